@@ -1,0 +1,383 @@
+/*
+ * lightgraph.h - C ABI of the MI355X (gfx950) LightGraph multiple-scattering path.
+ *
+ * The reference (asiroisvigneux/LightGraph) is a single C++ program; its hot path
+ * is the LightGraph estimator (src/LightGraph.h), the scatter-grid splat
+ * (src/ScatterGrid.h) and the per-pixel march (src/Renderer.h).  This header is the
+ * device boundary that the C++ host classes (lightgraph_amd/csrc/host) call.  Every
+ * entry point names the reference code it replaces.
+ *
+ * Conventions
+ *   - Every function returns int32_t: LG_OK (0) or a negative LG_ERR_* code.  The
+ *     message of the last failure on the calling thread is lg_last_error().
+ *   - No C++ exception crosses this boundary.
+ *   - Pointers named d_* are device pointers (allocated with lg_malloc or any other
+ *     HIP allocator on the context's device).  Pointers named h_* are host pointers.
+ *   - A context owns one HIP stream; all work of a context is ordered on that stream
+ *     and is asynchronous unless the entry point says it synchronises.  A context is
+ *     not thread-safe: use one per host thread.
+ *   - World space follows the reference: index space of the density grid maps to
+ *     world space by world = origin + voxel_size * index (uniform voxels,
+ *     Renderer.h:287-290 rejects non-uniform ones).
+ *   - Distances between graph vertices are optical depths tau >= 0 (float32);
+ *     +INFINITY means "no edge / unreachable".  Transmittance per channel is
+ *     exp(-sigma_t[c] * tau), with sigma_t = -extinction (Shader.h:133).
+ */
+#ifndef LIGHTGRAPH_H_
+#define LIGHTGRAPH_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LG_ABI_VERSION 1
+
+enum {
+    LG_OK = 0,
+    LG_ERR_INVALID = -1,     /* bad argument                                  */
+    LG_ERR_NOMEM = -2,       /* device allocation failed                      */
+    LG_ERR_HIP = -3,         /* HIP runtime error                             */
+    LG_ERR_BUDGET = -4,      /* request exceeds the context's HBM budget      */
+    LG_ERR_UNSUPPORTED = -5, /* valid request this build does not support     */
+    LG_ERR_OVERFLOW = -6     /* an output capacity given by the caller is too small */
+};
+
+enum { LG_F32 = 0, LG_F16 = 1, LG_BF16 = 2 };
+
+enum { LG_LIGHT_DIRECTIONAL = 0, LG_LIGHT_POINT = 1, LG_LIGHT_ENV = 2 };
+
+#define LG_MAX_RAMP 8
+#define LG_MAX_LIGHTS 8
+
+typedef struct lg_ctx lg_ctx;       /* opaque: device, stream, HBM budget   */
+typedef struct lg_grid lg_grid;     /* opaque: dense-brick voxel grid       */
+typedef struct lg_envmap lg_envmap; /* opaque: env map + FP64 sampling CDF  */
+typedef struct lg_solver lg_solver; /* opaque: per-solve device workspace   */
+
+/* Voxel grid description.  Replaces openvdb::FloatGrid + its linear transform
+ * (main.cc:491-502).  Index (x,y,z), 0 <= x < nx ..., addresses voxel centres. */
+typedef struct lg_grid_desc {
+    int32_t nx, ny, nz;
+    int32_t dtype;      /* LG_F32 | LG_F16 | LG_BF16: storage type of the bricks */
+    double voxel_size;  /* uniform voxel edge in world units                     */
+    double origin[3];   /* world position of index (0,0,0)                       */
+} lg_grid_desc;
+
+typedef struct lg_grid_info {
+    lg_grid_desc desc;
+    int32_t bricks[3];        /* ceil(n/8) per axis                               */
+    int32_t active_bricks;    /* bricks holding at least one non-zero voxel       */
+    int32_t bbox_min[3];      /* active bbox in voxel units at brick resolution,  */
+    int32_t bbox_max[3];      /* exclusive max: [8*bmin, 8*(bmax+1))              */
+    uint64_t active_voxels;   /* voxels != 0                                      */
+    uint64_t device_bytes;    /* HBM held by the grid                             */
+} lg_grid_info;
+
+/* Shader parameter block.  Replaces class Shader (Shader.h:100-274) and its
+ * ColorRamp (Shader.h:23-93); defaults are those of Shader::Shader (Shader.h:123-138). */
+typedef struct lg_shader {
+    double density_scale;          /* Shader.h:124 */
+    double density_max;            /* Shader.h:125 */
+    double shadow_density_scale;   /* Shader.h:126 */
+    double scatter_density_scale;  /* Shader.h:127 */
+    double scatter_density_min;    /* Shader.h:128 */
+    double albedo[3];              /* volumeColor*scattering/(scattering+absorption), Shader.h:132 */
+    double extinction[3];          /* -(scattering+absorption), Shader.h:133 (negative) */
+    double scattering[3];          /* Shader.h:130, used as 1/scattering in Renderer.h:516 */
+    double emission_scale;         /* Shader.h:134 */
+    double scatter_scale;          /* Shader.h:136 */
+    double scatter_density_mask_power; /* Shader.h:137 */
+    int32_t ramp_count;            /* 0: no ramp, emission colour is 0 (Shader.h:66-68) */
+    int32_t ramp_pad_;
+    double ramp_in, ramp_out;      /* ColorRamp mIn/mOut (Shader.h:26-27) */
+    double ramp_pos[LG_MAX_RAMP];
+    double ramp_color[LG_MAX_RAMP][3];
+} lg_shader;
+
+/* One light.  Replaces class Light and its subclasses (Light.h:26-235). */
+typedef struct lg_light {
+    int32_t type;        /* LG_LIGHT_*                                             */
+    int32_t samples;     /* Light::mSamples (1 for directional and point lights)   */
+    float intensity;     /* Light::mIntensity is a float in the reference (Light.h:31) */
+    float pad_;
+    double color[3];     /* Light::mColor                                          */
+    double dir[3];       /* directional: DistantLight::mDir (unit, Light.h:111)    */
+    double pos[3];       /* point: PointLight::mPos (Light.h:72)                   */
+    double xform[16];    /* env: Light::mXform, row-major, row-vector convention   */
+    const lg_envmap *env;/* env: map + CDF created by lg_envmap_create            */
+} lg_light;
+
+/* March settings.  Replaces the SceneSettings fields the marches read
+ * (SceneSettings.h:31-32). */
+typedef struct lg_march_params {
+    double cutoff;   /* SceneSettings::cutoff, default 0.005                      */
+    double step[2];  /* SceneSettings::step: primary, shadow (index units)        */
+} lg_march_params;
+
+/* Pinhole camera.  Replaces PerspectiveCamera (Renderer.h:129-248); the host
+ * fills it with lg_camera_setup or its own equivalent. */
+typedef struct lg_camera {
+    double screen_to_world[16]; /* AffineMap matrix, row-major, world = [x y z 1]*M */
+    double scale_width;         /* 0.5*aperture/focal (Renderer.h:162)              */
+    double scale_height;        /* scale_width*height/width (Renderer.h:163)        */
+    double znear, zfar;         /* ray time span (Renderer.h:234)                   */
+    int32_t width, height;
+} lg_camera;
+
+/* Device pointers to the four AOV films, each width*height RGBA float32 in the
+ * layout of Film::RGBA (Renderer.h:72-101).  Any of them may be NULL. */
+typedef struct lg_films {
+    float *d_rgb, *d_direct, *d_scatter, *d_emission;
+} lg_films;
+
+/* Scatter grid (multiple-scattering radiance, Vec3f) on a coarse lattice:
+ * world = index * voxel_size (no origin, like createLinearTransform in
+ * ScatterGrid.h:310).  Dense, x fastest, 3 floats per voxel. */
+typedef struct lg_scatter_grid {
+    int32_t nx, ny, nz, pad_;
+    double voxel_size;
+    float *d_rgb;          /* nx*ny*nz*3 floats                    */
+    uint8_t *d_active;     /* nx*ny*nz flags (1 = active voxel)    */
+} lg_scatter_grid;
+
+/* Procedural volumes used for the synthetic workloads. */
+enum { LG_SCENE_SPHERE_FBM = 0, LG_SCENE_BLOBS_FBM = 1, LG_SCENE_PLUME = 2 };
+typedef struct lg_scene_spec {
+    int32_t kind;        /* LG_SCENE_*                                     */
+    int32_t n;           /* cube side in voxels                            */
+    int32_t octaves;     /* fBm octaves                                    */
+    int32_t blobs;       /* BLOBS: number of ellipsoids                    */
+    uint64_t seed;
+    double radius;       /* SPHERE: sphere radius in voxels                */
+    double axis_min, axis_max;   /* BLOBS: semi-axis range in voxels       */
+    double active_fraction;      /* BLOBS: target fraction of active voxels (0 = no tuning) */
+    int32_t dtype;       /* storage type of the density bricks             */
+    int32_t with_temperature;    /* PLUME: also produce a temperature grid */
+} lg_scene_spec;
+
+/* Per-stage timing of the last lg_solve on a solver (milliseconds, HIP events). */
+typedef struct lg_solve_timing {
+    float sample_ms, graph_ms, march_ms, light_ms, fw_ms, diffuse_ms, total_ms;
+} lg_solve_timing;
+
+/* Device buffers owned by a solver (valid until the next lg_solve on it). */
+typedef struct lg_solver_buffers {
+    int32_t n, ld, n_edges, edge_cap;
+    float *d_points;      /* n*3, world space                                      */
+    int32_t *d_edges;     /* n_edges*2, (i,j) with i<j, in the reference's scan order */
+    float *d_weights;     /* n_edges edge optical depths (+INF = dropped)           */
+    float *d_D;           /* n x ld all-pairs optical depth after the solve          */
+    float *d_L;           /* n*3 direct light per vertex                             */
+    float *d_diffuse;     /* n*3 diffused radiance per vertex                        */
+    int32_t *d_pred;      /* n x ld canonical predecessors, or NULL                  */
+} lg_solver_buffers;
+
+/* ---------------------------------------------------------------- context ---- */
+const char *lg_last_error(void);
+int32_t lg_abi_version(void);
+/* hbm_budget_bytes = 0 means "no limit beyond the device". */
+int32_t lg_ctx_create(int32_t device, uint64_t hbm_budget_bytes, lg_ctx **out);
+int32_t lg_ctx_destroy(lg_ctx *ctx);
+int32_t lg_sync(lg_ctx *ctx);
+int32_t lg_ctx_stream(lg_ctx *ctx, void **hip_stream_out);
+int32_t lg_ctx_bytes_in_use(lg_ctx *ctx, uint64_t *bytes);
+/* Per-kernel HIP-event timing of the library's own launches (for benchmarks). */
+int32_t lg_ctx_set_kernel_timing(lg_ctx *ctx, int32_t enable);
+int32_t lg_kernel_time(lg_ctx *ctx, const char *kernel, uint64_t *launches, double *total_ms);
+int32_t lg_kernel_time_reset(lg_ctx *ctx);
+
+/* Device memory, charged against the context budget. */
+int32_t lg_malloc(lg_ctx *ctx, size_t bytes, void **d_out);
+int32_t lg_free(lg_ctx *ctx, void *d_ptr);
+int32_t lg_memcpy_h2d(lg_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int32_t lg_memcpy_d2h(lg_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+int32_t lg_memset_d(lg_ctx *ctx, void *d_dst, int32_t value, size_t bytes);
+
+/* Stream-ordered timer: lg_timer_mark records an event, lg_timer_elapsed syncs. */
+int32_t lg_timer_mark(lg_ctx *ctx, int32_t slot);
+int32_t lg_timer_elapsed(lg_ctx *ctx, int32_t slot_a, int32_t slot_b, float *ms);
+
+/* ------------------------------------------------------------------ grids ---- */
+/* Upload a dense host volume (x fastest, nx*ny*nz float32) into a dense-brick
+ * device grid.  Replaces io::File::readGrid + GridSampler set-up (main.cc:489-502).
+ * Voxels != 0 are active; bricks holding an active voxel exist (leaf nodes). */
+int32_t lg_grid_create(lg_ctx *ctx, const lg_grid_desc *desc, const float *h_dense, lg_grid **out);
+/* Generate a procedural grid on the device (no counterpart in the reference: it
+ * only reads VDB files).  temperature_out may be NULL. */
+int32_t lg_grid_generate(lg_ctx *ctx, const lg_scene_spec *spec, lg_grid **density_out,
+                         lg_grid **temperature_out);
+/* Host twin of lg_grid_generate's density (same arithmetic; for CPU-only use). */
+int32_t lg_scene_generate_host(const lg_scene_spec *spec, float *h_dense_out, int32_t threads);
+int32_t lg_grid_info_get(const lg_grid *grid, lg_grid_info *info);
+/* Download the grid as a dense float32 volume (x fastest, nx*ny*nz). */
+int32_t lg_grid_read_dense(lg_ctx *ctx, const lg_grid *grid, float *h_out);
+int32_t lg_grid_destroy(lg_ctx *ctx, lg_grid *grid);
+
+/* ---------------------------------------------------------- solve stages ---- */
+/* Kernel (a): density-weighted point sampling over the dense bricks.  Replaces
+ * the dart thrower LightGraph::scatterPoints (LightGraph.h:206-274) with the
+ * density-weighted sampler: voxel v is drawn with probability
+ * floor(rho_v*2^24)/sum, then jittered uniformly in its cell.  Stream: Philox4x32-10
+ * keyed by seed, counter (point, solve, block, 1).  d_pts_out: n*3 floats (world). */
+int32_t lg_sample_points(lg_ctx *ctx, const lg_grid *grid, uint64_t seed, uint32_t solve,
+                         int32_t n, float *d_pts_out);
+
+/* Graph build.  Replaces LightGraph::buildGraph (LightGraph.h:287-318): for i
+ * ascending, j>i ascending, add (i,j) when |p_i-p_j|^2 < radius^2 (FP64), skipping
+ * j whose count reached max_connect and ending row i when its count reaches it.
+ * max_connect <= 0 means uncapped.  d_edges_out: edge_cap*2 int32 in row-major
+ * order; *n_edges_out (host) receives the count (the call synchronises). */
+int32_t lg_build_edges(lg_ctx *ctx, const float *d_pts, int32_t n, double radius,
+                       int32_t max_connect, int32_t *d_edges_out, int32_t edge_cap,
+                       int32_t *n_edges_out);
+
+/* Kernel (b): edge optical depth.  Replaces LightGraph::raymarchConnection
+ * (LightGraph.h:347-418).  For edge e=(i,j): tau_e = sum over samples of
+ * max(rho', scatterDensityMin)*h, tau_e = +INF when the ray misses the grid bbox or
+ * the reference's dropout (|T|^2 < cutoff) fires.  Writes d_w[e] (may be NULL) and,
+ * when d_D is not NULL, D[i*ld+j] = D[j*ld+i] = tau_e. */
+int32_t lg_march_edges(lg_ctx *ctx, const lg_grid *grid, const lg_shader *shader,
+                       const lg_march_params *params, const float *d_pts,
+                       const int32_t *d_edges, int32_t n_edges, float *d_w, float *d_D,
+                       int32_t ld);
+
+/* Vertex lighting.  Replaces LightGraph::computeLighting (LightGraph.h:427-505):
+ * d_L_out[i] = sum_lights albedo*mean_s(I_s*T_s) + ramp(temp(p_i))*emissionScale.
+ * temperature may be NULL (all zero).  Env-light samples use Philox counter
+ * (vertex, solve, block, 4) with env_seed. */
+int32_t lg_vertex_light(lg_ctx *ctx, const lg_grid *density, const lg_grid *temperature,
+                        const lg_shader *shader, const lg_light *lights, int32_t n_lights,
+                        const lg_march_params *params, uint64_t env_seed, uint32_t solve,
+                        const float *d_pts, int32_t n, float *d_L_out);
+
+/* Fill D (n x ld, ld >= n) with +INF off the diagonal and 0 on it. */
+int32_t lg_fw_init(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld);
+/* Required leading dimension of D for lg_fw_solve with n vertices. */
+int32_t lg_fw_ld(int32_t n);
+
+/* Kernel (c): all-pairs min-plus closure of the symmetric matrix D (n x ld,
+ * ld = lg_fw_ld(n), zero diagonal, +INF = no edge) in place.  Replaces
+ * LightGraph::floydAlgo (LightGraph.h:512-542): the reference's per-channel
+ * (max, x) Floyd-Warshall on transmittance equals (min, +) on tau because
+ * T_c = exp(-sigma_t[c]*tau) with a global sigma_t.  Pivots are taken in
+ * ascending order in blocks of lg_fw_ld's tile.  If d_pred is not NULL it receives
+ * canonical predecessors: pred[i][j] = smallest k adjacent to j minimising
+ * D[i][k] + w(k,j) (D[i][i]=0), -1 if unreachable, i on the diagonal; this needs
+ * the edge list and its weights. */
+int32_t lg_fw_solve(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld, int32_t *d_pred,
+                    const int32_t *d_edges, const float *d_w, int32_t n_edges);
+
+/* Radiance diffusion.  Replaces LightGraph::diffuseLight (LightGraph.h:554-587):
+ * P_c = exp(-sigma_t[c]*D) (so P_ii = 1, LightGraph.h:530-534), columns normalised
+ * by max(sum_i P_ij, 1), out[i] = sum_j L[j] (x) P_ij. */
+int32_t lg_diffuse(lg_ctx *ctx, int32_t n, const float *d_D, int32_t ld,
+                   const double sigma_t[3], const float *d_L, float *d_out);
+
+/* ------------------------------------------------------------ full solve ---- */
+/* Per-solve workspace for graphs of up to n_max vertices.  with_pred allocates
+ * the predecessor matrix. */
+int32_t lg_solver_create(lg_ctx *ctx, int32_t n_max, int32_t edge_cap, int32_t with_pred,
+                         lg_solver **out);
+int32_t lg_solver_destroy(lg_ctx *ctx, lg_solver *solver);
+int32_t lg_solver_get(const lg_solver *solver, lg_solver_buffers *out);
+int32_t lg_solver_timing(const lg_solver *solver, lg_solve_timing *out);
+
+typedef struct lg_solve_params {
+    int32_t n;             /* points in the graph                                  */
+    int32_t max_connect;   /* GRAPH_MAX_CONNECT (main.cc:62) = 12; <= 0 uncapped    */
+    double radius;         /* edge radius in world units                           */
+    uint64_t seed;         /* point-sampling seed (reference engines: 854123+i)     */
+    uint64_t env_seed;     /* env-light sampling seed (Light.h:157: 4564446)        */
+    uint32_t solve;        /* solve index (the reference's LightGraph iteration i)  */
+    int32_t external_points; /* 1: use the points already in the solver's d_points */
+    int32_t timing;        /* 1: record per-stage HIP events                       */
+    int32_t pad_;
+} lg_solve_params;
+
+/* One LightGraph solve (LightGraph::operator() body, LightGraph.h:163-189):
+ * sample -> edges -> edge march -> Floyd-Warshall -> vertex light -> diffuse. */
+int32_t lg_solve(lg_ctx *ctx, lg_solver *solver, const lg_grid *density,
+                 const lg_grid *temperature, const lg_shader *shader, const lg_light *lights,
+                 int32_t n_lights, const lg_march_params *params, const lg_solve_params *sp);
+
+/* ----------------------------------------------------------- batched form ---- */
+/* Many independent min-plus closures (uneven n) in one persistent launch.
+ * d_D[g] is n[g] x ld[g] with ld[g] = lg_fw_ld(n[g]).  The host arrays are read
+ * during the call; the device matrices are updated in place. */
+int32_t lg_fw_solve_batched(lg_ctx *ctx, int32_t count, const int32_t *h_n,
+                            float *const *h_D, const int32_t *h_ld);
+
+/* ----------------------------------------------------------- splat/render ---- */
+/* Splat radius of ScatterGrid::operator() (ScatterGrid.h:71-73) with the voxel size
+ * of LightGraph::buildVdbPointGrid (LightGraph.h:713-714) restated on the host. */
+int32_t lg_splat_radius(const float *h_pts, int64_t count, int64_t pts_per_kernel_total,
+                        double *radius_out, double *point_voxel_out);
+/* Coarse topology (ScatterGrid::buildScatterGrid, ScatterGrid.h:105-144): a coarse
+ * voxel is active if any density voxel in its footprint is active, then the set is
+ * dilated by one voxel (26-neighbourhood).  Allocates sg->d_rgb / d_active. */
+int32_t lg_scatter_grid_create(lg_ctx *ctx, const lg_grid *density, int32_t res_factor,
+                               lg_scatter_grid *sg);
+int32_t lg_scatter_grid_destroy(lg_ctx *ctx, lg_scatter_grid *sg);
+/* Gaussian splat (ScatterGrid::operator(), ScatterGrid.h:61-98) of count points
+ * (world, float3) with radiance (float3) into every active coarse voxel. */
+int32_t lg_splat(lg_ctx *ctx, const float *d_pts, const float *d_rad, int64_t count,
+                 double radius, lg_scatter_grid *sg);
+
+/* Camera set-up of main.cc:96-102 + PerspectiveCamera ctor (Renderer.h:148-171). */
+int32_t lg_camera_setup(const double cam_rotate[3], const double cam_translate[3],
+                        double cam_y_rotate, double focal, double aperture, double znear,
+                        double zfar, int32_t width, int32_t height, lg_camera *out);
+
+/* Kernel (d): per-pixel march.  Replaces VolumeRender::operator()
+ * (Renderer.h:506-693).  samples = AA samples per pixel; the render stream is
+ * Philox counter (pixel, sample, block, 2) keyed by seed (reference: RandGen(1),
+ * Renderer.h:518); env lights use (pixel, sample, block, 3) keyed by env_seed. */
+int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *scatter,
+                  const lg_grid *temperature, const lg_camera *camera,
+                  const lg_shader *shader, const lg_light *lights, int32_t n_lights,
+                  const lg_march_params *params, int32_t samples, uint64_t seed,
+                  uint64_t env_seed, const lg_films *films);
+
+/* Environment map: width x height RGB float32 (row 0 = top), luminance*sin(theta)
+ * weighted 2D distribution (Light.h:165-176, Sampling.h:128-167) with its CDF
+ * prefix sums computed on the device in FP64. */
+int32_t lg_envmap_create(lg_ctx *ctx, int32_t width, int32_t height, const float *h_rgb,
+                         lg_envmap **out);
+int32_t lg_envmap_destroy(lg_ctx *ctx, lg_envmap *env);
+/* Download the FP64 CDFs: conditional (height rows x (width+1)), marginal (height+1),
+ * per-row integrals (height) and the total. */
+int32_t lg_envmap_read_cdf(lg_ctx *ctx, const lg_envmap *env, double *h_cond,
+                           double *h_marg, double *h_rowint, double *h_total);
+
+/* ------------------------------------------------- host-side parameters ---- */
+/* Message of the last failure of the functions in this section. */
+const char *lg_host_last_error(void);
+/* Shader::Shader defaults (Shader.h:123-138). */
+int32_t lg_shader_default(lg_shader *s);
+/* Shader::setAbsorption/setScattering/setVolumeColor (Shader.h:165-211). */
+int32_t lg_shader_set_coefficients(lg_shader *s, const double absorption[3], const double scattering[3],
+                                   const double volume_color[3]);
+/* parseColorRamp + Shader::setEmissionColorRamp (main.cc:280-319). */
+int32_t lg_shader_parse_ramp(lg_shader *s, const char *text);
+/* SceneSettings cutoff/step defaults (SceneSettings.h:51-52). */
+int32_t lg_march_params_default(lg_march_params *p);
+/* parseLight "dir" + DistantLight (main.cc:255-269, Light.h:106-111). */
+int32_t lg_light_directional(double y_rot, const double color[3], double intensity, const double rot[3],
+                             lg_light *out);
+/* PointLight (Light.h:56-91); the reference has no CLI form for it. */
+int32_t lg_light_point(double y_rot, const double color[3], double intensity, const double pos[3],
+                       lg_light *out);
+/* EnvLight (Light.h:132-153) over a map from lg_envmap_create. */
+int32_t lg_light_env(double y_rot, const double color[3], double intensity, int32_t samples,
+                     const lg_envmap *env, lg_light *out);
+/* parseLight (main.cc:234-274) for "dir:..." and "point:yRot:r,g,b:intensity:x,y,z". */
+int32_t lg_light_parse(const char *text, lg_light *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIGHTGRAPH_H_ */

@@ -1,0 +1,397 @@
+"""Python access to the MI355X LightGraph library (liblightgraph.so).
+
+The compute path is the HIP/C++ library behind the C ABI of
+include/lightgraph.h; this module only marshals numpy arrays and parameter
+blocks for tests, the benchmark and the smoke check.  Nothing here falls back to
+a CPU implementation: if the library or a GPU is missing, calls raise.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import (LG_F32, LG_F16, LG_BF16, LG_LIGHT_DIRECTIONAL, LG_LIGHT_POINT, LG_LIGHT_ENV,
+                   LG_SCENE_SPHERE_FBM, LG_SCENE_BLOBS_FBM, LG_SCENE_PLUME, lg_shader, lg_light,
+                   lg_march_params, lg_camera, lg_scene_spec, lg_grid_desc, lg_films, lg_scatter_grid,
+                   lg_solve_params)
+
+_LIB = None
+
+
+class LGError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[lightgraph {code}] {msg}")
+        self.code = code
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = _abi.load()
+    return _LIB
+
+
+def _check(rc, what=""):
+    if rc != 0:
+        msg = lib().lg_last_error()
+        raise LGError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+
+def _hcheck(rc, what=""):
+    if rc != 0:
+        msg = lib().lg_host_last_error()
+        raise LGError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+
+def _d3(v):
+    return (C.c_double * 3)(*[float(x) for x in v])
+
+
+def _vec3(v):
+    if np.isscalar(v):
+        return [float(v)] * 3
+    v = list(v)
+    return [float(x) for x in v] if len(v) == 3 else [float(v[0])] * 3
+
+
+# ------------------------------------------------------------ parameters ----
+def default_shader(absorption=None, scattering=None, volume_color=None, **fields):
+    """Shader::Shader defaults (Shader.h:123-138) plus overrides."""
+    s = lg_shader()
+    _hcheck(lib().lg_shader_default(C.byref(s)), "lg_shader_default")
+    if absorption is not None or scattering is not None or volume_color is not None:
+        a = _vec3(absorption if absorption is not None else 0.1)
+        sc = _vec3(scattering if scattering is not None else 1.5)
+        vc = _vec3(volume_color if volume_color is not None else 1.0)
+        _hcheck(lib().lg_shader_set_coefficients(C.byref(s), _d3(a), _d3(sc), _d3(vc)), "coefficients")
+    ramp = fields.pop("ramp", None)
+    if ramp is not None:
+        _hcheck(lib().lg_shader_parse_ramp(C.byref(s), ramp.encode()), "ramp")
+    for k, v in fields.items():
+        setattr(s, k, v)
+    return s
+
+
+def march_params(cutoff=0.005, primary_step=1.0, shadow_step=3.0):
+    p = lg_march_params()
+    p.cutoff = cutoff
+    p.step[0] = primary_step
+    p.step[1] = shadow_step
+    return p
+
+
+def directional_light(y_rot=0.0, color=(1, 1, 1), intensity=1.0, rot=(0, 0, 0)):
+    l = lg_light()
+    _hcheck(lib().lg_light_directional(y_rot, _d3(_vec3(color)), intensity, _d3(_vec3(rot)), C.byref(l)), "dir")
+    return l
+
+
+def point_light(y_rot=0.0, color=(1, 1, 1), intensity=1.0, pos=(0, 0, 0)):
+    l = lg_light()
+    _hcheck(lib().lg_light_point(y_rot, _d3(_vec3(color)), intensity, _d3(_vec3(pos)), C.byref(l)), "point")
+    return l
+
+
+def env_light(envmap, y_rot=0.0, color=(1, 1, 1), intensity=1.0, samples=3):
+    l = lg_light()
+    _hcheck(lib().lg_light_env(y_rot, _d3(_vec3(color)), intensity, samples, envmap.handle, C.byref(l)), "env")
+    return l
+
+
+def parse_light(text):
+    l = lg_light()
+    _hcheck(lib().lg_light_parse(text.encode(), C.byref(l)), f"light '{text}'")
+    return l
+
+
+def light_array(lights):
+    arr = (lg_light * max(1, len(lights)))()
+    for k, l in enumerate(lights):
+        arr[k] = l
+    return arr
+
+
+def scene_spec(kind, n, seed, octaves=4, radius=0.0, blobs=0, axis_min=30.0, axis_max=120.0,
+               active_fraction=0.0, dtype=LG_F32, with_temperature=0):
+    s = lg_scene_spec()
+    s.kind, s.n, s.seed, s.octaves = kind, n, seed, octaves
+    s.radius, s.blobs, s.axis_min, s.axis_max = radius, blobs, axis_min, axis_max
+    s.active_fraction, s.dtype, s.with_temperature = active_fraction, dtype, with_temperature
+    return s
+
+
+def generate_host(spec, threads=0):
+    """Host twin of the procedural generator: dense float32 volume [z, y, x]."""
+    out = np.zeros((spec.n, spec.n, spec.n), dtype=np.float32)
+    _check(lib().lg_scene_generate_host(C.byref(spec), out.ctypes.data, threads), "generate_host")
+    return out
+
+
+def fw_ld(n):
+    return lib().lg_fw_ld(n)
+
+
+# --------------------------------------------------------------- device -----
+class DeviceBuffer:
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        _check(lib().lg_malloc(ctx.handle, self.nbytes, C.byref(p)), "lg_malloc")
+        self.ptr = p.value
+
+    def free(self):
+        if self.ptr:
+            _check(lib().lg_free(self.ctx.handle, self.ptr), "lg_free")
+            self.ptr = None
+
+    def write(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes, "write exceeds buffer"
+        _check(lib().lg_memcpy_h2d(self.ctx.handle, self.ptr, arr.ctypes.data, arr.nbytes), "h2d")
+
+    def read(self, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        assert out.nbytes <= self.nbytes, "read exceeds buffer"
+        _check(lib().lg_memcpy_d2h(self.ctx.handle, out.ctypes.data, self.ptr, out.nbytes), "d2h")
+        return out
+
+
+class Context:
+    """One HIP device + stream (lg_ctx)."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        h = C.c_void_p()
+        _check(lib().lg_ctx_create(device, hbm_budget, C.byref(h)), "lg_ctx_create")
+        self.handle = h.value
+
+    def close(self):
+        if self.handle:
+            lib().lg_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        _check(lib().lg_sync(self.handle), "lg_sync")
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        b = DeviceBuffer(self, max(arr.nbytes, 4))
+        b.write(arr)
+        return b
+
+    def bytes_in_use(self):
+        v = C.c_uint64()
+        _check(lib().lg_ctx_bytes_in_use(self.handle, C.byref(v)), "bytes_in_use")
+        return v.value
+
+    def kernel_timing(self, enable=True):
+        _check(lib().lg_ctx_set_kernel_timing(self.handle, 1 if enable else 0), "kernel_timing")
+
+    def kernel_time(self, name):
+        n = C.c_uint64()
+        ms = C.c_double()
+        _check(lib().lg_kernel_time(self.handle, name.encode(), C.byref(n), C.byref(ms)), "kernel_time")
+        return n.value, ms.value
+
+    def kernel_time_reset(self):
+        _check(lib().lg_kernel_time_reset(self.handle), "kernel_time_reset")
+
+    def mark(self, slot):
+        _check(lib().lg_timer_mark(self.handle, slot), "timer_mark")
+
+    def elapsed_ms(self, a, b):
+        ms = C.c_float()
+        _check(lib().lg_timer_elapsed(self.handle, a, b, C.byref(ms)), "timer_elapsed")
+        return ms.value
+
+
+class Grid:
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    @staticmethod
+    def from_dense(ctx, dense_zyx, voxel_size=1.0, origin=(0.0, 0.0, 0.0), dtype=LG_F32):
+        a = np.ascontiguousarray(dense_zyx, dtype=np.float32)
+        d = lg_grid_desc()
+        d.nz, d.ny, d.nx = a.shape
+        d.dtype = dtype
+        d.voxel_size = voxel_size
+        for c in range(3):
+            d.origin[c] = origin[c]
+        h = C.c_void_p()
+        _check(lib().lg_grid_create(ctx.handle, C.byref(d), a.ctypes.data, C.byref(h)), "lg_grid_create")
+        return Grid(ctx, h.value)
+
+    @staticmethod
+    def generate(ctx, spec):
+        d = C.c_void_p()
+        t = C.c_void_p()
+        _check(lib().lg_grid_generate(ctx.handle, C.byref(spec), C.byref(d), C.byref(t)), "lg_grid_generate")
+        return Grid(ctx, d.value), (Grid(ctx, t.value) if t.value else None)
+
+    def info(self):
+        i = _abi.lg_grid_info()
+        _check(lib().lg_grid_info_get(self.handle, C.byref(i)), "grid_info")
+        return i
+
+    def read_dense(self):
+        i = self.info()
+        out = np.empty((i.desc.nz, i.desc.ny, i.desc.nx), dtype=np.float32)
+        _check(lib().lg_grid_read_dense(self.ctx.handle, self.handle, out.ctypes.data), "read_dense")
+        return out
+
+    def destroy(self):
+        if self.handle:
+            lib().lg_grid_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+
+class EnvMap:
+    def __init__(self, ctx, rgb_hw3):
+        self.ctx = ctx
+        a = np.ascontiguousarray(rgb_hw3, dtype=np.float32)
+        self.height, self.width = a.shape[:2]
+        self.rgb = a
+        h = C.c_void_p()
+        _check(lib().lg_envmap_create(ctx.handle, self.width, self.height, a.ctypes.data, C.byref(h)), "envmap")
+        self.handle = h.value
+
+    def read_cdf(self):
+        cond = np.empty((self.height, self.width + 1), dtype=np.float64)
+        marg = np.empty(self.height + 1, dtype=np.float64)
+        rowint = np.empty(self.height, dtype=np.float64)
+        total = C.c_double()
+        _check(lib().lg_envmap_read_cdf(self.ctx.handle, self.handle, cond.ctypes.data_as(_abi.DP),
+                                        marg.ctypes.data_as(_abi.DP), rowint.ctypes.data_as(_abi.DP),
+                                        C.byref(total)), "read_cdf")
+        return cond, marg, rowint, total.value
+
+    def destroy(self):
+        if self.handle:
+            lib().lg_envmap_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+
+# ---------------------------------------------------------- solve stages ----
+def sample_points(ctx, grid, seed, solve, n, out_buf):
+    _check(lib().lg_sample_points(ctx.handle, grid.handle, seed, solve, n, out_buf.ptr), "sample_points")
+
+
+def build_edges(ctx, pts_buf, n, radius, max_connect, edges_buf, edge_cap):
+    ne = C.c_int32()
+    _check(lib().lg_build_edges(ctx.handle, pts_buf.ptr, n, radius, max_connect, edges_buf.ptr, edge_cap,
+                                C.byref(ne)), "build_edges")
+    return ne.value
+
+
+def march_edges(ctx, grid, shader, params, pts_buf, edges_buf, n_edges, w_buf=None, D_buf=None, ld=0):
+    _check(lib().lg_march_edges(ctx.handle, grid.handle, C.byref(shader), C.byref(params), pts_buf.ptr,
+                                edges_buf.ptr if edges_buf else None, n_edges, w_buf.ptr if w_buf else None,
+                                D_buf.ptr if D_buf else None, ld), "march_edges")
+
+
+def vertex_light(ctx, grid, temp_grid, shader, lights, params, env_seed, solve, pts_buf, n, out_buf):
+    arr = light_array(lights)
+    _check(lib().lg_vertex_light(ctx.handle, grid.handle, temp_grid.handle if temp_grid else None,
+                                 C.byref(shader), arr, len(lights), C.byref(params), env_seed, solve,
+                                 pts_buf.ptr, n, out_buf.ptr), "vertex_light")
+
+
+def fw_init(ctx, n, D_buf, ld):
+    _check(lib().lg_fw_init(ctx.handle, n, D_buf.ptr, ld), "fw_init")
+
+
+def fw_solve(ctx, n, D_buf, ld, pred_buf=None, edges_buf=None, w_buf=None, n_edges=0):
+    _check(lib().lg_fw_solve(ctx.handle, n, D_buf.ptr, ld, pred_buf.ptr if pred_buf else None,
+                             edges_buf.ptr if edges_buf else None, w_buf.ptr if w_buf else None, n_edges),
+           "fw_solve")
+
+
+def fw_solve_batched(ctx, ns, D_bufs):
+    k = len(ns)
+    n_arr = (C.c_int32 * k)(*ns)
+    ld_arr = (C.c_int32 * k)(*[fw_ld(n) for n in ns])
+    p_arr = (C.c_void_p * k)(*[b.ptr for b in D_bufs])
+    _check(lib().lg_fw_solve_batched(ctx.handle, k, n_arr, p_arr, ld_arr), "fw_solve_batched")
+
+
+def diffuse(ctx, n, D_buf, ld, sigma_t, L_buf, out_buf):
+    sig = (C.c_double * 3)(*sigma_t)
+    _check(lib().lg_diffuse(ctx.handle, n, D_buf.ptr, ld, sig, L_buf.ptr, out_buf.ptr), "diffuse")
+
+
+class Solver:
+    """Per-solve workspace (lg_solver): LightGraph::operator() for one index."""
+
+    def __init__(self, ctx, n_max, edge_cap=0, with_pred=False):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(lib().lg_solver_create(ctx.handle, n_max, edge_cap, 1 if with_pred else 0, C.byref(h)),
+               "solver_create")
+        self.handle = h.value
+
+    def solve(self, density, shader, lights, params, n, radius, seed=854123, solve=0, max_connect=12,
+              temperature=None, env_seed=4564446, external_points=False, timing=False):
+        sp = lg_solve_params()
+        sp.n, sp.max_connect, sp.radius, sp.seed, sp.env_seed = n, max_connect, radius, seed, env_seed
+        sp.solve, sp.external_points, sp.timing = solve, 1 if external_points else 0, 1 if timing else 0
+        arr = light_array(lights)
+        _check(lib().lg_solve(self.ctx.handle, self.handle, density.handle,
+                              temperature.handle if temperature else None, C.byref(shader), arr, len(lights),
+                              C.byref(params), C.byref(sp)), "lg_solve")
+
+    def buffers(self):
+        b = _abi.lg_solver_buffers()
+        _check(lib().lg_solver_get(self.handle, C.byref(b)), "solver_get")
+        return b
+
+    def timing(self):
+        t = _abi.lg_solve_timing()
+        _check(lib().lg_solver_timing(self.handle, C.byref(t)), "solver_timing")
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def _read(self, ptr, dtype, shape):
+        out = np.empty(shape, dtype=dtype)
+        _check(lib().lg_memcpy_d2h(self.ctx.handle, out.ctypes.data, ptr, out.nbytes), "d2h")
+        return out
+
+    def write_points(self, pts):
+        b = self.buffers()
+        a = np.ascontiguousarray(pts, dtype=np.float32)
+        _check(lib().lg_memcpy_h2d(self.ctx.handle, b.d_points, a.ctypes.data, a.nbytes), "h2d")
+
+    def results(self, want_D=True):
+        b = self.buffers()
+        r = {"points": self._read(b.d_points, np.float32, (b.n, 3)),
+             "edges": self._read(b.d_edges, np.int32, (b.n_edges, 2)) if b.n_edges else np.zeros((0, 2), np.int32),
+             "weights": self._read(b.d_weights, np.float32, (b.n_edges,)) if b.n_edges else np.zeros(0, np.float32),
+             "L": self._read(b.d_L, np.float32, (b.n, 3)),
+             "diffuse": self._read(b.d_diffuse, np.float32, (b.n, 3))}
+        if want_D:
+            r["D"] = self._read(b.d_D, np.float32, (b.ld, b.ld))[: b.n, : b.n]
+        if b.d_pred:
+            r["pred"] = self._read(b.d_pred, np.int32, (b.ld, b.ld))[: b.n, : b.n]
+        return r
+
+    def destroy(self):
+        if self.handle:
+            lib().lg_solver_destroy(self.ctx.handle, self.handle)
+            self.handle = None
+
+
+def camera_setup(cam_rotate=(0, 0, 0), cam_translate=(0, 0, 0), cam_y_rotate=0.0, focal=50.0, aperture=36.0,
+                 znear=1e-3, zfar=3.4028234663852886e38, width=960, height=540):
+    cam = lg_camera()
+    _check(lib().lg_camera_setup(_d3(cam_rotate), _d3(cam_translate), cam_y_rotate, focal, aperture, znear, zfar,
+                                 width, height, C.byref(cam)), "camera_setup")
+    return cam

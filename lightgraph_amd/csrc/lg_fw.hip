@@ -1,0 +1,333 @@
+// Kernel (c): blocked Floyd-Warshall in the (min, +) semiring on optical depth
+// (the reference's (max, x) on per-channel transmittance, LightGraph.h:512-542).
+//
+// Schedule for pivot block K (tiles of 128 x 128, pivots in ascending order):
+//   1. pivot: close tile (K,K) in LDS, one pivot at a time (1024 threads);
+//   2. row panel: (K,J) = min((K,J), P*(K,K) (x) (K,J)) for every J != K, where P*
+//      is the closed pivot tile (its zero diagonal keeps the old value); the
+//      column panel (J,K) receives the transpose;
+//   3. remainder: for I <= J (both != K): (I,J) = min((I,J), (K,I)^T (x) (K,J)),
+//      and (J,I) receives the transpose.
+// D is symmetric (undirected graph), so phases 2 and 3 read only the row panel
+// and compute half of the tiles.  Phases 2 and 3 are one min-plus tile kernel:
+// C = min(C, X^T (x) Y) with X, Y row-panel tiles, 8x8 outputs per thread, two
+// pivots per step folded with v_min3_f32.  Tiles whose row-panel operand is all
+// +INF are skipped (the reference skips the same zero-transmittance entries,
+// LightGraph.h:518-520); skipping changes no value.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "lg_internal.h"
+
+namespace lg {
+
+constexpr int TB = FW_TILE;  // 128
+constexpr int KC = 32;       // pivots staged in LDS per step
+
+__global__ void __launch_bounds__(1024) k_fw_pivot(float *__restrict__ D, int ld, int kb) {
+    __shared__ float T[TB][TB];
+    const int tid = threadIdx.x;
+    const int c = tid & (TB - 1), r0 = tid >> 7;
+    float *base = D + (long long)kb * TB * ld + (long long)kb * TB;
+    float v[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int row = r0 + 8 * r;
+        v[r] = base[(long long)row * ld + c];
+        T[row][c] = v[r];
+    }
+    __syncthreads();
+    for (int k = 0; k < TB; ++k) {
+        const float dkj = T[k][c];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = r0 + 8 * r;
+            const float nv = T[row][k] + dkj;
+            if (nv < v[r]) {
+                v[r] = nv;
+                T[row][c] = nv;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) base[(long long)(r0 + 8 * r) * ld + c] = v[r];
+}
+
+__device__ __forceinline__ void tri_decode(int p, int T, int &a, int &b) {
+    // row-major upper triangle of a T x T matrix: row a holds T - a entries
+    const double tt = 2.0 * T + 1.0;
+    int r = (int)floor((tt - sqrt(tt * tt - 8.0 * (double)p)) * 0.5);
+    auto S = [&](int x) { return x * T - (x * (x - 1)) / 2; };
+    if (r < 0) r = 0;
+    if (r > T - 1) r = T - 1;
+    while (r + 1 <= T - 1 && S(r + 1) <= p) ++r;
+    while (r > 0 && S(r) > p) --r;
+    a = r;
+    b = r + (p - S(r));
+}
+
+template <bool PANEL>
+__global__ void __launch_bounds__(256) k_fw_minplus(float *__restrict__ D, int ld, int kb, int nt,
+                                                     int *__restrict__ flags) {
+    int Ib, Jb;
+    if (PANEL) {
+        Ib = kb;
+        Jb = (int)blockIdx.x < kb ? (int)blockIdx.x : (int)blockIdx.x + 1;
+    } else {
+        int a, b;
+        tri_decode((int)blockIdx.x, nt - 1, a, b);
+        Ib = a < kb ? a : a + 1;
+        Jb = b < kb ? b : b + 1;
+        if (!flags[Ib] || !flags[Jb]) return;
+    }
+    __shared__ float Xs[KC][TB];
+    __shared__ float Ys[KC][TB];
+    const int tid = threadIdx.x;
+    const int ty = tid >> 4, tx = tid & 15;
+    float acc[8][8];
+    float *Cb = D + (long long)Ib * TB * ld + (long long)Jb * TB;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int row = (r < 4) ? (ty * 4 + r) : (64 + ty * 4 + r - 4);
+        const float4 lo = *reinterpret_cast<const float4 *>(Cb + (long long)row * ld + tx * 4);
+        const float4 hi = *reinterpret_cast<const float4 *>(Cb + (long long)row * ld + 64 + tx * 4);
+        acc[r][0] = lo.x;
+        acc[r][1] = lo.y;
+        acc[r][2] = lo.z;
+        acc[r][3] = lo.w;
+        acc[r][4] = hi.x;
+        acc[r][5] = hi.y;
+        acc[r][6] = hi.z;
+        acc[r][7] = hi.w;
+    }
+    const float *Xg = D + (long long)kb * TB * ld + (long long)Ib * TB;
+    const float *Yg = D + (long long)kb * TB * ld + (long long)Jb * TB;
+    for (int kc = 0; kc < TB; kc += KC) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int idx = tid + q * 256;
+            const int kk = idx >> 5, c4 = idx & 31;
+            *reinterpret_cast<float4 *>(&Xs[kk][c4 * 4]) =
+                *reinterpret_cast<const float4 *>(Xg + (long long)(kc + kk) * ld + c4 * 4);
+            *reinterpret_cast<float4 *>(&Ys[kk][c4 * 4]) =
+                *reinterpret_cast<const float4 *>(Yg + (long long)(kc + kk) * ld + c4 * 4);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < KC; kk += 2) {
+            const float4 xa0 = *reinterpret_cast<const float4 *>(&Xs[kk][ty * 4]);
+            const float4 xa1 = *reinterpret_cast<const float4 *>(&Xs[kk][64 + ty * 4]);
+            const float4 yb0 = *reinterpret_cast<const float4 *>(&Ys[kk][tx * 4]);
+            const float4 yb1 = *reinterpret_cast<const float4 *>(&Ys[kk][64 + tx * 4]);
+            const float4 xc0 = *reinterpret_cast<const float4 *>(&Xs[kk + 1][ty * 4]);
+            const float4 xc1 = *reinterpret_cast<const float4 *>(&Xs[kk + 1][64 + ty * 4]);
+            const float4 yd0 = *reinterpret_cast<const float4 *>(&Ys[kk + 1][tx * 4]);
+            const float4 yd1 = *reinterpret_cast<const float4 *>(&Ys[kk + 1][64 + tx * 4]);
+            const float a[8] = {xa0.x, xa0.y, xa0.z, xa0.w, xa1.x, xa1.y, xa1.z, xa1.w};
+            const float b[8] = {yb0.x, yb0.y, yb0.z, yb0.w, yb1.x, yb1.y, yb1.z, yb1.w};
+            const float c[8] = {xc0.x, xc0.y, xc0.z, xc0.w, xc1.x, xc1.y, xc1.z, xc1.w};
+            const float d[8] = {yd0.x, yd0.y, yd0.z, yd0.w, yd1.x, yd1.y, yd1.z, yd1.w};
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+#pragma unroll
+                for (int s = 0; s < 8; ++s) acc[r][s] = fminf(acc[r][s], fminf(a[r] + b[s], c[r] + d[s]));
+        }
+    }
+    // write C
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int row = (r < 4) ? (ty * 4 + r) : (64 + ty * 4 + r - 4);
+        *reinterpret_cast<float4 *>(Cb + (long long)row * ld + tx * 4) =
+            make_float4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+        *reinterpret_cast<float4 *>(Cb + (long long)row * ld + 64 + tx * 4) =
+            make_float4(acc[r][4], acc[r][5], acc[r][6], acc[r][7]);
+    }
+    if (PANEL || Ib != Jb) {
+        // transpose into tile (Jb, Ib)
+        float *Tb = D + (long long)Jb * TB * ld + (long long)Ib * TB;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int col = (s < 4) ? (tx * 4 + s) : (64 + tx * 4 + s - 4);
+            *reinterpret_cast<float4 *>(Tb + (long long)col * ld + ty * 4) =
+                make_float4(acc[0][s], acc[1][s], acc[2][s], acc[3][s]);
+            *reinterpret_cast<float4 *>(Tb + (long long)col * ld + 64 + ty * 4) =
+                make_float4(acc[4][s], acc[5][s], acc[6][s], acc[7][s]);
+        }
+    }
+    if (PANEL) {
+        int finite = 0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int s = 0; s < 8; ++s) finite |= (acc[r][s] < INFINITY) ? 1 : 0;
+        finite = __syncthreads_or(finite);
+        if (tid == 0) flags[Jb] = finite;
+    }
+}
+
+int32_t fw_solve_impl(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld) {
+    LG_CHECK_ARG(ld % TB == 0 && ld >= n && n >= 0, "fw: ld=%d must be a multiple of %d and >= n=%d", ld, TB, n);
+    if (n <= 1) return LG_OK;
+    const int nt = ld / TB;
+    void *s = nullptr;
+    LG_TRY(scratch(ctx, (size_t)nt * sizeof(int), &s));
+    int *flags = (int *)s;
+    const int T = nt - 1;
+    const long long P = (long long)T * (T + 1) / 2;
+    LG_CHECK_ARG(P < (1LL << 31), "fw: matrix too large");
+    for (int kb = 0; kb < nt; ++kb) {
+        hipEvent_t e1, e2, e3;
+        ktimer_begin(ctx, "k_fw_pivot", &e1);
+        k_fw_pivot<<<1, 1024, 0, ctx->stream>>>(d_D, ld, kb);
+        LG_LAUNCHED("k_fw_pivot");
+        ktimer_end(ctx, "k_fw_pivot", e1);
+        if (nt == 1) break;
+        ktimer_begin(ctx, "k_fw_panel", &e2);
+        k_fw_minplus<true><<<nt - 1, 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
+        LG_LAUNCHED("k_fw_panel");
+        ktimer_end(ctx, "k_fw_panel", e2);
+        ktimer_begin(ctx, "k_fw_minplus", &e3);
+        k_fw_minplus<false><<<(unsigned)P, 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
+        LG_LAUNCHED("k_fw_minplus");
+        ktimer_end(ctx, "k_fw_minplus", e3);
+    }
+    return LG_OK;
+}
+
+// ------------------------------------------------------------ predecessors ----
+__global__ void k_adj_degree(const int32_t *__restrict__ edges, int ne, int32_t *__restrict__ deg) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    atomicAdd(&deg[edges[2 * e]], 1);
+    atomicAdd(&deg[edges[2 * e + 1]], 1);
+}
+
+__global__ void k_adj_fill(const int32_t *__restrict__ edges, const float *__restrict__ w, int ne,
+                           const int32_t *__restrict__ off, int32_t *__restrict__ cursor, int32_t *__restrict__ nb,
+                           float *__restrict__ nw) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+    const int i = edges[2 * e], j = edges[2 * e + 1];
+    const float we = w[e];
+    int p = off[i] + atomicAdd(&cursor[i], 1);
+    nb[p] = j;
+    nw[p] = we;
+    p = off[j] + atomicAdd(&cursor[j], 1);
+    nb[p] = i;
+    nw[p] = we;
+}
+
+__global__ void k_adj_sort(int n, const int32_t *__restrict__ off, int32_t *__restrict__ nb, float *__restrict__ nw) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n) return;
+    const int s = off[v], e = off[v + 1];
+    for (int a = s + 1; a < e; ++a) {
+        const int kn = nb[a];
+        const float kw = nw[a];
+        int b = a - 1;
+        while (b >= s && nb[b] > kn) {
+            nb[b + 1] = nb[b];
+            nw[b + 1] = nw[b];
+            --b;
+        }
+        nb[b + 1] = kn;
+        nw[b + 1] = kw;
+    }
+}
+
+// pred[i][j] = smallest neighbour k of j minimising D[i][k] + w(k,j).
+__global__ void k_pred(int n, int ld, const float *__restrict__ D, const int32_t *__restrict__ off,
+                       const int32_t *__restrict__ nb, const float *__restrict__ nw, int32_t *__restrict__ pred) {
+    const int i = blockIdx.x;
+    const float *row = D + (long long)i * ld;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+        int best_k = -1;
+        if (j == i) {
+            best_k = i;
+        } else if (row[j] < INFINITY) {
+            float best = INFINITY;
+            for (int p = off[j]; p < off[j + 1]; ++p) {
+                const int k = nb[p];
+                const float v = row[k] + nw[p];
+                if (v < best) {
+                    best = v;
+                    best_k = k;
+                }
+            }
+        }
+        pred[(long long)i * ld + j] = best_k;
+    }
+}
+
+__global__ void k_scan_small(const int32_t *__restrict__ in, int n, int32_t *__restrict__ out) {
+    // exclusive scan, single thread block of 1024 (n up to a few 1e5)
+    __shared__ int32_t part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int s = t * per, e = min(n, s + per);
+    int32_t acc = 0;
+    for (int k = s; k < e; ++k) acc += in[k];
+    part[t] = acc;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int32_t v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int32_t run = t ? part[t - 1] : 0;
+    for (int k = s; k < e; ++k) {
+        out[k] = run;
+        run += in[k];
+    }
+    if (t == 1023) out[n] = part[1023];
+}
+
+int32_t fw_pred_impl(lg_ctx *ctx, int32_t n, const float *d_D, int32_t ld, int32_t *d_pred, const int32_t *d_edges,
+                     const float *d_w, int32_t ne) {
+    int32_t *deg = nullptr, *off = nullptr, *cur = nullptr, *nb = nullptr;
+    float *nw = nullptr;
+    int32_t rc = dalloc(ctx, (size_t)n + 1, &deg);
+    if (rc == LG_OK) rc = dalloc(ctx, (size_t)n + 1, &off);
+    if (rc == LG_OK) rc = dalloc(ctx, (size_t)n + 1, &cur);
+    if (rc == LG_OK) rc = dalloc(ctx, (size_t)2 * ne + 1, &nb);
+    if (rc == LG_OK) rc = dalloc(ctx, (size_t)2 * ne + 1, &nw);
+    if (rc == LG_OK) {
+        (void)hipMemsetAsync(deg, 0, ((size_t)n + 1) * 4, ctx->stream);
+        (void)hipMemsetAsync(cur, 0, ((size_t)n + 1) * 4, ctx->stream);
+        if (ne > 0) k_adj_degree<<<(ne + 255) / 256, 256, 0, ctx->stream>>>(d_edges, ne, deg);
+        k_scan_small<<<1, 1024, 0, ctx->stream>>>(deg, n, off);
+        if (ne > 0) k_adj_fill<<<(ne + 255) / 256, 256, 0, ctx->stream>>>(d_edges, d_w, ne, off, cur, nb, nw);
+        k_adj_sort<<<(n + 255) / 256, 256, 0, ctx->stream>>>(n, off, nb, nw);
+        k_pred<<<n, 256, 0, ctx->stream>>>(n, ld, d_D, off, nb, nw, d_pred);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("predecessor kernels failed to launch");
+            rc = LG_ERR_HIP;
+        }
+    }
+    dfree(ctx, deg);
+    dfree(ctx, off);
+    dfree(ctx, cur);
+    dfree(ctx, nb);
+    dfree(ctx, nw);
+    return rc;
+}
+
+}  // namespace lg
+
+using namespace lg;
+
+extern "C" int32_t lg_fw_solve(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld, int32_t *d_pred,
+                               const int32_t *d_edges, const float *d_w, int32_t n_edges) {
+    LG_CHECK_ARG(ctx && d_D, "lg_fw_solve: NULL argument");
+    LG_CHECK_ARG(ld == fw_ld(n), "lg_fw_solve: ld must be lg_fw_ld(n)=%d, got %d", fw_ld(n), ld);
+    LG_TRY(fw_solve_impl(ctx, n, d_D, ld));
+    if (d_pred) {
+        LG_CHECK_ARG(n_edges >= 0 && (n_edges == 0 || (d_edges && d_w)), "lg_fw_solve: predecessors need edges and weights");
+        LG_TRY(fw_pred_impl(ctx, n, d_D, ld, d_pred, d_edges, d_w, n_edges));
+    }
+    return LG_OK;
+}

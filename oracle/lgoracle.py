@@ -1,0 +1,249 @@
+"""ctypes binding of the CPU oracle (oracle/liblgoracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the timed CPU baseline -- never
+by the product path.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblgoracle.so")
+_LIB = None
+
+P = C.c_void_p
+I32 = C.c_int32
+U32 = C.c_uint32
+U64 = C.c_uint64
+I64 = C.c_int64
+D = C.c_double
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "lgo_philox4x32": (None, [P, P, P]),
+            "lgo_mt19937_floats": (None, [U32, I64, P]),
+            "lgo_mt19937_nth": (U32, [U32, I64]),
+            "lgo_box_sample": (None, [P, P, I64, P, P]),
+            "lgo_spans": (I32, [P, P, P, P, D, D, I32, P]),
+            "lgo_sample_points": (I32, [P, P, U64, U32, I32, P]),
+            "lgo_build_edges": (I32, [P, I32, D, I32, P, I32]),
+            "lgo_march_edges": (None, [P, P, P, P, P, P, I32, P, P]),
+            "lgo_vertex_light": (None, [P, P, P, P, P, P, I32, P, P, P, U64, U32, P, I32, P]),
+            "lgo_floyd_maxprod": (None, [I32, P]),
+            "lgo_floyd_minplus": (None, [I32, I32, P]),
+            "lgo_floyd_minplus_f32": (None, [I32, I32, P]),
+            "lgo_predecessors": (None, [I32, I32, P, P, P, I32, P]),
+            "lgo_diffuse": (None, [I32, P, P, P]),
+            "lgo_solve_points": (I32, [P, P, P, P, P, P, I32, P, P, P, U64, U32, P, I32, D, I32, P, I32, P, P, P]),
+            "lgo_dart_throw": (I32, [P, P, U32, U32, U32, P, P, I32]),
+            "lgo_env_cdf": (None, [I32, I32, P, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data if a is not None else None
+
+
+def _desc(dense, voxel_size=1.0, origin=(0.0, 0.0, 0.0)):
+    from lightgraph_amd._abi import lg_grid_desc
+    d = lg_grid_desc()
+    d.nz, d.ny, d.nx = dense.shape
+    d.dtype = 0
+    d.voxel_size = voxel_size
+    for c in range(3):
+        d.origin[c] = origin[c]
+    return d
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.uint32)
+    lib().lgo_philox4x32(_p(c), _p(k), _p(out))
+    return out
+
+
+def mt19937_floats(seed, count):
+    out = np.zeros(count, dtype=np.float32)
+    lib().lgo_mt19937_floats(seed, count, _p(out))
+    return out
+
+
+def mt19937_nth(seed, nth):
+    return lib().lgo_mt19937_nth(seed, nth)
+
+
+def box_sample(dense, xyz, voxel_size=1.0, origin=(0, 0, 0)):
+    dense = np.ascontiguousarray(dense, dtype=np.float32)
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    out = np.zeros(len(xyz), dtype=np.float32)
+    d = _desc(dense, voxel_size, origin)
+    lib().lgo_box_sample(C.byref(d), _p(dense), len(xyz), _p(xyz), _p(out))
+    return out
+
+
+def spans(dense, eye, direction, t0=1e-9, t1=1.7976931348623157e308, cap=256, voxel_size=1.0, origin=(0, 0, 0)):
+    dense = np.ascontiguousarray(dense, dtype=np.float32)
+    d = _desc(dense, voxel_size, origin)
+    e = np.asarray(eye, dtype=np.float64)
+    v = np.asarray(direction, dtype=np.float64)
+    out = np.zeros((cap, 2), dtype=np.float64)
+    k = lib().lgo_spans(C.byref(d), _p(dense), _p(e), _p(v), t0, t1, cap, _p(out))
+    if k < 0:
+        return None
+    return out[:min(k, cap)].copy()
+
+
+def sample_points(dense, seed, solve, n, voxel_size=1.0, origin=(0, 0, 0)):
+    dense = np.ascontiguousarray(dense, dtype=np.float32)
+    d = _desc(dense, voxel_size, origin)
+    out = np.zeros((n, 3), dtype=np.float32)
+    rc = lib().lgo_sample_points(C.byref(d), _p(dense), seed, solve, n, _p(out))
+    if rc != 0:
+        raise ValueError("grid has no positive density")
+    return out
+
+
+def build_edges(pts, radius, cap):
+    pts = np.ascontiguousarray(pts, dtype=np.float32)
+    n = len(pts)
+    max_e = n * (n - 1) // 2 if cap <= 0 else n * cap
+    out = np.zeros((max(1, max_e), 2), dtype=np.int32)
+    ne = lib().lgo_build_edges(_p(pts), n, radius, cap, _p(out), len(out))
+    return out[:ne].copy()
+
+
+def march_edges(dense, shader, params, pts, edges, voxel_size=1.0, origin=(0, 0, 0)):
+    dense = np.ascontiguousarray(dense, dtype=np.float32)
+    pts = np.ascontiguousarray(pts, dtype=np.float32)
+    edges = np.ascontiguousarray(edges, dtype=np.int32)
+    d = _desc(dense, voxel_size, origin)
+    T = np.zeros((len(edges), 3), dtype=np.float64)
+    tau = np.zeros(len(edges), dtype=np.float64)
+    lib().lgo_march_edges(C.byref(d), _p(dense), C.byref(shader), C.byref(params), _p(pts), _p(edges),
+                          len(edges), _p(T), _p(tau))
+    return T, tau
+
+
+def _light_args(lights, envs):
+    from lightgraph_amd._abi import lg_light
+    arr = (lg_light * max(1, len(lights)))()
+    for k, l in enumerate(lights):
+        arr[k] = l
+    envs = envs or [None] * len(lights)
+    rgb = (C.c_void_p * max(1, len(lights)))()
+    w = (C.c_int32 * max(1, len(lights)))()
+    h = (C.c_int32 * max(1, len(lights)))()
+    keep = []
+    for k, e in enumerate(envs):
+        if e is not None:
+            a = np.ascontiguousarray(e, dtype=np.float32)
+            keep.append(a)
+            rgb[k] = a.ctypes.data
+            h[k], w[k] = a.shape[0], a.shape[1]
+    return arr, rgb, w, h, keep
+
+
+def vertex_light(dense, shader, params, lights, pts, solve=0, env_seed=4564446, temp_dense=None, envs=None,
+                 voxel_size=1.0, origin=(0, 0, 0)):
+    dense = np.ascontiguousarray(dense, dtype=np.float32)
+    tdense = np.ascontiguousarray(temp_dense, dtype=np.float32) if temp_dense is not None else None
+    pts = np.ascontiguousarray(pts, dtype=np.float32)
+    d = _desc(dense, voxel_size, origin)
+    arr, rgb, w, h, keep = _light_args(lights, envs)
+    out = np.zeros((len(pts), 3), dtype=np.float64)
+    lib().lgo_vertex_light(C.byref(d), _p(dense), _p(tdense), C.byref(shader), C.byref(params), arr, len(lights),
+                           rgb, w, h, env_seed, solve, _p(pts), len(pts), _p(out))
+    return out
+
+
+def floyd_maxprod(P):
+    P = np.ascontiguousarray(P, dtype=np.float64).copy()
+    lib().lgo_floyd_maxprod(P.shape[0], _p(P))
+    return P
+
+
+def floyd_minplus(Dm):
+    Dm = np.ascontiguousarray(Dm, dtype=np.float64).copy()
+    lib().lgo_floyd_minplus(Dm.shape[0], Dm.shape[1], _p(Dm))
+    return Dm
+
+
+def floyd_minplus_f32(Dm):
+    Dm = np.ascontiguousarray(Dm, dtype=np.float32).copy()
+    lib().lgo_floyd_minplus_f32(Dm.shape[0], Dm.shape[1], _p(Dm))
+    return Dm
+
+
+def predecessors(Dm, edges, w):
+    Dm = np.ascontiguousarray(Dm, dtype=np.float64)
+    edges = np.ascontiguousarray(edges, dtype=np.int32)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    out = np.zeros(Dm.shape, dtype=np.int32)
+    lib().lgo_predecessors(Dm.shape[0], Dm.shape[1], _p(Dm), _p(edges), _p(w), len(edges), _p(out))
+    return out
+
+
+def diffuse(P, L):
+    P = np.ascontiguousarray(P, dtype=np.float64)
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    out = np.zeros((P.shape[0], 3), dtype=np.float64)
+    lib().lgo_diffuse(P.shape[0], _p(P), _p(L), _p(out))
+    return out
+
+
+def solve_points(dense, shader, params, lights, pts, radius, cap=12, solve=0, env_seed=4564446, temp_dense=None,
+                 envs=None, want_P=False, voxel_size=1.0, origin=(0, 0, 0)):
+    dense = np.ascontiguousarray(dense, dtype=np.float32)
+    tdense = np.ascontiguousarray(temp_dense, dtype=np.float32) if temp_dense is not None else None
+    pts = np.ascontiguousarray(pts, dtype=np.float32)
+    n = len(pts)
+    d = _desc(dense, voxel_size, origin)
+    arr, rgb, w, h, keep = _light_args(lights, envs)
+    cap_e = n * max(cap, 1) if cap > 0 else n * (n - 1) // 2
+    edges = np.zeros((max(1, cap_e), 2), dtype=np.int32)
+    P = np.zeros((n, n, 3), dtype=np.float64) if want_P else None
+    L = np.zeros((n, 3), dtype=np.float64)
+    dif = np.zeros((n, 3), dtype=np.float64)
+    ne = lib().lgo_solve_points(C.byref(d), _p(dense), _p(tdense), C.byref(shader), C.byref(params), arr,
+                                len(lights), rgb, w, h, env_seed, solve, _p(pts), n, radius, cap, _p(edges),
+                                len(edges), _p(P), _p(L), _p(dif))
+    return {"edges": edges[:ne].copy(), "P": P, "L": L, "diffuse": dif}
+
+
+def dart_throw(dense, iteration, scatter_count=1000000, target=750, min_dist=None, voxel_size=1.0,
+               origin=(0, 0, 0), cap=100000):
+    dense = np.ascontiguousarray(dense, dtype=np.float32)
+    d = _desc(dense, voxel_size, origin)
+    md = C.c_float(min_dist if min_dist is not None else 0.0)
+    out = np.zeros((cap, 3), dtype=np.float32)
+    n = lib().lgo_dart_throw(C.byref(d), _p(dense), iteration, scatter_count, target, C.byref(md), _p(out), cap)
+    return out[:min(n, cap)].copy(), md.value
+
+
+def env_cdf(rgb):
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w = rgb.shape[:2]
+    cond = np.zeros((h, w + 1), dtype=np.float64)
+    marg = np.zeros(h + 1, dtype=np.float64)
+    lib().lgo_env_cdf(w, h, _p(rgb), _p(cond), _p(marg))
+    return cond, marg

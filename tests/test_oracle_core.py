@@ -1,0 +1,297 @@
+"""Pins of the CPU oracle against known answers, closed forms and brute force.
+
+The reference has no tests or fixtures (SURVEY: none exist), so every check here
+comes from a published KAT, a library result (SciPy), a closed form or an
+exhaustive enumeration on a tiny case.  CPU only.
+"""
+import itertools
+import os
+
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "fw_golden.npz")
+
+
+# ---------------------------------------------------------------- RNG --------
+def test_philox_random123_kats(oracle):
+    # Random123 kat_vectors, philox4x32_10
+    kats = [((0, 0, 0, 0), (0, 0), (0x6627e8d5, 0xe169c58d, 0xbc57ac4c, 0x9b00dbd8)),
+            ((0xffffffff,) * 4, (0xffffffff, 0xffffffff), (0x408f276d, 0x41c83b0e, 0xa20bc7c6, 0x6d5451fd)),
+            ((0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344), (0xa4093822, 0x299f31d0),
+             (0xd16cfe09, 0x94fdcceb, 0x5001e420, 0x24126ea1))]
+    for ctr, key, want in kats:
+        assert tuple(int(x) for x in oracle.philox(ctr, key)) == want
+
+
+def test_mt19937_kats(oracle):
+    # C++ standard [rand.predef]: 10000th output of a default-constructed mt19937
+    assert oracle.mt19937_nth(5489, 10000) == 4123659995
+    # libstdc++ uniform_real_distribution<float> at the reference's first engine seed
+    got = oracle.mt19937_floats(854123, 4)
+    np.testing.assert_array_equal(got, np.float32([0.554893553, 0.638520598, 0.235635638, 0.0805947259]))
+
+
+# ------------------------------------------------------------ sampler --------
+def _affine_grid(n=12):
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    return (0.25 + 0.5 * x + 0.125 * y + 0.0625 * z).astype(np.float32)
+
+
+def test_box_sample_exact_on_affine_field(oracle):
+    g = _affine_grid()
+    rng = np.random.default_rng(1)
+    xyz = rng.uniform(0.0, 10.99, size=(500, 3))
+    got = oracle.box_sample(g, xyz)
+    want = 0.25 + 0.5 * xyz[:, 0] + 0.125 * xyz[:, 1] + 0.0625 * xyz[:, 2]
+    assert rel_err(got, want) < 1e-6
+
+
+def test_box_sample_lattice_and_background(oracle):
+    g = _affine_grid()
+    pts = np.array([[3, 4, 5], [0, 0, 0], [11, 11, 11]], dtype=np.float64)
+    np.testing.assert_array_equal(oracle.box_sample(g, pts), g[pts[:, 2].astype(int), pts[:, 1].astype(int),
+                                                                 pts[:, 0].astype(int)])
+    # half-way past the last voxel the background (0) enters with weight 1/2
+    v = oracle.box_sample(g, np.array([[11.5, 2.0, 2.0]]))[0]
+    assert v == pytest.approx(0.5 * g[2, 2, 11], rel=1e-6)
+    assert oracle.box_sample(g, np.array([[-3.0, 2.0, 2.0], [40.0, 1.0, 1.0]])).tolist() == [0.0, 0.0]
+
+
+def test_box_sample_weight_order(oracle):
+    # a field varying only in z: the first lerp (innermost) must be along z
+    g = np.zeros((4, 4, 4), np.float32)
+    g[2, :, :] = 1.0
+    v = oracle.box_sample(g, np.array([[1.0, 1.0, 1.25], [1.0, 1.0, 1.75]]))
+    np.testing.assert_allclose(v, [0.25, 0.75], rtol=1e-7)
+
+
+# -------------------------------------------------------------- spans --------
+def test_spans_alternating_bricks(oracle):
+    g = np.zeros((8, 8, 32), np.float32)
+    g[:, :, 0:8] = 1.0
+    g[:, :, 16:24] = 1.0
+    s = oracle.spans(g, (-4.0, 4.5, 4.5), (1.0, 0.0, 0.0))
+    np.testing.assert_allclose(s, [[4.0, 12.0], [20.0, 28.0]], atol=1e-9)
+
+
+def test_spans_miss_and_inside_start(oracle):
+    g = np.zeros((16, 16, 16), np.float32)
+    g[8:, 8:, 8:] = 1.0  # one brick [8,16)^3
+    assert oracle.spans(g, (-4.0, 0.5, 0.5), (1.0, 0.0, 0.0)) is None  # bbox [8,16]^3 missed
+    s = oracle.spans(g, (10.0, 10.0, 10.0), (0.0, 0.0, 1.0))
+    np.testing.assert_allclose(s, [[1e-9, 6.0]], atol=1e-12)
+
+
+def test_spans_diagonal_single_brick_time(oracle):
+    g = np.zeros((8, 8, 8), np.float32)
+    g[3, 3, 3] = 1.0
+    d = np.array([1.0, 1.0, 1.0]) / np.sqrt(3.0)
+    s = oracle.spans(g, (-1.0, -1.0, -1.0), d)
+    # enters the [0,8]^3 box at t = sqrt(3), leaves at t = 9*sqrt(3)
+    np.testing.assert_allclose(s, [[np.sqrt(3.0), 9 * np.sqrt(3.0)]], rtol=1e-12)
+
+
+# ------------------------------------------------------------ marches --------
+def _shader(lga, **kw):
+    return lga.default_shader(**kw)
+
+
+def test_edge_march_constant_density_closed_form(oracle, lga):
+    rho0 = 0.1
+    g = np.full((32, 32, 32), rho0, np.float32)
+    sh = _shader(lga)
+    p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5)
+    pts = np.array([[5.3, 6.1, 7.7], [20.2, 18.9, 11.4], [9.0, 9.5, 25.25]], np.float32)
+    edges = np.array([[0, 1], [0, 2]], np.int32)
+    T, tau = oracle.march_edges(g, sh, p, pts, edges)
+    m = max(float(np.float32(rho0)) * 1.0 * 1.0 * 0.35, 0.025)
+    for e, (i, j) in enumerate(edges):
+        dist = float(np.float32(np.linalg.norm(pts[j].astype(np.float64) - pts[i].astype(np.float64))))
+        count = int(np.floor(dist / 0.5))
+        want = np.exp(np.array(sh.extinction) * m * 0.5 * count)
+        assert rel_err(T[e], want) < 1e-12
+        assert tau[e] == pytest.approx(count * m * 0.5, rel=1e-12)
+
+
+def test_edge_march_scatter_floor_and_dropout(oracle, lga):
+    g = np.zeros((32, 32, 32), np.float32)
+    g[0:8, 0:8, 0:8] = 1e-6  # one brick exists but the density is below the cutoff
+    sh = _shader(lga)
+    p = lga.march_params(shadow_step=0.5)
+    pts = np.array([[1.0, 1.0, 1.0], [6.0, 1.0, 1.0]], np.float32)
+    T, tau = oracle.march_edges(g, sh, p, pts, np.array([[0, 1]], np.int32))
+    # every sample takes the scatterDensityMin floor (LightGraph.h:390-395): 10 samples
+    assert tau[0] == pytest.approx(10 * 0.025 * 0.5, rel=1e-12)
+    # dense medium: the dropout |T|^2 < cutoff zeroes the edge (LightGraph.h:396-400)
+    g2 = np.full((32, 32, 32), 1.0, np.float32)
+    sh2 = _shader(lga, scatter_density_scale=1.0)
+    T2, tau2 = oracle.march_edges(g2, sh2, p, np.array([[2, 2, 2], [29, 2, 2]], np.float32), np.array([[0, 1]], np.int32))
+    assert np.all(T2[0] == 0.0) and np.isinf(tau2[0])
+
+
+def test_edge_march_dropout_threshold(oracle, lga):
+    # the last sample count before |T|^2 < cutoff, from the closed form
+    g = np.full((64, 8, 8), 1.0, np.float32)
+    sh = _shader(lga, scatter_density_scale=1.0)
+    p = lga.march_params(cutoff=0.005, shadow_step=0.5)
+    ext = np.array(sh.extinction)
+    k_drop = next(k for k in range(1, 1000) if np.sum(np.exp(2 * ext * 0.5 * k)) < 0.005)
+    for k in (k_drop - 1, k_drop):
+        pts = np.array([[2.0, 4.0, 4.0], [2.0 + 0.5 * k + 0.25, 4.0, 4.0]], np.float32)
+        T, tau = oracle.march_edges(g, sh, p, pts, np.array([[0, 1]], np.int32))
+        if k < k_drop:
+            assert rel_err(T[0], np.exp(ext * 0.5 * k)) < 1e-12
+        else:
+            assert np.all(T[0] == 0.0)
+
+
+def test_vertex_light_directional_closed_form(oracle, lga):
+    rho0 = 0.05
+    g = np.zeros((40, 40, 40), np.float32)
+    g[:32, :32, :32] = rho0
+    sh = _shader(lga)
+    p = lga.march_params(cutoff=0.005, shadow_step=1.0)
+    light = lga.lg_light()
+    light.type, light.samples, light.intensity = 0, 1, 2.0
+    light.color[:] = [1.0, 0.5, 0.25]
+    light.dir[:] = [0.0, -1.0, 0.0]  # shadow rays go +y
+    pts = np.array([[10.0, 3.0, 12.0]], np.float32)
+    L = oracle.vertex_light(g, sh, p, [light], pts)
+    # samples at y = 4..32 (t <= 29); y = 32 reads the zero background and is skipped
+    dens = float(np.float32(rho0)) * 1.0 * 1.0 * 0.35
+    T = np.exp(np.array(sh.extinction) * dens * 1.0 * 28)
+    want = np.array(sh.albedo) * np.array([1.0, 0.5, 0.25]) * 2.0 * T
+    assert rel_err(L[0], want) < 1e-12
+
+
+# --------------------------------------------------------------- graph -------
+def test_graph_greedy_cap_on_a_clique(oracle):
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(0.0, 0.1, size=(20, 3)).astype(np.float32)
+    e = oracle.build_edges(pts, radius=10.0, cap=12)
+    # rows 0..12 close a K13, rows 13..19 a K7 (LightGraph.h:297-310)
+    want = [(i, j) for i in range(13) for j in range(i + 1, 13)] + \
+           [(i, j) for i in range(13, 20) for j in range(i + 1, 20)]
+    assert [tuple(x) for x in e] == want
+    assert len(oracle.build_edges(pts, radius=10.0, cap=0)) == 190
+
+
+def test_graph_radius_is_strict_and_float(oracle):
+    pts = np.array([[0, 0, 0], [1, 0, 0], [2.5, 0, 0], [3.5, 0, 0]], np.float32)
+    e = oracle.build_edges(pts, radius=1.0, cap=12)  # |d|^2 < r^2 is strict
+    assert len(e) == 0
+    e = oracle.build_edges(pts, radius=1.5, cap=12)
+    assert [tuple(x) for x in e] == [(0, 1), (2, 3)]
+
+
+# ----------------------------------------------------------------- FW --------
+def _brute_force(W):
+    n = len(W)
+    best = np.full((n, n), np.inf)
+    for i in range(n):
+        best[i, i] = 0.0
+        for r in range(1, n):
+            for mid in itertools.permutations([k for k in range(n) if k != i], r):
+                cost, prev = 0.0, i
+                for v in mid:
+                    cost += W[prev, v]
+                    prev = v
+                j = mid[-1]
+                best[i, j] = min(best[i, j], cost)
+    return best
+
+
+def test_fw_brute_force_tiny(oracle):
+    rng = np.random.default_rng(5)
+    for n in (2, 4, 6):
+        W = np.full((n, n), np.inf)
+        np.fill_diagonal(W, 0.0)
+        for i in range(n):
+            for j in range(i + 1, n):
+                if rng.uniform() < 0.6:
+                    W[i, j] = W[j, i] = float(rng.integers(1, 20))
+        np.testing.assert_array_equal(oracle.floyd_minplus(W), _brute_force(W))
+
+
+def test_fw_scipy_golden(oracle):
+    gold = np.load(GOLDEN)
+    for name in ("int16", "int64", "int200", "flt64", "sparse130"):
+        W, Dm = gold[name + "_W"], gold[name + "_D"]
+        got = oracle.floyd_minplus(W)
+        if name.startswith("flt"):
+            assert rel_err(got[np.isfinite(Dm)], Dm[np.isfinite(Dm)]) < 1e-12
+            assert np.array_equal(np.isinf(got), np.isinf(Dm))
+        else:
+            np.testing.assert_array_equal(got, Dm)
+
+
+def test_fw_maxproduct_equals_minplus(oracle):
+    # The reference's (max, x) FW on per-channel transmittance (LightGraph.h:516-534)
+    # equals (min, +) on tau when T_c = exp(-sigma_c tau).
+    gold = np.load(GOLDEN)
+    W = gold["int64_W"] * 0.01
+    sigma = np.array([1.6, 1.5, 1.4])
+    P = np.where(np.isfinite(W)[..., None], np.exp(-W[..., None] * sigma), 0.0)
+    for i in range(len(W)):
+        P[i, i] = 0.0  # the reference starts with no self loops
+    Pc = oracle.floyd_maxprod(P)
+    Dm = oracle.floyd_minplus(W)
+    want = np.where(np.isfinite(Dm)[..., None], np.exp(-Dm[..., None] * sigma), 0.0)
+    assert rel_err(Pc, want, floor=1e-300) < 1e-12
+
+
+def test_predecessors_tie_break(oracle):
+    # square 0-1-3 / 0-2-3 with equal weights: the canonical predecessor of 3 from 0 is 1
+    edges = np.array([[0, 1], [0, 2], [1, 3], [2, 3]], np.int32)
+    w = np.array([1.0, 1.0, 2.0, 2.0])
+    W = np.full((4, 4), np.inf)
+    np.fill_diagonal(W, 0)
+    for (i, j), x in zip(edges, w):
+        W[i, j] = W[j, i] = x
+    Dm = oracle.floyd_minplus(W)
+    pr = oracle.predecessors(Dm, edges, w)
+    assert pr[0, 3] == 1 and pr[3, 0] == 1 and pr[0, 0] == 0 and pr[1, 2] == 0
+
+
+# ------------------------------------------------------------ diffusion -----
+def test_diffuse_closed_forms(oracle):
+    L = np.array([[1.0, 1.0, 1.0], [2.0, 2.0, 2.0], [4.0, 4.0, 4.0]])
+    # no edges: P = I, columns sum to 1 -> out = L
+    P = np.zeros((3, 3, 3))
+    for i in range(3):
+        P[i, i] = 1.0
+    np.testing.assert_allclose(oracle.diffuse(P, L), L, rtol=1e-15)
+    # fully transparent complete graph: every vertex gets the mean
+    np.testing.assert_allclose(oracle.diffuse(np.ones((3, 3, 3)), L), np.full((3, 3), 7.0 / 3.0), rtol=1e-15)
+    # path 0-1-2, T01 = 0.5, T12 = 0.25: columns sums 1.625, 1.75, 1.375 (LightGraph.h:562-579)
+    a, b = 0.5, 0.25
+    T = np.array([[1, a, a * b], [a, 1, b], [a * b, b, 1]])
+    P = np.repeat(T[..., None], 3, axis=2)
+    out = oracle.diffuse(P, L)
+    np.testing.assert_allclose(out[:, 0], [1 / 1.625 + 1.0 / 1.75 + 0.5 / 1.375,
+                                           0.5 / 1.625 + 2 / 1.75 + 1.0 / 1.375,
+                                           0.125 / 1.625 + 0.5 / 1.75 + 4 / 1.375], rtol=1e-14)
+
+
+# -------------------------------------------------------------- points -------
+def test_sampler_single_voxel(oracle):
+    g = np.zeros((16, 16, 16), np.float32)
+    g[5, 6, 7] = 0.7  # (z, y, x)
+    pts = oracle.sample_points(g, seed=9, solve=0, n=200)
+    assert np.all(pts >= np.array([7 - 0.5, 6 - 0.5, 5 - 0.5], np.float32))
+    assert np.all(pts < np.array([7 + 0.5, 6 + 0.5, 5 + 0.5], np.float32))
+
+
+def test_sampler_density_weighting(oracle):
+    g = np.zeros((16, 16, 16), np.float32)
+    g[2, 2, 2] = 1.0
+    g[12, 12, 12] = 3.0
+    pts = oracle.sample_points(g, seed=11, solve=3, n=20000)
+    frac = np.mean(pts[:, 0] < 8.0)
+    assert abs(frac - 0.25) < 0.015
+    # distinct solves give distinct streams, the same solve repeats exactly
+    np.testing.assert_array_equal(pts, oracle.sample_points(g, seed=11, solve=3, n=20000))
+    assert not np.array_equal(pts, oracle.sample_points(g, seed=11, solve=4, n=20000))
