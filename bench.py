@@ -1,0 +1,161 @@
+"""LightGraph solves/s on one MI355X.
+
+Workload (the plain method at production size): a 512^3 fp32 cloud made of 20
+seeded ellipsoid blobs (semi-axes 30-120 voxels) times 6-octave fBm, thresholded
+to ~30% active voxels in 8^3 bricks; sigma_s = (1.0, 0.9, 0.8), sigma_a = 0.1; a
+point light and a directional light; N = 8192 density-sampled points per solve,
+edges within 12 voxels (12 per vertex at most), edge march at 0.5 voxel.  A step
+is one LightGraph solve (sample -> edges -> edge march -> all-pairs -> vertex
+light -> diffusion) with the volume resident in HBM.
+
+Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+PEAK_FP32_TFLOPS = 157.3
+
+WORKLOAD = {"workload": "c2_plain_solve", "volume": "512^3 blobs20+fBm6, ~30% active, fp32 bricks",
+            "points": 8192, "edge_radius": 12.0, "max_connect": 12, "edge_step": 0.5,
+            "lights": "point + directional", "sigma_s": [1.0, 0.9, 0.8], "sigma_a": 0.1}
+
+
+def setup(lga, ctx, n_points):
+    spec = lga.scene_spec(lga.LG_SCENE_BLOBS_FBM, 512, seed=2, octaves=6, blobs=20, axis_min=30.0,
+                          axis_max=120.0, active_fraction=0.30)
+    grid, _ = lga.Grid.generate(ctx, spec)
+    shader = lga.default_shader(absorption=0.1, scattering=(1.0, 0.9, 0.8))
+    params = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5)
+    lights = [lga.point_light(0.0, (1.0, 0.9, 0.8), 4.0e5, (256.0, 700.0, 256.0)),
+              lga.parse_light("dir:0:1,0.89,0.79:0.7:-41.8,-180,0")]
+    return spec, grid, shader, params, lights
+
+
+def kernel_roofline(ctx, ld):
+    """Roofline object for the kernel with the largest total time in the timed region."""
+    tiles = ld // 128
+    # algorithmic FP32 ops per launch: one add + one min per (i, j, k) over the
+    # off-pivot block rows/cols of one pivot block (2 * 128 * (ld - 128)^2)
+    table = {
+        "k_fw_minplus": ("valu", 2.0 * 128 * (ld - 128) ** 2, "TFLOP/s", PEAK_FP32_TFLOPS),
+    }
+    best = None
+    for name in ("k_fw_minplus", "k_fw_panel", "k_fw_pivot", "k_march_edges", "k_vertex_light", "k_graph",
+                 "k_sample_points", "k_diffuse"):
+        launches, total = ctx.kernel_time(name)
+        if launches and (best is None or total > best[2]):
+            best = (name, launches, total)
+    if best is None:
+        return None, {}
+    name, launches, total = best
+    avg_s = total / launches / 1e3
+    times = {}
+    for nm in ("k_fw_minplus", "k_fw_panel", "k_fw_pivot", "k_march_edges", "k_vertex_light", "k_graph",
+               "k_sample_points", "k_diffuse"):
+        l, t = ctx.kernel_time(nm)
+        if l:
+            times[nm] = {"launches": l, "avg_ms": round(t / l, 5)}
+    if name in table:
+        bound, work, unit, peak = table[name]
+        achieved = work / avg_s / 1e12
+        return {"kernel": name, "bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
+                "frac": round(achieved / peak, 4), "traffic": None, "avg_launch_ms": round(avg_s * 1e3, 5)}, times
+    return {"kernel": name, "bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
+            "traffic": None, "avg_launch_ms": round(avg_s * 1e3, 5)}, times
+
+
+def cpu_baseline(lga, ctx, grid, shader, params, lights, n_points, radius, budget_s):
+    """The oracle's serial solve (max-product Floyd-Warshall on Vec3 transmittance,
+    as the reference) on the same volume and points, one solve per host thread,
+    as many threads as cores (the reference parallelises across solves)."""
+    import numpy as np
+    from oracle import lgoracle
+    dense = grid.read_dense()
+    cores = min(16, os.cpu_count() or 1)
+    solver = lga.Solver(ctx, n_points)
+    pts = []
+    for s in range(cores):
+        solver.solve(grid, shader, lights, params, n=n_points, radius=radius, seed=854123, solve=1000 + s)
+        pts.append(solver.results(want_D=False)["points"])
+    solver.destroy()
+    done = []
+
+    def one(k):
+        lgoracle.solve_points(dense, shader, params, lights, pts[k], radius=radius, cap=12)
+        done.append(k)
+
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=one, args=(k,)) for k in range(cores)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    dt = time.perf_counter() - t0
+    return {"value": round(len(done) / dt, 4), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} concurrent full oracle solves (N={n_points}, same volume, points and lights), "
+                      f"{dt:.1f} s wall"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--points", type=int, default=8192)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=60.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
+        return 2
+
+    import lightgraph_amd as lga
+    ctx = lga.Context(0)
+    spec, grid, shader, params, lights = setup(lga, ctx, args.points)
+    n = args.points
+    solver = lga.Solver(ctx, n)
+    radius = 12.0
+    for k in range(args.warmup):
+        solver.solve(grid, shader, lights, params, n=n, radius=radius, seed=854123, solve=k)
+    ctx.sync()
+    ctx.kernel_time_reset()
+    ctx.kernel_timing(True)
+    ctx.sync()
+    ctx.mark(0)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        solver.solve(grid, shader, lights, params, n=n, radius=radius, seed=854123, solve=args.warmup + k)
+    ctx.mark(1)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    gpu_ms = ctx.elapsed_ms(0, 1)
+    ctx.kernel_timing(False)
+    ms_per_step = gpu_ms / args.steps
+    roof, times = kernel_roofline(ctx, lga.fw_ld(n))
+    b = solver.buffers()
+    line = {"metric": "LightGraph solves/s", "value": round(1000.0 / ms_per_step, 3), "unit": "solves/s",
+            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
+            "wall_ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32 (fp64 ray/DDA positions)", "data": "synthetic",
+            "config": dict(WORKLOAD, edges_last_solve=b.n_edges), "roofline": roof, "kernels": times}
+    solver.destroy()
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(lga, ctx, grid, shader, params, lights, n, radius, args.cpu_budget)
+    else:
+        line["cpu_baseline"] = None
+    grid.destroy()
+    ctx.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
