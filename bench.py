@@ -20,6 +20,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+KERNELS = ("k_fw_minplus", "k_fw_panel", "k_fw_pivot", "k_march_edges", "k_vertex_light", "k_graph",
+           "k_sample_points", "k_diffuse", "k_components", "k_comp_small")
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
 
@@ -48,8 +50,7 @@ def kernel_roofline(ctx, ld):
         "k_fw_minplus": ("valu", 2.0 * 128 * (ld - 128) ** 2, "TFLOP/s", PEAK_FP32_TFLOPS),
     }
     best = None
-    for name in ("k_fw_minplus", "k_fw_panel", "k_fw_pivot", "k_march_edges", "k_vertex_light", "k_graph",
-                 "k_sample_points", "k_diffuse"):
+    for name in KERNELS:
         launches, total = ctx.kernel_time(name)
         if launches and (best is None or total > best[2]):
             best = (name, launches, total)
@@ -58,8 +59,7 @@ def kernel_roofline(ctx, ld):
     name, launches, total = best
     avg_s = total / launches / 1e3
     times = {}
-    for nm in ("k_fw_minplus", "k_fw_panel", "k_fw_pivot", "k_march_edges", "k_vertex_light", "k_graph",
-               "k_sample_points", "k_diffuse"):
+    for nm in KERNELS:
         l, t = ctx.kernel_time(nm)
         if l:
             times[nm] = {"launches": l, "avg_ms": round(t / l, 5)}

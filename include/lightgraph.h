@@ -165,11 +165,11 @@ typedef struct lg_solve_timing {
 
 /* Device buffers owned by a solver (valid until the next lg_solve on it). */
 typedef struct lg_solver_buffers {
-    int32_t n, ld, n_edges, edge_cap;
+    int32_t n, ld, n_edges, n_components;
     float *d_points;      /* n*3, world space                                      */
     int32_t *d_edges;     /* n_edges*2, (i,j) with i<j, in the reference's scan order */
     float *d_weights;     /* n_edges edge optical depths (+INF = dropped)           */
-    float *d_D;           /* n x ld all-pairs optical depth after the solve          */
+    float *d_D;           /* n x ld all-pairs optical depth (LG_SOLVE_DENSE), or NULL */
     float *d_L;           /* n*3 direct light per vertex                             */
     float *d_diffuse;     /* n*3 diffused radiance per vertex                        */
     int32_t *d_pred;      /* n x ld canonical predecessors, or NULL                  */
@@ -277,9 +277,11 @@ int32_t lg_diffuse(lg_ctx *ctx, int32_t n, const float *d_D, int32_t ld,
                    const double sigma_t[3], const float *d_L, float *d_out);
 
 /* ------------------------------------------------------------ full solve ---- */
-/* Per-solve workspace for graphs of up to n_max vertices.  with_pred allocates
- * the predecessor matrix. */
-int32_t lg_solver_create(lg_ctx *ctx, int32_t n_max, int32_t edge_cap, int32_t with_pred,
+/* Per-solve workspace for graphs of up to n_max vertices.  flags:
+ * LG_SOLVER_DENSE_D allocates the dense n x ld optical-depth matrix,
+ * LG_SOLVER_PRED also the predecessor matrix.  edge_cap is ignored (the edge
+ * buffers grow on demand); it is kept for callers that size their own copies. */
+int32_t lg_solver_create(lg_ctx *ctx, int32_t n_max, int32_t edge_cap, int32_t flags,
                          lg_solver **out);
 int32_t lg_solver_destroy(lg_ctx *ctx, lg_solver *solver);
 int32_t lg_solver_get(const lg_solver *solver, lg_solver_buffers *out);
@@ -294,11 +296,16 @@ typedef struct lg_solve_params {
     uint32_t solve;        /* solve index (the reference's LightGraph iteration i)  */
     int32_t external_points; /* 1: use the points already in the solver's d_points */
     int32_t timing;        /* 1: record per-stage HIP events                       */
-    int32_t pad_;
+    int32_t flags;         /* LG_SOLVE_DENSE: also write the dense n x ld outputs   */
 } lg_solve_params;
 
+enum { LG_SOLVE_DENSE = 1 };
+enum { LG_SOLVER_DENSE_D = 1, LG_SOLVER_PRED = 2 };
+
 /* One LightGraph solve (LightGraph::operator() body, LightGraph.h:163-189):
- * sample -> edges -> edge march -> Floyd-Warshall -> vertex light -> diffuse. */
+ * sample -> edges -> edge march -> Floyd-Warshall -> vertex light -> diffuse.
+ * The all-pairs closure runs per connected component of the graph (optical
+ * depth across components is +INF, the reference's zero transmittance). */
 int32_t lg_solve(lg_ctx *ctx, lg_solver *solver, const lg_grid *density,
                  const lg_grid *temperature, const lg_shader *shader, const lg_light *lights,
                  int32_t n_lights, const lg_march_params *params, const lg_solve_params *sp);

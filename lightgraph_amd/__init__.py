@@ -333,18 +333,23 @@ def diffuse(ctx, n, D_buf, ld, sigma_t, L_buf, out_buf):
 class Solver:
     """Per-solve workspace (lg_solver): LightGraph::operator() for one index."""
 
-    def __init__(self, ctx, n_max, edge_cap=0, with_pred=False):
+    def __init__(self, ctx, n_max, dense=False, with_pred=False):
         self.ctx = ctx
         h = C.c_void_p()
-        _check(lib().lg_solver_create(ctx.handle, n_max, edge_cap, 1 if with_pred else 0, C.byref(h)),
-               "solver_create")
+        flags = (1 if dense else 0) | (2 if with_pred else 0)
+        _check(lib().lg_solver_create(ctx.handle, n_max, 0, flags, C.byref(h)), "solver_create")
         self.handle = h.value
+        self.dense = dense or with_pred
+        self.last_dense = False
 
     def solve(self, density, shader, lights, params, n, radius, seed=854123, solve=0, max_connect=12,
-              temperature=None, env_seed=4564446, external_points=False, timing=False):
+              temperature=None, env_seed=4564446, external_points=False, timing=False, dense=None):
         sp = lg_solve_params()
         sp.n, sp.max_connect, sp.radius, sp.seed, sp.env_seed = n, max_connect, radius, seed, env_seed
         sp.solve, sp.external_points, sp.timing = solve, 1 if external_points else 0, 1 if timing else 0
+        dense = self.dense if dense is None else dense
+        sp.flags = 1 if dense else 0
+        self.last_dense = dense
         arr = light_array(lights)
         _check(lib().lg_solve(self.ctx.handle, self.handle, density.handle,
                               temperature.handle if temperature else None, C.byref(shader), arr, len(lights),
@@ -377,10 +382,11 @@ class Solver:
              "weights": self._read(b.d_weights, np.float32, (b.n_edges,)) if b.n_edges else np.zeros(0, np.float32),
              "L": self._read(b.d_L, np.float32, (b.n, 3)),
              "diffuse": self._read(b.d_diffuse, np.float32, (b.n, 3))}
-        if want_D:
-            r["D"] = self._read(b.d_D, np.float32, (b.ld, b.ld))[: b.n, : b.n]
-        if b.d_pred:
-            r["pred"] = self._read(b.d_pred, np.int32, (b.ld, b.ld))[: b.n, : b.n]
+        r["n_components"] = b.n_components
+        if want_D and self.last_dense and b.d_D:
+            r["D"] = self._read(b.d_D, np.float32, (b.n, b.ld))[:, : b.n]
+        if self.last_dense and b.d_pred:
+            r["pred"] = self._read(b.d_pred, np.int32, (b.n, b.ld))[:, : b.n]
         return r
 
     def destroy(self):

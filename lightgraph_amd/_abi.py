@@ -80,7 +80,7 @@ class lg_solve_timing(C.Structure):
 
 
 class lg_solver_buffers(C.Structure):
-    _fields_ = [("n", C.c_int32), ("ld", C.c_int32), ("n_edges", C.c_int32), ("edge_cap", C.c_int32),
+    _fields_ = [("n", C.c_int32), ("ld", C.c_int32), ("n_edges", C.c_int32), ("n_components", C.c_int32),
                 ("d_points", C.c_void_p), ("d_edges", C.c_void_p), ("d_weights", C.c_void_p),
                 ("d_D", C.c_void_p), ("d_L", C.c_void_p), ("d_diffuse", C.c_void_p),
                 ("d_pred", C.c_void_p)]
@@ -89,7 +89,7 @@ class lg_solver_buffers(C.Structure):
 class lg_solve_params(C.Structure):
     _fields_ = [("n", C.c_int32), ("max_connect", C.c_int32), ("radius", C.c_double),
                 ("seed", C.c_uint64), ("env_seed", C.c_uint64), ("solve", C.c_uint32),
-                ("external_points", C.c_int32), ("timing", C.c_int32), ("pad_", C.c_int32)]
+                ("external_points", C.c_int32), ("timing", C.c_int32), ("flags", C.c_int32)]
 
 
 P = C.c_void_p

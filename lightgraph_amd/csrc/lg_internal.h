@@ -78,6 +78,7 @@ struct lg_ctx {
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
     int cu_count = 256;
+    void *stage_pipeline = nullptr;  // lg::Pipeline used by the single-stage entry points
 };
 
 // Dense-brick grid.  Storage holds (nb+2)^3 bricks of 8^3 voxels: one empty brick
@@ -112,19 +113,11 @@ struct lg_envmap {
 };
 
 struct lg_solver {
-    int n_max = 0, ld_max = 0, edge_cap = 0;
-    bool with_pred = false;
+    int n_max = 0, ld_max = 0;
+    bool with_pred = false, with_dense = false, timed = false;
     lg_solver_buffers buf{};
-    // graph-build workspace
-    int32_t *cand_count = nullptr;   // n_max+1
-    int32_t *cand_off = nullptr;     // n_max+1
-    int32_t *cand = nullptr;         // candidate capacity
-    int64_t cand_cap = 0;
-    int32_t *d_nedges = nullptr;     // device counter
-    double *d_colsum = nullptr;      // diffusion workspace
-    float *d_Lnorm = nullptr;
-    hipEvent_t ev[8] = {};
-    lg_solve_timing timing{};
+    void *pipeline = nullptr;  // lg::Pipeline
+    hipEvent_t ev[6] = {};
 };
 
 namespace lg {
@@ -145,6 +138,9 @@ int32_t scratch(lg_ctx *ctx, size_t bytes, void **out);
 // kernel timing hooks (no-ops unless lg_ctx_set_kernel_timing enabled it)
 void ktimer_begin(lg_ctx *ctx, const char *name, hipEvent_t *a);
 void ktimer_end(lg_ctx *ctx, const char *name, hipEvent_t a);
+
+// frees the host side of ctx->stage_pipeline (device memory goes with the ctx)
+void destroy_stage_pipeline(lg_ctx *ctx);
 
 // grid helpers (lg_runtime.hip)
 int32_t grid_alloc(lg_ctx *ctx, const lg_grid_desc *desc, lg_grid **out);

@@ -1,0 +1,289 @@
+// Shared declarations of the solve pipeline: per-solve descriptors, march and
+// lighting device functions, and the reusable device workspace.
+#pragma once
+
+#include <vector>
+
+#include "lg_internal.h"
+#include "lg_lights.h"
+
+namespace lg {
+
+constexpr int kSmallComp = 128;  // components up to this size are closed in LDS
+
+// Per-solve descriptor (device array, one entry per solve of a batch).
+struct SolveDesc {
+    lgd::GridView g;              // density
+    lgd::GridView tg;             // temperature (valid if has_temp)
+    const int32_t *active_list;   // sampling tables of the density grid
+    const uint64_t *mass_prefix;
+    uint64_t Q;
+    int nact;
+    int n, voff;                  // vertices of this solve: [voff, voff + n)
+    uint32_t solve;               // the reference's LightGraph iteration index
+    int has_temp;
+    int dtype;
+    int ld;                       // leading dimension of the optional dense outputs
+    float *dense_D;               // optional n x ld optical depth (solve-local indices)
+    int32_t *dense_pred;          // optional n x ld canonical predecessors
+};
+
+struct MarchConsts {
+    double ds_sds_scds[3];  // densityScale, shadowDensityScale, scatterDensityScale
+    double dmax, sdm, cutoff, h, tau_star;
+};
+
+struct LightConsts {
+    double ds_sds_scds[3];
+    double dmax, cutoff, h, tau_star;
+    double albedo[3], ext[3];
+    double emission_scale;
+    double ramp_in, ramp_out;
+    int ramp_n;
+    double ramp_pos[LG_MAX_RAMP];
+    double ramp_col[LG_MAX_RAMP][3];
+};
+
+MarchConsts march_consts(const lg_shader *sh, const lg_march_params *mp);
+LightConsts light_consts(const lg_shader *sh, const lg_march_params *mp);
+
+// Edge optical depth (raymarchConnection, LightGraph.h:364-406): +INF when the
+// ray misses the grid bbox or the dropout |T|^2 < cutoff fires.
+template <typename T>
+__device__ __forceinline__ float march_edge_one(const lgd::GridView &g, const MarchConsts &mc, const float *pi,
+                                                const float *pj) {
+    const double sp[3] = {(double)pi[0], (double)pi[1], (double)pi[2]};
+    double dir[3] = {(double)pj[0] - sp[0], (double)pj[1] - sp[1], (double)pj[2] - sp[2]};
+    const double len = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    const float dist = (float)len;
+    if (!(fabs(len - 0.0) <= 1e-7)) {  // Vec3::normalize(eps=1e-7): *this *= 1/len
+        const double inv = 1.0 / len;
+        for (int c = 0; c < 3; ++c) dir[c] = dir[c] * inv;
+    }
+    lgd::Isect s;
+    if (!lgd::isect_world_ray(g, s, sp, dir, lgd::kDeltaD, (double)dist)) return INFINITY;
+    double tau = 0.0;
+    double t0, t1;
+    while (lgd::isect_march(g, s, t0, t1)) {
+        for (double t = mc.h * ceil(t0 / mc.h); t <= t1; t += mc.h) {
+            const float rho = lgd::trilinear<T>(g, s.e[0] + s.d[0] * t, s.e[1] + s.d[1] * t, s.e[2] + s.d[2] * t);
+            double dens = (double)rho * mc.ds_sds_scds[0] * mc.ds_sds_scds[1] * mc.ds_sds_scds[2];
+            dens = (mc.dmax < dens) ? mc.dmax : dens;
+            if (dens < mc.cutoff && mc.sdm < mc.cutoff) continue;
+            const double m = (dens < mc.sdm) ? mc.sdm : dens;
+            tau += m * mc.h;
+            if (tau > mc.tau_star) return INFINITY;
+        }
+    }
+    return (float)tau;
+}
+
+// ColorRamp::getColor (Shader.h:63-92) in FP64, NaN behaviour included.
+__device__ __forceinline__ void ramp_color(const LightConsts &c, double x, double out[3]) {
+    if (c.ramp_n == 0) {
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    double hp = (x - c.ramp_in) / (c.ramp_out - c.ramp_in);
+    hp = (hp < 0.0) ? 0.0 : hp;
+    hp = (1.0 < hp) ? 1.0 : hp;
+    if (c.ramp_n == 1) {
+        for (int k = 0; k < 3; ++k) out[k] = c.ramp_col[0][k];
+        return;
+    }
+    int prev = 0, next = 0;
+    for (int i = 0; i < c.ramp_n; ++i) {
+        if (c.ramp_pos[i] <= hp) {
+            prev = i;
+        } else {
+            next = i;
+            break;
+        }
+    }
+    const double range = c.ramp_pos[next] - c.ramp_pos[prev];
+    const double blend = (hp - c.ramp_pos[prev]) / range;
+    for (int k = 0; k < 3; ++k) out[k] = c.ramp_col[prev][k] * (1.0 - blend) + c.ramp_col[next][k] * blend;
+}
+
+// Shadow optical depth toward a light (LightGraph.h:461-491: no floor, no
+// jitter).  Returns false when the ray misses the grid bbox (the reference then
+// adds nothing for this sample); *dropped reports the |T|^2 < cutoff dropout.
+template <typename T>
+__device__ __forceinline__ bool shadow_tau(const lgd::GridView &g, const LightConsts &lc, const double p[3],
+                                           const double ldir[3], double distance, double *tau_out, bool *dropped) {
+    const double sd[3] = {-ldir[0], -ldir[1], -ldir[2]};
+    lgd::Isect is;
+    if (!lgd::isect_world_ray(g, is, p, sd, lgd::kDeltaD, distance)) return false;
+    double tau = 0.0;
+    double t0, t1;
+    *dropped = false;
+    while (lgd::isect_march(g, is, t0, t1)) {
+        for (double t = lc.h * ceil(t0 / lc.h); t <= t1; t += lc.h) {
+            const float rho = lgd::trilinear<T>(g, is.e[0] + is.d[0] * t, is.e[1] + is.d[1] * t, is.e[2] + is.d[2] * t);
+            double dens = (double)rho * lc.ds_sds_scds[0] * lc.ds_sds_scds[1] * lc.ds_sds_scds[2];
+            dens = (lc.dmax < dens) ? lc.dmax : dens;
+            if (dens < lc.cutoff) continue;
+            tau += dens * lc.h;
+            if (tau > lc.tau_star) {
+                *dropped = true;
+                *tau_out = tau;
+                return true;
+            }
+        }
+    }
+    *tau_out = tau;
+    return true;
+}
+
+// computeLighting for one vertex (LightGraph.h:438-497).
+template <typename T>
+__device__ __forceinline__ void vertex_light_one(const lgd::GridView &g, const lgd::GridView &tg, int has_temp,
+                                                 const LightConsts &lc, const lgl::DevLights &L, uint32_t ek0,
+                                                 uint32_t ek1, uint32_t vertex, uint32_t solve, const double p[3],
+                                                 double out[3]) {
+    double emission[3] = {0.0, 0.0, 0.0};
+    if (lc.ramp_n > 0) {
+        double temp = 0.0;
+        if (has_temp) {
+            const double x = (p[0] - tg.origin[0]) * tg.inv_voxel, y = (p[1] - tg.origin[1]) * tg.inv_voxel,
+                         z = (p[2] - tg.origin[2]) * tg.inv_voxel;
+            temp = (double)lgd::trilinear<float>(tg, x, y, z);
+        }
+        ramp_color(lc, temp, emission);
+        for (int k = 0; k < 3; ++k) emission[k] = emission[k] * lc.emission_scale;
+    }
+    lgd::Stream env_rng;
+    env_rng.init(vertex, solve, lgd::DOM_VERTEX_ENV, ((uint64_t)ek1 << 32) | ek0);
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int li = 0; li < L.n; ++li) {
+        const lgl::DevLight &lt = L.l[li];
+        int samples = lt.samples;
+        if (samples > 1) samples *= 10;  // LightGraph.h:450-452
+        double lumi[3] = {0.0, 0.0, 0.0};
+        for (int s = 0; s < samples; ++s) {
+            double ldir[3], inten[3], distance;
+            lgl::illuminate(lt, p, env_rng, ldir, inten, distance);
+            double tau;
+            bool dropped;
+            if (!shadow_tau<T>(g, lc, p, ldir, distance, &tau, &dropped)) continue;
+            if (dropped) continue;  // transmittance set to zero
+            for (int k = 0; k < 3; ++k) lumi[k] += inten[k] * (double)expf((float)(lc.ext[k] * tau));
+        }
+        for (int k = 0; k < 3; ++k) acc[k] += lc.albedo[k] * (lumi[k] / (double)samples);
+    }
+    for (int k = 0; k < 3; ++k) out[k] = acc[k] + emission[k];
+}
+
+// Kernel (a) for point i of solve d: the density-weighted sampler.
+template <typename T>
+__device__ __forceinline__ void sample_point(const SolveDesc &d, int i, uint32_t k0, uint32_t k1, float *out) {
+    const lgd::GridView &g = d.g;
+    const lgd::U4 r0 = lgd::philox(lgd::U4{(uint32_t)i, d.solve, 0u, lgd::DOM_SAMPLE}, k0, k1);
+    const lgd::U4 r1 = lgd::philox(lgd::U4{(uint32_t)i, d.solve, 1u, lgd::DOM_SAMPLE}, k0, k1);
+    const uint64_t u = ((uint64_t)r0.y << 32) | (uint64_t)r0.x;
+    const uint64_t target = __umul64hi(u, d.Q);
+    int lo = 0, hi = d.nact - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (d.mass_prefix[mid] <= target)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const int sb = d.active_list[lo];
+    const uint64_t local = target - d.mass_prefix[lo];
+    const T *p = static_cast<const T *>(g.data) + ((long long)sb << 9);
+    int v = 511;
+    uint64_t cum = 0;
+    for (int k = 0; k < 512; ++k) {
+        const float f = lgd::to_f32(p[k]);
+        if (f > 0.0f) cum += (uint64_t)fmin((double)f * 16777216.0, 549755813888.0 * 16777216.0);
+        if (local < cum) {
+            v = k;
+            break;
+        }
+    }
+    const int sbx = sb % g.NBx, sby = (sb / g.NBx) % g.NBy, sbz = sb / (g.NBx * g.NBy);
+    const int vx = (sbx - 1) * 8 + (v & 7), vy = (sby - 1) * 8 + ((v >> 3) & 7), vz = (sbz - 1) * 8 + (v >> 6);
+    const double px = (double)vx - 0.5 + (double)lgd::u01(r0.z);
+    const double py = (double)vy - 0.5 + (double)lgd::u01(r0.w);
+    const double pz = (double)vz - 0.5 + (double)lgd::u01(r1.x);
+    out[0] = (float)(g.origin[0] + g.voxel * px);
+    out[1] = (float)(g.origin[1] + g.voxel * py);
+    out[2] = (float)(g.origin[2] + g.voxel * pz);
+}
+
+struct PipelineParams {
+    float *pts;             // V x 3 points (input when external_points, else output)
+    float *L;               // V x 3 vertex light (output)
+    float *out;             // V x 3 diffused radiance (output)
+    uint64_t seed, env_seed;
+    int external_points;
+    int max_connect;
+    double r2;              // squared edge radius (float arithmetic of the reference)
+    double ext[3];          // shader extinction (negative)
+    MarchConsts mc;
+    LightConsts lc;
+    lgl::DevLights lights;
+    hipEvent_t ev_after_sample = nullptr;
+    hipEvent_t ev_after_light = nullptr;
+};
+
+struct PipelineResult {
+    int Ktot = 0, n_edges_total = 0, n_components = 0, n_big = 0;
+    int *edges = nullptr;   // global (i, j), row-major, solve-major
+    float *w = nullptr;
+    int *epos = nullptr;    // candidate -> edge index (exclusive scan of accepted)
+    int *off = nullptr;     // vertex -> first candidate
+};
+
+// Device workspace reused across calls; grows on demand.
+struct Pipeline {
+    int v_cap = 0, s_cap = 0;
+    SolveDesc *sd = nullptr;
+    int *sid = nullptr, *rowcnt = nullptr, *off = nullptr, *colcnt = nullptr;
+    uint8_t *risky = nullptr;
+    int *visit = nullptr, *visit_pos = nullptr, *visit_list = nullptr;
+    int *adeg = nullptr, *aoff = nullptr, *acur = nullptr;
+    int *parent = nullptr, *root = nullptr;
+    uint64_t *keys = nullptr, *keys_sorted = nullptr;
+    int *flag = nullptr, *cid = nullptr, *member = nullptr, *local = nullptr, *comp_off = nullptr;
+    int *big_list = nullptr, *counters = nullptr;
+    void *tmp_vertex = nullptr;
+    size_t tmp_vertex_bytes = 0;
+    // candidate / edge sized
+    int *cand = nullptr, *accepted = nullptr, *epos = nullptr, *edges_ = nullptr, *anb = nullptr;
+    float *w_ = nullptr, *aw = nullptr;
+    uint8_t *tmp_cand = nullptr;
+    size_t cand_cap = 0, accepted_cap = 0, epos_cap = 0, tmp_cand_cap = 0, edges_cap = 0, w_cap = 0, anb_cap = 0,
+           aw_cap = 0;
+    // large components
+    float *big_T = nullptr, *big_L = nullptr, *big_out = nullptr, *big_Ln = nullptr;
+    double *big_sums = nullptr;
+    size_t bigT_cap = 0, bigL_cap = 0, bigout_cap = 0, bigsums_cap = 0, bigLn_cap = 0;
+
+    int32_t reserve_vertices(lg_ctx *ctx, int V, int S);
+    void release_vertex(lg_ctx *ctx);
+    void release(lg_ctx *ctx);
+};
+
+int32_t run_pipeline(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, const PipelineParams &pp,
+                     PipelineResult *res);
+// stages of run_pipeline
+int32_t pl_begin(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, int *V_out, int *nmax_out);
+int32_t pl_sample(lg_ctx *ctx, Pipeline &pl, int V, int dtype, uint64_t seed, float *pts);
+int32_t pl_light(lg_ctx *ctx, Pipeline &pl, int V, int dtype, const LightConsts &lc, const lgl::DevLights &lights,
+                 uint64_t env_seed, const float *pts, float *L);
+int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float *pts, double r2, int max_connect,
+                 int *Ktot_out);
+int32_t pl_march(lg_ctx *ctx, Pipeline &pl, int dtype, const MarchConsts &mc, const float *pts, int Ktot);
+int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, int V, int Ktot,
+                      const double ext[3], const float *L, float *out, PipelineResult *res);
+
+// helpers from lg_solve.hip
+int32_t fw_init_impl(lg_ctx *ctx, int n, float *d_D, int ld);
+int32_t diffuse_impl(lg_ctx *ctx, int n, const float *d_D, int ld, const double sigma_t[3], const float *d_L,
+                     float *d_out, double *d_sums, float *d_Ln);
+SolveDesc make_desc(lg_grid *density, const lg_grid *temperature, int n, int voff, uint32_t solve);
+
+}  // namespace lg

@@ -1,0 +1,852 @@
+// The LightGraph solve pipeline for S >= 1 independent solves at once
+// (LightGraph::operator(), LightGraph.h:158-196), organised around the
+// connected components of each light graph:
+//
+//   sample (a) -> vertex light -> candidate CSR -> capped greedy selection ->
+//   edge march (b) -> union-find over finite edges -> radix sort by component ->
+//   per component: min-plus closure (c) + column-normalised diffusion.
+//
+// Optical depth between vertices of different components is +INF (the
+// reference's zero transmittance), so closing each component separately gives
+// the same matrix as closing the whole graph, with the same pivot order inside a
+// component (members keep ascending vertex order).  Components of at most 128
+// vertices are closed and diffused in LDS by one workgroup each, all in one
+// launch; larger ones go through the blocked Floyd-Warshall kernels.
+//
+// The greedy degree cap (LightGraph.h:297-310) is order dependent, but only for
+// edges touching a vertex whose candidate degree exceeds the cap ("risky"):
+// every other candidate edge is always accepted.  Only rows holding such edges
+// are swept sequentially, one wave per solve.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+
+#include "lg_internal.h"
+#include "lg_lights.h"
+#include "lg_pipeline.h"
+
+namespace lg {
+
+// ------------------------------------------------------------- utilities ----
+__device__ __forceinline__ int atomic_load_i32(int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_store_i32(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void k_solve_ids(const SolveDesc *__restrict__ sd, int S, int V, int *__restrict__ sid) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    int lo = 0, hi = S - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (sd[mid].voff <= v)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    sid[v] = lo;
+}
+
+// ------------------------------------------------------------ sampling ------
+template <typename T>
+__global__ void k_sample(const SolveDesc *__restrict__ sd, const int *__restrict__ sid, int V, uint32_t k0,
+                         uint32_t k1, float *__restrict__ pts) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const SolveDesc &d = sd[sid[v]];
+    sample_point<T>(d, v - d.voff, k0, k1, pts + 3 * (long long)v);
+}
+
+// ----------------------------------------------------- vertex lighting ------
+template <typename T>
+__global__ void k_light(const SolveDesc *__restrict__ sd, const int *__restrict__ sid, int V, LightConsts lc,
+                        lgl::DevLights L, uint32_t ek0, uint32_t ek1, const float *__restrict__ pts,
+                        float *__restrict__ out) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const SolveDesc &d = sd[sid[v]];
+    const int i = v - d.voff;
+    const double p[3] = {(double)pts[3 * v], (double)pts[3 * v + 1], (double)pts[3 * v + 2]};
+    double acc[3];
+    vertex_light_one<T>(d.g, d.tg, d.has_temp, lc, L, ek0, ek1, (uint32_t)i, d.solve, p, acc);
+    for (int k = 0; k < 3; ++k) out[3 * v + k] = (float)acc[k];
+}
+
+// ---------------------------------------------------------- candidates ------
+__device__ __forceinline__ double pdist2(const float *__restrict__ pts, int i, int j) {
+    const double dx = (double)pts[3 * i + 0] - (double)pts[3 * j + 0];
+    const double dy = (double)pts[3 * i + 1] - (double)pts[3 * j + 1];
+    const double dz = (double)pts[3 * i + 2] - (double)pts[3 * j + 2];
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// One wave per global vertex i: candidates j > i of the same solve with
+// |p_i - p_j|^2 < r2, ascending.  fill == nullptr: count only.
+__global__ void k_cand(const SolveDesc *__restrict__ sd, const int *__restrict__ sid, int V, const float *__restrict__ pts,
+                       double r2, const int *__restrict__ off, int *__restrict__ count, int *__restrict__ fill) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (i >= V) return;
+    const SolveDesc &d = sd[sid[i]];
+    const int end = d.voff + d.n;
+    int c = 0;
+    const int base_out = fill ? off[i] : 0;
+    for (int base = i + 1; base < end; base += 64) {
+        const int j = base + lane;
+        const bool hit = j < end && pdist2(pts, i, j) < r2;
+        const uint64_t m = __ballot(hit);
+        if (fill && hit) fill[base_out + c + __popcll(m & ((1ull << lane) - 1ull))] = j;
+        c += __popcll(m);
+    }
+    if (!fill && lane == 0) count[i] = c;
+}
+
+__global__ void k_colcount(int V, const int *__restrict__ off, const int *__restrict__ cand, int *__restrict__ colcnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    for (int k = off[i]; k < off[i + 1]; ++k) atomicAdd(&colcnt[cand[k]], 1);
+}
+
+__global__ void k_risky(int V, int cap, const int *__restrict__ off, const int *__restrict__ colcnt,
+                        uint8_t *__restrict__ risky) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    risky[i] = (off[i + 1] - off[i] + colcnt[i]) > cap ? 1 : 0;
+}
+
+// accepted = 1 for candidates with no risky endpoint; visit = rows the greedy
+// sweep has to look at.  Uncapped (cap <= 0): everything accepted.
+__global__ void k_classify(int V, int cap, const int *__restrict__ off, const int *__restrict__ cand,
+                           const uint8_t *__restrict__ risky, int *__restrict__ accepted, int *__restrict__ visit) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    const int s = off[i], e = off[i + 1];
+    if (cap <= 0) {
+        for (int k = s; k < e; ++k) accepted[k] = 1;
+        visit[i] = 0;
+        return;
+    }
+    const bool ri = risky[i] != 0;
+    bool any = false;
+    for (int k = s; k < e; ++k) {
+        const bool rj = risky[cand[k]] != 0;
+        accepted[k] = (!ri && !rj) ? 1 : 0;
+        any |= rj;
+    }
+    visit[i] = (e > s && (ri || any)) ? 1 : 0;
+}
+
+__global__ void k_compact_rows(int V, const int *__restrict__ flag, const int *__restrict__ pos, int *__restrict__ list) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    if (flag[i]) list[pos[i]] = i;
+}
+
+__device__ __forceinline__ int lower_bound_i32(const int *a, int n, int x) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (a[mid] < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Sequential greedy over contested edges, one wave per solve.
+__global__ void __launch_bounds__(64) k_greedy(const SolveDesc *__restrict__ sd, int cap, const int *__restrict__ off,
+                                               const int *__restrict__ cand, const uint8_t *__restrict__ risky,
+                                               const int *__restrict__ visit_list, const int *__restrict__ n_visit,
+                                               int *__restrict__ accepted) {
+    extern __shared__ uint8_t cnt[];
+    const int s = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int n = sd[s].n, v0 = sd[s].voff;
+    for (int k = lane; k < n; k += 64) cnt[k] = 0;
+    __syncthreads();
+    const int nv = *n_visit;
+    const int lo = lower_bound_i32(visit_list, nv, v0), hi = lower_bound_i32(visit_list, nv, v0 + n);
+    for (int idx = lo; idx < hi; ++idx) {
+        const int i = visit_list[idx];
+        const bool ri = risky[i] != 0;
+        int ci = ri ? (int)cnt[i - v0] : 0;
+        if (ri && ci >= cap) continue;
+        const int e = off[i + 1];
+        for (int base = off[i]; base < e; base += 64) {
+            const int k = base + lane;
+            const bool valid = k < e;
+            const int j = valid ? cand[k] : v0;
+            const bool rj = valid && risky[j] != 0;
+            const bool contested = valid && (ri || rj);
+            const bool elig = contested && (!rj || (int)cnt[j - v0] < cap);
+            bool sel;
+            int taken = 0;
+            if (ri) {
+                const uint64_t m = __ballot(elig);
+                const int need = cap - ci;
+                sel = elig && __popcll(m & ((1ull << lane) - 1ull)) < need;
+                taken = min(__popcll(m), need);
+            } else {
+                sel = elig;
+            }
+            if (sel) {
+                accepted[k] = 1;
+                if (rj) cnt[j - v0] = (uint8_t)(cnt[j - v0] + 1);
+            }
+            ci += taken;
+            __syncthreads();
+            if (ri && ci >= cap) break;
+        }
+        if (ri && lane == 0) cnt[i - v0] = (uint8_t)ci;
+        __syncthreads();
+    }
+}
+
+// Edge list (i, j) in row-major order from the accepted candidates.
+__global__ void k_edges(int V, const int *__restrict__ off, const int *__restrict__ cand,
+                        const int *__restrict__ accepted, const int *__restrict__ epos, int *__restrict__ edges) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= V) return;
+    for (int k = off[i]; k < off[i + 1]; ++k)
+        if (accepted[k]) {
+            const int e = epos[k];
+            edges[2 * e] = i;
+            edges[2 * e + 1] = cand[k];
+        }
+}
+
+// ---------------------------------------------------------- edge march ------
+template <typename T>
+__global__ void k_march(const SolveDesc *__restrict__ sd, const int *__restrict__ sid, MarchConsts mc,
+                        const float *__restrict__ pts, const int *__restrict__ edges, int edge_limit,
+                        const int *__restrict__ n_edges, float *__restrict__ w) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= min(*n_edges, edge_limit)) return;
+    const int i = edges[2 * e], j = edges[2 * e + 1];
+    w[e] = march_edge_one<T>(sd[sid[i]].g, mc, pts + 3 * (long long)i, pts + 3 * (long long)j);
+}
+
+// ----------------------------------------------------------- adjacency ------
+__global__ void k_adj_deg(const int *__restrict__ edges, const float *__restrict__ w, const int *__restrict__ n_edges,
+                          int *__restrict__ deg) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= *n_edges || !(w[e] < INFINITY)) return;
+    atomicAdd(&deg[edges[2 * e]], 1);
+    atomicAdd(&deg[edges[2 * e + 1]], 1);
+}
+
+__global__ void k_adj_fill(const int *__restrict__ edges, const float *__restrict__ w, const int *__restrict__ n_edges,
+                           const int *__restrict__ aoff, int *__restrict__ cursor, int *__restrict__ nb,
+                           float *__restrict__ nw) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= *n_edges || !(w[e] < INFINITY)) return;
+    const int i = edges[2 * e], j = edges[2 * e + 1];
+    const float we = w[e];
+    int p = aoff[i] + atomicAdd(&cursor[i], 1);
+    nb[p] = j;
+    nw[p] = we;
+    p = aoff[j] + atomicAdd(&cursor[j], 1);
+    nb[p] = i;
+    nw[p] = we;
+}
+
+// ----------------------------------------------------------- components -----
+__device__ __forceinline__ int uf_find(int *P, int x) {
+    for (;;) {
+        const int p = atomic_load_i32(&P[x]);
+        if (p == x) return x;
+        const int g = atomic_load_i32(&P[p]);
+        if (g == p) return p;
+        atomic_store_i32(&P[x], g);  // path halving; pointers only decrease
+        x = g;
+    }
+}
+
+// Union-find over the finite edges of one solve per workgroup: hook the larger
+// root under the smaller until a full pass changes nothing; the root of a
+// component is its smallest vertex.
+__global__ void __launch_bounds__(1024) k_components(const SolveDesc *__restrict__ sd, const int *__restrict__ off,
+                                                     const int *__restrict__ epos, const int *__restrict__ edges,
+                                                     const float *__restrict__ w, int *__restrict__ P,
+                                                     int *__restrict__ root) {
+    const int s = blockIdx.x;
+    const int v0 = sd[s].voff, n = sd[s].n;
+    const int e0 = epos[off[v0]], e1 = epos[off[v0 + n]];
+    for (int v = v0 + threadIdx.x; v < v0 + n; v += blockDim.x) atomic_store_i32(&P[v], v);
+    __syncthreads();
+    int changed;
+    do {
+        changed = 0;
+        for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+            if (!(w[e] < INFINITY)) continue;
+            const int ru = uf_find(P, edges[2 * e]), rv = uf_find(P, edges[2 * e + 1]);
+            if (ru != rv) {
+                atomicMin(&P[max(ru, rv)], min(ru, rv));
+                changed = 1;
+            }
+        }
+        changed = __syncthreads_or(changed);
+    } while (changed);
+    for (int v = v0 + threadIdx.x; v < v0 + n; v += blockDim.x) root[v] = uf_find(P, v);
+}
+
+__global__ void k_keys(int V, int bits, const int *__restrict__ root, uint64_t *__restrict__ keys) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    keys[v] = ((uint64_t)root[v] << bits) | (uint64_t)v;
+}
+
+__global__ void k_comp_flags(int V, int bits, const uint64_t *__restrict__ sorted, int *__restrict__ flag,
+                             int *__restrict__ member) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= V) return;
+    const uint64_t mask = (1ull << bits) - 1ull;
+    member[k] = (int)(sorted[k] & mask);
+    flag[k] = (k == 0 || (sorted[k] >> bits) != (sorted[k - 1] >> bits)) ? 1 : 0;
+}
+
+__global__ void k_comp_offsets(int V, const int *__restrict__ flag, const int *__restrict__ cid_excl,
+                               int *__restrict__ comp_off, const int *__restrict__ n_comp) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) comp_off[*n_comp] = V;
+    if (k >= V) return;
+    if (flag[k]) comp_off[cid_excl[k]] = k;
+}
+
+// local index of every vertex inside its component and the list of components
+// too large for the LDS kernel.
+__global__ void k_comp_local(int V, const int *__restrict__ flag, const int *__restrict__ cid_excl,
+                             const int *__restrict__ comp_off, const int *__restrict__ member, int *__restrict__ local,
+                             int *__restrict__ big_list, int *__restrict__ n_big) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= V) return;
+    const int c = cid_excl[k] + flag[k] - 1;
+    local[member[k]] = k - comp_off[c];
+    if (flag[k] && comp_off[c + 1] - comp_off[c] > kSmallComp) big_list[atomicAdd(n_big, 1)] = c;
+}
+
+// ------------------------------------------------ small-component closure ----
+// One workgroup per component of m <= 128 vertices: build the m x m optical
+// depth matrix in LDS from the adjacency, close it pivot by pivot (ascending
+// vertex order), then diffuse (LightGraph.h:554-579) and write the outputs.
+__global__ void __launch_bounds__(256) k_comp_small(const SolveDesc *__restrict__ sd, const int *__restrict__ sid,
+                                                    const int *__restrict__ comp_off, const int *__restrict__ member,
+                                                    const int *__restrict__ local, const int *__restrict__ aoff,
+                                                    const int *__restrict__ anb, const float *__restrict__ aw,
+                                                    const float *__restrict__ L, float e0, float e1, float e2,
+                                                    float *__restrict__ out) {
+    __shared__ float T[kSmallComp][kSmallComp + 1];
+    __shared__ int mem[kSmallComp];
+    __shared__ float Ln[kSmallComp][3];
+    const int c = blockIdx.x;
+    const int a0 = comp_off[c];
+    const int m = comp_off[c + 1] - a0;
+    if (m > kSmallComp) return;
+    const int tid = threadIdx.x;
+    if (m == 1) {
+        if (tid == 0) {
+            const int v = member[a0];
+            for (int k = 0; k < 3; ++k) out[3 * v + k] = L[3 * v + k];  // P = [1]: diffuse = L
+            const SolveDesc &d = sd[sid[v]];
+            const int li = v - d.voff;
+            if (d.dense_D) d.dense_D[(long long)li * d.ld + li] = 0.0f;
+            if (d.dense_pred) d.dense_pred[(long long)li * d.ld + li] = li;
+        }
+        return;
+    }
+    for (int a = tid; a < m; a += blockDim.x) mem[a] = member[a0 + a];
+    for (int e = tid; e < m * m; e += blockDim.x) {
+        const int i = e / m, j = e - i * m;
+        T[i][j] = (i == j) ? 0.0f : INFINITY;
+    }
+    __syncthreads();
+    for (int a = tid; a < m; a += blockDim.x) {
+        const int v = mem[a];
+        for (int p = aoff[v]; p < aoff[v + 1]; ++p) T[a][local[anb[p]]] = aw[p];
+    }
+    __syncthreads();
+    const int ty = tid >> 4, tx = tid & 15;
+    for (int k = 0; k < m; ++k) {
+        for (int i = ty; i < m; i += 16) {
+            const float dik = T[i][k];
+            if (!(dik < INFINITY)) continue;
+            for (int j = tx; j < m; j += 16) {
+                const float nv = dik + T[k][j];
+                if (nv < T[i][j]) T[i][j] = nv;
+            }
+        }
+        __syncthreads();
+    }
+    const float ex[3] = {e0, e1, e2};
+    for (int j = tid; j < m; j += blockDim.x) {
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int i = 0; i < m; ++i) {
+            const float dij = T[i][j];
+            for (int q = 0; q < 3; ++q) s[q] += (double)expf(ex[q] * dij);
+        }
+        const int v = mem[j];
+        for (int q = 0; q < 3; ++q) Ln[j][q] = (float)((double)L[3 * v + q] / (s[q] < 1.0 ? 1.0 : s[q]));
+    }
+    __syncthreads();
+    for (int i = tid; i < m; i += blockDim.x) {
+        double o[3] = {0.0, 0.0, 0.0};
+        for (int j = 0; j < m; ++j) {
+            const float dij = T[i][j];
+            for (int q = 0; q < 3; ++q) o[q] += (double)Ln[j][q] * (double)expf(ex[q] * dij);
+        }
+        const int v = mem[i];
+        for (int q = 0; q < 3; ++q) out[3 * v + q] = (float)o[q];
+    }
+    const SolveDesc &d = sd[sid[mem[0]]];
+    if (d.dense_D) {
+        for (int e = tid; e < m * m; e += blockDim.x) {
+            const int i = e / m, j = e - i * m;
+            d.dense_D[(long long)(mem[i] - d.voff) * d.ld + (mem[j] - d.voff)] = T[i][j];
+        }
+    }
+    if (d.dense_pred) {
+        // canonical predecessor: smallest neighbour k of j minimising D[i][k] + w(k, j)
+        for (int e = tid; e < m * m; e += blockDim.x) {
+            const int i = e / m, j = e - i * m;
+            int best_k = -1;
+            if (i == j) {
+                best_k = mem[i] - d.voff;
+            } else if (T[i][j] < INFINITY) {
+                float best = INFINITY;
+                const int vj = mem[j];
+                for (int p = aoff[vj]; p < aoff[vj + 1]; ++p) {
+                    const int kv = anb[p];
+                    const float val = T[i][local[kv]] + aw[p];
+                    const int kl = kv - d.voff;
+                    if (val < best || (val == best && kl < best_k)) {
+                        best = val;
+                        best_k = kl;
+                    }
+                }
+            }
+            d.dense_pred[(long long)(mem[i] - d.voff) * d.ld + (mem[j] - d.voff)] = best_k;
+        }
+    }
+}
+
+// ------------------------------------------------ large-component path ------
+__global__ void k_big_fill(int m, int ld, int a0, const int *__restrict__ member, const int *__restrict__ local,
+                           const int *__restrict__ aoff, const int *__restrict__ anb, const float *__restrict__ aw,
+                           float *__restrict__ Tm) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= m) return;
+    const int v = member[a0 + a];
+    for (int p = aoff[v]; p < aoff[v + 1]; ++p) Tm[(long long)a * ld + local[anb[p]]] = aw[p];
+}
+
+__global__ void k_big_gather_L(int m, int a0, const int *__restrict__ member, const float *__restrict__ L,
+                               float *__restrict__ Lm) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= m) return;
+    const int v = member[a0 + a];
+    for (int q = 0; q < 3; ++q) Lm[3 * a + q] = L[3 * v + q];
+}
+
+__global__ void k_big_scatter_out(int m, int a0, const int *__restrict__ member, const float *__restrict__ om,
+                                  float *__restrict__ out) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= m) return;
+    const int v = member[a0 + a];
+    for (int q = 0; q < 3; ++q) out[3 * v + q] = om[3 * a + q];
+}
+
+__global__ void k_big_scatter_dense(int m, int ldm, int a0, const int *__restrict__ member,
+                                    const float *__restrict__ Tm, int voff, float *__restrict__ D, int ld) {
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long long)m * m) return;
+    const int i = (int)(e / m), j = (int)(e - (long long)i * m);
+    D[(long long)(member[a0 + i] - voff) * ld + (member[a0 + j] - voff)] = Tm[(long long)i * ldm + j];
+}
+
+__global__ void k_big_pred(int m, int ldm, int a0, const int *__restrict__ member, const int *__restrict__ local,
+                           const int *__restrict__ aoff, const int *__restrict__ anb, const float *__restrict__ aw,
+                           const float *__restrict__ Tm, int voff, int *__restrict__ pred, int ld) {
+    const int i = blockIdx.x;
+    if (i >= m) return;
+    const float *row = Tm + (long long)i * ldm;
+    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+        int best_k = -1;
+        if (i == j) {
+            best_k = member[a0 + i] - voff;
+        } else if (row[j] < INFINITY) {
+            float best = INFINITY;
+            const int vj = member[a0 + j];
+            for (int p = aoff[vj]; p < aoff[vj + 1]; ++p) {
+                const int kv = anb[p];
+                const float val = row[local[kv]] + aw[p];
+                const int kl = kv - voff;
+                if (val < best || (val == best && kl < best_k)) {
+                    best = val;
+                    best_k = kl;
+                }
+            }
+        }
+        pred[(long long)(member[a0 + i] - voff) * ld + (member[a0 + j] - voff)] = best_k;
+    }
+}
+
+__global__ void k_pred_init(int n, int ld, int *__restrict__ pred) {
+    const long long total = (long long)n * ld;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (long long)gridDim.x * blockDim.x) {
+        const int i = (int)(k / ld), j = (int)(k % ld);
+        pred[k] = (i == j) ? i : -1;
+    }
+}
+
+// ------------------------------------------------------------- driver -------
+template <typename F>
+static void for_dtype(int dtype, F &&f) {
+    if (dtype == LG_F32)
+        f(float{});
+    else if (dtype == LG_F16)
+        f(_Float16{});
+    else
+        f(lgd::bf16_t{});
+}
+
+template <typename T>
+static int32_t grow(lg_ctx *ctx, T **p, size_t *cap, size_t want) {
+    if (want <= *cap && *p) return LG_OK;
+    dfree(ctx, *p);
+    *p = nullptr;
+    *cap = 0;
+    const size_t n = std::max<size_t>(want + want / 4, 256);
+    LG_TRY(dalloc(ctx, n, p));
+    *cap = n;
+    return LG_OK;
+}
+
+int32_t Pipeline::reserve_vertices(lg_ctx *ctx, int V, int S) {
+    if (V <= v_cap && S <= s_cap) return LG_OK;
+    release_vertex(ctx);
+    v_cap = std::max(V, 1);
+    s_cap = std::max(S, 1);
+    int32_t rc = LG_OK;
+    auto A = [&](auto **p, size_t n) {
+        if (rc == LG_OK) rc = dalloc(ctx, n, p);
+    };
+    A(&sd, (size_t)s_cap);
+    A(&sid, (size_t)v_cap);
+    A(&rowcnt, (size_t)v_cap + 1);
+    A(&off, (size_t)v_cap + 1);
+    A(&colcnt, (size_t)v_cap);
+    A(&risky, (size_t)v_cap);
+    A(&visit, (size_t)v_cap + 1);
+    A(&visit_pos, (size_t)v_cap + 1);
+    A(&visit_list, (size_t)v_cap);
+    A(&adeg, (size_t)v_cap + 1);
+    A(&aoff, (size_t)v_cap + 1);
+    A(&acur, (size_t)v_cap);
+    A(&parent, (size_t)v_cap);
+    A(&root, (size_t)v_cap);
+    A(&keys, (size_t)v_cap);
+    A(&keys_sorted, (size_t)v_cap);
+    A(&flag, (size_t)v_cap + 1);
+    A(&cid, (size_t)v_cap + 1);
+    A(&member, (size_t)v_cap);
+    A(&local, (size_t)v_cap);
+    A(&comp_off, (size_t)v_cap + 1);
+    A(&big_list, (size_t)v_cap);
+    A(&counters, 16);
+    // rocPRIM temporary storage for scans and sorts over up to v_cap keys
+    size_t b1 = 0, b2 = 0;
+    rocprim::radix_sort_keys(nullptr, b1, keys, keys_sorted, (size_t)v_cap, 0, 64, ctx->stream);
+    rocprim::exclusive_scan(nullptr, b2, rowcnt, off, 0, (size_t)v_cap + 1, rocprim::plus<int>(), ctx->stream);
+    tmp_vertex_bytes = std::max(b1, b2);
+    A((uint8_t **)&tmp_vertex, tmp_vertex_bytes + 256);
+    return rc;
+}
+
+void Pipeline::release_vertex(lg_ctx *ctx) {
+    void *ps[] = {sd, sid, rowcnt, off, colcnt, risky, visit, visit_pos, visit_list, adeg, aoff, acur, parent,
+                  root, keys, keys_sorted, flag, cid, member, local, comp_off, big_list, counters, tmp_vertex};
+    for (void *p : ps) dfree(ctx, p);
+    sd = nullptr;
+    sid = rowcnt = off = colcnt = visit = visit_pos = visit_list = adeg = aoff = acur = parent = root = nullptr;
+    flag = cid = member = local = comp_off = big_list = counters = nullptr;
+    risky = nullptr;
+    keys = keys_sorted = nullptr;
+    tmp_vertex = nullptr;
+    v_cap = s_cap = 0;
+}
+
+void Pipeline::release(lg_ctx *ctx) {
+    release_vertex(ctx);
+    void *ps[] = {cand, accepted, epos, tmp_cand, edges_, w_, anb, aw, big_T, big_L, big_out, big_sums, big_Ln};
+    for (void *p : ps) dfree(ctx, p);
+    cand = accepted = epos = edges_ = anb = nullptr;
+    w_ = aw = big_T = big_L = big_out = big_Ln = nullptr;
+    big_sums = nullptr;
+    tmp_cand = nullptr;
+    cand_cap = accepted_cap = epos_cap = tmp_cand_cap = edges_cap = w_cap = anb_cap = aw_cap = 0;
+    bigT_cap = bigL_cap = bigout_cap = bigsums_cap = bigLn_cap = 0;
+}
+
+static int32_t scan_excl(lg_ctx *ctx, Pipeline &pl, const int *in, int *out, size_t n, bool cand_sized) {
+    size_t bytes = 0;
+    LG_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, 0, n, rocprim::plus<int>(), ctx->stream));
+    void *tmp = nullptr;
+    if (!cand_sized && bytes <= pl.tmp_vertex_bytes) {
+        tmp = pl.tmp_vertex;
+    } else {
+        LG_TRY(grow(ctx, (uint8_t **)&pl.tmp_cand, &pl.tmp_cand_cap, bytes));
+        tmp = pl.tmp_cand;
+    }
+    LG_HIP(rocprim::exclusive_scan(tmp, bytes, in, out, 0, n, rocprim::plus<int>(), ctx->stream));
+    return LG_OK;
+}
+
+int32_t pl_begin(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, int *V_out, int *nmax_out) {
+    const int S = (int)descs.size();
+    LG_CHECK_ARG(S >= 1, "pipeline: no solve");
+    int V = 0, nmax = 0;
+    for (auto &d : descs) {
+        LG_CHECK_ARG(d.n >= 1 && d.n <= 65536, "pipeline: n=%d outside [1, 65536]", d.n);
+        LG_CHECK_ARG(d.voff == V, "pipeline: vertex offsets must be contiguous");
+        LG_CHECK_ARG(d.dtype == descs[0].dtype, "pipeline: all volumes of a batch share one brick type");
+        V += d.n;
+        nmax = std::max(nmax, d.n);
+    }
+    LG_TRY(pl.reserve_vertices(ctx, V, S));
+    LG_HIP(hipMemcpyAsync(pl.sd, descs.data(), sizeof(SolveDesc) * S, hipMemcpyHostToDevice, ctx->stream));
+    k_solve_ids<<<(unsigned)((V + 255) / 256), 256, 0, ctx->stream>>>(pl.sd, S, V, pl.sid);
+    LG_LAUNCHED("k_solve_ids");
+    *V_out = V;
+    if (nmax_out) *nmax_out = nmax;
+    return LG_OK;
+}
+
+int32_t pl_sample(lg_ctx *ctx, Pipeline &pl, int V, int dtype, uint64_t seed, float *pts) {
+    hipEvent_t ev;
+    ktimer_begin(ctx, "k_sample_points", &ev);
+    for_dtype(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        k_sample<T><<<(unsigned)((V + 255) / 256), 256, 0, ctx->stream>>>(pl.sd, pl.sid, V, (uint32_t)seed,
+                                                                         (uint32_t)(seed >> 32), pts);
+    });
+    LG_LAUNCHED("k_sample");
+    ktimer_end(ctx, "k_sample_points", ev);
+    return LG_OK;
+}
+
+int32_t pl_light(lg_ctx *ctx, Pipeline &pl, int V, int dtype, const LightConsts &lc, const lgl::DevLights &lights,
+                 uint64_t env_seed, const float *pts, float *L) {
+    hipEvent_t ev;
+    ktimer_begin(ctx, "k_vertex_light", &ev);
+    for_dtype(dtype, [&](auto tag) {
+        using T = decltype(tag);
+        k_light<T><<<(unsigned)((V + 127) / 128), 128, 0, ctx->stream>>>(pl.sd, pl.sid, V, lc, lights,
+                                                                        (uint32_t)env_seed, (uint32_t)(env_seed >> 32),
+                                                                        pts, L);
+    });
+    LG_LAUNCHED("k_light");
+    ktimer_end(ctx, "k_vertex_light", ev);
+    return LG_OK;
+}
+
+// Candidate CSR, greedy cap and the edge list (pl.edges_, count at pl.epos[Ktot]).
+int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float *pts, double r2, int max_connect,
+                 int *Ktot_out) {
+    hipStream_t st = ctx->stream;
+    const unsigned vb = (unsigned)((V + 255) / 256);
+    hipEvent_t ev;
+    ktimer_begin(ctx, "k_graph", &ev);
+    const unsigned wb = (unsigned)((V + 3) / 4);
+    LG_HIP(hipMemsetAsync(pl.rowcnt + V, 0, sizeof(int), st));
+    k_cand<<<wb, 256, 0, st>>>(pl.sd, pl.sid, V, pts, r2, nullptr, pl.rowcnt, nullptr);
+    LG_LAUNCHED("k_cand");
+    LG_TRY(scan_excl(ctx, pl, pl.rowcnt, pl.off, (size_t)V + 1, false));
+    int Ktot = 0;
+    LG_HIP(hipMemcpyAsync(&Ktot, pl.off + V, sizeof(int), hipMemcpyDeviceToHost, st));
+    LG_HIP(hipStreamSynchronize(st));
+    LG_TRY(grow(ctx, &pl.cand, &pl.cand_cap, (size_t)Ktot + 1));
+    LG_TRY(grow(ctx, &pl.accepted, &pl.accepted_cap, (size_t)Ktot + 1));
+    LG_TRY(grow(ctx, &pl.epos, &pl.epos_cap, (size_t)Ktot + 1));
+    LG_TRY(grow(ctx, &pl.edges_, &pl.edges_cap, (size_t)2 * Ktot + 2));
+    LG_TRY(grow(ctx, &pl.w_, &pl.w_cap, (size_t)Ktot + 1));
+    k_cand<<<wb, 256, 0, st>>>(pl.sd, pl.sid, V, pts, r2, pl.off, nullptr, pl.cand);
+    LG_LAUNCHED("k_cand");
+    const int cap = (max_connect > 0 && max_connect < 255) ? max_connect : 0;
+    if (cap > 0) {
+        LG_HIP(hipMemsetAsync(pl.colcnt, 0, sizeof(int) * V, st));
+        k_colcount<<<vb, 256, 0, st>>>(V, pl.off, pl.cand, pl.colcnt);
+        k_risky<<<vb, 256, 0, st>>>(V, cap, pl.off, pl.colcnt, pl.risky);
+    }
+    k_classify<<<vb, 256, 0, st>>>(V, cap, pl.off, pl.cand, pl.risky, pl.accepted, pl.visit);
+    LG_LAUNCHED("k_classify");
+    if (cap > 0) {
+        LG_HIP(hipMemsetAsync(pl.visit + V, 0, sizeof(int), st));
+        LG_TRY(scan_excl(ctx, pl, pl.visit, pl.visit_pos, (size_t)V + 1, false));
+        k_compact_rows<<<vb, 256, 0, st>>>(V, pl.visit, pl.visit_pos, pl.visit_list);
+        k_greedy<<<S, 64, (size_t)nmax, st>>>(pl.sd, cap, pl.off, pl.cand, pl.risky, pl.visit_list, pl.visit_pos + V,
+                                              pl.accepted);
+        LG_LAUNCHED("k_greedy");
+    }
+    LG_HIP(hipMemsetAsync(pl.accepted + Ktot, 0, sizeof(int), st));
+    LG_TRY(scan_excl(ctx, pl, pl.accepted, pl.epos, (size_t)Ktot + 1, true));
+    k_edges<<<vb, 256, 0, st>>>(V, pl.off, pl.cand, pl.accepted, pl.epos, pl.edges_);
+    LG_LAUNCHED("k_edges");
+    ktimer_end(ctx, "k_graph", ev);
+    *Ktot_out = Ktot;
+    return LG_OK;
+}
+
+int32_t pl_march(lg_ctx *ctx, Pipeline &pl, int dtype, const MarchConsts &mc, const float *pts, int Ktot) {
+    hipEvent_t ev;
+    ktimer_begin(ctx, "k_march_edges", &ev);
+    if (Ktot > 0) {
+        for_dtype(dtype, [&](auto tag) {
+            using T = decltype(tag);
+            k_march<T><<<(unsigned)((Ktot + 127) / 128), 128, 0, ctx->stream>>>(pl.sd, pl.sid, mc, pts, pl.edges_,
+                                                                               Ktot, pl.epos + Ktot, pl.w_);
+        });
+        LG_LAUNCHED("k_march");
+    }
+    ktimer_end(ctx, "k_march_edges", ev);
+    return LG_OK;
+}
+
+// Components, closure (c) and diffusion; dense outputs when the descriptors ask.
+int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, int V, int Ktot,
+                      const double ext[3], const float *L, float *out, PipelineResult *res) {
+    hipStream_t st = ctx->stream;
+    const int S = (int)descs.size();
+    const unsigned vb = (unsigned)((V + 255) / 256);
+    const int *d_nedges = pl.epos + Ktot;
+    hipEvent_t ev;
+    ktimer_begin(ctx, "k_components", &ev);
+    LG_HIP(hipMemsetAsync(pl.adeg, 0, sizeof(int) * ((size_t)V + 1), st));
+    LG_HIP(hipMemsetAsync(pl.acur, 0, sizeof(int) * (size_t)V, st));
+    const unsigned eb = (unsigned)((Ktot + 255) / 256 + 1);
+    k_adj_deg<<<eb, 256, 0, st>>>(pl.edges_, pl.w_, d_nedges, pl.adeg);
+    LG_TRY(scan_excl(ctx, pl, pl.adeg, pl.aoff, (size_t)V + 1, false));
+    LG_TRY(grow(ctx, &pl.anb, &pl.anb_cap, (size_t)2 * Ktot + 2));
+    LG_TRY(grow(ctx, &pl.aw, &pl.aw_cap, (size_t)2 * Ktot + 2));
+    k_adj_fill<<<eb, 256, 0, st>>>(pl.edges_, pl.w_, d_nedges, pl.aoff, pl.acur, pl.anb, pl.aw);
+    k_components<<<S, 1024, 0, st>>>(pl.sd, pl.off, pl.epos, pl.edges_, pl.w_, pl.parent, pl.root);
+    LG_LAUNCHED("k_components");
+    int bits = 1;
+    while ((1 << bits) < V) ++bits;
+    k_keys<<<vb, 256, 0, st>>>(V, bits, pl.root, pl.keys);
+    size_t sbytes = pl.tmp_vertex_bytes;
+    LG_HIP(rocprim::radix_sort_keys(pl.tmp_vertex, sbytes, pl.keys, pl.keys_sorted, (size_t)V, 0, 2 * bits, st));
+    k_comp_flags<<<vb, 256, 0, st>>>(V, bits, pl.keys_sorted, pl.flag, pl.member);
+    LG_HIP(hipMemsetAsync(pl.flag + V, 0, sizeof(int), st));
+    LG_TRY(scan_excl(ctx, pl, pl.flag, pl.cid, (size_t)V + 1, false));
+    k_comp_offsets<<<vb, 256, 0, st>>>(V, pl.flag, pl.cid, pl.comp_off, pl.cid + V);
+    LG_HIP(hipMemsetAsync(pl.counters, 0, sizeof(int) * 16, st));
+    k_comp_local<<<vb, 256, 0, st>>>(V, pl.flag, pl.cid, pl.comp_off, pl.member, pl.local, pl.big_list, pl.counters);
+    LG_LAUNCHED("k_comp_local");
+    int host_counts[3] = {0, 0, 0};
+    LG_HIP(hipMemcpyAsync(&host_counts[0], pl.cid + V, sizeof(int), hipMemcpyDeviceToHost, st));
+    LG_HIP(hipMemcpyAsync(&host_counts[1], pl.counters, sizeof(int), hipMemcpyDeviceToHost, st));
+    LG_HIP(hipMemcpyAsync(&host_counts[2], d_nedges, sizeof(int), hipMemcpyDeviceToHost, st));
+    LG_HIP(hipStreamSynchronize(st));
+    ktimer_end(ctx, "k_components", ev);
+    const int C = host_counts[0], nbig = host_counts[1];
+    res->n_edges_total = host_counts[2];
+    res->n_components = C;
+    res->n_big = nbig;
+    res->Ktot = Ktot;
+
+    // dense outputs (optional) start as +INF / -1 with the diagonal
+    for (auto &d : descs) {
+        if (d.dense_D) LG_TRY(fw_init_impl(ctx, d.n, d.dense_D, d.ld));
+        if (d.dense_pred)
+            k_pred_init<<<(unsigned)std::min<long long>(((long long)d.n * d.ld + 255) / 256, 65536), 256, 0, st>>>(
+                d.n, d.ld, d.dense_pred);
+    }
+
+    // (c) + diffusion: small components in one launch
+    const float e0 = (float)ext[0], e1 = (float)ext[1], e2 = (float)ext[2];
+    ktimer_begin(ctx, "k_comp_small", &ev);
+    if (C > 0) {
+        k_comp_small<<<C, 256, 0, st>>>(pl.sd, pl.sid, pl.comp_off, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, L, e0,
+                                        e1, e2, out);
+        LG_LAUNCHED("k_comp_small");
+    }
+    ktimer_end(ctx, "k_comp_small", ev);
+
+    // large components: blocked Floyd-Warshall on a gathered matrix
+    if (nbig > 0) {
+        std::vector<int> big(nbig), coff((size_t)C + 1), first(nbig);
+        LG_HIP(hipMemcpyAsync(big.data(), pl.big_list, sizeof(int) * nbig, hipMemcpyDeviceToHost, st));
+        LG_HIP(hipMemcpyAsync(coff.data(), pl.comp_off, sizeof(int) * ((size_t)C + 1), hipMemcpyDeviceToHost, st));
+        LG_HIP(hipStreamSynchronize(st));
+        std::sort(big.begin(), big.end());
+        for (int b = 0; b < nbig; ++b)
+            LG_HIP(hipMemcpyAsync(&first[b], pl.member + coff[big[b]], sizeof(int), hipMemcpyDeviceToHost, st));
+        LG_HIP(hipStreamSynchronize(st));
+        const double sig[3] = {-ext[0], -ext[1], -ext[2]};
+        for (int b = 0; b < nbig; ++b) {
+            const int a0 = coff[big[b]], m = coff[big[b] + 1] - coff[big[b]];
+            const int ldm = fw_ld(m);
+            LG_TRY(grow(ctx, &pl.big_T, &pl.bigT_cap, (size_t)ldm * ldm));
+            LG_TRY(grow(ctx, &pl.big_L, &pl.bigL_cap, (size_t)3 * m));
+            LG_TRY(grow(ctx, &pl.big_out, &pl.bigout_cap, (size_t)3 * m));
+            LG_TRY(grow(ctx, &pl.big_sums, &pl.bigsums_cap, (size_t)3 * m));
+            LG_TRY(grow(ctx, &pl.big_Ln, &pl.bigLn_cap, (size_t)3 * m));
+            LG_TRY(fw_init_impl(ctx, m, pl.big_T, ldm));
+            k_big_fill<<<(m + 127) / 128, 128, 0, st>>>(m, ldm, a0, pl.member, pl.local, pl.aoff, pl.anb, pl.aw,
+                                                        pl.big_T);
+            LG_LAUNCHED("k_big_fill");
+            LG_TRY(fw_solve_impl(ctx, m, pl.big_T, ldm));
+            k_big_gather_L<<<(m + 127) / 128, 128, 0, st>>>(m, a0, pl.member, L, pl.big_L);
+            LG_TRY(diffuse_impl(ctx, m, pl.big_T, ldm, sig, pl.big_L, pl.big_out, pl.big_sums, pl.big_Ln));
+            k_big_scatter_out<<<(m + 127) / 128, 128, 0, st>>>(m, a0, pl.member, pl.big_out, out);
+            LG_LAUNCHED("k_big_scatter_out");
+            int s = 0;
+            while (s + 1 < S && descs[s + 1].voff <= first[b]) ++s;
+            const SolveDesc &d = descs[s];
+            if (d.dense_D) {
+                k_big_scatter_dense<<<(unsigned)(((long long)m * m + 255) / 256), 256, 0, st>>>(
+                    m, ldm, a0, pl.member, pl.big_T, d.voff, d.dense_D, d.ld);
+                LG_LAUNCHED("k_big_scatter_dense");
+            }
+            if (d.dense_pred) {
+                k_big_pred<<<m, 256, 0, st>>>(m, ldm, a0, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, pl.big_T,
+                                              d.voff, d.dense_pred, d.ld);
+                LG_LAUNCHED("k_big_pred");
+            }
+        }
+    }
+    res->edges = pl.edges_;
+    res->w = pl.w_;
+    res->epos = pl.epos;
+    res->off = pl.off;
+    return LG_OK;
+}
+
+void destroy_stage_pipeline(lg_ctx *ctx) {
+    delete static_cast<Pipeline *>(ctx->stage_pipeline);
+    ctx->stage_pipeline = nullptr;
+}
+
+int32_t run_pipeline(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, const PipelineParams &pp,
+                     PipelineResult *res) {
+    int V = 0, nmax = 0, Ktot = 0;
+    LG_TRY(pl_begin(ctx, pl, descs, &V, &nmax));
+    const int dtype = descs[0].dtype;
+    if (!pp.external_points) LG_TRY(pl_sample(ctx, pl, V, dtype, pp.seed, pp.pts));
+    if (pp.ev_after_sample) LG_HIP(hipEventRecord(pp.ev_after_sample, ctx->stream));
+    LG_TRY(pl_light(ctx, pl, V, dtype, pp.lc, pp.lights, pp.env_seed, pp.pts, pp.L));
+    if (pp.ev_after_light) LG_HIP(hipEventRecord(pp.ev_after_light, ctx->stream));
+    LG_TRY(pl_graph(ctx, pl, V, (int)descs.size(), nmax, pp.pts, pp.r2, pp.max_connect, &Ktot));
+    LG_TRY(pl_march(ctx, pl, dtype, pp.mc, pp.pts, Ktot));
+    LG_TRY(pl_components(ctx, pl, descs, V, Ktot, pp.ext, pp.L, pp.out, res));
+    return LG_OK;
+}
+
+}  // namespace lg

@@ -311,6 +311,7 @@ int32_t lg_ctx_destroy(lg_ctx *ctx) {
         (void)hipEventDestroy(p.b);
     }
     for (auto e : ctx->ktimer.pool) (void)hipEventDestroy(e);
+    destroy_stage_pipeline(ctx);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LG_OK;

@@ -263,7 +263,7 @@ def test_full_solve_c1_against_oracle(lga, ctx, oracle, c1):
     sh = _c1_shader(lga)
     p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5)
     lights = [lga.parse_light("dir:0:1,1,1:1:-45,30,0")]
-    s = lga.Solver(ctx, 1024, with_pred=True)
+    s = lga.Solver(ctx, 1024, dense=True, with_pred=True)
     s.solve(grid, sh, lights, p, n=1024, radius=8.0, seed=854123, solve=5)
     r = s.results()
     ref = oracle.solve_points(dense, sh, p, lights, r["points"], radius=8.0, cap=12, want_P=True)
@@ -280,4 +280,74 @@ def test_full_solve_c1_against_oracle(lga, ctx, oracle, c1):
     r2 = s.results()
     for k in ("points", "edges", "D", "L", "diffuse", "pred"):
         np.testing.assert_array_equal(r[k], r2[k])
+    s.destroy()
+
+
+def _pred_rule(D, edges, w):
+    """Canonical predecessor in float32 arithmetic: smallest neighbour k of j
+    minimising D[i][k] + w(k, j); -1 if unreachable; i on the diagonal."""
+    n = len(D)
+    nbrs = [[] for _ in range(n)]
+    for (a, b), x in zip(edges, w):
+        if np.isfinite(x):
+            nbrs[a].append((b, np.float32(x)))
+            nbrs[b].append((a, np.float32(x)))
+    out = np.full((n, n), -1, np.int32)
+    for j in range(n):
+        ks = np.array(sorted(k for k, _ in nbrs[j]), dtype=np.int64)
+        ws = dict(nbrs[j])
+        wv = np.array([ws[k] for k in ks], dtype=np.float32)
+        for i in range(n):
+            if i == j:
+                out[i, j] = i
+            elif np.isfinite(D[i, j]) and len(ks):
+                vals = (D[i, ks] + wv).astype(np.float32)
+                out[i, j] = ks[int(np.argmin(vals))]
+    return out
+
+
+def test_full_solve_sparse_components(lga, ctx, oracle):
+    """A sparse graph (many small components, closed in LDS) against the oracle's
+    dense max-product solve; predecessors against the canonical rule."""
+    spec = lga.scene_spec(lga.LG_SCENE_BLOBS_FBM, 128, seed=9, octaves=6, blobs=20, axis_min=10, axis_max=40,
+                          active_fraction=0.3)
+    grid, _ = lga.Grid.generate(ctx, spec)
+    dense = grid.read_dense()
+    sh = lga.default_shader(absorption=0.1, scattering=(1.0, 0.9, 0.8))
+    p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5)
+    lights = [lga.point_light(0.0, (1.0, 0.9, 0.8), 4.0e4, (64.0, 200.0, 64.0)),
+              lga.parse_light("dir:0:1,0.89,0.79:0.7:-41.8,-180,0")]
+    n = 1500
+    s = lga.Solver(ctx, n, dense=True, with_pred=True)
+    s.solve(grid, sh, lights, p, n=n, radius=5.0, seed=77, solve=3)
+    r = s.results()
+    assert r["n_components"] > 50
+    ref = oracle.solve_points(dense, sh, p, lights, r["points"], radius=5.0, cap=12, want_P=True)
+    np.testing.assert_array_equal(r["edges"], ref["edges"])
+    assert rel_err(r["L"], ref["L"]) < 1e-5
+    sig0 = -sh.extinction[0]
+    tau_ref = np.where(ref["P"][..., 0] > 0, -np.log(np.maximum(ref["P"][..., 0], 1e-300)) / sig0, np.inf)
+    assert np.array_equal(np.isinf(r["D"]), np.isinf(tau_ref))
+    fin = np.isfinite(tau_ref)
+    assert rel_err(r["D"][fin], tau_ref[fin], floor=1e-30) < 1e-5
+    assert rel_err(r["diffuse"], ref["diffuse"]) < 1e-5
+    np.testing.assert_array_equal(r["pred"], _pred_rule(r["D"], r["edges"], r["weights"]))
+    # without the dense outputs the radiance is the same
+    s.solve(grid, sh, lights, p, n=n, radius=5.0, seed=77, solve=3, dense=False)
+    np.testing.assert_array_equal(s.results(want_D=False)["diffuse"], r["diffuse"])
+    s.destroy()
+    grid.destroy()
+
+
+def test_full_solve_giant_component_pred(lga, ctx, oracle, c1):
+    """C1 graph: one component far above the LDS limit goes through the blocked
+    kernels; its predecessors follow the canonical rule."""
+    grid, dense, _ = c1
+    sh = _c1_shader(lga)
+    p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5)
+    s = lga.Solver(ctx, 600, dense=True, with_pred=True)
+    s.solve(grid, sh, [lga.parse_light("dir:0:1,1,1:1:-45,30,0")], p, n=600, radius=8.0, seed=3, solve=1)
+    r = s.results()
+    assert r["n_components"] < 10
+    np.testing.assert_array_equal(r["pred"], _pred_rule(r["D"], r["edges"], r["weights"]))
     s.destroy()
