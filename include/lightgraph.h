@@ -135,9 +135,11 @@ typedef struct lg_films {
 
 /* Scatter grid (multiple-scattering radiance, Vec3f) on a coarse lattice:
  * world = index * voxel_size (no origin, like createLinearTransform in
- * ScatterGrid.h:310).  Dense, x fastest, 3 floats per voxel. */
+ * ScatterGrid.h:310).  Dense storage covers coarse indices [i0, i0 + n) per axis,
+ * x fastest, 3 floats per voxel; voxels outside it or inactive read 0. */
 typedef struct lg_scatter_grid {
-    int32_t nx, ny, nz, pad_;
+    int32_t nx, ny, nz;
+    int32_t i0[3];         /* coarse index of storage element (0,0,0) */
     double voxel_size;
     float *d_rgb;          /* nx*ny*nz*3 floats                    */
     uint8_t *d_active;     /* nx*ny*nz flags (1 = active voxel)    */

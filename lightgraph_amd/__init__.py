@@ -401,3 +401,71 @@ def camera_setup(cam_rotate=(0, 0, 0), cam_translate=(0, 0, 0), cam_y_rotate=0.0
     _check(lib().lg_camera_setup(_d3(cam_rotate), _d3(cam_translate), cam_y_rotate, focal, aperture, znear, zfar,
                                  width, height, C.byref(cam)), "camera_setup")
     return cam
+
+
+# ---------------------------------------------------------- splat / render ---
+def splat_radius(points, kernel_total):
+    """(radius, point-grid voxel size) of the splat kernel (ScatterGrid.h:71-73)."""
+    p = np.ascontiguousarray(points, dtype=np.float32).reshape(-1, 3)
+    r = C.c_double()
+    v = C.c_double()
+    _check(lib().lg_splat_radius(p.ctypes.data, len(p), int(kernel_total), C.byref(r), C.byref(v)), "splat_radius")
+    return r.value, v.value
+
+
+class ScatterGrid:
+    """Coarse Vec3f multiple-scattering grid (ScatterGrid.h)."""
+
+    def __init__(self, ctx, density, res_factor=4):
+        self.ctx = ctx
+        self.sg = lg_scatter_grid()
+        _check(lib().lg_scatter_grid_create(ctx.handle, density.handle, res_factor, C.byref(self.sg)),
+               "scatter_grid_create")
+
+    @property
+    def shape(self):
+        return (self.sg.nz, self.sg.ny, self.sg.nx)
+
+    def splat(self, pts_buf, rad_buf, count, radius):
+        _check(lib().lg_splat(self.ctx.handle, pts_buf.ptr if pts_buf else None, rad_buf.ptr if rad_buf else None,
+                              int(count), float(radius), C.byref(self.sg)), "splat")
+
+    def read(self):
+        nz, ny, nx = self.shape
+        rgb = np.empty((nz, ny, nx, 3), np.float32)
+        act = np.empty((nz, ny, nx), np.uint8)
+        _check(lib().lg_memcpy_d2h(self.ctx.handle, rgb.ctypes.data, self.sg.d_rgb, rgb.nbytes), "d2h")
+        _check(lib().lg_memcpy_d2h(self.ctx.handle, act.ctypes.data, self.sg.d_active, act.nbytes), "d2h")
+        return rgb, act
+
+    def write_rgb(self, rgb):
+        a = np.ascontiguousarray(rgb, dtype=np.float32)
+        _check(lib().lg_memcpy_h2d(self.ctx.handle, self.sg.d_rgb, a.ctypes.data, a.nbytes), "h2d")
+
+    def destroy(self):
+        if self.sg.d_rgb:
+            lib().lg_scatter_grid_destroy(self.ctx.handle, C.byref(self.sg))
+
+
+class Films:
+    """The four RGBA float32 AOV films (Renderer.h:68-121) on the device."""
+
+    def __init__(self, ctx, width, height):
+        self.ctx, self.width, self.height = ctx, width, height
+        self.bufs = [ctx.alloc(width * height * 16) for _ in range(4)]
+        self.s = lg_films(*[b.ptr for b in self.bufs])
+
+    def read(self):
+        return np.stack([b.read(np.float32, (self.height, self.width, 4)) for b in self.bufs])
+
+    def free(self):
+        for b in self.bufs:
+            b.free()
+
+
+def render(ctx, density, scatter, temperature, camera, shader, lights, params, samples, films, seed=1,
+           env_seed=4564446):
+    arr = light_array(lights)
+    _check(lib().lg_render(ctx.handle, density.handle, C.byref(scatter.sg) if scatter else None,
+                           temperature.handle if temperature else None, C.byref(camera), C.byref(shader), arr,
+                           len(lights), C.byref(params), samples, seed, env_seed, C.byref(films.s)), "render")

@@ -63,7 +63,7 @@ class lg_films(C.Structure):
 
 
 class lg_scatter_grid(C.Structure):
-    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("pad_", C.c_int32),
+    _fields_ = [("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("i0", C.c_int32 * 3),
                 ("voxel_size", C.c_double), ("d_rgb", C.c_void_p), ("d_active", C.c_void_p)]
 
 

@@ -7,6 +7,8 @@
 #include <unordered_set>
 #include <vector>
 
+#include <rocprim/device/device_scan.hpp>
+
 #include "lg_internal.h"
 #include "lg_lights.h"
 
@@ -75,6 +77,395 @@ __global__ void k_env_marg(const double *__restrict__ rowint, int h, double *__r
         for (int i = 1; i <= h; ++i) marg[i] = marg[i] / I;
     }
     *total = I;
+}
+
+// ----------------------------------------------------------- scatter grid ----
+struct CoarseView {
+    int nx, ny, nz;
+    int i0x, i0y, i0z;
+    double cv, inv_cv;
+    const float *rgb;
+};
+
+inline CoarseView coarse_view(const lg_scatter_grid *sg) {
+    CoarseView c{};
+    c.nx = sg->nx;
+    c.ny = sg->ny;
+    c.nz = sg->nz;
+    c.i0x = sg->i0[0];
+    c.i0y = sg->i0[1];
+    c.i0z = sg->i0[2];
+    c.cv = sg->voxel_size;
+    c.inv_cv = 1.0 / sg->voxel_size;
+    c.rgb = sg->d_rgb;
+    return c;
+}
+
+__device__ __forceinline__ void coarse_tap(const CoarseView &c, int x, int y, int z, float v[3]) {
+    x -= c.i0x;
+    y -= c.i0y;
+    z -= c.i0z;
+    if (x < 0 || y < 0 || z < 0 || x >= c.nx || y >= c.ny || z >= c.nz) {
+        v[0] = v[1] = v[2] = 0.0f;
+        return;
+    }
+    const float *p = c.rgb + 3 * (((long long)z * c.ny + y) * c.nx + x);
+    v[0] = p[0];
+    v[1] = p[1];
+    v[2] = p[2];
+}
+
+// BoxSampler on the Vec3f scatter grid at coarse index position (x, y, z).
+__device__ __forceinline__ void coarse_sample(const CoarseView &c, double x, double y, double z, float out[3]) {
+    const int ix = (int)floor(x), iy = (int)floor(y), iz = (int)floor(z);
+    const double u = x - (double)ix, v = y - (double)iy, w = z - (double)iz;
+    float d[2][2][2][3];
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b)
+            for (int e = 0; e < 2; ++e) coarse_tap(c, ix + a, iy + b, iz + e, d[a][b][e]);
+    for (int ch = 0; ch < 3; ++ch) {
+        const float c00 = lgd::box_lerp(d[0][0][0][ch], d[0][0][1][ch], w);
+        const float c01 = lgd::box_lerp(d[0][1][0][ch], d[0][1][1][ch], w);
+        const float c10 = lgd::box_lerp(d[1][0][0][ch], d[1][0][1][ch], w);
+        const float c11 = lgd::box_lerp(d[1][1][0][ch], d[1][1][1][ch], w);
+        out[ch] = lgd::box_lerp(lgd::box_lerp(c00, c01, v), lgd::box_lerp(c10, c11, v), u);
+    }
+}
+
+// Coarse topology: a coarse voxel is active when a density voxel of its
+// footprint (f voxels centred on it) is non-zero.
+template <typename T>
+__global__ void k_coarse_topology(lgd::GridView g, int nxd, int nyd, int nzd, int f, CoarseView c,
+                                  uint8_t *__restrict__ act) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)c.nx * c.ny * c.nz;
+    if (k >= total) return;
+    const int x = (int)(k % c.nx), y = (int)((k / c.nx) % c.ny), z = (int)(k / ((long long)c.nx * c.ny));
+    const int C[3] = {x + c.i0x, y + c.i0y, z + c.i0z};
+    const int nd[3] = {nxd, nyd, nzd};
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        const double centre = ((double)C[a] * c.cv - g.origin[a]) * g.inv_voxel;
+        lo[a] = (int)floor(centre - 0.5 * f + 0.5);
+        hi[a] = lo[a] + f;
+        lo[a] = lo[a] < 0 ? 0 : lo[a];
+        hi[a] = hi[a] > nd[a] ? nd[a] : hi[a];
+    }
+    const T *d = static_cast<const T *>(g.data);
+    uint8_t on = 0;
+    for (int zz = lo[2]; zz < hi[2] && !on; ++zz)
+        for (int yy = lo[1]; yy < hi[1] && !on; ++yy)
+            for (int xx = lo[0]; xx < hi[0]; ++xx)
+                if (lgd::to_f32(d[lgd::voxel_offset(g, xx, yy, zz)]) != 0.0f) {
+                    on = 1;
+                    break;
+                }
+    act[k] = on;
+}
+
+// dilateActiveValues(NN_FACE_EDGE_VERTEX, 1 iteration)
+__global__ void k_dilate26(int nx, int ny, int nz, const uint8_t *__restrict__ in, uint8_t *__restrict__ out) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)nx * ny * nz;
+    if (k >= total) return;
+    const int x = (int)(k % nx), y = (int)((k / nx) % ny), z = (int)(k / ((long long)nx * ny));
+    uint8_t on = 0;
+    for (int dz = -1; dz <= 1 && !on; ++dz)
+        for (int dy = -1; dy <= 1 && !on; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int xx = x + dx, yy = y + dy, zz = z + dz;
+                if (xx < 0 || yy < 0 || zz < 0 || xx >= nx || yy >= ny || zz >= nz) continue;
+                if (in[((long long)zz * ny + yy) * nx + xx]) {
+                    on = 1;
+                    break;
+                }
+            }
+    out[k] = on;
+}
+
+// ------------------------------------------------------------------ splat ----
+struct BinGrid {
+    double lo[3];
+    double inv_cs;
+    int n[3];
+};
+
+__device__ __forceinline__ int bin_axis(const BinGrid &b, int a, double p) {
+    int i = (int)floor((p - b.lo[a]) * b.inv_cs);
+    return i < 0 ? 0 : (i >= b.n[a] ? b.n[a] - 1 : i);
+}
+
+__device__ __forceinline__ long long bin_of(const BinGrid &b, const float *p) {
+    const int x = bin_axis(b, 0, p[0]), y = bin_axis(b, 1, p[1]), z = bin_axis(b, 2, p[2]);
+    return ((long long)z * b.n[1] + y) * b.n[0] + x;
+}
+
+__global__ void k_bin_count(const float *__restrict__ pts, long long count, BinGrid b, int *__restrict__ cnt,
+                            int *__restrict__ cell) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const long long c = bin_of(b, pts + 3 * i);
+    cell[i] = (int)c;
+    atomicAdd(&cnt[c], 1);
+}
+
+__global__ void k_bin_fill(long long count, const int *__restrict__ cell, const int *__restrict__ off,
+                           int *__restrict__ cursor, int *__restrict__ idx) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const int c = cell[i];
+    idx[off[c] + atomicAdd(&cursor[c], 1)] = (int)i;
+}
+
+// ScatterGrid::operator() (ScatterGrid.h:61-98) for every active coarse voxel:
+// Gaussian-weighted mean of the radiance of the points within radius r.
+__global__ void k_splat(CoarseView c, const uint8_t *__restrict__ act, const float *__restrict__ pts,
+                        const float *__restrict__ rad, BinGrid b, int reach, const int *__restrict__ off,
+                        const int *__restrict__ idx, double r2, double inv2sig2, float *__restrict__ out) {
+    const long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long total = (long long)c.nx * c.ny * c.nz;
+    if (k >= total) return;
+    float *o = out + 3 * k;
+    if (!act[k]) {
+        o[0] = o[1] = o[2] = 0.0f;
+        return;
+    }
+    const int x = (int)(k % c.nx), y = (int)((k / c.nx) % c.ny), z = (int)(k / ((long long)c.nx * c.ny));
+    const double w[3] = {(double)(x + c.i0x) * c.cv, (double)(y + c.i0y) * c.cv, (double)(z + c.i0z) * c.cv};
+    const int bx = bin_axis(b, 0, w[0]), by = bin_axis(b, 1, w[1]), bz = bin_axis(b, 2, w[2]);
+    double tot = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int zz = max(0, bz - reach); zz <= min(b.n[2] - 1, bz + reach); ++zz)
+        for (int yy = max(0, by - reach); yy <= min(b.n[1] - 1, by + reach); ++yy)
+            for (int xx = max(0, bx - reach); xx <= min(b.n[0] - 1, bx + reach); ++xx) {
+                const long long cell = ((long long)zz * b.n[1] + yy) * b.n[0] + xx;
+                for (int q = off[cell]; q < off[cell + 1]; ++q) {
+                    const int i = idx[q];
+                    const double dx = w[0] - (double)pts[3 * i], dy = w[1] - (double)pts[3 * i + 1],
+                                 dz = w[2] - (double)pts[3 * i + 2];
+                    if (!(dx * dx + dy * dy + dz * dz < r2)) continue;
+                    const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+                    const float d2 = fx * fx + fy * fy + fz * fz;  // Vec3s::lengthSqr
+                    const double wt = exp(-inv2sig2 * (double)d2);
+                    tot += wt;
+                    s0 += (double)rad[3 * i] * wt;
+                    s1 += (double)rad[3 * i + 1] * wt;
+                    s2 += (double)rad[3 * i + 2] * wt;
+                }
+            }
+    if (tot > 0.0) {
+        s0 /= tot;
+        s1 /= tot;
+        s2 /= tot;
+    }
+    o[0] = (float)s0;
+    o[1] = (float)s1;
+    o[2] = (float)s2;
+}
+
+// ------------------------------------------------------------------ render ---
+struct RenderConsts {
+    double ds, sds, dmax, cutoff, step0, step1, tau_star;
+    double albedo[3], ext[3], inv_scat[3];
+    double scatter_scale, mask_pow, emission_scale;
+    double ramp_in, ramp_out;
+    int ramp_n;
+    double ramp_pos[LG_MAX_RAMP];
+    double ramp_col[LG_MAX_RAMP][3];
+    double M[16];  // screen to world
+    double scale_w, scale_h, znear, zfar;
+    int width, height, samples;
+    double ratio_aa;
+};
+
+__device__ __forceinline__ void render_ramp(const RenderConsts &c, double x, double out[3]) {
+    if (c.ramp_n == 0) {
+        out[0] = out[1] = out[2] = 0.0;
+        return;
+    }
+    double hp = (x - c.ramp_in) / (c.ramp_out - c.ramp_in);
+    hp = (hp < 0.0) ? 0.0 : hp;
+    hp = (1.0 < hp) ? 1.0 : hp;
+    if (c.ramp_n == 1) {
+        for (int k = 0; k < 3; ++k) out[k] = c.ramp_col[0][k];
+        return;
+    }
+    int prev = 0, next = 0;
+    for (int i = 0; i < c.ramp_n; ++i) {
+        if (c.ramp_pos[i] <= hp) {
+            prev = i;
+        } else {
+            next = i;
+            break;
+        }
+    }
+    const double range = c.ramp_pos[next] - c.ramp_pos[prev];
+    const double blend = (hp - c.ramp_pos[prev]) / range;
+    for (int k = 0; k < 3; ++k) out[k] = c.ramp_col[prev][k] * (1.0 - blend) + c.ramp_col[next][k] * blend;
+}
+
+// Shadow march of the render (Renderer.h:615-643): jittered step redrawn at
+// each span and after each sample that passed the cutoff.  Returns false when
+// the ray misses the bbox; *T receives the transmittance (0 after dropout).
+template <typename T>
+__device__ __forceinline__ bool render_shadow(const lgd::GridView &g, const RenderConsts &c, const double p[3],
+                                              const double ldir[3], double distance, lgd::Stream &rng,
+                                              double *tau_out, bool *dropped) {
+    const double sd[3] = {-ldir[0], -ldir[1], -ldir[2]};
+    lgd::Isect is;
+    if (!lgd::isect_world_ray(g, is, p, sd, lgd::kDeltaD, distance)) return false;
+    double tau = 0.0;
+    double t0, t1;
+    *dropped = false;
+    while (lgd::isect_march(g, is, t0, t1)) {
+        double step = c.step1 * (1.0 + ((double)rng.next() - 0.5) * 0.5);
+        for (double t = c.step1 * ceil(t0 / c.step1); t <= t1; t += step) {
+            const float rho = lgd::trilinear<T>(g, is.e[0] + is.d[0] * t, is.e[1] + is.d[1] * t, is.e[2] + is.d[2] * t);
+            double dens = (double)rho * c.ds * c.sds;
+            dens = (c.dmax < dens) ? c.dmax : dens;
+            if (dens < c.cutoff) continue;
+            tau += dens * step;
+            if (tau > c.tau_star) {
+                *dropped = true;
+                *tau_out = tau;
+                return true;
+            }
+            step = c.step1 * (1.0 + ((double)rng.next() - 0.5) * 0.5);
+        }
+    }
+    *tau_out = tau;
+    return true;
+}
+
+// Kernel (d): one lane per pixel, AA samples in sequence (Renderer.h:524-689).
+template <typename T>
+__global__ void __launch_bounds__(256) k_render(lgd::GridView g, lgd::GridView tg, int has_temp, CoarseView sc,
+                                                int has_scatter, RenderConsts c, lgl::DevLights L, uint32_t k0,
+                                                uint32_t k1, uint32_t ek0, uint32_t ek1, float4 *__restrict__ rgb,
+                                                float4 *__restrict__ direct, float4 *__restrict__ scatter,
+                                                float4 *__restrict__ emission) {
+    const int i = blockIdx.x * 16 + (threadIdx.x & 15);
+    const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (i >= c.width || j >= c.height) return;
+    const uint32_t pix = (uint32_t)j * (uint32_t)c.width + (uint32_t)i;
+    // camera ray origin and axis (PerspectiveCamera::initRay, Renderer.h:232-239)
+    const double eye[3] = {c.M[12], c.M[13], c.M[14]};
+    const double axis[3] = {0.0 * c.M[0] + 0.0 * c.M[4] + -1.0 * c.M[8], 0.0 * c.M[1] + 0.0 * c.M[5] + -1.0 * c.M[9],
+                            0.0 * c.M[2] + 0.0 * c.M[6] + -1.0 * c.M[10]};
+    double lumAA[3] = {0, 0, 0}, dirAA[3] = {0, 0, 0}, scaAA[3] = {0, 0, 0}, emiAA[3] = {0, 0, 0};
+    double transAA[3] = {0, 0, 0};
+    for (int s = 0; s < c.samples; ++s) {
+        lgd::Stream rng, env;
+        rng.init(pix, (uint32_t)s, lgd::DOM_RENDER, ((uint64_t)k1 << 32) | k0);
+        env.init(pix, (uint32_t)s, lgd::DOM_RENDER_ENV, ((uint64_t)ek1 << 32) | ek0);
+        // PerspectiveCamera::getRay (Renderer.h:216-229)
+        const double ri = (double)i + 0.5 + ((double)rng.next() - 0.5);
+        const double rj = (double)j + 0.5 + ((double)rng.next() - 0.5);
+        const double sx = (2.0 * ri / (double)c.width - 1.0) * c.scale_w;
+        const double sy = (1.0 - 2.0 * rj / (double)c.height) * c.scale_h;
+        const double sz = -1.0;
+        double dir[3];
+        for (int a = 0; a < 3; ++a) dir[a] = sx * c.M[a] + sy * c.M[4 + a] + sz * c.M[8 + a];
+        const double dl = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+        if (!(fabs(dl - 0.0) <= 1e-7)) {
+            const double inv = 1.0 / dl;
+            for (int a = 0; a < 3; ++a) dir[a] = dir[a] * inv;
+        }
+        const double scale = 1.0 / (dir[0] * axis[0] + dir[1] * axis[1] + dir[2] * axis[2]);
+        lgd::Isect pr;
+        if (!lgd::isect_world_ray(g, pr, eye, dir, c.znear * scale, c.zfar * scale)) {
+            for (int a = 0; a < 3; ++a) transAA[a] += 1.0 * c.ratio_aa;
+            continue;
+        }
+        double lum[3] = {0, 0, 0}, dl_[3] = {0, 0, 0}, sl[3] = {0, 0, 0}, el[3] = {0, 0, 0};
+        double tau_p = 0.0;
+        bool pdrop = false;
+        double t0, t1;
+        while (!pdrop && lgd::isect_march(g, pr, t0, t1)) {
+            double step = c.step0 * (1.0 + ((double)rng.next() - 0.5) * 0.5);
+            for (double t = c.step0 * ceil(t0 / c.step0); t <= t1; t += step) {
+                const double ip[3] = {pr.e[0] + pr.d[0] * t, pr.e[1] + pr.d[1] * t, pr.e[2] + pr.d[2] * t};
+                const double wp[3] = {g.origin[0] + g.voxel * ip[0], g.origin[1] + g.voxel * ip[1],
+                                      g.origin[2] + g.voxel * ip[2]};
+                const float rho = lgd::trilinear<T>(g, ip[0], ip[1], ip[2]);
+                double pd = (double)rho * c.ds;
+                pd = (c.dmax < pd) ? c.dmax : pd;
+                if (pd < c.cutoff) continue;
+                // single scattering, multiple scattering and emission terms
+                const float trans_f[3] = {expf((float)(c.ext[0] * tau_p)), expf((float)(c.ext[1] * tau_p)),
+                                          expf((float)(c.ext[2] * tau_p))};
+                double one_m_dt[3];
+                for (int a = 0; a < 3; ++a) one_m_dt[a] = (double)(-expm1f((float)(c.ext[a] * pd * step)));
+                for (int li = 0; li < L.n; ++li) {
+                    const lgl::DevLight &lt = L.l[li];
+                    double tl[3] = {0, 0, 0};
+                    for (int ls = 0; ls < lt.samples; ++ls) {
+                        double ldir[3], inten[3], dist;
+                        lgl::illuminate(lt, wp, env, ldir, inten, dist);
+                        double tau_s;
+                        bool sdrop;
+                        if (!render_shadow<T>(g, c, wp, ldir, dist, rng, &tau_s, &sdrop)) continue;
+                        if (sdrop) continue;
+                        for (int a = 0; a < 3; ++a) tl[a] += inten[a] * (double)expf((float)(c.ext[a] * tau_s));
+                    }
+                    for (int a = 0; a < 3; ++a) {
+                        const double v = c.albedo[a] * (tl[a] / (double)lt.samples) * (double)trans_f[a] * one_m_dt[a];
+                        lum[a] += v;
+                        dl_[a] += v;
+                    }
+                }
+                double msc[3] = {0, 0, 0};
+                if (has_scatter) {
+                    float sv[3];
+                    coarse_sample(sc, wp[0] * sc.inv_cv, wp[1] * sc.inv_cv, wp[2] * sc.inv_cv, sv);
+                    const double pc = pd < 0.0 ? 0.0 : (pd > 1.0 ? 1.0 : pd);
+                    const double mask = (double)powf((float)pc, (float)c.mask_pow);
+                    for (int a = 0; a < 3; ++a) msc[a] = (double)sv[a] * c.inv_scat[a] * mask * c.scatter_scale;
+                }
+                double emi[3] = {0, 0, 0};
+                if (c.ramp_n > 0) {
+                    double temp = 0.0;
+                    if (has_temp) temp = (double)lgd::trilinear<float>(tg, ip[0], ip[1], ip[2]);
+                    render_ramp(c, temp, emi);
+                    for (int a = 0; a < 3; ++a) emi[a] *= c.emission_scale;
+                }
+                for (int a = 0; a < 3; ++a) {
+                    const double ts = msc[a] * (double)trans_f[a] * one_m_dt[a];
+                    const double te = emi[a] * (double)trans_f[a] * one_m_dt[a];
+                    sl[a] += ts;
+                    el[a] += te;
+                    lum[a] += ts + te;
+                }
+                tau_p += pd * step;
+                if (tau_p > c.tau_star) {
+                    pdrop = true;
+                    break;
+                }
+                step = c.step0 * (1.0 + ((double)rng.next() - 0.5) * 0.5);
+            }
+        }
+        for (int a = 0; a < 3; ++a) {
+            lumAA[a] += lum[a] * c.ratio_aa;
+            dirAA[a] += dl_[a] * c.ratio_aa;
+            scaAA[a] += sl[a] * c.ratio_aa;
+            emiAA[a] += el[a] * c.ratio_aa;
+            transAA[a] += (double)expf((float)(c.ext[a] * tau_p)) * c.ratio_aa;  // pTrans is kept at dropout
+        }
+    }
+    const float alpha = (float)(1.0f - (transAA[0] + transAA[1] + transAA[2]) / 3.0f);
+    if (rgb) rgb[pix] = make_float4((float)lumAA[0], (float)lumAA[1], (float)lumAA[2], alpha);
+    if (direct) direct[pix] = make_float4((float)dirAA[0], (float)dirAA[1], (float)dirAA[2], 0.0f);
+    if (scatter) scatter[pix] = make_float4((float)scaAA[0], (float)scaAA[1], (float)scaAA[2], 0.0f);
+    if (emission) emission[pix] = make_float4((float)emiAA[0], (float)emiAA[1], (float)emiAA[2], 0.0f);
+}
+
+template <typename F>
+static void for_dtype_r(int dtype, F &&f) {
+    if (dtype == LG_F32)
+        f(float{});
+    else if (dtype == LG_F16)
+        f(_Float16{});
+    else
+        f(lgd::bf16_t{});
 }
 
 }  // namespace lg
@@ -185,34 +576,257 @@ int32_t lg_envmap_read_cdf(lg_ctx *ctx, const lg_envmap *e, double *cond, double
     return LG_OK;
 }
 
+// computeVoxelSize(points, 8) (used by LightGraph::buildVdbPointGrid,
+// LightGraph.h:713-714) restated: voxel volume = bbox volume / (count / 8),
+// refined by counting the occupied voxels of a cell-centred mask until the
+// estimate stops shrinking by more than 10%, truncated to 5 decimals.  Then the
+// splat radius of ScatterGrid.h:71-73.  Parity unpinned against OpenVDB.
 int32_t lg_splat_radius(const float *h_pts, int64_t count, int64_t kernel_total, double *radius_out,
                         double *voxel_out) {
-    (void)h_pts;
-    (void)count;
-    (void)kernel_total;
-    (void)radius_out;
-    (void)voxel_out;
-    set_error("lg_splat_radius: not built yet");
-    return LG_ERR_UNSUPPORTED;
+    LG_CHECK_ARG(h_pts && radius_out && count >= 0 && kernel_total >= 0, "lg_splat_radius: bad argument");
+    float vs = 0.1f;
+    if (count > 1) {
+        int64_t target = count / 8;
+        if (target == 0) target = 1;
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int64_t i = 0; i < count; ++i)
+            for (int c = 0; c < 3; ++c) {
+                lo[c] = std::min(lo[c], (double)h_pts[3 * i + c]);
+                hi[c] = std::max(hi[c], (double)h_pts[3 * i + c]);
+            }
+        double ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+        bool coincident = ext[0] <= 1e-7 && ext[1] <= 1e-7 && ext[2] <= 1e-7;
+        if (!coincident) {
+            double volume = ext[0] * ext[1] * ext[2];
+            if (volume <= 1e-7) {  // collinear or coplanar points
+                std::sort(ext, ext + 3);
+                volume = (ext[1] <= 1e-7) ? ext[2] * ext[2] * ext[2] : ext[2] * ext[2] * ext[1];
+            }
+            auto from_volume = [&](double vol, float &v) {
+                double vv = vol / (double)target;
+                bool ok = true;
+                if (vv < 3e-15) {
+                    vv = 3e-15;
+                    ok = false;
+                }
+                v = (float)pow(vv, 1.0 / 3.0);
+                return ok;
+            };
+            double prev_volume = volume;
+            if (from_volume(volume, vs)) {
+                int64_t prev_count = 0, vcount = 1;
+                std::unordered_set<uint64_t> cells;
+                while (vcount > prev_count) {
+                    cells.clear();
+                    const double inv = 1.0 / (double)vs;
+                    for (int64_t i = 0; i < count; ++i) {
+                        uint64_t key = 0;
+                        for (int c = 0; c < 3; ++c) {
+                            const int64_t q = (int64_t)floor((double)h_pts[3 * i + c] * inv + 0.5);
+                            key = key * 2097152ull + (uint64_t)((q + 1048576) & 2097151);
+                        }
+                        cells.insert(key);
+                    }
+                    prev_count = vcount;
+                    vcount = (int64_t)cells.size();
+                    volume = (double)(vs * vs * vs * (float)vcount);
+                    if (volume >= prev_volume) break;
+                    prev_volume = volume;
+                    const float prev_vs = vs;
+                    if (!from_volume(volume, vs)) {
+                        vs = prev_vs;
+                        break;
+                    }
+                    if (vs / prev_vs > 0.9f) break;
+                }
+            }
+            for (int d = 5; d < 11; ++d) {  // truncate for readability (5 decimals)
+                const double t = pow(10.0, d);
+                const float tv = (float)(floor((double)vs * t + 0.5) / t);
+                if (tv != 0.0f) {
+                    vs = tv;
+                    break;
+                }
+            }
+        }
+    }
+    const double v = (double)vs;
+    *voxel_out = v;
+    *radius_out = pow((3.0 * (double)kernel_total * v * v * v) / (32.0 * M_PI), 1.0 / 3.0);
+    return LG_OK;
 }
 
-int32_t lg_scatter_grid_create(lg_ctx *, const lg_grid *, int32_t, lg_scatter_grid *) {
-    set_error("lg_scatter_grid_create: not built yet");
-    return LG_ERR_UNSUPPORTED;
+int32_t lg_scatter_grid_create(lg_ctx *ctx, const lg_grid *density, int32_t res_factor, lg_scatter_grid *sg) {
+    LG_CHECK_ARG(ctx && density && sg && res_factor >= 1, "lg_scatter_grid_create: bad argument");
+    memset(sg, 0, sizeof(*sg));
+    const lg_grid_desc &d = density->desc;
+    const double cv = d.voxel_size * (double)res_factor;  // ScatterGrid.h:309-310
+    const int nd[3] = {d.nx, d.ny, d.nz};
+    int n[3], i0[3];
+    for (int a = 0; a < 3; ++a) {
+        const double wlo = d.origin[a] - 0.5 * d.voxel_size, whi = d.origin[a] + d.voxel_size * (nd[a] - 0.5);
+        i0[a] = (int)floor(wlo / cv) - 2;
+        n[a] = (int)ceil(whi / cv) + 2 - i0[a] + 1;
+    }
+    sg->nx = n[0];
+    sg->ny = n[1];
+    sg->nz = n[2];
+    for (int a = 0; a < 3; ++a) sg->i0[a] = i0[a];
+    sg->voxel_size = cv;
+    const size_t vox = (size_t)n[0] * n[1] * n[2];
+    uint8_t *tmp = nullptr;
+    LG_TRY(dalloc(ctx, vox * 3, &sg->d_rgb));
+    int32_t rc = dalloc(ctx, vox, &sg->d_active);
+    if (rc == LG_OK) rc = dalloc(ctx, vox, &tmp);
+    if (rc == LG_OK) {
+        LG_HIP(hipMemsetAsync(sg->d_rgb, 0, vox * 12, ctx->stream));
+        const CoarseView c = coarse_view(sg);
+        const lgd::GridView g = make_view(density);
+        const unsigned blocks = (unsigned)((vox + 255) / 256);
+        for_dtype_r(d.dtype, [&](auto tag) {
+            using T = decltype(tag);
+            k_coarse_topology<T><<<blocks, 256, 0, ctx->stream>>>(g, d.nx, d.ny, d.nz, res_factor, c, tmp);
+        });
+        k_dilate26<<<blocks, 256, 0, ctx->stream>>>(n[0], n[1], n[2], tmp, sg->d_active);
+        if (hipGetLastError() != hipSuccess) {
+            set_error("scatter grid topology kernels failed to launch");
+            rc = LG_ERR_HIP;
+        }
+    }
+    dfree(ctx, tmp);
+    if (rc != LG_OK) lg_scatter_grid_destroy(ctx, sg);
+    return rc;
 }
 
-int32_t lg_scatter_grid_destroy(lg_ctx *, lg_scatter_grid *) { return LG_OK; }
-
-int32_t lg_splat(lg_ctx *, const float *, const float *, int64_t, double, lg_scatter_grid *) {
-    set_error("lg_splat: not built yet");
-    return LG_ERR_UNSUPPORTED;
+int32_t lg_scatter_grid_destroy(lg_ctx *ctx, lg_scatter_grid *sg) {
+    if (!sg) return LG_OK;
+    dfree(ctx, sg->d_rgb);
+    dfree(ctx, sg->d_active);
+    sg->d_rgb = nullptr;
+    sg->d_active = nullptr;
+    return LG_OK;
 }
 
-int32_t lg_render(lg_ctx *, const lg_grid *, const lg_scatter_grid *, const lg_grid *, const lg_camera *,
-                  const lg_shader *, const lg_light *, int32_t, const lg_march_params *, int32_t, uint64_t, uint64_t,
-                  const lg_films *) {
-    set_error("lg_render: not built yet");
-    return LG_ERR_UNSUPPORTED;
+int32_t lg_splat(lg_ctx *ctx, const float *d_pts, const float *d_rad, int64_t count, double radius,
+                 lg_scatter_grid *sg) {
+    LG_CHECK_ARG(ctx && sg && sg->d_rgb && sg->d_active && count >= 0, "lg_splat: bad argument");
+    LG_CHECK_ARG(count == 0 || (d_pts && d_rad), "lg_splat: NULL points");
+    LG_CHECK_ARG(radius > 0.0 && count < (1ll << 31), "lg_splat: radius must be positive");
+    const CoarseView c = coarse_view(sg);
+    // bins of edge >= radius over the coarse lattice's world extent
+    BinGrid b{};
+    const double cs = std::max(radius, 1e-6);
+    b.inv_cs = 1.0 / cs;
+    const int ci0[3] = {c.i0x, c.i0y, c.i0z}, cn[3] = {c.nx, c.ny, c.nz};
+    long long nbins = 1;
+    for (int a = 0; a < 3; ++a) {
+        b.lo[a] = (double)ci0[a] * c.cv;
+        const double extent = (double)(cn[a] - 1) * c.cv;
+        b.n[a] = std::max(1, (int)ceil(extent / cs) + 1);
+        nbins *= b.n[a];
+    }
+    LG_CHECK_ARG(nbins < (1ll << 28), "lg_splat: radius too small for the grid");
+    int *cnt = nullptr, *off = nullptr, *cur = nullptr, *cell = nullptr, *idx = nullptr;
+    int32_t rc = dalloc(ctx, (size_t)nbins + 1, &cnt);
+    if (rc == LG_OK) rc = dalloc(ctx, (size_t)nbins + 1, &off);
+    if (rc == LG_OK) rc = dalloc(ctx, (size_t)nbins + 1, &cur);
+    if (rc == LG_OK) rc = dalloc(ctx, (size_t)std::max<int64_t>(count, 1), &cell);
+    if (rc == LG_OK) rc = dalloc(ctx, (size_t)std::max<int64_t>(count, 1), &idx);
+    if (rc == LG_OK) {
+        hipEvent_t ev;
+        ktimer_begin(ctx, "k_splat", &ev);
+        (void)hipMemsetAsync(cnt, 0, sizeof(int) * ((size_t)nbins + 1), ctx->stream);
+        (void)hipMemsetAsync(cur, 0, sizeof(int) * ((size_t)nbins + 1), ctx->stream);
+        if (count > 0) k_bin_count<<<(unsigned)((count + 255) / 256), 256, 0, ctx->stream>>>(d_pts, count, b, cnt, cell);
+        size_t bytes = 0;
+        (void)rocprim::exclusive_scan(nullptr, bytes, cnt, off, 0, (size_t)nbins + 1, rocprim::plus<int>(), ctx->stream);
+        void *tmp = nullptr;
+        rc = scratch(ctx, bytes, &tmp);
+        if (rc == LG_OK) {
+            (void)rocprim::exclusive_scan(tmp, bytes, cnt, off, 0, (size_t)nbins + 1, rocprim::plus<int>(), ctx->stream);
+            if (count > 0) k_bin_fill<<<(unsigned)((count + 255) / 256), 256, 0, ctx->stream>>>(count, cell, off, cur, idx);
+            const long long vox = (long long)c.nx * c.ny * c.nz;
+            const double r2 = radius * radius;
+            // sigma^2 = -r^2 / (2 ln 0.1) (ScatterGrid.h:75), weight exp(-d^2 / (2 sigma^2))
+            const double sigma2 = -(radius * radius) / (2.0 * log(0.1));
+            const double inv2sig2 = 1.0 / (2.0 * sigma2);
+            const int reach = (int)ceil(radius * b.inv_cs);
+            k_splat<<<(unsigned)((vox + 127) / 128), 128, 0, ctx->stream>>>(c, sg->d_active, d_pts, d_rad, b, reach, off,
+                                                                           idx, r2, inv2sig2, sg->d_rgb);
+            if (hipGetLastError() != hipSuccess) {
+                set_error("splat kernels failed to launch");
+                rc = LG_ERR_HIP;
+            }
+        }
+        ktimer_end(ctx, "k_splat", ev);
+    }
+    dfree(ctx, cnt);
+    dfree(ctx, off);
+    dfree(ctx, cur);
+    dfree(ctx, cell);
+    dfree(ctx, idx);
+    return rc;
+}
+
+int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *scatter, const lg_grid *temperature,
+                  const lg_camera *camera, const lg_shader *sh, const lg_light *lights, int32_t n_lights,
+                  const lg_march_params *mp, int32_t samples, uint64_t seed, uint64_t env_seed, const lg_films *films) {
+    LG_CHECK_ARG(ctx && density && camera && sh && mp && films, "lg_render: NULL argument");
+    LG_CHECK_ARG(camera->width > 0 && camera->height > 0, "lg_render: empty image");
+    LG_CHECK_ARG(samples >= 0, "lg_render: samples must be >= 0");
+    LG_CHECK_ARG(mp->step[0] > 0 && mp->step[1] > 0, "lg_render: steps must be positive");
+    LG_CHECK_ARG(!temperature || temperature->desc.dtype == LG_F32, "temperature grid must be float32");
+    lgl::DevLights L{};
+    LG_TRY(lgl::make_dev_lights(lights, n_lights, &L));
+    RenderConsts c{};
+    c.ds = sh->density_scale;
+    c.sds = sh->shadow_density_scale;
+    c.dmax = sh->density_max;
+    c.cutoff = mp->cutoff;
+    c.step0 = mp->step[0];
+    c.step1 = mp->step[1];
+    c.tau_star = dropout_tau(sh->extinction, mp->cutoff);
+    for (int a = 0; a < 3; ++a) {
+        c.albedo[a] = sh->albedo[a];
+        c.ext[a] = sh->extinction[a];
+        c.inv_scat[a] = 1.0 / sh->scattering[a];  // Renderer.h:516
+    }
+    c.scatter_scale = sh->scatter_scale;
+    c.mask_pow = sh->scatter_density_mask_power;
+    c.emission_scale = sh->emission_scale;
+    c.ramp_in = sh->ramp_in;
+    c.ramp_out = sh->ramp_out;
+    c.ramp_n = std::max(0, std::min(sh->ramp_count, LG_MAX_RAMP));
+    for (int k = 0; k < c.ramp_n; ++k) {
+        c.ramp_pos[k] = sh->ramp_pos[k];
+        for (int a = 0; a < 3; ++a) c.ramp_col[k][a] = sh->ramp_color[k][a];
+    }
+    memcpy(c.M, camera->screen_to_world, sizeof(c.M));
+    c.scale_w = camera->scale_width;
+    c.scale_h = camera->scale_height;
+    c.znear = camera->znear;
+    c.zfar = camera->zfar;
+    c.width = camera->width;
+    c.height = camera->height;
+    c.samples = samples;
+    c.ratio_aa = 1.0 / (double)samples;
+    const lgd::GridView g = make_view(density);
+    const lgd::GridView tg = temperature ? make_view(temperature) : lgd::GridView{};
+    const CoarseView sc = scatter && scatter->d_rgb ? coarse_view(scatter) : CoarseView{};
+    dim3 grid((c.width + 15) / 16, (c.height + 15) / 16);
+    hipEvent_t ev;
+    ktimer_begin(ctx, "k_render", &ev);
+    for_dtype_r(density->desc.dtype, [&](auto tag) {
+        using T = decltype(tag);
+        k_render<T><<<grid, 256, 0, ctx->stream>>>(g, tg, temperature ? 1 : 0, sc, (scatter && scatter->d_rgb) ? 1 : 0,
+                                                   c, L, (uint32_t)seed, (uint32_t)(seed >> 32), (uint32_t)env_seed,
+                                                   (uint32_t)(env_seed >> 32), (float4 *)films->d_rgb,
+                                                   (float4 *)films->d_direct, (float4 *)films->d_scatter,
+                                                   (float4 *)films->d_emission);
+    });
+    LG_LAUNCHED("k_render");
+    ktimer_end(ctx, "k_render", ev);
+    return LG_OK;
 }
 
 int32_t lg_fw_solve_batched(lg_ctx *ctx, int32_t count, const int32_t *h_n, float *const *h_D, const int32_t *h_ld) {

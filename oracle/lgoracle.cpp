@@ -988,3 +988,329 @@ extern "C" void lgo_env_cdf(int32_t w, int32_t h, const float *rgb, double *cond
     memcpy(cond, sc.envs[0].cond.data(), sc.envs[0].cond.size() * 8);
     memcpy(marg, sc.envs[0].marg.data(), sc.envs[0].marg.size() * 8);
 }
+
+// ====================================================== splat and render =====
+namespace {
+
+struct Coarse {
+    int n[3], i0[3];
+    double cv;
+    const float *rgb;  // n[0]*n[1]*n[2]*3
+    Vec3 value(int x, int y, int z) const {
+        x -= i0[0];
+        y -= i0[1];
+        z -= i0[2];
+        if (x < 0 || y < 0 || z < 0 || x >= n[0] || y >= n[1] || z >= n[2]) return Vec3(0.0);
+        const float *p = rgb + 3 * (((size_t)z * n[1] + y) * n[0] + x);
+        return Vec3(p[0], p[1], p[2]);
+    }
+    // BoxSampler on a Vec3fGrid: per-channel float lerps with double weights
+    Vec3 sample(const Vec3 &world) const {
+        const double x = world[0] * (1.0 / cv), y = world[1] * (1.0 / cv), z = world[2] * (1.0 / cv);
+        const int i = (int)std::floor(x), j = (int)std::floor(y), k = (int)std::floor(z);
+        const double u = x - i, v = y - j, w = z - k;
+        Vec3 d[2][2][2];
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b)
+                for (int c = 0; c < 2; ++c) d[a][b][c] = value(i + a, j + b, k + c);
+        Vec3 out;
+        for (int ch = 0; ch < 3; ++ch) {
+            auto f = [&](int a, int b, int c) { return (float)d[a][b][c][ch]; };
+            out[ch] = interpolate(interpolate(interpolate(f(0, 0, 0), f(0, 0, 1), w), interpolate(f(0, 1, 0), f(0, 1, 1), w), v),
+                                  interpolate(interpolate(f(1, 0, 0), f(1, 0, 1), w), interpolate(f(1, 1, 0), f(1, 1, 1), w), v),
+                                  u);
+        }
+        return out;
+    }
+};
+
+struct Camera {
+    const lg_camera *c;
+    // PerspectiveCamera::getRay (Renderer.h:216-229) with the two jitter draws
+    Ray getRay(int i, int j, CounterStream &rng) const {
+        const double *M = c->screen_to_world;
+        const Vec3 eye(M[12], M[13], M[14]);
+        const Vec3 axis(0.0 * M[0] + 0.0 * M[4] + -1.0 * M[8], 0.0 * M[1] + 0.0 * M[5] + -1.0 * M[9],
+                        0.0 * M[2] + 0.0 * M[6] + -1.0 * M[10]);
+        const double ri = (double)i + 0.5 + ((double)rng() - 0.5);
+        const double rj = (double)j + 0.5 + ((double)rng() - 0.5);
+        const Vec3 scr((2 * ri / double(c->width) - 1) * c->scale_width,
+                       (1 - 2 * rj / double(c->height)) * c->scale_height, -1.0);
+        Vec3 dir(scr[0] * M[0] + scr[1] * M[4] + scr[2] * M[8], scr[0] * M[1] + scr[1] * M[5] + scr[2] * M[9],
+                 scr[0] * M[2] + scr[1] * M[6] + scr[2] * M[10]);
+        const double d = dir.length();
+        if (!(std::fabs(d - 0.0) <= 1e-7)) dir = dir * (1.0 / d);
+        const double s = 1.0 / dir.dot(axis);
+        return Ray(eye, dir, c->znear * s, c->zfar * s);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+// Coarse topology of ScatterGrid::buildScatterGrid (ScatterGrid.h:105-144) as
+// restated by the library: footprint of f density voxels centred on the coarse
+// voxel, then a 26-neighbour dilation.  Parity unpinned against OpenVDB's
+// GridTransformer.
+void lgo_scatter_topology(const lg_grid_desc *d, const float *dense, int32_t f, const int32_t n[3],
+                          const int32_t i0[3], double cv, uint8_t *active) {
+    Grid g(*d, dense);
+    std::vector<uint8_t> raw((size_t)n[0] * n[1] * n[2], 0);
+    for (int z = 0; z < n[2]; ++z)
+        for (int y = 0; y < n[1]; ++y)
+            for (int x = 0; x < n[0]; ++x) {
+                const int C[3] = {x + i0[0], y + i0[1], z + i0[2]};
+                int lo[3];
+                for (int a = 0; a < 3; ++a) lo[a] = (int)std::floor(((double)C[a] * cv - g.origin[a]) * g.inv_voxel - 0.5 * f + 0.5);
+                bool on = false;
+                for (int dz = 0; dz < f && !on; ++dz)
+                    for (int dy = 0; dy < f && !on; ++dy)
+                        for (int dx = 0; dx < f && !on; ++dx) on = g.value(lo[0] + dx, lo[1] + dy, lo[2] + dz) != 0.0f;
+                raw[((size_t)z * n[1] + y) * n[0] + x] = on;
+            }
+    for (int z = 0; z < n[2]; ++z)
+        for (int y = 0; y < n[1]; ++y)
+            for (int x = 0; x < n[0]; ++x) {
+                bool on = false;
+                for (int dz = -1; dz <= 1; ++dz)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dx = -1; dx <= 1; ++dx) {
+                            const int xx = x + dx, yy = y + dy, zz = z + dz;
+                            if (xx < 0 || yy < 0 || zz < 0 || xx >= n[0] || yy >= n[1] || zz >= n[2]) continue;
+                            on = on || raw[((size_t)zz * n[1] + yy) * n[0] + xx];
+                        }
+                active[((size_t)z * n[1] + y) * n[0] + x] = on;
+            }
+}
+
+// ScatterGrid::operator() (ScatterGrid.h:61-98) over every active coarse voxel,
+// by brute force over the points.
+void lgo_splat(const float *pts, const float *rad, int64_t count, double radius, const int32_t n[3],
+               const int32_t i0[3], double cv, const uint8_t *active, float *rgb_out) {
+    const double sigmaSquared = -(radius * radius) / (2 * std::log(0.1));
+    const size_t vox = (size_t)n[0] * n[1] * n[2];
+    for (size_t k = 0; k < vox; ++k) {
+        float *o = rgb_out + 3 * k;
+        o[0] = o[1] = o[2] = 0.0f;
+        if (!active[k]) continue;
+        const int x = (int)(k % n[0]), y = (int)((k / n[0]) % n[1]), z = (int)(k / ((size_t)n[0] * n[1]));
+        const Vec3 worldPos((x + i0[0]) * cv, (y + i0[1]) * cv, (z + i0[2]) * cv);
+        Vec3 scatter(0.0);
+        double totalWeigths = 0.0;
+        for (int64_t p = 0; p < count; ++p) {
+            const Vec3 pos(pts[3 * p], pts[3 * p + 1], pts[3 * p + 2]);
+            if (!((pos - worldPos).lengthSqr() < radius * radius)) continue;
+            const Vec3 dd = worldPos - pos;
+            const float fx = (float)dd[0], fy = (float)dd[1], fz = (float)dd[2];
+            const float l2 = fx * fx + fy * fy + fz * fz;
+            const double weigth = std::exp(-(1.0 / (2.0 * sigmaSquared)) * l2);
+            totalWeigths += weigth;
+            scatter += Vec3(rad[3 * p], rad[3 * p + 1], rad[3 * p + 2]) * weigth;
+        }
+        if (totalWeigths > 0.0) scatter = scatter / totalWeigths;
+        for (int c = 0; c < 3; ++c) o[c] = (float)scatter[c];
+    }
+}
+
+// VolumeRender::operator() (Renderer.h:506-693), rows split over threads.
+// films: 4 consecutive width*height RGBA float arrays (rgb, direct, scatter,
+// emission).  scatter_rgb may be NULL (no multiple scattering term).
+void lgo_render(const lg_grid_desc *d, const float *dense, const float *temp_dense, const float *scatter_rgb,
+                const int32_t sn[3], const int32_t si0[3], double scv, const lg_camera *cam, const lg_shader *sh,
+                const lg_light *lights, int32_t nl, const float *const *env_rgb, const int32_t *env_w,
+                const int32_t *env_h, const lg_march_params *mp, int32_t samples, uint64_t seed, uint64_t env_seed,
+                int32_t threads, float *films) {
+    Grid g(*d, dense);
+    std::unique_ptr<Grid> tg;
+    if (temp_dense) tg.reset(new Grid(*d, temp_dense));
+    Scene sc = makeScene(&g, tg.get(), sh, mp, lights, nl, env_rgb, env_w, env_h);
+    Coarse coarse{};
+    if (scatter_rgb) {
+        for (int a = 0; a < 3; ++a) {
+            coarse.n[a] = sn[a];
+            coarse.i0[a] = si0[a];
+        }
+        coarse.cv = scv;
+        coarse.rgb = scatter_rgb;
+    }
+    const Camera camera{cam};
+    const int W = cam->width, H = cam->height;
+    float *rgbF = films, *dirF = films + 4 * (size_t)W * H, *scaF = films + 8 * (size_t)W * H,
+          *emiF = films + 12 * (size_t)W * H;
+    const Vec3 One(1.0);
+    const Vec3 invScatter = Vec3(1.0) / Vec3(sh->scattering[0], sh->scattering[1], sh->scattering[2]);
+    const Vec3 ext = sc.sh.ext(), albedo = sc.sh.albedo();
+    const double jitterAmp = 0.5, cutoff = mp->cutoff, h0 = mp->step[0], h1 = mp->step[1];
+    auto rows = [&](int j0, int j1) {
+        for (int j = j0; j < j1; ++j)
+            for (int i = 0; i < W; ++i) {
+                const uint32_t pix = (uint32_t)j * (uint32_t)W + (uint32_t)i;
+                Vec3 pTransAA(0.0), pLumiAA(0.0), directLumiAA(0.0), scatterLumiAA(0.0), emitLumiAA(0.0);
+                const double ratioAA = 1.0 / (double)samples;
+                for (int sampleAA = 0; sampleAA < samples; sampleAA++) {
+                    CounterStream rng(pix, (uint32_t)sampleAA, 2u, seed);
+                    CounterStream envRng(pix, (uint32_t)sampleAA, 3u, env_seed);
+                    const Ray pRay = camera.getRay(i, j, rng);
+                    Vec3 pTrans(1.0);
+                    Intersector primary(g), shadow(g);
+                    if (!primary.hitGridBbox(pRay)) {
+                        pTransAA += pTrans * ratioAA;
+                        continue;
+                    }
+                    Vec3 pLumi(0.0), directLumi(0.0), scatterLumi(0.0), emitLumi(0.0);
+                    double pT0, pT1;
+                    while (primary.march(pT0, pT1)) {
+                        bool pDropout = false;
+                        double pStep = h0 * (1.0 + ((double)rng() - 0.5) * jitterAmp);
+                        double pT = h0 * std::ceil(pT0 / h0);
+                        for (; pT <= pT1; pT += pStep) {
+                            const Vec3 ip = primary.indexPos(pT);
+                            const Vec3 pPos(g.origin[0] + g.voxel * ip[0], g.origin[1] + g.voxel * ip[1],
+                                            g.origin[2] + g.voxel * ip[2]);
+                            const double pDensity = std::min((double)sampleIndex(g, ip) * sh->density_scale,
+                                                             sh->density_max);
+                            Vec3 pScatter(0.0);
+                            if (scatter_rgb) pScatter = coarse.sample(pPos);
+                            const Vec3 scatter = pScatter * invScatter *
+                                                 std::pow(std::min(std::max(pDensity, 0.0), 1.0),
+                                                          sh->scatter_density_mask_power) *
+                                                 sh->scatter_scale;
+                            const double pTemp = tg ? (double)sampleIndex(*tg, ip) : 0.0;
+                            const Vec3 emission = sc.sh.emissionColor(pTemp) * sh->emission_scale;
+                            if (pDensity < cutoff) continue;
+                            const Vec3 dT = expv(ext * pDensity * pStep);
+                            for (int k = 0; k < nl; k++) {
+                                Vec3 tmpLumi(0.0);
+                                for (int s = 0; s < lights[k].samples; s++) {
+                                    Vec3 sTrans(1.0), lightDir, lightIntensity;
+                                    double distance;
+                                    illuminate(lights[k], &sc.envs[k], pPos, &envRng, lightDir, lightIntensity, distance);
+                                    const Ray sRay(pPos, -lightDir, kDelta, distance);
+                                    if (!shadow.hitGridBbox(sRay)) continue;
+                                    double sT0, sT1;
+                                    while (shadow.march(sT0, sT1)) {
+                                        bool sDropout = false;
+                                        double sStep = h1 * (1.0 + ((double)rng() - 0.5) * jitterAmp);
+                                        double sT = h1 * std::ceil(sT0 / h1);
+                                        for (; sT <= sT1; sT += sStep) {
+                                            const double sDensity =
+                                                std::min((double)sampleIndex(g, shadow.indexPos(sT)) *
+                                                             sh->density_scale * sh->shadow_density_scale,
+                                                         sh->density_max);
+                                            if (sDensity < cutoff) continue;
+                                            sTrans *= expv(ext * sDensity * sStep);
+                                            if (sTrans.lengthSqr() < cutoff) {
+                                                sTrans = Vec3(0.0);
+                                                sDropout = true;
+                                                break;
+                                            }
+                                            sStep = h1 * (1.0 + ((double)rng() - 0.5) * jitterAmp);
+                                        }
+                                        if (sDropout) break;
+                                    }
+                                    tmpLumi += lightIntensity * sTrans;
+                                }
+                                const Vec3 tmpDirectLumi = albedo * (tmpLumi / (double)lights[k].samples) * pTrans * (One - dT);
+                                pLumi += tmpDirectLumi;
+                                directLumi += tmpDirectLumi;
+                            }
+                            const Vec3 tmpScatterLumi = scatter * pTrans * (One - dT);
+                            const Vec3 tmpEmitLumi = emission * pTrans * (One - dT);
+                            scatterLumi += tmpScatterLumi;
+                            emitLumi += tmpEmitLumi;
+                            pLumi += tmpScatterLumi + tmpEmitLumi;
+                            pTrans *= dT;
+                            if (pTrans.lengthSqr() < cutoff) {
+                                pDropout = true;
+                                break;
+                            }
+                            pStep = h0 * (1.0 + ((double)rng() - 0.5) * jitterAmp);
+                        }
+                        if (pDropout) break;
+                    }
+                    pLumiAA += pLumi * ratioAA;
+                    directLumiAA += directLumi * ratioAA;
+                    scatterLumiAA += scatterLumi * ratioAA;
+                    emitLumiAA += emitLumi * ratioAA;
+                    pTransAA += pTrans * ratioAA;
+                }
+                const size_t o = 4 * (size_t)pix;
+                for (int c = 0; c < 3; ++c) {
+                    rgbF[o + c] = (float)pLumiAA[c];
+                    dirF[o + c] = (float)directLumiAA[c];
+                    scaF[o + c] = (float)scatterLumiAA[c];
+                    emiF[o + c] = (float)emitLumiAA[c];
+                }
+                rgbF[o + 3] = (float)(1.0f - pTransAA.sum() / 3.0f);
+                dirF[o + 3] = scaF[o + 3] = emiF[o + 3] = 0.0f;
+            }
+    };
+    if (threads <= 1) {
+        rows(0, H);
+    } else {
+        std::vector<std::thread> pool;
+        const int per = (H + threads - 1) / threads;
+        for (int t = 0; t < threads; ++t) {
+            const int a = t * per, b = std::min(H, a + per);
+            if (a < b) pool.emplace_back(rows, a, b);
+        }
+        for (auto &th : pool) th.join();
+    }
+}
+
+// computeVoxelSize restatement for the splat radius (see lg_splat_radius).
+double lgo_point_voxel_size(const float *pts, int64_t count) {
+    if (count <= 1) return (double)0.1f;
+    int64_t target = std::max<int64_t>(count / 8, 1);
+    Vec3 lo(1e300), hi(-1e300);
+    for (int64_t i = 0; i < count; ++i)
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = std::min(lo[c], (double)pts[3 * i + c]);
+            hi[c] = std::max(hi[c], (double)pts[3 * i + c]);
+        }
+    double e[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+    if (e[0] <= 1e-7 && e[1] <= 1e-7 && e[2] <= 1e-7) return (double)0.1f;
+    double volume = e[0] * e[1] * e[2];
+    if (volume <= 1e-7) {
+        std::sort(e, e + 3);
+        volume = (e[1] <= 1e-7) ? e[2] * e[2] * e[2] : e[2] * e[2] * e[1];
+    }
+    float vs = (float)std::pow(std::max(volume / (double)target, 3e-15), 1.0 / 3.0);
+    double prevVolume = volume;
+    int64_t prevCount = 0, vcount = 1;
+    while (vcount > prevCount) {
+        std::vector<uint64_t> keys((size_t)count);
+        const double inv = 1.0 / (double)vs;
+        for (int64_t i = 0; i < count; ++i) {
+            uint64_t key = 0;
+            for (int c = 0; c < 3; ++c) {
+                const int64_t q = (int64_t)std::floor((double)pts[3 * i + c] * inv + 0.5);
+                key = key * 2097152ull + (uint64_t)((q + 1048576) & 2097151);
+            }
+            keys[(size_t)i] = key;
+        }
+        std::sort(keys.begin(), keys.end());
+        prevCount = vcount;
+        vcount = (int64_t)(std::unique(keys.begin(), keys.end()) - keys.begin());
+        volume = (double)(vs * vs * vs * (float)vcount);
+        if (volume >= prevVolume) break;
+        prevVolume = volume;
+        const float prev = vs;
+        const double vv = volume / (double)target;
+        if (vv < 3e-15) {
+            vs = prev;
+            break;
+        }
+        vs = (float)std::pow(vv, 1.0 / 3.0);
+        if (vs / prev > 0.9f) break;
+    }
+    for (int dd = 5; dd < 11; ++dd) {
+        const double t = std::pow(10.0, dd);
+        const float tv = (float)(std::floor((double)vs * t + 0.5) / t);
+        if (tv != 0.0f) return (double)tv;
+    }
+    return (double)vs;
+}
+
+}  // extern "C"

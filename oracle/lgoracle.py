@@ -247,3 +247,65 @@ def env_cdf(rgb):
     marg = np.zeros(h + 1, dtype=np.float64)
     lib().lgo_env_cdf(w, h, _p(rgb), _p(cond), _p(marg))
     return cond, marg
+
+
+def _sig_render():
+    L = lib()
+    if not hasattr(L, "_render_sig"):
+        L.lgo_scatter_topology.restype = None
+        L.lgo_scatter_topology.argtypes = [P, P, I32, P, P, D, P]
+        L.lgo_splat.restype = None
+        L.lgo_splat.argtypes = [P, P, I64, D, P, P, D, P, P]
+        L.lgo_render.restype = None
+        L.lgo_render.argtypes = [P, P, P, P, P, P, D, P, P, P, I32, P, P, P, P, I32, U64, U64, I32, P]
+        L.lgo_point_voxel_size.restype = D
+        L.lgo_point_voxel_size.argtypes = [P, I64]
+        L._render_sig = True
+    return L
+
+
+def scatter_topology(dense, factor, n_xyz, i0_xyz, cv, voxel_size=1.0, origin=(0, 0, 0)):
+    dense = np.ascontiguousarray(dense, dtype=np.float32)
+    d = _desc(dense, voxel_size, origin)
+    n = np.asarray(n_xyz, dtype=np.int32)
+    i0 = np.asarray(i0_xyz, dtype=np.int32)
+    out = np.zeros((n[2], n[1], n[0]), dtype=np.uint8)
+    _sig_render().lgo_scatter_topology(C.byref(d), _p(dense), factor, _p(n), _p(i0), cv, _p(out))
+    return out
+
+
+def splat(pts, rad, radius, n_xyz, i0_xyz, cv, active):
+    pts = np.ascontiguousarray(pts, dtype=np.float32)
+    rad = np.ascontiguousarray(rad, dtype=np.float32)
+    n = np.asarray(n_xyz, dtype=np.int32)
+    i0 = np.asarray(i0_xyz, dtype=np.int32)
+    act = np.ascontiguousarray(active, dtype=np.uint8)
+    out = np.zeros((n[2], n[1], n[0], 3), dtype=np.float32)
+    _sig_render().lgo_splat(_p(pts), _p(rad), len(pts), radius, _p(n), _p(i0), cv, _p(act), _p(out))
+    return out
+
+
+def point_voxel_size(pts):
+    pts = np.ascontiguousarray(pts, dtype=np.float32)
+    return _sig_render().lgo_point_voxel_size(_p(pts), len(pts))
+
+
+def render(dense, camera, shader, lights, params, samples, seed=1, env_seed=4564446, temp_dense=None,
+           scatter=None, envs=None, threads=8, voxel_size=1.0, origin=(0, 0, 0)):
+    """scatter = (rgb[z,y,x,3], n_xyz, i0_xyz, cv) or None.  Returns films [4, H, W, 4]."""
+    dense = np.ascontiguousarray(dense, dtype=np.float32)
+    tdense = np.ascontiguousarray(temp_dense, dtype=np.float32) if temp_dense is not None else None
+    d = _desc(dense, voxel_size, origin)
+    arr, rgb, w, h, keep = _light_args(lights, envs)
+    srgb = sn = si0 = None
+    scv = 0.0
+    if scatter is not None:
+        srgb = np.ascontiguousarray(scatter[0], dtype=np.float32)
+        sn = np.asarray(scatter[1], dtype=np.int32)
+        si0 = np.asarray(scatter[2], dtype=np.int32)
+        scv = float(scatter[3])
+    films = np.zeros((4, camera.height, camera.width, 4), dtype=np.float32)
+    _sig_render().lgo_render(C.byref(d), _p(dense), _p(tdense), _p(srgb), _p(sn), _p(si0), scv, C.byref(camera),
+                             C.byref(shader), arr, len(lights), rgb, w, h, C.byref(params), samples, seed, env_seed,
+                             threads, _p(films))
+    return films
