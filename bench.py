@@ -8,6 +8,14 @@ edges within 12 voxels (12 per vertex at most), edge march at 0.5 voxel.  A step
 is one LightGraph solve (sample -> edges -> edge march -> all-pairs -> vertex
 light -> diffusion) with the volume resident in HBM.
 
+Beside the headline, the line carries:
+  * "latency": p50/p99 of one solve issued alone (host clock, solve + sync);
+  * "batched": the batching workload -- 64 seeded blob+fBm volumes of side
+    128..384 (step 8, seeds 3000+k), N drawn from [512, 6144] -- solved one graph
+    at a time (plain) and as one batch, solves/s of each, their ratio, and the
+    largest difference between a graph's batched and plain radiance (0 when
+    the batch reproduces every graph exactly).
+
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
 import argparse
@@ -39,6 +47,67 @@ def setup(lga, ctx, n_points):
     lights = [lga.point_light(0.0, (1.0, 0.9, 0.8), 4.0e5, (256.0, 700.0, 256.0)),
               lga.parse_light("dir:0:1,0.89,0.79:0.7:-41.8,-180,0")]
     return spec, grid, shader, params, lights
+
+
+def batch_workload(lga, ctx, count, seed=3000):
+    """count volumes: side U{128..384} step 8, N U{512..6144}, volume seeds seed+k."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sides = rng.integers(0, (384 - 128) // 8 + 1, count) * 8 + 128
+    ns = rng.integers(512, 6145, count)
+    grids = []
+    for k in range(count):
+        spec = lga.scene_spec(lga.LG_SCENE_BLOBS_FBM, int(sides[k]), seed=seed + k, octaves=6, blobs=20,
+                              axis_min=30.0, axis_max=120.0, active_fraction=0.30)
+        g, _ = lga.Grid.generate(ctx, spec)
+        grids.append(g)
+    return grids, [int(x) for x in sides], [int(x) for x in ns]
+
+
+def bench_batched(lga, ctx, shader, params, lights, count, reps):
+    import numpy as np
+    radius = 12.0
+    grids, sides, ns = batch_workload(lga, ctx, count)
+    items = [(g, n, 5000 + k) for k, (g, n) in enumerate(zip(grids, ns))]
+    plain = lga.Solver(ctx, max(ns))
+    batch = lga.Solver(ctx, sum(ns))
+
+    def run_plain(keep=False):
+        outs = []
+        for g, n, sv in items:
+            plain.solve(g, shader, lights, params, n=n, radius=radius, seed=854123, solve=sv)
+            if keep:
+                outs.append(plain.results(want_D=False)["diffuse"])
+        return outs
+
+    def run_batch():
+        batch.solve_batch(items, shader, lights, params, radius=radius, seed=854123)
+
+    ref = run_plain(keep=True)
+    run_batch()
+    got = [r["diffuse"] for r in batch.batch_results()]
+    max_diff = max(float(np.max(np.abs(a - b))) if len(a) else 0.0 for a, b in zip(got, ref))
+    ctx.sync()
+    ctx.mark(2)
+    for _ in range(reps):
+        run_plain()
+    ctx.mark(3)
+    for _ in range(reps):
+        run_batch()
+    ctx.mark(4)
+    ctx.sync()
+    t_plain = ctx.elapsed_ms(2, 3) / reps
+    t_batch = ctx.elapsed_ms(3, 4) / reps
+    plain.destroy()
+    batch.destroy()
+    for g in grids:
+        g.destroy()
+    return {"workload": "c3_batch", "volumes": count, "side_range": [min(sides), max(sides)],
+            "points_total": int(sum(ns)), "n_range": [min(ns), max(ns)], "edge_radius": radius, "reps": reps,
+            "plain_ms": round(t_plain, 3), "batched_ms": round(t_batch, 3),
+            "plain_solves_per_s": round(count * 1000.0 / t_plain, 2),
+            "batched_solves_per_s": round(count * 1000.0 / t_batch, 2),
+            "speedup": round(t_plain / t_batch, 3), "max_abs_diff_batched_vs_plain": max_diff}
 
 
 def kernel_roofline(ctx, ld):
@@ -112,6 +181,9 @@ def main():
     ap.add_argument("--points", type=int, default=8192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=60.0)
+    ap.add_argument("--batch-volumes", type=int, default=64)
+    ap.add_argument("--batch-reps", type=int, default=3)
+    ap.add_argument("--latency-solves", type=int, default=50)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
@@ -141,12 +213,25 @@ def main():
     ms_per_step = gpu_ms / args.steps
     roof, times = kernel_roofline(ctx, lga.fw_ld(n))
     b = solver.buffers()
+    lat = []
+    for k in range(args.latency_solves):
+        t1 = time.perf_counter()
+        solver.solve(grid, shader, lights, params, n=n, radius=radius, seed=854123, solve=10000 + k)
+        ctx.sync()
+        lat.append((time.perf_counter() - t1) * 1e3)
+    lat.sort()
+
+    def pct(q):
+        return round(lat[min(len(lat) - 1, int(q * len(lat)))], 4) if lat else None
     line = {"metric": "LightGraph solves/s", "value": round(1000.0 / ms_per_step, 3), "unit": "solves/s",
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
             "wall_ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "fp32 (fp64 ray/DDA positions)", "data": "synthetic",
-            "config": dict(WORKLOAD, edges_last_solve=b.n_edges), "roofline": roof, "kernels": times}
+            "config": dict(WORKLOAD, edges_last_solve=b.n_edges), "roofline": roof, "kernels": times,
+            "latency": {"p50_ms": pct(0.50), "p99_ms": pct(0.99), "solves": len(lat), "mode": "one solve alone"}}
     solver.destroy()
+    if args.batch_volumes > 0:
+        line["batched"] = bench_batched(lga, ctx, shader, params, lights, args.batch_volumes, args.batch_reps)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(lga, ctx, grid, shader, params, lights, n, radius, args.cpu_budget)
     else:

@@ -279,9 +279,10 @@ int32_t lg_diffuse(lg_ctx *ctx, int32_t n, const float *d_D, int32_t ld,
                    const double sigma_t[3], const float *d_L, float *d_out);
 
 /* ------------------------------------------------------------ full solve ---- */
-/* Per-solve workspace for graphs of up to n_max vertices.  flags:
- * LG_SOLVER_DENSE_D allocates the dense n x ld optical-depth matrix,
- * LG_SOLVER_PRED also the predecessor matrix.  edge_cap is ignored (the edge
+/* Per-solve workspace for graphs (or batches, lg_solve_batch) of up to n_max
+ * vertices in total, n_max <= 2^24.  flags: LG_SOLVER_DENSE_D allocates the dense
+ * n x ld optical-depth matrix, LG_SOLVER_PRED also the predecessor matrix; both
+ * need n_max <= 65536.  edge_cap is ignored (the edge
  * buffers grow on demand); it is kept for callers that size their own copies. */
 int32_t lg_solver_create(lg_ctx *ctx, int32_t n_max, int32_t edge_cap, int32_t flags,
                          lg_solver **out);
@@ -313,11 +314,36 @@ int32_t lg_solve(lg_ctx *ctx, lg_solver *solver, const lg_grid *density,
                  int32_t n_lights, const lg_march_params *params, const lg_solve_params *sp);
 
 /* ----------------------------------------------------------- batched form ---- */
-/* Many independent min-plus closures (uneven n) in one persistent launch.
- * d_D[g] is n[g] x ld[g] with ld[g] = lg_fw_ld(n[g]).  The host arrays are read
- * during the call; the device matrices are updated in place. */
+/* Many independent min-plus closures of uneven n (lg_fw_solve for each matrix)
+ * in one launch sequence: each 128-pivot round closes the pivot tiles of every
+ * matrix still that large in one launch, then all their panels, then all their
+ * remainder tiles.  d_D[g] is n[g] x ld[g] with ld[g] = lg_fw_ld(n[g]).  The host
+ * arrays are read during the call; the device matrices are updated in place.
+ * Replaces the per-solve floydAlgo calls of a tbb::parallel_for over solves
+ * (LightGraph.h:512-542 under main.cc:534-538). */
 int32_t lg_fw_solve_batched(lg_ctx *ctx, int32_t count, const int32_t *h_n,
                             float *const *h_D, const int32_t *h_ld);
+
+/* One graph of a batched solve. */
+typedef struct lg_batch_item {
+    const lg_grid *density;      /* density grid of this solve (all items: one dtype) */
+    const lg_grid *temperature;  /* float32 temperature grid or NULL                  */
+    int32_t n;                   /* points in this graph                              */
+    uint32_t solve;              /* solve index: selects the RNG stream               */
+} lg_batch_item;
+
+/* count independent LightGraph solves (LightGraph::operator() for every index of
+ * the tbb::parallel_for at main.cc:534-538, each on its own volume) as one
+ * pipeline: one sampling launch, one vertex-light launch, one graph build, one
+ * fused edge-march launch, one small-component launch and the batched
+ * Floyd-Warshall for the components above 128 vertices.  Item k owns vertices
+ * [voff_k, voff_k + n_k) of the solver's buffers, voff_k = n_0 + ... + n_{k-1};
+ * edges carry global vertex ids.  shader, lights, params and sp (except n, solve
+ * and flags, which must be 0) are shared.  Each item's outputs equal those of
+ * lg_solve on that item alone. */
+int32_t lg_solve_batch(lg_ctx *ctx, lg_solver *solver, const lg_batch_item *items, int32_t count,
+                       const lg_shader *shader, const lg_light *lights, int32_t n_lights,
+                       const lg_march_params *params, const lg_solve_params *sp);
 
 /* ----------------------------------------------------------- splat/render ---- */
 /* Splat radius of ScatterGrid::operator() (ScatterGrid.h:71-73) with the voxel size

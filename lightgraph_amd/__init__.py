@@ -355,6 +355,39 @@ class Solver:
                               temperature.handle if temperature else None, C.byref(shader), arr, len(lights),
                               C.byref(params), C.byref(sp)), "lg_solve")
 
+    def solve_batch(self, items, shader, lights, params, radius, seed=854123, max_connect=12, env_seed=4564446,
+                    timing=False):
+        """items: sequence of (density Grid, n, solve index[, temperature Grid]).
+        Item k owns vertices [voff_k, voff_k + n_k) of the buffers."""
+        arr = (_abi.lg_batch_item * len(items))()
+        self.batch_n = []
+        for k, it in enumerate(items):
+            g, n, solve = it[0], it[1], it[2]
+            t = it[3] if len(it) > 3 else None
+            arr[k].density, arr[k].temperature = g.handle, (t.handle if t else None)
+            arr[k].n, arr[k].solve = n, solve
+            self.batch_n.append(n)
+        sp = lg_solve_params()
+        sp.max_connect, sp.radius, sp.seed, sp.env_seed = max_connect, radius, seed, env_seed
+        sp.timing = 1 if timing else 0
+        self.last_dense = False
+        la = light_array(lights)
+        _check(lib().lg_solve_batch(self.ctx.handle, self.handle, arr, len(items), C.byref(shader), la, len(lights),
+                                    C.byref(params), C.byref(sp)), "lg_solve_batch")
+
+    def batch_results(self):
+        """Per-item dicts (points, edges with item-local ids, weights, L, diffuse)."""
+        r = self.results(want_D=False)
+        out, voff = [], 0
+        e = r["edges"]
+        for n in self.batch_n:
+            sel = (e[:, 0] >= voff) & (e[:, 0] < voff + n)
+            out.append({"points": r["points"][voff:voff + n], "L": r["L"][voff:voff + n],
+                        "diffuse": r["diffuse"][voff:voff + n], "edges": e[sel] - voff,
+                        "weights": r["weights"][sel]})
+            voff += n
+        return out
+
     def buffers(self):
         b = _abi.lg_solver_buffers()
         _check(lib().lg_solver_get(self.handle, C.byref(b)), "solver_get")

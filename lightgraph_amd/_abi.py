@@ -92,6 +92,10 @@ class lg_solve_params(C.Structure):
                 ("external_points", C.c_int32), ("timing", C.c_int32), ("flags", C.c_int32)]
 
 
+class lg_batch_item(C.Structure):
+    _fields_ = [("density", C.c_void_p), ("temperature", C.c_void_p), ("n", C.c_int32), ("solve", C.c_uint32)]
+
+
 P = C.c_void_p
 I32 = C.c_int32
 U32 = C.c_uint32
@@ -143,6 +147,8 @@ SIGNATURES = {
     "lg_solve": (I32, [P, P, P, P, C.POINTER(lg_shader), C.POINTER(lg_light), I32,
                        C.POINTER(lg_march_params), C.POINTER(lg_solve_params)]),
     "lg_fw_solve_batched": (I32, [P, I32, C.POINTER(I32), C.POINTER(C.c_void_p), C.POINTER(I32)]),
+    "lg_solve_batch": (I32, [P, P, C.POINTER(lg_batch_item), I32, C.POINTER(lg_shader), C.POINTER(lg_light), I32,
+                             C.POINTER(lg_march_params), C.POINTER(lg_solve_params)]),
     "lg_splat_radius": (I32, [P, I64, I64, DP, DP]),
     "lg_scatter_grid_create": (I32, [P, P, I32, C.POINTER(lg_scatter_grid)]),
     "lg_scatter_grid_destroy": (I32, [P, C.POINTER(lg_scatter_grid)]),

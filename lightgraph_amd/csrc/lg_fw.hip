@@ -25,7 +25,7 @@ namespace lg {
 constexpr int TB = FW_TILE;  // 128
 constexpr int KC = 32;       // pivots staged in LDS per step
 
-__global__ void __launch_bounds__(1024) k_fw_pivot(float *__restrict__ D, int ld, int kb) {
+__device__ __forceinline__ void fw_pivot_tile(float *__restrict__ D, int ld, int kb) {
     __shared__ float T[TB][TB];
     const int tid = threadIdx.x;
     const int c = tid & (TB - 1), r0 = tid >> 7;
@@ -55,6 +55,21 @@ __global__ void __launch_bounds__(1024) k_fw_pivot(float *__restrict__ D, int ld
     for (int r = 0; r < 16; ++r) base[(long long)(r0 + 8 * r) * ld + c] = v[r];
 }
 
+__global__ void __launch_bounds__(1024) k_fw_pivot(float *__restrict__ D, int ld, int kb) { fw_pivot_tile(D, ld, kb); }
+
+// One matrix of a batch: D (ld x ld, ld = 128 * nt), its tile flags at flags[flag_off].
+struct FwJob {
+    float *D;
+    int ld, nt, flag_off, pad;
+};
+
+// Batched pivot closure: block b closes the pivot tile of job active[b].
+__global__ void __launch_bounds__(1024) k_fw_pivot_b(const FwJob *__restrict__ jobs, const int *__restrict__ active,
+                                                     int kb) {
+    const FwJob j = jobs[active[blockIdx.x]];
+    fw_pivot_tile(j.D, j.ld, kb);
+}
+
 __device__ __forceinline__ void tri_decode(int p, int T, int &a, int &b) {
     // row-major upper triangle of a T x T matrix: row a holds T - a entries
     const double tt = 2.0 * T + 1.0;
@@ -69,15 +84,15 @@ __device__ __forceinline__ void tri_decode(int p, int T, int &a, int &b) {
 }
 
 template <bool PANEL>
-__global__ void __launch_bounds__(256) k_fw_minplus(float *__restrict__ D, int ld, int kb, int nt,
-                                                     int *__restrict__ flags) {
+__device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, int kb, int nt, int *__restrict__ flags,
+                                                int tile) {
     int Ib, Jb;
     if (PANEL) {
         Ib = kb;
-        Jb = (int)blockIdx.x < kb ? (int)blockIdx.x : (int)blockIdx.x + 1;
+        Jb = tile < kb ? tile : tile + 1;
     } else {
         int a, b;
-        tri_decode((int)blockIdx.x, nt - 1, a, b);
+        tri_decode(tile, nt - 1, a, b);
         Ib = a < kb ? a : a + 1;
         Jb = b < kb ? b : b + 1;
         if (!flags[Ib] || !flags[Jb]) return;
@@ -168,6 +183,30 @@ __global__ void __launch_bounds__(256) k_fw_minplus(float *__restrict__ D, int l
     }
 }
 
+template <bool PANEL>
+__global__ void __launch_bounds__(256) k_fw_minplus(float *__restrict__ D, int ld, int kb, int nt,
+                                                     int *__restrict__ flags) {
+    fw_minplus_tile<PANEL>(D, ld, kb, nt, flags, (int)blockIdx.x);
+}
+
+// Batched panel / remainder launch: prefix[a] is the first block of active job a.
+template <bool PANEL>
+__global__ void __launch_bounds__(256) k_fw_minplus_b(const FwJob *__restrict__ jobs, const int *__restrict__ active,
+                                                       const int *__restrict__ prefix, int n_active, int kb,
+                                                       int *__restrict__ flags) {
+    const int b = (int)blockIdx.x;
+    int lo = 0, hi = n_active - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const FwJob j = jobs[active[lo]];
+    fw_minplus_tile<PANEL>(j.D, j.ld, kb, j.nt, flags + j.flag_off, b - prefix[lo]);
+}
+
 int32_t fw_solve_impl(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld) {
     LG_CHECK_ARG(ld % TB == 0 && ld >= n && n >= 0, "fw: ld=%d must be a multiple of %d and >= n=%d", ld, TB, n);
     if (n <= 1) return LG_OK;
@@ -192,6 +231,76 @@ int32_t fw_solve_impl(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld) {
         ktimer_begin(ctx, "k_fw_minplus", &e3);
         k_fw_minplus<false><<<(unsigned)P, 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
         LG_LAUNCHED("k_fw_minplus");
+        ktimer_end(ctx, "k_fw_minplus", e3);
+    }
+    return LG_OK;
+}
+
+// Uneven batch: round kb runs the pivot tiles of every matrix with more than kb
+// tile rows in one launch, then all their panels, then all their remainders.
+int32_t fw_solve_batched_impl(lg_ctx *ctx, const std::vector<float *> &Ds, const std::vector<int> &ns) {
+    const int count = (int)Ds.size();
+    std::vector<FwJob> jobs;
+    int max_nt = 0, flag_total = 0;
+    for (int g = 0; g < count; ++g) {
+        if (ns[g] <= 1) continue;
+        const int ld = fw_ld(ns[g]);
+        jobs.push_back(FwJob{Ds[g], ld, ld / TB, flag_total, 0});
+        flag_total += ld / TB;
+        max_nt = std::max(max_nt, ld / TB);
+    }
+    if (jobs.empty()) return LG_OK;
+    const int J = (int)jobs.size();
+    // per round: active job ids, panel prefix, remainder prefix
+    std::vector<int> table;
+    std::vector<int> round_off(max_nt), round_n(max_nt), panel_blocks(max_nt), rem_blocks(max_nt);
+    for (int kb = 0; kb < max_nt; ++kb) {
+        round_off[kb] = (int)table.size();
+        std::vector<int> act;
+        for (int j = 0; j < J; ++j)
+            if (jobs[j].nt > kb) act.push_back(j);
+        round_n[kb] = (int)act.size();
+        table.insert(table.end(), act.begin(), act.end());
+        int pp = 0, rp = 0;
+        std::vector<int> ppre, rpre;
+        for (int j : act) {
+            ppre.push_back(pp);
+            rpre.push_back(rp);
+            const long long T = jobs[j].nt - 1;
+            pp += (int)T;
+            rp += (int)(T * (T + 1) / 2);
+        }
+        panel_blocks[kb] = pp;
+        rem_blocks[kb] = rp;
+        table.insert(table.end(), ppre.begin(), ppre.end());
+        table.insert(table.end(), rpre.begin(), rpre.end());
+    }
+    const size_t jobs_bytes = sizeof(FwJob) * J;
+    const size_t table_bytes = sizeof(int) * table.size();
+    const size_t flag_bytes = sizeof(int) * (size_t)flag_total;
+    void *s = nullptr;
+    LG_TRY(scratch(ctx, jobs_bytes + table_bytes + flag_bytes + 1024, &s));
+    FwJob *d_jobs = (FwJob *)s;
+    int *d_table = (int *)((char *)s + ((jobs_bytes + 255) / 256) * 256);
+    int *d_flags = d_table + table.size() + 64;
+    LG_HIP(hipMemcpyAsync(d_jobs, jobs.data(), jobs_bytes, hipMemcpyHostToDevice, ctx->stream));
+    LG_HIP(hipMemcpyAsync(d_table, table.data(), table_bytes, hipMemcpyHostToDevice, ctx->stream));
+    for (int kb = 0; kb < max_nt; ++kb) {
+        const int *act = d_table + round_off[kb];
+        const int n = round_n[kb];
+        hipEvent_t e1, e2, e3;
+        ktimer_begin(ctx, "k_fw_pivot", &e1);
+        k_fw_pivot_b<<<n, 1024, 0, ctx->stream>>>(d_jobs, act, kb);
+        LG_LAUNCHED("k_fw_pivot_b");
+        ktimer_end(ctx, "k_fw_pivot", e1);
+        if (panel_blocks[kb] == 0) continue;
+        ktimer_begin(ctx, "k_fw_panel", &e2);
+        k_fw_minplus_b<true><<<panel_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + n, n, kb, d_flags);
+        LG_LAUNCHED("k_fw_panel_b");
+        ktimer_end(ctx, "k_fw_panel", e2);
+        ktimer_begin(ctx, "k_fw_minplus", &e3);
+        k_fw_minplus_b<false><<<rem_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + 2 * n, n, kb, d_flags);
+        LG_LAUNCHED("k_fw_minplus_b");
         ktimer_end(ctx, "k_fw_minplus", e3);
     }
     return LG_OK;

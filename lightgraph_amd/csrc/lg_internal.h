@@ -149,6 +149,9 @@ int32_t grid_finalize(lg_ctx *ctx, lg_grid *g);
 // helpers implemented in the stage translation units
 int32_t grid_build_sampling_tables(lg_ctx *ctx, lg_grid *g);
 int32_t fw_solve_impl(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld);
+// Blocked Floyd-Warshall over many matrices of uneven size n[g] (ld = fw_ld(n[g]))
+// with one pivot, one panel and one remainder launch per 128-pivot round.
+int32_t fw_solve_batched_impl(lg_ctx *ctx, const std::vector<float *> &Ds, const std::vector<int> &ns);
 int32_t fw_pred_impl(lg_ctx *ctx, int32_t n, const float *d_D, int32_t ld, int32_t *d_pred,
                      const int32_t *d_edges, const float *d_w, int32_t n_edges);
 

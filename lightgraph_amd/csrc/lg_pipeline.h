@@ -258,9 +258,10 @@ struct Pipeline {
     size_t cand_cap = 0, accepted_cap = 0, epos_cap = 0, tmp_cand_cap = 0, edges_cap = 0, w_cap = 0, anb_cap = 0,
            aw_cap = 0;
     // large components
-    float *big_T = nullptr, *big_L = nullptr, *big_out = nullptr, *big_Ln = nullptr;
+    float *big_T = nullptr;
     double *big_sums = nullptr;
-    size_t bigT_cap = 0, bigL_cap = 0, bigout_cap = 0, bigsums_cap = 0, bigLn_cap = 0;
+    uint8_t *big_meta = nullptr;
+    size_t bigT_cap = 0, bigsums_cap = 0, bigmeta_cap = 0;
 
     int32_t reserve_vertices(lg_ctx *ctx, int V, int S);
     void release_vertex(lg_ctx *ctx);

@@ -320,8 +320,8 @@ __global__ void k_comp_offsets(int V, const int *__restrict__ flag, const int *_
     if (flag[k]) comp_off[cid_excl[k]] = k;
 }
 
-// local index of every vertex inside its component and the list of components
-// too large for the LDS kernel.
+// local index of every vertex inside its component and the (component, size)
+// pairs of the components too large for the LDS kernel.
 __global__ void k_comp_local(int V, const int *__restrict__ flag, const int *__restrict__ cid_excl,
                              const int *__restrict__ comp_off, const int *__restrict__ member, int *__restrict__ local,
                              int *__restrict__ big_list, int *__restrict__ n_big) {
@@ -329,7 +329,12 @@ __global__ void k_comp_local(int V, const int *__restrict__ flag, const int *__r
     if (k >= V) return;
     const int c = cid_excl[k] + flag[k] - 1;
     local[member[k]] = k - comp_off[c];
-    if (flag[k] && comp_off[c + 1] - comp_off[c] > kSmallComp) big_list[atomicAdd(n_big, 1)] = c;
+    const int m = comp_off[c + 1] - comp_off[c];
+    if (flag[k] && m > kSmallComp) {
+        const int slot = atomicAdd(n_big, 1);
+        big_list[2 * slot] = c;
+        big_list[2 * slot + 1] = m;
+    }
 }
 
 // ------------------------------------------------ small-component closure ----
@@ -437,63 +442,152 @@ __global__ void __launch_bounds__(256) k_comp_small(const SolveDesc *__restrict_
 }
 
 // ------------------------------------------------ large-component path ------
-__global__ void k_big_fill(int m, int ld, int a0, const int *__restrict__ member, const int *__restrict__ local,
-                           const int *__restrict__ aoff, const int *__restrict__ anb, const float *__restrict__ aw,
-                           float *__restrict__ Tm) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= m) return;
-    const int v = member[a0 + a];
-    for (int p = aoff[v]; p < aoff[v + 1]; ++p) Tm[(long long)a * ld + local[anb[p]]] = aw[p];
+// Components above kSmallComp vertices are gathered into per-component matrices
+// (one arena, ldm = 128-padded size) and closed together by the batched blocked
+// Floyd-Warshall.  Every kernel below covers all large components of the call in
+// one launch: block b handles row b of the concatenated row space and finds its
+// component by binary search over the row prefix.
+struct BigJob {
+    long long toff;  // first element of the component's matrix in the arena
+    int a0;          // first member in the sorted member list
+    int m, ldm;
+    int row0;        // first row of the component in the concatenated m-row space
+    int pad0, pad1;
+};
+
+__device__ __forceinline__ int big_find(const int *__restrict__ prefix, int J, int b) {
+    int lo = 0, hi = J - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
 }
 
-__global__ void k_big_gather_L(int m, int a0, const int *__restrict__ member, const float *__restrict__ L,
-                               float *__restrict__ Lm) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= m) return;
-    const int v = member[a0 + a];
-    for (int q = 0; q < 3; ++q) Lm[3 * a + q] = L[3 * v + q];
+__global__ void k_big_a0(int J, const int *__restrict__ comp, const int *__restrict__ comp_off,
+                         BigJob *__restrict__ jobs) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < J) jobs[b].a0 = comp_off[comp[b]];
 }
 
-__global__ void k_big_scatter_out(int m, int a0, const int *__restrict__ member, const float *__restrict__ om,
-                                  float *__restrict__ out) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= m) return;
-    const int v = member[a0 + a];
-    for (int q = 0; q < 3; ++q) out[3 * v + q] = om[3 * a + q];
+// rows of ldm: +INF off the diagonal, 0 on it, then the component's edges.
+__global__ void __launch_bounds__(256) k_big_fill(const BigJob *__restrict__ jobs, const int *__restrict__ ld_prefix,
+                                                  int J, const int *__restrict__ member, const int *__restrict__ local,
+                                                  const int *__restrict__ aoff, const int *__restrict__ anb,
+                                                  const float *__restrict__ aw, float *__restrict__ T) {
+    const int q = big_find(ld_prefix, J, (int)blockIdx.x);
+    const BigJob jb = jobs[q];
+    const int r = (int)blockIdx.x - ld_prefix[q];
+    float *row = T + jb.toff + (long long)r * jb.ldm;
+    for (int j = threadIdx.x; j < jb.ldm; j += blockDim.x) row[j] = (j == r) ? 0.0f : INFINITY;
+    __syncthreads();
+    if (r >= jb.m) return;
+    const int v = member[jb.a0 + r];
+    for (int p = aoff[v] + threadIdx.x; p < aoff[v + 1]; p += blockDim.x) row[local[anb[p]]] = aw[p];
 }
 
-__global__ void k_big_scatter_dense(int m, int ldm, int a0, const int *__restrict__ member,
-                                    const float *__restrict__ Tm, int voff, float *__restrict__ D, int ld) {
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (long long)m * m) return;
-    const int i = (int)(e / m), j = (int)(e - (long long)i * m);
-    D[(long long)(member[a0 + i] - voff) * ld + (member[a0 + j] - voff)] = Tm[(long long)i * ldm + j];
+// per-row sums of exp(ext_c * D) over the component (D symmetric: = column sums)
+__global__ void __launch_bounds__(256) k_big_rowsum(const BigJob *__restrict__ jobs, const int *__restrict__ m_prefix,
+                                                    int J, const float *__restrict__ T, float e0, float e1, float e2,
+                                                    double *__restrict__ sums) {
+    const int q = big_find(m_prefix, J, (int)blockIdx.x);
+    const BigJob jb = jobs[q];
+    const int i = (int)blockIdx.x - m_prefix[q];
+    const float *row = T + jb.toff + (long long)i * jb.ldm;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    for (int j = threadIdx.x; j < jb.m; j += blockDim.x) {
+        const float d = row[j];
+        a0 += expf(e0 * d);
+        a1 += expf(e1 * d);
+        a2 += expf(e2 * d);
+    }
+    __shared__ double s[3][256];
+    s[0][threadIdx.x] = a0;
+    s[1][threadIdx.x] = a1;
+    s[2][threadIdx.x] = a2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int c = 0; c < 3; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) sums[3 * (long long)(jb.row0 + i) + c] = s[c][0];
 }
 
-__global__ void k_big_pred(int m, int ldm, int a0, const int *__restrict__ member, const int *__restrict__ local,
-                           const int *__restrict__ aoff, const int *__restrict__ anb, const float *__restrict__ aw,
-                           const float *__restrict__ Tm, int voff, int *__restrict__ pred, int ld) {
-    const int i = blockIdx.x;
-    if (i >= m) return;
-    const float *row = Tm + (long long)i * ldm;
-    for (int j = threadIdx.x; j < m; j += blockDim.x) {
+// diffuse_i = sum_j (L_j / max(colsum_j, 1)) * exp(ext * D_ij), scattered to the
+// component's global vertex ids.
+__global__ void __launch_bounds__(256) k_big_diffuse(const BigJob *__restrict__ jobs, const int *__restrict__ m_prefix,
+                                                     int J, const float *__restrict__ T, const int *__restrict__ member,
+                                                     const float *__restrict__ L, const double *__restrict__ sums,
+                                                     float e0, float e1, float e2, float *__restrict__ out) {
+    const int q = big_find(m_prefix, J, (int)blockIdx.x);
+    const BigJob jb = jobs[q];
+    const int i = (int)blockIdx.x - m_prefix[q];
+    const float *row = T + jb.toff + (long long)i * jb.ldm;
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int j = threadIdx.x; j < jb.m; j += blockDim.x) {
+        const float d = row[j];
+        const int vj = member[jb.a0 + j];
+        const double *sj = sums + 3 * (long long)(jb.row0 + j);
+        float ln[3];
+        for (int c = 0; c < 3; ++c) ln[c] = (float)((double)L[3 * vj + c] / (sj[c] < 1.0 ? 1.0 : sj[c]));
+        a0 += (double)(ln[0] * expf(e0 * d));
+        a1 += (double)(ln[1] * expf(e1 * d));
+        a2 += (double)(ln[2] * expf(e2 * d));
+    }
+    __shared__ double s[3][256];
+    s[0][threadIdx.x] = a0;
+    s[1][threadIdx.x] = a1;
+    s[2][threadIdx.x] = a2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int c = 0; c < 3; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int vi = member[jb.a0 + i];
+        for (int c = 0; c < 3; ++c) out[3 * vi + c] = (float)s[c][0];
+    }
+}
+
+// optional dense outputs of the solve that owns the component: D and canonical
+// predecessors (smallest neighbour k of j minimising D_ik + w_kj).
+__global__ void __launch_bounds__(256) k_big_dense(const BigJob *__restrict__ jobs, const int *__restrict__ m_prefix,
+                                                   int J, const float *__restrict__ T, const SolveDesc *__restrict__ sd,
+                                                   const int *__restrict__ sid, const int *__restrict__ member,
+                                                   const int *__restrict__ local, const int *__restrict__ aoff,
+                                                   const int *__restrict__ anb, const float *__restrict__ aw) {
+    const int q = big_find(m_prefix, J, (int)blockIdx.x);
+    const BigJob jb = jobs[q];
+    const int i = (int)blockIdx.x - m_prefix[q];
+    const SolveDesc &d = sd[sid[member[jb.a0]]];
+    const float *row = T + jb.toff + (long long)i * jb.ldm;
+    const int vi = member[jb.a0 + i] - d.voff;
+    for (int j = threadIdx.x; j < jb.m; j += blockDim.x) {
+        const int vj = member[jb.a0 + j];
+        if (d.dense_D) d.dense_D[(long long)vi * d.ld + (vj - d.voff)] = row[j];
+        if (!d.dense_pred) continue;
         int best_k = -1;
         if (i == j) {
-            best_k = member[a0 + i] - voff;
+            best_k = vi;
         } else if (row[j] < INFINITY) {
             float best = INFINITY;
-            const int vj = member[a0 + j];
             for (int p = aoff[vj]; p < aoff[vj + 1]; ++p) {
                 const int kv = anb[p];
                 const float val = row[local[kv]] + aw[p];
-                const int kl = kv - voff;
+                const int kl = kv - d.voff;
                 if (val < best || (val == best && kl < best_k)) {
                     best = val;
                     best_k = kl;
                 }
             }
         }
-        pred[(long long)(member[a0 + i] - voff) * ld + (member[a0 + j] - voff)] = best_k;
+        d.dense_pred[(long long)vi * d.ld + (vj - d.voff)] = best_k;
     }
 }
 
@@ -558,12 +652,17 @@ int32_t Pipeline::reserve_vertices(lg_ctx *ctx, int V, int S) {
     A(&member, (size_t)v_cap);
     A(&local, (size_t)v_cap);
     A(&comp_off, (size_t)v_cap + 1);
-    A(&big_list, (size_t)v_cap);
+    A(&big_list, (size_t)v_cap + 2);  // (component, size) pairs: at most v_cap / 129 of them
     A(&counters, 16);
     // rocPRIM temporary storage for scans and sorts over up to v_cap keys
     size_t b1 = 0, b2 = 0;
-    rocprim::radix_sort_keys(nullptr, b1, keys, keys_sorted, (size_t)v_cap, 0, 64, ctx->stream);
-    rocprim::exclusive_scan(nullptr, b2, rowcnt, off, 0, (size_t)v_cap + 1, rocprim::plus<int>(), ctx->stream);
+    if (rc == LG_OK &&
+        (rocprim::radix_sort_keys(nullptr, b1, keys, keys_sorted, (size_t)v_cap, 0, 64, ctx->stream) != hipSuccess ||
+         rocprim::exclusive_scan(nullptr, b2, rowcnt, off, 0, (size_t)v_cap + 1, rocprim::plus<int>(), ctx->stream) !=
+             hipSuccess)) {
+        set_error("rocPRIM temporary-size query failed");
+        rc = LG_ERR_HIP;
+    }
     tmp_vertex_bytes = std::max(b1, b2);
     A((uint8_t **)&tmp_vertex, tmp_vertex_bytes + 256);
     return rc;
@@ -584,14 +683,15 @@ void Pipeline::release_vertex(lg_ctx *ctx) {
 
 void Pipeline::release(lg_ctx *ctx) {
     release_vertex(ctx);
-    void *ps[] = {cand, accepted, epos, tmp_cand, edges_, w_, anb, aw, big_T, big_L, big_out, big_sums, big_Ln};
+    void *ps[] = {cand, accepted, epos, tmp_cand, edges_, w_, anb, aw, big_T, big_sums, big_meta};
     for (void *p : ps) dfree(ctx, p);
     cand = accepted = epos = edges_ = anb = nullptr;
-    w_ = aw = big_T = big_L = big_out = big_Ln = nullptr;
+    w_ = aw = big_T = nullptr;
+    big_meta = nullptr;
     big_sums = nullptr;
     tmp_cand = nullptr;
     cand_cap = accepted_cap = epos_cap = tmp_cand_cap = edges_cap = w_cap = anb_cap = aw_cap = 0;
-    bigT_cap = bigL_cap = bigout_cap = bigsums_cap = bigLn_cap = 0;
+    bigT_cap = bigsums_cap = bigmeta_cap = 0;
 }
 
 static int32_t scan_excl(lg_ctx *ctx, Pipeline &pl, const int *in, int *out, size_t n, bool cand_sized) {
@@ -749,24 +849,34 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     LG_HIP(hipMemsetAsync(pl.counters, 0, sizeof(int) * 16, st));
     k_comp_local<<<vb, 256, 0, st>>>(V, pl.flag, pl.cid, pl.comp_off, pl.member, pl.local, pl.big_list, pl.counters);
     LG_LAUNCHED("k_comp_local");
-    int host_counts[3] = {0, 0, 0};
-    LG_HIP(hipMemcpyAsync(&host_counts[0], pl.cid + V, sizeof(int), hipMemcpyDeviceToHost, st));
-    LG_HIP(hipMemcpyAsync(&host_counts[1], pl.counters, sizeof(int), hipMemcpyDeviceToHost, st));
-    LG_HIP(hipMemcpyAsync(&host_counts[2], d_nedges, sizeof(int), hipMemcpyDeviceToHost, st));
+    // counts and the large-component pairs come back in one synchronisation
+    const int max_big = V / (kSmallComp + 1);
+    std::vector<int> host((size_t)3 + 2 * (size_t)max_big);
+    LG_HIP(hipMemcpyAsync(&host[0], pl.cid + V, sizeof(int), hipMemcpyDeviceToHost, st));
+    LG_HIP(hipMemcpyAsync(&host[1], pl.counters, sizeof(int), hipMemcpyDeviceToHost, st));
+    LG_HIP(hipMemcpyAsync(&host[2], d_nedges, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (max_big > 0)
+        LG_HIP(hipMemcpyAsync(&host[3], pl.big_list, sizeof(int) * 2 * (size_t)max_big, hipMemcpyDeviceToHost, st));
     LG_HIP(hipStreamSynchronize(st));
     ktimer_end(ctx, "k_components", ev);
-    const int C = host_counts[0], nbig = host_counts[1];
-    res->n_edges_total = host_counts[2];
+    const int C = host[0], nbig = host[1];
+    res->n_edges_total = host[2];
     res->n_components = C;
     res->n_big = nbig;
     res->Ktot = Ktot;
+    if (nbig > max_big) {
+        set_error("pl_components: %d large components exceed the bound %d", nbig, max_big);
+        return LG_ERR_HIP;
+    }
 
     // dense outputs (optional) start as +INF / -1 with the diagonal
+    bool any_dense = false;
     for (auto &d : descs) {
         if (d.dense_D) LG_TRY(fw_init_impl(ctx, d.n, d.dense_D, d.ld));
         if (d.dense_pred)
             k_pred_init<<<(unsigned)std::min<long long>(((long long)d.n * d.ld + 255) / 256, 65536), 256, 0, st>>>(
                 d.n, d.ld, d.dense_pred);
+        any_dense = any_dense || d.dense_D || d.dense_pred;
     }
 
     // (c) + diffusion: small components in one launch
@@ -778,49 +888,67 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
         LG_LAUNCHED("k_comp_small");
     }
     ktimer_end(ctx, "k_comp_small", ev);
+    if (nbig == 0) {
+        res->edges = pl.edges_;
+        res->w = pl.w_;
+        res->epos = pl.epos;
+        res->off = pl.off;
+        return LG_OK;
+    }
 
-    // large components: blocked Floyd-Warshall on a gathered matrix
-    if (nbig > 0) {
-        std::vector<int> big(nbig), coff((size_t)C + 1), first(nbig);
-        LG_HIP(hipMemcpyAsync(big.data(), pl.big_list, sizeof(int) * nbig, hipMemcpyDeviceToHost, st));
-        LG_HIP(hipMemcpyAsync(coff.data(), pl.comp_off, sizeof(int) * ((size_t)C + 1), hipMemcpyDeviceToHost, st));
-        LG_HIP(hipStreamSynchronize(st));
-        std::sort(big.begin(), big.end());
-        for (int b = 0; b < nbig; ++b)
-            LG_HIP(hipMemcpyAsync(&first[b], pl.member + coff[big[b]], sizeof(int), hipMemcpyDeviceToHost, st));
-        LG_HIP(hipStreamSynchronize(st));
-        const double sig[3] = {-ext[0], -ext[1], -ext[2]};
-        for (int b = 0; b < nbig; ++b) {
-            const int a0 = coff[big[b]], m = coff[big[b] + 1] - coff[big[b]];
-            const int ldm = fw_ld(m);
-            LG_TRY(grow(ctx, &pl.big_T, &pl.bigT_cap, (size_t)ldm * ldm));
-            LG_TRY(grow(ctx, &pl.big_L, &pl.bigL_cap, (size_t)3 * m));
-            LG_TRY(grow(ctx, &pl.big_out, &pl.bigout_cap, (size_t)3 * m));
-            LG_TRY(grow(ctx, &pl.big_sums, &pl.bigsums_cap, (size_t)3 * m));
-            LG_TRY(grow(ctx, &pl.big_Ln, &pl.bigLn_cap, (size_t)3 * m));
-            LG_TRY(fw_init_impl(ctx, m, pl.big_T, ldm));
-            k_big_fill<<<(m + 127) / 128, 128, 0, st>>>(m, ldm, a0, pl.member, pl.local, pl.aoff, pl.anb, pl.aw,
-                                                        pl.big_T);
-            LG_LAUNCHED("k_big_fill");
-            LG_TRY(fw_solve_impl(ctx, m, pl.big_T, ldm));
-            k_big_gather_L<<<(m + 127) / 128, 128, 0, st>>>(m, a0, pl.member, L, pl.big_L);
-            LG_TRY(diffuse_impl(ctx, m, pl.big_T, ldm, sig, pl.big_L, pl.big_out, pl.big_sums, pl.big_Ln));
-            k_big_scatter_out<<<(m + 127) / 128, 128, 0, st>>>(m, a0, pl.member, pl.big_out, out);
-            LG_LAUNCHED("k_big_scatter_out");
-            int s = 0;
-            while (s + 1 < S && descs[s + 1].voff <= first[b]) ++s;
-            const SolveDesc &d = descs[s];
-            if (d.dense_D) {
-                k_big_scatter_dense<<<(unsigned)(((long long)m * m + 255) / 256), 256, 0, st>>>(
-                    m, ldm, a0, pl.member, pl.big_T, d.voff, d.dense_D, d.ld);
-                LG_LAUNCHED("k_big_scatter_dense");
-            }
-            if (d.dense_pred) {
-                k_big_pred<<<m, 256, 0, st>>>(m, ldm, a0, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, pl.big_T,
-                                              d.voff, d.dense_pred, d.ld);
-                LG_LAUNCHED("k_big_pred");
-            }
-        }
+    // large components: one arena, batched blocked Floyd-Warshall, batched diffusion
+    std::vector<std::pair<int, int>> big((size_t)nbig);
+    for (int b = 0; b < nbig; ++b) big[b] = {host[3 + 2 * b], host[4 + 2 * b]};
+    std::sort(big.begin(), big.end());  // deterministic order (slots come from atomics)
+    std::vector<BigJob> jobs((size_t)nbig);
+    std::vector<int> ld_prefix((size_t)nbig), m_prefix((size_t)nbig);
+    std::vector<float *> mats((size_t)nbig);
+    std::vector<int> ms((size_t)nbig);
+    long long toff = 0;
+    int rows = 0, mrows = 0;
+    for (int b = 0; b < nbig; ++b) {
+        const int m = big[b].second, ldm = fw_ld(m);
+        jobs[b] = BigJob{toff, 0, m, ldm, mrows, 0, 0};
+        ld_prefix[b] = rows;
+        m_prefix[b] = mrows;
+        rows += ldm;
+        mrows += m;
+        toff += (long long)ldm * ldm;
+        ms[b] = m;
+    }
+    // a0 = comp_off[c]: gathered on the device from the component ids
+    LG_TRY(grow(ctx, &pl.big_T, &pl.bigT_cap, (size_t)toff));
+    LG_TRY(grow(ctx, &pl.big_sums, &pl.bigsums_cap, (size_t)3 * mrows));
+    const size_t jb_bytes = sizeof(BigJob) * nbig, pre_bytes = sizeof(int) * nbig;
+    LG_TRY(grow(ctx, &pl.big_meta, &pl.bigmeta_cap, jb_bytes + 2 * pre_bytes + sizeof(int) * nbig + 256));
+    BigJob *d_jobs = (BigJob *)pl.big_meta;
+    int *d_ldp = (int *)(pl.big_meta + jb_bytes);
+    int *d_mp = d_ldp + nbig;
+    int *d_comp = d_mp + nbig;
+    std::vector<int> comps((size_t)nbig);
+    for (int b = 0; b < nbig; ++b) comps[b] = big[b].first;
+    LG_HIP(hipMemcpyAsync(d_jobs, jobs.data(), jb_bytes, hipMemcpyHostToDevice, st));
+    LG_HIP(hipMemcpyAsync(d_ldp, ld_prefix.data(), pre_bytes, hipMemcpyHostToDevice, st));
+    LG_HIP(hipMemcpyAsync(d_mp, m_prefix.data(), pre_bytes, hipMemcpyHostToDevice, st));
+    LG_HIP(hipMemcpyAsync(d_comp, comps.data(), pre_bytes, hipMemcpyHostToDevice, st));
+    k_big_a0<<<(nbig + 255) / 256, 256, 0, st>>>(nbig, d_comp, pl.comp_off, d_jobs);
+    LG_LAUNCHED("k_big_a0");
+    ktimer_begin(ctx, "k_big_fill", &ev);
+    k_big_fill<<<rows, 256, 0, st>>>(d_jobs, d_ldp, nbig, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, pl.big_T);
+    LG_LAUNCHED("k_big_fill");
+    ktimer_end(ctx, "k_big_fill", ev);
+    for (int b = 0; b < nbig; ++b) mats[b] = pl.big_T + jobs[b].toff;
+    LG_TRY(fw_solve_batched_impl(ctx, mats, ms));
+    ktimer_begin(ctx, "k_diffuse", &ev);
+    k_big_rowsum<<<mrows, 256, 0, st>>>(d_jobs, d_mp, nbig, pl.big_T, e0, e1, e2, pl.big_sums);
+    LG_LAUNCHED("k_big_rowsum");
+    k_big_diffuse<<<mrows, 256, 0, st>>>(d_jobs, d_mp, nbig, pl.big_T, pl.member, L, pl.big_sums, e0, e1, e2, out);
+    LG_LAUNCHED("k_big_diffuse");
+    ktimer_end(ctx, "k_diffuse", ev);
+    if (any_dense) {
+        k_big_dense<<<mrows, 256, 0, st>>>(d_jobs, d_mp, nbig, pl.big_T, pl.sd, pl.sid, pl.member, pl.local, pl.aoff,
+                                           pl.anb, pl.aw);
+        LG_LAUNCHED("k_big_dense");
     }
     res->edges = pl.edges_;
     res->w = pl.w_;

@@ -831,11 +831,14 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
 
 int32_t lg_fw_solve_batched(lg_ctx *ctx, int32_t count, const int32_t *h_n, float *const *h_D, const int32_t *h_ld) {
     LG_CHECK_ARG(ctx && count >= 0 && (count == 0 || (h_n && h_D && h_ld)), "lg_fw_solve_batched: bad argument");
+    std::vector<float *> Ds((size_t)count);
+    std::vector<int> ns((size_t)count);
     for (int g = 0; g < count; ++g) {
-        LG_CHECK_ARG(h_ld[g] == fw_ld(h_n[g]), "lg_fw_solve_batched: ld[%d] must be lg_fw_ld(n)", g);
-        LG_TRY(fw_solve_impl(ctx, h_n[g], h_D[g], h_ld[g]));
+        LG_CHECK_ARG(h_n[g] >= 0 && h_D[g] && h_ld[g] == fw_ld(h_n[g]), "lg_fw_solve_batched: graph %d: bad n/D/ld", g);
+        Ds[g] = h_D[g];
+        ns[g] = h_n[g];
     }
-    return LG_OK;
+    return fw_solve_batched_impl(ctx, Ds, ns);
 }
 
 }  // extern "C"

@@ -301,6 +301,53 @@ int32_t diffuse_impl(lg_ctx *ctx, int n, const float *d_D, int ld, const double 
     return LG_OK;
 }
 
+// Runs the pipeline over the solves in descs (vertex offsets voff, one shared
+// shader / light set / march parameters) into the solver's buffers.
+static int32_t solve_descs(lg_ctx *ctx, lg_solver *s, const std::vector<SolveDesc> &descs, const lg_shader *shader,
+                           const lg_light *lights, int32_t n_lights, const lg_march_params *params,
+                           const lg_solve_params *sp) {
+    Pipeline &pl = *static_cast<Pipeline *>(s->pipeline);
+    PipelineParams pp{};
+    pp.pts = s->buf.d_points;
+    pp.L = s->buf.d_L;
+    pp.out = s->buf.d_diffuse;
+    pp.seed = sp->seed;
+    pp.env_seed = sp->env_seed;
+    pp.external_points = sp->external_points;
+    pp.max_connect = sp->max_connect;
+    const float rf = (float)sp->radius;
+    const float r2f = rf * rf;  // float, as searchRadiusConnectSquared (LightGraph.h:171)
+    pp.r2 = (double)r2f;
+    for (int c = 0; c < 3; ++c) pp.ext[c] = shader->extinction[c];
+    pp.mc = march_consts(shader, params);
+    pp.lc = light_consts(shader, params);
+    LG_TRY(lgl::make_dev_lights(lights, n_lights, &pp.lights));
+    const bool t = sp->timing != 0;
+    const int dtype = descs[0].dtype;
+    int V = 0, nmax = 0, Ktot = 0;
+    if (t) LG_HIP(hipEventRecord(s->ev[0], ctx->stream));
+    LG_TRY(pl_begin(ctx, pl, descs, &V, &nmax));
+    if (!pp.external_points) LG_TRY(pl_sample(ctx, pl, V, dtype, pp.seed, pp.pts));
+    if (t) LG_HIP(hipEventRecord(s->ev[1], ctx->stream));
+    LG_TRY(pl_light(ctx, pl, V, dtype, pp.lc, pp.lights, pp.env_seed, pp.pts, pp.L));
+    if (t) LG_HIP(hipEventRecord(s->ev[2], ctx->stream));
+    LG_TRY(pl_graph(ctx, pl, V, (int)descs.size(), nmax, pp.pts, pp.r2, pp.max_connect, &Ktot));
+    if (t) LG_HIP(hipEventRecord(s->ev[3], ctx->stream));
+    LG_TRY(pl_march(ctx, pl, dtype, pp.mc, pp.pts, Ktot));
+    if (t) LG_HIP(hipEventRecord(s->ev[4], ctx->stream));
+    PipelineResult res;
+    LG_TRY(pl_components(ctx, pl, descs, V, Ktot, pp.ext, pp.L, pp.out, &res));
+    if (t) LG_HIP(hipEventRecord(s->ev[5], ctx->stream));
+    s->timed = t;
+    s->buf.n = V;
+    s->buf.ld = descs.size() == 1 ? descs[0].ld : 0;
+    s->buf.n_edges = res.n_edges_total;
+    s->buf.n_components = res.n_components;
+    s->buf.d_edges = res.edges;
+    s->buf.d_weights = res.w;
+    return LG_OK;
+}
+
 Pipeline &stage_pipeline(lg_ctx *ctx) {
     if (!ctx->stage_pipeline) ctx->stage_pipeline = new Pipeline();
     return *static_cast<Pipeline *>(ctx->stage_pipeline);
@@ -426,7 +473,10 @@ int32_t lg_diffuse(lg_ctx *ctx, int32_t n, const float *d_D, int32_t ld, const d
 }
 
 int32_t lg_solver_create(lg_ctx *ctx, int32_t n_max, int32_t edge_cap, int32_t flags, lg_solver **out) {
-    LG_CHECK_ARG(ctx && out && n_max >= 1 && n_max <= 65536 && edge_cap >= 0, "lg_solver_create: bad argument");
+    LG_CHECK_ARG(ctx && out && n_max >= 1 && edge_cap >= 0, "lg_solver_create: bad argument");
+    LG_CHECK_ARG(n_max <= (1 << 24), "lg_solver_create: n_max %d above 2^24", n_max);
+    LG_CHECK_ARG(!(flags & (LG_SOLVER_DENSE_D | LG_SOLVER_PRED)) || n_max <= 65536,
+                 "lg_solver_create: dense outputs need n_max <= 65536");
     *out = nullptr;
     lg_solver *s = new lg_solver();
     s->n_max = n_max;
@@ -511,53 +561,41 @@ int32_t lg_solve(lg_ctx *ctx, lg_solver *s, const lg_grid *density, const lg_gri
     LG_CHECK_ARG(!temperature || temperature->desc.dtype == LG_F32, "temperature grid must be float32");
     const bool dense = (sp->flags & LG_SOLVE_DENSE) != 0;
     LG_CHECK_ARG(!dense || s->with_dense, "lg_solve: dense outputs need a solver created with LG_SOLVER_DENSE_D");
+    LG_CHECK_ARG(!dense || fw_ld(n) <= s->ld_max, "lg_solve: dense outputs larger than the solver's");
     lg_grid *g = const_cast<lg_grid *>(density);
     LG_TRY(grid_build_sampling_tables(ctx, g));
-    Pipeline &pl = *static_cast<Pipeline *>(s->pipeline);
-    const int ld = fw_ld(n);
     SolveDesc d = make_desc(g, temperature, n, 0, sp->solve);
-    d.ld = ld;
+    d.ld = fw_ld(n);
     d.dense_D = dense ? s->buf.d_D : nullptr;
     d.dense_pred = (dense && s->with_pred) ? s->buf.d_pred : nullptr;
-    std::vector<SolveDesc> descs = {d};
-    PipelineParams pp{};
-    pp.pts = s->buf.d_points;
-    pp.L = s->buf.d_L;
-    pp.out = s->buf.d_diffuse;
-    pp.seed = sp->seed;
-    pp.env_seed = sp->env_seed;
-    pp.external_points = sp->external_points;
-    pp.max_connect = sp->max_connect;
-    const float rf = (float)sp->radius;
-    const float r2f = rf * rf;  // float, as searchRadiusConnectSquared (LightGraph.h:171)
-    pp.r2 = (double)r2f;
-    for (int c = 0; c < 3; ++c) pp.ext[c] = shader->extinction[c];
-    pp.mc = march_consts(shader, params);
-    pp.lc = light_consts(shader, params);
-    LG_TRY(lgl::make_dev_lights(lights, n_lights, &pp.lights));
-    const bool t = sp->timing != 0;
-    int V = 0, nmax = 0, Ktot = 0;
-    if (t) LG_HIP(hipEventRecord(s->ev[0], ctx->stream));
-    LG_TRY(pl_begin(ctx, pl, descs, &V, &nmax));
-    if (!pp.external_points) LG_TRY(pl_sample(ctx, pl, V, d.dtype, pp.seed, pp.pts));
-    if (t) LG_HIP(hipEventRecord(s->ev[1], ctx->stream));
-    LG_TRY(pl_light(ctx, pl, V, d.dtype, pp.lc, pp.lights, pp.env_seed, pp.pts, pp.L));
-    if (t) LG_HIP(hipEventRecord(s->ev[2], ctx->stream));
-    LG_TRY(pl_graph(ctx, pl, V, 1, nmax, pp.pts, pp.r2, pp.max_connect, &Ktot));
-    if (t) LG_HIP(hipEventRecord(s->ev[3], ctx->stream));
-    LG_TRY(pl_march(ctx, pl, d.dtype, pp.mc, pp.pts, Ktot));
-    if (t) LG_HIP(hipEventRecord(s->ev[4], ctx->stream));
-    PipelineResult res;
-    LG_TRY(pl_components(ctx, pl, descs, V, Ktot, pp.ext, pp.L, pp.out, &res));
-    if (t) LG_HIP(hipEventRecord(s->ev[5], ctx->stream));
-    s->timed = t;
-    s->buf.n = n;
-    s->buf.ld = ld;
-    s->buf.n_edges = res.n_edges_total;
-    s->buf.n_components = res.n_components;
-    s->buf.d_edges = res.edges;
-    s->buf.d_weights = res.w;
-    return LG_OK;
+    return solve_descs(ctx, s, {d}, shader, lights, n_lights, params, sp);
+}
+
+int32_t lg_solve_batch(lg_ctx *ctx, lg_solver *s, const lg_batch_item *items, int32_t count, const lg_shader *shader,
+                       const lg_light *lights, int32_t n_lights, const lg_march_params *params,
+                       const lg_solve_params *sp) {
+    LG_CHECK_ARG(ctx && s && items && sp && count >= 1, "lg_solve_batch: bad argument");
+    LG_TRY(check_params(shader, params));
+    LG_CHECK_ARG((sp->flags & LG_SOLVE_DENSE) == 0, "lg_solve_batch: dense outputs are per-solve only (use lg_solve)");
+    std::vector<SolveDesc> descs;
+    descs.reserve((size_t)count);
+    long long V = 0;
+    for (int k = 0; k < count; ++k) {
+        const lg_batch_item &it = items[k];
+        LG_CHECK_ARG(it.density && it.n >= 1, "lg_solve_batch: item %d: NULL density or n < 1", k);
+        LG_CHECK_ARG(!it.temperature || it.temperature->desc.dtype == LG_F32,
+                     "lg_solve_batch: item %d: temperature grid must be float32", k);
+        LG_CHECK_ARG(it.density->desc.dtype == items[0].density->desc.dtype,
+                     "lg_solve_batch: item %d: every density grid of a batch needs the same element type", k);
+        lg_grid *g = const_cast<lg_grid *>(it.density);
+        LG_TRY(grid_build_sampling_tables(ctx, g));
+        SolveDesc d = make_desc(g, it.temperature, it.n, (int)V, it.solve);
+        d.ld = fw_ld(it.n);
+        descs.push_back(d);
+        V += it.n;
+        LG_CHECK_ARG(V <= s->n_max, "lg_solve_batch: %lld points exceed the solver's n_max %d", V, s->n_max);
+    }
+    return solve_descs(ctx, s, descs, shader, lights, n_lights, params, sp);
 }
 
 }  // extern "C"

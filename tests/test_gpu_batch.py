@@ -1,0 +1,132 @@
+"""Batched form: uneven graphs in one launch sequence give, graph by graph, the
+same results as the plain per-graph path (bit-identical: every graph is
+computed by the same per-component arithmetic whatever else is in the batch),
+and the batched blocked Floyd-Warshall matches the oracle."""
+import numpy as np
+import pytest
+from scipy.sparse import csr_matrix
+from scipy.sparse.csgraph import connected_components
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_graph(rng, n, deg, integer):
+    W = np.full((n, n), np.inf, np.float32)
+    np.fill_diagonal(W, 0.0)
+    m = int(deg * n / 2)
+    a = rng.integers(0, n, m)
+    b = rng.integers(0, n, m)
+    x = rng.integers(1, 20, m).astype(np.float32) if integer else rng.uniform(0.01, 3.0, m).astype(np.float32)
+    for i, j, v in zip(a, b, x):
+        if i != j:
+            W[i, j] = W[j, i] = min(W[i, j], v)
+    return W
+
+
+def _padded(lga, W):
+    n = W.shape[0]
+    ld = lga.fw_ld(n)
+    Dm = np.full((ld, ld), np.inf, np.float32)
+    np.fill_diagonal(Dm, 0.0)
+    Dm[:n, :n] = W
+    return Dm, ld
+
+
+SIZES = [1, 7, 128, 129, 300, 700, 1030, 2]
+
+
+def test_fw_batched_equals_single_and_oracle(lga, ctx, oracle):
+    rng = np.random.default_rng(21)
+    Ws = [_random_graph(rng, n, 2.5, integer=(k % 2 == 0)) for k, n in enumerate(SIZES)]
+    single, batch = [], []
+    for W in Ws:
+        Dm, ld = _padded(lga, W)
+        b1, b2 = ctx.upload(Dm), ctx.upload(Dm)
+        lga.fw_solve(ctx, W.shape[0], b1, ld)
+        single.append(b1)
+        batch.append(b2)
+    lga.fw_solve_batched(ctx, SIZES, batch)
+    for k, W in enumerate(Ws):
+        n = W.shape[0]
+        ld = lga.fw_ld(n)
+        got = batch[k].read(np.float32, (ld, ld))
+        ref = single[k].read(np.float32, (ld, ld))
+        np.testing.assert_array_equal(got, ref)
+        want = oracle.floyd_minplus(W.astype(np.float64))
+        g = got[:n, :n].astype(np.float64)
+        if k % 2 == 0:
+            np.testing.assert_array_equal(g, want)
+        else:
+            assert np.array_equal(np.isinf(g), np.isinf(want))
+            fin = np.isfinite(want)
+            np.testing.assert_allclose(g[fin], want[fin], rtol=1e-5, atol=0)
+        single[k].free()
+        batch[k].free()
+
+
+def test_fw_batched_empty_and_bad_ld(lga, ctx):
+    lga.fw_solve_batched(ctx, [], [])
+    b = ctx.alloc(4)
+    with pytest.raises(lga.LGError):
+        k = 1
+        import ctypes as C
+        n_arr = (C.c_int32 * k)(5)
+        ld_arr = (C.c_int32 * k)(5)  # must be lg_fw_ld(5) = 128
+        p_arr = (C.c_void_p * k)(b.ptr)
+        lga._check(lga.lib().lg_fw_solve_batched(ctx.handle, k, n_arr, p_arr, ld_arr), "fw_solve_batched")
+    b.free()
+
+
+@pytest.fixture(scope="module")
+def volumes(lga, ctx):
+    """Six blob+fBm volumes of uneven side (64..160) with seeds 3000+k."""
+    sides = [64, 96, 160, 72, 128, 88]
+    grids = []
+    for k, side in enumerate(sides):
+        spec = lga.scene_spec(lga.LG_SCENE_BLOBS_FBM, side, seed=3000 + k, octaves=6, blobs=20,
+                              axis_min=side * 30 // 512 + 4, axis_max=side * 120 // 512 + 8, active_fraction=0.3)
+        g, _ = lga.Grid.generate(ctx, spec)
+        grids.append(g)
+    yield grids
+    for g in grids:
+        g.destroy()
+
+
+def test_solve_batch_equals_plain(lga, ctx, volumes):
+    sh = lga.default_shader(absorption=0.1, scattering=(1.0, 0.9, 0.8))
+    p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5)
+    lights = [lga.point_light(0.0, (1.0, 0.9, 0.8), 4.0e4, (64.0, 200.0, 64.0)),
+              lga.parse_light("dir:0:1,0.89,0.79:0.7:-41.8,-180,0")]
+    ns = [512, 1400, 3000, 700, 2100, 900]
+    radius = 6.0
+    items = [(g, n, 40 + k) for k, (g, n) in enumerate(zip(volumes, ns))]
+    bs = lga.Solver(ctx, sum(ns))
+    bs.solve_batch(items, sh, lights, p, radius=radius, seed=854123)
+    got = bs.batch_results()
+    plain = lga.Solver(ctx, max(ns))
+    largest = 0
+    for k, (g, n, solve) in enumerate(items):
+        plain.solve(g, sh, lights, p, n=n, radius=radius, seed=854123, solve=solve)
+        ref = plain.results(want_D=False)
+        for key in ("points", "L", "diffuse", "edges", "weights"):
+            np.testing.assert_array_equal(got[k][key], ref[key], err_msg=f"item {k} ({key})")
+        assert np.all(np.isfinite(ref["diffuse"])) and ref["diffuse"].max() > 0
+        e = ref["edges"][np.isfinite(ref["weights"])]
+        A = csr_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(n, n))
+        _, lab = connected_components(A, directed=False)
+        largest = max(largest, np.bincount(lab).max())
+    assert largest > 128, "the batch should hold a component for the blocked Floyd-Warshall"
+    bs.destroy()
+    plain.destroy()
+
+
+def test_solve_batch_rejects_mixed_dtypes(lga, ctx, volumes):
+    d = volumes[0].read_dense()
+    g16 = lga.Grid.from_dense(ctx, d, dtype=lga.LG_F16)
+    sh = lga.default_shader()
+    s = lga.Solver(ctx, 400)
+    with pytest.raises(lga.LGError):
+        s.solve_batch([(volumes[0], 200, 0), (g16, 200, 1)], sh, [lga.parse_light("dir:0:1,1,1:1:0,0,0")],
+                      lga.march_params(), radius=4.0)
+    s.destroy()
+    g16.destroy()
