@@ -56,6 +56,12 @@ struct Stream {
         block = 0;
         used = 4;
     }
+    // position the stream at word w (as if w words had been drawn)
+    __device__ void skip(uint32_t w) {
+        block = w >> 2;
+        used = 4;
+        for (uint32_t r = 0; r < (w & 3u); ++r) (void)next();
+    }
     __device__ float next() {
         if (used == 4) {
             cache = philox(U4{a, b, block, domain}, k0, k1);

@@ -42,7 +42,14 @@ struct LightConsts {
     int ramp_n;
     double ramp_pos[LG_MAX_RAMP];
     double ramp_col[LG_MAX_RAMP][3];
+    int exact_steps;  // 1: t0 + k*h is exact for every sample (see step_is_exact)
 };
+
+// True when h is a multiple of 2^-30 below 2^23: every multiple of h below 2^23
+// is then a multiple of 2^-30 below 2^23, which FP64 holds exactly, so for sample
+// times under 2^23 index units the sequential recurrence t += h and the direct
+// form h*ceil(t0/h) + k*h give the same bits.
+bool step_is_exact(double h);
 
 MarchConsts march_consts(const lg_shader *sh, const lg_march_params *mp);
 LightConsts light_consts(const lg_shader *sh, const lg_march_params *mp);
@@ -135,13 +142,59 @@ __device__ __forceinline__ bool shadow_tau(const lgd::GridView &g, const LightCo
     return true;
 }
 
-// computeLighting for one vertex (LightGraph.h:438-497).
+// Wave-cooperative form of shadow_tau for exact steps: the 64 lanes of a wave
+// take consecutive samples t = ts + k*h of one span, the optical depth is an FP64
+// wave prefix sum (same sample set and densities as the sequential loop; the
+// summation order differs only in rounding), and the dropout test fires at the
+// first sample whose running depth exceeds tau_star.  Every lane returns the
+// same values.
 template <typename T>
-__device__ __forceinline__ void vertex_light_one(const lgd::GridView &g, const lgd::GridView &tg, int has_temp,
-                                                 const LightConsts &lc, const lgl::DevLights &L, uint32_t ek0,
-                                                 uint32_t ek1, uint32_t vertex, uint32_t solve, const double p[3],
-                                                 double out[3]) {
-    double emission[3] = {0.0, 0.0, 0.0};
+__device__ __forceinline__ bool shadow_tau_wave(const lgd::GridView &g, const LightConsts &lc, const double p[3],
+                                                const double ldir[3], double distance, double *tau_out,
+                                                bool *dropped) {
+    const int lane = threadIdx.x & 63;
+    const double sd[3] = {-ldir[0], -ldir[1], -ldir[2]};
+    lgd::Isect is;
+    if (!lgd::isect_world_ray(g, is, p, sd, lgd::kDeltaD, distance)) return false;
+    double tau = 0.0;
+    double t0, t1;
+    *dropped = false;
+    while (lgd::isect_march(g, is, t0, t1)) {
+        const double ts = lc.h * ceil(t0 / lc.h);
+        for (int base = 0;; base += 64) {
+            const double t = ts + (double)(base + lane) * lc.h;
+            const bool act = t <= t1;
+            double x = 0.0;
+            if (act) {
+                const float rho =
+                    lgd::trilinear<T>(g, is.e[0] + is.d[0] * t, is.e[1] + is.d[1] * t, is.e[2] + is.d[2] * t);
+                double dens = (double)rho * lc.ds_sds_scds[0] * lc.ds_sds_scds[1] * lc.ds_sds_scds[2];
+                dens = (lc.dmax < dens) ? lc.dmax : dens;
+                if (!(dens < lc.cutoff)) x = dens * lc.h;
+            }
+            double pre = x;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const double y = __shfl_up(pre, o, 64);
+                if (lane >= o) pre += y;
+            }
+            if (__ballot(act && tau + pre > lc.tau_star) != 0ull) {
+                *dropped = true;
+                *tau_out = tau;
+                return true;
+            }
+            tau += __shfl(pre, 63, 64);
+            if (!__shfl((int)act, 63, 64)) break;  // t grows with the lane: the span is done
+        }
+    }
+    *tau_out = tau;
+    return true;
+}
+
+// Emission term of computeLighting (LightGraph.h:494-496).
+__device__ __forceinline__ void vertex_emission(const lgd::GridView &tg, int has_temp, const LightConsts &lc,
+                                                const double p[3], double emission[3]) {
+    emission[0] = emission[1] = emission[2] = 0.0;
     if (lc.ramp_n > 0) {
         double temp = 0.0;
         if (has_temp) {
@@ -152,6 +205,34 @@ __device__ __forceinline__ void vertex_light_one(const lgd::GridView &g, const l
         ramp_color(lc, temp, emission);
         for (int k = 0; k < 3; ++k) emission[k] = emission[k] * lc.emission_scale;
     }
+}
+
+// Light sample t (0 <= t < total samples) of a light list: light index, sample
+// within the light and env-stream words drawn before it (2 per env sample,
+// LightGraph.h:449-452 scales multi-sample lights by 10).
+__device__ __forceinline__ void light_task(const lgl::DevLights &L, int t, int *li_out, uint32_t *env_words) {
+    uint32_t w = 0;
+    int li = 0;
+    for (; li < L.n; ++li) {
+        int ns = L.l[li].samples;
+        if (ns > 1) ns *= 10;
+        if (t < ns) break;
+        t -= ns;
+        if (L.l[li].type == LG_LIGHT_ENV) w += 2u * (uint32_t)ns;
+    }
+    if (li < L.n && L.l[li].type == LG_LIGHT_ENV) w += 2u * (uint32_t)t;
+    *li_out = li;
+    *env_words = w;
+}
+
+// computeLighting for one vertex (LightGraph.h:438-497).
+template <typename T>
+__device__ __forceinline__ void vertex_light_one(const lgd::GridView &g, const lgd::GridView &tg, int has_temp,
+                                                 const LightConsts &lc, const lgl::DevLights &L, uint32_t ek0,
+                                                 uint32_t ek1, uint32_t vertex, uint32_t solve, const double p[3],
+                                                 double out[3]) {
+    double emission[3];
+    vertex_emission(tg, has_temp, lc, p, emission);
     lgd::Stream env_rng;
     env_rng.init(vertex, solve, lgd::DOM_VERTEX_ENV, ((uint64_t)ek1 << 32) | ek0);
     double acc[3] = {0.0, 0.0, 0.0};
@@ -260,6 +341,8 @@ struct Pipeline {
     // large components
     float *big_T = nullptr;
     double *big_sums = nullptr;
+    double *ray = nullptr;  // per (vertex, light sample) shadowed intensity
+    size_t ray_cap = 0;
     uint8_t *big_meta = nullptr;
     size_t bigT_cap = 0, bigsums_cap = 0, bigmeta_cap = 0;
 

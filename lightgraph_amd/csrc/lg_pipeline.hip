@@ -78,6 +78,58 @@ __global__ void k_light(const SolveDesc *__restrict__ sd, const int *__restrict_
     for (int k = 0; k < 3; ++k) out[3 * v + k] = (float)acc[k];
 }
 
+// Vertex lighting for exact steps: one wave per (vertex, light sample) shadow
+// ray, then one lane per vertex sums the samples in the order of
+// computeLighting (LightGraph.h:446-497).
+template <typename T>
+__global__ void __launch_bounds__(256) k_light_rays(const SolveDesc *__restrict__ sd, const int *__restrict__ sid,
+                                                    int V, int nt, LightConsts lc, lgl::DevLights L, uint32_t ek0,
+                                                    uint32_t ek1, const float *__restrict__ pts,
+                                                    double *__restrict__ ray) {
+    const long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= (long long)V * nt) return;
+    const int v = (int)(w / nt), t = (int)(w - (long long)v * nt);
+    int li;
+    uint32_t ew;
+    light_task(L, t, &li, &ew);
+    const SolveDesc &d = sd[sid[v]];
+    const double p[3] = {(double)pts[3 * v], (double)pts[3 * v + 1], (double)pts[3 * v + 2]};
+    lgd::Stream rng;
+    rng.init((uint32_t)(v - d.voff), d.solve, lgd::DOM_VERTEX_ENV, ((uint64_t)ek1 << 32) | ek0);
+    rng.skip(ew);
+    double ldir[3], inten[3], distance;
+    lgl::illuminate(L.l[li], p, rng, ldir, inten, distance);
+    double tau;
+    bool dropped;
+    double r[3] = {0.0, 0.0, 0.0};
+    if (shadow_tau_wave<T>(d.g, lc, p, ldir, distance, &tau, &dropped) && !dropped)
+        for (int k = 0; k < 3; ++k) r[k] = inten[k] * (double)expf((float)(lc.ext[k] * tau));
+    if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < 3; ++k) ray[3 * w + k] = r[k];
+}
+
+__global__ void k_light_combine(const SolveDesc *__restrict__ sd, const int *__restrict__ sid, int V, int nt,
+                                LightConsts lc, lgl::DevLights L, const float *__restrict__ pts,
+                                const double *__restrict__ ray, float *__restrict__ out) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const SolveDesc &d = sd[sid[v]];
+    const double p[3] = {(double)pts[3 * v], (double)pts[3 * v + 1], (double)pts[3 * v + 2]};
+    double emission[3];
+    vertex_emission(d.tg, d.has_temp, lc, p, emission);
+    const double *r = ray + 3 * (long long)v * nt;
+    double acc[3] = {0.0, 0.0, 0.0};
+    for (int li = 0; li < L.n; ++li) {
+        int ns = L.l[li].samples;
+        if (ns > 1) ns *= 10;
+        double lumi[3] = {0.0, 0.0, 0.0};
+        for (int s = 0; s < ns; ++s, r += 3)
+            for (int k = 0; k < 3; ++k) lumi[k] += r[k];
+        for (int k = 0; k < 3; ++k) acc[k] += lc.albedo[k] * (lumi[k] / (double)ns);
+    }
+    for (int k = 0; k < 3; ++k) out[3 * v + k] = (float)(acc[k] + emission[k]);
+}
+
 // ---------------------------------------------------------- candidates ------
 __device__ __forceinline__ double pdist2(const float *__restrict__ pts, int i, int j) {
     const double dx = (double)pts[3 * i + 0] - (double)pts[3 * j + 0];
@@ -683,15 +735,16 @@ void Pipeline::release_vertex(lg_ctx *ctx) {
 
 void Pipeline::release(lg_ctx *ctx) {
     release_vertex(ctx);
-    void *ps[] = {cand, accepted, epos, tmp_cand, edges_, w_, anb, aw, big_T, big_sums, big_meta};
+    void *ps[] = {cand, accepted, epos, tmp_cand, edges_, w_, anb, aw, big_T, big_sums, big_meta, ray};
     for (void *p : ps) dfree(ctx, p);
     cand = accepted = epos = edges_ = anb = nullptr;
     w_ = aw = big_T = nullptr;
     big_meta = nullptr;
     big_sums = nullptr;
+    ray = nullptr;
     tmp_cand = nullptr;
     cand_cap = accepted_cap = epos_cap = tmp_cand_cap = edges_cap = w_cap = anb_cap = aw_cap = 0;
-    bigT_cap = bigsums_cap = bigmeta_cap = 0;
+    bigT_cap = bigsums_cap = bigmeta_cap = ray_cap = 0;
 }
 
 static int32_t scan_excl(lg_ctx *ctx, Pipeline &pl, const int *in, int *out, size_t n, bool cand_sized) {
@@ -745,13 +798,31 @@ int32_t pl_light(lg_ctx *ctx, Pipeline &pl, int V, int dtype, const LightConsts 
                  uint64_t env_seed, const float *pts, float *L) {
     hipEvent_t ev;
     ktimer_begin(ctx, "k_vertex_light", &ev);
-    for_dtype(dtype, [&](auto tag) {
-        using T = decltype(tag);
-        k_light<T><<<(unsigned)((V + 127) / 128), 128, 0, ctx->stream>>>(pl.sd, pl.sid, V, lc, lights,
-                                                                        (uint32_t)env_seed, (uint32_t)(env_seed >> 32),
-                                                                        pts, L);
-    });
-    LG_LAUNCHED("k_light");
+    int nt = 0;
+    for (int li = 0; li < lights.n; ++li) nt += lights.l[li].samples > 1 ? lights.l[li].samples * 10 : 1;
+    const uint32_t ek0 = (uint32_t)env_seed, ek1 = (uint32_t)(env_seed >> 32);
+    if (lc.exact_steps) {
+        const long long rays = (long long)V * nt;
+        if (rays > 0) {
+            LG_TRY(grow(ctx, &pl.ray, &pl.ray_cap, (size_t)rays * 3));
+            for_dtype(dtype, [&](auto tag) {
+                using T = decltype(tag);
+                k_light_rays<T><<<(unsigned)((rays + 3) / 4), 256, 0, ctx->stream>>>(pl.sd, pl.sid, V, nt, lc, lights,
+                                                                                   ek0, ek1, pts, pl.ray);
+            });
+            LG_LAUNCHED("k_light_rays");
+        }
+        k_light_combine<<<(unsigned)((V + 255) / 256), 256, 0, ctx->stream>>>(pl.sd, pl.sid, V, nt, lc, lights, pts,
+                                                                            pl.ray, L);
+        LG_LAUNCHED("k_light_combine");
+    } else {
+        for_dtype(dtype, [&](auto tag) {
+            using T = decltype(tag);
+            k_light<T><<<(unsigned)((V + 127) / 128), 128, 0, ctx->stream>>>(pl.sd, pl.sid, V, lc, lights, ek0, ek1,
+                                                                            pts, L);
+        });
+        LG_LAUNCHED("k_light");
+    }
     ktimer_end(ctx, "k_vertex_light", ev);
     return LG_OK;
 }

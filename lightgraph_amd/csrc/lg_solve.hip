@@ -135,6 +135,12 @@ SolveDesc make_desc(lg_grid *density, const lg_grid *temperature, int n, int vof
 }
 
 // ------------------------------------------------------- march constants ----
+bool step_is_exact(double h) {
+    if (!(h > 0.0) || !std::isfinite(h)) return false;
+    const double m = ldexp(h, 30);
+    return m == floor(m) && h < 8388608.0;
+}
+
 MarchConsts march_consts(const lg_shader *sh, const lg_march_params *mp) {
     MarchConsts mc{};
     mc.ds_sds_scds[0] = sh->density_scale;
@@ -161,6 +167,7 @@ LightConsts light_consts(const lg_shader *sh, const lg_march_params *mp) {
         lc.albedo[c] = sh->albedo[c];
         lc.ext[c] = sh->extinction[c];
     }
+    lc.exact_steps = step_is_exact(lc.h) ? 1 : 0;
     lc.emission_scale = sh->emission_scale;
     lc.ramp_in = sh->ramp_in;
     lc.ramp_out = sh->ramp_out;
@@ -192,18 +199,6 @@ __global__ void k_march_list(lgd::GridView g, MarchConsts mc, const float *__res
         D[(long long)i * ld + j] = tau;
         D[(long long)j * ld + i] = tau;
     }
-}
-
-template <typename T>
-__global__ void k_light_list(lgd::GridView g, lgd::GridView tg, int has_temp, LightConsts lc, lgl::DevLights L,
-                             uint32_t ek0, uint32_t ek1, uint32_t solve, const float *__restrict__ pts, int n,
-                             float *__restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const double p[3] = {(double)pts[3 * i], (double)pts[3 * i + 1], (double)pts[3 * i + 2]};
-    double acc[3];
-    vertex_light_one<T>(g, tg, has_temp, lc, L, ek0, ek1, (uint32_t)i, solve, p, acc);
-    for (int k = 0; k < 3; ++k) out[3 * i + k] = (float)acc[k];
 }
 
 // ------------------------------------------------------------ D init --------
@@ -433,22 +428,14 @@ int32_t lg_vertex_light(lg_ctx *ctx, const lg_grid *density, const lg_grid *temp
     LG_CHECK_ARG(ctx && density && d_pts && d_L_out, "lg_vertex_light: NULL argument");
     LG_TRY(check_params(shader, params));
     if (n <= 0) return LG_OK;
+    LG_CHECK_ARG(n <= 65536, "lg_vertex_light: n=%d above 65536", n);
     lgl::DevLights L{};
     LG_TRY(lgl::make_dev_lights(lights, n_lights, &L));
     LG_CHECK_ARG(!temperature || temperature->desc.dtype == LG_F32, "temperature grid must be float32");
-    const lgd::GridView g = make_view(density);
-    const lgd::GridView tg = temperature ? make_view(temperature) : lgd::GridView{};
-    const LightConsts lc = light_consts(shader, params);
-    hipEvent_t ev;
-    ktimer_begin(ctx, "k_vertex_light", &ev);
-    for_dtype(density->desc.dtype, [&](auto tag) {
-        using T = decltype(tag);
-        k_light_list<T><<<(unsigned)((n + 127) / 128), 128, 0, ctx->stream>>>(
-            g, tg, temperature ? 1 : 0, lc, L, (uint32_t)env_seed, (uint32_t)(env_seed >> 32), solve, d_pts, n, d_L_out);
-    });
-    LG_LAUNCHED("k_light_list");
-    ktimer_end(ctx, "k_vertex_light", ev);
-    return LG_OK;
+    Pipeline &pl = stage_pipeline(ctx);
+    int V = 0;
+    LG_TRY(pl_begin(ctx, pl, {make_desc(const_cast<lg_grid *>(density), temperature, n, 0, solve)}, &V, nullptr));
+    return pl_light(ctx, pl, V, density->desc.dtype, light_consts(shader, params), L, env_seed, d_pts, d_L_out);
 }
 
 int32_t lg_fw_ld(int32_t n) { return n <= 0 ? 0 : fw_ld(n); }

@@ -137,12 +137,15 @@ def _lights(lga):
     return [lga.parse_light("dir:0:1,1,1:1:-45,30,0"), lga.point_light(0, (1.0, 0.8, 0.6), 2000.0, (20, 70, 30))]
 
 
-def test_vertex_light_parity(lga, ctx, oracle, c1):
+@pytest.mark.parametrize("step", [0.5, 3.0, 0.3])
+def test_vertex_light_parity(lga, ctx, oracle, c1, step):
+    """Steps 0.5 and 3 take the wave-per-ray kernel (exact sample positions),
+    0.3 the one-lane-per-vertex sequential march."""
     grid, dense, _ = c1
     n = 1024
     pts = oracle.sample_points(dense, 854123, 0, n)
     sh = _c1_shader(lga)
-    p = lga.march_params(cutoff=0.005, shadow_step=0.5)
+    p = lga.march_params(cutoff=0.005, shadow_step=step)
     lights = _lights(lga)
     pb = ctx.upload(pts)
     lb = ctx.alloc(n * 12)
@@ -152,6 +155,41 @@ def test_vertex_light_parity(lga, ctx, oracle, c1):
     assert rel_err(got, want) < 1e-5
     pb.free()
     lb.free()
+
+
+def _sky(h=32, w=64):
+    """Sky gradient with one bright sun texel block."""
+    y = np.linspace(0.0, 1.0, h, dtype=np.float32)[:, None, None]
+    rgb = np.broadcast_to(np.concatenate([0.2 + 0.6 * y, 0.3 + 0.5 * y, 0.9 - 0.2 * y], axis=-1), (h, w, 3)).copy()
+    rgb[6:8, 40:42] = 500.0
+    return rgb
+
+
+@pytest.mark.parametrize("step", [0.5, 0.3])
+def test_vertex_light_env_and_multisample(lga, ctx, oracle, c1, step):
+    """Env light (2 draws per sample, 10x samples), a multi-sample point light
+    and a directional light in one list: the per-ray kernel's stream offsets and
+    the per-vertex sum order against the oracle."""
+    grid, dense, _ = c1
+    n = 512
+    pts = oracle.sample_points(dense, 854123, 2, n)
+    sh = _c1_shader(lga)
+    p = lga.march_params(cutoff=0.005, shadow_step=step)
+    rgb = _sky()
+    env = lga.EnvMap(ctx, rgb)
+    lights = [lga.point_light(0, (1.0, 0.8, 0.6), 2000.0, (20, 70, 30)),
+              lga.env_light(env, y_rot=30.0, color=(1.0, 0.9, 0.8), intensity=0.5, samples=2),
+              lga.parse_light("dir:0:1,1,1:1:-45,30,0")]
+    lights[0].samples = 2
+    pb = ctx.upload(pts)
+    lb = ctx.alloc(n * 12)
+    lga.vertex_light(ctx, grid, None, sh, lights, p, 4564446, 7, pb, n, lb)
+    got = lb.read(np.float32, (n, 3))
+    want = oracle.vertex_light(dense, sh, p, lights, pts, solve=7, envs=[None, rgb, None])
+    assert rel_err(got, want) < 1e-5
+    pb.free()
+    lb.free()
+    env.destroy()
 
 
 def _fw_device(lga, ctx, W):
