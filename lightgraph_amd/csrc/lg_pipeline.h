@@ -26,6 +26,16 @@ struct SolveDesc {
     int ld;                       // leading dimension of the optional dense outputs
     float *dense_D;               // optional n x ld optical depth (solve-local indices)
     int32_t *dense_pred;          // optional n x ld canonical predecessors
+    int cell_A;                   // neighbour grid: at most cell_A + 1 cells per axis
+    int cell_base;                // first cell id of this solve in the batch's cell space
+};
+
+// Uniform neighbour grid of one solve (cells at least the edge radius wide).
+struct CellDesc {
+    double lo[3];
+    double inv[3];
+    int n[3];
+    int base;
 };
 
 struct MarchConsts {
@@ -343,6 +353,13 @@ struct Pipeline {
     double *big_sums = nullptr;
     double *ray = nullptr;  // per (vertex, light sample) shadowed intensity
     size_t ray_cap = 0;
+    // neighbour grid
+    CellDesc *cd = nullptr;
+    uint32_t *ckeys = nullptr, *ckeys_sorted = nullptr;
+    int *cvals = nullptr, *cvals_sorted = nullptr;
+    int *cstart = nullptr, *cend = nullptr;
+    size_t cells_cap = 0, cend_cap = 0;
+    long long total_cells = 0;
     uint8_t *big_meta = nullptr;
     size_t bigT_cap = 0, bigsums_cap = 0, bigmeta_cap = 0;
 

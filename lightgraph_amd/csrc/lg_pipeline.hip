@@ -131,6 +131,88 @@ __global__ void k_light_combine(const SolveDesc *__restrict__ sd, const int *__r
 }
 
 // ---------------------------------------------------------- candidates ------
+
+// ------------------------------------------------------- neighbour grid ----
+// Candidates j > i with |p_i - p_j|^2 < r2 are found through a uniform grid of
+// cells at least r wide per solve: points within r of each other sit in the same
+// or adjacent cells (cell indices are clamped to the grid, which only merges
+// cells, so adjacency still holds).  Points are radix-sorted by cell; the 3 cells
+// of one x-run are consecutive in that order.
+__device__ __forceinline__ int cell_axis(double p, double lo, double inv, int n) {
+    const double f = floor((p - lo) * inv);
+    if (!(f >= 0.0)) return 0;
+    if (f > (double)(n - 1)) return n - 1;
+    return (int)f;
+}
+
+// Block s: bounding box of solve s's points and its cell grid.
+__global__ void __launch_bounds__(256) k_cell_setup(const SolveDesc *__restrict__ sd, const float *__restrict__ pts,
+                                                    double cell_min, CellDesc *__restrict__ cd) {
+    const SolveDesc &d = sd[blockIdx.x];
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int v = d.voff + threadIdx.x; v < d.voff + d.n; v += blockDim.x)
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = fminf(lo[c], pts[3 * v + c]);
+            hi[c] = fmaxf(hi[c], pts[3 * v + c]);
+        }
+    __shared__ float s_lo[3][256], s_hi[3][256];
+    for (int c = 0; c < 3; ++c) {
+        s_lo[c][threadIdx.x] = lo[c];
+        s_hi[c][threadIdx.x] = hi[c];
+    }
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int c = 0; c < 3; ++c) {
+                s_lo[c][threadIdx.x] = fminf(s_lo[c][threadIdx.x], s_lo[c][threadIdx.x + o]);
+                s_hi[c][threadIdx.x] = fmaxf(s_hi[c][threadIdx.x], s_hi[c][threadIdx.x + o]);
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        CellDesc c{};
+        for (int a = 0; a < 3; ++a) {
+            const double l = (double)s_lo[a][0], h = (double)s_hi[a][0];
+            const double ext = h - l;
+            if (!(ext >= 0.0) || !isfinite(ext) || !isfinite(l)) {
+                c.lo[a] = 0.0;
+                c.inv[a] = 0.0;
+                c.n[a] = 1;
+                continue;
+            }
+            const double cell = fmax(cell_min, ext / (double)d.cell_A);
+            int n = (int)floor(ext / cell) + 1;
+            if (n > d.cell_A + 1) n = d.cell_A + 1;
+            if (n < 1) n = 1;
+            c.lo[a] = l;
+            c.inv[a] = 1.0 / cell;
+            c.n[a] = n;
+        }
+        c.base = d.cell_base;
+        cd[blockIdx.x] = c;
+    }
+}
+
+__global__ void k_cell_keys(const CellDesc *__restrict__ cd, const int *__restrict__ sid, int V,
+                            const float *__restrict__ pts, uint32_t *__restrict__ keys, int *__restrict__ vals) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const CellDesc c = cd[sid[v]];
+    const int x = cell_axis((double)pts[3 * v], c.lo[0], c.inv[0], c.n[0]);
+    const int y = cell_axis((double)pts[3 * v + 1], c.lo[1], c.inv[1], c.n[1]);
+    const int z = cell_axis((double)pts[3 * v + 2], c.lo[2], c.inv[2], c.n[2]);
+    keys[v] = (uint32_t)(c.base + (z * c.n[1] + y) * c.n[0] + x);
+    vals[v] = v;
+}
+
+__global__ void k_cell_bounds(int V, const uint32_t *__restrict__ sk, int *__restrict__ cstart, int *__restrict__ cend) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= V) return;
+    const uint32_t key = sk[k];
+    if (k == 0 || sk[k - 1] != key) cstart[key] = k;
+    if (k == V - 1 || sk[k + 1] != key) cend[key] = k + 1;
+}
+
 __device__ __forceinline__ double pdist2(const float *__restrict__ pts, int i, int j) {
     const double dx = (double)pts[3 * i + 0] - (double)pts[3 * j + 0];
     const double dy = (double)pts[3 * i + 1] - (double)pts[3 * j + 1];
@@ -138,25 +220,95 @@ __device__ __forceinline__ double pdist2(const float *__restrict__ pts, int i, i
     return dx * dx + dy * dy + dz * dz;
 }
 
-// One wave per global vertex i: candidates j > i of the same solve with
-// |p_i - p_j|^2 < r2, ascending.  fill == nullptr: count only.
-__global__ void k_cand(const SolveDesc *__restrict__ sd, const int *__restrict__ sid, int V, const float *__restrict__ pts,
-                       double r2, const int *__restrict__ off, int *__restrict__ count, int *__restrict__ fill) {
-    const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+constexpr int kCandLds = 512;  // candidates per row sorted in LDS; longer rows take the ordered scan
+
+// One wave per global vertex i.  FILL = false: count candidates into count[i];
+// FILL = true: write them, ascending, at off[i].
+template <bool FILL>
+__global__ void __launch_bounds__(256) k_cand(const SolveDesc *__restrict__ sd, const CellDesc *__restrict__ cd,
+                                              const int *__restrict__ sid, int V, const float *__restrict__ pts,
+                                              double r2, const int *__restrict__ cstart, const int *__restrict__ cend,
+                                              const int *__restrict__ svals, const int *__restrict__ off,
+                                              int *__restrict__ count, int *__restrict__ fill) {
+    __shared__ int buf[4][kCandLds];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = blockIdx.x * 4 + wv;
     if (i >= V) return;
+    const CellDesc c = cd[sid[i]];
+    const int cx = cell_axis((double)pts[3 * i], c.lo[0], c.inv[0], c.n[0]);
+    const int cy = cell_axis((double)pts[3 * i + 1], c.lo[1], c.inv[1], c.n[1]);
+    const int cz = cell_axis((double)pts[3 * i + 2], c.lo[2], c.inv[2], c.n[2]);
+    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < c.n[0] ? cx + 1 : c.n[0] - 1;
+    // the 9 x-runs as position ranges of the sorted points
+    int rs[9], pre[10];
+    pre[0] = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+        const int y = cy + (r % 3) - 1, z = cz + (r / 3) - 1;
+        int s0 = 0x7fffffff, e0 = 0;
+        if (y >= 0 && y < c.n[1] && z >= 0 && z < c.n[2]) {
+            const int k0 = c.base + (z * c.n[1] + y) * c.n[0];
+            for (int x = x0; x <= x1; ++x) {
+                const int e = cend[k0 + x];
+                if (e > 0) {
+                    s0 = min(s0, cstart[k0 + x]);
+                    e0 = max(e0, e);
+                }
+            }
+        }
+        rs[r] = s0;
+        pre[r + 1] = pre[r] + (e0 > 0 ? e0 - s0 : 0);
+    }
+    const int total = pre[9];
+    int cnt = 0;
+    for (int f0 = 0; f0 < total; f0 += 64) {
+        const int f = f0 + lane;
+        bool hit = false;
+        int j = -1;
+        if (f < total) {
+            int pos = 0;
+#pragma unroll
+            for (int r = 0; r < 9; ++r)
+                if (f >= pre[r] && f < pre[r + 1]) pos = rs[r] + (f - pre[r]);
+            j = svals[pos];
+            hit = j > i && pdist2(pts, i, j) < r2;
+        }
+        const uint64_t m = __ballot(hit);
+        if (FILL && hit) {
+            const int slot = cnt + __popcll(m & ((1ull << lane) - 1ull));
+            if (slot < kCandLds) buf[wv][slot] = j;
+        }
+        cnt += __popcll(m);
+    }
+    if (!FILL) {
+        if (lane == 0) count[i] = cnt;
+        return;
+    }
+    const int o = off[i];
+    if (cnt <= kCandLds) {
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // rank of each candidate among the row's (distinct) candidates
+        for (int e = lane; e < cnt; e += 64) {
+            const int val = buf[wv][e];
+            int rank = 0;
+            for (int x = 0; x < cnt; ++x) rank += buf[wv][x] < val;
+            fill[o + rank] = val;
+        }
+        return;
+    }
+    // long row: ordered scan over the whole solve
     const SolveDesc &d = sd[sid[i]];
     const int end = d.voff + d.n;
-    int c = 0;
-    const int base_out = fill ? off[i] : 0;
+    int w = 0;
     for (int base = i + 1; base < end; base += 64) {
-        const int j = base + lane;
-        const bool hit = j < end && pdist2(pts, i, j) < r2;
-        const uint64_t m = __ballot(hit);
-        if (fill && hit) fill[base_out + c + __popcll(m & ((1ull << lane) - 1ull))] = j;
-        c += __popcll(m);
+        const int jj = base + lane;
+        const bool h = jj < end && pdist2(pts, i, jj) < r2;
+        const uint64_t m = __ballot(h);
+        if (h) fill[o + w + __popcll(m & ((1ull << lane) - 1ull))] = jj;
+        w += __popcll(m);
     }
-    if (!fill && lane == 0) count[i] = c;
 }
 
 __global__ void k_colcount(int V, const int *__restrict__ off, const int *__restrict__ cand, int *__restrict__ colcnt) {
@@ -706,25 +858,37 @@ int32_t Pipeline::reserve_vertices(lg_ctx *ctx, int V, int S) {
     A(&comp_off, (size_t)v_cap + 1);
     A(&big_list, (size_t)v_cap + 2);  // (component, size) pairs: at most v_cap / 129 of them
     A(&counters, 16);
+    A(&cd, (size_t)s_cap);
+    A(&ckeys, (size_t)v_cap);
+    A(&ckeys_sorted, (size_t)v_cap);
+    A(&cvals, (size_t)v_cap);
+    A(&cvals_sorted, (size_t)v_cap);
     // rocPRIM temporary storage for scans and sorts over up to v_cap keys
-    size_t b1 = 0, b2 = 0;
+    size_t b1 = 0, b2 = 0, b3 = 0;
     if (rc == LG_OK &&
         (rocprim::radix_sort_keys(nullptr, b1, keys, keys_sorted, (size_t)v_cap, 0, 64, ctx->stream) != hipSuccess ||
          rocprim::exclusive_scan(nullptr, b2, rowcnt, off, 0, (size_t)v_cap + 1, rocprim::plus<int>(), ctx->stream) !=
-             hipSuccess)) {
+             hipSuccess ||
+         rocprim::radix_sort_pairs(nullptr, b3, ckeys, ckeys_sorted, cvals, cvals_sorted, (size_t)v_cap, 0, 32,
+                                   ctx->stream) != hipSuccess)) {
         set_error("rocPRIM temporary-size query failed");
         rc = LG_ERR_HIP;
     }
-    tmp_vertex_bytes = std::max(b1, b2);
+    tmp_vertex_bytes = std::max(std::max(b1, b2), b3);
     A((uint8_t **)&tmp_vertex, tmp_vertex_bytes + 256);
     return rc;
 }
 
 void Pipeline::release_vertex(lg_ctx *ctx) {
-    void *ps[] = {sd, sid, rowcnt, off, colcnt, risky, visit, visit_pos, visit_list, adeg, aoff, acur, parent,
-                  root, keys, keys_sorted, flag, cid, member, local, comp_off, big_list, counters, tmp_vertex};
+    void *ps[] = {sd,     sid,  rowcnt,      off,    colcnt,       risky, visit,     visit_pos, visit_list,
+                  adeg,   aoff, acur,        parent, root,         keys,  keys_sorted, flag,    cid,
+                  member, local, comp_off,   big_list, counters,   tmp_vertex, cd,     ckeys, ckeys_sorted,
+                  cvals,  cvals_sorted};
     for (void *p : ps) dfree(ctx, p);
     sd = nullptr;
+    cd = nullptr;
+    ckeys = ckeys_sorted = nullptr;
+    cvals = cvals_sorted = nullptr;
     sid = rowcnt = off = colcnt = visit = visit_pos = visit_list = adeg = aoff = acur = parent = root = nullptr;
     flag = cid = member = local = comp_off = big_list = counters = nullptr;
     risky = nullptr;
@@ -735,8 +899,10 @@ void Pipeline::release_vertex(lg_ctx *ctx) {
 
 void Pipeline::release(lg_ctx *ctx) {
     release_vertex(ctx);
-    void *ps[] = {cand, accepted, epos, tmp_cand, edges_, w_, anb, aw, big_T, big_sums, big_meta, ray};
+    void *ps[] = {cand, accepted, epos, tmp_cand, edges_, w_, anb, aw, big_T, big_sums, big_meta, ray, cstart, cend};
     for (void *p : ps) dfree(ctx, p);
+    cstart = cend = nullptr;
+    cells_cap = cend_cap = 0;
     cand = accepted = epos = edges_ = anb = nullptr;
     w_ = aw = big_T = nullptr;
     big_meta = nullptr;
@@ -773,7 +939,18 @@ int32_t pl_begin(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs,
         nmax = std::max(nmax, d.n);
     }
     LG_TRY(pl.reserve_vertices(ctx, V, S));
-    LG_HIP(hipMemcpyAsync(pl.sd, descs.data(), sizeof(SolveDesc) * S, hipMemcpyHostToDevice, ctx->stream));
+    // neighbour-grid cell budget: at most (A + 1)^3 cells per solve, A = cbrt(2n) + 1
+    std::vector<SolveDesc> up(descs);
+    long long cells = 0;
+    for (auto &d : up) {
+        const int A = (int)floor(cbrt(2.0 * d.n)) + 1;
+        d.cell_A = A;
+        d.cell_base = (int)cells;
+        cells += (long long)(A + 1) * (A + 1) * (A + 1);
+    }
+    LG_CHECK_ARG(cells < (1ll << 31), "pipeline: neighbour grid too large (%lld cells)", cells);
+    pl.total_cells = cells;
+    LG_HIP(hipMemcpyAsync(pl.sd, up.data(), sizeof(SolveDesc) * S, hipMemcpyHostToDevice, ctx->stream));
     k_solve_ids<<<(unsigned)((V + 255) / 256), 256, 0, ctx->stream>>>(pl.sd, S, V, pl.sid);
     LG_LAUNCHED("k_solve_ids");
     *V_out = V;
@@ -835,8 +1012,27 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
     hipEvent_t ev;
     ktimer_begin(ctx, "k_graph", &ev);
     const unsigned wb = (unsigned)((V + 3) / 4);
+    // neighbour grid: cells at least r wide (a hair wider so that rounding in the
+    // cell index cannot separate two points closer than r by two cells)
+    const double cell_min = sqrt(r2) * (1.0 + 1e-6) + 1e-30;
+    k_cell_setup<<<S, 256, 0, st>>>(pl.sd, pts, cell_min, pl.cd);
+    LG_LAUNCHED("k_cell_setup");
+    k_cell_keys<<<vb, 256, 0, st>>>(pl.cd, pl.sid, V, pts, pl.ckeys, pl.cvals);
+    LG_LAUNCHED("k_cell_keys");
+    int kbits = 1;
+    while ((1ll << kbits) < pl.total_cells) ++kbits;
+    size_t sbytes = pl.tmp_vertex_bytes;
+    LG_HIP(rocprim::radix_sort_pairs(pl.tmp_vertex, sbytes, pl.ckeys, pl.ckeys_sorted, pl.cvals, pl.cvals_sorted,
+                                     (size_t)V, 0, kbits, st));
+    LG_TRY(grow(ctx, &pl.cstart, &pl.cells_cap, (size_t)pl.total_cells));
+    LG_TRY(grow(ctx, &pl.cend, &pl.cend_cap, (size_t)pl.total_cells));
+    LG_HIP(hipMemsetAsync(pl.cstart, 0, sizeof(int) * (size_t)pl.total_cells, st));
+    LG_HIP(hipMemsetAsync(pl.cend, 0, sizeof(int) * (size_t)pl.total_cells, st));
+    k_cell_bounds<<<vb, 256, 0, st>>>(V, pl.ckeys_sorted, pl.cstart, pl.cend);
+    LG_LAUNCHED("k_cell_bounds");
     LG_HIP(hipMemsetAsync(pl.rowcnt + V, 0, sizeof(int), st));
-    k_cand<<<wb, 256, 0, st>>>(pl.sd, pl.sid, V, pts, r2, nullptr, pl.rowcnt, nullptr);
+    k_cand<false><<<wb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted, nullptr,
+                                      pl.rowcnt, nullptr);
     LG_LAUNCHED("k_cand");
     LG_TRY(scan_excl(ctx, pl, pl.rowcnt, pl.off, (size_t)V + 1, false));
     int Ktot = 0;
@@ -847,7 +1043,8 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
     LG_TRY(grow(ctx, &pl.epos, &pl.epos_cap, (size_t)Ktot + 1));
     LG_TRY(grow(ctx, &pl.edges_, &pl.edges_cap, (size_t)2 * Ktot + 2));
     LG_TRY(grow(ctx, &pl.w_, &pl.w_cap, (size_t)Ktot + 1));
-    k_cand<<<wb, 256, 0, st>>>(pl.sd, pl.sid, V, pts, r2, pl.off, nullptr, pl.cand);
+    k_cand<true><<<wb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted, pl.off,
+                                     nullptr, pl.cand);
     LG_LAUNCHED("k_cand");
     const int cap = (max_connect > 0 && max_connect < 255) ? max_connect : 0;
     if (cap > 0) {
