@@ -61,6 +61,10 @@ struct LightConsts {
 // form h*ceil(t0/h) + k*h give the same bits.
 bool step_is_exact(double h);
 
+// Transmittance exp(ext * tau) of one channel (ext = -sigma_t <= 0); +INF (no
+// path) gives 0 also for a channel without extinction.
+__device__ __forceinline__ float trans_f(float ext, float tau) { return tau < INFINITY ? expf(ext * tau) : 0.0f; }
+
 MarchConsts march_consts(const lg_shader *sh, const lg_march_params *mp);
 LightConsts light_consts(const lg_shader *sh, const lg_march_params *mp);
 
@@ -340,6 +344,7 @@ struct Pipeline {
     uint64_t *keys = nullptr, *keys_sorted = nullptr;
     int *flag = nullptr, *cid = nullptr, *member = nullptr, *local = nullptr, *comp_off = nullptr;
     int *big_list = nullptr, *counters = nullptr;
+    int *cls16 = nullptr, *cls48 = nullptr, *cls128 = nullptr;  // components of 2-32, 33-64, 65-128 vertices
     void *tmp_vertex = nullptr;
     size_t tmp_vertex_bytes = 0;
     // candidate / edge sized

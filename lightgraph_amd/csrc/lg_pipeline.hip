@@ -364,6 +364,15 @@ __device__ __forceinline__ int lower_bound_i32(const int *a, int n, int x) {
     return lo;
 }
 
+// Orders the LDS accesses of the lanes of one wave (a one-wave workgroup needs no
+// s_barrier, and __syncthreads would also wait for every outstanding global
+// store, a full memory round trip per call).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Sequential greedy over contested edges, one wave per solve.
 __global__ void __launch_bounds__(64) k_greedy(const SolveDesc *__restrict__ sd, int cap, const int *__restrict__ off,
                                                const int *__restrict__ cand, const uint8_t *__restrict__ risky,
@@ -405,11 +414,169 @@ __global__ void __launch_bounds__(64) k_greedy(const SolveDesc *__restrict__ sd,
                 if (rj) cnt[j - v0] = (uint8_t)(cnt[j - v0] + 1);
             }
             ci += taken;
-            __syncthreads();
+            wave_sync();
             if (ri && ci >= cap) break;
         }
         if (ri && lane == 0) cnt[i - v0] = (uint8_t)ci;
-        __syncthreads();
+        wave_sync();
+    }
+}
+
+// The same sweep with the rows staged in LDS: per batch of up to 64 rows (and
+// kGreedyBuf candidates), one wave loads the row headers, then every candidate
+// and its risky flag with 32 loads in flight per lane, then runs the sequential
+// decisions out of LDS.  Row order and every decision are those of k_greedy.
+constexpr int kGreedyBuf = 2048;
+
+__global__ void __launch_bounds__(64) k_greedy_staged(const SolveDesc *__restrict__ sd, int cap,
+                                                      const int *__restrict__ off, const int *__restrict__ cand,
+                                                      const uint8_t *__restrict__ risky,
+                                                      const int *__restrict__ visit_list, const int *__restrict__ n_visit,
+                                                      int *__restrict__ accepted) {
+    extern __shared__ uint8_t cnt[];
+    __shared__ uint32_t cbuf[kGreedyBuf];
+    const int s = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int n = sd[s].n, v0 = sd[s].voff;
+    for (int k = lane; k < n; k += 64) cnt[k] = 0;
+    const int nv = *n_visit;
+    const int lo = lower_bound_i32(visit_list, nv, v0), hi = lower_bound_i32(visit_list, nv, v0 + n);
+    __syncthreads();
+    int idx = lo;
+    while (idx < hi) {
+        // row headers
+        const bool have = idx + lane < hi;
+        int i = 0, k0 = 0, len = 0, rf = 0;
+        if (have) {
+            i = visit_list[idx + lane];
+            k0 = off[i];
+            len = off[i + 1] - k0;
+            rf = risky[i];
+        }
+        int pre = len;  // inclusive scan
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(pre, o, 64);
+            if (lane >= o) pre += y;
+        }
+        const int pos = pre - len;
+        const uint64_t fits = __ballot(have && pre <= kGreedyBuf);
+        const int rows = __popcll(fits);  // rows form a prefix: pre grows with the lane
+        if (rows == 0) {
+            // a row longer than the buffer: the direct sweep for that row alone
+            const int ii = __shfl(i, 0, 64);
+            const bool ri = risky[ii] != 0;
+            int ci = ri ? (int)cnt[ii - v0] : 0;
+            if (!(ri && ci >= cap)) {
+                const int e = off[ii + 1];
+                for (int base = off[ii]; base < e; base += 64) {
+                    const int k = base + lane;
+                    const bool valid = k < e;
+                    const int j = valid ? cand[k] : v0;
+                    const bool rj = valid && risky[j] != 0;
+                    const bool contested = valid && (ri || rj);
+                    const bool elig = contested && (!rj || (int)cnt[j - v0] < cap);
+                    bool sel;
+                    int taken = 0;
+                    if (ri) {
+                        const uint64_t m = __ballot(elig);
+                        const int need = cap - ci;
+                        sel = elig && __popcll(m & ((1ull << lane) - 1ull)) < need;
+                        taken = min(__popcll(m), need);
+                    } else {
+                        sel = elig;
+                    }
+                    if (sel) {
+                        accepted[k] = 1;
+                        if (rj) cnt[j - v0] = (uint8_t)(cnt[j - v0] + 1);
+                    }
+                    ci += taken;
+                    wave_sync();
+                    if (ri && ci >= cap) break;
+                }
+                if (ri && lane == 0) cnt[ii - v0] = (uint8_t)ci;
+            }
+            wave_sync();
+            ++idx;
+            continue;
+        }
+        const int total = __shfl(pre, rows - 1, 64);
+        // row of every flat candidate position: mark row starts, then a running
+        // max over the positions (rows start in increasing order)
+        for (int q = lane; q < total; q += 64) cbuf[q] = 0u;
+        wave_sync();
+        if (lane < rows) cbuf[pos] = (uint32_t)lane;
+        wave_sync();
+        // candidates and their risky flags, 32 loads in flight per lane
+        int jv[kGreedyBuf / 64];
+        int carry = 0;
+#pragma unroll
+        for (int u = 0; u < kGreedyBuf / 64; ++u) {
+            const int q = lane + 64 * u;
+            int r = q < total ? (int)cbuf[q] : 0;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int y = __shfl_up(r, o, 64);
+                if (lane >= o) r = max(r, y);
+            }
+            r = max(r, carry);
+            carry = __shfl(r, 63, 64);
+            const int kb = __shfl(k0, r, 64), pb = __shfl(pos, r, 64);  // all lanes active
+            jv[u] = -1;
+            if (q < total) jv[u] = cand[kb + (q - pb)];
+        }
+        wave_sync();
+#pragma unroll
+        for (int u = 0; u < kGreedyBuf / 64; ++u) {
+            const int q = lane + 64 * u;
+            if (q < total) cbuf[q] = (uint32_t)(jv[u] - v0) | ((risky[jv[u]] != 0) ? 0x80000000u : 0u);
+        }
+        wave_sync();
+        // row headers stay in the lanes that loaded them (readlane); the first
+        // candidate chunk of the next row is fetched while this row decides
+        const int mr = i - v0;
+        uint32_t wnext = lane < __builtin_amdgcn_readlane(len, 0) ? cbuf[__builtin_amdgcn_readlane(pos, 0) + lane] : 0u;
+        for (int r = 0; r < rows; ++r) {
+            const int il = __builtin_amdgcn_readlane(mr, r), p0 = __builtin_amdgcn_readlane(pos, r);
+            const int ln = __builtin_amdgcn_readlane(len, r), kk0 = __builtin_amdgcn_readlane(k0, r);
+            const bool ri = __builtin_amdgcn_readlane(rf, r) != 0;
+            const uint32_t w0 = wnext;
+            if (r + 1 < rows) {
+                const int p1 = __builtin_amdgcn_readlane(pos, r + 1), l1 = __builtin_amdgcn_readlane(len, r + 1);
+                wnext = lane < l1 ? cbuf[p1 + lane] : 0u;
+            }
+            int ci = ri ? (int)cnt[il] : 0;
+            if (ri && ci >= cap) continue;
+            for (int t0 = 0; t0 < ln; t0 += 64) {
+                const int t = t0 + lane;
+                const bool valid = t < ln;
+                const uint32_t w = t0 == 0 ? w0 : (valid ? cbuf[p0 + t] : 0u);
+                const int jl = (int)(w & 0x7fffffffu);
+                const bool rj = valid && (w >> 31) != 0;
+                const bool contested = valid && (ri || rj);
+                const int cj = rj ? (int)cnt[jl] : 0;
+                const bool elig = contested && (!rj || cj < cap);
+                bool sel;
+                int taken = 0;
+                if (ri) {
+                    const uint64_t m = __ballot(elig);
+                    const int need = cap - ci;
+                    sel = elig && __popcll(m & ((1ull << lane) - 1ull)) < need;
+                    taken = min(__popcll(m), need);
+                } else {
+                    sel = elig;
+                }
+                if (sel) {
+                    accepted[kk0 + t] = 1;
+                    if (rj) cnt[jl] = (uint8_t)(cj + 1);
+                }
+                ci += taken;
+                if (ri && ci >= cap) break;
+            }
+            if (ri && lane == 0) cnt[il] = (uint8_t)ci;
+            wave_sync();
+        }
+        idx += rows;
     }
 }
 
@@ -524,20 +691,39 @@ __global__ void k_comp_offsets(int V, const int *__restrict__ flag, const int *_
     if (flag[k]) comp_off[cid_excl[k]] = k;
 }
 
-// local index of every vertex inside its component and the (component, size)
-// pairs of the components too large for the LDS kernel.
-__global__ void k_comp_local(int V, const int *__restrict__ flag, const int *__restrict__ cid_excl,
+// local index of every vertex inside its component, the (component, size) pairs
+// of the components too large for the LDS kernels, and the lists of the other
+// components by size class (2..32, 33..64, 65..128).  An isolated vertex has
+// P = [1], so its diffused radiance is its own light; it is written here.
+__global__ void k_comp_local(int V, const SolveDesc *__restrict__ sd, const int *__restrict__ sid,
+                             const int *__restrict__ flag, const int *__restrict__ cid_excl,
                              const int *__restrict__ comp_off, const int *__restrict__ member, int *__restrict__ local,
-                             int *__restrict__ big_list, int *__restrict__ n_big) {
+                             int *__restrict__ big_list, int *__restrict__ counters, int *__restrict__ cls16,
+                             int *__restrict__ cls48, int *__restrict__ cls128, const float *__restrict__ L,
+                             float *__restrict__ out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= V) return;
     const int c = cid_excl[k] + flag[k] - 1;
     local[member[k]] = k - comp_off[c];
+    if (!flag[k]) return;
     const int m = comp_off[c + 1] - comp_off[c];
-    if (flag[k] && m > kSmallComp) {
-        const int slot = atomicAdd(n_big, 1);
+    if (m > kSmallComp) {
+        const int slot = atomicAdd(&counters[0], 1);
         big_list[2 * slot] = c;
         big_list[2 * slot + 1] = m;
+    } else if (m > 64) {
+        cls128[atomicAdd(&counters[3], 1)] = c;
+    } else if (m > 32) {
+        cls48[atomicAdd(&counters[2], 1)] = c;
+    } else if (m > 1) {
+        cls16[atomicAdd(&counters[1], 1)] = c;
+    } else {
+        const int v = member[k];
+        for (int q = 0; q < 3; ++q) out[3 * v + q] = L[3 * v + q];
+        const SolveDesc &d = sd[sid[v]];
+        const int li = v - d.voff;
+        if (d.dense_D) d.dense_D[(long long)li * d.ld + li] = 0.0f;
+        if (d.dense_pred) d.dense_pred[(long long)li * d.ld + li] = li;
     }
 }
 
@@ -545,84 +731,155 @@ __global__ void k_comp_local(int V, const int *__restrict__ flag, const int *__r
 // One workgroup per component of m <= 128 vertices: build the m x m optical
 // depth matrix in LDS from the adjacency, close it pivot by pivot (ascending
 // vertex order), then diffuse (LightGraph.h:554-579) and write the outputs.
-__global__ void __launch_bounds__(256) k_comp_small(const SolveDesc *__restrict__ sd, const int *__restrict__ sid,
-                                                    const int *__restrict__ comp_off, const int *__restrict__ member,
-                                                    const int *__restrict__ local, const int *__restrict__ aoff,
-                                                    const int *__restrict__ anb, const float *__restrict__ aw,
-                                                    const float *__restrict__ L, float e0, float e1, float e2,
-                                                    float *__restrict__ out) {
-    __shared__ float T[kSmallComp][kSmallComp + 1];
-    __shared__ int mem[kSmallComp];
-    __shared__ float Ln[kSmallComp][3];
-    const int c = blockIdx.x;
+// One workgroup of (MS/4)^2 threads per component of at most MS vertices.  The
+// component's optical-depth matrix is built in LDS from the adjacency (padding
+// vertices are isolated: +INF rows, zero diagonal), then closed in registers
+// (4 x 4 per thread) with the four-pivots-per-round-trip scheme of the pivot
+// tile (fw_pivot_tile): pivots in ascending vertex order, bit-identical to the
+// one-pivot loop.  Diffusion (LightGraph.h:554-587) follows from the registers:
+// column sums = row sums (D is symmetric), each reduced over the row's lanes in
+// a fixed order.
+template <int MS>
+__global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
+    const SolveDesc *__restrict__ sd, const int *__restrict__ sid, const int *__restrict__ cls,
+    const int *__restrict__ comp_off, const int *__restrict__ member, const int *__restrict__ local,
+    const int *__restrict__ aoff, const int *__restrict__ anb, const float *__restrict__ aw,
+    const float *__restrict__ L, float e0, float e1, float e2, float *__restrict__ out) {
+    constexpr int CG = MS / 4;  // column groups = lanes sharing a row group
+    constexpr int NT = CG * CG;
+    __shared__ float T[MS][MS + 1];
+    __shared__ float P[4][MS];
+    __shared__ float G[4][MS];
+    __shared__ int mem[MS];
+    __shared__ float Ln[MS][3];
+    const int c = cls[blockIdx.x];
     const int a0 = comp_off[c];
     const int m = comp_off[c + 1] - a0;
-    if (m > kSmallComp) return;
+    if (m > MS) return;
     const int tid = threadIdx.x;
-    if (m == 1) {
-        if (tid == 0) {
-            const int v = member[a0];
-            for (int k = 0; k < 3; ++k) out[3 * v + k] = L[3 * v + k];  // P = [1]: diffuse = L
-            const SolveDesc &d = sd[sid[v]];
-            const int li = v - d.voff;
-            if (d.dense_D) d.dense_D[(long long)li * d.ld + li] = 0.0f;
-            if (d.dense_pred) d.dense_pred[(long long)li * d.ld + li] = li;
-        }
-        return;
-    }
-    for (int a = tid; a < m; a += blockDim.x) mem[a] = member[a0 + a];
-    for (int e = tid; e < m * m; e += blockDim.x) {
-        const int i = e / m, j = e - i * m;
+    const int rg = tid / CG, cg = tid % CG;
+    for (int a = tid; a < m; a += NT) mem[a] = member[a0 + a];
+    for (int e = tid; e < MS * MS; e += NT) {
+        const int i = e / MS, j = e - i * MS;
         T[i][j] = (i == j) ? 0.0f : INFINITY;
     }
     __syncthreads();
-    for (int a = tid; a < m; a += blockDim.x) {
+    for (int a = tid; a < m; a += NT) {
         const int v = mem[a];
         for (int p = aoff[v]; p < aoff[v + 1]; ++p) T[a][local[anb[p]]] = aw[p];
     }
     __syncthreads();
-    const int ty = tid >> 4, tx = tid & 15;
-    for (int k = 0; k < m; ++k) {
-        for (int i = ty; i < m; i += 16) {
-            const float dik = T[i][k];
-            if (!(dik < INFINITY)) continue;
-            for (int j = tx; j < m; j += 16) {
-                const float nv = dik + T[k][j];
-                if (nv < T[i][j]) T[i][j] = nv;
-            }
+    float v[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) v[a][b] = T[4 * rg + a][4 * cg + b];
+    const int groups = (m + 3) / 4;
+    for (int g = 0; g < groups; ++g) {
+        const int k = 4 * g;
+        if (rg == g) {
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) P[a][4 * cg + b] = v[a][b];
         }
         __syncthreads();
-    }
-    const float ex[3] = {e0, e1, e2};
-    for (int j = tid; j < m; j += blockDim.x) {
-        double s[3] = {0.0, 0.0, 0.0};
-        for (int i = 0; i < m; ++i) {
-            const float dij = T[i][j];
-            for (int q = 0; q < 3; ++q) s[q] += (double)expf(ex[q] * dij);
+        if (tid < MS) {
+            const int x = tid;
+            const float s01 = P[0][k + 1], s02 = P[0][k + 2], s03 = P[0][k + 3];
+            const float s12 = fminf(P[1][k + 2], s01 + s02);
+            const float s13 = fminf(P[1][k + 3], s01 + s03);
+            const float s23 = fminf(fminf(P[2][k + 3], s02 + s03), s12 + s13);
+            const float g0 = P[0][x];
+            const float g1 = fminf(P[1][x], s01 + g0);
+            const float g2 = fminf(fminf(P[2][x], s02 + g0), s12 + g1);
+            const float g3 = fminf(fminf(fminf(P[3][x], s03 + g0), s13 + g1), s23 + g2);
+            G[0][x] = g0;
+            G[1][x] = g1;
+            G[2][x] = g2;
+            G[3][x] = g3;
         }
-        const int v = mem[j];
-        for (int q = 0; q < 3; ++q) Ln[j][q] = (float)((double)L[3 * v + q] / (s[q] < 1.0 ? 1.0 : s[q]));
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int i = 4 * rg + a, j = 4 * cg + b;
+                const float m01 = fminf(G[0][i] + G[0][j], G[1][i] + G[1][j]);
+                const float m23 = fminf(G[2][i] + G[2][j], G[3][i] + G[3][j]);
+                v[a][b] = fminf(v[a][b], fminf(m01, m23));
+            }
+    }
+    // diffusion: two reductions over the row's lanes (transmittances kept in
+    // registers between them where the register budget allows)
+    const float ex[3] = {e0, e1, e2};
+    constexpr bool kKeep = MS < 128;
+    float P3[3][4][4];
+    auto tr = [&](int q, int a, int b) {
+        return kKeep ? P3[q][a][b] : ((4 * cg + b < m) ? trans_f(ex[q], v[a][b]) : 0.0f);
+    };
+    if (kKeep) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+#pragma unroll
+                for (int q = 0; q < 3; ++q) P3[q][a][b] = (4 * cg + b < m) ? trans_f(ex[q], v[a][b]) : 0.0f;
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double sq[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            sq[q] = 0.0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) sq[q] += (double)tr(q, a, b);
+#pragma unroll
+            for (int o = CG / 2; o > 0; o >>= 1) sq[q] += __shfl_xor(sq[q], o, CG);
+        }
+        const int i = 4 * rg + a;
+        if (cg == 0 && i < m) {
+            const int vi = mem[i];
+            for (int q = 0; q < 3; ++q) Ln[i][q] = (float)((double)L[3 * vi + q] / (sq[q] < 1.0 ? 1.0 : sq[q]));
+        }
     }
     __syncthreads();
-    for (int i = tid; i < m; i += blockDim.x) {
-        double o[3] = {0.0, 0.0, 0.0};
-        for (int j = 0; j < m; ++j) {
-            const float dij = T[i][j];
-            for (int q = 0; q < 3; ++q) o[q] += (double)Ln[j][q] * (double)expf(ex[q] * dij);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+        double oq[3];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            oq[q] = 0.0;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = 4 * cg + b;
+                if (j < m) oq[q] += (double)Ln[j][q] * (double)tr(q, a, b);
+            }
+#pragma unroll
+            for (int o = CG / 2; o > 0; o >>= 1) oq[q] += __shfl_xor(oq[q], o, CG);
         }
-        const int v = mem[i];
-        for (int q = 0; q < 3; ++q) out[3 * v + q] = (float)o[q];
+        const int i = 4 * rg + a;
+        if (cg == 0 && i < m) {
+            const int vi = mem[i];
+            for (int q = 0; q < 3; ++q) out[3 * vi + q] = (float)oq[q];
+        }
     }
     const SolveDesc &d = sd[sid[mem[0]]];
+    if (!d.dense_D && !d.dense_pred) return;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) T[4 * rg + a][4 * cg + b] = v[a][b];
+    __syncthreads();
     if (d.dense_D) {
-        for (int e = tid; e < m * m; e += blockDim.x) {
+        for (int e = tid; e < m * m; e += NT) {
             const int i = e / m, j = e - i * m;
             d.dense_D[(long long)(mem[i] - d.voff) * d.ld + (mem[j] - d.voff)] = T[i][j];
         }
     }
     if (d.dense_pred) {
         // canonical predecessor: smallest neighbour k of j minimising D[i][k] + w(k, j)
-        for (int e = tid; e < m * m; e += blockDim.x) {
+        for (int e = tid; e < m * m; e += NT) {
             const int i = e / m, j = e - i * m;
             int best_k = -1;
             if (i == j) {
@@ -704,9 +961,9 @@ __global__ void __launch_bounds__(256) k_big_rowsum(const BigJob *__restrict__ j
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
     for (int j = threadIdx.x; j < jb.m; j += blockDim.x) {
         const float d = row[j];
-        a0 += expf(e0 * d);
-        a1 += expf(e1 * d);
-        a2 += expf(e2 * d);
+        a0 += trans_f(e0, d);
+        a1 += trans_f(e1, d);
+        a2 += trans_f(e2, d);
     }
     __shared__ double s[3][256];
     s[0][threadIdx.x] = a0;
@@ -739,9 +996,9 @@ __global__ void __launch_bounds__(256) k_big_diffuse(const BigJob *__restrict__ 
         const double *sj = sums + 3 * (long long)(jb.row0 + j);
         float ln[3];
         for (int c = 0; c < 3; ++c) ln[c] = (float)((double)L[3 * vj + c] / (sj[c] < 1.0 ? 1.0 : sj[c]));
-        a0 += (double)(ln[0] * expf(e0 * d));
-        a1 += (double)(ln[1] * expf(e1 * d));
-        a2 += (double)(ln[2] * expf(e2 * d));
+        a0 += (double)(ln[0] * trans_f(e0, d));
+        a1 += (double)(ln[1] * trans_f(e1, d));
+        a2 += (double)(ln[2] * trans_f(e2, d));
     }
     __shared__ double s[3][256];
     s[0][threadIdx.x] = a0;
@@ -859,6 +1116,9 @@ int32_t Pipeline::reserve_vertices(lg_ctx *ctx, int V, int S) {
     A(&big_list, (size_t)v_cap + 2);  // (component, size) pairs: at most v_cap / 129 of them
     A(&counters, 16);
     A(&cd, (size_t)s_cap);
+    A(&cls16, (size_t)v_cap);
+    A(&cls48, (size_t)v_cap);
+    A(&cls128, (size_t)v_cap);
     A(&ckeys, (size_t)v_cap);
     A(&ckeys_sorted, (size_t)v_cap);
     A(&cvals, (size_t)v_cap);
@@ -883,8 +1143,9 @@ void Pipeline::release_vertex(lg_ctx *ctx) {
     void *ps[] = {sd,     sid,  rowcnt,      off,    colcnt,       risky, visit,     visit_pos, visit_list,
                   adeg,   aoff, acur,        parent, root,         keys,  keys_sorted, flag,    cid,
                   member, local, comp_off,   big_list, counters,   tmp_vertex, cd,     ckeys, ckeys_sorted,
-                  cvals,  cvals_sorted};
+                  cvals,  cvals_sorted, cls16, cls48, cls128};
     for (void *p : ps) dfree(ctx, p);
+    cls16 = cls48 = cls128 = nullptr;
     sd = nullptr;
     cd = nullptr;
     ckeys = ckeys_sorted = nullptr;
@@ -1015,6 +1276,8 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
     // neighbour grid: cells at least r wide (a hair wider so that rounding in the
     // cell index cannot separate two points closer than r by two cells)
     const double cell_min = sqrt(r2) * (1.0 + 1e-6) + 1e-30;
+    hipEvent_t ev_c;
+    ktimer_begin(ctx, "k_cand", &ev_c);
     k_cell_setup<<<S, 256, 0, st>>>(pl.sd, pts, cell_min, pl.cd);
     LG_LAUNCHED("k_cell_setup");
     k_cell_keys<<<vb, 256, 0, st>>>(pl.cd, pl.sid, V, pts, pl.ckeys, pl.cvals);
@@ -1046,6 +1309,7 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
     k_cand<true><<<wb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted, pl.off,
                                      nullptr, pl.cand);
     LG_LAUNCHED("k_cand");
+    ktimer_end(ctx, "k_cand", ev_c);
     const int cap = (max_connect > 0 && max_connect < 255) ? max_connect : 0;
     if (cap > 0) {
         LG_HIP(hipMemsetAsync(pl.colcnt, 0, sizeof(int) * V, st));
@@ -1058,9 +1322,16 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
         LG_HIP(hipMemsetAsync(pl.visit + V, 0, sizeof(int), st));
         LG_TRY(scan_excl(ctx, pl, pl.visit, pl.visit_pos, (size_t)V + 1, false));
         k_compact_rows<<<vb, 256, 0, st>>>(V, pl.visit, pl.visit_pos, pl.visit_list);
-        k_greedy<<<S, 64, (size_t)nmax, st>>>(pl.sd, cap, pl.off, pl.cand, pl.risky, pl.visit_list, pl.visit_pos + V,
-                                              pl.accepted);
+        hipEvent_t ev_g;
+        ktimer_begin(ctx, "k_greedy", &ev_g);
+        if (nmax <= 48 * 1024)
+            k_greedy_staged<<<S, 64, (size_t)nmax, st>>>(pl.sd, cap, pl.off, pl.cand, pl.risky, pl.visit_list,
+                                                         pl.visit_pos + V, pl.accepted);
+        else
+            k_greedy<<<S, 64, (size_t)nmax, st>>>(pl.sd, cap, pl.off, pl.cand, pl.risky, pl.visit_list,
+                                                  pl.visit_pos + V, pl.accepted);
         LG_LAUNCHED("k_greedy");
+        ktimer_end(ctx, "k_greedy", ev_g);
     }
     LG_HIP(hipMemsetAsync(pl.accepted + Ktot, 0, sizeof(int), st));
     LG_TRY(scan_excl(ctx, pl, pl.accepted, pl.epos, (size_t)Ktot + 1, true));
@@ -1115,20 +1386,21 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     LG_TRY(scan_excl(ctx, pl, pl.flag, pl.cid, (size_t)V + 1, false));
     k_comp_offsets<<<vb, 256, 0, st>>>(V, pl.flag, pl.cid, pl.comp_off, pl.cid + V);
     LG_HIP(hipMemsetAsync(pl.counters, 0, sizeof(int) * 16, st));
-    k_comp_local<<<vb, 256, 0, st>>>(V, pl.flag, pl.cid, pl.comp_off, pl.member, pl.local, pl.big_list, pl.counters);
+    k_comp_local<<<vb, 256, 0, st>>>(V, pl.sd, pl.sid, pl.flag, pl.cid, pl.comp_off, pl.member, pl.local, pl.big_list,
+                                     pl.counters, pl.cls16, pl.cls48, pl.cls128, L, out);
     LG_LAUNCHED("k_comp_local");
     // counts and the large-component pairs come back in one synchronisation
     const int max_big = V / (kSmallComp + 1);
-    std::vector<int> host((size_t)3 + 2 * (size_t)max_big);
+    std::vector<int> host((size_t)6 + 2 * (size_t)max_big);
     LG_HIP(hipMemcpyAsync(&host[0], pl.cid + V, sizeof(int), hipMemcpyDeviceToHost, st));
-    LG_HIP(hipMemcpyAsync(&host[1], pl.counters, sizeof(int), hipMemcpyDeviceToHost, st));
-    LG_HIP(hipMemcpyAsync(&host[2], d_nedges, sizeof(int), hipMemcpyDeviceToHost, st));
+    LG_HIP(hipMemcpyAsync(&host[1], pl.counters, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
+    LG_HIP(hipMemcpyAsync(&host[5], d_nedges, sizeof(int), hipMemcpyDeviceToHost, st));
     if (max_big > 0)
-        LG_HIP(hipMemcpyAsync(&host[3], pl.big_list, sizeof(int) * 2 * (size_t)max_big, hipMemcpyDeviceToHost, st));
+        LG_HIP(hipMemcpyAsync(&host[6], pl.big_list, sizeof(int) * 2 * (size_t)max_big, hipMemcpyDeviceToHost, st));
     LG_HIP(hipStreamSynchronize(st));
     ktimer_end(ctx, "k_components", ev);
-    const int C = host[0], nbig = host[1];
-    res->n_edges_total = host[2];
+    const int C = host[0], nbig = host[1], n16 = host[2], n48 = host[3], n128 = host[4];
+    res->n_edges_total = host[5];
     res->n_components = C;
     res->n_big = nbig;
     res->Ktot = Ktot;
@@ -1150,10 +1422,20 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     // (c) + diffusion: small components in one launch
     const float e0 = (float)ext[0], e1 = (float)ext[1], e2 = (float)ext[2];
     ktimer_begin(ctx, "k_comp_small", &ev);
-    if (C > 0) {
-        k_comp_small<<<C, 256, 0, st>>>(pl.sd, pl.sid, pl.comp_off, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, L, e0,
-                                        e1, e2, out);
-        LG_LAUNCHED("k_comp_small");
+    if (n128 > 0) {
+        k_comp_small<kSmallComp><<<n128, 1024, 0, st>>>(pl.sd, pl.sid, pl.cls128, pl.comp_off, pl.member, pl.local,
+                                                        pl.aoff, pl.anb, pl.aw, L, e0, e1, e2, out);
+        LG_LAUNCHED("k_comp_small<128>");
+    }
+    if (n48 > 0) {
+        k_comp_small<64><<<n48, 256, 0, st>>>(pl.sd, pl.sid, pl.cls48, pl.comp_off, pl.member, pl.local, pl.aoff,
+                                               pl.anb, pl.aw, L, e0, e1, e2, out);
+        LG_LAUNCHED("k_comp_small<64>");
+    }
+    if (n16 > 0) {
+        k_comp_small<32><<<n16, 64, 0, st>>>(pl.sd, pl.sid, pl.cls16, pl.comp_off, pl.member, pl.local, pl.aoff,
+                                              pl.anb, pl.aw, L, e0, e1, e2, out);
+        LG_LAUNCHED("k_comp_small<32>");
     }
     ktimer_end(ctx, "k_comp_small", ev);
     if (nbig == 0) {
@@ -1166,7 +1448,7 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
 
     // large components: one arena, batched blocked Floyd-Warshall, batched diffusion
     std::vector<std::pair<int, int>> big((size_t)nbig);
-    for (int b = 0; b < nbig; ++b) big[b] = {host[3 + 2 * b], host[4 + 2 * b]};
+    for (int b = 0; b < nbig; ++b) big[b] = {host[6 + 2 * b], host[7 + 2 * b]};
     std::sort(big.begin(), big.end());  // deterministic order (slots come from atomics)
     std::vector<BigJob> jobs((size_t)nbig);
     std::vector<int> ld_prefix((size_t)nbig), m_prefix((size_t)nbig);

@@ -228,9 +228,9 @@ __global__ void k_rowsum(const float *__restrict__ D, int n, int ld, float e0, f
     float a0 = 0.f, a1 = 0.f, a2 = 0.f;
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
         const float d = row[j];
-        a0 += expf(e0 * d);
-        a1 += expf(e1 * d);
-        a2 += expf(e2 * d);
+        a0 += trans_f(e0, d);
+        a1 += trans_f(e1, d);
+        a2 += trans_f(e2, d);
     }
     __shared__ double s[3][256];
     s[0][threadIdx.x] = a0;
@@ -262,9 +262,9 @@ __global__ void k_diffuse_out(const float *__restrict__ D, int n, int ld, float 
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int j = threadIdx.x; j < n; j += blockDim.x) {
         const float d = row[j];
-        a0 += (double)(Ln[3 * j + 0] * expf(e0 * d));
-        a1 += (double)(Ln[3 * j + 1] * expf(e1 * d));
-        a2 += (double)(Ln[3 * j + 2] * expf(e2 * d));
+        a0 += (double)(Ln[3 * j + 0] * trans_f(e0, d));
+        a1 += (double)(Ln[3 * j + 1] * trans_f(e1, d));
+        a2 += (double)(Ln[3 * j + 2] * trans_f(e2, d));
     }
     __shared__ double s[3][256];
     s[0][threadIdx.x] = a0;
