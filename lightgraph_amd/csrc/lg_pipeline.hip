@@ -640,32 +640,36 @@ __device__ __forceinline__ int uf_find(int *P, int x) {
     }
 }
 
-// Union-find over the finite edges of one solve per workgroup: hook the larger
-// root under the smaller until a full pass changes nothing; the root of a
-// component is its smallest vertex.
-__global__ void __launch_bounds__(1024) k_components(const SolveDesc *__restrict__ sd, const int *__restrict__ off,
-                                                     const int *__restrict__ epos, const int *__restrict__ edges,
-                                                     const float *__restrict__ w, int *__restrict__ P,
-                                                     int *__restrict__ root) {
-    const int s = blockIdx.x;
-    const int v0 = sd[s].voff, n = sd[s].n;
-    const int e0 = epos[off[v0]], e1 = epos[off[v0 + n]];
-    for (int v = v0 + threadIdx.x; v < v0 + n; v += blockDim.x) atomic_store_i32(&P[v], v);
-    __syncthreads();
-    int changed;
-    do {
-        changed = 0;
-        for (int e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-            if (!(w[e] < INFINITY)) continue;
-            const int ru = uf_find(P, edges[2 * e]), rv = uf_find(P, edges[2 * e + 1]);
-            if (ru != rv) {
-                atomicMin(&P[max(ru, rv)], min(ru, rv));
-                changed = 1;
-            }
+// Lock-free union-find over the finite edges of every solve of the call, one
+// thread per edge: hook the larger root under the smaller with a CAS that only
+// succeeds while it is still a root, so one pass joins every edge.  The final root
+// of a component is its smallest vertex, whatever the interleaving.
+__global__ void k_uf_init(int V, int *__restrict__ P) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) atomic_store_i32(&P[v], v);
+}
+
+__global__ void k_uf_union(const int *__restrict__ edges, const float *__restrict__ w,
+                           const int *__restrict__ n_edges, int *__restrict__ P) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= *n_edges || !(w[e] < INFINITY)) return;
+    int a = edges[2 * e], b = edges[2 * e + 1];
+    for (;;) {
+        a = uf_find(P, a);
+        b = uf_find(P, b);
+        if (a == b) return;
+        if (a < b) {
+            const int t = a;
+            a = b;
+            b = t;
         }
-        changed = __syncthreads_or(changed);
-    } while (changed);
-    for (int v = v0 + threadIdx.x; v < v0 + n; v += blockDim.x) root[v] = uf_find(P, v);
+        if (atomicCAS(&P[a], a, b) == a) return;
+    }
+}
+
+__global__ void k_uf_root(int V, int *__restrict__ P, int *__restrict__ root) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v < V) root[v] = uf_find(P, v);
 }
 
 __global__ void k_keys(int V, int bits, const int *__restrict__ root, uint64_t *__restrict__ keys) {
@@ -1361,7 +1365,6 @@ int32_t pl_march(lg_ctx *ctx, Pipeline &pl, int dtype, const MarchConsts &mc, co
 int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, int V, int Ktot,
                       const double ext[3], const float *L, float *out, PipelineResult *res) {
     hipStream_t st = ctx->stream;
-    const int S = (int)descs.size();
     const unsigned vb = (unsigned)((V + 255) / 256);
     const int *d_nedges = pl.epos + Ktot;
     hipEvent_t ev;
@@ -1374,8 +1377,15 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     LG_TRY(grow(ctx, &pl.anb, &pl.anb_cap, (size_t)2 * Ktot + 2));
     LG_TRY(grow(ctx, &pl.aw, &pl.aw_cap, (size_t)2 * Ktot + 2));
     k_adj_fill<<<eb, 256, 0, st>>>(pl.edges_, pl.w_, d_nedges, pl.aoff, pl.acur, pl.anb, pl.aw);
-    k_components<<<S, 1024, 0, st>>>(pl.sd, pl.off, pl.epos, pl.edges_, pl.w_, pl.parent, pl.root);
-    LG_LAUNCHED("k_components");
+    hipEvent_t ev_u;
+    ktimer_begin(ctx, "k_union_find", &ev_u);
+    k_uf_init<<<vb, 256, 0, st>>>(V, pl.parent);
+    k_uf_union<<<eb, 256, 0, st>>>(pl.edges_, pl.w_, d_nedges, pl.parent);
+    k_uf_root<<<vb, 256, 0, st>>>(V, pl.parent, pl.root);
+    LG_LAUNCHED("k_uf_root");
+    ktimer_end(ctx, "k_union_find", ev_u);
+    hipEvent_t ev_s;
+    ktimer_begin(ctx, "k_comp_sort", &ev_s);
     int bits = 1;
     while ((1 << bits) < V) ++bits;
     k_keys<<<vb, 256, 0, st>>>(V, bits, pl.root, pl.keys);
@@ -1389,6 +1399,7 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     k_comp_local<<<vb, 256, 0, st>>>(V, pl.sd, pl.sid, pl.flag, pl.cid, pl.comp_off, pl.member, pl.local, pl.big_list,
                                      pl.counters, pl.cls16, pl.cls48, pl.cls128, L, out);
     LG_LAUNCHED("k_comp_local");
+    ktimer_end(ctx, "k_comp_sort", ev_s);
     // counts and the large-component pairs come back in one synchronisation
     const int max_big = V / (kSmallComp + 1);
     std::vector<int> host((size_t)6 + 2 * (size_t)max_big);
