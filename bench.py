@@ -28,10 +28,27 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-KERNELS = ("k_fw_minplus", "k_fw_panel", "k_fw_pivot", "k_march_edges", "k_vertex_light", "k_graph",
-           "k_sample_points", "k_diffuse", "k_components", "k_comp_small")
+# Kernel-level timers (one kernel, or one kernel per size class) that can carry
+# the roofline, and stage timers reported beside them.
+KERNELS = ("k_light_rays", "k_march_edges", "k_fw_minplus", "k_fw_panel", "k_fw_pivot", "k_comp_small",
+           "k_sample_points", "k_big_fill", "k_diffuse")
+STAGES = ("k_vertex_light", "k_graph", "k_cand", "k_greedy", "k_components", "k_union_find", "k_comp_sort")
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
+# kernel -> (bound, work counter of lg_solver_stats, algorithmic FP32 ops per unit)
+#   marches: 32 FP32 ops per trilinear sample (index transform + 8-tap trilinear 23,
+#            scale/clamp/floor 3, accumulate 2, amortised exp 4);
+#   min-plus: one add + one min per (i, j, k) of each executed 128^3 tile update;
+#   small components: one add + one min per (i, j, k) of each component's closure.
+ROOF = {
+    "k_light_rays": ("valu", "shadow_samples", 32.0),
+    "k_march_edges": ("valu", "edge_samples", 32.0),
+    "k_fw_minplus": ("valu", "fw_tiles", 2.0 * 128 ** 3),
+    "k_fw_panel": ("valu", "fw_panel_tiles", 2.0 * 128 ** 3),
+    "k_fw_pivot": ("valu", "fw_pivots", 2.0 * 128 ** 3),
+    "k_comp_small": ("valu", "small_updates", 2.0),
+}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 
 WORKLOAD = {"workload": "c2_plain_solve", "volume": "512^3 blobs20+fBm6, ~30% active, fp32 bricks",
             "points": 8192, "edge_radius": 12.0, "max_connect": 12, "edge_step": 0.5,
@@ -92,12 +109,19 @@ def bench_batched(lga, ctx, shader, params, lights, count, reps):
     for _ in range(reps):
         run_plain()
     ctx.mark(3)
+    ctx.kernel_time_reset()
+    ctx.kernel_timing(True)
     for _ in range(reps):
         run_batch()
     ctx.mark(4)
     ctx.sync()
+    ctx.kernel_timing(False)
     t_plain = ctx.elapsed_ms(2, 3) / reps
     t_batch = ctx.elapsed_ms(3, 4) / reps
+    btimes = kernel_times(ctx, KERNELS + STAGES)
+    batch.solve_batch(items, shader, lights, params, radius=radius, seed=854123, stats=True)
+    bwork = {k: v * reps for k, v in batch.stats().items()}
+    broof = kernel_roofline(btimes, bwork, "c3_batch")
     plain.destroy()
     batch.destroy()
     for g in grids:
@@ -107,38 +131,45 @@ def bench_batched(lga, ctx, shader, params, lights, count, reps):
             "plain_ms": round(t_plain, 3), "batched_ms": round(t_batch, 3),
             "plain_solves_per_s": round(count * 1000.0 / t_plain, 2),
             "batched_solves_per_s": round(count * 1000.0 / t_batch, 2),
-            "speedup": round(t_plain / t_batch, 3), "max_abs_diff_batched_vs_plain": max_diff}
+            "speedup": round(t_plain / t_batch, 3), "max_abs_diff_batched_vs_plain": max_diff,
+            "roofline": broof,
+            "kernels": {k: {"launches": v["launches"], "avg_ms": v["avg_ms"]} for k, v in btimes.items()}}
 
 
-def kernel_roofline(ctx, ld):
-    """Roofline object for the kernel with the largest total time in the timed region."""
-    tiles = ld // 128
-    # algorithmic FP32 ops per launch: one add + one min per (i, j, k) over the
-    # off-pivot block rows/cols of one pivot block (2 * 128 * (ld - 128)^2)
-    table = {
-        "k_fw_minplus": ("valu", 2.0 * 128 * (ld - 128) ** 2, "TFLOP/s", PEAK_FP32_TFLOPS),
-    }
-    best = None
-    for name in KERNELS:
-        launches, total = ctx.kernel_time(name)
-        if launches and (best is None or total > best[2]):
-            best = (name, launches, total)
-    if best is None:
-        return None, {}
-    name, launches, total = best
-    avg_s = total / launches / 1e3
-    times = {}
-    for nm in KERNELS:
-        l, t = ctx.kernel_time(nm)
-        if l:
-            times[nm] = {"launches": l, "avg_ms": round(t / l, 5)}
-    if name in table:
-        bound, work, unit, peak = table[name]
-        achieved = work / avg_s / 1e12
-        return {"kernel": name, "bound": bound, "achieved": round(achieved, 3), "peak": peak, "unit": unit,
-                "frac": round(achieved / peak, 4), "traffic": None, "avg_launch_ms": round(avg_s * 1e3, 5)}, times
-    return {"kernel": name, "bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
-            "traffic": None, "avg_launch_ms": round(avg_s * 1e3, 5)}, times
+def kernel_times(ctx, names):
+    out = {}
+    for nm in names:
+        launches, total = ctx.kernel_time(nm)
+        if launches:
+            out[nm] = {"launches": launches, "total_ms": total, "avg_ms": round(total / launches, 5)}
+    return out
+
+
+def kernel_roofline(times, work, workload):
+    """Roofline object for the kernel with the largest total time among KERNELS:
+    achieved = algorithmic work of its launches (device work counters of the same
+    solves) / their summed HIP-event duration."""
+    cands = [(t["total_ms"], nm) for nm, t in times.items() if nm in KERNELS]
+    if not cands:
+        return None
+    _, name = max(cands)
+    t = times[name]
+    roof = {"kernel": name, "bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
+            "traffic": None, "avg_launch_ms": t["avg_ms"], "launches": t["launches"]}
+    if name in ROOF:
+        bound, key, ops = ROOF[name]
+        units = work.get(key, 0)
+        achieved = units * ops / (t["total_ms"] * 1e-3) / 1e12
+        roof.update({"bound": bound, "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "work_unit": key,
+                     "units_per_launch": round(units / t["launches"], 1), "ops_per_unit": ops})
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            tr = json.load(f).get(workload, {}).get(name)
+        if tr:
+            roof["traffic"] = tr["hbm_bytes_per_launch"]
+            roof["traffic_gbs"] = round(tr["hbm_bytes_per_launch"] / (t["avg_ms"] * 1e-3) / 1e9, 1)
+    return roof
 
 
 def cpu_baseline(lga, ctx, grid, shader, params, lights, n_points, radius, budget_s):
@@ -184,6 +215,9 @@ def main():
     ap.add_argument("--batch-volumes", type=int, default=64)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--latency-solves", type=int, default=50)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device three times with k_copy "
+                         "(the known-size kernel that traffic counters are scaled against)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
@@ -211,7 +245,15 @@ def main():
     gpu_ms = ctx.elapsed_ms(0, 1)
     ctx.kernel_timing(False)
     ms_per_step = gpu_ms / args.steps
-    roof, times = kernel_roofline(ctx, lga.fw_ld(n))
+    times = kernel_times(ctx, KERNELS + STAGES)
+    # the same solves again, untimed, with the device work counters on
+    work = {}
+    for k in range(args.steps):
+        solver.solve(grid, shader, lights, params, n=n, radius=radius, seed=854123, solve=args.warmup + k,
+                     stats=True)
+        for key, v in solver.stats().items():
+            work[key] = work.get(key, 0) + v
+    roof = kernel_roofline(times, work, WORKLOAD["workload"])
     b = solver.buffers()
     lat = []
     for k in range(args.latency_solves):
@@ -227,7 +269,9 @@ def main():
             "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
             "wall_ms_per_step": round(wall * 1e3 / args.steps, 4), "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "fp32 (fp64 ray/DDA positions)", "data": "synthetic",
-            "config": dict(WORKLOAD, edges_last_solve=b.n_edges), "roofline": roof, "kernels": times,
+            "config": dict(WORKLOAD, edges_last_solve=b.n_edges), "roofline": roof,
+            "kernels": {k: {"launches": v["launches"], "avg_ms": v["avg_ms"]} for k, v in times.items()},
+            "work_per_solve": {k: round(v / args.steps, 1) for k, v in work.items()},
             "latency": {"p50_ms": pct(0.50), "p99_ms": pct(0.99), "solves": len(lat), "mode": "one solve alone"}}
     solver.destroy()
     if args.batch_volumes > 0:
@@ -236,6 +280,16 @@ def main():
         line["cpu_baseline"] = cpu_baseline(lga, ctx, grid, shader, params, lights, n, radius, args.cpu_budget)
     else:
         line["cpu_baseline"] = None
+    if args.copy_calibration_mib > 0:
+        nbytes = args.copy_calibration_mib << 20
+        a, b2 = ctx.alloc(nbytes), ctx.alloc(nbytes)
+        lga.lib().lg_memset_d(ctx.handle, a.ptr, 1, nbytes)
+        for _ in range(3):
+            lga._check(lga.lib().lg_copy_d(ctx.handle, b2.ptr, a.ptr, nbytes), "copy")
+        ctx.sync()
+        a.free()
+        b2.free()
+        line["copy_calibration_bytes"] = nbytes
     grid.destroy()
     ctx.close()
     print(json.dumps(line), flush=True)

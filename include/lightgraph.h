@@ -197,6 +197,10 @@ int32_t lg_free(lg_ctx *ctx, void *d_ptr);
 int32_t lg_memcpy_h2d(lg_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
 int32_t lg_memcpy_d2h(lg_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
 int32_t lg_memset_d(lg_ctx *ctx, void *d_dst, int32_t value, size_t bytes);
+/* Stream-ordered device-to-device copy by a plain streaming kernel (k_copy);
+ * bytes a multiple of 16, pointers 16-byte aligned.  No reference counterpart:
+ * it is the known-size kernel that HBM traffic counters are calibrated on. */
+int32_t lg_copy_d(lg_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
 
 /* Stream-ordered timer: lg_timer_mark records an event, lg_timer_elapsed syncs. */
 int32_t lg_timer_mark(lg_ctx *ctx, int32_t slot);
@@ -299,10 +303,29 @@ typedef struct lg_solve_params {
     uint32_t solve;        /* solve index (the reference's LightGraph iteration i)  */
     int32_t external_points; /* 1: use the points already in the solver's d_points */
     int32_t timing;        /* 1: record per-stage HIP events                       */
-    int32_t flags;         /* LG_SOLVE_DENSE: also write the dense n x ld outputs   */
+    int32_t flags;         /* LG_SOLVE_DENSE: also write the dense n x ld outputs;
+                            * LG_SOLVE_STATS: count the work (lg_solver_stats)       */
 } lg_solve_params;
 
-enum { LG_SOLVE_DENSE = 1 };
+enum { LG_SOLVE_DENSE = 1, LG_SOLVE_STATS = 2 };
+
+/* Work done by the last lg_solve / lg_solve_batch call, counted on the device
+ * when it ran with LG_SOLVE_STATS (otherwise the counters read 0).  The counts are
+ * the algorithmic units of the kernels' rooflines. */
+typedef struct lg_solve_stats {
+    uint64_t points;          /* vertices of the call                                   */
+    uint64_t edges;           /* accepted edges (kernel b rays)                         */
+    uint64_t edge_samples;    /* trilinear samples of the edge march                    */
+    uint64_t shadow_rays;     /* vertex-light shadow rays                               */
+    uint64_t shadow_samples;  /* their trilinear samples                                */
+    uint64_t fw_tiles;        /* 128x128x128 min-plus remainder tile updates (kernel c) */
+    uint64_t fw_panel_tiles;  /* the same for the row-panel tiles                       */
+    uint64_t fw_pivots;       /* 128x128 pivot-tile closures                            */
+    uint64_t small_updates;   /* sum of m^3 over the components closed in LDS (m <= 128) */
+    uint64_t components;      /* connected components                                   */
+    uint64_t big_components;  /* components above 128 vertices                          */
+} lg_solve_stats;
+int32_t lg_solver_stats(lg_ctx *ctx, const lg_solver *solver, lg_solve_stats *out);
 enum { LG_SOLVER_DENSE_D = 1, LG_SOLVER_PRED = 2 };
 
 /* One LightGraph solve (LightGraph::operator() body, LightGraph.h:163-189):

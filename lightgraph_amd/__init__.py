@@ -343,12 +343,12 @@ class Solver:
         self.last_dense = False
 
     def solve(self, density, shader, lights, params, n, radius, seed=854123, solve=0, max_connect=12,
-              temperature=None, env_seed=4564446, external_points=False, timing=False, dense=None):
+              temperature=None, env_seed=4564446, external_points=False, timing=False, dense=None, stats=False):
         sp = lg_solve_params()
         sp.n, sp.max_connect, sp.radius, sp.seed, sp.env_seed = n, max_connect, radius, seed, env_seed
         sp.solve, sp.external_points, sp.timing = solve, 1 if external_points else 0, 1 if timing else 0
         dense = self.dense if dense is None else dense
-        sp.flags = 1 if dense else 0
+        sp.flags = (1 if dense else 0) | (2 if stats else 0)
         self.last_dense = dense
         arr = light_array(lights)
         _check(lib().lg_solve(self.ctx.handle, self.handle, density.handle,
@@ -356,7 +356,7 @@ class Solver:
                               C.byref(params), C.byref(sp)), "lg_solve")
 
     def solve_batch(self, items, shader, lights, params, radius, seed=854123, max_connect=12, env_seed=4564446,
-                    timing=False):
+                    timing=False, stats=False):
         """items: sequence of (density Grid, n, solve index[, temperature Grid]).
         Item k owns vertices [voff_k, voff_k + n_k) of the buffers."""
         arr = (_abi.lg_batch_item * len(items))()
@@ -370,6 +370,7 @@ class Solver:
         sp = lg_solve_params()
         sp.max_connect, sp.radius, sp.seed, sp.env_seed = max_connect, radius, seed, env_seed
         sp.timing = 1 if timing else 0
+        sp.flags = 2 if stats else 0
         self.last_dense = False
         la = light_array(lights)
         _check(lib().lg_solve_batch(self.ctx.handle, self.handle, arr, len(items), C.byref(shader), la, len(lights),
@@ -392,6 +393,12 @@ class Solver:
         b = _abi.lg_solver_buffers()
         _check(lib().lg_solver_get(self.handle, C.byref(b)), "solver_get")
         return b
+
+    def stats(self):
+        """Work counters of the last call made with stats=True (lg_solver_stats)."""
+        st = _abi.lg_solve_stats()
+        _check(lib().lg_solver_stats(self.ctx.handle, self.handle, C.byref(st)), "solver_stats")
+        return {k: int(getattr(st, k)) for k, _ in st._fields_}
 
     def timing(self):
         t = _abi.lg_solve_timing()

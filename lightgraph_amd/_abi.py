@@ -92,6 +92,11 @@ class lg_solve_params(C.Structure):
                 ("external_points", C.c_int32), ("timing", C.c_int32), ("flags", C.c_int32)]
 
 
+class lg_solve_stats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("points", "edges", "edge_samples", "shadow_rays", "shadow_samples",
+                                          "fw_tiles", "fw_panel_tiles", "fw_pivots", "small_updates", "components", "big_components")]
+
+
 class lg_batch_item(C.Structure):
     _fields_ = [("density", C.c_void_p), ("temperature", C.c_void_p), ("n", C.c_int32), ("solve", C.c_uint32)]
 
@@ -122,6 +127,7 @@ SIGNATURES = {
     "lg_free": (I32, [P, P]),
     "lg_memcpy_h2d": (I32, [P, P, P, SZ]),
     "lg_memcpy_d2h": (I32, [P, P, P, SZ]),
+    "lg_copy_d": (I32, [P, P, P, SZ]),
     "lg_memset_d": (I32, [P, P, I32, SZ]),
     "lg_timer_mark": (I32, [P, I32]),
     "lg_timer_elapsed": (I32, [P, I32, I32, C.POINTER(C.c_float)]),
@@ -147,6 +153,7 @@ SIGNATURES = {
     "lg_solve": (I32, [P, P, P, P, C.POINTER(lg_shader), C.POINTER(lg_light), I32,
                        C.POINTER(lg_march_params), C.POINTER(lg_solve_params)]),
     "lg_fw_solve_batched": (I32, [P, I32, C.POINTER(I32), C.POINTER(C.c_void_p), C.POINTER(I32)]),
+    "lg_solver_stats": (I32, [P, P, C.POINTER(lg_solve_stats)]),
     "lg_solve_batch": (I32, [P, P, C.POINTER(lg_batch_item), I32, C.POINTER(lg_shader), C.POINTER(lg_light), I32,
                              C.POINTER(lg_march_params), C.POINTER(lg_solve_params)]),
     "lg_splat_radius": (I32, [P, I64, I64, DP, DP]),

@@ -114,8 +114,9 @@ struct FwJob {
 
 // Batched pivot closure: block b closes the pivot tile of job active[b].
 __global__ void __launch_bounds__(1024) k_fw_pivot_b(const FwJob *__restrict__ jobs, const int *__restrict__ active,
-                                                     int kb) {
+                                                     int kb, unsigned long long *__restrict__ stats) {
     const FwJob j = jobs[active[blockIdx.x]];
+    if (stats && threadIdx.x == 0) stat_add(stats, ST_FW_PIVOTS, 1ull);
     fw_pivot_tile(j.D, j.ld, kb);
 }
 
@@ -134,7 +135,7 @@ __device__ __forceinline__ void tri_decode(int p, int T, int &a, int &b) {
 
 template <bool PANEL>
 __device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, int kb, int nt, int *__restrict__ flags,
-                                                int tile) {
+                                                int tile, unsigned long long *__restrict__ stats = nullptr) {
     int Ib, Jb;
     if (PANEL) {
         Ib = kb;
@@ -146,6 +147,7 @@ __device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, i
         Jb = b < kb ? b : b + 1;
         if (!flags[Ib] || !flags[Jb]) return;
     }
+    if (stats && threadIdx.x == 0) stat_add(stats, PANEL ? ST_FW_PANEL_TILES : ST_FW_TILES, 1ull);
     __shared__ float Xs[KC][TB];
     __shared__ float Ys[KC][TB];
     const int tid = threadIdx.x;
@@ -242,7 +244,7 @@ __global__ void __launch_bounds__(256) k_fw_minplus(float *__restrict__ D, int l
 template <bool PANEL>
 __global__ void __launch_bounds__(256) k_fw_minplus_b(const FwJob *__restrict__ jobs, const int *__restrict__ active,
                                                        const int *__restrict__ prefix, int n_active, int kb,
-                                                       int *__restrict__ flags) {
+                                                       int *__restrict__ flags, unsigned long long *__restrict__ stats) {
     const int b = (int)blockIdx.x;
     int lo = 0, hi = n_active - 1;
     while (lo < hi) {
@@ -253,7 +255,7 @@ __global__ void __launch_bounds__(256) k_fw_minplus_b(const FwJob *__restrict__ 
             hi = mid - 1;
     }
     const FwJob j = jobs[active[lo]];
-    fw_minplus_tile<PANEL>(j.D, j.ld, kb, j.nt, flags + j.flag_off, b - prefix[lo]);
+    fw_minplus_tile<PANEL>(j.D, j.ld, kb, j.nt, flags + j.flag_off, b - prefix[lo], stats);
 }
 
 int32_t fw_solve_impl(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld) {
@@ -287,7 +289,8 @@ int32_t fw_solve_impl(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld) {
 
 // Uneven batch: round kb runs the pivot tiles of every matrix with more than kb
 // tile rows in one launch, then all their panels, then all their remainders.
-int32_t fw_solve_batched_impl(lg_ctx *ctx, const std::vector<float *> &Ds, const std::vector<int> &ns) {
+int32_t fw_solve_batched_impl(lg_ctx *ctx, const std::vector<float *> &Ds, const std::vector<int> &ns,
+                              unsigned long long *stats) {
     const int count = (int)Ds.size();
     std::vector<FwJob> jobs;
     int max_nt = 0, flag_total = 0;
@@ -339,16 +342,16 @@ int32_t fw_solve_batched_impl(lg_ctx *ctx, const std::vector<float *> &Ds, const
         const int n = round_n[kb];
         hipEvent_t e1, e2, e3;
         ktimer_begin(ctx, "k_fw_pivot", &e1);
-        k_fw_pivot_b<<<n, 1024, 0, ctx->stream>>>(d_jobs, act, kb);
+        k_fw_pivot_b<<<n, 1024, 0, ctx->stream>>>(d_jobs, act, kb, stats);
         LG_LAUNCHED("k_fw_pivot_b");
         ktimer_end(ctx, "k_fw_pivot", e1);
         if (panel_blocks[kb] == 0) continue;
         ktimer_begin(ctx, "k_fw_panel", &e2);
-        k_fw_minplus_b<true><<<panel_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + n, n, kb, d_flags);
+        k_fw_minplus_b<true><<<panel_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + n, n, kb, d_flags, stats);
         LG_LAUNCHED("k_fw_panel_b");
         ktimer_end(ctx, "k_fw_panel", e2);
         ktimer_begin(ctx, "k_fw_minplus", &e3);
-        k_fw_minplus_b<false><<<rem_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + 2 * n, n, kb, d_flags);
+        k_fw_minplus_b<false><<<rem_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + 2 * n, n, kb, d_flags, stats);
         LG_LAUNCHED("k_fw_minplus_b");
         ktimer_end(ctx, "k_fw_minplus", e3);
     }

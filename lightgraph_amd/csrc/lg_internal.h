@@ -114,7 +114,8 @@ struct lg_envmap {
 
 struct lg_solver {
     int n_max = 0, ld_max = 0;
-    bool with_pred = false, with_dense = false, timed = false;
+    bool with_pred = false, with_dense = false, timed = false, stats_valid = false;
+    int n_big = 0;
     lg_solver_buffers buf{};
     void *pipeline = nullptr;  // lg::Pipeline
     hipEvent_t ev[6] = {};
@@ -148,10 +149,21 @@ int32_t grid_finalize(lg_ctx *ctx, lg_grid *g);
 
 // helpers implemented in the stage translation units
 int32_t grid_build_sampling_tables(lg_ctx *ctx, lg_grid *g);
+// Work counters of one solve call (LG_SOLVE_STATS), slots of a device u64 array.
+// Each counter has kStatSlots slots picked by block index (spreads the atomics);
+// the host sums them.
+enum : int { ST_EDGE_SAMPLES = 0, ST_SHADOW_RAYS, ST_SHADOW_SAMPLES, ST_FW_TILES, ST_FW_PIVOTS, ST_SMALL_UPDATES,
+             ST_FW_PANEL_TILES, ST_COUNT = 8 };
+constexpr int kStatSlots = 64;
+__device__ __forceinline__ void stat_add(unsigned long long *stats, int counter, unsigned long long v) {
+    atomicAdd(&stats[counter * kStatSlots + (blockIdx.x & (kStatSlots - 1))], v);
+}
+
 int32_t fw_solve_impl(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld);
 // Blocked Floyd-Warshall over many matrices of uneven size n[g] (ld = fw_ld(n[g]))
 // with one pivot, one panel and one remainder launch per 128-pivot round.
-int32_t fw_solve_batched_impl(lg_ctx *ctx, const std::vector<float *> &Ds, const std::vector<int> &ns);
+int32_t fw_solve_batched_impl(lg_ctx *ctx, const std::vector<float *> &Ds, const std::vector<int> &ns,
+                              unsigned long long *stats = nullptr);
 int32_t fw_pred_impl(lg_ctx *ctx, int32_t n, const float *d_D, int32_t ld, int32_t *d_pred,
                      const int32_t *d_edges, const float *d_w, int32_t n_edges);
 

@@ -72,7 +72,7 @@ LightConsts light_consts(const lg_shader *sh, const lg_march_params *mp);
 // ray misses the grid bbox or the dropout |T|^2 < cutoff fires.
 template <typename T>
 __device__ __forceinline__ float march_edge_one(const lgd::GridView &g, const MarchConsts &mc, const float *pi,
-                                                const float *pj) {
+                                                const float *pj, int *nsamp = nullptr) {
     const double sp[3] = {(double)pi[0], (double)pi[1], (double)pi[2]};
     double dir[3] = {(double)pj[0] - sp[0], (double)pj[1] - sp[1], (double)pj[2] - sp[2]};
     const double len = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
@@ -85,8 +85,11 @@ __device__ __forceinline__ float march_edge_one(const lgd::GridView &g, const Ma
     if (!lgd::isect_world_ray(g, s, sp, dir, lgd::kDeltaD, (double)dist)) return INFINITY;
     double tau = 0.0;
     double t0, t1;
+    int count = 0;
     while (lgd::isect_march(g, s, t0, t1)) {
         for (double t = mc.h * ceil(t0 / mc.h); t <= t1; t += mc.h) {
+            ++count;
+            if (nsamp) *nsamp = count;
             const float rho = lgd::trilinear<T>(g, s.e[0] + s.d[0] * t, s.e[1] + s.d[1] * t, s.e[2] + s.d[2] * t);
             double dens = (double)rho * mc.ds_sds_scds[0] * mc.ds_sds_scds[1] * mc.ds_sds_scds[2];
             dens = (mc.dmax < dens) ? mc.dmax : dens;
@@ -165,7 +168,7 @@ __device__ __forceinline__ bool shadow_tau(const lgd::GridView &g, const LightCo
 template <typename T>
 __device__ __forceinline__ bool shadow_tau_wave(const lgd::GridView &g, const LightConsts &lc, const double p[3],
                                                 const double ldir[3], double distance, double *tau_out,
-                                                bool *dropped) {
+                                                bool *dropped, int *nsamp) {
     const int lane = threadIdx.x & 63;
     const double sd[3] = {-ldir[0], -ldir[1], -ldir[2]};
     lgd::Isect is;
@@ -178,6 +181,7 @@ __device__ __forceinline__ bool shadow_tau_wave(const lgd::GridView &g, const Li
         for (int base = 0;; base += 64) {
             const double t = ts + (double)(base + lane) * lc.h;
             const bool act = t <= t1;
+            *nsamp += __popcll(__ballot(act));
             double x = 0.0;
             if (act) {
                 const float rho =
@@ -356,6 +360,8 @@ struct Pipeline {
     // large components
     float *big_T = nullptr;
     double *big_sums = nullptr;
+    unsigned long long *stats = nullptr;  // ST_COUNT counters; kernels add to them when stats_on
+    bool stats_on = false;
     double *ray = nullptr;  // per (vertex, light sample) shadowed intensity
     size_t ray_cap = 0;
     // neighbour grid

@@ -85,7 +85,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_light_rays(const SolveDesc *__restrict__ sd, const int *__restrict__ sid,
                                                     int V, int nt, LightConsts lc, lgl::DevLights L, uint32_t ek0,
                                                     uint32_t ek1, const float *__restrict__ pts,
-                                                    double *__restrict__ ray) {
+                                                    double *__restrict__ ray, unsigned long long *__restrict__ stats) {
     const long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (w >= (long long)V * nt) return;
     const int v = (int)(w / nt), t = (int)(w - (long long)v * nt);
@@ -102,10 +102,16 @@ __global__ void __launch_bounds__(256) k_light_rays(const SolveDesc *__restrict_
     double tau;
     bool dropped;
     double r[3] = {0.0, 0.0, 0.0};
-    if (shadow_tau_wave<T>(d.g, lc, p, ldir, distance, &tau, &dropped) && !dropped)
+    int ns = 0;
+    if (shadow_tau_wave<T>(d.g, lc, p, ldir, distance, &tau, &dropped, &ns) && !dropped)
         for (int k = 0; k < 3; ++k) r[k] = inten[k] * (double)expf((float)(lc.ext[k] * tau));
-    if ((threadIdx.x & 63) == 0)
+    if ((threadIdx.x & 63) == 0) {
         for (int k = 0; k < 3; ++k) ray[3 * w + k] = r[k];
+        if (stats) {
+            stat_add(stats, ST_SHADOW_RAYS, 1ull);
+            stat_add(stats, ST_SHADOW_SAMPLES, (unsigned long long)ns);
+        }
+    }
 }
 
 __global__ void k_light_combine(const SolveDesc *__restrict__ sd, const int *__restrict__ sid, int V, int nt,
@@ -597,11 +603,18 @@ __global__ void k_edges(int V, const int *__restrict__ off, const int *__restric
 template <typename T>
 __global__ void k_march(const SolveDesc *__restrict__ sd, const int *__restrict__ sid, MarchConsts mc,
                         const float *__restrict__ pts, const int *__restrict__ edges, int edge_limit,
-                        const int *__restrict__ n_edges, float *__restrict__ w) {
+                        const int *__restrict__ n_edges, float *__restrict__ w, unsigned long long *__restrict__ stats) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= min(*n_edges, edge_limit)) return;
-    const int i = edges[2 * e], j = edges[2 * e + 1];
-    w[e] = march_edge_one<T>(sd[sid[i]].g, mc, pts + 3 * (long long)i, pts + 3 * (long long)j);
+    int ns = 0;
+    if (e < min(*n_edges, edge_limit)) {
+        const int i = edges[2 * e], j = edges[2 * e + 1];
+        w[e] = march_edge_one<T>(sd[sid[i]].g, mc, pts + 3 * (long long)i, pts + 3 * (long long)j, &ns);
+    }
+    if (stats) {  // one atomic per wave
+        unsigned long long v = (unsigned long long)ns;
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if ((threadIdx.x & 63) == 0 && v) stat_add(stats, ST_EDGE_SAMPLES, v);
+    }
 }
 
 // ----------------------------------------------------------- adjacency ------
@@ -748,7 +761,8 @@ __global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
     const SolveDesc *__restrict__ sd, const int *__restrict__ sid, const int *__restrict__ cls,
     const int *__restrict__ comp_off, const int *__restrict__ member, const int *__restrict__ local,
     const int *__restrict__ aoff, const int *__restrict__ anb, const float *__restrict__ aw,
-    const float *__restrict__ L, float e0, float e1, float e2, float *__restrict__ out) {
+    const float *__restrict__ L, float e0, float e1, float e2, float *__restrict__ out,
+    unsigned long long *__restrict__ stats) {
     constexpr int CG = MS / 4;  // column groups = lanes sharing a row group
     constexpr int NT = CG * CG;
     __shared__ float T[MS][MS + 1];
@@ -762,6 +776,7 @@ __global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
     if (m > MS) return;
     const int tid = threadIdx.x;
     const int rg = tid / CG, cg = tid % CG;
+    if (stats && tid == 0) stat_add(stats, ST_SMALL_UPDATES, (unsigned long long)m * m * m);
     for (int a = tid; a < m; a += NT) mem[a] = member[a0 + a];
     for (int e = tid; e < MS * MS; e += NT) {
         const int i = e / MS, j = e - i * MS;
@@ -1164,8 +1179,10 @@ void Pipeline::release_vertex(lg_ctx *ctx) {
 
 void Pipeline::release(lg_ctx *ctx) {
     release_vertex(ctx);
-    void *ps[] = {cand, accepted, epos, tmp_cand, edges_, w_, anb, aw, big_T, big_sums, big_meta, ray, cstart, cend};
+    void *ps[] = {cand, accepted, epos, tmp_cand, edges_, w_, anb, aw, big_T, big_sums, big_meta, ray, cstart, cend,
+                  stats};
     for (void *p : ps) dfree(ctx, p);
+    stats = nullptr;
     cstart = cend = nullptr;
     cells_cap = cend_cap = 0;
     cand = accepted = epos = edges_ = anb = nullptr;
@@ -1247,12 +1264,15 @@ int32_t pl_light(lg_ctx *ctx, Pipeline &pl, int V, int dtype, const LightConsts 
         const long long rays = (long long)V * nt;
         if (rays > 0) {
             LG_TRY(grow(ctx, &pl.ray, &pl.ray_cap, (size_t)rays * 3));
+            hipEvent_t ev_r;
+            ktimer_begin(ctx, "k_light_rays", &ev_r);
             for_dtype(dtype, [&](auto tag) {
                 using T = decltype(tag);
-                k_light_rays<T><<<(unsigned)((rays + 3) / 4), 256, 0, ctx->stream>>>(pl.sd, pl.sid, V, nt, lc, lights,
-                                                                                   ek0, ek1, pts, pl.ray);
+                k_light_rays<T><<<(unsigned)((rays + 3) / 4), 256, 0, ctx->stream>>>(
+                    pl.sd, pl.sid, V, nt, lc, lights, ek0, ek1, pts, pl.ray, pl.stats_on ? pl.stats : nullptr);
             });
             LG_LAUNCHED("k_light_rays");
+            ktimer_end(ctx, "k_light_rays", ev_r);
         }
         k_light_combine<<<(unsigned)((V + 255) / 256), 256, 0, ctx->stream>>>(pl.sd, pl.sid, V, nt, lc, lights, pts,
                                                                             pl.ray, L);
@@ -1352,8 +1372,8 @@ int32_t pl_march(lg_ctx *ctx, Pipeline &pl, int dtype, const MarchConsts &mc, co
     if (Ktot > 0) {
         for_dtype(dtype, [&](auto tag) {
             using T = decltype(tag);
-            k_march<T><<<(unsigned)((Ktot + 127) / 128), 128, 0, ctx->stream>>>(pl.sd, pl.sid, mc, pts, pl.edges_,
-                                                                               Ktot, pl.epos + Ktot, pl.w_);
+            k_march<T><<<(unsigned)((Ktot + 127) / 128), 128, 0, ctx->stream>>>(
+                pl.sd, pl.sid, mc, pts, pl.edges_, Ktot, pl.epos + Ktot, pl.w_, pl.stats_on ? pl.stats : nullptr);
         });
         LG_LAUNCHED("k_march");
     }
@@ -1435,17 +1455,20 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     ktimer_begin(ctx, "k_comp_small", &ev);
     if (n128 > 0) {
         k_comp_small<kSmallComp><<<n128, 1024, 0, st>>>(pl.sd, pl.sid, pl.cls128, pl.comp_off, pl.member, pl.local,
-                                                        pl.aoff, pl.anb, pl.aw, L, e0, e1, e2, out);
+                                                        pl.aoff, pl.anb, pl.aw, L, e0, e1, e2, out,
+            pl.stats_on ? pl.stats : nullptr);
         LG_LAUNCHED("k_comp_small<128>");
     }
     if (n48 > 0) {
         k_comp_small<64><<<n48, 256, 0, st>>>(pl.sd, pl.sid, pl.cls48, pl.comp_off, pl.member, pl.local, pl.aoff,
-                                               pl.anb, pl.aw, L, e0, e1, e2, out);
+                                               pl.anb, pl.aw, L, e0, e1, e2, out,
+            pl.stats_on ? pl.stats : nullptr);
         LG_LAUNCHED("k_comp_small<64>");
     }
     if (n16 > 0) {
         k_comp_small<32><<<n16, 64, 0, st>>>(pl.sd, pl.sid, pl.cls16, pl.comp_off, pl.member, pl.local, pl.aoff,
-                                              pl.anb, pl.aw, L, e0, e1, e2, out);
+                                              pl.anb, pl.aw, L, e0, e1, e2, out,
+            pl.stats_on ? pl.stats : nullptr);
         LG_LAUNCHED("k_comp_small<32>");
     }
     ktimer_end(ctx, "k_comp_small", ev);
@@ -1499,7 +1522,7 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     LG_LAUNCHED("k_big_fill");
     ktimer_end(ctx, "k_big_fill", ev);
     for (int b = 0; b < nbig; ++b) mats[b] = pl.big_T + jobs[b].toff;
-    LG_TRY(fw_solve_batched_impl(ctx, mats, ms));
+    LG_TRY(fw_solve_batched_impl(ctx, mats, ms, pl.stats_on ? pl.stats : nullptr));
     ktimer_begin(ctx, "k_diffuse", &ev);
     k_big_rowsum<<<mrows, 256, 0, st>>>(d_jobs, d_mp, nbig, pl.big_T, e0, e1, e2, pl.big_sums);
     LG_LAUNCHED("k_big_rowsum");

@@ -381,6 +381,28 @@ int32_t lg_memcpy_d2h(lg_ctx *ctx, void *h_dst, const void *d_src, size_t bytes)
     return LG_OK;
 }
 
+// Plain streaming copy, 16 B per lane per step: the reference point that HBM
+// traffic counters are scaled against (its bytes are known exactly).
+__global__ void __launch_bounds__(256) k_copy(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n16) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+int32_t lg_copy_d(lg_ctx *ctx, void *d_dst, const void *d_src, size_t bytes) {
+    LG_CHECK_ARG(ctx && d_dst && d_src && bytes % 16 == 0, "lg_copy_d: NULL pointer or size not a multiple of 16");
+    LG_CHECK_ARG(((uintptr_t)d_dst | (uintptr_t)d_src) % 16 == 0, "lg_copy_d: pointers must be 16-byte aligned");
+    if (bytes == 0) return LG_OK;
+    const size_t n16 = bytes / 16;
+    const unsigned blocks =
+        (unsigned)std::max<size_t>(1, std::min<size_t>((n16 + 255) / 256, (size_t)std::max(ctx->cu_count, 1) * 16));
+    hipEvent_t ev;
+    ktimer_begin(ctx, "k_copy", &ev);
+    k_copy<<<blocks, 256, 0, ctx->stream>>>((const uint4 *)d_src, (uint4 *)d_dst, n16);
+    LG_LAUNCHED("k_copy");
+    ktimer_end(ctx, "k_copy", ev);
+    return LG_OK;
+}
+
 int32_t lg_memset_d(lg_ctx *ctx, void *d_dst, int32_t value, size_t bytes) {
     LG_CHECK_ARG(ctx && d_dst, "lg_memset_d: NULL argument");
     LG_HIP(hipMemsetAsync(d_dst, value, bytes, ctx->stream));

@@ -319,6 +319,11 @@ static int32_t solve_descs(lg_ctx *ctx, lg_solver *s, const std::vector<SolveDes
     LG_TRY(lgl::make_dev_lights(lights, n_lights, &pp.lights));
     const bool t = sp->timing != 0;
     const int dtype = descs[0].dtype;
+    pl.stats_on = (sp->flags & LG_SOLVE_STATS) != 0;
+    if (pl.stats_on) {
+        if (!pl.stats) LG_TRY(dalloc(ctx, (size_t)ST_COUNT * kStatSlots, &pl.stats));
+        LG_HIP(hipMemsetAsync(pl.stats, 0, sizeof(unsigned long long) * ST_COUNT * kStatSlots, ctx->stream));
+    }
     int V = 0, nmax = 0, Ktot = 0;
     if (t) LG_HIP(hipEventRecord(s->ev[0], ctx->stream));
     LG_TRY(pl_begin(ctx, pl, descs, &V, &nmax));
@@ -338,6 +343,8 @@ static int32_t solve_descs(lg_ctx *ctx, lg_solver *s, const std::vector<SolveDes
     s->buf.ld = descs.size() == 1 ? descs[0].ld : 0;
     s->buf.n_edges = res.n_edges_total;
     s->buf.n_components = res.n_components;
+    s->n_big = res.n_big;
+    s->stats_valid = pl.stats_on;
     s->buf.d_edges = res.edges;
     s->buf.d_weights = res.w;
     return LG_OK;
@@ -491,6 +498,30 @@ int32_t lg_solver_create(lg_ctx *ctx, int32_t n_max, int32_t edge_cap, int32_t f
         return rc;
     }
     *out = s;
+    return LG_OK;
+}
+
+int32_t lg_solver_stats(lg_ctx *ctx, const lg_solver *s, lg_solve_stats *out) {
+    LG_CHECK_ARG(ctx && s && out, "lg_solver_stats: NULL argument");
+    std::memset(out, 0, sizeof(*out));
+    const Pipeline &pl = *static_cast<const Pipeline *>(s->pipeline);
+    out->points = (uint64_t)s->buf.n;
+    out->edges = (uint64_t)s->buf.n_edges;
+    out->components = (uint64_t)s->buf.n_components;
+    out->big_components = (uint64_t)s->n_big;
+    if (!s->stats_valid || !pl.stats) return LG_OK;
+    unsigned long long slots[ST_COUNT * kStatSlots], h[ST_COUNT] = {};
+    LG_HIP(hipMemcpyAsync(slots, pl.stats, sizeof(slots), hipMemcpyDeviceToHost, ctx->stream));
+    LG_HIP(hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < ST_COUNT; ++c)
+        for (int k = 0; k < kStatSlots; ++k) h[c] += slots[c * kStatSlots + k];
+    out->edge_samples = h[ST_EDGE_SAMPLES];
+    out->shadow_rays = h[ST_SHADOW_RAYS];
+    out->shadow_samples = h[ST_SHADOW_SAMPLES];
+    out->fw_tiles = h[ST_FW_TILES];
+    out->fw_panel_tiles = h[ST_FW_PANEL_TILES];
+    out->fw_pivots = h[ST_FW_PIVOTS];
+    out->small_updates = h[ST_SMALL_UPDATES];
     return LG_OK;
 }
 

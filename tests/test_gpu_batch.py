@@ -130,3 +130,33 @@ def test_solve_batch_rejects_mixed_dtypes(lga, ctx, volumes):
                       lga.march_params(), radius=4.0)
     s.destroy()
     g16.destroy()
+
+
+def test_solve_stats_counts(lga, ctx, volumes):
+    """Device work counters: rays = vertices x light samples, LDS closure work =
+    sum of m^3 over the components of 2..128 vertices, tiles only for larger ones."""
+    sh = lga.default_shader(absorption=0.1, scattering=(1.0, 0.9, 0.8))
+    p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5)
+    lights = [lga.point_light(0.0, (1.0, 0.9, 0.8), 4.0e4, (64.0, 200.0, 64.0)),
+              lga.parse_light("dir:0:1,0.89,0.79:0.7:-41.8,-180,0")]
+    n = 1400
+    s = lga.Solver(ctx, n)
+    s.solve(volumes[1], sh, lights, p, n=n, radius=6.0, seed=854123, solve=3, stats=True)
+    st = s.stats()
+    r = s.results(want_D=False)
+    fin = np.isfinite(r["weights"])
+    e = r["edges"][fin]
+    A = csr_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(n, n))
+    nc, lab = connected_components(A, directed=False)
+    sizes = np.bincount(lab)
+    assert st["points"] == n and st["edges"] == len(r["edges"]) and st["components"] == nc
+    assert st["shadow_rays"] == 2 * n and st["shadow_samples"] > st["shadow_rays"]
+    assert st["edge_samples"] > 0
+    small = sizes[(sizes >= 2) & (sizes <= 128)].astype(np.int64)
+    assert st["small_updates"] == int((small ** 3).sum())
+    assert st["big_components"] == int((sizes > 128).sum())
+    assert (st["fw_tiles"] + st["fw_panel_tiles"] > 0) == (st["big_components"] > 0)
+    # without the flag the counters are not collected
+    s.solve(volumes[1], sh, lights, p, n=n, radius=6.0, seed=854123, solve=3)
+    assert s.stats()["shadow_samples"] == 0
+    s.destroy()
