@@ -136,6 +136,78 @@ def bench_batched(lga, ctx, shader, params, lights, count, reps):
             "kernels": {k: {"launches": v["launches"], "avg_ms": v["avg_ms"]} for k, v in btimes.items()}}
 
 
+def bench_render(lga, ctx, grid, shader, lights, solver, n, reps):
+    """The C2 image: 1920 x 1080, 4 AA samples, the scatter grid splatted from one
+    solve's radiance (3 points per kernel, factor-4 coarse grid), primary step 1,
+    shadow step 3 (SceneSettings defaults)."""
+    solver.solve(grid, shader, lights, lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5),
+                 n=n, radius=12.0, seed=854123, solve=777)
+    r = solver.results(want_D=False)
+    sg = lga.ScatterGrid(ctx, grid, 4)
+    radius, _ = lga.splat_radius(r["points"], 3)
+    pb, rb = ctx.upload(r["points"]), ctx.upload(r["diffuse"])
+    ctx.sync()
+    ctx.mark(5)
+    sg.splat(pb, rb, len(r["points"]), radius)
+    ctx.mark(6)
+    ctx.sync()
+    splat_ms = ctx.elapsed_ms(5, 6)
+    w, h, spp = 1920, 1080, 4
+    cam = lga.camera_setup(cam_translate=(256.0, 256.0, 1400.0), focal=50.0, width=w, height=h)
+    films = lga.Films(ctx, w, h)
+    rp = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=3.0)
+    lga.render(ctx, grid, sg, None, cam, shader, lights, rp, spp, films, seed=1)  # warm-up
+    ctx.sync()
+    ctx.mark(5)
+    for _ in range(reps):
+        lga.render(ctx, grid, sg, None, cam, shader, lights, rp, spp, films, seed=1)
+    ctx.mark(6)
+    ctx.sync()
+    ms = ctx.elapsed_ms(5, 6) / reps
+    img = films.read()
+    alpha_mean = float(img[0][..., 3].mean())
+    films.free()
+    sg.destroy()
+    pb.free()
+    rb.free()
+    return {"workload": "c2_render", "width": w, "height": h, "spp": spp, "ms": round(ms, 3),
+            "camera_rays_per_s": round(w * h * spp / (ms * 1e-3), 1), "splat_ms": round(splat_ms, 4),
+            "primary_step": 1.0, "shadow_step": 3.0, "reps": reps, "mean_alpha": round(alpha_mean, 4)}
+
+
+def bench_fw_dense(lga, ctx, solver, n):
+    """Kernel (c) at the configured size: blocked Floyd-Warshall on the dense
+    n x n optical-depth matrix of one C2 graph (edges of the last solve), the
+    whole matrix rather than its components.  Algorithmic ops: 2 n^3."""
+    import numpy as np
+    r = solver.results(want_D=False)
+    ld = lga.fw_ld(n)
+    W = np.full((ld, ld), np.inf, np.float32)
+    np.fill_diagonal(W, 0.0)
+    e, w = r["edges"], r["weights"]
+    W[e[:, 0], e[:, 1]] = w
+    W[e[:, 1], e[:, 0]] = w
+    buf = ctx.upload(W)
+    del W
+    ctx.kernel_time_reset()
+    ctx.kernel_timing(True)
+    ctx.sync()
+    ctx.mark(5)
+    lga.fw_solve(ctx, n, buf, ld)
+    ctx.mark(6)
+    ctx.sync()
+    ctx.kernel_timing(False)
+    ms = ctx.elapsed_ms(5, 6)
+    t = kernel_times(ctx, ("k_fw_minplus", "k_fw_panel", "k_fw_pivot"))
+    buf.free()
+    alg = 2.0 * float(n) ** 3
+    return {"workload": "c2_fw_dense", "n": n, "ms": round(ms, 3),
+            "algorithmic_tflops": round(alg / (ms * 1e-3) / 1e12, 2),
+            "frac_of_fp32_peak": round(alg / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
+            "note": "2 n^3 ops as the reference's triple loop; the symmetric blocked kernel computes half the tiles",
+            "kernels": {k: {"launches": v["launches"], "avg_ms": v["avg_ms"]} for k, v in t.items()}}
+
+
 def kernel_times(ctx, names):
     out = {}
     for nm in names:
@@ -215,6 +287,8 @@ def main():
     ap.add_argument("--batch-volumes", type=int, default=64)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--latency-solves", type=int, default=50)
+    ap.add_argument("--render-reps", type=int, default=3, help="0: skip the 1080p render timing")
+    ap.add_argument("--no-fw-dense", action="store_true", help="skip the dense n x n Floyd-Warshall timing")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device three times with k_copy "
                          "(the known-size kernel that traffic counters are scaled against)")
@@ -273,6 +347,10 @@ def main():
             "kernels": {k: {"launches": v["launches"], "avg_ms": v["avg_ms"]} for k, v in times.items()},
             "work_per_solve": {k: round(v / args.steps, 1) for k, v in work.items()},
             "latency": {"p50_ms": pct(0.50), "p99_ms": pct(0.99), "solves": len(lat), "mode": "one solve alone"}}
+    if not args.no_fw_dense:
+        line["fw_dense"] = bench_fw_dense(lga, ctx, solver, n)
+    if args.render_reps > 0:
+        line["render"] = bench_render(lga, ctx, grid, shader, lights, solver, n, args.render_reps)
     solver.destroy()
     if args.batch_volumes > 0:
         line["batched"] = bench_batched(lga, ctx, shader, params, lights, args.batch_volumes, args.batch_reps)
