@@ -1,0 +1,91 @@
+"""The command-line front end on the device: a procedural scene through
+LightGraph iterations, splat and render equals, film for film and bit for bit,
+the same stages called through the Python mirror of the C ABI; EXR output with
+and without compression holds the same pixels; env lights load from EXR."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import exrio
+
+pytestmark = pytest.mark.gpu
+
+CLI = os.path.join(os.path.dirname(__file__), "..", "lightgraph_amd", "lightgraph")
+AOVS = [("", ("R", "G", "B", "A")), ("direct.", ("R", "G", "B")), ("scatter.", ("R", "G", "B")),
+        ("emission.", ("R", "G", "B"))]
+
+
+def _cli(*args):
+    r = subprocess.run([CLI, *args], capture_output=True, text=True, timeout=180)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def _api_films(lga, ctx, size, seed, n, iters, radius, w, h, spp, lights, sh, p, cam_pos, focal):
+    spec = lga.scene_spec(lga.LG_SCENE_SPHERE_FBM, size, seed=seed, octaves=4, radius=0.375 * size)
+    grid, _ = lga.Grid.generate(ctx, spec)
+    s = lga.Solver(ctx, n * iters)
+    s.solve_batch([(grid, n, i) for i in range(iters)], sh, lights, p, radius=radius)
+    res = s.batch_results()
+    pts = np.concatenate([r["points"] for r in res])
+    dif = np.concatenate([r["diffuse"] for r in res])
+    sg = lga.ScatterGrid(ctx, grid, 4)
+    rad, _ = lga.splat_radius(pts, 3 * iters)
+    pb, rb = ctx.upload(pts), ctx.upload(dif)
+    sg.splat(pb, rb, len(pts), rad)
+    cam = lga.camera_setup(cam_translate=cam_pos, focal=focal, width=w, height=h)
+    films = lga.Films(ctx, w, h)
+    lga.render(ctx, grid, sg, None, cam, sh, lights, p, spp, films, seed=1)
+    out = films.read()
+    films.free()
+    for b in (pb, rb):
+        b.free()
+    sg.destroy()
+    s.destroy()
+    grid.destroy()
+    return out
+
+
+def test_cli_equals_api(lga, ctx, tmp_path):
+    common = ["-scene", "sphere", "-sceneSize", "64", "-sceneSeed", "3", "-points", "400", "-iter", "3",
+              "-radius", "6", "-resWidth", "64", "-resHeight", "48", "-samples", "2", "-camPos", "31.5,31.5,150",
+              "-focal", "60", "-light", "dir:0:1,1,1:1:-45,30,0", "-light", "point:0:1,0.8,0.6:2000:20,70,30",
+              "-scattering", "1", "-absorption", "0.1", "-shadowStep", "2"]
+    a, b = str(tmp_path / "none.exr"), str(tmp_path / "zip.exr")
+    out = _cli(*common, "-compression", "none", "-exrOut", a)
+    for stage in ("LightGraphs Calculation:", "Rasterize LightGraphs to Grid:", "Render:"):
+        assert stage in out
+    _cli(*common, "-exrOut", b)  # default zip
+    ca, ha = exrio.read(a)
+    cb, hb = exrio.read(b)
+    assert ha["compression"] == 0 and hb["compression"] == 3
+    assert len(ha["channels"]) == 13
+    for k in ca:
+        np.testing.assert_array_equal(ca[k], cb[k])
+    sh = lga.default_shader(absorption=0.1, scattering=1.0)
+    p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=2.0)
+    lights = [lga.parse_light("dir:0:1,1,1:1:-45,30,0"), lga.parse_light("point:0:1,0.8,0.6:2000:20,70,30")]
+    want = _api_films(lga, ctx, 64, 3, 400, 3, 6.0, 64, 48, 2, lights, sh, p, (31.5, 31.5, 150.0), 60.0)
+    assert want[0][..., 3].max() > 0.3
+    for f, (prefix, comps) in enumerate(AOVS):
+        for c, comp in enumerate(comps):
+            np.testing.assert_array_equal(ca[prefix + comp], want[f][..., c], err_msg=prefix + comp)
+
+
+def test_cli_env_light_from_exr(lga, ctx, tmp_path):
+    h, w = 16, 32
+    y = np.linspace(0, 1, h, dtype=np.float32)[:, None, None]
+    sky = np.broadcast_to(np.concatenate([0.3 + 0.5 * y, 0.4 + 0.4 * y, 0.9 - 0.3 * y], -1), (h, w, 3)).copy()
+    sky[3, 20] = 400.0
+    env_path = str(tmp_path / "sky.exr")
+    exrio.write_rgb(env_path, sky, compression=3)
+    back, _ = exrio.read(env_path)
+    np.testing.assert_array_equal(back["R"], sky[..., 0])
+    out = str(tmp_path / "env.exr")
+    _cli("-scene", "sphere", "-sceneSize", "48", "-points", "300", "-iter", "2", "-radius", "5", "-resWidth", "40",
+         "-resHeight", "30", "-camPos", "23.5,23.5,120", "-focal", "60", "-light", f"env:0:1,1,1:0.5:1,{env_path}",
+         "-compression", "none", "-exrOut", out)
+    c, _ = exrio.read(out)
+    assert np.all(np.isfinite(c["direct.R"])) and c["direct.R"].max() > 0
