@@ -368,6 +368,59 @@ int32_t lg_solve_batch(lg_ctx *ctx, lg_solver *solver, const lg_batch_item *item
                        const lg_shader *shader, const lg_light *lights, int32_t n_lights,
                        const lg_march_params *params, const lg_solve_params *sp);
 
+/* ------------------------------------------------------------ serving ---- */
+/* A render job: volume -> `iterations` LightGraph solves of `points` points ->
+ * splat -> render (main.cc:505-128 as one request).  Iteration i uses solve
+ * index solve_base + i. */
+typedef struct lg_job {
+    lg_scene_spec scene;          /* procedural volume (kept resident between jobs) */
+    int32_t points, iterations;
+    double radius;                /* edge radius, world units                       */
+    int32_t max_connect;
+    int32_t samples;              /* AA samples per pixel                           */
+    uint64_t seed, env_seed, render_seed;
+    uint32_t solve_base;
+    int32_t pts_per_kernel;       /* SceneSettings::ptsPerKernel                    */
+    int32_t scatter_res_factor;   /* SceneSettings::scatterGridResFactor            */
+    int32_t n_lights;
+    lg_shader shader;
+    lg_march_params march;
+    lg_camera camera;
+    lg_light lights[LG_MAX_LIGHTS];
+} lg_job;
+
+typedef struct lg_job_result {
+    int32_t status;               /* LG_OK or the failing call's code               */
+    int32_t batch;                /* jobs that shared this job's solve launch       */
+    double t_submit, t_admit, t_start, t_done;   /* seconds on the server clock     */
+    uint64_t reserved_bytes;      /* device memory the job was admitted with        */
+} lg_job_result;
+
+typedef struct lg_server_stats {
+    uint64_t queued, completed, failed, batches, batched_jobs, solves;
+    uint64_t reserved_bytes, peak_reserved_bytes, budget_bytes;
+    double clock_s;
+} lg_server_stats;
+
+typedef struct lg_server lg_server;
+
+/* Request loop with continuous batching (no reference counterpart: the reference
+ * renders one scene per process).  `workers` threads, each with its own HIP
+ * stream, take the oldest admissible job plus queued jobs that can share its
+ * solve launch (same brick type, shader, lights, march and graph parameters), up
+ * to max_batch, while the estimated device memory fits hbm_budget; resident
+ * volumes are evicted least-recently-used when the budget needs room. */
+int32_t lg_server_create(int32_t device, uint64_t hbm_budget, int32_t workers, int32_t max_batch,
+                         lg_server **out);
+/* Queues a job; the four films (width*height RGBA float32 each, NULL to skip) are
+ * written when it completes.  LG_ERR_BUDGET: the job alone exceeds the budget. */
+int32_t lg_server_submit(lg_server *srv, const lg_job *job, float *const h_films[4], uint64_t *id_out);
+int32_t lg_server_wait(lg_server *srv, uint64_t id, lg_job_result *out);
+int32_t lg_server_poll(lg_server *srv, uint64_t id, lg_job_result *out, int32_t *done);
+int32_t lg_server_get_stats(lg_server *srv, lg_server_stats *out);
+/* Finishes every submitted job, then stops the workers and frees the device memory. */
+int32_t lg_server_destroy(lg_server *srv);
+
 /* ----------------------------------------------------------- splat/render ---- */
 /* Splat radius of ScatterGrid::operator() (ScatterGrid.h:71-73) with the voxel size
  * of LightGraph::buildVdbPointGrid (LightGraph.h:713-714) restated on the host. */

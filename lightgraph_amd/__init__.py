@@ -509,3 +509,56 @@ def render(ctx, density, scatter, temperature, camera, shader, lights, params, s
     _check(lib().lg_render(ctx.handle, density.handle, C.byref(scatter.sg) if scatter else None,
                            temperature.handle if temperature else None, C.byref(camera), C.byref(shader), arr,
                            len(lights), C.byref(params), samples, seed, env_seed, C.byref(films.s)), "render")
+
+
+# ------------------------------------------------------------- serving -------
+def job(scene, points, camera, shader, lights, march, iterations=1, radius=12.0, max_connect=12, samples=1,
+        seed=854123, env_seed=4564446, render_seed=1, solve_base=0, pts_per_kernel=3, scatter_res_factor=4):
+    """An lg_job: volume -> `iterations` solves -> splat -> render."""
+    j = _abi.lg_job()
+    j.scene, j.points, j.iterations, j.radius, j.max_connect = scene, points, iterations, radius, max_connect
+    j.samples, j.seed, j.env_seed, j.render_seed, j.solve_base = samples, seed, env_seed, render_seed, solve_base
+    j.pts_per_kernel, j.scatter_res_factor = pts_per_kernel, scatter_res_factor
+    j.shader, j.march, j.camera = shader, march, camera
+    if len(lights) > len(j.lights):
+        raise LGError(-1, f"at most {len(j.lights)} lights")
+    j.n_lights = len(lights)
+    for k, l in enumerate(lights):
+        j.lights[k] = l
+    return j
+
+
+class Server:
+    """lg_server: continuous-batching request loop with HBM-budget admission."""
+
+    def __init__(self, budget_bytes, workers=2, max_batch=8, device=0):
+        h = C.c_void_p()
+        _check(lib().lg_server_create(device, int(budget_bytes), workers, max_batch, C.byref(h)), "server_create")
+        self.handle = h.value
+        self._films = {}
+
+    def submit(self, j):
+        w, h = j.camera.width, j.camera.height
+        films = np.zeros((4, h, w, 4), np.float32)
+        ptrs = (C.c_void_p * 4)(*[films[k].ctypes.data for k in range(4)])
+        jid = C.c_uint64()
+        _check(lib().lg_server_submit(self.handle, C.byref(j), ptrs, C.byref(jid)), "server_submit")
+        self._films[jid.value] = films
+        return jid.value
+
+    def wait(self, jid):
+        r = _abi.lg_job_result()
+        _check(lib().lg_server_wait(self.handle, jid, C.byref(r)), "server_wait")
+        res = {k: getattr(r, k) for k, _ in r._fields_}
+        res["films"] = self._films.pop(jid)
+        return res
+
+    def stats(self):
+        st = _abi.lg_server_stats()
+        _check(lib().lg_server_get_stats(self.handle, C.byref(st)), "server_stats")
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
+    def close(self):
+        if self.handle:
+            lib().lg_server_destroy(self.handle)
+            self.handle = None

@@ -97,6 +97,28 @@ class lg_solve_stats(C.Structure):
                                           "fw_tiles", "fw_panel_tiles", "fw_pivots", "small_updates", "components", "big_components")]
 
 
+LG_MAX_LIGHTS_ = 8
+
+
+class lg_job(C.Structure):
+    _fields_ = [("scene", lg_scene_spec), ("points", C.c_int32), ("iterations", C.c_int32), ("radius", C.c_double),
+                ("max_connect", C.c_int32), ("samples", C.c_int32), ("seed", C.c_uint64), ("env_seed", C.c_uint64),
+                ("render_seed", C.c_uint64), ("solve_base", C.c_uint32), ("pts_per_kernel", C.c_int32),
+                ("scatter_res_factor", C.c_int32), ("n_lights", C.c_int32), ("shader", lg_shader),
+                ("march", lg_march_params), ("camera", lg_camera), ("lights", lg_light * LG_MAX_LIGHTS_)]
+
+
+class lg_job_result(C.Structure):
+    _fields_ = [("status", C.c_int32), ("batch", C.c_int32), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_start", C.c_double), ("t_done", C.c_double), ("reserved_bytes", C.c_uint64)]
+
+
+class lg_server_stats(C.Structure):
+    _fields_ = [(k, C.c_uint64) for k in ("queued", "completed", "failed", "batches", "batched_jobs", "solves",
+                                          "reserved_bytes", "peak_reserved_bytes", "budget_bytes")] + \
+               [("clock_s", C.c_double)]
+
+
 class lg_batch_item(C.Structure):
     _fields_ = [("density", C.c_void_p), ("temperature", C.c_void_p), ("n", C.c_int32), ("solve", C.c_uint32)]
 
@@ -154,6 +176,12 @@ SIGNATURES = {
                        C.POINTER(lg_march_params), C.POINTER(lg_solve_params)]),
     "lg_fw_solve_batched": (I32, [P, I32, C.POINTER(I32), C.POINTER(C.c_void_p), C.POINTER(I32)]),
     "lg_solver_stats": (I32, [P, P, C.POINTER(lg_solve_stats)]),
+    "lg_server_create": (I32, [I32, U64, I32, I32, PP]),
+    "lg_server_submit": (I32, [P, C.POINTER(lg_job), C.POINTER(C.c_void_p), C.POINTER(U64)]),
+    "lg_server_wait": (I32, [P, U64, C.POINTER(lg_job_result)]),
+    "lg_server_poll": (I32, [P, U64, C.POINTER(lg_job_result), C.POINTER(I32)]),
+    "lg_server_get_stats": (I32, [P, C.POINTER(lg_server_stats)]),
+    "lg_server_destroy": (I32, [P]),
     "lg_solve_batch": (I32, [P, P, C.POINTER(lg_batch_item), I32, C.POINTER(lg_shader), C.POINTER(lg_light), I32,
                              C.POINTER(lg_march_params), C.POINTER(lg_solve_params)]),
     "lg_splat_radius": (I32, [P, I64, I64, DP, DP]),
