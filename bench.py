@@ -208,6 +208,75 @@ def bench_fw_dense(lga, ctx, solver, n):
             "kernels": {k: {"launches": v["launches"], "avg_ms": v["avg_ms"]} for k, v in t.items()}}
 
 
+def bench_serving(lga, ctx, rates, jobs_per_rate, workers, max_batch, seed=5):
+    """Serving workload (scaled-down C5): Poisson arrivals at each rate of render
+    jobs drawn from 12 seeded blob volumes of side 128..768 (step 64), N in
+    [1000, 16000], images 512^2 / 1280x720 / 1920x1080 at 1-4 spp, one of three
+    light sets (directional, point, environment); one LightGraph solve per job.
+    Latency = arrival -> films on the host.  The same job list is also run
+    through a one-worker, one-job-per-batch server (the plain path, back to back)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sides = [int(x) for x in rng.integers(0, (768 - 128) // 64 + 1, 12) * 64 + 128]
+    pool = [lga.scene_spec(lga.LG_SCENE_BLOBS_FBM, sides[k], seed=7000 + k, octaves=6, blobs=20,
+                           axis_min=sides[k] * 30 / 512, axis_max=sides[k] * 120 / 512, active_fraction=0.30)
+            for k in range(12)]
+    h, w = 128, 256
+    y = np.linspace(0, 1, h, dtype=np.float32)[:, None, None]
+    sky = np.broadcast_to(np.concatenate([0.2 + 0.6 * y, 0.3 + 0.5 * y, 0.9 - 0.2 * y], -1), (h, w, 3)).copy()
+    sky[20:23, 150:153] = 5000.0
+    env = lga.EnvMap(ctx, sky)
+    sh = lga.default_shader(absorption=0.1, scattering=(1.0, 0.9, 0.8))
+    mp = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=3.0)
+    light_sets = [[lga.parse_light("dir:0:1,0.89,0.79:0.7:-41.8,-180,0")],
+                  [lga.point_light(0, (1, 0.9, 0.8), 4e5, (256, 900, 256))],
+                  [lga.env_light(env, 0.0, (1, 1, 1), 1.0, samples=1)]]
+    images = [(512, 512), (1280, 720), (1920, 1080)]
+
+    def make_jobs(n):
+        out = []
+        for k in range(n):
+            v = pool[int(rng.integers(12))]
+            iw, ih = images[int(rng.integers(3))]
+            cam = lga.camera_setup(cam_translate=(v.n / 2, v.n / 2, 2.8 * v.n), focal=50.0, width=iw, height=ih)
+            out.append(lga.job(v, int(rng.integers(1000, 16001)), cam, sh, light_sets[int(rng.integers(3))], mp,
+                               radius=12.0, samples=int(rng.integers(1, 5)), solve_base=k))
+        return out
+
+    def run(jobs, arrivals, nworkers, nbatch):
+        srv = lga.Server(160 << 30, workers=nworkers, max_batch=nbatch)
+        t0 = time.perf_counter()
+        ids = []
+        for j, a in zip(jobs, arrivals):
+            dt = a - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(j))
+        rs = [srv.wait(i) for i in ids]
+        st = srv.stats()
+        srv.close()
+        lat = sorted(r["t_done"] - r["t_submit"] for r in rs)
+        span = max(r["t_done"] for r in rs) - min(r["t_submit"] for r in rs)
+        pct = lambda q: round(1e3 * lat[min(len(lat) - 1, int(q * len(lat)))], 2)
+        return {"jobs": len(rs), "failed": sum(r["status"] != 0 for r in rs), "solves_per_s": round(len(rs) / span, 3),
+                "p50_ms": pct(0.5), "p99_ms": pct(0.99), "mean_batch": round(float(np.mean([r["batch"] for r in rs])), 2),
+                "peak_reserved_gb": round(st["peak_reserved_bytes"] / 2 ** 30, 2)}
+
+    out = {"workload": "c5_serving", "volumes": {"count": 12, "sides": sides}, "n_range": [1000, 16000],
+           "images": ["512x512", "1280x720", "1920x1080"], "spp": [1, 4], "workers": workers, "max_batch": max_batch,
+           "rates": {}}
+    for lam in rates:
+        jobs = make_jobs(jobs_per_rate)
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, jobs_per_rate))
+        out["rates"][str(lam)] = run(jobs, arrivals, workers, max_batch)
+    jobs = make_jobs(jobs_per_rate)
+    out["saturated"] = run(jobs, np.zeros(len(jobs)), workers, max_batch)
+    out["plain"] = run(jobs, np.zeros(len(jobs)), 1, 1)
+    out["served_vs_plain"] = round(out["saturated"]["solves_per_s"] / out["plain"]["solves_per_s"], 3)
+    env.destroy()
+    return out
+
+
 def kernel_times(ctx, names):
     out = {}
     for nm in names:
@@ -288,6 +357,10 @@ def main():
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--latency-solves", type=int, default=50)
     ap.add_argument("--render-reps", type=int, default=3, help="0: skip the 1080p render timing")
+    ap.add_argument("--serve-jobs", type=int, default=40, help="jobs per arrival rate; 0: skip serving")
+    ap.add_argument("--serve-rates", default="2,10,50", help="Poisson arrival rates, jobs/s")
+    ap.add_argument("--serve-workers", type=int, default=3)
+    ap.add_argument("--serve-batch", type=int, default=8)
     ap.add_argument("--no-fw-dense", action="store_true", help="skip the dense n x n Floyd-Warshall timing")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device three times with k_copy "
@@ -354,6 +427,9 @@ def main():
     solver.destroy()
     if args.batch_volumes > 0:
         line["batched"] = bench_batched(lga, ctx, shader, params, lights, args.batch_volumes, args.batch_reps)
+    if args.serve_jobs > 0:
+        rates = [float(x) for x in args.serve_rates.split(",") if x]
+        line["serving"] = bench_serving(lga, ctx, rates, args.serve_jobs, args.serve_workers, args.serve_batch)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(lga, ctx, grid, shader, params, lights, n, radius, args.cpu_budget)
     else:
