@@ -277,6 +277,61 @@ def bench_serving(lga, ctx, rates, jobs_per_rate, workers, max_batch, seed=5):
     return out
 
 
+def bench_precision(lga, ctx, reps=3):
+    """Precision workload (C4): 1024^3 plume with temperature, bricks stored as
+    FP32 / FP16 / BF16; solve (N = 12288, r = 12, env map 1024 x 512 with a 5000x
+    sun + directional light) and a 960 x 540 render per brick type.  Errors
+    against the FP64 oracle for the same workload are measured by
+    tests/test_gpu_precision.py."""
+    import numpy as np
+    h, w = 512, 1024
+    y = np.linspace(0, 1, h, dtype=np.float32)[:, None, None]
+    sky = np.broadcast_to(np.concatenate([0.25 + 0.5 * y, 0.35 + 0.45 * y, 0.8 - 0.2 * y], -1), (h, w, 3)).copy()
+    yy, xx = np.mgrid[0:h, 0:w]
+    sky[(xx - 0.7 * w) ** 2 + (yy - 0.22 * h) ** 2 <= (0.012 * w) ** 2 + 1] *= 5000.0
+    env = lga.EnvMap(ctx, sky)
+    ramp = "0.0,0.33,0.66,1.0/0,0,0:1.0,0.076,0.0:1.0,0.322,0.0:1.0,0.45,0.15/0.35/1.5"
+    sh = lga.default_shader(absorption=0.1, scattering=1.0, emission_scale=2.0, ramp=ramp)
+    p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=1.0)
+    lights = [lga.env_light(env, 20.0, (1, 1, 1), 0.5, samples=2), lga.parse_light("dir:0:1,0.9,0.8:0.8:-40,30,0")]
+    out = {"workload": "c4_precision", "volume": "plume 1024^3 + temperature", "points": 12288, "radius": 12.0,
+           "image": "960x540x1", "types": {}}
+    temp = None
+    for name, dt in (("fp32", lga.LG_F32), ("fp16", lga.LG_F16), ("bf16", lga.LG_BF16)):
+        spec = lga.scene_spec(lga.LG_SCENE_PLUME, 1024, seed=4, octaves=6, with_temperature=1, dtype=dt)
+        grid, t = lga.Grid.generate(ctx, spec)
+        if temp is None:
+            temp = t
+        else:
+            t.destroy()
+        s = lga.Solver(ctx, 12288)
+        s.solve(grid, sh, lights, p, n=12288, radius=12.0, solve=0, temperature=temp)
+        ctx.sync()
+        ctx.mark(5)
+        for k in range(reps):
+            s.solve(grid, sh, lights, p, n=12288, radius=12.0, solve=1 + k, temperature=temp)
+        ctx.mark(6)
+        ctx.sync()
+        solve_ms = ctx.elapsed_ms(5, 6) / reps
+        s.destroy()
+        cam = lga.camera_setup(cam_translate=(512.0, 460.0, 2450.0), focal=50.0, width=960, height=540)
+        films = lga.Films(ctx, 960, 540)
+        lga.render(ctx, grid, None, temp, cam, sh, lights, p, 1, films, seed=1)
+        ctx.sync()
+        ctx.mark(5)
+        lga.render(ctx, grid, None, temp, cam, sh, lights, p, 1, films, seed=1)
+        ctx.mark(6)
+        ctx.sync()
+        render_ms = ctx.elapsed_ms(5, 6)
+        films.free()
+        out["types"][name] = {"grid_gb": round(grid.info().device_bytes / 2 ** 30, 3), "solve_ms": round(solve_ms, 3),
+                              "render_ms": round(render_ms, 3)}
+        grid.destroy()
+    temp.destroy()
+    env.destroy()
+    return out
+
+
 def kernel_times(ctx, names):
     out = {}
     for nm in names:
@@ -357,6 +412,7 @@ def main():
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--latency-solves", type=int, default=50)
     ap.add_argument("--render-reps", type=int, default=3, help="0: skip the 1080p render timing")
+    ap.add_argument("--no-precision", action="store_true", help="skip the FP32/FP16/BF16 brick timing")
     ap.add_argument("--serve-jobs", type=int, default=40, help="jobs per arrival rate; 0: skip serving")
     ap.add_argument("--serve-rates", default="2,10,50", help="Poisson arrival rates, jobs/s")
     ap.add_argument("--serve-workers", type=int, default=3)
@@ -427,6 +483,8 @@ def main():
     solver.destroy()
     if args.batch_volumes > 0:
         line["batched"] = bench_batched(lga, ctx, shader, params, lights, args.batch_volumes, args.batch_reps)
+    if not args.no_precision:
+        line["precision"] = bench_precision(lga, ctx)
     if args.serve_jobs > 0:
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         line["serving"] = bench_serving(lga, ctx, rates, args.serve_jobs, args.serve_workers, args.serve_batch)

@@ -37,6 +37,11 @@ BARS = {"fp32": {"tau": (0, 1e-6), "D": (0, 1e-5), "L": (0, 1e-5), "diffuse": (0
         "bf16": {"tau": (0, 8e-3), "D": (0, 8e-3), "L": (3, 2e-2), "diffuse": (3, 2e-2), "pixels": (3, 1e-1)}}
 
 
+def _say(msg):
+    if FULL:
+        print(msg, flush=True)
+
+
 def _rel(got, want, floor_frac=1e-6):
     """(max, 99.9th percentile, mean) relative error and the largest error
     relative to the reference's maximum."""
@@ -90,6 +95,7 @@ def test_precision_errors_vs_fp64_oracle(lga, ctx, oracle, setup):
     ref = None
     for name, dtype in (("fp32", lga.LG_F32), ("fp16", lga.LG_F16), ("bf16", lga.LG_BF16)):
         spec = lga.scene_spec(lga.LG_SCENE_PLUME, SIZE, seed=4, octaves=6, with_temperature=1, dtype=dtype)
+        _say(f"precision: {name} solve")
         grid, temp = lga.Grid.generate(ctx, spec)
         temp.destroy()
         s = lga.Solver(ctx, n, dense=True)
@@ -101,10 +107,13 @@ def test_precision_errors_vs_fp64_oracle(lga, ctx, oracle, setup):
         if ref is None:
             # FP64 oracle on the FP32 volume, same points and edges
             e = r["edges"]
+            _say(f"precision: oracle edges ({len(e)})")
             T_o, tau_o = oracle.march_edges(S["dense"], S["sh"], S["p"], S["pts"], e)
             tau_o = np.where(T_o[:, 0] > 0, tau_o, np.inf)
+            _say("precision: oracle vertex light")
             L_o = oracle.vertex_light(S["dense"], S["sh"], S["p"], S["lights"], S["pts"], solve=0,
                                       temp_dense=S["tdense"], envs=S["envs"])
+            _say("precision: oracle all-pairs")
             fin = np.isfinite(tau_o)
             A = csr_matrix((tau_o[fin], (e[fin, 0], e[fin, 1])), shape=(n, n))
             D_o = dijkstra(A, directed=False)
@@ -129,6 +138,7 @@ def test_precision_errors_vs_fp64_oracle(lga, ctx, oracle, setup):
         got = films.read()
         films.free()
         if "pix" not in ref:
+            _say("precision: oracle render")
             ref["pix"] = oracle.render(S["dense"], cam, S["sh"], S["lights"], S["p"], 1, seed=1,
                                        temp_dense=S["tdense"], envs=S["envs"], threads=16)
         errs = [_rel(got[k][..., :3], ref["pix"][k][..., :3]) for k in (0, 1, 3)]
