@@ -443,14 +443,36 @@ int32_t lg_camera_setup(const double cam_rotate[3], const double cam_translate[3
                         double zfar, int32_t width, int32_t height, lg_camera *out);
 
 /* Kernel (d): per-pixel march.  Replaces VolumeRender::operator()
- * (Renderer.h:506-693).  samples = AA samples per pixel; the render stream is
- * Philox counter (pixel, sample, block, 2) keyed by seed (reference: RandGen(1),
- * Renderer.h:518); env lights use (pixel, sample, block, 3) keyed by env_seed. */
+ * (Renderer.h:506-693).  samples = AA samples per pixel, in [1, 2^20), and
+ * width * height * samples < 2^32.  Streams (reference: one RandGen(1) per TBB
+ * range, Renderer.h:518): the primary ray of (pixel, sample s) draws camera
+ * jitter and steps from Philox counter (pixel, s, block, 2) keyed by seed; at its
+ * q-th shading point, light k draws shadow steps from (pixel, q, block,
+ * 5 | k << 8 | s << 12) keyed by seed and env-map samples from (pixel, q, block,
+ * 3 | k << 8 | s << 12) keyed by env_seed.  Device scratch: lg_render_workspace. */
 int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *scatter,
                   const lg_grid *temperature, const lg_camera *camera,
                   const lg_shader *shader, const lg_light *lights, int32_t n_lights,
                   const lg_march_params *params, int32_t samples, uint64_t seed,
                   uint64_t env_seed, const lg_films *films);
+/* Device scratch an lg_render of this size keeps on its context (for admission
+ * control).  It is exceeded only when one block of 64 pixels holds more than
+ * 256 MiB of shading points.  No reference counterpart. */
+int32_t lg_render_workspace(int32_t width, int32_t height, int32_t samples, int32_t n_lights,
+                            uint64_t *bytes);
+
+/* Work of the last lg_render on a context, counted on the device while kernel
+ * timing is on (lg_ctx_set_kernel_timing), otherwise 0: the algorithmic units of
+ * kernel (d)'s roofline.  No reference counterpart. */
+typedef struct lg_render_stats {
+    uint64_t camera_rays;      /* (pixel, AA sample) primary rays                     */
+    uint64_t primary_samples;  /* trilinear samples of the primary march              */
+    uint64_t shading_points;   /* primary samples that passed the cutoff              */
+    uint64_t shadow_rays;      /* shadow rays (shading points x lights x samples)     */
+    uint64_t shadow_samples;   /* their trilinear samples                             */
+    uint64_t chunks;           /* shading-point chunks the render ran in              */
+} lg_render_stats;
+int32_t lg_render_get_stats(lg_ctx *ctx, lg_render_stats *out);
 
 /* Environment map: width x height RGB float32 (row 0 = top), luminance*sin(theta)
  * weighted 2D distribution (Light.h:165-176, Sampling.h:128-167) with its CDF

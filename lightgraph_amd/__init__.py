@@ -511,6 +511,20 @@ def render(ctx, density, scatter, temperature, camera, shader, lights, params, s
                            len(lights), C.byref(params), samples, seed, env_seed, C.byref(films.s)), "render")
 
 
+def render_stats(ctx):
+    """Work counters of the last render on ctx (counted while kernel timing is on)."""
+    st = _abi.lg_render_stats()
+    _check(lib().lg_render_get_stats(ctx.handle, C.byref(st)), "render_get_stats")
+    return {k: int(getattr(st, k)) for k, _ in st._fields_}
+
+
+def render_workspace(width, height, samples, n_lights):
+    """Device scratch bytes a render of this size keeps on its context."""
+    b = C.c_uint64()
+    _check(lib().lg_render_workspace(width, height, samples, n_lights, C.byref(b)), "render_workspace")
+    return int(b.value)
+
+
 # ------------------------------------------------------------- serving -------
 def job(scene, points, camera, shader, lights, march, iterations=1, radius=12.0, max_connect=12, samples=1,
         seed=854123, env_seed=4564446, render_seed=1, solve_base=0, pts_per_kernel=3, scatter_res_factor=4):

@@ -74,7 +74,9 @@ struct Stream {
     }
 };
 
-enum : uint32_t { DOM_SAMPLE = 1, DOM_RENDER = 2, DOM_RENDER_ENV = 3, DOM_VERTEX_ENV = 4 };
+// Stream domains (fourth counter word).  The render's shadow streams carry
+// (AA sample << 12 | light << 8) above the domain byte.
+enum : uint32_t { DOM_SAMPLE = 1, DOM_RENDER = 2, DOM_RENDER_ENV = 3, DOM_VERTEX_ENV = 4, DOM_RENDER_SHADOW = 5 };
 
 // ----------------------------------------------------------- grid view ------
 struct bf16_t {

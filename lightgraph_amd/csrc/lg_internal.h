@@ -74,9 +74,15 @@ struct lg_ctx {
     bool timing = false;
     lg::KernelTimer ktimer;
     hipEvent_t marks[16] = {};
-    // scratch reused by entry points that need temporary device memory
+    // scratch reused by entry points that need temporary device memory (slot 0),
+    // and a second slot for a call that needs two live at once (the render)
     void *scratch = nullptr;
     size_t scratch_bytes = 0;
+    void *scratch1 = nullptr;
+    size_t scratch1_bytes = 0;
+    // work counters of the last lg_render (kernel timing on), device u64 slots
+    unsigned long long *render_stats = nullptr;
+    uint64_t render_chunks = 0;
     int cu_count = 256;
     void *stage_pipeline = nullptr;  // lg::Pipeline used by the single-stage entry points
 };
@@ -133,8 +139,8 @@ int32_t dalloc(lg_ctx *ctx, size_t count, T **out) {
     *out = static_cast<T *>(p);
     return rc;
 }
-// scratch that lives until the next call that asks for scratch
-int32_t scratch(lg_ctx *ctx, size_t bytes, void **out);
+// scratch that lives until the next call that asks for scratch of the same slot
+int32_t scratch(lg_ctx *ctx, size_t bytes, void **out, int slot = 0);
 
 // kernel timing hooks (no-ops unless lg_ctx_set_kernel_timing enabled it)
 void ktimer_begin(lg_ctx *ctx, const char *name, hipEvent_t *a);

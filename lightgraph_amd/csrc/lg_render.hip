@@ -303,13 +303,224 @@ __device__ __forceinline__ void render_ramp(const RenderConsts &c, double x, dou
     for (int k = 0; k < 3; ++k) out[k] = c.ramp_col[prev][k] * (1.0 - blend) + c.ramp_col[next][k] * blend;
 }
 
+// Kernel (d), the per-pixel march of Renderer.h:506-693, in four phases over
+// the (pixel, AA sample) rays, ray r = pixel * samples + sample:
+//   count   one lane per ray marches the primary ray and counts its shading
+//           points (samples that pass the cutoff, Renderer.h:595);
+//   fill    per chunk of pixels, one lane per ray marches again, writes each
+//           shading point (world position, T before it, 1 - dT) and sums the
+//           scatter and emission terms and the final transmittance of the ray;
+//   shadow  one lane per (light, shading point) runs that light's samples
+//           (illuminate + shadow march, Renderer.h:609-651) and writes the
+//           point's direct term;
+//   combine one lane per pixel sums its rays' terms and AA samples into the films.
+// The shadow rays of a shading point draw from their own streams, so the
+// primary march does not wait on them and the shadow phase runs at full
+// occupancy with neighbouring lanes on neighbouring points of one light.
+struct ShadePoint {
+    double wp[3];     // world position
+    float tf[3];      // T before the sample, expf(ext * tau) (Renderer.h:603)
+    float om[3];      // 1 - dT, -expm1f(ext * density * step)
+    uint32_t ray;     // r = pixel * samples + sample
+    uint32_t ord;     // ordinal of the shading point on its ray
+};
+struct RayAcc {
+    double sl[3], el[3];  // scatter and emission sums of the ray
+    double tr[3];         // final transmittance of the ray
+};
+
+enum : int { RS_CAMERA_RAYS = 0, RS_PRIMARY_SAMPLES, RS_SHADING_POINTS, RS_SHADOW_RAYS, RS_SHADOW_SAMPLES, RS_COUNT };
+
+__device__ __forceinline__ void render_stat(unsigned long long *stats, int counter, unsigned long long v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0 && v) stat_add(stats, counter, v);
+}
+
+__device__ __forceinline__ uint32_t shadow_tag(uint32_t sample, int light) {
+    return (sample << 12) | ((uint32_t)light << 8);
+}
+
+// PerspectiveCamera::getRay (Renderer.h:216-229) for pixel (i, j) with the
+// jitter from rng, clipped to the grid (VolumeRayIntersector::setWorldRay).
+__device__ __forceinline__ bool camera_ray(const lgd::GridView &g, const RenderConsts &c, int i, int j,
+                                           lgd::Stream &rng, lgd::Isect &pr) {
+    const double eye[3] = {c.M[12], c.M[13], c.M[14]};
+    const double axis[3] = {0.0 * c.M[0] + 0.0 * c.M[4] + -1.0 * c.M[8], 0.0 * c.M[1] + 0.0 * c.M[5] + -1.0 * c.M[9],
+                            0.0 * c.M[2] + 0.0 * c.M[6] + -1.0 * c.M[10]};
+    const double ri = (double)i + 0.5 + ((double)rng.next() - 0.5);
+    const double rj = (double)j + 0.5 + ((double)rng.next() - 0.5);
+    const double sx = (2.0 * ri / (double)c.width - 1.0) * c.scale_w;
+    const double sy = (1.0 - 2.0 * rj / (double)c.height) * c.scale_h;
+    const double sz = -1.0;
+    double dir[3];
+    for (int a = 0; a < 3; ++a) dir[a] = sx * c.M[a] + sy * c.M[4 + a] + sz * c.M[8 + a];
+    const double dl = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    if (!(fabs(dl - 0.0) <= 1e-7)) {
+        const double inv = 1.0 / dl;
+        for (int a = 0; a < 3; ++a) dir[a] = dir[a] * inv;
+    }
+    const double scale = 1.0 / (dir[0] * axis[0] + dir[1] * axis[1] + dir[2] * axis[2]);
+    return lgd::isect_world_ray(g, pr, eye, dir, c.znear * scale, c.zfar * scale);
+}
+
+// The primary march (Renderer.h:548-687) without the shadow rays: f(ip, pd,
+// step, tau) runs at every sample that passes the cutoff, tau being the optical
+// depth before it; the step is redrawn at each span and after each such sample.
+// Returns the final optical depth; *samples counts the trilinear samples.
+template <typename T, typename F>
+__device__ __forceinline__ double primary_march(const lgd::GridView &g, const RenderConsts &c, lgd::Isect &pr,
+                                                lgd::Stream &rng, uint32_t *samples, F &&f) {
+    double tau = 0.0;
+    bool drop = false;
+    double t0, t1;
+    while (!drop && lgd::isect_march(g, pr, t0, t1)) {
+        double step = c.step0 * (1.0 + ((double)rng.next() - 0.5) * 0.5);
+        for (double t = c.step0 * ceil(t0 / c.step0); t <= t1; t += step) {
+            const double ip[3] = {pr.e[0] + pr.d[0] * t, pr.e[1] + pr.d[1] * t, pr.e[2] + pr.d[2] * t};
+            const float rho = lgd::trilinear<T>(g, ip[0], ip[1], ip[2]);
+            ++*samples;
+            double pd = (double)rho * c.ds;
+            pd = (c.dmax < pd) ? c.dmax : pd;
+            if (pd < c.cutoff) continue;
+            f(ip, pd, step, tau);
+            tau += pd * step;
+            if (tau > c.tau_star) {
+                drop = true;
+                break;
+            }
+            step = c.step0 * (1.0 + ((double)rng.next() - 0.5) * 0.5);
+        }
+    }
+    return tau;
+}
+
+__device__ __forceinline__ void ray_pixel(const RenderConsts &c, uint32_t r, uint32_t &pix, uint32_t &s, int &i,
+                                          int &j) {
+    pix = r / (uint32_t)c.samples;
+    s = r - pix * (uint32_t)c.samples;
+    j = (int)(pix / (uint32_t)c.width);
+    i = (int)(pix - (uint32_t)j * (uint32_t)c.width);
+}
+
+// Phase count: shading points of every ray (0 when no light needs them).
+template <typename T>
+__global__ void __launch_bounds__(256) k_render_count(lgd::GridView g, RenderConsts c, int n_lights, uint32_t k0,
+                                                      uint32_t k1, uint32_t nrays, int *__restrict__ cnt,
+                                                      unsigned long long *__restrict__ stats) {
+    const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    uint32_t n = 0, smp = 0;
+    if (r < nrays) {
+        uint32_t pix, s;
+        int i, j;
+        ray_pixel(c, r, pix, s, i, j);
+        lgd::Stream rng;
+        rng.init(pix, s, lgd::DOM_RENDER, ((uint64_t)k1 << 32) | k0);
+        lgd::Isect pr;
+        if (n_lights > 0 && camera_ray(g, c, i, j, rng, pr))
+            (void)primary_march<T>(g, c, pr, rng, &smp, [&](const double *, double, double, double) { ++n; });
+        cnt[r] = (int)n;
+    }
+    (void)smp;
+}
+
+// Shading points of each block of 64 pixels.
+__global__ void __launch_bounds__(256) k_render_block_sum(const int *__restrict__ cnt, uint32_t nrays,
+                                                          uint32_t rays_per_block, uint32_t nblocks,
+                                                          unsigned long long *__restrict__ bsum) {
+    const uint32_t b = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (b >= nblocks) return;  // whole waves leave together
+    const uint64_t r0 = (uint64_t)b * rays_per_block;
+    const uint64_t r1 = std::min<uint64_t>(r0 + rays_per_block, nrays);
+    unsigned long long v = 0;
+    for (uint64_t r = r0 + lane; r < r1; r += 64) v += (unsigned long long)cnt[r];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) bsum[b] = v;
+}
+
+// Phase fill over rays [ray0, ray0 + nrays).
+template <typename T>
+__global__ void __launch_bounds__(256) k_render_fill(lgd::GridView g, lgd::GridView tg, int has_temp, CoarseView sc,
+                                                     int has_scatter, RenderConsts c, uint32_t k0, uint32_t k1,
+                                                     uint32_t ray0, uint32_t nrays, const int *__restrict__ cnt,
+                                                     const int *__restrict__ off, ShadePoint *__restrict__ pts,
+                                                     RayAcc *__restrict__ acc, unsigned long long *__restrict__ stats) {
+    const uint32_t rl = blockIdx.x * 256u + threadIdx.x;
+    uint32_t smp = 0, npts = 0, cam = 0;
+    if (rl < nrays) {
+        const uint32_t r = ray0 + rl;
+        uint32_t pix, s;
+        int i, j;
+        ray_pixel(c, r, pix, s, i, j);
+        lgd::Stream rng;
+        rng.init(pix, s, lgd::DOM_RENDER, ((uint64_t)k1 << 32) | k0);
+        RayAcc a{};
+        a.tr[0] = a.tr[1] = a.tr[2] = 1.0;
+        lgd::Isect pr;
+        cam = 1;
+        if (camera_ray(g, c, i, j, rng, pr)) {
+            const int q0 = off[rl], qn = cnt[r];
+            uint32_t ord = 0;
+            const double tau = primary_march<T>(g, c, pr, rng, &smp, [&](const double *ip, double pd, double step,
+                                                                         double tau_before) {
+                const double wp[3] = {g.origin[0] + g.voxel * ip[0], g.origin[1] + g.voxel * ip[1],
+                                      g.origin[2] + g.voxel * ip[2]};
+                float tf[3], om[3];
+                for (int k = 0; k < 3; ++k) {
+                    tf[k] = expf((float)(c.ext[k] * tau_before));
+                    om[k] = -expm1f((float)(c.ext[k] * pd * step));
+                }
+                if ((int)ord < qn) {
+                    ShadePoint &p = pts[q0 + (int)ord];
+                    for (int k = 0; k < 3; ++k) {
+                        p.wp[k] = wp[k];
+                        p.tf[k] = tf[k];
+                        p.om[k] = om[k];
+                    }
+                    p.ray = r;
+                    p.ord = ord;
+                }
+                ++ord;
+                double msc[3] = {0, 0, 0};
+                if (has_scatter) {
+                    float sv[3];
+                    coarse_sample(sc, wp[0] * sc.inv_cv, wp[1] * sc.inv_cv, wp[2] * sc.inv_cv, sv);
+                    const double pc = pd < 0.0 ? 0.0 : (pd > 1.0 ? 1.0 : pd);
+                    const double mask = (double)powf((float)pc, (float)c.mask_pow);
+                    for (int k = 0; k < 3; ++k) msc[k] = (double)sv[k] * c.inv_scat[k] * mask * c.scatter_scale;
+                }
+                double emi[3] = {0, 0, 0};
+                if (c.ramp_n > 0) {
+                    double temp = 0.0;
+                    if (has_temp) temp = (double)lgd::trilinear<float>(tg, ip[0], ip[1], ip[2]);
+                    render_ramp(c, temp, emi);
+                    for (int k = 0; k < 3; ++k) emi[k] *= c.emission_scale;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    a.sl[k] += msc[k] * (double)tf[k] * (double)om[k];
+                    a.el[k] += emi[k] * (double)tf[k] * (double)om[k];
+                }
+            });
+            npts = ord;
+            for (int k = 0; k < 3; ++k) a.tr[k] = (double)expf((float)(c.ext[k] * tau));  // kept at dropout
+        }
+        acc[rl] = a;
+    }
+    if (stats) {
+        render_stat(stats, RS_CAMERA_RAYS, cam);
+        render_stat(stats, RS_PRIMARY_SAMPLES, smp);
+        render_stat(stats, RS_SHADING_POINTS, npts);
+    }
+}
+
 // Shadow march of the render (Renderer.h:615-643): jittered step redrawn at
 // each span and after each sample that passed the cutoff.  Returns false when
-// the ray misses the bbox; *T receives the transmittance (0 after dropout).
+// the ray misses the bbox; *tau_out receives the optical depth, *dropped the
+// dropout (transmittance 0).
 template <typename T>
 __device__ __forceinline__ bool render_shadow(const lgd::GridView &g, const RenderConsts &c, const double p[3],
                                               const double ldir[3], double distance, lgd::Stream &rng,
-                                              double *tau_out, bool *dropped) {
+                                              double *tau_out, bool *dropped, uint32_t *samples) {
     const double sd[3] = {-ldir[0], -ldir[1], -ldir[2]};
     lgd::Isect is;
     if (!lgd::isect_world_ray(g, is, p, sd, lgd::kDeltaD, distance)) return false;
@@ -320,6 +531,7 @@ __device__ __forceinline__ bool render_shadow(const lgd::GridView &g, const Rend
         double step = c.step1 * (1.0 + ((double)rng.next() - 0.5) * 0.5);
         for (double t = c.step1 * ceil(t0 / c.step1); t <= t1; t += step) {
             const float rho = lgd::trilinear<T>(g, is.e[0] + is.d[0] * t, is.e[1] + is.d[1] * t, is.e[2] + is.d[2] * t);
+            ++*samples;
             double dens = (double)rho * c.ds * c.sds;
             dens = (c.dmax < dens) ? c.dmax : dens;
             if (dens < c.cutoff) continue;
@@ -336,119 +548,85 @@ __device__ __forceinline__ bool render_shadow(const lgd::GridView &g, const Rend
     return true;
 }
 
-// Kernel (d): one lane per pixel, AA samples in sequence (Renderer.h:524-689).
+// Phase shadow: task t = light * P_pad + point (P_pad a multiple of 64, so a
+// wave holds one light); out[3 t ..] = albedo * (sum of the light's samples /
+// samples) * T * (1 - dT) (Renderer.h:644-648).
 template <typename T>
-__global__ void __launch_bounds__(256) k_render(lgd::GridView g, lgd::GridView tg, int has_temp, CoarseView sc,
-                                                int has_scatter, RenderConsts c, lgl::DevLights L, uint32_t k0,
-                                                uint32_t k1, uint32_t ek0, uint32_t ek1, float4 *__restrict__ rgb,
-                                                float4 *__restrict__ direct, float4 *__restrict__ scatter,
-                                                float4 *__restrict__ emission) {
-    const int i = blockIdx.x * 16 + (threadIdx.x & 15);
-    const int j = blockIdx.y * 16 + (threadIdx.x >> 4);
-    if (i >= c.width || j >= c.height) return;
-    const uint32_t pix = (uint32_t)j * (uint32_t)c.width + (uint32_t)i;
-    // camera ray origin and axis (PerspectiveCamera::initRay, Renderer.h:232-239)
-    const double eye[3] = {c.M[12], c.M[13], c.M[14]};
-    const double axis[3] = {0.0 * c.M[0] + 0.0 * c.M[4] + -1.0 * c.M[8], 0.0 * c.M[1] + 0.0 * c.M[5] + -1.0 * c.M[9],
-                            0.0 * c.M[2] + 0.0 * c.M[6] + -1.0 * c.M[10]};
+__global__ void __launch_bounds__(256) k_render_shadow(lgd::GridView g, RenderConsts c, lgl::DevLights L,
+                                                       uint32_t k0, uint32_t k1, uint32_t ek0, uint32_t ek1,
+                                                       const ShadePoint *__restrict__ pts, uint32_t P,
+                                                       uint32_t P_pad, double *__restrict__ out,
+                                                       unsigned long long *__restrict__ stats) {
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const int li = __builtin_amdgcn_readfirstlane((int)(t / P_pad));
+    const uint32_t q = t - (uint32_t)li * P_pad;
+    uint32_t rays = 0, smp = 0;
+    if (li < L.n && q < P) {
+        const lgl::DevLight &lt = L.l[li];
+        const ShadePoint sp = pts[q];
+        const uint32_t pix = sp.ray / (uint32_t)c.samples;
+        const uint32_t s = sp.ray - pix * (uint32_t)c.samples;
+        const uint32_t tag = shadow_tag(s, li);
+        lgd::Stream rng, env;
+        rng.init(pix, sp.ord, lgd::DOM_RENDER_SHADOW | tag, ((uint64_t)k1 << 32) | k0);
+        env.init(pix, sp.ord, lgd::DOM_RENDER_ENV | tag, ((uint64_t)ek1 << 32) | ek0);
+        double tl[3] = {0, 0, 0};
+        for (int ls = 0; ls < lt.samples; ++ls) {
+            double ldir[3], inten[3], dist;
+            lgl::illuminate(lt, sp.wp, env, ldir, inten, dist);
+            double tau_s;
+            bool sdrop;
+            ++rays;
+            if (!render_shadow<T>(g, c, sp.wp, ldir, dist, rng, &tau_s, &sdrop, &smp)) continue;
+            if (sdrop) continue;
+            for (int a = 0; a < 3; ++a) tl[a] += inten[a] * (double)expf((float)(c.ext[a] * tau_s));
+        }
+        double *o = out + 3ull * t;
+        for (int a = 0; a < 3; ++a)
+            o[a] = c.albedo[a] * (tl[a] / (double)lt.samples) * (double)sp.tf[a] * (double)sp.om[a];
+    }
+    if (stats) {
+        render_stat(stats, RS_SHADOW_RAYS, rays);
+        render_stat(stats, RS_SHADOW_SAMPLES, smp);
+    }
+}
+
+// Phase combine over pixels [pix0, pix0 + npix) (Renderer.h:670-689).
+__global__ void __launch_bounds__(256) k_render_combine(RenderConsts c, int n_lights, uint32_t pix0, uint32_t npix,
+                                                        const int *__restrict__ cnt, const int *__restrict__ off,
+                                                        const RayAcc *__restrict__ acc,
+                                                        const double *__restrict__ direct_terms, uint32_t P_pad,
+                                                        float4 *__restrict__ rgb, float4 *__restrict__ direct,
+                                                        float4 *__restrict__ scatter, float4 *__restrict__ emission) {
+    const uint32_t pl = blockIdx.x * 256u + threadIdx.x;
+    if (pl >= npix) return;
+    const uint32_t pix = pix0 + pl;
+    const uint32_t ray0 = pix0 * (uint32_t)c.samples;
     double lumAA[3] = {0, 0, 0}, dirAA[3] = {0, 0, 0}, scaAA[3] = {0, 0, 0}, emiAA[3] = {0, 0, 0};
     double transAA[3] = {0, 0, 0};
     for (int s = 0; s < c.samples; ++s) {
-        lgd::Stream rng, env;
-        rng.init(pix, (uint32_t)s, lgd::DOM_RENDER, ((uint64_t)k1 << 32) | k0);
-        env.init(pix, (uint32_t)s, lgd::DOM_RENDER_ENV, ((uint64_t)ek1 << 32) | ek0);
-        // PerspectiveCamera::getRay (Renderer.h:216-229)
-        const double ri = (double)i + 0.5 + ((double)rng.next() - 0.5);
-        const double rj = (double)j + 0.5 + ((double)rng.next() - 0.5);
-        const double sx = (2.0 * ri / (double)c.width - 1.0) * c.scale_w;
-        const double sy = (1.0 - 2.0 * rj / (double)c.height) * c.scale_h;
-        const double sz = -1.0;
-        double dir[3];
-        for (int a = 0; a < 3; ++a) dir[a] = sx * c.M[a] + sy * c.M[4 + a] + sz * c.M[8 + a];
-        const double dl = sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
-        if (!(fabs(dl - 0.0) <= 1e-7)) {
-            const double inv = 1.0 / dl;
-            for (int a = 0; a < 3; ++a) dir[a] = dir[a] * inv;
-        }
-        const double scale = 1.0 / (dir[0] * axis[0] + dir[1] * axis[1] + dir[2] * axis[2]);
-        lgd::Isect pr;
-        if (!lgd::isect_world_ray(g, pr, eye, dir, c.znear * scale, c.zfar * scale)) {
-            for (int a = 0; a < 3; ++a) transAA[a] += 1.0 * c.ratio_aa;
-            continue;
-        }
-        double lum[3] = {0, 0, 0}, dl_[3] = {0, 0, 0}, sl[3] = {0, 0, 0}, el[3] = {0, 0, 0};
-        double tau_p = 0.0;
-        bool pdrop = false;
-        double t0, t1;
-        while (!pdrop && lgd::isect_march(g, pr, t0, t1)) {
-            double step = c.step0 * (1.0 + ((double)rng.next() - 0.5) * 0.5);
-            for (double t = c.step0 * ceil(t0 / c.step0); t <= t1; t += step) {
-                const double ip[3] = {pr.e[0] + pr.d[0] * t, pr.e[1] + pr.d[1] * t, pr.e[2] + pr.d[2] * t};
-                const double wp[3] = {g.origin[0] + g.voxel * ip[0], g.origin[1] + g.voxel * ip[1],
-                                      g.origin[2] + g.voxel * ip[2]};
-                const float rho = lgd::trilinear<T>(g, ip[0], ip[1], ip[2]);
-                double pd = (double)rho * c.ds;
-                pd = (c.dmax < pd) ? c.dmax : pd;
-                if (pd < c.cutoff) continue;
-                // single scattering, multiple scattering and emission terms
-                const float trans_f[3] = {expf((float)(c.ext[0] * tau_p)), expf((float)(c.ext[1] * tau_p)),
-                                          expf((float)(c.ext[2] * tau_p))};
-                double one_m_dt[3];
-                for (int a = 0; a < 3; ++a) one_m_dt[a] = (double)(-expm1f((float)(c.ext[a] * pd * step)));
-                for (int li = 0; li < L.n; ++li) {
-                    const lgl::DevLight &lt = L.l[li];
-                    double tl[3] = {0, 0, 0};
-                    for (int ls = 0; ls < lt.samples; ++ls) {
-                        double ldir[3], inten[3], dist;
-                        lgl::illuminate(lt, wp, env, ldir, inten, dist);
-                        double tau_s;
-                        bool sdrop;
-                        if (!render_shadow<T>(g, c, wp, ldir, dist, rng, &tau_s, &sdrop)) continue;
-                        if (sdrop) continue;
-                        for (int a = 0; a < 3; ++a) tl[a] += inten[a] * (double)expf((float)(c.ext[a] * tau_s));
-                    }
-                    for (int a = 0; a < 3; ++a) {
-                        const double v = c.albedo[a] * (tl[a] / (double)lt.samples) * (double)trans_f[a] * one_m_dt[a];
-                        lum[a] += v;
-                        dl_[a] += v;
+        const uint32_t r = pix * (uint32_t)c.samples + (uint32_t)s;
+        const uint32_t rl = r - ray0;
+        const RayAcc a = acc[rl];
+        double lum[3] = {0, 0, 0}, dl[3] = {0, 0, 0};
+        if (n_lights > 0) {
+            const int q0 = off[rl], qn = cnt[r];
+            for (int q = q0; q < q0 + qn; ++q)
+                for (int li = 0; li < n_lights; ++li) {
+                    const double *v = direct_terms + 3ull * ((uint64_t)li * P_pad + (uint32_t)q);
+                    for (int k = 0; k < 3; ++k) {
+                        lum[k] += v[k];
+                        dl[k] += v[k];
                     }
                 }
-                double msc[3] = {0, 0, 0};
-                if (has_scatter) {
-                    float sv[3];
-                    coarse_sample(sc, wp[0] * sc.inv_cv, wp[1] * sc.inv_cv, wp[2] * sc.inv_cv, sv);
-                    const double pc = pd < 0.0 ? 0.0 : (pd > 1.0 ? 1.0 : pd);
-                    const double mask = (double)powf((float)pc, (float)c.mask_pow);
-                    for (int a = 0; a < 3; ++a) msc[a] = (double)sv[a] * c.inv_scat[a] * mask * c.scatter_scale;
-                }
-                double emi[3] = {0, 0, 0};
-                if (c.ramp_n > 0) {
-                    double temp = 0.0;
-                    if (has_temp) temp = (double)lgd::trilinear<float>(tg, ip[0], ip[1], ip[2]);
-                    render_ramp(c, temp, emi);
-                    for (int a = 0; a < 3; ++a) emi[a] *= c.emission_scale;
-                }
-                for (int a = 0; a < 3; ++a) {
-                    const double ts = msc[a] * (double)trans_f[a] * one_m_dt[a];
-                    const double te = emi[a] * (double)trans_f[a] * one_m_dt[a];
-                    sl[a] += ts;
-                    el[a] += te;
-                    lum[a] += ts + te;
-                }
-                tau_p += pd * step;
-                if (tau_p > c.tau_star) {
-                    pdrop = true;
-                    break;
-                }
-                step = c.step0 * (1.0 + ((double)rng.next() - 0.5) * 0.5);
-            }
         }
-        for (int a = 0; a < 3; ++a) {
-            lumAA[a] += lum[a] * c.ratio_aa;
-            dirAA[a] += dl_[a] * c.ratio_aa;
-            scaAA[a] += sl[a] * c.ratio_aa;
-            emiAA[a] += el[a] * c.ratio_aa;
-            transAA[a] += (double)expf((float)(c.ext[a] * tau_p)) * c.ratio_aa;  // pTrans is kept at dropout
+        for (int k = 0; k < 3; ++k) {
+            lum[k] += a.sl[k] + a.el[k];
+            lumAA[k] += lum[k] * c.ratio_aa;
+            dirAA[k] += dl[k] * c.ratio_aa;
+            scaAA[k] += a.sl[k] * c.ratio_aa;
+            emiAA[k] += a.el[k] * c.ratio_aa;
+            transAA[k] += a.tr[k] * c.ratio_aa;
         }
     }
     const float alpha = (float)(1.0f - (transAA[0] + transAA[1] + transAA[2]) / 3.0f);
@@ -456,6 +634,45 @@ __global__ void __launch_bounds__(256) k_render(lgd::GridView g, lgd::GridView t
     if (direct) direct[pix] = make_float4((float)dirAA[0], (float)dirAA[1], (float)dirAA[2], 0.0f);
     if (scatter) scatter[pix] = make_float4((float)scaAA[0], (float)scaAA[1], (float)scaAA[2], 0.0f);
     if (emission) emission[pix] = make_float4((float)emiAA[0], (float)emiAA[1], (float)emiAA[2], 0.0f);
+}
+
+// Render workspace.  Slot 0 holds the shading-point count of every ray and the
+// per-block sums; slot 1 holds one chunk: ray offsets, ray sums, scan storage,
+// shading points and their direct terms.  A chunk is a run of whole blocks of
+// 64 pixels with at most kRenderRayCap rays and about kRenderPointBytes of
+// shading-point data (a single block may exceed the latter).
+constexpr uint32_t kRenderBlockPixels = 64;
+constexpr uint64_t kRenderRayCap = 1u << 20;
+constexpr uint64_t kRenderPointBytes = 256ull << 20;
+
+inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+struct RenderLayout {
+    uint64_t npix, nrays, nblocks, rays_per_block, ray_cap, point_cap;
+    size_t bsum_off, head_bytes;
+    size_t acc_off, scan_off, scan_bytes, ray_area_bytes;
+    size_t point_bytes;  // per shading point, record and direct terms
+};
+
+inline RenderLayout render_layout(int w, int h, int samples, int n_lights) {
+    RenderLayout l{};
+    l.npix = (uint64_t)w * (uint64_t)h;
+    l.nrays = l.npix * (uint64_t)samples;
+    l.nblocks = (l.npix + kRenderBlockPixels - 1) / kRenderBlockPixels;
+    l.rays_per_block = (uint64_t)kRenderBlockPixels * (uint64_t)samples;
+    l.ray_cap = std::min(std::max(kRenderRayCap, l.rays_per_block), l.nblocks * l.rays_per_block);
+    l.point_bytes = sizeof(ShadePoint) + 24ull * (uint64_t)n_lights;
+    l.point_cap = n_lights > 0 ? kRenderPointBytes / l.point_bytes : 0;
+    l.bsum_off = align256(4 * l.nrays);
+    l.head_bytes = l.bsum_off + align256(8 * l.nblocks);
+    size_t sb = 0;
+    (void)rocprim::exclusive_scan(nullptr, sb, (const int *)nullptr, (int *)nullptr, 0, (size_t)l.ray_cap,
+                                  rocprim::plus<int>());
+    l.scan_bytes = sb;
+    l.acc_off = align256(4 * l.ray_cap);
+    l.scan_off = l.acc_off + align256(sizeof(RayAcc) * l.ray_cap);
+    l.ray_area_bytes = l.scan_off + align256(sb);
+    return l;
 }
 
 template <typename F>
@@ -773,7 +990,9 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
                   const lg_march_params *mp, int32_t samples, uint64_t seed, uint64_t env_seed, const lg_films *films) {
     LG_CHECK_ARG(ctx && density && camera && sh && mp && films, "lg_render: NULL argument");
     LG_CHECK_ARG(camera->width > 0 && camera->height > 0, "lg_render: empty image");
-    LG_CHECK_ARG(samples >= 0, "lg_render: samples must be >= 0");
+    LG_CHECK_ARG(samples >= 1 && samples < (1 << 20), "lg_render: samples must be in [1, 2^20)");
+    LG_CHECK_ARG((uint64_t)camera->width * (uint64_t)camera->height * (uint64_t)samples < (1ull << 32),
+                 "lg_render: width * height * samples must be below 2^32");
     LG_CHECK_ARG(mp->step[0] > 0 && mp->step[1] > 0, "lg_render: steps must be positive");
     LG_CHECK_ARG(!temperature || temperature->desc.dtype == LG_F32, "temperature grid must be float32");
     lgl::DevLights L{};
@@ -813,19 +1032,139 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
     const lgd::GridView g = make_view(density);
     const lgd::GridView tg = temperature ? make_view(temperature) : lgd::GridView{};
     const CoarseView sc = scatter && scatter->d_rgb ? coarse_view(scatter) : CoarseView{};
-    dim3 grid((c.width + 15) / 16, (c.height + 15) / 16);
+    const int has_sc = (scatter && scatter->d_rgb) ? 1 : 0;
+    const RenderLayout lay = render_layout(c.width, c.height, samples, L.n);
+    const uint32_t nrays = (uint32_t)lay.nrays;
+    ctx->render_chunks = 0;
+    unsigned long long *stats = nullptr;
+    if (ctx->timing) {
+        if (!ctx->render_stats) LG_TRY(dalloc(ctx, (size_t)RS_COUNT * kStatSlots, &ctx->render_stats));
+        stats = ctx->render_stats;
+        LG_HIP(hipMemsetAsync(stats, 0, sizeof(unsigned long long) * RS_COUNT * kStatSlots, ctx->stream));
+    }
+    if (nrays == 0) return LG_OK;
+    // phase count over the whole image, then the shading points of every block of pixels
+    void *head = nullptr;
+    LG_TRY(scratch(ctx, lay.head_bytes, &head, 0));
+    int *cnt = (int *)head;
+    unsigned long long *bsum = (unsigned long long *)((char *)head + lay.bsum_off);
+    const uint32_t u0 = (uint32_t)seed, u1 = (uint32_t)(seed >> 32);
+    const uint32_t e0 = (uint32_t)env_seed, e1 = (uint32_t)(env_seed >> 32);
     hipEvent_t ev;
-    ktimer_begin(ctx, "k_render", &ev);
+    ktimer_begin(ctx, "k_render_count", &ev);
     for_dtype_r(density->desc.dtype, [&](auto tag) {
         using T = decltype(tag);
-        k_render<T><<<grid, 256, 0, ctx->stream>>>(g, tg, temperature ? 1 : 0, sc, (scatter && scatter->d_rgb) ? 1 : 0,
-                                                   c, L, (uint32_t)seed, (uint32_t)(seed >> 32), (uint32_t)env_seed,
-                                                   (uint32_t)(env_seed >> 32), (float4 *)films->d_rgb,
-                                                   (float4 *)films->d_direct, (float4 *)films->d_scatter,
-                                                   (float4 *)films->d_emission);
+        k_render_count<T><<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(g, c, L.n, u0, u1, nrays, cnt, stats);
     });
-    LG_LAUNCHED("k_render");
-    ktimer_end(ctx, "k_render", ev);
+    LG_LAUNCHED("k_render_count");
+    ktimer_end(ctx, "k_render_count", ev);
+    const uint32_t nblocks = (uint32_t)lay.nblocks;
+    k_render_block_sum<<<(nblocks + 3) / 4, 256, 0, ctx->stream>>>(cnt, nrays, (uint32_t)lay.rays_per_block, nblocks,
+                                                                    bsum);
+    LG_LAUNCHED("k_render_block_sum");
+    std::vector<unsigned long long> h_bsum(nblocks);
+    LG_HIP(hipMemcpyAsync(h_bsum.data(), bsum, sizeof(unsigned long long) * nblocks, hipMemcpyDeviceToHost,
+                          ctx->stream));
+    LG_HIP(hipStreamSynchronize(ctx->stream));
+    // chunks of whole blocks within the ray and point caps
+    struct Chunk {
+        uint32_t b0, b1;
+        uint64_t points;
+    };
+    std::vector<Chunk> chunks;
+    uint64_t max_points = 0;
+    {
+        Chunk cur{0, 0, 0};
+        for (uint32_t b = 0; b < nblocks; ++b) {
+            const uint64_t rays_after = (uint64_t)(b + 1 - cur.b0) * lay.rays_per_block;
+            if (cur.b1 > cur.b0 && (rays_after > lay.ray_cap || cur.points + h_bsum[b] > lay.point_cap)) {
+                chunks.push_back(cur);
+                cur = Chunk{b, b, 0};
+            }
+            cur.b1 = b + 1;
+            cur.points += h_bsum[b];
+        }
+        chunks.push_back(cur);
+        for (const Chunk &ch : chunks) max_points = std::max(max_points, ch.points);
+    }
+    LG_CHECK_ARG(max_points < (1ull << 31) / (uint64_t)std::max(L.n, 1),
+                 "lg_render: %llu shading points in one block of %u pixels", (unsigned long long)max_points,
+                 kRenderBlockPixels);
+    const uint64_t pad_cap = (max_points + 63) & ~63ull;
+    const size_t ray_bytes = lay.ray_area_bytes;
+    const size_t pts_off = ray_bytes;
+    const size_t dir_off = pts_off + align256(sizeof(ShadePoint) * max_points);
+    const size_t body_bytes = dir_off + align256(24ull * (uint64_t)L.n * pad_cap);
+    void *body = nullptr;
+    LG_TRY(scratch(ctx, body_bytes, &body, 1));
+    int *off = (int *)body;
+    RayAcc *acc = (RayAcc *)((char *)body + lay.acc_off);
+    void *scan_tmp = (char *)body + lay.scan_off;
+    ShadePoint *pts = (ShadePoint *)((char *)body + pts_off);
+    double *dterms = (double *)((char *)body + dir_off);
+    for (const Chunk &ch : chunks) {
+        const uint32_t pix0 = ch.b0 * kRenderBlockPixels;
+        const uint32_t pix1 = std::min<uint64_t>((uint64_t)ch.b1 * kRenderBlockPixels, lay.npix);
+        const uint32_t ray0 = pix0 * (uint32_t)samples, nr = (pix1 - pix0) * (uint32_t)samples;
+        const uint32_t P = (uint32_t)ch.points, P_pad = (P + 63u) & ~63u;
+        size_t tb = lay.scan_bytes;
+        LG_HIP(rocprim::exclusive_scan(scan_tmp, tb, cnt + ray0, off, 0, (size_t)nr, rocprim::plus<int>(),
+                                       ctx->stream));
+        ktimer_begin(ctx, "k_render_fill", &ev);
+        for_dtype_r(density->desc.dtype, [&](auto tag) {
+            using T = decltype(tag);
+            k_render_fill<T><<<(nr + 255) / 256, 256, 0, ctx->stream>>>(g, tg, temperature ? 1 : 0, sc, has_sc, c, u0,
+                                                                        u1, ray0, nr, cnt, off, pts, acc, stats);
+        });
+        LG_LAUNCHED("k_render_fill");
+        ktimer_end(ctx, "k_render_fill", ev);
+        if (P > 0 && L.n > 0) {
+            const uint64_t tasks = (uint64_t)L.n * P_pad;
+            ktimer_begin(ctx, "k_render_shadow", &ev);
+            for_dtype_r(density->desc.dtype, [&](auto tag) {
+                using T = decltype(tag);
+                k_render_shadow<T><<<(unsigned)((tasks + 255) / 256), 256, 0, ctx->stream>>>(
+                    g, c, L, u0, u1, e0, e1, pts, P, P_pad, dterms, stats);
+            });
+            LG_LAUNCHED("k_render_shadow");
+            ktimer_end(ctx, "k_render_shadow", ev);
+        }
+        ktimer_begin(ctx, "k_render_combine", &ev);
+        k_render_combine<<<(pix1 - pix0 + 255) / 256, 256, 0, ctx->stream>>>(
+            c, L.n, pix0, pix1 - pix0, cnt, off, acc, dterms, P_pad, (float4 *)films->d_rgb, (float4 *)films->d_direct,
+            (float4 *)films->d_scatter, (float4 *)films->d_emission);
+        LG_LAUNCHED("k_render_combine");
+        ktimer_end(ctx, "k_render_combine", ev);
+    }
+    ctx->render_chunks = chunks.size();
+    return LG_OK;
+}
+
+int32_t lg_render_get_stats(lg_ctx *ctx, lg_render_stats *out) {
+    LG_CHECK_ARG(ctx && out, "lg_render_get_stats: NULL argument");
+    memset(out, 0, sizeof(*out));
+    out->chunks = ctx->render_chunks;
+    if (!ctx->render_stats) return LG_OK;
+    std::vector<unsigned long long> h((size_t)RS_COUNT * kStatSlots);
+    LG_HIP(hipMemcpyAsync(h.data(), ctx->render_stats, sizeof(unsigned long long) * h.size(), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    LG_HIP(hipStreamSynchronize(ctx->stream));
+    uint64_t v[RS_COUNT] = {};
+    for (int k = 0; k < RS_COUNT; ++k)
+        for (int q = 0; q < kStatSlots; ++q) v[k] += h[(size_t)k * kStatSlots + q];
+    out->camera_rays = v[RS_CAMERA_RAYS];
+    out->primary_samples = v[RS_PRIMARY_SAMPLES];
+    out->shading_points = v[RS_SHADING_POINTS];
+    out->shadow_rays = v[RS_SHADOW_RAYS];
+    out->shadow_samples = v[RS_SHADOW_SAMPLES];
+    return LG_OK;
+}
+
+int32_t lg_render_workspace(int32_t width, int32_t height, int32_t samples, int32_t n_lights, uint64_t *bytes) {
+    LG_CHECK_ARG(bytes && width > 0 && height > 0 && samples >= 1 && n_lights >= 0 && n_lights <= LG_MAX_LIGHTS,
+                 "lg_render_workspace: bad argument");
+    const RenderLayout l = render_layout(width, height, samples, n_lights);
+    *bytes = l.head_bytes + l.ray_area_bytes + align256(l.point_cap * l.point_bytes) + 512;
     return LG_OK;
 }
 

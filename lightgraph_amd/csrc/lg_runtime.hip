@@ -53,16 +53,18 @@ int32_t dfree(lg_ctx *ctx, void *p) {
     return LG_OK;
 }
 
-int32_t scratch(lg_ctx *ctx, size_t bytes, void **out) {
-    if (bytes > ctx->scratch_bytes) {
-        if (ctx->scratch) LG_TRY(dfree(ctx, ctx->scratch));
-        ctx->scratch = nullptr;
-        ctx->scratch_bytes = 0;
+int32_t scratch(lg_ctx *ctx, size_t bytes, void **out, int slot) {
+    void *&p = slot == 0 ? ctx->scratch : ctx->scratch1;
+    size_t &have = slot == 0 ? ctx->scratch_bytes : ctx->scratch1_bytes;
+    if (bytes > have) {
+        if (p) LG_TRY(dfree(ctx, p));
+        p = nullptr;
+        have = 0;
         size_t want = std::max(bytes, (size_t)1 << 20);
-        LG_TRY(dmalloc(ctx, want, &ctx->scratch));
-        ctx->scratch_bytes = want;
+        LG_TRY(dmalloc(ctx, want, &p));
+        have = want;
     }
-    *out = ctx->scratch;
+    *out = p;
     return LG_OK;
 }
 

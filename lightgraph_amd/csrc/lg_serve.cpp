@@ -61,15 +61,20 @@ uint64_t volume_bytes(const lg_scene_spec &s) {
 }
 
 // transient device memory of one job besides its volume (scatter grid, films,
-// solver workspace with room for a component holding a quarter of the points)
+// render workspace, solver workspace with room for a component holding a
+// quarter of the points)
 uint64_t job_bytes(const lg_job &j) {
     const uint64_t V = (uint64_t)std::max(j.points, 1) * (uint64_t)std::max(j.iterations, 1);
     const uint64_t nc = (uint64_t)(j.scene.n / std::max(j.scatter_res_factor, 1)) + 6;
     const uint64_t scatter = nc * nc * nc * 13;
     const uint64_t films = (uint64_t)j.camera.width * (uint64_t)j.camera.height * 64;
+    uint64_t render = 0;
+    if (lg_render_workspace(std::max(j.camera.width, 1), std::max(j.camera.height, 1), std::max(j.samples, 1),
+                            std::min(std::max(j.n_lights, 0), LG_MAX_LIGHTS), &render) != LG_OK)
+        render = 0;
     const uint64_t big = V / 4 + 128;
     const uint64_t solver = V * 2048 + big * big * 4;
-    return scatter + films + solver;
+    return scatter + films + render + solver;
 }
 
 double now_s(std::chrono::steady_clock::time_point t0) {

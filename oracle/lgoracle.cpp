@@ -10,6 +10,10 @@
 // Substitutions against the reference (each is also made by the device path):
 //   - random streams are Philox4x32-10 counter streams instead of std::mt19937
 //     engines (RandGen.h), keyed so that no stream is shared between threads;
+//     in the render, the primary ray of (pixel, AA sample) owns one stream
+//     (camera jitter, primary steps), and each shadow ray batch of (pixel, AA
+//     sample, shading point ordinal, light) owns a step stream and an env
+//     stream, so shadow rays draw nothing from the primary ray's stream;
 //   - points come from the density-weighted sampler instead of dart throwing
 //     (LightGraph.h:206-274); the dart thrower is kept here as lgo_dart_throw
 //     for the reference-mode CPU baseline;
@@ -1116,6 +1120,12 @@ void lgo_splat(const float *pts, const float *rad, int64_t count, double radius,
 // VolumeRender::operator() (Renderer.h:506-693), rows split over threads.
 // films: 4 consecutive width*height RGBA float arrays (rgb, direct, scatter,
 // emission).  scatter_rgb may be NULL (no multiple scattering term).
+// Streams: the primary ray of (pix, AA sample s) draws its camera jitter and
+// its steps from (pix, s, 2; seed).  At the shading point with ordinal q (the
+// q-th sample that passed the cutoff on that ray), light k draws its shadow
+// steps from (pix, q, 5 | k << 8 | s << 12; seed) and its env-map samples from
+// (pix, q, 3 | k << 8 | s << 12; env_seed), light samples in order.  The
+// reference draws all of these from one engine per TBB range (Renderer.h:518).
 void lgo_render(const lg_grid_desc *d, const float *dense, const float *temp_dense, const float *scatter_rgb,
                 const int32_t sn[3], const int32_t si0[3], double scv, const lg_camera *cam, const lg_shader *sh,
                 const lg_light *lights, int32_t nl, const float *const *env_rgb, const int32_t *env_w,
@@ -1150,7 +1160,7 @@ void lgo_render(const lg_grid_desc *d, const float *dense, const float *temp_den
                 const double ratioAA = 1.0 / (double)samples;
                 for (int sampleAA = 0; sampleAA < samples; sampleAA++) {
                     CounterStream rng(pix, (uint32_t)sampleAA, 2u, seed);
-                    CounterStream envRng(pix, (uint32_t)sampleAA, 3u, env_seed);
+                    uint32_t ordinal = 0;  // shading points taken so far on this ray
                     const Ray pRay = camera.getRay(i, j, rng);
                     Vec3 pTrans(1.0);
                     Intersector primary(g), shadow(g);
@@ -1181,6 +1191,9 @@ void lgo_render(const lg_grid_desc *d, const float *dense, const float *temp_den
                             if (pDensity < cutoff) continue;
                             const Vec3 dT = expv(ext * pDensity * pStep);
                             for (int k = 0; k < nl; k++) {
+                                const uint32_t tag = ((uint32_t)sampleAA << 12) | ((uint32_t)k << 8);
+                                CounterStream sRng(pix, ordinal, 5u | tag, seed);
+                                CounterStream envRng(pix, ordinal, 3u | tag, env_seed);
                                 Vec3 tmpLumi(0.0);
                                 for (int s = 0; s < lights[k].samples; s++) {
                                     Vec3 sTrans(1.0), lightDir, lightIntensity;
@@ -1191,7 +1204,7 @@ void lgo_render(const lg_grid_desc *d, const float *dense, const float *temp_den
                                     double sT0, sT1;
                                     while (shadow.march(sT0, sT1)) {
                                         bool sDropout = false;
-                                        double sStep = h1 * (1.0 + ((double)rng() - 0.5) * jitterAmp);
+                                        double sStep = h1 * (1.0 + ((double)sRng() - 0.5) * jitterAmp);
                                         double sT = h1 * std::ceil(sT0 / h1);
                                         for (; sT <= sT1; sT += sStep) {
                                             const double sDensity =
@@ -1205,7 +1218,7 @@ void lgo_render(const lg_grid_desc *d, const float *dense, const float *temp_den
                                                 sDropout = true;
                                                 break;
                                             }
-                                            sStep = h1 * (1.0 + ((double)rng() - 0.5) * jitterAmp);
+                                            sStep = h1 * (1.0 + ((double)sRng() - 0.5) * jitterAmp);
                                         }
                                         if (sDropout) break;
                                     }
@@ -1220,6 +1233,7 @@ void lgo_render(const lg_grid_desc *d, const float *dense, const float *temp_den
                             scatterLumi += tmpScatterLumi;
                             emitLumi += tmpEmitLumi;
                             pLumi += tmpScatterLumi + tmpEmitLumi;
+                            ++ordinal;
                             pTrans *= dT;
                             if (pTrans.lengthSqr() < cutoff) {
                                 pDropout = true;
