@@ -456,10 +456,12 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
                   const lg_march_params *params, int32_t samples, uint64_t seed,
                   uint64_t env_seed, const lg_films *films);
 /* Device scratch an lg_render of this size keeps on its context (for admission
- * control).  It is exceeded only when one block of 64 pixels holds more than
- * 256 MiB of shading points.  No reference counterpart. */
+ * control), given the most shading points one camera ray can take (<= 0:
+ * unknown; a ray through n voxels at primary step h takes at most n / (0.75 h)
+ * + 1).  It is exceeded only when one block of 64 pixels holds more than 512 MiB
+ * of shading points.  No reference counterpart. */
 int32_t lg_render_workspace(int32_t width, int32_t height, int32_t samples, int32_t n_lights,
-                            uint64_t *bytes);
+                            int64_t max_points_per_ray, uint64_t *bytes);
 
 /* Work of the last lg_render on a context, counted on the device while kernel
  * timing is on (lg_ctx_set_kernel_timing), otherwise 0: the algorithmic units of
@@ -470,6 +472,8 @@ typedef struct lg_render_stats {
     uint64_t shading_points;   /* primary samples that passed the cutoff              */
     uint64_t shadow_rays;      /* shadow rays (shading points x lights x samples)     */
     uint64_t shadow_samples;   /* their trilinear samples                             */
+    uint64_t shadow_lane_slots; /* 64 x the most samples of a lane, summed over the
+                                 * shadow waves: shadow_samples / this = lane efficiency */
     uint64_t chunks;           /* shading-point chunks the render ran in              */
 } lg_render_stats;
 int32_t lg_render_get_stats(lg_ctx *ctx, lg_render_stats *out);

@@ -518,10 +518,11 @@ def render_stats(ctx):
     return {k: int(getattr(st, k)) for k, _ in st._fields_}
 
 
-def render_workspace(width, height, samples, n_lights):
+def render_workspace(width, height, samples, n_lights, max_points_per_ray=0):
     """Device scratch bytes a render of this size keeps on its context."""
     b = C.c_uint64()
-    _check(lib().lg_render_workspace(width, height, samples, n_lights, C.byref(b)), "render_workspace")
+    _check(lib().lg_render_workspace(width, height, samples, n_lights, max_points_per_ray, C.byref(b)),
+           "render_workspace")
     return int(b.value)
 
 

@@ -99,7 +99,7 @@ class lg_solve_stats(C.Structure):
 
 class lg_render_stats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("camera_rays", "primary_samples", "shading_points", "shadow_rays",
-                                          "shadow_samples", "chunks")]
+                                          "shadow_samples", "shadow_lane_slots", "chunks")]
 
 
 LG_MAX_LIGHTS_ = 8
@@ -197,7 +197,7 @@ SIGNATURES = {
     "lg_render": (I32, [P, P, C.POINTER(lg_scatter_grid), P, C.POINTER(lg_camera), C.POINTER(lg_shader),
                         C.POINTER(lg_light), I32, C.POINTER(lg_march_params), I32, U64, U64,
                         C.POINTER(lg_films)]),
-    "lg_render_workspace": (I32, [I32, I32, I32, I32, C.POINTER(C.c_uint64)]),
+    "lg_render_workspace": (I32, [I32, I32, I32, I32, I64, C.POINTER(C.c_uint64)]),
     "lg_render_get_stats": (I32, [P, C.POINTER(lg_render_stats)]),
     "lg_envmap_create": (I32, [P, I32, I32, P, PP]),
     "lg_envmap_destroy": (I32, [P, P]),

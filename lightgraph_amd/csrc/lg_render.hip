@@ -7,7 +7,9 @@
 #include <unordered_set>
 #include <vector>
 
+#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include "lg_internal.h"
 #include "lg_lights.h"
@@ -303,37 +305,45 @@ __device__ __forceinline__ void render_ramp(const RenderConsts &c, double x, dou
     for (int k = 0; k < 3; ++k) out[k] = c.ramp_col[prev][k] * (1.0 - blend) + c.ramp_col[next][k] * blend;
 }
 
-// Kernel (d), the per-pixel march of Renderer.h:506-693, in four phases over
-// the (pixel, AA sample) rays, ray r = pixel * samples + sample:
+// Kernel (d), the per-pixel march of Renderer.h:506-693, in phases over the
+// (pixel, AA sample) rays, ray r = pixel * samples + sample:
 //   count   one lane per ray marches the primary ray and counts its shading
 //           points (samples that pass the cutoff, Renderer.h:595);
-//   fill    per chunk of pixels, one lane per ray marches again, writes each
-//           shading point (world position, T before it, 1 - dT) and sums the
-//           scatter and emission terms and the final transmittance of the ray;
+//   then per chunk of pixels:
+//   fill    one lane per ray marches again and writes each shading point's
+//           world position, ray and ordinal;
 //   shadow  one lane per (light, shading point) runs that light's samples
-//           (illuminate + shadow march, Renderer.h:609-651) and writes the
-//           point's direct term;
-//   combine one lane per pixel sums its rays' terms and AA samples into the films.
+//           (illuminate + shadow march, Renderer.h:609-643) and writes the sum
+//           of intensity * transmittance;
+//   combine one lane per ray marches again and shades every point in the
+//           reference's order: direct term per light, scatter, emission
+//           (Renderer.h:644-666), into the ray's four AOVs and transmittance;
+//   aa      one lane per pixel sums its rays (Renderer.h:670-689).
 // The shadow rays of a shading point draw from their own streams, so the
-// primary march does not wait on them and the shadow phase runs at full
-// occupancy with neighbouring lanes on neighbouring points of one light.
+// primary march does not wait on them and the shadow phase runs with
+// neighbouring lanes on neighbouring points of one light.
 struct ShadePoint {
-    double wp[3];     // world position
-    float tf[3];      // T before the sample, expf(ext * tau) (Renderer.h:603)
-    float om[3];      // 1 - dT, -expm1f(ext * density * step)
-    uint32_t ray;     // r = pixel * samples + sample
-    uint32_t ord;     // ordinal of the shading point on its ray
+    double wp[3];  // world position
+    uint32_t ray;  // r = pixel * samples + sample
+    uint32_t ord;  // ordinal of the shading point on its ray
 };
-struct RayAcc {
-    double sl[3], el[3];  // scatter and emission sums of the ray
-    double tr[3];         // final transmittance of the ray
+struct RayOut {
+    double lum[3], dl[3], sl[3], el[3];  // beauty, direct, scatter, emission
+    double tr[3];                        // final transmittance
 };
 
-enum : int { RS_CAMERA_RAYS = 0, RS_PRIMARY_SAMPLES, RS_SHADING_POINTS, RS_SHADOW_RAYS, RS_SHADOW_SAMPLES, RS_COUNT };
+enum : int { RS_CAMERA_RAYS = 0, RS_PRIMARY_SAMPLES, RS_SHADING_POINTS, RS_SHADOW_RAYS, RS_SHADOW_SAMPLES,
+             RS_SHADOW_LANE_SLOTS, RS_COUNT };
 
 __device__ __forceinline__ void render_stat(unsigned long long *stats, int counter, unsigned long long v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if ((threadIdx.x & 63) == 0 && v) stat_add(stats, counter, v);
+}
+
+// 64 x the largest v of the wave: the lane-iterations the wave spent on v
+__device__ __forceinline__ void render_stat_wave_max(unsigned long long *stats, int counter, uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    if ((threadIdx.x & 63) == 0 && v) stat_add(stats, counter, 64ull * v);
 }
 
 __device__ __forceinline__ uint32_t shadow_tag(uint32_t sample, int light) {
@@ -405,22 +415,19 @@ __device__ __forceinline__ void ray_pixel(const RenderConsts &c, uint32_t r, uin
 // Phase count: shading points of every ray (0 when no light needs them).
 template <typename T>
 __global__ void __launch_bounds__(256) k_render_count(lgd::GridView g, RenderConsts c, int n_lights, uint32_t k0,
-                                                      uint32_t k1, uint32_t nrays, int *__restrict__ cnt,
-                                                      unsigned long long *__restrict__ stats) {
+                                                      uint32_t k1, uint32_t nrays, int *__restrict__ cnt) {
     const uint32_t r = blockIdx.x * 256u + threadIdx.x;
+    if (r >= nrays) return;
+    uint32_t pix, s;
+    int i, j;
+    ray_pixel(c, r, pix, s, i, j);
+    lgd::Stream rng;
+    rng.init(pix, s, lgd::DOM_RENDER, ((uint64_t)k1 << 32) | k0);
+    lgd::Isect pr;
     uint32_t n = 0, smp = 0;
-    if (r < nrays) {
-        uint32_t pix, s;
-        int i, j;
-        ray_pixel(c, r, pix, s, i, j);
-        lgd::Stream rng;
-        rng.init(pix, s, lgd::DOM_RENDER, ((uint64_t)k1 << 32) | k0);
-        lgd::Isect pr;
-        if (n_lights > 0 && camera_ray(g, c, i, j, rng, pr))
-            (void)primary_march<T>(g, c, pr, rng, &smp, [&](const double *, double, double, double) { ++n; });
-        cnt[r] = (int)n;
-    }
-    (void)smp;
+    if (n_lights > 0 && camera_ray(g, c, i, j, rng, pr))
+        (void)primary_march<T>(g, c, pr, rng, &smp, [&](const double *, double, double, double) { ++n; });
+    cnt[r] = (int)n;
 }
 
 // Shading points of each block of 64 pixels.
@@ -438,79 +445,43 @@ __global__ void __launch_bounds__(256) k_render_block_sum(const int *__restrict_
     if (lane == 0) bsum[b] = v;
 }
 
-// Phase fill over rays [ray0, ray0 + nrays).
+// Phase fill over rays [ray0, ray0 + nrays): shading point q0 + k of ray r is
+// its k-th, q0 = off[r - ray0].  keys[q] = the point's ordinal (capped): the
+// shadow phase visits points in (ordinal, ray) order, so that a wave holds
+// points at the same depth of neighbouring rays, whose shadow rays are alike.
+constexpr uint32_t kOrdKeyMax = 4095;
+constexpr int kOrdKeyBits = 12;
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_render_fill(lgd::GridView g, lgd::GridView tg, int has_temp, CoarseView sc,
-                                                     int has_scatter, RenderConsts c, uint32_t k0, uint32_t k1,
+__global__ void __launch_bounds__(256) k_render_fill(lgd::GridView g, RenderConsts c, uint32_t k0, uint32_t k1,
                                                      uint32_t ray0, uint32_t nrays, const int *__restrict__ cnt,
                                                      const int *__restrict__ off, ShadePoint *__restrict__ pts,
-                                                     RayAcc *__restrict__ acc, unsigned long long *__restrict__ stats) {
+                                                     uint32_t *__restrict__ keys) {
     const uint32_t rl = blockIdx.x * 256u + threadIdx.x;
-    uint32_t smp = 0, npts = 0, cam = 0;
-    if (rl < nrays) {
-        const uint32_t r = ray0 + rl;
-        uint32_t pix, s;
-        int i, j;
-        ray_pixel(c, r, pix, s, i, j);
-        lgd::Stream rng;
-        rng.init(pix, s, lgd::DOM_RENDER, ((uint64_t)k1 << 32) | k0);
-        RayAcc a{};
-        a.tr[0] = a.tr[1] = a.tr[2] = 1.0;
-        lgd::Isect pr;
-        cam = 1;
-        if (camera_ray(g, c, i, j, rng, pr)) {
-            const int q0 = off[rl], qn = cnt[r];
-            uint32_t ord = 0;
-            const double tau = primary_march<T>(g, c, pr, rng, &smp, [&](const double *ip, double pd, double step,
-                                                                         double tau_before) {
-                const double wp[3] = {g.origin[0] + g.voxel * ip[0], g.origin[1] + g.voxel * ip[1],
-                                      g.origin[2] + g.voxel * ip[2]};
-                float tf[3], om[3];
-                for (int k = 0; k < 3; ++k) {
-                    tf[k] = expf((float)(c.ext[k] * tau_before));
-                    om[k] = -expm1f((float)(c.ext[k] * pd * step));
-                }
-                if ((int)ord < qn) {
-                    ShadePoint &p = pts[q0 + (int)ord];
-                    for (int k = 0; k < 3; ++k) {
-                        p.wp[k] = wp[k];
-                        p.tf[k] = tf[k];
-                        p.om[k] = om[k];
-                    }
-                    p.ray = r;
-                    p.ord = ord;
-                }
-                ++ord;
-                double msc[3] = {0, 0, 0};
-                if (has_scatter) {
-                    float sv[3];
-                    coarse_sample(sc, wp[0] * sc.inv_cv, wp[1] * sc.inv_cv, wp[2] * sc.inv_cv, sv);
-                    const double pc = pd < 0.0 ? 0.0 : (pd > 1.0 ? 1.0 : pd);
-                    const double mask = (double)powf((float)pc, (float)c.mask_pow);
-                    for (int k = 0; k < 3; ++k) msc[k] = (double)sv[k] * c.inv_scat[k] * mask * c.scatter_scale;
-                }
-                double emi[3] = {0, 0, 0};
-                if (c.ramp_n > 0) {
-                    double temp = 0.0;
-                    if (has_temp) temp = (double)lgd::trilinear<float>(tg, ip[0], ip[1], ip[2]);
-                    render_ramp(c, temp, emi);
-                    for (int k = 0; k < 3; ++k) emi[k] *= c.emission_scale;
-                }
-                for (int k = 0; k < 3; ++k) {
-                    a.sl[k] += msc[k] * (double)tf[k] * (double)om[k];
-                    a.el[k] += emi[k] * (double)tf[k] * (double)om[k];
-                }
-            });
-            npts = ord;
-            for (int k = 0; k < 3; ++k) a.tr[k] = (double)expf((float)(c.ext[k] * tau));  // kept at dropout
+    if (rl >= nrays) return;
+    const uint32_t r = ray0 + rl;
+    const int qn = cnt[r];
+    if (qn == 0) return;
+    uint32_t pix, s;
+    int i, j;
+    ray_pixel(c, r, pix, s, i, j);
+    lgd::Stream rng;
+    rng.init(pix, s, lgd::DOM_RENDER, ((uint64_t)k1 << 32) | k0);
+    lgd::Isect pr;
+    if (!camera_ray(g, c, i, j, rng, pr)) return;
+    const int q0 = off[rl];
+    ShadePoint *out = pts + q0;
+    uint32_t ord = 0, smp = 0;
+    (void)primary_march<T>(g, c, pr, rng, &smp, [&](const double *ip, double, double, double) {
+        if ((int)ord < qn) {
+            ShadePoint &p = out[ord];
+            for (int k = 0; k < 3; ++k) p.wp[k] = g.origin[k] + g.voxel * ip[k];
+            p.ray = r;
+            p.ord = ord;
+            keys[q0 + (int)ord] = ord < kOrdKeyMax ? ord : kOrdKeyMax;
         }
-        acc[rl] = a;
-    }
-    if (stats) {
-        render_stat(stats, RS_CAMERA_RAYS, cam);
-        render_stat(stats, RS_PRIMARY_SAMPLES, smp);
-        render_stat(stats, RS_SHADING_POINTS, npts);
-    }
+        ++ord;
+    });
 }
 
 // Shadow march of the render (Renderer.h:615-643): jittered step redrawn at
@@ -548,32 +519,40 @@ __device__ __forceinline__ bool render_shadow(const lgd::GridView &g, const Rend
     return true;
 }
 
-// Phase shadow: task t = light * P_pad + point (P_pad a multiple of 64, so a
-// wave holds one light); out[3 t ..] = albedo * (sum of the light's samples /
-// samples) * T * (1 - dT) (Renderer.h:644-648).
+// Phase shadow: lane light * P_pad + k takes point q = order[k] (P_pad a
+// multiple of 64, so a wave holds one light) and writes out[3 t ..], t =
+// light * P_pad + q: the sum over the light's samples of intensity *
+// transmittance (Renderer.h:609-643).
 template <typename T>
-__global__ void __launch_bounds__(256) k_render_shadow(lgd::GridView g, RenderConsts c, lgl::DevLights L,
+__global__ void __launch_bounds__(256, 4) k_render_shadow(lgd::GridView g, RenderConsts c, lgl::DevLights L,
                                                        uint32_t k0, uint32_t k1, uint32_t ek0, uint32_t ek1,
-                                                       const ShadePoint *__restrict__ pts, uint32_t P,
+                                                       const ShadePoint *__restrict__ pts,
+                                                       const uint32_t *__restrict__ order, uint32_t P,
                                                        uint32_t P_pad, double *__restrict__ out,
                                                        unsigned long long *__restrict__ stats) {
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    const int li = __builtin_amdgcn_readfirstlane((int)(t / P_pad));
-    const uint32_t q = t - (uint32_t)li * P_pad;
+    const uint32_t tid = blockIdx.x * 256u + threadIdx.x;
+    const int li = __builtin_amdgcn_readfirstlane((int)(tid / P_pad));
+    const uint32_t k = tid - (uint32_t)li * P_pad;
     uint32_t rays = 0, smp = 0;
-    if (li < L.n && q < P) {
+    if (li < L.n && k < P) {
+        const uint32_t q = order[k];
+        const uint64_t t = (uint64_t)li * P_pad + q;
         const lgl::DevLight &lt = L.l[li];
         const ShadePoint sp = pts[q];
         const uint32_t pix = sp.ray / (uint32_t)c.samples;
-        const uint32_t s = sp.ray - pix * (uint32_t)c.samples;
-        const uint32_t tag = shadow_tag(s, li);
-        lgd::Stream rng, env;
+        const uint32_t tag = shadow_tag(sp.ray - pix * (uint32_t)c.samples, li);
+        lgd::Stream rng;
         rng.init(pix, sp.ord, lgd::DOM_RENDER_SHADOW | tag, ((uint64_t)k1 << 32) | k0);
-        env.init(pix, sp.ord, lgd::DOM_RENDER_ENV | tag, ((uint64_t)ek1 << 32) | ek0);
         double tl[3] = {0, 0, 0};
         for (int ls = 0; ls < lt.samples; ++ls) {
             double ldir[3], inten[3], dist;
-            lgl::illuminate(lt, sp.wp, env, ldir, inten, dist);
+            {
+                // an env sample draws 2 values: sample ls starts at draw 2 ls
+                lgd::Stream env;
+                env.init(pix, sp.ord, lgd::DOM_RENDER_ENV | tag, ((uint64_t)ek1 << 32) | ek0);
+                if (lt.type == LG_LIGHT_ENV && ls > 0) env.skip(2u * (uint32_t)ls);
+                lgl::illuminate(lt, sp.wp, env, ldir, inten, dist);
+            }
             double tau_s;
             bool sdrop;
             ++rays;
@@ -581,77 +560,141 @@ __global__ void __launch_bounds__(256) k_render_shadow(lgd::GridView g, RenderCo
             if (sdrop) continue;
             for (int a = 0; a < 3; ++a) tl[a] += inten[a] * (double)expf((float)(c.ext[a] * tau_s));
         }
-        double *o = out + 3ull * t;
-        for (int a = 0; a < 3; ++a)
-            o[a] = c.albedo[a] * (tl[a] / (double)lt.samples) * (double)sp.tf[a] * (double)sp.om[a];
+        double *o = out + 3 * t;
+        for (int a = 0; a < 3; ++a) o[a] = tl[a];
     }
     if (stats) {
         render_stat(stats, RS_SHADOW_RAYS, rays);
         render_stat(stats, RS_SHADOW_SAMPLES, smp);
+        render_stat_wave_max(stats, RS_SHADOW_LANE_SLOTS, smp);
     }
 }
 
-// Phase combine over pixels [pix0, pix0 + npix) (Renderer.h:670-689).
-__global__ void __launch_bounds__(256) k_render_combine(RenderConsts c, int n_lights, uint32_t pix0, uint32_t npix,
-                                                        const int *__restrict__ cnt, const int *__restrict__ off,
-                                                        const RayAcc *__restrict__ acc,
-                                                        const double *__restrict__ direct_terms, uint32_t P_pad,
-                                                        float4 *__restrict__ rgb, float4 *__restrict__ direct,
-                                                        float4 *__restrict__ scatter, float4 *__restrict__ emission) {
-    const uint32_t pl = blockIdx.x * 256u + threadIdx.x;
-    if (pl >= npix) return;
-    const uint32_t pix = pix0 + pl;
-    const uint32_t ray0 = pix0 * (uint32_t)c.samples;
-    double lumAA[3] = {0, 0, 0}, dirAA[3] = {0, 0, 0}, scaAA[3] = {0, 0, 0}, emiAA[3] = {0, 0, 0};
-    double transAA[3] = {0, 0, 0};
-    for (int s = 0; s < c.samples; ++s) {
-        const uint32_t r = pix * (uint32_t)c.samples + (uint32_t)s;
-        const uint32_t rl = r - ray0;
-        const RayAcc a = acc[rl];
-        double lum[3] = {0, 0, 0}, dl[3] = {0, 0, 0};
-        if (n_lights > 0) {
+// Phase combine over rays [ray0, ray0 + nrays) (Renderer.h:580-668).
+template <typename T>
+__global__ void __launch_bounds__(256) k_render_combine(lgd::GridView g, lgd::GridView tg, int has_temp,
+                                                        CoarseView sc, int has_scatter, RenderConsts c,
+                                                        lgl::DevLights L, uint32_t k0, uint32_t k1, uint32_t ray0,
+                                                        uint32_t nrays, const int *__restrict__ cnt,
+                                                        const int *__restrict__ off, const double *__restrict__ sums,
+                                                        uint32_t P_pad, RayOut *__restrict__ rout,
+                                                        unsigned long long *__restrict__ stats) {
+    const uint32_t rl = blockIdx.x * 256u + threadIdx.x;
+    uint32_t cam = 0, smp = 0, npts = 0;
+    if (rl < nrays) {
+        const uint32_t r = ray0 + rl;
+        uint32_t pix, s;
+        int i, j;
+        ray_pixel(c, r, pix, s, i, j);
+        lgd::Stream rng;
+        rng.init(pix, s, lgd::DOM_RENDER, ((uint64_t)k1 << 32) | k0);
+        RayOut o{};
+        o.tr[0] = o.tr[1] = o.tr[2] = 1.0;
+        lgd::Isect pr;
+        cam = 1;
+        if (camera_ray(g, c, i, j, rng, pr)) {
             const int q0 = off[rl], qn = cnt[r];
-            for (int q = q0; q < q0 + qn; ++q)
-                for (int li = 0; li < n_lights; ++li) {
-                    const double *v = direct_terms + 3ull * ((uint64_t)li * P_pad + (uint32_t)q);
-                    for (int k = 0; k < 3; ++k) {
-                        lum[k] += v[k];
-                        dl[k] += v[k];
+            int ord = 0;
+            const double tau = primary_march<T>(g, c, pr, rng, &smp, [&](const double *ip, double pd, double step,
+                                                                         double tau_before) {
+                const double wp[3] = {g.origin[0] + g.voxel * ip[0], g.origin[1] + g.voxel * ip[1],
+                                      g.origin[2] + g.voxel * ip[2]};
+                double tf[3], om[3];
+                for (int k = 0; k < 3; ++k) {
+                    tf[k] = (double)expf((float)(c.ext[k] * tau_before));
+                    om[k] = (double)(-expm1f((float)(c.ext[k] * pd * step)));
+                }
+                if (ord < qn) {
+                    for (int li = 0; li < L.n; ++li) {
+                        const double *tl = sums + 3ull * ((uint64_t)li * P_pad + (uint32_t)(q0 + ord));
+                        const double ns = (double)L.l[li].samples;
+                        for (int k = 0; k < 3; ++k) {
+                            const double v = c.albedo[k] * (tl[k] / ns) * tf[k] * om[k];
+                            o.lum[k] += v;
+                            o.dl[k] += v;
+                        }
                     }
                 }
+                ++ord;
+                double msc[3] = {0, 0, 0};
+                if (has_scatter) {
+                    float sv[3];
+                    coarse_sample(sc, wp[0] * sc.inv_cv, wp[1] * sc.inv_cv, wp[2] * sc.inv_cv, sv);
+                    const double pc = pd < 0.0 ? 0.0 : (pd > 1.0 ? 1.0 : pd);
+                    const double mask = (double)powf((float)pc, (float)c.mask_pow);
+                    for (int k = 0; k < 3; ++k) msc[k] = (double)sv[k] * c.inv_scat[k] * mask * c.scatter_scale;
+                }
+                double emi[3] = {0, 0, 0};
+                if (c.ramp_n > 0) {
+                    double temp = 0.0;
+                    if (has_temp) temp = (double)lgd::trilinear<float>(tg, ip[0], ip[1], ip[2]);
+                    render_ramp(c, temp, emi);
+                    for (int k = 0; k < 3; ++k) emi[k] *= c.emission_scale;
+                }
+                for (int k = 0; k < 3; ++k) {
+                    const double ts = msc[k] * tf[k] * om[k];
+                    const double te = emi[k] * tf[k] * om[k];
+                    o.sl[k] += ts;
+                    o.el[k] += te;
+                    o.lum[k] += ts + te;
+                }
+            });
+            npts = (uint32_t)ord;
+            for (int k = 0; k < 3; ++k) o.tr[k] = (double)expf((float)(c.ext[k] * tau));  // kept at dropout
         }
+        rout[rl] = o;
+    }
+    if (stats) {
+        render_stat(stats, RS_CAMERA_RAYS, cam);
+        render_stat(stats, RS_PRIMARY_SAMPLES, smp);
+        render_stat(stats, RS_SHADING_POINTS, npts);
+    }
+}
+
+// Phase aa over pixels [pix0, pix0 + npix): AA samples summed in order.
+__global__ void __launch_bounds__(256) k_render_aa(RenderConsts c, uint32_t pix0, uint32_t npix,
+                                                   const RayOut *__restrict__ rout, float4 *__restrict__ rgb,
+                                                   float4 *__restrict__ direct, float4 *__restrict__ scatter,
+                                                   float4 *__restrict__ emission) {
+    const uint32_t pl = blockIdx.x * 256u + threadIdx.x;
+    if (pl >= npix) return;
+    double lum[3] = {0, 0, 0}, dl[3] = {0, 0, 0}, sl[3] = {0, 0, 0}, el[3] = {0, 0, 0}, tr[3] = {0, 0, 0};
+    const RayOut *ro = rout + (uint64_t)pl * (uint32_t)c.samples;
+    for (int s = 0; s < c.samples; ++s) {
+        const RayOut &o = ro[s];
         for (int k = 0; k < 3; ++k) {
-            lum[k] += a.sl[k] + a.el[k];
-            lumAA[k] += lum[k] * c.ratio_aa;
-            dirAA[k] += dl[k] * c.ratio_aa;
-            scaAA[k] += a.sl[k] * c.ratio_aa;
-            emiAA[k] += a.el[k] * c.ratio_aa;
-            transAA[k] += a.tr[k] * c.ratio_aa;
+            lum[k] += o.lum[k] * c.ratio_aa;
+            dl[k] += o.dl[k] * c.ratio_aa;
+            sl[k] += o.sl[k] * c.ratio_aa;
+            el[k] += o.el[k] * c.ratio_aa;
+            tr[k] += o.tr[k] * c.ratio_aa;
         }
     }
-    const float alpha = (float)(1.0f - (transAA[0] + transAA[1] + transAA[2]) / 3.0f);
-    if (rgb) rgb[pix] = make_float4((float)lumAA[0], (float)lumAA[1], (float)lumAA[2], alpha);
-    if (direct) direct[pix] = make_float4((float)dirAA[0], (float)dirAA[1], (float)dirAA[2], 0.0f);
-    if (scatter) scatter[pix] = make_float4((float)scaAA[0], (float)scaAA[1], (float)scaAA[2], 0.0f);
-    if (emission) emission[pix] = make_float4((float)emiAA[0], (float)emiAA[1], (float)emiAA[2], 0.0f);
+    const uint32_t pix = pix0 + pl;
+    const float alpha = (float)(1.0f - (tr[0] + tr[1] + tr[2]) / 3.0f);
+    if (rgb) rgb[pix] = make_float4((float)lum[0], (float)lum[1], (float)lum[2], alpha);
+    if (direct) direct[pix] = make_float4((float)dl[0], (float)dl[1], (float)dl[2], 0.0f);
+    if (scatter) scatter[pix] = make_float4((float)sl[0], (float)sl[1], (float)sl[2], 0.0f);
+    if (emission) emission[pix] = make_float4((float)el[0], (float)el[1], (float)el[2], 0.0f);
 }
 
 // Render workspace.  Slot 0 holds the shading-point count of every ray and the
-// per-block sums; slot 1 holds one chunk: ray offsets, ray sums, scan storage,
-// shading points and their direct terms.  A chunk is a run of whole blocks of
-// 64 pixels with at most kRenderRayCap rays and about kRenderPointBytes of
-// shading-point data (a single block may exceed the latter).
+// per-block sums; slot 1 holds one chunk: ray offsets, scan storage, ray
+// outputs, shading points, their sort keys and order, sort storage and their
+// light sums.  A chunk is a run of whole
+// blocks of 64 pixels with at most kRenderRayCap rays and about
+// kRenderPointBytes of shading-point data (a single block may exceed the latter).
 constexpr uint32_t kRenderBlockPixels = 64;
 constexpr uint64_t kRenderRayCap = 1u << 20;
-constexpr uint64_t kRenderPointBytes = 256ull << 20;
+constexpr uint64_t kRenderPointBytes = 512ull << 20;
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 struct RenderLayout {
     uint64_t npix, nrays, nblocks, rays_per_block, ray_cap, point_cap;
-    size_t bsum_off, head_bytes;
-    size_t acc_off, scan_off, scan_bytes, ray_area_bytes;
-    size_t point_bytes;  // per shading point, record and direct terms
+    size_t bsum_off, head_bytes;                          // slot 0
+    size_t scan_off, scan_bytes, out_off, ray_area_bytes;  // slot 1, before the points
+    size_t point_bytes;                                   // per shading point: record and light sums
 };
 
 inline RenderLayout render_layout(int w, int h, int samples, int n_lights) {
@@ -661,7 +704,7 @@ inline RenderLayout render_layout(int w, int h, int samples, int n_lights) {
     l.nblocks = (l.npix + kRenderBlockPixels - 1) / kRenderBlockPixels;
     l.rays_per_block = (uint64_t)kRenderBlockPixels * (uint64_t)samples;
     l.ray_cap = std::min(std::max(kRenderRayCap, l.rays_per_block), l.nblocks * l.rays_per_block);
-    l.point_bytes = sizeof(ShadePoint) + 24ull * (uint64_t)n_lights;
+    l.point_bytes = sizeof(ShadePoint) + 12 + 24ull * (uint64_t)n_lights;
     l.point_cap = n_lights > 0 ? kRenderPointBytes / l.point_bytes : 0;
     l.bsum_off = align256(4 * l.nrays);
     l.head_bytes = l.bsum_off + align256(8 * l.nblocks);
@@ -669,9 +712,9 @@ inline RenderLayout render_layout(int w, int h, int samples, int n_lights) {
     (void)rocprim::exclusive_scan(nullptr, sb, (const int *)nullptr, (int *)nullptr, 0, (size_t)l.ray_cap,
                                   rocprim::plus<int>());
     l.scan_bytes = sb;
-    l.acc_off = align256(4 * l.ray_cap);
-    l.scan_off = l.acc_off + align256(sizeof(RayAcc) * l.ray_cap);
-    l.ray_area_bytes = l.scan_off + align256(sb);
+    l.scan_off = align256(4 * l.ray_cap);
+    l.out_off = l.scan_off + align256(sb);
+    l.ray_area_bytes = l.out_off + align256(sizeof(RayOut) * l.ray_cap);
     return l;
 }
 
@@ -1054,7 +1097,7 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
     ktimer_begin(ctx, "k_render_count", &ev);
     for_dtype_r(density->desc.dtype, [&](auto tag) {
         using T = decltype(tag);
-        k_render_count<T><<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(g, c, L.n, u0, u1, nrays, cnt, stats);
+        k_render_count<T><<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(g, c, L.n, u0, u1, nrays, cnt);
     });
     LG_LAUNCHED("k_render_count");
     ktimer_end(ctx, "k_render_count", ev);
@@ -1091,50 +1134,72 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
                  "lg_render: %llu shading points in one block of %u pixels", (unsigned long long)max_points,
                  kRenderBlockPixels);
     const uint64_t pad_cap = (max_points + 63) & ~63ull;
-    const size_t ray_bytes = lay.ray_area_bytes;
-    const size_t pts_off = ray_bytes;
-    const size_t dir_off = pts_off + align256(sizeof(ShadePoint) * max_points);
-    const size_t body_bytes = dir_off + align256(24ull * (uint64_t)L.n * pad_cap);
+    size_t sort_bytes = 0;
+    LG_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                     rocprim::counting_iterator<uint32_t>(0), (uint32_t *)nullptr,
+                                     (size_t)std::max<uint64_t>(max_points, 1), 0, kOrdKeyBits));
+    const size_t pts_off = lay.ray_area_bytes;
+    const size_t key_off = pts_off + align256(sizeof(ShadePoint) * max_points);
+    const size_t keys_off = key_off + align256(4 * max_points);
+    const size_t ord_off = keys_off + align256(4 * max_points);
+    const size_t stmp_off = ord_off + align256(4 * max_points);
+    const size_t sum_off = stmp_off + align256(sort_bytes);
+    const size_t body_bytes = sum_off + align256(24ull * (uint64_t)L.n * pad_cap);
     void *body = nullptr;
     LG_TRY(scratch(ctx, body_bytes, &body, 1));
     int *off = (int *)body;
-    RayAcc *acc = (RayAcc *)((char *)body + lay.acc_off);
     void *scan_tmp = (char *)body + lay.scan_off;
+    RayOut *rout = (RayOut *)((char *)body + lay.out_off);
     ShadePoint *pts = (ShadePoint *)((char *)body + pts_off);
-    double *dterms = (double *)((char *)body + dir_off);
+    uint32_t *keys = (uint32_t *)((char *)body + key_off);
+    uint32_t *keys_sorted = (uint32_t *)((char *)body + keys_off);
+    uint32_t *order = (uint32_t *)((char *)body + ord_off);
+    void *sort_tmp = (char *)body + stmp_off;
+    double *sums = (double *)((char *)body + sum_off);
     for (const Chunk &ch : chunks) {
         const uint32_t pix0 = ch.b0 * kRenderBlockPixels;
         const uint32_t pix1 = std::min<uint64_t>((uint64_t)ch.b1 * kRenderBlockPixels, lay.npix);
         const uint32_t ray0 = pix0 * (uint32_t)samples, nr = (pix1 - pix0) * (uint32_t)samples;
         const uint32_t P = (uint32_t)ch.points, P_pad = (P + 63u) & ~63u;
-        size_t tb = lay.scan_bytes;
-        LG_HIP(rocprim::exclusive_scan(scan_tmp, tb, cnt + ray0, off, 0, (size_t)nr, rocprim::plus<int>(),
-                                       ctx->stream));
-        ktimer_begin(ctx, "k_render_fill", &ev);
-        for_dtype_r(density->desc.dtype, [&](auto tag) {
-            using T = decltype(tag);
-            k_render_fill<T><<<(nr + 255) / 256, 256, 0, ctx->stream>>>(g, tg, temperature ? 1 : 0, sc, has_sc, c, u0,
-                                                                        u1, ray0, nr, cnt, off, pts, acc, stats);
-        });
-        LG_LAUNCHED("k_render_fill");
-        ktimer_end(ctx, "k_render_fill", ev);
-        if (P > 0 && L.n > 0) {
+        if (P > 0) {
+            size_t tb = lay.scan_bytes;
+            LG_HIP(rocprim::exclusive_scan(scan_tmp, tb, cnt + ray0, off, 0, (size_t)nr, rocprim::plus<int>(),
+                                           ctx->stream));
+            ktimer_begin(ctx, "k_render_fill", &ev);
+            for_dtype_r(density->desc.dtype, [&](auto tag) {
+                using T = decltype(tag);
+                k_render_fill<T><<<(nr + 255) / 256, 256, 0, ctx->stream>>>(g, c, u0, u1, ray0, nr, cnt, off, pts,
+                                                                            keys);
+            });
+            LG_LAUNCHED("k_render_fill");
+            ktimer_end(ctx, "k_render_fill", ev);
+            size_t sb = sort_bytes;
+            ktimer_begin(ctx, "k_render_sort", &ev);
+            LG_HIP(rocprim::radix_sort_pairs(sort_tmp, sb, keys, keys_sorted, rocprim::counting_iterator<uint32_t>(0),
+                                             order, (size_t)P, 0, kOrdKeyBits, ctx->stream));
+            ktimer_end(ctx, "k_render_sort", ev);
             const uint64_t tasks = (uint64_t)L.n * P_pad;
             ktimer_begin(ctx, "k_render_shadow", &ev);
             for_dtype_r(density->desc.dtype, [&](auto tag) {
                 using T = decltype(tag);
                 k_render_shadow<T><<<(unsigned)((tasks + 255) / 256), 256, 0, ctx->stream>>>(
-                    g, c, L, u0, u1, e0, e1, pts, P, P_pad, dterms, stats);
+                    g, c, L, u0, u1, e0, e1, pts, order, P, P_pad, sums, stats);
             });
             LG_LAUNCHED("k_render_shadow");
             ktimer_end(ctx, "k_render_shadow", ev);
         }
         ktimer_begin(ctx, "k_render_combine", &ev);
-        k_render_combine<<<(pix1 - pix0 + 255) / 256, 256, 0, ctx->stream>>>(
-            c, L.n, pix0, pix1 - pix0, cnt, off, acc, dterms, P_pad, (float4 *)films->d_rgb, (float4 *)films->d_direct,
-            (float4 *)films->d_scatter, (float4 *)films->d_emission);
+        for_dtype_r(density->desc.dtype, [&](auto tag) {
+            using T = decltype(tag);
+            k_render_combine<T><<<(nr + 255) / 256, 256, 0, ctx->stream>>>(
+                g, tg, temperature ? 1 : 0, sc, has_sc, c, L, u0, u1, ray0, nr, cnt, off, sums, P_pad, rout, stats);
+        });
         LG_LAUNCHED("k_render_combine");
         ktimer_end(ctx, "k_render_combine", ev);
+        k_render_aa<<<(pix1 - pix0 + 255) / 256, 256, 0, ctx->stream>>>(
+            c, pix0, pix1 - pix0, rout, (float4 *)films->d_rgb, (float4 *)films->d_direct, (float4 *)films->d_scatter,
+            (float4 *)films->d_emission);
+        LG_LAUNCHED("k_render_aa");
     }
     ctx->render_chunks = chunks.size();
     return LG_OK;
@@ -1157,14 +1222,26 @@ int32_t lg_render_get_stats(lg_ctx *ctx, lg_render_stats *out) {
     out->shading_points = v[RS_SHADING_POINTS];
     out->shadow_rays = v[RS_SHADOW_RAYS];
     out->shadow_samples = v[RS_SHADOW_SAMPLES];
+    out->shadow_lane_slots = v[RS_SHADOW_LANE_SLOTS];
     return LG_OK;
 }
 
-int32_t lg_render_workspace(int32_t width, int32_t height, int32_t samples, int32_t n_lights, uint64_t *bytes) {
+int32_t lg_render_workspace(int32_t width, int32_t height, int32_t samples, int32_t n_lights,
+                            int64_t max_points_per_ray, uint64_t *bytes) {
     LG_CHECK_ARG(bytes && width > 0 && height > 0 && samples >= 1 && n_lights >= 0 && n_lights <= LG_MAX_LIGHTS,
                  "lg_render_workspace: bad argument");
     const RenderLayout l = render_layout(width, height, samples, n_lights);
-    *bytes = l.head_bytes + l.ray_area_bytes + align256(l.point_cap * l.point_bytes) + 512;
+    uint64_t pts = l.point_cap;
+    if (max_points_per_ray > 0) {
+        const uint64_t chunk_rays = std::max<uint64_t>(l.ray_cap, l.rays_per_block);
+        pts = std::min<uint64_t>(pts, chunk_rays * (uint64_t)max_points_per_ray);
+    }
+    size_t sort_bytes = 0;
+    if (pts > 0)
+        (void)rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                        rocprim::counting_iterator<uint32_t>(0), (uint32_t *)nullptr, (size_t)pts,
+                                        0, kOrdKeyBits);
+    *bytes = l.head_bytes + l.ray_area_bytes + align256(pts * l.point_bytes) + align256(sort_bytes) + 2048;
     return LG_OK;
 }
 

@@ -68,9 +68,12 @@ uint64_t job_bytes(const lg_job &j) {
     const uint64_t nc = (uint64_t)(j.scene.n / std::max(j.scatter_res_factor, 1)) + 6;
     const uint64_t scatter = nc * nc * nc * 13;
     const uint64_t films = (uint64_t)j.camera.width * (uint64_t)j.camera.height * 64;
+    // a camera ray crosses at most sqrt(3) n voxels; jittered steps are >= 0.75 h
+    const double h = j.march.step[0] > 0 ? j.march.step[0] : 1.0;
+    const int64_t per_ray = (int64_t)std::ceil(1.7320508075688772 * (double)j.scene.n / (0.75 * h)) + 2;
     uint64_t render = 0;
     if (lg_render_workspace(std::max(j.camera.width, 1), std::max(j.camera.height, 1), std::max(j.samples, 1),
-                            std::min(std::max(j.n_lights, 0), LG_MAX_LIGHTS), &render) != LG_OK)
+                            std::min(std::max(j.n_lights, 0), LG_MAX_LIGHTS), per_ray, &render) != LG_OK)
         render = 0;
     const uint64_t big = V / 4 + 128;
     const uint64_t solver = V * 2048 + big * big * 4;

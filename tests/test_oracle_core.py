@@ -94,6 +94,78 @@ def test_spans_diagonal_single_brick_time(oracle):
     np.testing.assert_allclose(s, [[np.sqrt(3.0), 9 * np.sqrt(3.0)]], rtol=1e-12)
 
 
+def _brute_spans(occ, eye, d, lo, hi):
+    """Runs of existing 8^3 bricks along eye + t d, from the plane crossings: clip
+    to [lo, hi], cut at every brick plane, classify each piece by its midpoint."""
+    t0, t1 = 1e-9, np.inf
+    for a in range(3):
+        if abs(d[a]) < 1e-15:
+            if not lo[a] <= eye[a] <= hi[a]:
+                return []
+            continue
+        u, v = sorted(((lo[a] - eye[a]) / d[a], (hi[a] - eye[a]) / d[a]))
+        t0, t1 = max(t0, u), min(t1, v)
+    if t0 > t1:
+        return []
+    cuts = [t0, t1]
+    for a in range(3):
+        if abs(d[a]) < 1e-15:
+            continue
+        for k in range(int(lo[a] // 8), int(hi[a] // 8) + 1):
+            t = (8.0 * k - eye[a]) / d[a]
+            if t0 < t < t1:
+                cuts.append(t)
+    cuts = sorted(cuts)
+    out = []
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        if b - a < 1e-12:
+            continue
+        m = eye + d * (0.5 * (a + b))
+        bi = np.floor(m / 8.0).astype(int)
+        on = bool(np.all(bi >= 0) and np.all(bi < occ.shape[::-1]) and occ[bi[2], bi[1], bi[0]])
+        if on and out and abs(out[-1][1] - a) < 1e-9:
+            out[-1][1] = b
+        elif on:
+            out.append([a, b])
+    return out
+
+
+def test_spans_match_plane_crossings(oracle):
+    """The brick DDA gives the runs of existing bricks that the brick planes
+    define, for random rays over a 96^3 grid whose 32^3 regions are empty, full
+    or a random mix of bricks."""
+    rng = np.random.default_rng(11)
+    nb = 12
+    occ = np.zeros((nb, nb, nb), bool)
+    kinds = rng.integers(0, 3, size=(3, 3, 3))
+    for z in range(3):
+        for y in range(3):
+            for x in range(3):
+                blk = (slice(4 * z, 4 * z + 4), slice(4 * y, 4 * y + 4), slice(4 * x, 4 * x + 4))
+                occ[blk] = [False, True, None][kinds[z, y, x]] if kinds[z, y, x] < 2 else rng.random((4, 4, 4)) < 0.5
+    assert {0, 1, 2} <= set(kinds.ravel().tolist())
+    g = np.zeros((8 * nb,) * 3, np.float32)
+    for bz, by, bx in zip(*np.nonzero(occ)):
+        g[8 * bz + 3, 8 * by + 3, 8 * bx + 3] = 1.0
+    bz, by, bx = np.nonzero(occ)
+    lo = 8.0 * np.array([bx.min(), by.min(), bz.min()])
+    hi = 8.0 * (np.array([bx.max(), by.max(), bz.max()]) + 1)
+    n_checked = 0
+    for _ in range(300):
+        eye = rng.uniform(-40, 8 * nb + 40, 3)
+        target = rng.uniform(0, 8 * nb, 3)
+        d = target - eye
+        d /= np.linalg.norm(d)
+        want = _brute_spans(occ, eye, d, lo, hi)
+        got = oracle.spans(g, eye, d)
+        got = [] if got is None else got.tolist()
+        assert len(got) == len(want), (eye, d, got, want)
+        if want:
+            np.testing.assert_allclose(np.array(got), np.array(want), rtol=0, atol=1e-6)
+            n_checked += 1
+    assert n_checked > 100
+
+
 # ------------------------------------------------------------ marches --------
 def _shader(lga, **kw):
     return lga.default_shader(**kw)
