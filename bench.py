@@ -49,6 +49,7 @@ ROOF = {
     "k_comp_small": ("valu", "small_updates", 2.0),
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+RENDER_KERNELS = ("k_render_count", "k_render_fill", "k_render_sort", "k_render_shadow", "k_render_combine")
 
 WORKLOAD = {"workload": "c2_plain_solve", "volume": "512^3 blobs20+fBm6, ~30% active, fp32 bricks",
             "points": 8192, "edge_radius": 12.0, "max_connect": 12, "edge_step": 0.5,
@@ -166,13 +167,36 @@ def bench_render(lga, ctx, grid, shader, lights, solver, n, reps):
     ms = ctx.elapsed_ms(5, 6) / reps
     img = films.read()
     alpha_mean = float(img[0][..., 3].mean())
+    # one more render with kernel timing and the device work counters on
+    ctx.kernel_time_reset()
+    ctx.kernel_timing(True)
+    lga.render(ctx, grid, sg, None, cam, shader, lights, rp, spp, films, seed=1)
+    ctx.sync()
+    ctx.kernel_timing(False)
+    st = lga.render_stats(ctx)
+    phases = {}
+    for nm in RENDER_KERNELS:
+        launches, tot = ctx.kernel_time(nm)
+        if launches:
+            phases[nm] = {"launches": launches, "ms": round(tot, 3)}
     films.free()
     sg.destroy()
     pb.free()
     rb.free()
-    return {"workload": "c2_render", "width": w, "height": h, "spp": spp, "ms": round(ms, 3),
-            "camera_rays_per_s": round(w * h * spp / (ms * 1e-3), 1), "splat_ms": round(splat_ms, 4),
-            "primary_step": 1.0, "shadow_step": 3.0, "reps": reps, "mean_alpha": round(alpha_mean, 4)}
+    out = {"workload": "c2_render", "width": w, "height": h, "spp": spp, "ms": round(ms, 3),
+           "camera_rays_per_s": round(w * h * spp / (ms * 1e-3), 1), "splat_ms": round(splat_ms, 4),
+           "primary_step": 1.0, "shadow_step": 3.0, "reps": reps, "mean_alpha": round(alpha_mean, 4),
+           "phases": phases, "work": {k: v for k, v in st.items() if k != "shadow_lane_slots"},
+           "shadow_lane_efficiency": round(st["shadow_samples"] / max(st["shadow_lane_slots"], 1), 4)}
+    sh_ms = phases.get("k_render_shadow", {}).get("ms")
+    if sh_ms:
+        # the shadow phase: trilinear samples of the shadow marches (32 ops each, as k_light_rays)
+        achieved = st["shadow_samples"] * 32.0 / (sh_ms * 1e-3) / 1e12
+        out["roofline"] = {"kernel": "k_render_shadow", "bound": "valu", "achieved": round(achieved, 3),
+                           "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                           "traffic": None, "work_unit": "shadow_samples", "ops_per_unit": 32.0,
+                           "shadow_samples_per_s": round(st["shadow_samples"] / (sh_ms * 1e-3), 1)}
+    return out
 
 
 def bench_fw_dense(lga, ctx, solver, n):
