@@ -49,7 +49,7 @@ ROOF = {
     "k_comp_small": ("valu", "small_updates", 2.0),
 }
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
-RENDER_KERNELS = ("k_render_count", "k_render_fill", "k_render_sort", "k_render_shadow", "k_render_combine")
+RENDER_KERNELS = ("k_render_count", "k_render_fill", "k_render_shadow", "k_render_combine")
 
 WORKLOAD = {"workload": "c2_plain_solve", "volume": "512^3 blobs20+fBm6, ~30% active, fp32 bricks",
             "points": 8192, "edge_radius": 12.0, "max_connect": 12, "edge_step": 0.5,

@@ -7,9 +7,7 @@
 #include <unordered_set>
 #include <vector>
 
-#include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
-#include <rocprim/iterator/counting_iterator.hpp>
 
 #include "lg_internal.h"
 #include "lg_lights.h"
@@ -320,8 +318,11 @@ __device__ __forceinline__ void render_ramp(const RenderConsts &c, double x, dou
 //           (Renderer.h:644-666), into the ray's four AOVs and transmittance;
 //   aa      one lane per pixel sums its rays (Renderer.h:670-689).
 // The shadow rays of a shading point draw from their own streams, so the
-// primary march does not wait on them and the shadow phase runs with
-// neighbouring lanes on neighbouring points of one light.
+// primary march does not wait on them.  The points of each group of 64 rays
+// (one wave of fill and combine) are stored together, ordinal-major: the k-th
+// points of the group's rays in ray order, then the (k+1)-th, ...  A shadow wave
+// then holds points at the same depth of neighbouring rays, whose shadow rays
+// are alike, and reads and writes its points in order.
 struct ShadePoint {
     double wp[3];  // world position
     uint32_t ray;  // r = pixel * samples + sample
@@ -445,22 +446,64 @@ __global__ void __launch_bounds__(256) k_render_block_sum(const int *__restrict_
     if (lane == 0) bsum[b] = v;
 }
 
-// Phase fill over rays [ray0, ray0 + nrays): shading point q0 + k of ray r is
-// its k-th, q0 = off[r - ray0].  keys[q] = the point's ordinal (capped): the
-// shadow phase visits points in (ordinal, ray) order, so that a wave holds
-// points at the same depth of neighbouring rays, whose shadow rays are alike.
-constexpr uint32_t kOrdKeyMax = 4095;
-constexpr int kOrdKeyBits = 12;
+// Position of the k-th shading point of lane j's ray within its group of 64
+// rays (counts c_i): S(k) + #{i < j : c_i > k}, S(k) = sum_i min(c_i, k), the
+// points of ordinals below k.  A per-wave table holds S(k) and the ballot of
+// c_i > k for k < kOrdTable; deeper ordinals are counted from the c_i.
+constexpr int kOrdTable = 256;
+struct OrdTable {
+    unsigned long long mask[kOrdTable];
+    int S[kOrdTable];
+    int c[64];
+};
 
+// All 64 lanes of the wave call this with their count (0 for lanes past the
+// last ray); the caller synchronises before ord_pos reads the table.
+__device__ __forceinline__ void ord_table_build(OrdTable &t, int c) {
+    const int lane = threadIdx.x & 63;
+    int maxc = c;
+    for (int o = 32; o > 0; o >>= 1) maxc = max(maxc, __shfl_xor(maxc, o, 64));
+    t.c[lane] = c;
+    const int kt = maxc < kOrdTable ? maxc : kOrdTable;
+    int S = 0;
+    for (int k = 0; k < kt; ++k) {
+        const unsigned long long m = __ballot(c > k);
+        if (lane == 0) {
+            t.mask[k] = m;
+            t.S[k] = S;
+        }
+        S += __popcll(m);
+    }
+}
+
+__device__ __forceinline__ int ord_pos(const OrdTable &t, int k) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (k < kOrdTable) return t.S[k] + __popcll(t.mask[k] & below);
+    int S = 0, rank = 0;
+    for (int i = 0; i < 64; ++i) {
+        const int ci = t.c[i];
+        S += ci < k ? ci : k;
+        rank += (i < lane && ci > k) ? 1 : 0;
+    }
+    return S + rank;
+}
+
+// Phase fill over rays [ray0, ray0 + nrays): the k-th shading point of ray r
+// goes to group base + ord_pos(k), the group base being off[] of its first ray.
 template <typename T>
 __global__ void __launch_bounds__(256) k_render_fill(lgd::GridView g, RenderConsts c, uint32_t k0, uint32_t k1,
                                                      uint32_t ray0, uint32_t nrays, const int *__restrict__ cnt,
-                                                     const int *__restrict__ off, ShadePoint *__restrict__ pts,
-                                                     uint32_t *__restrict__ keys) {
+                                                     const int *__restrict__ off, ShadePoint *__restrict__ pts) {
+    __shared__ OrdTable tab[4];
     const uint32_t rl = blockIdx.x * 256u + threadIdx.x;
-    if (rl >= nrays) return;
     const uint32_t r = ray0 + rl;
-    const int qn = cnt[r];
+    const bool valid = rl < nrays;
+    const int qn = valid ? cnt[r] : 0;
+    const int base = __shfl(valid ? off[rl] : 0, 0, 64);
+    OrdTable &t = tab[threadIdx.x >> 6];
+    ord_table_build(t, qn);
+    __syncthreads();
     if (qn == 0) return;
     uint32_t pix, s;
     int i, j;
@@ -469,16 +512,13 @@ __global__ void __launch_bounds__(256) k_render_fill(lgd::GridView g, RenderCons
     rng.init(pix, s, lgd::DOM_RENDER, ((uint64_t)k1 << 32) | k0);
     lgd::Isect pr;
     if (!camera_ray(g, c, i, j, rng, pr)) return;
-    const int q0 = off[rl];
-    ShadePoint *out = pts + q0;
     uint32_t ord = 0, smp = 0;
     (void)primary_march<T>(g, c, pr, rng, &smp, [&](const double *ip, double, double, double) {
         if ((int)ord < qn) {
-            ShadePoint &p = out[ord];
+            ShadePoint &p = pts[base + ord_pos(t, (int)ord)];
             for (int k = 0; k < 3; ++k) p.wp[k] = g.origin[k] + g.voxel * ip[k];
             p.ray = r;
             p.ord = ord;
-            keys[q0 + (int)ord] = ord < kOrdKeyMax ? ord : kOrdKeyMax;
         }
         ++ord;
     });
@@ -519,24 +559,20 @@ __device__ __forceinline__ bool render_shadow(const lgd::GridView &g, const Rend
     return true;
 }
 
-// Phase shadow: lane light * P_pad + k takes point q = order[k] (P_pad a
-// multiple of 64, so a wave holds one light) and writes out[3 t ..], t =
-// light * P_pad + q: the sum over the light's samples of intensity *
-// transmittance (Renderer.h:609-643).
+// Phase shadow: task t = light * P_pad + point (P_pad a multiple of 64, so a
+// wave holds one light); out[3 t ..] = the sum over the light's samples of
+// intensity * transmittance (Renderer.h:609-643).
 template <typename T>
 __global__ void __launch_bounds__(256, 4) k_render_shadow(lgd::GridView g, RenderConsts c, lgl::DevLights L,
                                                        uint32_t k0, uint32_t k1, uint32_t ek0, uint32_t ek1,
-                                                       const ShadePoint *__restrict__ pts,
-                                                       const uint32_t *__restrict__ order, uint32_t P,
+                                                       const ShadePoint *__restrict__ pts, uint32_t P,
                                                        uint32_t P_pad, double *__restrict__ out,
                                                        unsigned long long *__restrict__ stats) {
-    const uint32_t tid = blockIdx.x * 256u + threadIdx.x;
-    const int li = __builtin_amdgcn_readfirstlane((int)(tid / P_pad));
-    const uint32_t k = tid - (uint32_t)li * P_pad;
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const int li = __builtin_amdgcn_readfirstlane((int)(t / P_pad));
+    const uint32_t q = t - (uint32_t)li * P_pad;
     uint32_t rays = 0, smp = 0;
-    if (li < L.n && k < P) {
-        const uint32_t q = order[k];
-        const uint64_t t = (uint64_t)li * P_pad + q;
+    if (li < L.n && q < P) {
         const lgl::DevLight &lt = L.l[li];
         const ShadePoint sp = pts[q];
         const uint32_t pix = sp.ray / (uint32_t)c.samples;
@@ -560,7 +596,7 @@ __global__ void __launch_bounds__(256, 4) k_render_shadow(lgd::GridView g, Rende
             if (sdrop) continue;
             for (int a = 0; a < 3; ++a) tl[a] += inten[a] * (double)expf((float)(c.ext[a] * tau_s));
         }
-        double *o = out + 3 * t;
+        double *o = out + 3ull * t;
         for (int a = 0; a < 3; ++a) o[a] = tl[a];
     }
     if (stats) {
@@ -572,16 +608,23 @@ __global__ void __launch_bounds__(256, 4) k_render_shadow(lgd::GridView g, Rende
 
 // Phase combine over rays [ray0, ray0 + nrays) (Renderer.h:580-668).
 template <typename T>
-__global__ void __launch_bounds__(256) k_render_combine(lgd::GridView g, lgd::GridView tg, int has_temp,
+__global__ void __launch_bounds__(256, 3) k_render_combine(lgd::GridView g, lgd::GridView tg, int has_temp,
                                                         CoarseView sc, int has_scatter, RenderConsts c,
                                                         lgl::DevLights L, uint32_t k0, uint32_t k1, uint32_t ray0,
                                                         uint32_t nrays, const int *__restrict__ cnt,
                                                         const int *__restrict__ off, const double *__restrict__ sums,
                                                         uint32_t P_pad, RayOut *__restrict__ rout,
                                                         unsigned long long *__restrict__ stats) {
+    __shared__ OrdTable tab[4];
     const uint32_t rl = blockIdx.x * 256u + threadIdx.x;
+    const bool valid = rl < nrays;
+    const int qn = valid ? cnt[ray0 + rl] : 0;
+    const int base = __shfl(valid ? off[rl] : 0, 0, 64);
+    OrdTable &tb = tab[threadIdx.x >> 6];
+    ord_table_build(tb, qn);
+    __syncthreads();
     uint32_t cam = 0, smp = 0, npts = 0;
-    if (rl < nrays) {
+    if (valid) {
         const uint32_t r = ray0 + rl;
         uint32_t pix, s;
         int i, j;
@@ -593,7 +636,6 @@ __global__ void __launch_bounds__(256) k_render_combine(lgd::GridView g, lgd::Gr
         lgd::Isect pr;
         cam = 1;
         if (camera_ray(g, c, i, j, rng, pr)) {
-            const int q0 = off[rl], qn = cnt[r];
             int ord = 0;
             const double tau = primary_march<T>(g, c, pr, rng, &smp, [&](const double *ip, double pd, double step,
                                                                          double tau_before) {
@@ -605,8 +647,9 @@ __global__ void __launch_bounds__(256) k_render_combine(lgd::GridView g, lgd::Gr
                     om[k] = (double)(-expm1f((float)(c.ext[k] * pd * step)));
                 }
                 if (ord < qn) {
+                    const uint32_t q = (uint32_t)(base + ord_pos(tb, ord));
                     for (int li = 0; li < L.n; ++li) {
-                        const double *tl = sums + 3ull * ((uint64_t)li * P_pad + (uint32_t)(q0 + ord));
+                        const double *tl = sums + 3ull * ((uint64_t)li * P_pad + q);
                         const double ns = (double)L.l[li].samples;
                         for (int k = 0; k < 3; ++k) {
                             const double v = c.albedo[k] * (tl[k] / ns) * tf[k] * om[k];
@@ -680,8 +723,7 @@ __global__ void __launch_bounds__(256) k_render_aa(RenderConsts c, uint32_t pix0
 
 // Render workspace.  Slot 0 holds the shading-point count of every ray and the
 // per-block sums; slot 1 holds one chunk: ray offsets, scan storage, ray
-// outputs, shading points, their sort keys and order, sort storage and their
-// light sums.  A chunk is a run of whole
+// outputs, shading points and their light sums.  A chunk is a run of whole
 // blocks of 64 pixels with at most kRenderRayCap rays and about
 // kRenderPointBytes of shading-point data (a single block may exceed the latter).
 constexpr uint32_t kRenderBlockPixels = 64;
@@ -704,7 +746,7 @@ inline RenderLayout render_layout(int w, int h, int samples, int n_lights) {
     l.nblocks = (l.npix + kRenderBlockPixels - 1) / kRenderBlockPixels;
     l.rays_per_block = (uint64_t)kRenderBlockPixels * (uint64_t)samples;
     l.ray_cap = std::min(std::max(kRenderRayCap, l.rays_per_block), l.nblocks * l.rays_per_block);
-    l.point_bytes = sizeof(ShadePoint) + 12 + 24ull * (uint64_t)n_lights;
+    l.point_bytes = sizeof(ShadePoint) + 24ull * (uint64_t)n_lights;
     l.point_cap = n_lights > 0 ? kRenderPointBytes / l.point_bytes : 0;
     l.bsum_off = align256(4 * l.nrays);
     l.head_bytes = l.bsum_off + align256(8 * l.nblocks);
@@ -1134,16 +1176,8 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
                  "lg_render: %llu shading points in one block of %u pixels", (unsigned long long)max_points,
                  kRenderBlockPixels);
     const uint64_t pad_cap = (max_points + 63) & ~63ull;
-    size_t sort_bytes = 0;
-    LG_HIP(rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                     rocprim::counting_iterator<uint32_t>(0), (uint32_t *)nullptr,
-                                     (size_t)std::max<uint64_t>(max_points, 1), 0, kOrdKeyBits));
     const size_t pts_off = lay.ray_area_bytes;
-    const size_t key_off = pts_off + align256(sizeof(ShadePoint) * max_points);
-    const size_t keys_off = key_off + align256(4 * max_points);
-    const size_t ord_off = keys_off + align256(4 * max_points);
-    const size_t stmp_off = ord_off + align256(4 * max_points);
-    const size_t sum_off = stmp_off + align256(sort_bytes);
+    const size_t sum_off = pts_off + align256(sizeof(ShadePoint) * max_points);
     const size_t body_bytes = sum_off + align256(24ull * (uint64_t)L.n * pad_cap);
     void *body = nullptr;
     LG_TRY(scratch(ctx, body_bytes, &body, 1));
@@ -1151,10 +1185,6 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
     void *scan_tmp = (char *)body + lay.scan_off;
     RayOut *rout = (RayOut *)((char *)body + lay.out_off);
     ShadePoint *pts = (ShadePoint *)((char *)body + pts_off);
-    uint32_t *keys = (uint32_t *)((char *)body + key_off);
-    uint32_t *keys_sorted = (uint32_t *)((char *)body + keys_off);
-    uint32_t *order = (uint32_t *)((char *)body + ord_off);
-    void *sort_tmp = (char *)body + stmp_off;
     double *sums = (double *)((char *)body + sum_off);
     for (const Chunk &ch : chunks) {
         const uint32_t pix0 = ch.b0 * kRenderBlockPixels;
@@ -1168,22 +1198,16 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
             ktimer_begin(ctx, "k_render_fill", &ev);
             for_dtype_r(density->desc.dtype, [&](auto tag) {
                 using T = decltype(tag);
-                k_render_fill<T><<<(nr + 255) / 256, 256, 0, ctx->stream>>>(g, c, u0, u1, ray0, nr, cnt, off, pts,
-                                                                            keys);
+                k_render_fill<T><<<(nr + 255) / 256, 256, 0, ctx->stream>>>(g, c, u0, u1, ray0, nr, cnt, off, pts);
             });
             LG_LAUNCHED("k_render_fill");
             ktimer_end(ctx, "k_render_fill", ev);
-            size_t sb = sort_bytes;
-            ktimer_begin(ctx, "k_render_sort", &ev);
-            LG_HIP(rocprim::radix_sort_pairs(sort_tmp, sb, keys, keys_sorted, rocprim::counting_iterator<uint32_t>(0),
-                                             order, (size_t)P, 0, kOrdKeyBits, ctx->stream));
-            ktimer_end(ctx, "k_render_sort", ev);
             const uint64_t tasks = (uint64_t)L.n * P_pad;
             ktimer_begin(ctx, "k_render_shadow", &ev);
             for_dtype_r(density->desc.dtype, [&](auto tag) {
                 using T = decltype(tag);
                 k_render_shadow<T><<<(unsigned)((tasks + 255) / 256), 256, 0, ctx->stream>>>(
-                    g, c, L, u0, u1, e0, e1, pts, order, P, P_pad, sums, stats);
+                    g, c, L, u0, u1, e0, e1, pts, P, P_pad, sums, stats);
             });
             LG_LAUNCHED("k_render_shadow");
             ktimer_end(ctx, "k_render_shadow", ev);
@@ -1236,12 +1260,7 @@ int32_t lg_render_workspace(int32_t width, int32_t height, int32_t samples, int3
         const uint64_t chunk_rays = std::max<uint64_t>(l.ray_cap, l.rays_per_block);
         pts = std::min<uint64_t>(pts, chunk_rays * (uint64_t)max_points_per_ray);
     }
-    size_t sort_bytes = 0;
-    if (pts > 0)
-        (void)rocprim::radix_sort_pairs(nullptr, sort_bytes, (uint32_t *)nullptr, (uint32_t *)nullptr,
-                                        rocprim::counting_iterator<uint32_t>(0), (uint32_t *)nullptr, (size_t)pts,
-                                        0, kOrdKeyBits);
-    *bytes = l.head_bytes + l.ray_area_bytes + align256(pts * l.point_bytes) + align256(sort_bytes) + 2048;
+    *bytes = l.head_bytes + l.ray_area_bytes + align256(pts * l.point_bytes) + 1024;
     return LG_OK;
 }
 
