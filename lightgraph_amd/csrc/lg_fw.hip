@@ -17,6 +17,7 @@
 // LightGraph.h:518-520); skipping changes no value.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "lg_internal.h"
@@ -332,11 +333,15 @@ int32_t fw_solve_batched_impl(lg_ctx *ctx, const std::vector<float *> &Ds, const
     const size_t flag_bytes = sizeof(int) * (size_t)flag_total;
     void *s = nullptr;
     LG_TRY(scratch(ctx, jobs_bytes + table_bytes + flag_bytes + 1024, &s));
+    const size_t table_at = ((jobs_bytes + 255) / 256) * 256;
     FwJob *d_jobs = (FwJob *)s;
-    int *d_table = (int *)((char *)s + ((jobs_bytes + 255) / 256) * 256);
+    int *d_table = (int *)((char *)s + table_at);
     int *d_flags = d_table + table.size() + 64;
-    LG_HIP(hipMemcpyAsync(d_jobs, jobs.data(), jobs_bytes, hipMemcpyHostToDevice, ctx->stream));
-    LG_HIP(hipMemcpyAsync(d_table, table.data(), table_bytes, hipMemcpyHostToDevice, ctx->stream));
+    // the job list and the round tables in one upload
+    std::vector<uint8_t> up(table_at + table_bytes, 0);
+    memcpy(up.data(), jobs.data(), jobs_bytes);
+    memcpy(up.data() + table_at, table.data(), table_bytes);
+    LG_HIP(hipMemcpyAsync(s, up.data(), up.size(), hipMemcpyHostToDevice, ctx->stream));
     for (int kb = 0; kb < max_nt; ++kb) {
         const int *act = d_table + round_off[kb];
         const int n = round_n[kb];

@@ -273,35 +273,60 @@ __device__ __forceinline__ void vertex_light_one(const lgd::GridView &g, const l
     for (int k = 0; k < 3; ++k) out[k] = acc[k] + emission[k];
 }
 
-// Kernel (a) for point i of solve d: the density-weighted sampler.
+// Kernel (a) for point i of solve d: the density-weighted sampler, run by one
+// wave (all 64 lanes call it; lane 0 writes).  The point's u64 target in
+// [0, total mass) selects the last active brick whose mass prefix is <= it
+// (a 64-ary search: the same index as a binary search), then the first voxel
+// of the brick, in storage order, at which the running sum of the integer
+// voxel masses floor(rho 2^24) exceeds the rest (8 steps of a wave prefix sum
+// over 64 voxels: the same voxel as the voxel-by-voxel scan).
 template <typename T>
-__device__ __forceinline__ void sample_point(const SolveDesc &d, int i, uint32_t k0, uint32_t k1, float *out) {
+__device__ __forceinline__ void sample_point_wave(const SolveDesc &d, int i, uint32_t k0, uint32_t k1, float *out) {
     const lgd::GridView &g = d.g;
+    const int lane = threadIdx.x & 63;
     const lgd::U4 r0 = lgd::philox(lgd::U4{(uint32_t)i, d.solve, 0u, lgd::DOM_SAMPLE}, k0, k1);
     const lgd::U4 r1 = lgd::philox(lgd::U4{(uint32_t)i, d.solve, 1u, lgd::DOM_SAMPLE}, k0, k1);
     const uint64_t u = ((uint64_t)r0.y << 32) | (uint64_t)r0.x;
     const uint64_t target = __umul64hi(u, d.Q);
+    // largest lo in [0, nact - 1] with mass_prefix[lo] <= target (mass_prefix[0] = 0)
     int lo = 0, hi = d.nact - 1;
     while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (d.mass_prefix[mid] <= target)
-            lo = mid;
-        else
-            hi = mid - 1;
+        // 64 probes at lo + 1 + k (hi - lo) / 64, k = 0..63: the last probe that
+        // passes bounds the answer from below, the next one from above
+        const long long span = (long long)(hi - lo);
+        const int probe = lo + 1 + (int)(((long long)lane * span) / 64);
+        const bool ok = probe <= hi && d.mass_prefix[probe] <= target;
+        const unsigned long long m = __ballot(ok);
+        if (m == 0ull) {
+            hi = lo;  // the first probe, lo + 1, failed
+        } else {
+            const int last = 63 - __clzll((long long)m);
+            const int plast = lo + 1 + (int)(((long long)last * span) / 64);
+            const int pnext = last < 63 ? lo + 1 + (int)(((long long)(last + 1) * span) / 64) : hi + 1;
+            lo = plast;
+            hi = (pnext - 1 < hi) ? pnext - 1 : hi;
+        }
     }
     const int sb = d.active_list[lo];
     const uint64_t local = target - d.mass_prefix[lo];
     const T *p = static_cast<const T *>(g.data) + ((long long)sb << 9);
     int v = 511;
-    uint64_t cum = 0;
-    for (int k = 0; k < 512; ++k) {
-        const float f = lgd::to_f32(p[k]);
-        if (f > 0.0f) cum += (uint64_t)fmin((double)f * 16777216.0, 549755813888.0 * 16777216.0);
-        if (local < cum) {
-            v = k;
+    uint64_t carry = 0;
+    for (int c = 0; c < 8; ++c) {
+        const float f = lgd::to_f32(p[64 * c + lane]);
+        uint64_t q = (f > 0.0f) ? (uint64_t)fmin((double)f * 16777216.0, 549755813888.0 * 16777216.0) : 0ull;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint64_t t = __shfl_up(q, o, 64);
+            if (lane >= o) q += t;
+        }
+        const unsigned long long hit = __ballot(local < carry + q);
+        if (hit) {
+            v = 64 * c + (__ffsll((long long)hit) - 1);
             break;
         }
+        carry += __shfl(q, 63, 64);
     }
+    if (lane != 0) return;
     const int sbx = sb % g.NBx, sby = (sb / g.NBx) % g.NBy, sbz = sb / (g.NBx * g.NBy);
     const int vx = (sbx - 1) * 8 + (v & 7), vy = (sby - 1) * 8 + ((v >> 3) & 7), vz = (sbz - 1) * 8 + (v >> 6);
     const double px = (double)vx - 0.5 + (double)lgd::u01(r0.z);

@@ -54,13 +54,14 @@ __global__ void k_solve_ids(const SolveDesc *__restrict__ sd, int S, int V, int 
 }
 
 // ------------------------------------------------------------ sampling ------
+// one wave per point
 template <typename T>
-__global__ void k_sample(const SolveDesc *__restrict__ sd, const int *__restrict__ sid, int V, uint32_t k0,
-                         uint32_t k1, float *__restrict__ pts) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
+__global__ void __launch_bounds__(256) k_sample(const SolveDesc *__restrict__ sd, const int *__restrict__ sid, int V,
+                                                uint32_t k0, uint32_t k1, float *__restrict__ pts) {
+    const int v = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (v >= V) return;  // whole waves leave together
     const SolveDesc &d = sd[sid[v]];
-    sample_point<T>(d, v - d.voff, k0, k1, pts + 3 * (long long)v);
+    sample_point_wave<T>(d, v - d.voff, k0, k1, pts + 3 * (long long)v);
 }
 
 // ----------------------------------------------------- vertex lighting ------
@@ -717,8 +718,12 @@ __global__ void k_comp_local(int V, const SolveDesc *__restrict__ sd, const int 
                              const int *__restrict__ comp_off, const int *__restrict__ member, int *__restrict__ local,
                              int *__restrict__ big_list, int *__restrict__ counters, int *__restrict__ cls16,
                              int *__restrict__ cls48, int *__restrict__ cls128, const float *__restrict__ L,
-                             float *__restrict__ out) {
+                             float *__restrict__ out, const int *__restrict__ n_comp, const int *__restrict__ n_edges) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) {  // for the host's one readback
+        counters[8] = *n_comp;
+        counters[9] = *n_edges;
+    }
     if (k >= V) return;
     const int c = cid_excl[k] + flag[k] - 1;
     local[member[k]] = k - comp_off[c];
@@ -1132,8 +1137,10 @@ int32_t Pipeline::reserve_vertices(lg_ctx *ctx, int V, int S) {
     A(&member, (size_t)v_cap);
     A(&local, (size_t)v_cap);
     A(&comp_off, (size_t)v_cap + 1);
-    A(&big_list, (size_t)v_cap + 2);  // (component, size) pairs: at most v_cap / 129 of them
-    A(&counters, 16);
+    // counters[0..15] then the (component, size) pairs of the large components:
+    // at most v_cap / (kSmallComp + 1) of them
+    A(&counters, 16 + 2 * ((size_t)v_cap / (kSmallComp + 1) + 1));
+    big_list = counters ? counters + 16 : nullptr;
     A(&cd, (size_t)s_cap);
     A(&cls16, (size_t)v_cap);
     A(&cls48, (size_t)v_cap);
@@ -1161,7 +1168,7 @@ int32_t Pipeline::reserve_vertices(lg_ctx *ctx, int V, int S) {
 void Pipeline::release_vertex(lg_ctx *ctx) {
     void *ps[] = {sd,     sid,  rowcnt,      off,    colcnt,       risky, visit,     visit_pos, visit_list,
                   adeg,   aoff, acur,        parent, root,         keys,  keys_sorted, flag,    cid,
-                  member, local, comp_off,   big_list, counters,   tmp_vertex, cd,     ckeys, ckeys_sorted,
+                  member, local, comp_off,   counters,   tmp_vertex, cd,     ckeys, ckeys_sorted,
                   cvals,  cvals_sorted, cls16, cls48, cls128};
     for (void *p : ps) dfree(ctx, p);
     cls16 = cls48 = cls128 = nullptr;
@@ -1195,7 +1202,73 @@ void Pipeline::release(lg_ctx *ctx) {
     bigT_cap = bigsums_cap = bigmeta_cap = ray_cap = 0;
 }
 
+// Clears up to 8 int regions in one launch (a stage's counters and flags
+// together instead of one fill dispatch each).
+struct ZeroList {
+    int *p[8];
+    long long n[8];
+    int count;
+};
+
+__global__ void k_zero(ZeroList z) {
+    const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (int r = 0; r < z.count; ++r)
+        for (long long i = i0; i < z.n[r]; i += stride) z.p[r][i] = 0;
+}
+
+static int32_t zero_ints(lg_ctx *ctx, std::initializer_list<std::pair<int *, size_t>> regions) {
+    ZeroList z{};
+    size_t most = 0;
+    for (const auto &r : regions) {
+        if (!r.first || r.second == 0) continue;
+        LG_CHECK_ARG(z.count < 8, "zero_ints: more than 8 regions");
+        z.p[z.count] = r.first;
+        z.n[z.count] = (long long)r.second;
+        ++z.count;
+        most = std::max(most, r.second);
+    }
+    if (z.count == 0) return LG_OK;
+    const unsigned blocks = (unsigned)std::min<size_t>(1024, (most + 255) / 256);
+    k_zero<<<blocks, 256, 0, ctx->stream>>>(z);
+    LG_LAUNCHED("k_zero");
+    return LG_OK;
+}
+
+// Exclusive scan of up to kSmallScan ints in one workgroup: thread t sums its
+// run of ceil(n / 1024) elements, the run sums are scanned in LDS, then each
+// thread writes its run.  One dispatch instead of a device-wide scan's two.
+constexpr size_t kSmallScan = 65536;
+
+__global__ void __launch_bounds__(1024) k_scan_small(const int *__restrict__ in, int *__restrict__ out, int n) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = t * per, e = min(b + per, n);
+    int sum = 0;
+    for (int i = b; i < e; ++i) sum += in[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const int v = t >= o ? part[t - o] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int i = b; i < e; ++i) {
+        const int x = in[i];
+        out[i] = run;
+        run += x;
+    }
+}
+
 static int32_t scan_excl(lg_ctx *ctx, Pipeline &pl, const int *in, int *out, size_t n, bool cand_sized) {
+    if (n <= kSmallScan) {
+        if (n > 0) k_scan_small<<<1, 1024, 0, ctx->stream>>>(in, out, (int)n);
+        LG_LAUNCHED("k_scan_small");
+        return LG_OK;
+    }
     size_t bytes = 0;
     LG_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, 0, n, rocprim::plus<int>(), ctx->stream));
     void *tmp = nullptr;
@@ -1245,7 +1318,7 @@ int32_t pl_sample(lg_ctx *ctx, Pipeline &pl, int V, int dtype, uint64_t seed, fl
     ktimer_begin(ctx, "k_sample_points", &ev);
     for_dtype(dtype, [&](auto tag) {
         using T = decltype(tag);
-        k_sample<T><<<(unsigned)((V + 255) / 256), 256, 0, ctx->stream>>>(pl.sd, pl.sid, V, (uint32_t)seed,
+        k_sample<T><<<(unsigned)((V + 3) / 4), 256, 0, ctx->stream>>>(pl.sd, pl.sid, V, (uint32_t)seed,
                                                                          (uint32_t)(seed >> 32), pts);
     });
     LG_LAUNCHED("k_sample");
@@ -1313,11 +1386,10 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
                                      (size_t)V, 0, kbits, st));
     LG_TRY(grow(ctx, &pl.cstart, &pl.cells_cap, (size_t)pl.total_cells));
     LG_TRY(grow(ctx, &pl.cend, &pl.cend_cap, (size_t)pl.total_cells));
-    LG_HIP(hipMemsetAsync(pl.cstart, 0, sizeof(int) * (size_t)pl.total_cells, st));
-    LG_HIP(hipMemsetAsync(pl.cend, 0, sizeof(int) * (size_t)pl.total_cells, st));
+    LG_TRY(zero_ints(ctx, {{pl.cstart, (size_t)pl.total_cells}, {pl.cend, (size_t)pl.total_cells},
+                           {pl.rowcnt + V, 1}}));
     k_cell_bounds<<<vb, 256, 0, st>>>(V, pl.ckeys_sorted, pl.cstart, pl.cend);
     LG_LAUNCHED("k_cell_bounds");
-    LG_HIP(hipMemsetAsync(pl.rowcnt + V, 0, sizeof(int), st));
     k_cand<false><<<wb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted, nullptr,
                                       pl.rowcnt, nullptr);
     LG_LAUNCHED("k_cand");
@@ -1330,20 +1402,19 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
     LG_TRY(grow(ctx, &pl.epos, &pl.epos_cap, (size_t)Ktot + 1));
     LG_TRY(grow(ctx, &pl.edges_, &pl.edges_cap, (size_t)2 * Ktot + 2));
     LG_TRY(grow(ctx, &pl.w_, &pl.w_cap, (size_t)Ktot + 1));
+    const int cap = (max_connect > 0 && max_connect < 255) ? max_connect : 0;
+    LG_TRY(zero_ints(ctx, {{cap > 0 ? pl.colcnt : nullptr, (size_t)V}, {pl.visit + V, 1}, {pl.accepted + Ktot, 1}}));
     k_cand<true><<<wb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted, pl.off,
                                      nullptr, pl.cand);
     LG_LAUNCHED("k_cand");
     ktimer_end(ctx, "k_cand", ev_c);
-    const int cap = (max_connect > 0 && max_connect < 255) ? max_connect : 0;
     if (cap > 0) {
-        LG_HIP(hipMemsetAsync(pl.colcnt, 0, sizeof(int) * V, st));
         k_colcount<<<vb, 256, 0, st>>>(V, pl.off, pl.cand, pl.colcnt);
         k_risky<<<vb, 256, 0, st>>>(V, cap, pl.off, pl.colcnt, pl.risky);
     }
     k_classify<<<vb, 256, 0, st>>>(V, cap, pl.off, pl.cand, pl.risky, pl.accepted, pl.visit);
     LG_LAUNCHED("k_classify");
     if (cap > 0) {
-        LG_HIP(hipMemsetAsync(pl.visit + V, 0, sizeof(int), st));
         LG_TRY(scan_excl(ctx, pl, pl.visit, pl.visit_pos, (size_t)V + 1, false));
         k_compact_rows<<<vb, 256, 0, st>>>(V, pl.visit, pl.visit_pos, pl.visit_list);
         hipEvent_t ev_g;
@@ -1357,7 +1428,6 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
         LG_LAUNCHED("k_greedy");
         ktimer_end(ctx, "k_greedy", ev_g);
     }
-    LG_HIP(hipMemsetAsync(pl.accepted + Ktot, 0, sizeof(int), st));
     LG_TRY(scan_excl(ctx, pl, pl.accepted, pl.epos, (size_t)Ktot + 1, true));
     k_edges<<<vb, 256, 0, st>>>(V, pl.off, pl.cand, pl.accepted, pl.epos, pl.edges_);
     LG_LAUNCHED("k_edges");
@@ -1389,8 +1459,7 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     const int *d_nedges = pl.epos + Ktot;
     hipEvent_t ev;
     ktimer_begin(ctx, "k_components", &ev);
-    LG_HIP(hipMemsetAsync(pl.adeg, 0, sizeof(int) * ((size_t)V + 1), st));
-    LG_HIP(hipMemsetAsync(pl.acur, 0, sizeof(int) * (size_t)V, st));
+    LG_TRY(zero_ints(ctx, {{pl.adeg, (size_t)V + 1}, {pl.acur, (size_t)V}, {pl.flag + V, 1}, {pl.counters, 16}}));
     const unsigned eb = (unsigned)((Ktot + 255) / 256 + 1);
     k_adj_deg<<<eb, 256, 0, st>>>(pl.edges_, pl.w_, d_nedges, pl.adeg);
     LG_TRY(scan_excl(ctx, pl, pl.adeg, pl.aoff, (size_t)V + 1, false));
@@ -1412,26 +1481,22 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     size_t sbytes = pl.tmp_vertex_bytes;
     LG_HIP(rocprim::radix_sort_keys(pl.tmp_vertex, sbytes, pl.keys, pl.keys_sorted, (size_t)V, 0, 2 * bits, st));
     k_comp_flags<<<vb, 256, 0, st>>>(V, bits, pl.keys_sorted, pl.flag, pl.member);
-    LG_HIP(hipMemsetAsync(pl.flag + V, 0, sizeof(int), st));
     LG_TRY(scan_excl(ctx, pl, pl.flag, pl.cid, (size_t)V + 1, false));
     k_comp_offsets<<<vb, 256, 0, st>>>(V, pl.flag, pl.cid, pl.comp_off, pl.cid + V);
-    LG_HIP(hipMemsetAsync(pl.counters, 0, sizeof(int) * 16, st));
     k_comp_local<<<vb, 256, 0, st>>>(V, pl.sd, pl.sid, pl.flag, pl.cid, pl.comp_off, pl.member, pl.local, pl.big_list,
-                                     pl.counters, pl.cls16, pl.cls48, pl.cls128, L, out);
+                                     pl.counters, pl.cls16, pl.cls48, pl.cls128, L, out, pl.cid + V, d_nedges);
     LG_LAUNCHED("k_comp_local");
     ktimer_end(ctx, "k_comp_sort", ev_s);
     // counts and the large-component pairs come back in one synchronisation
+    // (counters[0..3], component count, edge count, then the large-component
+    // pairs: big_list = counters + 16)
     const int max_big = V / (kSmallComp + 1);
-    std::vector<int> host((size_t)6 + 2 * (size_t)max_big);
-    LG_HIP(hipMemcpyAsync(&host[0], pl.cid + V, sizeof(int), hipMemcpyDeviceToHost, st));
-    LG_HIP(hipMemcpyAsync(&host[1], pl.counters, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
-    LG_HIP(hipMemcpyAsync(&host[5], d_nedges, sizeof(int), hipMemcpyDeviceToHost, st));
-    if (max_big > 0)
-        LG_HIP(hipMemcpyAsync(&host[6], pl.big_list, sizeof(int) * 2 * (size_t)max_big, hipMemcpyDeviceToHost, st));
+    std::vector<int> rb((size_t)16 + 2 * (size_t)max_big);
+    LG_HIP(hipMemcpyAsync(rb.data(), pl.counters, sizeof(int) * rb.size(), hipMemcpyDeviceToHost, st));
     LG_HIP(hipStreamSynchronize(st));
     ktimer_end(ctx, "k_components", ev);
-    const int C = host[0], nbig = host[1], n16 = host[2], n48 = host[3], n128 = host[4];
-    res->n_edges_total = host[5];
+    const int C = rb[8], nbig = rb[0], n16 = rb[1], n48 = rb[2], n128 = rb[3];
+    res->n_edges_total = rb[9];
     res->n_components = C;
     res->n_big = nbig;
     res->Ktot = Ktot;
@@ -1482,7 +1547,7 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
 
     // large components: one arena, batched blocked Floyd-Warshall, batched diffusion
     std::vector<std::pair<int, int>> big((size_t)nbig);
-    for (int b = 0; b < nbig; ++b) big[b] = {host[6 + 2 * b], host[7 + 2 * b]};
+    for (int b = 0; b < nbig; ++b) big[b] = {rb[16 + 2 * b], rb[17 + 2 * b]};
     std::sort(big.begin(), big.end());  // deterministic order (slots come from atomics)
     std::vector<BigJob> jobs((size_t)nbig);
     std::vector<int> ld_prefix((size_t)nbig), m_prefix((size_t)nbig);
@@ -1509,12 +1574,14 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     int *d_ldp = (int *)(pl.big_meta + jb_bytes);
     int *d_mp = d_ldp + nbig;
     int *d_comp = d_mp + nbig;
-    std::vector<int> comps((size_t)nbig);
-    for (int b = 0; b < nbig; ++b) comps[b] = big[b].first;
-    LG_HIP(hipMemcpyAsync(d_jobs, jobs.data(), jb_bytes, hipMemcpyHostToDevice, st));
-    LG_HIP(hipMemcpyAsync(d_ldp, ld_prefix.data(), pre_bytes, hipMemcpyHostToDevice, st));
-    LG_HIP(hipMemcpyAsync(d_mp, m_prefix.data(), pre_bytes, hipMemcpyHostToDevice, st));
-    LG_HIP(hipMemcpyAsync(d_comp, comps.data(), pre_bytes, hipMemcpyHostToDevice, st));
+    // one upload of the job table, the two prefixes and the component ids
+    std::vector<uint8_t> up(jb_bytes + 3 * pre_bytes);
+    memcpy(up.data(), jobs.data(), jb_bytes);
+    memcpy(up.data() + jb_bytes, ld_prefix.data(), pre_bytes);
+    memcpy(up.data() + jb_bytes + pre_bytes, m_prefix.data(), pre_bytes);
+    int *up_comp = (int *)(up.data() + jb_bytes + 2 * pre_bytes);
+    for (int b = 0; b < nbig; ++b) up_comp[b] = big[b].first;
+    LG_HIP(hipMemcpyAsync(d_jobs, up.data(), up.size(), hipMemcpyHostToDevice, st));
     k_big_a0<<<(nbig + 255) / 256, 256, 0, st>>>(nbig, d_comp, pl.comp_off, d_jobs);
     LG_LAUNCHED("k_big_a0");
     ktimer_begin(ctx, "k_big_fill", &ev);
