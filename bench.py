@@ -507,6 +507,11 @@ def main():
     solver.destroy()
     if args.batch_volumes > 0:
         line["batched"] = bench_batched(lga, ctx, shader, params, lights, args.batch_volumes, args.batch_reps)
+    t = lga.fw_tuning(ctx)
+    line["fw_tuning"] = {"split_below": t["split_below"], "use_graph": t["use_graph"], "tuned": t["tuned"],
+                         "tiles": t["tiles"], "whole_us": [round(x, 2) for x in t["whole_us"]],
+                         "split_us": [round(x, 2) for x in t["split_us"]], "stream_ms": round(t["stream_ms"], 4),
+                         "graph_ms": round(t["graph_ms"], 4), "capture_ms": round(t["capture_ms"], 4)}
     if not args.no_precision:
         line["precision"] = bench_precision(lga, ctx)
     if args.serve_jobs > 0:

@@ -347,6 +347,31 @@ int32_t lg_solve(lg_ctx *ctx, lg_solver *solver, const lg_grid *density,
 int32_t lg_fw_solve_batched(lg_ctx *ctx, int32_t count, const int32_t *h_n,
                             float *const *h_D, const int32_t *h_ld);
 
+/* Tile and schedule choices of the batched Floyd-Warshall, picked by a
+ * measurement on the device (no reference counterpart: the reference runs one
+ * scalar loop per solve).  split_below: panel and remainder launches of fewer
+ * 128 x 128 tiles run four 64 x 64 workgroups per tile (results are identical
+ * either way).  use_graph: the rounds of a batch are captured once into a
+ * hipGraph and replayed while the schedule's shape repeats.  The first batched
+ * closure on a device runs lg_fw_autotune unless the environment sets
+ * LG_FW_AUTOTUNE=0 (then the defaults hold); LG_FW_SPLIT_BELOW and LG_FW_GRAPH
+ * override the two choices. */
+typedef struct lg_fw_tuning {
+    int32_t split_below;
+    int32_t use_graph;
+    int32_t tuned;          /* 1: measured on this device                           */
+    int32_t points;         /* entries of the tables below                           */
+    int32_t tiles[16];      /* tile counts of one remainder launch                   */
+    double whole_us[16];    /* its time with one workgroup per tile                  */
+    double split_us[16];    /* and with four                                         */
+    double stream_ms;       /* a synthetic 24-matrix schedule launched on the stream */
+    double graph_ms;        /* the same schedule replayed as a captured hipGraph     */
+    double capture_ms;      /* capturing and instantiating that graph                */
+} lg_fw_tuning;
+int32_t lg_fw_autotune(lg_ctx *ctx, lg_fw_tuning *out);
+int32_t lg_fw_get_tuning(lg_ctx *ctx, lg_fw_tuning *out);
+int32_t lg_fw_set_tuning(lg_ctx *ctx, const lg_fw_tuning *tuning);
+
 /* One graph of a batched solve. */
 typedef struct lg_batch_item {
     const lg_grid *density;      /* density grid of this solve (all items: one dtype) */

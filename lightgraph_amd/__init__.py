@@ -325,6 +325,33 @@ def fw_solve_batched(ctx, ns, D_bufs):
     _check(lib().lg_fw_solve_batched(ctx.handle, k, n_arr, p_arr, ld_arr), "fw_solve_batched")
 
 
+def _tuning_dict(t):
+    k = t.points
+    return {"split_below": t.split_below, "use_graph": t.use_graph, "tuned": t.tuned,
+            "tiles": list(t.tiles[:k]), "whole_us": list(t.whole_us[:k]), "split_us": list(t.split_us[:k]),
+            "stream_ms": t.stream_ms, "graph_ms": t.graph_ms, "capture_ms": t.capture_ms}
+
+
+def fw_autotune(ctx):
+    """Measure the batched Floyd-Warshall choices on this device now (lg_fw_autotune)."""
+    t = _abi.lg_fw_tuning()
+    _check(lib().lg_fw_autotune(ctx.handle, C.byref(t)), "fw_autotune")
+    return _tuning_dict(t)
+
+
+def fw_tuning(ctx):
+    t = _abi.lg_fw_tuning()
+    _check(lib().lg_fw_get_tuning(ctx.handle, C.byref(t)), "fw_get_tuning")
+    return _tuning_dict(t)
+
+
+def set_fw_tuning(ctx, split_below, use_graph):
+    t = _abi.lg_fw_tuning()
+    _check(lib().lg_fw_get_tuning(ctx.handle, C.byref(t)), "fw_get_tuning")
+    t.split_below, t.use_graph = int(split_below), int(use_graph)
+    _check(lib().lg_fw_set_tuning(ctx.handle, C.byref(t)), "fw_set_tuning")
+
+
 def diffuse(ctx, n, D_buf, ld, sigma_t, L_buf, out_buf):
     sig = (C.c_double * 3)(*sigma_t)
     _check(lib().lg_diffuse(ctx.handle, n, D_buf.ptr, ld, sig, L_buf.ptr, out_buf.ptr), "diffuse")

@@ -97,6 +97,12 @@ class lg_solve_stats(C.Structure):
                                           "fw_tiles", "fw_panel_tiles", "fw_pivots", "small_updates", "components", "big_components")]
 
 
+class lg_fw_tuning(C.Structure):
+    _fields_ = [("split_below", C.c_int32), ("use_graph", C.c_int32), ("tuned", C.c_int32), ("points", C.c_int32),
+                ("tiles", C.c_int32 * 16), ("whole_us", C.c_double * 16), ("split_us", C.c_double * 16),
+                ("stream_ms", C.c_double), ("graph_ms", C.c_double), ("capture_ms", C.c_double)]
+
+
 class lg_render_stats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("camera_rays", "primary_samples", "shading_points", "shadow_rays",
                                           "shadow_samples", "shadow_lane_slots", "chunks")]
@@ -197,6 +203,9 @@ SIGNATURES = {
     "lg_render": (I32, [P, P, C.POINTER(lg_scatter_grid), P, C.POINTER(lg_camera), C.POINTER(lg_shader),
                         C.POINTER(lg_light), I32, C.POINTER(lg_march_params), I32, U64, U64,
                         C.POINTER(lg_films)]),
+    "lg_fw_autotune": (I32, [P, C.POINTER(lg_fw_tuning)]),
+    "lg_fw_get_tuning": (I32, [P, C.POINTER(lg_fw_tuning)]),
+    "lg_fw_set_tuning": (I32, [P, C.POINTER(lg_fw_tuning)]),
     "lg_render_workspace": (I32, [I32, I32, I32, I32, I64, C.POINTER(C.c_uint64)]),
     "lg_render_get_stats": (I32, [P, C.POINTER(lg_render_stats)]),
     "lg_envmap_create": (I32, [P, I32, I32, P, PP]),

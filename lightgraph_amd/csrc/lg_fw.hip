@@ -16,6 +16,10 @@
 // +INF are skipped (the reference skips the same zero-transmittance entries,
 // LightGraph.h:518-520); skipping changes no value.
 #include <algorithm>
+#include <mutex>
+#include <map>
+#include <cstdlib>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -114,10 +118,13 @@ struct FwJob {
 };
 
 // Batched pivot closure: block b closes the pivot tile of job active[b].
+// It also clears the job's tile flags for the round's panel launch.
 __global__ void __launch_bounds__(1024) k_fw_pivot_b(const FwJob *__restrict__ jobs, const int *__restrict__ active,
-                                                     int kb, unsigned long long *__restrict__ stats) {
+                                                     int kb, int *__restrict__ flags,
+                                                     unsigned long long *__restrict__ stats) {
     const FwJob j = jobs[active[blockIdx.x]];
     if (stats && threadIdx.x == 0) stat_add(stats, ST_FW_PIVOTS, 1ull);
+    for (int t = threadIdx.x; t < j.nt; t += blockDim.x) flags[j.flag_off + t] = 0;
     fw_pivot_tile(j.D, j.ld, kb);
 }
 
@@ -235,18 +242,107 @@ __device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, i
     }
 }
 
+// The same update for one 64 x 64 quadrant q (rows 64 (q >> 1).., columns
+// 64 (q & 1).. of the output tile), 4 x 4 outputs per thread: four workgroups
+// share a tile when a launch holds too few tiles to fill the GPU.  Each output
+// folds the pivots in the same pairs and order as fw_minplus_tile, so the
+// result is bit-identical.  A panel quadrant with a finite value sets flags[Jb]
+// to 1; the caller has cleared the flags.
+template <bool PANEL>
+__device__ __forceinline__ void fw_minplus_quad(float *__restrict__ D, int ld, int kb, int nt, int *__restrict__ flags,
+                                                int tile, int q, unsigned long long *__restrict__ stats) {
+    int Ib, Jb;
+    if (PANEL) {
+        Ib = kb;
+        Jb = tile < kb ? tile : tile + 1;
+    } else {
+        int a, b;
+        tri_decode(tile, nt - 1, a, b);
+        Ib = a < kb ? a : a + 1;
+        Jb = b < kb ? b : b + 1;
+        if (!flags[Ib] || !flags[Jb]) return;
+    }
+    if (stats && q == 0 && threadIdx.x == 0) stat_add(stats, PANEL ? ST_FW_PANEL_TILES : ST_FW_TILES, 1ull);
+    __shared__ float Xs[KC][64];
+    __shared__ float Ys[KC][64];
+    const int tid = threadIdx.x;
+    const int ty = tid >> 4, tx = tid & 15;
+    const int r0 = 64 * (q >> 1), c0 = 64 * (q & 1);
+    float acc[4][4];
+    float *Cb = D + (long long)Ib * TB * ld + (long long)Jb * TB;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const float4 v = *reinterpret_cast<const float4 *>(Cb + (long long)(r0 + ty * 4 + r) * ld + c0 + tx * 4);
+        acc[r][0] = v.x;
+        acc[r][1] = v.y;
+        acc[r][2] = v.z;
+        acc[r][3] = v.w;
+    }
+    const float *Xg = D + (long long)kb * TB * ld + (long long)Ib * TB + r0;
+    const float *Yg = D + (long long)kb * TB * ld + (long long)Jb * TB + c0;
+    for (int kc = 0; kc < TB; kc += KC) {
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int idx = tid + h * 256;
+            const int kk = idx >> 4, c4 = idx & 15;
+            *reinterpret_cast<float4 *>(&Xs[kk][c4 * 4]) =
+                *reinterpret_cast<const float4 *>(Xg + (long long)(kc + kk) * ld + c4 * 4);
+            *reinterpret_cast<float4 *>(&Ys[kk][c4 * 4]) =
+                *reinterpret_cast<const float4 *>(Yg + (long long)(kc + kk) * ld + c4 * 4);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < KC; kk += 2) {
+            const float4 xa = *reinterpret_cast<const float4 *>(&Xs[kk][ty * 4]);
+            const float4 yb = *reinterpret_cast<const float4 *>(&Ys[kk][tx * 4]);
+            const float4 xc = *reinterpret_cast<const float4 *>(&Xs[kk + 1][ty * 4]);
+            const float4 yd = *reinterpret_cast<const float4 *>(&Ys[kk + 1][tx * 4]);
+            const float a[4] = {xa.x, xa.y, xa.z, xa.w};
+            const float b[4] = {yb.x, yb.y, yb.z, yb.w};
+            const float c[4] = {xc.x, xc.y, xc.z, xc.w};
+            const float d[4] = {yd.x, yd.y, yd.z, yd.w};
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc[r][t] = fminf(acc[r][t], fminf(a[r] + b[t], c[r] + d[t]));
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        *reinterpret_cast<float4 *>(Cb + (long long)(r0 + ty * 4 + r) * ld + c0 + tx * 4) =
+            make_float4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+    if (PANEL || Ib != Jb) {
+        float *Tb = D + (long long)Jb * TB * ld + (long long)Ib * TB;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+            *reinterpret_cast<float4 *>(Tb + (long long)(c0 + tx * 4 + t) * ld + r0 + ty * 4) =
+                make_float4(acc[0][t], acc[1][t], acc[2][t], acc[3][t]);
+    }
+    if (PANEL) {
+        int finite = 0;
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) finite |= (acc[r][t] < INFINITY) ? 1 : 0;
+        finite = __syncthreads_or(finite);
+        if (tid == 0 && finite) flags[Jb] = 1;
+    }
+}
+
 template <bool PANEL>
 __global__ void __launch_bounds__(256) k_fw_minplus(float *__restrict__ D, int ld, int kb, int nt,
                                                      int *__restrict__ flags) {
     fw_minplus_tile<PANEL>(D, ld, kb, nt, flags, (int)blockIdx.x);
 }
 
-// Batched panel / remainder launch: prefix[a] is the first block of active job a.
-template <bool PANEL>
+// Batched panel / remainder launch: prefix[a] is the first tile of active job
+// a; SPLIT 2 runs four quadrant workgroups per tile.
+template <bool PANEL, int SPLIT>
 __global__ void __launch_bounds__(256) k_fw_minplus_b(const FwJob *__restrict__ jobs, const int *__restrict__ active,
                                                        const int *__restrict__ prefix, int n_active, int kb,
                                                        int *__restrict__ flags, unsigned long long *__restrict__ stats) {
-    const int b = (int)blockIdx.x;
+    const int b = (int)blockIdx.x / (SPLIT * SPLIT);
     int lo = 0, hi = n_active - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -256,7 +352,11 @@ __global__ void __launch_bounds__(256) k_fw_minplus_b(const FwJob *__restrict__ 
             hi = mid - 1;
     }
     const FwJob j = jobs[active[lo]];
-    fw_minplus_tile<PANEL>(j.D, j.ld, kb, j.nt, flags + j.flag_off, b - prefix[lo], stats);
+    if (SPLIT == 1)
+        fw_minplus_tile<PANEL>(j.D, j.ld, kb, j.nt, flags + j.flag_off, b - prefix[lo], stats);
+    else
+        fw_minplus_quad<PANEL>(j.D, j.ld, kb, j.nt, flags + j.flag_off, b - prefix[lo],
+                               (int)blockIdx.x % (SPLIT * SPLIT), stats);
 }
 
 int32_t fw_solve_impl(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld) {
@@ -288,10 +388,30 @@ int32_t fw_solve_impl(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld) {
     return LG_OK;
 }
 
-// Uneven batch: round kb runs the pivot tiles of every matrix with more than kb
-// tile rows in one launch, then all their panels, then all their remainders.
+// ------------------------------------------------------------------ tuning ----
+// The batched closure's two choices (lg_fw_tuning): which launches split tiles
+// into quadrants, and whether the rounds run as a captured hipGraph.
+struct FwTuningChoice {
+    int split_below = 0;
+    int use_graph = 0;
+};
+static int32_t fw_tuning_active(lg_ctx *ctx, FwTuningChoice *out);
+template <typename F>
+static int32_t fw_graph_capture(lg_ctx *ctx, const std::vector<uint64_t> &key, F &&rounds);
+static int32_t fw_batched_run(lg_ctx *ctx, const std::vector<float *> &Ds, const std::vector<int> &ns,
+                              unsigned long long *stats, const FwTuningChoice &tc);
+
 int32_t fw_solve_batched_impl(lg_ctx *ctx, const std::vector<float *> &Ds, const std::vector<int> &ns,
                               unsigned long long *stats) {
+    FwTuningChoice tc;
+    LG_TRY(fw_tuning_active(ctx, &tc));
+    return fw_batched_run(ctx, Ds, ns, stats, tc);
+}
+
+// Uneven batch: round kb runs the pivot tiles of every matrix with more than kb
+// tile rows in one launch, then all their panels, then all their remainders.
+static int32_t fw_batched_run(lg_ctx *ctx, const std::vector<float *> &Ds, const std::vector<int> &ns,
+                              unsigned long long *stats, const FwTuningChoice &tc) {
     const int count = (int)Ds.size();
     std::vector<FwJob> jobs;
     int max_nt = 0, flag_total = 0;
@@ -342,24 +462,277 @@ int32_t fw_solve_batched_impl(lg_ctx *ctx, const std::vector<float *> &Ds, const
     memcpy(up.data(), jobs.data(), jobs_bytes);
     memcpy(up.data() + table_at, table.data(), table_bytes);
     LG_HIP(hipMemcpyAsync(s, up.data(), up.size(), hipMemcpyHostToDevice, ctx->stream));
+    auto rounds = [&](bool timers) -> int32_t {
+        for (int kb = 0; kb < max_nt; ++kb) {
+            const int *act = d_table + round_off[kb];
+            const int n = round_n[kb];
+            hipEvent_t e1 = nullptr, e2 = nullptr, e3 = nullptr;
+            if (timers) ktimer_begin(ctx, "k_fw_pivot", &e1);
+            k_fw_pivot_b<<<n, 1024, 0, ctx->stream>>>(d_jobs, act, kb, d_flags, stats);
+            LG_LAUNCHED("k_fw_pivot_b");
+            if (timers) ktimer_end(ctx, "k_fw_pivot", e1);
+            if (panel_blocks[kb] == 0) continue;
+            if (timers) ktimer_begin(ctx, "k_fw_panel", &e2);
+            if (panel_blocks[kb] < tc.split_below)
+                k_fw_minplus_b<true, 2><<<4 * panel_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + n, n, kb,
+                                                                                     d_flags, stats);
+            else
+                k_fw_minplus_b<true, 1><<<panel_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + n, n, kb,
+                                                                                 d_flags, stats);
+            LG_LAUNCHED("k_fw_panel_b");
+            if (timers) ktimer_end(ctx, "k_fw_panel", e2);
+            if (timers) ktimer_begin(ctx, "k_fw_minplus", &e3);
+            if (rem_blocks[kb] < tc.split_below)
+                k_fw_minplus_b<false, 2><<<4 * rem_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + 2 * n, n,
+                                                                                     kb, d_flags, stats);
+            else
+                k_fw_minplus_b<false, 1><<<rem_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + 2 * n, n, kb,
+                                                                                 d_flags, stats);
+            LG_LAUNCHED("k_fw_minplus_b");
+            if (timers) ktimer_end(ctx, "k_fw_minplus", e3);
+        }
+        return LG_OK;
+    };
+    if (!tc.use_graph || ctx->timing) return rounds(ctx->timing);
+    // graph schedule: the launches depend on the device tables' address, the
+    // round shapes, the split choice and the counter pointer; the tables'
+    // contents are read when the graph runs, so one capture serves every batch
+    // of the same shape
+    std::vector<uint64_t> key = {(uint64_t)(uintptr_t)s, (uint64_t)(uintptr_t)stats, (uint64_t)tc.split_below,
+                                 (uint64_t)max_nt, (uint64_t)table_at};
     for (int kb = 0; kb < max_nt; ++kb) {
-        const int *act = d_table + round_off[kb];
-        const int n = round_n[kb];
-        hipEvent_t e1, e2, e3;
-        ktimer_begin(ctx, "k_fw_pivot", &e1);
-        k_fw_pivot_b<<<n, 1024, 0, ctx->stream>>>(d_jobs, act, kb, stats);
-        LG_LAUNCHED("k_fw_pivot_b");
-        ktimer_end(ctx, "k_fw_pivot", e1);
-        if (panel_blocks[kb] == 0) continue;
-        ktimer_begin(ctx, "k_fw_panel", &e2);
-        k_fw_minplus_b<true><<<panel_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + n, n, kb, d_flags, stats);
-        LG_LAUNCHED("k_fw_panel_b");
-        ktimer_end(ctx, "k_fw_panel", e2);
-        ktimer_begin(ctx, "k_fw_minplus", &e3);
-        k_fw_minplus_b<false><<<rem_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + 2 * n, n, kb, d_flags, stats);
-        LG_LAUNCHED("k_fw_minplus_b");
-        ktimer_end(ctx, "k_fw_minplus", e3);
+        key.push_back((uint64_t)round_off[kb]);
+        key.push_back(((uint64_t)round_n[kb] << 32) | (uint32_t)panel_blocks[kb]);
+        key.push_back((uint64_t)rem_blocks[kb]);
     }
+    if (!ctx->fw_graph || key != ctx->fw_graph_key) LG_TRY(fw_graph_capture(ctx, key, rounds));
+    LG_HIP(hipGraphLaunch(ctx->fw_graph, ctx->stream));
+    return LG_OK;
+}
+
+// Replaces the context's captured schedule by a capture of `rounds`.
+template <typename F>
+static int32_t fw_graph_capture(lg_ctx *ctx, const std::vector<uint64_t> &key, F &&rounds) {
+    if (ctx->fw_graph) {
+        LG_HIP(hipStreamSynchronize(ctx->stream));  // the old graph may still run
+        (void)hipGraphExecDestroy(ctx->fw_graph);
+        ctx->fw_graph = nullptr;
+        ctx->fw_graph_key.clear();
+    }
+    LG_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int32_t rc = rounds(false);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (rc != LG_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        if (rc != LG_OK) return rc;
+        set_error("hipStreamEndCapture: %s", hipGetErrorString(e));
+        return LG_ERR_HIP;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+        set_error("hipGraphInstantiate: %s", hipGetErrorString(ei));
+        return LG_ERR_HIP;
+    }
+    ctx->fw_graph = exec;
+    ctx->fw_graph_key = key;
+    return LG_OK;
+}
+
+// Defaults without a measurement: split launches of fewer than two tiles per CU.
+constexpr int kFwSplitBelowDefault = 256;
+
+__global__ void k_tune_fill(float *__restrict__ D, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t h = (uint32_t)i * 2654435761u;
+    h ^= h >> 15;
+    D[i] = 1.0f + (float)(h & 1023u) * (1.0f / 1024.0f);
+}
+
+__global__ void k_tune_flags(int *__restrict__ f, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) f[i] = 1;
+}
+
+static int env_int(const char *name, int dflt) {
+    const char *v = getenv(name);
+    return (v && *v) ? atoi(v) : dflt;
+}
+
+// Measures, on synthetic finite matrices: (1) one remainder launch of T tiles
+// with one and with four workgroups per tile, T = the remainder tiles of round 0
+// of an nt-tile matrix, giving split_below = the first T at which splitting no
+// longer wins; (2) the closure of 24 matrices of 2..7 tile rows launched on the
+// stream and as a captured graph, giving use_graph when a capture pays for
+// itself in one use.
+static int32_t fw_autotune_run(lg_ctx *ctx, lg_fw_tuning *t) {
+    *t = lg_fw_tuning{};
+    t->split_below = kFwSplitBelowDefault;
+    static const int nts[] = {2, 3, 4, 6, 8, 12, 16, 23, 32, 46, 64, 91};
+    const int npts = (int)(sizeof(nts) / sizeof(nts[0]));
+    const int ld = 91 * TB;
+    float *D = nullptr;
+    int *meta = nullptr;
+    if (dalloc(ctx, (size_t)ld * ld, &D) != LG_OK) return LG_OK;  // no room: keep the defaults
+    if (dalloc(ctx, 256, &meta) != LG_OK) {
+        dfree(ctx, D);
+        return LG_OK;
+    }
+    hipStream_t st = ctx->stream;
+    const long long nel = (long long)ld * ld;
+    k_tune_fill<<<(unsigned)((nel + 255) / 256), 256, 0, st>>>(D, nel);
+    k_tune_flags<<<1, 64, 0, st>>>(meta + 64, 64);
+    hipEvent_t a = nullptr, b = nullptr;
+    int32_t rc = LG_OK;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) rc = LG_ERR_HIP;
+    // meta: [0] FwJob, [32] active = {0}, [33] prefix = {0}, [64..] flags
+    auto time_launch = [&](auto launch) -> double {
+        float best = 1e30f;
+        for (int rep = 0; rep < 4; ++rep) {
+            (void)hipEventRecord(a, st);
+            launch();
+            (void)hipEventRecord(b, st);
+            (void)hipEventSynchronize(b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            if (rep > 0) best = std::min(best, ms);  // rep 0 warms up
+        }
+        return (double)best * 1e3;
+    };
+    for (int k = 0; k < npts && rc == LG_OK; ++k) {
+        const int nt = nts[k];
+        const int T = (nt - 1) * nt / 2;
+        FwJob job{D, ld, nt, 0, 0};
+        int ap[2] = {0, 0};
+        if (hipMemcpyAsync(meta, &job, sizeof(job), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipMemcpyAsync(meta + 32, ap, sizeof(ap), hipMemcpyHostToDevice, st) != hipSuccess) {
+            rc = LG_ERR_HIP;
+            break;
+        }
+        const FwJob *dj = (const FwJob *)meta;
+        const int *act = meta + 32;
+        t->tiles[k] = T;
+        t->whole_us[k] = time_launch([&] {
+            k_fw_minplus_b<false, 1><<<T, 256, 0, st>>>(dj, act, act + 1, 1, 0, meta + 64, nullptr);
+        });
+        t->split_us[k] = time_launch([&] {
+            k_fw_minplus_b<false, 2><<<4 * T, 256, 0, st>>>(dj, act, act + 1, 1, 0, meta + 64, nullptr);
+        });
+        t->points = k + 1;
+    }
+    if (rc == LG_OK && hipGetLastError() != hipSuccess) rc = LG_ERR_HIP;
+    if (rc == LG_OK) {
+        int sb = 1 << 30;  // splitting won at every measured size
+        for (int k = 0; k < t->points; ++k)
+            if (t->whole_us[k] <= t->split_us[k]) {
+                sb = t->tiles[k];
+                break;
+            }
+        t->split_below = sb;
+    }
+    // (2) stream against graph, 24 matrices of 2..7 tile rows carved from D
+    if (rc == LG_OK) {
+        std::vector<float *> Ds;
+        std::vector<int> ns;
+        size_t off = 0;
+        for (int g = 0; g < 24; ++g) {
+            const int n = TB * (2 + g % 6);
+            Ds.push_back(D + off);
+            ns.push_back(n);
+            off += (size_t)n * n;
+        }
+        FwTuningChoice tc;
+        tc.split_below = t->split_below;
+        float best = 1e30f;
+        for (int rep = 0; rep < 4 && rc == LG_OK; ++rep) {
+            (void)hipEventRecord(a, st);
+            rc = fw_batched_run(ctx, Ds, ns, nullptr, tc);
+            (void)hipEventRecord(b, st);
+            (void)hipEventSynchronize(b);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            if (rep > 0) best = std::min(best, ms);
+        }
+        t->stream_ms = best;
+        tc.use_graph = 1;
+        const bool timing = ctx->timing;
+        ctx->timing = false;
+        if (rc == LG_OK) {
+            (void)hipStreamSynchronize(st);
+            const auto c0 = std::chrono::steady_clock::now();
+            rc = fw_batched_run(ctx, Ds, ns, nullptr, tc);  // captures
+            (void)hipStreamSynchronize(st);
+            const double first = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c0).count();
+            best = 1e30f;
+            for (int rep = 0; rep < 3 && rc == LG_OK; ++rep) {
+                (void)hipEventRecord(a, st);
+                rc = fw_batched_run(ctx, Ds, ns, nullptr, tc);  // replays
+                (void)hipEventRecord(b, st);
+                (void)hipEventSynchronize(b);
+                float ms = 0.f;
+                (void)hipEventElapsedTime(&ms, a, b);
+                best = std::min(best, ms);
+            }
+            t->graph_ms = best;
+            t->capture_ms = std::max(0.0, first - (double)best);
+        }
+        ctx->timing = timing;
+        if (ctx->fw_graph) {
+            (void)hipStreamSynchronize(st);
+            (void)hipGraphExecDestroy(ctx->fw_graph);
+            ctx->fw_graph = nullptr;
+            ctx->fw_graph_key.clear();
+        }
+        t->use_graph = (rc == LG_OK && t->graph_ms + t->capture_ms < t->stream_ms) ? 1 : 0;
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+    dfree(ctx, meta);
+    dfree(ctx, D);
+    if (rc != LG_OK) return rc;
+    t->tuned = 1;
+    return LG_OK;
+}
+
+static std::mutex g_fw_tune_mu;
+static std::map<int, lg_fw_tuning> g_fw_tuned;  // per device, measured once per process
+
+int32_t fw_tuning_init(lg_ctx *ctx) {
+    if (ctx->fw_tune_set) return LG_OK;
+    lg_fw_tuning t{};
+    t.split_below = kFwSplitBelowDefault;
+    if (env_int("LG_FW_AUTOTUNE", 1) != 0) {
+        std::lock_guard<std::mutex> lk(g_fw_tune_mu);
+        auto it = g_fw_tuned.find(ctx->device);
+        if (it != g_fw_tuned.end()) {
+            t = it->second;
+        } else {
+            lg_fw_tuning m{};
+            if (fw_autotune_run(ctx, &m) == LG_OK && m.tuned) {
+                t = m;
+                g_fw_tuned[ctx->device] = m;
+            }
+        }
+    }
+    ctx->fw_tune = t;
+    ctx->fw_tune_set = true;
+    return LG_OK;
+}
+
+int32_t fw_autotune(lg_ctx *ctx, lg_fw_tuning *out) {
+    std::lock_guard<std::mutex> lk(g_fw_tune_mu);
+    LG_TRY(fw_autotune_run(ctx, out));
+    if (out->tuned) g_fw_tuned[ctx->device] = *out;
+    return LG_OK;
+}
+
+static int32_t fw_tuning_active(lg_ctx *ctx, FwTuningChoice *out) {
+    LG_TRY(fw_tuning_init(ctx));
+    out->split_below = env_int("LG_FW_SPLIT_BELOW", ctx->fw_tune.split_below);
+    out->use_graph = env_int("LG_FW_GRAPH", ctx->fw_tune.use_graph) != 0 ? 1 : 0;
     return LG_OK;
 }
 

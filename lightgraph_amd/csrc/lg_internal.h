@@ -83,6 +83,11 @@ struct lg_ctx {
     // work counters of the last lg_render (kernel timing on), device u64 slots
     unsigned long long *render_stats = nullptr;
     uint64_t render_chunks = 0;
+    // batched Floyd-Warshall choices (lg_fw_tuning) and the last captured schedule
+    lg_fw_tuning fw_tune{};
+    bool fw_tune_set = false;
+    hipGraphExec_t fw_graph = nullptr;
+    std::vector<uint64_t> fw_graph_key;
     int cu_count = 256;
     void *stage_pipeline = nullptr;  // lg::Pipeline used by the single-stage entry points
 };
@@ -170,6 +175,10 @@ int32_t fw_solve_impl(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld);
 // with one pivot, one panel and one remainder launch per 128-pivot round.
 int32_t fw_solve_batched_impl(lg_ctx *ctx, const std::vector<float *> &Ds, const std::vector<int> &ns,
                               unsigned long long *stats = nullptr);
+// lg_fw_tuning of the context: set it from the per-device measurement (taken on
+// first use), or measure again
+int32_t fw_tuning_init(lg_ctx *ctx);
+int32_t fw_autotune(lg_ctx *ctx, lg_fw_tuning *out);
 int32_t fw_pred_impl(lg_ctx *ctx, int32_t n, const float *d_D, int32_t ld, int32_t *d_pred,
                      const int32_t *d_edges, const float *d_w, int32_t n_edges);
 

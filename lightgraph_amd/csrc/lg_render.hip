@@ -1264,6 +1264,31 @@ int32_t lg_render_workspace(int32_t width, int32_t height, int32_t samples, int3
     return LG_OK;
 }
 
+int32_t lg_fw_autotune(lg_ctx *ctx, lg_fw_tuning *out) {
+    LG_CHECK_ARG(ctx, "lg_fw_autotune: ctx is NULL");
+    lg_fw_tuning t{};
+    LG_TRY(fw_autotune(ctx, &t));
+    ctx->fw_tune = t;
+    ctx->fw_tune_set = true;
+    if (out) *out = t;
+    return LG_OK;
+}
+
+int32_t lg_fw_get_tuning(lg_ctx *ctx, lg_fw_tuning *out) {
+    LG_CHECK_ARG(ctx && out, "lg_fw_get_tuning: NULL argument");
+    if (!ctx->fw_tune_set) LG_TRY(fw_tuning_init(ctx));
+    *out = ctx->fw_tune;
+    return LG_OK;
+}
+
+int32_t lg_fw_set_tuning(lg_ctx *ctx, const lg_fw_tuning *t) {
+    LG_CHECK_ARG(ctx && t && t->split_below >= 0 && (t->use_graph == 0 || t->use_graph == 1),
+                 "lg_fw_set_tuning: bad argument");
+    ctx->fw_tune = *t;
+    ctx->fw_tune_set = true;
+    return LG_OK;
+}
+
 int32_t lg_fw_solve_batched(lg_ctx *ctx, int32_t count, const int32_t *h_n, float *const *h_D, const int32_t *h_ld) {
     LG_CHECK_ARG(ctx && count >= 0 && (count == 0 || (h_n && h_D && h_ld)), "lg_fw_solve_batched: bad argument");
     std::vector<float *> Ds((size_t)count);

@@ -304,6 +304,7 @@ int32_t lg_ctx_destroy(lg_ctx *ctx) {
     if (!ctx) return LG_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->fw_graph) (void)hipGraphExecDestroy(ctx->fw_graph);
     for (auto &kv : ctx->allocs) (void)hipFree(kv.first);
     ctx->allocs.clear();
     for (auto &m : ctx->marks)

@@ -64,6 +64,60 @@ def test_fw_batched_equals_single_and_oracle(lga, ctx, oracle):
         batch[k].free()
 
 
+def test_fw_split_and_graph_schedules_are_identical(lga, ctx):
+    """Quadrant-split tiles and the hipGraph-captured round schedule give the
+    bytes of the plain schedule; the captured graph is replayed three times on
+    fresh inputs of the same shape, with eager work and a synchronisation
+    between replays, and every replay equals the eager result."""
+    rng = np.random.default_rng(5)
+    sizes = [300, 129, 700, 1030, 256, 7, 513]
+    Ws = [_random_graph(rng, n, 2.5, integer=(k % 2 == 1)) for k, n in enumerate(sizes)]
+    inputs = [_padded(lga, W)[0] for W in Ws]
+
+    def run(split_below, use_graph):
+        lga.set_fw_tuning(ctx, split_below, use_graph)
+        bufs = [ctx.upload(Dm) for Dm in inputs]
+        lga.fw_solve_batched(ctx, sizes, bufs)
+        out = [b.read(np.float32, Dm.shape) for b, Dm in zip(bufs, inputs)]
+        for b in bufs:
+            b.free()
+        return out
+
+    saved = lga.fw_tuning(ctx)
+    try:
+        plain = run(0, 0)
+        split = run(1 << 30, 0)
+        for a, b in zip(plain, split):
+            np.testing.assert_array_equal(a, b)
+        for rep in range(3):  # capture, then two replays of the same schedule
+            graph = run(1 << 30, 1)
+            for a, b in zip(plain, graph):
+                np.testing.assert_array_equal(a, b)
+            eager = run(1 << 30, 0)  # eager work between replays
+            for a, b in zip(plain, eager):
+                np.testing.assert_array_equal(a, b)
+            ctx.sync()
+        for a, b in zip(plain, run(0, 1)):  # whole tiles, captured
+            np.testing.assert_array_equal(a, b)
+    finally:
+        lga.set_fw_tuning(ctx, saved["split_below"], saved["use_graph"])
+
+
+def test_fw_autotune_measures_both_choices(lga, ctx):
+    t = lga.fw_autotune(ctx)
+    assert t["tuned"] == 1 and len(t["tiles"]) == 12 and t["tiles"][0] == 1 and t["tiles"][-1] == 4095
+    assert all(x > 0 for x in t["whole_us"] + t["split_us"])
+    assert t["stream_ms"] > 0 and t["graph_ms"] > 0
+    assert 1 <= t["split_below"] and t["use_graph"] in (0, 1)
+    # the choice follows the table: split below the first size where it stops winning
+    wins = [s < w for s, w in zip(t["split_us"], t["whole_us"])]
+    want = next((n for n, w in zip(t["tiles"], wins) if not w), 1 << 30)
+    assert t["split_below"] == want
+    # whole-tile launches grow with the tile count once the GPU is full
+    assert t["whole_us"][-1] > t["whole_us"][0]
+    assert lga.fw_tuning(ctx)["split_below"] == t["split_below"]
+
+
 def test_fw_batched_empty_and_bad_ld(lga, ctx):
     lga.fw_solve_batched(ctx, [], [])
     b = ctx.alloc(4)
