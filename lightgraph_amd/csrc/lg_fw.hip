@@ -109,7 +109,12 @@ __device__ __forceinline__ void fw_pivot_tile(float *__restrict__ D, int ld, int
             make_float4(v[a][0], v[a][1], v[a][2], v[a][3]);
 }
 
-__global__ void __launch_bounds__(1024) k_fw_pivot(float *__restrict__ D, int ld, int kb) { fw_pivot_tile(D, ld, kb); }
+// One matrix: also clears the tile flags for the round's panel launch.
+__global__ void __launch_bounds__(1024) k_fw_pivot(float *__restrict__ D, int ld, int kb, int *__restrict__ flags,
+                                                   int nt) {
+    for (int t = threadIdx.x; t < nt; t += blockDim.x) flags[t] = 0;
+    fw_pivot_tile(D, ld, kb);
+}
 
 // One matrix of a batch: D (ld x ld, ld = 128 * nt), its tile flags at flags[flag_off].
 struct FwJob {
@@ -330,10 +335,13 @@ __device__ __forceinline__ void fw_minplus_quad(float *__restrict__ D, int ld, i
     }
 }
 
-template <bool PANEL>
+template <bool PANEL, int SPLIT>
 __global__ void __launch_bounds__(256) k_fw_minplus(float *__restrict__ D, int ld, int kb, int nt,
                                                      int *__restrict__ flags) {
-    fw_minplus_tile<PANEL>(D, ld, kb, nt, flags, (int)blockIdx.x);
+    if (SPLIT == 1)
+        fw_minplus_tile<PANEL>(D, ld, kb, nt, flags, (int)blockIdx.x);
+    else
+        fw_minplus_quad<PANEL>(D, ld, kb, nt, flags, (int)blockIdx.x / 4, (int)blockIdx.x % 4, nullptr);
 }
 
 // Batched panel / remainder launch: prefix[a] is the first tile of active job
@@ -359,6 +367,15 @@ __global__ void __launch_bounds__(256) k_fw_minplus_b(const FwJob *__restrict__ 
                                (int)blockIdx.x % (SPLIT * SPLIT), stats);
 }
 
+// ------------------------------------------------------------------ tuning ----
+// The closures's two choices (lg_fw_tuning): which launches split tiles
+// into quadrants, and whether the rounds run as a captured hipGraph.
+struct FwTuningChoice {
+    int split_below = 0;
+    int use_graph = 0;
+};
+static int32_t fw_tuning_active(lg_ctx *ctx, FwTuningChoice *out);
+
 int32_t fw_solve_impl(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld) {
     LG_CHECK_ARG(ld % TB == 0 && ld >= n && n >= 0, "fw: ld=%d must be a multiple of %d and >= n=%d", ld, TB, n);
     if (n <= 1) return LG_OK;
@@ -369,33 +386,33 @@ int32_t fw_solve_impl(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld) {
     const int T = nt - 1;
     const long long P = (long long)T * (T + 1) / 2;
     LG_CHECK_ARG(P < (1LL << 31), "fw: matrix too large");
+    FwTuningChoice tc;
+    LG_TRY(fw_tuning_active(ctx, &tc));
     for (int kb = 0; kb < nt; ++kb) {
         hipEvent_t e1, e2, e3;
         ktimer_begin(ctx, "k_fw_pivot", &e1);
-        k_fw_pivot<<<1, 1024, 0, ctx->stream>>>(d_D, ld, kb);
+        k_fw_pivot<<<1, 1024, 0, ctx->stream>>>(d_D, ld, kb, flags, nt);
         LG_LAUNCHED("k_fw_pivot");
         ktimer_end(ctx, "k_fw_pivot", e1);
         if (nt == 1) break;
         ktimer_begin(ctx, "k_fw_panel", &e2);
-        k_fw_minplus<true><<<nt - 1, 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
+        if (nt - 1 < tc.split_below)
+            k_fw_minplus<true, 2><<<4 * (nt - 1), 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
+        else
+            k_fw_minplus<true, 1><<<nt - 1, 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
         LG_LAUNCHED("k_fw_panel");
         ktimer_end(ctx, "k_fw_panel", e2);
         ktimer_begin(ctx, "k_fw_minplus", &e3);
-        k_fw_minplus<false><<<(unsigned)P, 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
+        if (P < tc.split_below && 4 * P < (1LL << 31))
+            k_fw_minplus<false, 2><<<(unsigned)(4 * P), 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
+        else
+            k_fw_minplus<false, 1><<<(unsigned)P, 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
         LG_LAUNCHED("k_fw_minplus");
         ktimer_end(ctx, "k_fw_minplus", e3);
     }
     return LG_OK;
 }
 
-// ------------------------------------------------------------------ tuning ----
-// The batched closure's two choices (lg_fw_tuning): which launches split tiles
-// into quadrants, and whether the rounds run as a captured hipGraph.
-struct FwTuningChoice {
-    int split_below = 0;
-    int use_graph = 0;
-};
-static int32_t fw_tuning_active(lg_ctx *ctx, FwTuningChoice *out);
 template <typename F>
 static int32_t fw_graph_capture(lg_ctx *ctx, const std::vector<uint64_t> &key, F &&rounds);
 static int32_t fw_batched_run(lg_ctx *ctx, const std::vector<float *> &Ds, const std::vector<int> &ns,
@@ -473,7 +490,7 @@ static int32_t fw_batched_run(lg_ctx *ctx, const std::vector<float *> &Ds, const
             if (timers) ktimer_end(ctx, "k_fw_pivot", e1);
             if (panel_blocks[kb] == 0) continue;
             if (timers) ktimer_begin(ctx, "k_fw_panel", &e2);
-            if (panel_blocks[kb] < tc.split_below)
+            if (panel_blocks[kb] < tc.split_below && panel_blocks[kb] < (1 << 29))
                 k_fw_minplus_b<true, 2><<<4 * panel_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + n, n, kb,
                                                                                      d_flags, stats);
             else
@@ -482,7 +499,7 @@ static int32_t fw_batched_run(lg_ctx *ctx, const std::vector<float *> &Ds, const
             LG_LAUNCHED("k_fw_panel_b");
             if (timers) ktimer_end(ctx, "k_fw_panel", e2);
             if (timers) ktimer_begin(ctx, "k_fw_minplus", &e3);
-            if (rem_blocks[kb] < tc.split_below)
+            if (rem_blocks[kb] < tc.split_below && rem_blocks[kb] < (1 << 29))
                 k_fw_minplus_b<false, 2><<<4 * rem_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + 2 * n, n,
                                                                                      kb, d_flags, stats);
             else
