@@ -195,7 +195,7 @@ __device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, i
                 *reinterpret_cast<const float4 *>(Yg + (long long)(kc + kk) * ld + c4 * 4);
         }
         __syncthreads();
-#pragma unroll 4
+#pragma unroll 1  // unrolled, the compiler folds only a third of the pairs into v_min3
         for (int kk = 0; kk < KC; kk += 2) {
             const float4 xa0 = *reinterpret_cast<const float4 *>(&Xs[kk][ty * 4]);
             const float4 xa1 = *reinterpret_cast<const float4 *>(&Xs[kk][64 + ty * 4]);
@@ -297,7 +297,7 @@ __device__ __forceinline__ void fw_minplus_quad(float *__restrict__ D, int ld, i
                 *reinterpret_cast<const float4 *>(Yg + (long long)(kc + kk) * ld + c4 * 4);
         }
         __syncthreads();
-#pragma unroll 4
+#pragma unroll 1  // unrolled, the compiler folds only a third of the pairs into v_min3
         for (int kk = 0; kk < KC; kk += 2) {
             const float4 xa = *reinterpret_cast<const float4 *>(&Xs[kk][ty * 4]);
             const float4 yb = *reinterpret_cast<const float4 *>(&Ys[kk][tx * 4]);
