@@ -40,9 +40,12 @@ PEAK_FP32_TFLOPS = 157.3
 #            scale/clamp/floor 3, accumulate 2, amortised exp 4);
 #   min-plus: one add + one min per (i, j, k) of each executed 128^3 tile update;
 #   small components: one add + one min per (i, j, k) of each component's closure.
+# kernel: (bound, work counter, algorithmic amount per unit).  The marches are
+# gathers: per trilinear sample 8 taps of 4 B (FP32 bricks) against HBM; the
+# Floyd-Warshall kernels are 2 VALU ops (add + min) per (i, j, k) update.
 ROOF = {
-    "k_light_rays": ("valu", "shadow_samples", 32.0),
-    "k_march_edges": ("valu", "edge_samples", 32.0),
+    "k_light_rays": ("hbm", "shadow_samples", 32.0),
+    "k_march_edges": ("hbm", "edge_samples", 32.0),
     "k_fw_minplus": ("valu", "fw_tiles", 2.0 * 128 ** 3),
     "k_fw_panel": ("valu", "fw_panel_tiles", 2.0 * 128 ** 3),
     "k_fw_pivot": ("valu", "fw_pivots", 2.0 * 128 ** 3),
@@ -190,12 +193,13 @@ def bench_render(lga, ctx, grid, shader, lights, solver, n, reps):
            "shadow_lane_efficiency": round(st["shadow_samples"] / max(st["shadow_lane_slots"], 1), 4)}
     sh_ms = phases.get("k_render_shadow", {}).get("ms")
     if sh_ms:
-        # the shadow phase: trilinear samples of the shadow marches (32 ops each, as k_light_rays)
-        achieved = st["shadow_samples"] * 32.0 / (sh_ms * 1e-3) / 1e12
-        out["roofline"] = {"kernel": "k_render_shadow", "bound": "valu", "achieved": round(achieved, 3),
-                           "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-                           "traffic": None, "work_unit": "shadow_samples", "ops_per_unit": 32.0,
-                           "shadow_samples_per_s": round(st["shadow_samples"] / (sh_ms * 1e-3), 1)}
+        # the shadow phase: trilinear samples of the shadow marches, 8 taps of 4 B each
+        achieved = st["shadow_samples"] * 32.0 / (sh_ms * 1e-3) / 1e9
+        out["roofline"] = {"kernel": "k_render_shadow", "bound": "hbm", "achieved": round(achieved, 2),
+                           "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                           "traffic": None, "work_unit": "shadow_samples", "bytes_per_unit": 32.0,
+                           "shadow_samples_per_s": round(st["shadow_samples"] / (sh_ms * 1e-3), 1),
+                           "note": "issue-bound on VALU (profiles/k_render_c2_1080p_kernel_trace.md); the taps hit L2"}
     return out
 
 
@@ -377,12 +381,18 @@ def kernel_roofline(times, work, workload):
     roof = {"kernel": name, "bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
             "traffic": None, "avg_launch_ms": t["avg_ms"], "launches": t["launches"]}
     if name in ROOF:
-        bound, key, ops = ROOF[name]
+        bound, key, per = ROOF[name]
         units = work.get(key, 0)
-        achieved = units * ops / (t["total_ms"] * 1e-3) / 1e12
-        roof.update({"bound": bound, "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "work_unit": key,
-                     "units_per_launch": round(units / t["launches"], 1), "ops_per_unit": ops})
+        if bound == "hbm":
+            achieved = units * per / (t["total_ms"] * 1e-3) / 1e9
+            roof.update({"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": round(achieved / PEAK_HBM_GBS, 4), "work_unit": key,
+                         "units_per_launch": round(units / t["launches"], 1), "bytes_per_unit": per})
+        else:
+            achieved = units * per / (t["total_ms"] * 1e-3) / 1e12
+            roof.update({"bound": bound, "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "work_unit": key,
+                         "units_per_launch": round(units / t["launches"], 1), "ops_per_unit": per})
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f).get(workload, {}).get(name)
