@@ -6,6 +6,12 @@
 #define LIGHTGRAPH_HOST_HPP_
 
 #include <algorithm>
+#include <cstdio>
+#include <sstream>
+#include <iostream>
+#include <fstream>
+#include <chrono>
+#include <sys/stat.h>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -191,6 +197,7 @@ struct SceneSettings {
     int threads = 0;
     bool verbose = false;
     bool geoDump = false;
+    std::string geoName = "volume";  // file-name stem of the geometry dumps (the vdb file's stem)
     // additions
     uint32_t points = 750;   // graph points per LightGraph iteration (TARGET_PC_SIZE, LightGraph.h:26)
     double radius = 0.0;     // edge radius in world units; 0: derived from the bbox (see LightGraph)
@@ -268,6 +275,85 @@ class LightGraph {
                 D[v] = Vec3d(dif[o], dif[o + 1], dif[o + 2]);
             }
         }
+        if (opts_.geoDump)
+            for (size_t k = 0; k < count; ++k) dumpGeo(begin + k);
+    }
+
+    // LightGraph::outputDebugGeo (LightGraph.h:604-696): five comma-separated
+    // text files per iteration under geoDump/: points, edges (i < j), the
+    // all-pairs transmittance of each edge after diffusion's column
+    // normalisation, vertex light and diffused radiance.  The iteration is solved
+    // again alone with the dense optical-depth output (the batched solve gives
+    // the same numbers); transmittance T_c = exp(extinction_c D) per channel,
+    // normalised by max(1, column sum) as diffuseLight does (LightGraph.h:563-571).
+    void dumpGeo(size_t iter) const {
+        const auto t0 = std::chrono::steady_clock::now();
+        Context &ctx = grid_.context();
+        const uint32_t n = opts_.points;
+        lg_solver *solver = nullptr;
+        check(lg_solver_create(ctx.get(), (int32_t)n, 0, LG_SOLVER_DENSE_D, &solver), "lg_solver_create");
+        std::unique_ptr<lg_solver, std::function<void(lg_solver *)>> guard(
+            solver, [&ctx](lg_solver *s) { lg_solver_destroy(ctx.get(), s); });
+        lg_solve_params sp{};
+        sp.n = (int32_t)n;
+        sp.max_connect = opts_.maxConnect;
+        sp.radius = radius();
+        sp.seed = opts_.seed;
+        sp.env_seed = opts_.envSeed;
+        sp.solve = (uint32_t)iter;
+        sp.flags = LG_SOLVE_DENSE;
+        const lg_march_params mp = opts_.march();
+        check(lg_solve(ctx.get(), solver, grid_.get(), temp_ ? temp_->get() : nullptr, &shader_.params(),
+                       opts_.light.data(), (int32_t)opts_.light.size(), &mp, &sp),
+              "lg_solve");
+        lg_solver_buffers b{};
+        check(lg_solver_get(solver, &b), "lg_solver_get");
+        std::vector<float> pts((size_t)n * 3), L((size_t)n * 3), dif((size_t)n * 3), Dm((size_t)n * b.ld);
+        std::vector<int32_t> edges((size_t)b.n_edges * 2);
+        check(lg_memcpy_d2h(ctx.get(), pts.data(), b.d_points, pts.size() * 4), "d2h points");
+        check(lg_memcpy_d2h(ctx.get(), L.data(), b.d_L, L.size() * 4), "d2h light");
+        check(lg_memcpy_d2h(ctx.get(), dif.data(), b.d_diffuse, dif.size() * 4), "d2h diffuse");
+        check(lg_memcpy_d2h(ctx.get(), Dm.data(), b.d_D, Dm.size() * 4), "d2h D");
+        if (b.n_edges > 0) check(lg_memcpy_d2h(ctx.get(), edges.data(), b.d_edges, edges.size() * 4), "d2h edges");
+        const double *ext = shader_.params().extinction;
+        // column normalisation of the transmittance matrix (D is symmetric)
+        std::vector<double> accum((size_t)n * 3, 0.0);
+        for (uint32_t j = 0; j < n; ++j)
+            for (uint32_t i = 0; i < n; ++i) {
+                const float d = Dm[(size_t)i * b.ld + j];
+                if (!(d < INFINITY)) continue;
+                for (int c = 0; c < 3; ++c) accum[3 * (size_t)j + c] += std::exp(ext[c] * (double)d);
+            }
+        for (auto &a : accum) a = std::max(a, 1.0);
+        struct stat st_dir {};
+        if (stat("geoDump", &st_dir) != 0 && mkdir("geoDump", 0755) == 0) std::cout << "Success" << "\n";
+        char tag[16];
+        std::snprintf(tag, sizeof(tag), "%04zu", iter);
+        auto path = [&](const char *kind) { return "geoDump/" + opts_.geoName + "_" + kind + "_" + tag + ".txt"; };
+        auto vec3 = [&](const char *kind, const std::vector<float> &a) {
+            std::ofstream f(path(kind));
+            for (uint32_t v = 0; v < n; ++v)
+                f << (double)a[3 * v] << "," << (double)a[3 * v + 1] << "," << (double)a[3 * v + 2] << "\n";
+        };
+        vec3("pc", pts);
+        {
+            std::ofstream fe(path("edge")), fm(path("marching"));
+            for (int32_t e = 0; e < b.n_edges; ++e) {
+                const int32_t i = edges[2 * e], j = edges[2 * e + 1];
+                fe << i << "," << j << "\n";
+                const float d = Dm[(size_t)i * b.ld + j];
+                double t[3] = {0.0, 0.0, 0.0};
+                if (d < INFINITY)
+                    for (int c = 0; c < 3; ++c) t[c] = std::exp(ext[c] * (double)d) / accum[3 * (size_t)j + c];
+                fm << t[0] << "," << t[1] << "," << t[2] << "\n";
+            }
+        }
+        vec3("light", L);
+        vec3("diffuse", dif);
+        if (opts_.verbose)
+            std::cout << "Geo Dump to Disk: "
+                      << std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() << " sec"
+                      << std::endl;
     }
 
    private:

@@ -52,7 +52,7 @@ void printHelp() {
         {"-tempColorRamp", "Define a color ramp to remap temperature values to emission colors",
          "float,...,float/float,float,float:...:float,float,float/float/float"},
         {"-v", "Enable verbose output", ""},
-        {"-geoDump", "Write LightGraph geometry to disk as ASCII files (not written by this build)", ""},
+        {"-geoDump", "Write LightGraph geometry to disk as ASCII files", ""},
         {"-h", "Print this help", ""},
         {"-help", "Print this help", ""},
         {"--help", "Print this help", ""},
@@ -157,7 +157,6 @@ int main(int argc, char *argv[]) {
             return 1;
         }
         if (imgFilename.empty()) imgFilename = scene + ".exr";
-        if (opts.geoDump) cout << "-geoDump: geometry dumps are not written by this build" << endl;
 
         lg_scene_spec spec{};
         spec.n = sceneSize;
@@ -182,6 +181,7 @@ int main(int argc, char *argv[]) {
         } else {
             throw Error("-scene must be sphere, blobs or plume", LG_ERR_INVALID);
         }
+        opts.geoName = scene;  // the dumps' stem (the reference uses the vdb file's stem)
         Context ctx(0);
         // lights: dir/point parse on the host; env needs its map on the device
         std::vector<std::unique_ptr<EnvMap>> envMaps;

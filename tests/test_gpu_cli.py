@@ -89,3 +89,48 @@ def test_cli_env_light_from_exr(lga, ctx, tmp_path):
          "-compression", "none", "-exrOut", out)
     c, _ = exrio.read(out)
     assert np.all(np.isfinite(c["direct.R"])) and c["direct.R"].max() > 0
+
+
+def test_cli_geo_dump(lga, ctx, tmp_path):
+    """-geoDump writes, per iteration, the reference's five comma-separated
+    files (LightGraph.h:604-696); their values are the API's solve outputs and
+    the edges' transmittance exp(-sigma D) normalised by max(1, column sum)."""
+    n, iters = 300, 2
+    args = ["-scene", "sphere", "-sceneSize", "48", "-sceneSeed", "5", "-points", str(n), "-iter", str(iters),
+            "-radius", "6", "-resWidth", "16", "-resHeight", "12", "-camPos", "23.5,23.5,120",
+            "-light", "dir:0:1,1,1:1:-45,30,0", "-scattering", "1,0.9,0.8", "-absorption", "0.1",
+            "-exrOut", str(tmp_path / "o.exr"), "-geoDump"]
+    r = subprocess.run([os.path.abspath(CLI), *args], capture_output=True, text=True, timeout=180, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d = tmp_path / "geoDump"
+    names = sorted(os.listdir(d))
+    assert names == sorted(f"sphere_{k}_{i:04d}.txt" for k in ("pc", "edge", "marching", "light", "diffuse")
+                           for i in range(iters))
+
+    def load(kind, i, dtype=float):
+        return np.loadtxt(d / f"sphere_{kind}_{i:04d}.txt", delimiter=",", dtype=dtype, ndmin=2)
+
+    spec = lga.scene_spec(lga.LG_SCENE_SPHERE_FBM, 48, seed=5, octaves=4, radius=0.375 * 48)
+    grid, _ = lga.Grid.generate(ctx, spec)
+    sh = lga.default_shader(absorption=0.1, scattering=(1.0, 0.9, 0.8))
+    p = lga.march_params()
+    lights = [lga.parse_light("dir:0:1,1,1:1:-45,30,0")]
+    s = lga.Solver(ctx, n, dense=True)
+    ext = np.array(sh.extinction)
+    for i in range(iters):
+        s.solve(grid, sh, lights, p, n=n, radius=6.0, solve=i)
+        res = s.results()
+        for kind, key in (("pc", "points"), ("light", "L"), ("diffuse", "diffuse")):
+            # the files hold 6 significant digits (the stream default of boost::format)
+            np.testing.assert_allclose(load(kind, i), res[key], rtol=1e-5, atol=1e-30, err_msg=kind)
+        e = load("edge", i, int)
+        np.testing.assert_array_equal(e, res["edges"])
+        D = res["D"][:n, :n].astype(np.float64)
+        T = np.exp(D[..., None] * ext[None, None, :])  # +INF -> 0
+        col = np.maximum(T.sum(axis=0), 1.0)
+        want = T[e[:, 0], e[:, 1]] / col[e[:, 1]]
+        got = load("marching", i)
+        assert got.shape == (len(e), 3) and np.all(got >= 0) and np.all(got <= 1)
+        np.testing.assert_allclose(got, want, rtol=2e-5, atol=1e-9)
+    s.destroy()
+    grid.destroy()
