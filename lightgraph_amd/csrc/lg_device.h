@@ -211,36 +211,54 @@ __device__ __forceinline__ bool isect_world_ray(const GridView &g, Isect &s, con
     return true;
 }
 
+// DDA<Ray, 3> set-up at time t0 (math/DDA.h): the brick of the entry point, and
+// per axis the step, the time of the first plane crossing and the time between
+// crossings.  The m-th crossing of axis c is at next0[c] + m * delta[c]: a closed
+// form (OpenVDB accumulates next += delta; the plane is the same), so that lanes
+// can time different crossings independently.
+__device__ __forceinline__ void dda_setup(const Isect &s, double t0, int vox[3], int step[3], double next0[3],
+                                          double delta[3]) {
+    double pos[3];
+    for (int c = 0; c < 3; ++c) {
+        pos[c] = s.e[c] + s.d[c] * t0;
+        vox[c] = ((int)floor(pos[c])) & ~7;
+    }
+    for (int c = 0; c < 3; ++c) {
+        if (s.d[c] == 0.0) {
+            step[c] = 0;
+            next0[c] = kDblMax;
+            delta[c] = kDblMax;
+        } else if (s.inv[c] > 0) {
+            step[c] = 8;
+            next0[c] = t0 + ((double)(vox[c] + 8) - pos[c]) * s.inv[c];
+            delta[c] = 8.0 * s.inv[c];
+        } else {
+            step[c] = -8;
+            next0[c] = t0 + ((double)vox[c] - pos[c]) * s.inv[c];
+            delta[c] = -8.0 * s.inv[c];
+        }
+    }
+}
+
+// math::MinIndex: the axis whose crossing comes first (ties: z, then y, then x).
+__device__ __forceinline__ int dda_min_axis(double n0, double n1, double n2) {
+    const int key = ((n0 < n1) << 2) + ((n0 < n2) << 1) + (n1 < n2);
+    return (key >= 6) ? 0 : ((key == 1 || key == 3) ? 1 : 2);
+}
+
 // One VolumeRayIntersector::march (Renderer.h:345-369) over the brick level:
 // the next span of consecutive existing 8^3 bricks along the ray.  Returns true
 // with [t0, t1] when a span longer than Delta<double> was found.
 __device__ __forceinline__ bool isect_march(const GridView &g, Isect &s, double &t0, double &t1) {
     double st0 = -1.0, st1 = -1.0;
     if (s.t1 - s.t0 > kDeltaF) {
-        // DDA<Ray, 3>::init
         double mT0 = s.t0;
         const double mT1 = s.t1;
-        double pos[3], next[3], delta[3];
+        double next0[3], delta[3];
         int vox[3], step[3];
-        for (int c = 0; c < 3; ++c) {
-            pos[c] = s.e[c] + s.d[c] * mT0;
-            vox[c] = ((int)floor(pos[c])) & ~7;
-        }
-        for (int c = 0; c < 3; ++c) {
-            if (s.d[c] == 0.0) {
-                step[c] = 0;
-                next[c] = kDblMax;
-                delta[c] = kDblMax;
-            } else if (s.inv[c] > 0) {
-                step[c] = 8;
-                next[c] = mT0 + ((double)(vox[c] + 8) - pos[c]) * s.inv[c];
-                delta[c] = 8.0 * s.inv[c];
-            } else {
-                step[c] = -8;
-                next[c] = mT0 + ((double)vox[c] - pos[c]) * s.inv[c];
-                delta[c] = -8.0 * s.inv[c];
-            }
-        }
+        dda_setup(s, mT0, vox, step, next0, delta);
+        double n0 = next0[0], n1 = next0[1], n2 = next0[2];
+        int m0 = 0, m1 = 0, m2 = 0;
         for (;;) {
             if (brick_exists(g, vox[0], vox[1], vox[2])) {
                 if (st0 < 0) st0 = mT0;
@@ -249,15 +267,146 @@ __device__ __forceinline__ bool isect_march(const GridView &g, Isect &s, double 
                 if (st1 - st0 > kDeltaD) break;
                 st0 = st1 = -1.0;
             }
-            // DDA::step with math::MinIndex
-            const int key = ((next[0] < next[1]) << 2) + ((next[0] < next[2]) << 1) + (next[1] < next[2]);
-            const int ax = (key >= 6) ? 0 : ((key == 1 || key == 3) ? 1 : 2);
-            mT0 = next[ax];
-            next[ax] += delta[ax];
-            vox[ax] += step[ax];
+            // DDA::step (constant indices keep the state in registers)
+            const int ax = dda_min_axis(n0, n1, n2);
+            if (ax == 0) {
+                mT0 = n0;
+                n0 = next0[0] + (double)(++m0) * delta[0];
+                vox[0] += step[0];
+            } else if (ax == 1) {
+                mT0 = n1;
+                n1 = next0[1] + (double)(++m1) * delta[1];
+                vox[1] += step[1];
+            } else {
+                mT0 = n2;
+                n2 = next0[2] + (double)(++m2) * delta[2];
+                vox[2] += step[2];
+            }
             if (!(mT0 <= mT1)) {
                 if (st0 >= 0) st1 = mT1;
                 break;
+            }
+        }
+    }
+    if (st1 > 0) {
+        s.t0 = st1 + kDeltaD;
+        s.t1 = s.tmax;
+    }
+    t0 = st0;
+    t1 = st1;
+    return (st1 - st0) > kDeltaD;
+}
+
+// isect_march for a wave whose 64 lanes share one ray (the same s in every
+// lane; every lane gets the same span).  Cells are taken 64 at a time: lane l
+// times the l-th coming crossing of each axis, each crossing's rank in the
+// merged order is its index plus the crossings of the other axes before it
+// (found by binary search over the lanes, ties ordered z, y, x as MinIndex
+// does), the crossings are placed by rank, and lane k then holds the k-th cell
+// of the walk and loads its brick flag; the span rules run on the flag mask.
+// The cells, times and spans are those of isect_march.  Workgroups of at most
+// 256 threads (a 64-entry exchange buffer per wave).
+__device__ __forceinline__ bool isect_march_wave(const GridView &g, Isect &s, double &t0, double &t1) {
+    __shared__ double ev_t[4][64];
+    __shared__ int ev_ax[4][64];
+    const int lane = threadIdx.x & 63, wv = (threadIdx.x >> 6) & 3;
+    double st0 = -1.0, st1 = -1.0;
+    if (s.t1 - s.t0 > kDeltaF) {
+        const double mT1 = s.t1;
+        double next0[3], delta[3];
+        int vox[3], step[3];
+        dda_setup(s, s.t0, vox, step, next0, delta);
+        int mc[3] = {0, 0, 0};  // crossings taken per axis
+        double tcur = s.t0;     // entry time of the current cell (cell 0 of the batch)
+        bool found = false, done = false;
+        while (!found && !done) {
+            // lane l: crossing mc + l of each axis
+            double c[3];
+            for (int a = 0; a < 3; ++a) c[a] = next0[a] + (double)(mc[a] + lane) * delta[a];
+            // rank of crossing (a, lane) = lane + crossings of the other axes before it
+            int rank[3];
+            for (int a = 0; a < 3; ++a) {
+                int r = lane;
+                for (int b = 0; b < 3; ++b) {
+                    if (b == a) continue;
+                    // count of j in [0, 64) with (c_b[j], b) before (c[a], a): ties put z, y, x first
+                    int lo = 0, hi = 64;
+                    for (int it = 0; it < 7; ++it) {
+                        const int mid = (lo + hi) >> 1;
+                        const double v = __shfl(c[b], mid < 64 ? mid : 63, 64);
+                        const bool before = mid < hi && (v < c[a] || (v == c[a] && b > a));
+                        lo = before ? mid + 1 : lo;
+                        hi = before ? hi : mid;
+                    }
+                    r += lo;
+                }
+                rank[a] = r;
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (int a = 0; a < 3; ++a)
+                if (rank[a] < 64) {
+                    ev_t[wv][rank[a]] = c[a];
+                    ev_ax[wv][rank[a]] = a;
+                }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // event k (k = 0..63) is the crossing into cell k + 1; cell k is entered at
+            // tcur (k = 0) or at event k - 1
+            const double te = ev_t[wv][lane];
+            const int ea = ev_ax[wv][lane];
+            const double tprev = __shfl_up(te, 1, 64);  // every lane takes part in the shuffle
+            const double tcell = lane == 0 ? tcur : tprev;
+            const unsigned long long mx = __ballot(ea == 0), my = __ballot(ea == 1), mz = __ballot(ea == 2);
+            const unsigned long long below = (1ull << lane) - 1ull;  // events before cell `lane`
+            const int cx = __popcll(mx & below), cy = __popcll(my & below), cz = __popcll(mz & below);
+            const int vx = vox[0] + cx * step[0], vy = vox[1] + cy * step[1], vz = vox[2] + cz * step[2];
+            const bool valid = tcell <= mT1;  // cell 0 always (t1 - t0 > Delta<float>)
+            const bool ex = valid && brick_exists(g, vx, vy, vz);
+            const unsigned long long V = __ballot(valid), E = __ballot(ex);
+            const int K = __popcll(V);  // cells of this batch on the ray (a prefix)
+            // span rules over cells 0..K-1 in order
+            int k = 0;
+            while (k < K) {
+                const unsigned long long from = (k >= 64) ? 0ull : ~((1ull << k) - 1ull);
+                if (st0 < 0) {
+                    const unsigned long long rem = E & from;
+                    if (!rem) break;
+                    k = __ffsll((long long)rem) - 1;
+                    if (k >= K) break;
+                    st0 = __shfl(tcell, k, 64);
+                    ++k;
+                } else {
+                    const unsigned long long rem = ~E & V & from;
+                    if (!rem) break;
+                    k = __ffsll((long long)rem) - 1;
+                    st1 = __shfl(tcell, k, 64);
+                    if (st1 - st0 > kDeltaD) {
+                        found = true;
+                        break;
+                    }
+                    st0 = st1 = -1.0;
+                    ++k;
+                }
+            }
+            if (found) break;
+            if (K < 64) {  // the walk left [t0, t1] inside this batch
+                if (st0 >= 0) st1 = mT1;
+                done = true;
+                break;
+            }
+            // next batch: cell 64, entered at event 63
+            const double t63 = __shfl(te, 63, 64);
+            vox[0] += __popcll(mx) * step[0];
+            vox[1] += __popcll(my) * step[1];
+            vox[2] += __popcll(mz) * step[2];
+            mc[0] += __popcll(mx);
+            mc[1] += __popcll(my);
+            mc[2] += __popcll(mz);
+            tcur = t63;
+            if (!(tcur <= mT1)) {
+                if (st0 >= 0) st1 = mT1;
+                done = true;
             }
         }
     }

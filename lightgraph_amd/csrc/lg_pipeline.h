@@ -176,7 +176,7 @@ __device__ __forceinline__ bool shadow_tau_wave(const lgd::GridView &g, const Li
     double tau = 0.0;
     double t0, t1;
     *dropped = false;
-    while (lgd::isect_march(g, is, t0, t1)) {
+    while (lgd::isect_march_wave(g, is, t0, t1)) {
         const double ts = lc.h * ceil(t0 / lc.h);
         for (int base = 0;; base += 64) {
             const double t = ts + (double)(base + lane) * lc.h;

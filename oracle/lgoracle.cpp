@@ -243,28 +243,34 @@ struct Ray {
 };
 
 // math::DDA<Ray, 3> (leaf level, 8^3 cells)
+// math::DDA over 8^3 bricks.  The m-th plane crossing of axis a is timed in
+// closed form, next0[a] + m * delta[a], where OpenVDB accumulates next += delta
+// (the same plane; the closed form lets 64 lanes of a wave time 64 crossings at
+// once on the device).
 struct DDA {
-    int voxel[3], step[3];
-    double t0, t1, next[3], delta[3];
+    int voxel[3], step[3], m[3];
+    double t0, t1, next0[3], next[3], delta[3];
     void init(const Ray &r) {
         t0 = r.t0;
         t1 = r.t1;
         const Vec3 pos = r.at(t0);
         for (int a = 0; a < 3; ++a) voxel[a] = ((int)std::floor(pos[a])) & ~7;
         for (int a = 0; a < 3; ++a) {
+            m[a] = 0;
             if (r.dir[a] == 0.0) {
                 step[a] = 0;
-                next[a] = DBL_MAX;
+                next0[a] = DBL_MAX;
                 delta[a] = DBL_MAX;
             } else if (r.inv[a] > 0) {
                 step[a] = 8;
-                next[a] = t0 + ((double)(voxel[a] + 8) - pos[a]) * r.inv[a];
+                next0[a] = t0 + ((double)(voxel[a] + 8) - pos[a]) * r.inv[a];
                 delta[a] = step[a] * r.inv[a];
             } else {
                 step[a] = -8;
-                next[a] = t0 + ((double)voxel[a] - pos[a]) * r.inv[a];
+                next0[a] = t0 + ((double)voxel[a] - pos[a]) * r.inv[a];
                 delta[a] = step[a] * r.inv[a];
             }
+            next[a] = next0[a];
         }
     }
     static int minIndex(const double v[3]) {
@@ -275,7 +281,8 @@ struct DDA {
     bool stepOnce() {
         const int a = minIndex(next);
         t0 = next[a];
-        next[a] += delta[a];
+        ++m[a];
+        next[a] = next0[a] + (double)m[a] * delta[a];
         voxel[a] += step[a];
         return t0 <= t1;
     }
