@@ -86,6 +86,10 @@ def batch_workload(lga, ctx, count, seed=3000):
 
 
 def bench_batched(lga, ctx, shader, params, lights, count, reps):
+    """C3: the 64 volumes solved one by one (plain, ascending pivot order: the
+    drop-in path) and as one batch, each in both pivot orders (ascending and
+    breadth-first, LG_SOLVE_BFS_ORDER).  Every speedup is against the plain
+    ascending-order pass; the roofline is that of the breadth-first batch."""
     import numpy as np
     radius = 12.0
     grids, sides, ns = batch_workload(lga, ctx, count)
@@ -93,49 +97,80 @@ def bench_batched(lga, ctx, shader, params, lights, count, reps):
     plain = lga.Solver(ctx, max(ns))
     batch = lga.Solver(ctx, sum(ns))
 
-    def run_plain(keep=False):
+    def run_plain(order, keep=False):
         outs = []
         for g, n, sv in items:
-            plain.solve(g, shader, lights, params, n=n, radius=radius, seed=854123, solve=sv)
+            plain.solve(g, shader, lights, params, n=n, radius=radius, seed=854123, solve=sv, order=order)
             if keep:
                 outs.append(plain.results(want_D=False)["diffuse"])
         return outs
 
-    def run_batch():
-        batch.solve_batch(items, shader, lights, params, radius=radius, seed=854123)
+    def run_batch(order, stats=False):
+        batch.solve_batch(items, shader, lights, params, radius=radius, seed=854123, order=order, stats=stats)
 
-    ref = run_plain(keep=True)
-    run_batch()
-    got = [r["diffuse"] for r in batch.batch_results()]
-    max_diff = max(float(np.max(np.abs(a - b))) if len(a) else 0.0 for a, b in zip(got, ref))
-    ctx.sync()
-    ctx.mark(2)
-    for _ in range(reps):
-        run_plain()
-    ctx.mark(3)
-    ctx.kernel_time_reset()
-    ctx.kernel_timing(True)
-    for _ in range(reps):
-        run_batch()
-    ctx.mark(4)
-    ctx.sync()
-    ctx.kernel_timing(False)
-    t_plain = ctx.elapsed_ms(2, 3) / reps
-    t_batch = ctx.elapsed_ms(3, 4) / reps
+    def rel_diff(got, ref):
+        worst = 0.0
+        for a, b in zip(got, ref):
+            if len(a):
+                a64, b64 = a.astype(np.float64), b.astype(np.float64)
+                worst = max(worst, float(np.max(np.abs(a64 - b64) / np.maximum(np.abs(b64), 1e-6 * np.abs(b64).max()))))
+        return worst
+
+    ref = run_plain("reference", keep=True)
+    ref_bfs = run_plain("bfs", keep=True)
+    diffs = {}
+    for order in ("reference", "bfs"):
+        run_batch(order)
+        got = [r["diffuse"] for r in batch.batch_results()]
+        same = ref if order == "reference" else ref_bfs
+        diffs[order] = {"max_abs_diff_vs_plain_same_order": max(float(np.max(np.abs(a - b))) if len(a) else 0.0
+                                                                for a, b in zip(got, same)),
+                        "max_rel_diff_vs_plain_reference_order": rel_diff(got, ref)}
+    ms = {}
+    for name, fn in (("plain", run_plain), ("batched", run_batch)):
+        for order in ("reference", "bfs"):
+            ctx.sync()
+            ctx.mark(2)
+            if name == "batched" and order == "bfs":
+                ctx.kernel_time_reset()
+                ctx.kernel_timing(True)
+            for _ in range(reps):
+                fn(order)
+            ctx.mark(3)
+            ctx.sync()
+            ctx.kernel_timing(False)
+            ms[(name, order)] = ctx.elapsed_ms(2, 3) / reps
     btimes = kernel_times(ctx, KERNELS + STAGES)
-    batch.solve_batch(items, shader, lights, params, radius=radius, seed=854123, stats=True)
-    bwork = {k: v * reps for k, v in batch.stats().items()}
-    broof = kernel_roofline(btimes, bwork, "c3_batch")
+    tiles = {}
+    for order in ("reference", "bfs"):
+        run_batch(order, stats=True)
+        st = batch.stats()
+        tiles[order] = int(st["fw_tiles"])
+    bwork = {k: v * reps for k, v in st.items()}
+    broof = kernel_roofline(btimes, bwork, "c3_batch_bfs_order")
     plain.destroy()
     batch.destroy()
     for g in grids:
         g.destroy()
+    t_plain = ms[("plain", "reference")]
+    per_s = {f"{a}_{b}": round(count * 1000.0 / t, 2) for (a, b), t in ms.items()}
     return {"workload": "c3_batch", "volumes": count, "side_range": [min(sides), max(sides)],
             "points_total": int(sum(ns)), "n_range": [min(ns), max(ns)], "edge_radius": radius, "reps": reps,
-            "plain_ms": round(t_plain, 3), "batched_ms": round(t_batch, 3),
-            "plain_solves_per_s": round(count * 1000.0 / t_plain, 2),
-            "batched_solves_per_s": round(count * 1000.0 / t_batch, 2),
-            "speedup": round(t_plain / t_batch, 3), "max_abs_diff_batched_vs_plain": max_diff,
+            "ms": {f"{a}_{b}": round(t, 3) for (a, b), t in ms.items()},
+            "solves_per_s": per_s,
+            "plain_ms": round(t_plain, 3), "batched_ms": round(ms[("batched", "reference")], 3),
+            "batched_bfs_ms": round(ms[("batched", "bfs")], 3),
+            "plain_solves_per_s": per_s["plain_reference"],
+            "batched_solves_per_s": per_s["batched_reference"],
+            "batched_bfs_solves_per_s": per_s["batched_bfs"],
+            "speedup": round(t_plain / ms[("batched", "reference")], 3),
+            "speedup_bfs": round(t_plain / ms[("batched", "bfs")], 3),
+            "speedup_note": "plain = one lg_solve per volume in ascending pivot order (the drop-in path); "
+                            "speedup = batched ascending order, speedup_bfs = batched breadth-first order, "
+                            "both over plain ascending order",
+            "fw_remainder_tiles_per_pass": tiles,
+            "diffs": diffs,
+            "max_abs_diff_batched_vs_plain": diffs["reference"]["max_abs_diff_vs_plain_same_order"],
             "roofline": broof,
             "kernels": {k: {"launches": v["launches"], "avg_ms": v["avg_ms"]} for k, v in btimes.items()}}
 

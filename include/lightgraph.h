@@ -304,10 +304,20 @@ typedef struct lg_solve_params {
     int32_t external_points; /* 1: use the points already in the solver's d_points */
     int32_t timing;        /* 1: record per-stage HIP events                       */
     int32_t flags;         /* LG_SOLVE_DENSE: also write the dense n x ld outputs;
-                            * LG_SOLVE_STATS: count the work (lg_solver_stats)       */
+                            * LG_SOLVE_STATS: count the work (lg_solver_stats);
+                            * LG_SOLVE_BFS_ORDER: see below                           */
 } lg_solve_params;
 
-enum { LG_SOLVE_DENSE = 1, LG_SOLVE_STATS = 2 };
+/* LG_SOLVE_BFS_ORDER (no reference counterpart; off in the drop-in path): close
+ * each component of more than 128 vertices in breadth-first vertex order (level
+ * from the component's smallest vertex, then vertex index) instead of ascending
+ * vertex order.  The shortest paths are the same; the blocked closure skips the
+ * many remainder tiles whose panel operands are still +INF in that order, and the
+ * FP32 sums are rounded in the new pivot order (distances and diffused radiance
+ * agree with the ascending order to FP32 rounding, bit-exact for integer
+ * weights).  Points, edges and the outputs keep the vertex indices of the call;
+ * at most 2^21 vertices per call. */
+enum { LG_SOLVE_DENSE = 1, LG_SOLVE_STATS = 2, LG_SOLVE_BFS_ORDER = 4 };
 
 /* Work done by the last lg_solve / lg_solve_batch call, counted on the device
  * when it ran with LG_SOLVE_STATS (otherwise the counters read 0).  The counts are
@@ -350,7 +360,8 @@ int32_t lg_fw_solve_batched(lg_ctx *ctx, int32_t count, const int32_t *h_n,
 /* Tile and schedule choices of the batched Floyd-Warshall, picked by a
  * measurement on the device (no reference counterpart: the reference runs one
  * scalar loop per solve).  split_below: panel and remainder launches of fewer
- * 128 x 128 tiles run four 64 x 64 workgroups per tile (results are identical
+ * 128 x 128 tiles run two 128 x 64 column-half workgroups per panel tile and
+ * four 64 x 64 quadrant workgroups per remainder tile (results are identical
  * either way).  use_graph: the rounds of a batch are captured once into a
  * hipGraph and replayed while the schedule's shape repeats.  The first batched
  * closure on a device runs lg_fw_autotune unless the environment sets

@@ -357,6 +357,17 @@ def diffuse(ctx, n, D_buf, ld, sigma_t, L_buf, out_buf):
     _check(lib().lg_diffuse(ctx.handle, n, D_buf.ptr, ld, sig, L_buf.ptr, out_buf.ptr), "diffuse")
 
 
+LG_SOLVE_DENSE, LG_SOLVE_STATS, LG_SOLVE_BFS_ORDER = 1, 2, 4
+
+
+def _order_flag(order):
+    if order == "reference":
+        return 0
+    if order == "bfs":
+        return LG_SOLVE_BFS_ORDER
+    raise ValueError(f"order must be 'reference' or 'bfs', not {order!r}")
+
+
 class Solver:
     """Per-solve workspace (lg_solver): LightGraph::operator() for one index."""
 
@@ -370,12 +381,16 @@ class Solver:
         self.last_dense = False
 
     def solve(self, density, shader, lights, params, n, radius, seed=854123, solve=0, max_connect=12,
-              temperature=None, env_seed=4564446, external_points=False, timing=False, dense=None, stats=False):
+              temperature=None, env_seed=4564446, external_points=False, timing=False, dense=None, stats=False,
+              order="reference"):
+        """order: "reference" closes each component in ascending vertex order (the
+        reference's pivot order); "bfs" closes the components of more than 128
+        vertices in breadth-first order (LG_SOLVE_BFS_ORDER)."""
         sp = lg_solve_params()
         sp.n, sp.max_connect, sp.radius, sp.seed, sp.env_seed = n, max_connect, radius, seed, env_seed
         sp.solve, sp.external_points, sp.timing = solve, 1 if external_points else 0, 1 if timing else 0
         dense = self.dense if dense is None else dense
-        sp.flags = (1 if dense else 0) | (2 if stats else 0)
+        sp.flags = (LG_SOLVE_DENSE if dense else 0) | (LG_SOLVE_STATS if stats else 0) | _order_flag(order)
         self.last_dense = dense
         arr = light_array(lights)
         _check(lib().lg_solve(self.ctx.handle, self.handle, density.handle,
@@ -383,9 +398,10 @@ class Solver:
                               C.byref(params), C.byref(sp)), "lg_solve")
 
     def solve_batch(self, items, shader, lights, params, radius, seed=854123, max_connect=12, env_seed=4564446,
-                    timing=False, stats=False):
+                    timing=False, stats=False, order="reference"):
         """items: sequence of (density Grid, n, solve index[, temperature Grid]).
-        Item k owns vertices [voff_k, voff_k + n_k) of the buffers."""
+        Item k owns vertices [voff_k, voff_k + n_k) of the buffers.  order as in
+        solve."""
         arr = (_abi.lg_batch_item * len(items))()
         self.batch_n = []
         for k, it in enumerate(items):
@@ -397,7 +413,7 @@ class Solver:
         sp = lg_solve_params()
         sp.max_connect, sp.radius, sp.seed, sp.env_seed = max_connect, radius, seed, env_seed
         sp.timing = 1 if timing else 0
-        sp.flags = 2 if stats else 0
+        sp.flags = (LG_SOLVE_STATS if stats else 0) | _order_flag(order)
         self.last_dense = False
         la = light_array(lights)
         _check(lib().lg_solve_batch(self.ctx.handle, self.handle, arr, len(items), C.byref(shader), la, len(lights),

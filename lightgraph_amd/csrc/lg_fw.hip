@@ -160,7 +160,7 @@ __device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, i
         Jb = b < kb ? b : b + 1;
         if (!flags[Ib] || !flags[Jb]) return;
     }
-    if (stats && threadIdx.x == 0) stat_add(stats, PANEL ? ST_FW_PANEL_TILES : ST_FW_TILES, 1ull);
+    if (!PANEL && stats && threadIdx.x == 0) stat_add(stats, ST_FW_TILES, 1ull);
     __shared__ float Xs[KC][TB];
     __shared__ float Ys[KC][TB];
     const int tid = threadIdx.x;
@@ -180,6 +180,17 @@ __device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, i
         acc[r][5] = hi.y;
         acc[r][6] = hi.z;
         acc[r][7] = hi.w;
+    }
+    if (PANEL) {
+        // the panel tile is also the Y operand: all +INF leaves it (and its
+        // flag, cleared by the caller) unchanged
+        int finite = 0;
+#pragma unroll
+        for (int r = 0; r < 8; ++r)
+#pragma unroll
+            for (int t = 0; t < 8; ++t) finite |= (acc[r][t] < INFINITY) ? 1 : 0;
+        if (!__syncthreads_or(finite)) return;
+        if (stats && threadIdx.x == 0) stat_add(stats, ST_FW_PANEL_TILES, 1ull);
     }
     const float *Xg = D + (long long)kb * TB * ld + (long long)Ib * TB;
     const float *Yg = D + (long long)kb * TB * ld + (long long)Jb * TB;
@@ -247,27 +258,113 @@ __device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, i
     }
 }
 
-// The same update for one 64 x 64 quadrant q (rows 64 (q >> 1).., columns
-// 64 (q & 1).. of the output tile), 4 x 4 outputs per thread: four workgroups
-// share a tile when a launch holds too few tiles to fill the GPU.  Each output
-// folds the pivots in the same pairs and order as fw_minplus_tile, so the
-// result is bit-identical.  A panel quadrant with a finite value sets flags[Jb]
-// to 1; the caller has cleared the flags.
-template <bool PANEL>
+// Panel update of one 128 x 64 column half h of the row-panel tile (kb, Jb),
+// 8 x 4 outputs per thread: two workgroups share a panel tile when a launch
+// holds too few tiles to fill the GPU.  The panel tile is its own Y operand, so
+// a workgroup may only read the columns it writes: a split by rows would let one
+// half read rows that the other has already rewritten.  Pivots fold in the same
+// pairs and order as fw_minplus_tile (bit-identical results).  A half with a
+// finite value sets flags[Jb] to 1; the caller has cleared the flags.
+__device__ __forceinline__ void fw_panel_half(float *__restrict__ D, int ld, int kb, int *__restrict__ flags,
+                                              int tile, int h, unsigned long long *__restrict__ stats) {
+    const int Jb = tile < kb ? tile : tile + 1;
+    __shared__ float Xs[KC][TB];
+    __shared__ float Ys[KC][64];
+    const int tid = threadIdx.x;
+    const int ty = tid >> 4, tx = tid & 15;
+    const int c0 = 64 * h;
+    float acc[8][4];
+    float *Cb = D + (long long)kb * TB * ld + (long long)Jb * TB + c0;
+    int finite = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int row = (r < 4) ? (ty * 4 + r) : (64 + ty * 4 + r - 4);
+        const float4 v = *reinterpret_cast<const float4 *>(Cb + (long long)row * ld + tx * 4);
+        acc[r][0] = v.x;
+        acc[r][1] = v.y;
+        acc[r][2] = v.z;
+        acc[r][3] = v.w;
+        finite |= (v.x < INFINITY) | (v.y < INFINITY) | (v.z < INFINITY) | (v.w < INFINITY);
+    }
+    // all +INF (the half is its own Y operand): nothing changes
+    if (!__syncthreads_or(finite)) return;
+    if (stats && h == 0 && threadIdx.x == 0) stat_add(stats, ST_FW_PANEL_TILES, 1ull);
+    const float *Xg = D + (long long)kb * TB * ld + (long long)kb * TB;
+    for (int kc = 0; kc < TB; kc += KC) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int idx = tid + q * 256;
+            const int kk = idx >> 5, c4 = idx & 31;
+            *reinterpret_cast<float4 *>(&Xs[kk][c4 * 4]) =
+                *reinterpret_cast<const float4 *>(Xg + (long long)(kc + kk) * ld + c4 * 4);
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const int idx = tid + q * 256;
+            const int kk = idx >> 4, c4 = idx & 15;
+            *reinterpret_cast<float4 *>(&Ys[kk][c4 * 4]) =
+                *reinterpret_cast<const float4 *>(Cb + (long long)(kc + kk) * ld + c4 * 4);
+        }
+        __syncthreads();
+#pragma unroll 1  // unrolled, the compiler folds only a third of the pairs into v_min3
+        for (int kk = 0; kk < KC; kk += 2) {
+            const float4 xa0 = *reinterpret_cast<const float4 *>(&Xs[kk][ty * 4]);
+            const float4 xa1 = *reinterpret_cast<const float4 *>(&Xs[kk][64 + ty * 4]);
+            const float4 yb = *reinterpret_cast<const float4 *>(&Ys[kk][tx * 4]);
+            const float4 xc0 = *reinterpret_cast<const float4 *>(&Xs[kk + 1][ty * 4]);
+            const float4 xc1 = *reinterpret_cast<const float4 *>(&Xs[kk + 1][64 + ty * 4]);
+            const float4 yd = *reinterpret_cast<const float4 *>(&Ys[kk + 1][tx * 4]);
+            const float a[8] = {xa0.x, xa0.y, xa0.z, xa0.w, xa1.x, xa1.y, xa1.z, xa1.w};
+            const float b[4] = {yb.x, yb.y, yb.z, yb.w};
+            const float c[8] = {xc0.x, xc0.y, xc0.z, xc0.w, xc1.x, xc1.y, xc1.z, xc1.w};
+            const float d[4] = {yd.x, yd.y, yd.z, yd.w};
+#pragma unroll
+            for (int r = 0; r < 8; ++r)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc[r][t] = fminf(acc[r][t], fminf(a[r] + b[t], c[r] + d[t]));
+        }
+    }
+    // every read of this half is done (the last chunk was staged before the
+    // barrier above), so the in-place writes cannot feed this workgroup's loads
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int row = (r < 4) ? (ty * 4 + r) : (64 + ty * 4 + r - 4);
+        *reinterpret_cast<float4 *>(Cb + (long long)row * ld + tx * 4) =
+            make_float4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+    }
+    // transpose into the column panel tile (Jb, kb)
+    float *Tb = D + (long long)Jb * TB * ld + (long long)kb * TB;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const long long col = c0 + tx * 4 + t;
+        *reinterpret_cast<float4 *>(Tb + col * ld + ty * 4) = make_float4(acc[0][t], acc[1][t], acc[2][t], acc[3][t]);
+        *reinterpret_cast<float4 *>(Tb + col * ld + 64 + ty * 4) =
+            make_float4(acc[4][t], acc[5][t], acc[6][t], acc[7][t]);
+    }
+    finite = 0;
+#pragma unroll
+    for (int r = 0; r < 8; ++r)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) finite |= (acc[r][t] < INFINITY) ? 1 : 0;
+    finite = __syncthreads_or(finite);
+    if (tid == 0 && finite) flags[Jb] = 1;
+}
+
+// The remainder update for one 64 x 64 quadrant q (rows 64 (q >> 1)..,
+// columns 64 (q & 1).. of the output tile), 4 x 4 outputs per thread: four
+// workgroups share a tile when a launch holds too few tiles to fill the GPU.
+// Its operands are panel tiles, which no remainder workgroup writes.  Each
+// output folds the pivots in the same pairs and order as fw_minplus_tile, so
+// the result is bit-identical.
 __device__ __forceinline__ void fw_minplus_quad(float *__restrict__ D, int ld, int kb, int nt, int *__restrict__ flags,
                                                 int tile, int q, unsigned long long *__restrict__ stats) {
-    int Ib, Jb;
-    if (PANEL) {
-        Ib = kb;
-        Jb = tile < kb ? tile : tile + 1;
-    } else {
-        int a, b;
-        tri_decode(tile, nt - 1, a, b);
-        Ib = a < kb ? a : a + 1;
-        Jb = b < kb ? b : b + 1;
-        if (!flags[Ib] || !flags[Jb]) return;
-    }
-    if (stats && q == 0 && threadIdx.x == 0) stat_add(stats, PANEL ? ST_FW_PANEL_TILES : ST_FW_TILES, 1ull);
+    int a, b;
+    tri_decode(tile, nt - 1, a, b);
+    const int Ib = a < kb ? a : a + 1;
+    const int Jb = b < kb ? b : b + 1;
+    if (!flags[Ib] || !flags[Jb]) return;
+    if (stats && q == 0 && threadIdx.x == 0) stat_add(stats, ST_FW_TILES, 1ull);
     __shared__ float Xs[KC][64];
     __shared__ float Ys[KC][64];
     const int tid = threadIdx.x;
@@ -285,6 +382,7 @@ __device__ __forceinline__ void fw_minplus_quad(float *__restrict__ D, int ld, i
     }
     const float *Xg = D + (long long)kb * TB * ld + (long long)Ib * TB + r0;
     const float *Yg = D + (long long)kb * TB * ld + (long long)Jb * TB + c0;
+
     for (int kc = 0; kc < TB; kc += KC) {
         __syncthreads();
 #pragma unroll
@@ -317,40 +415,42 @@ __device__ __forceinline__ void fw_minplus_quad(float *__restrict__ D, int ld, i
     for (int r = 0; r < 4; ++r)
         *reinterpret_cast<float4 *>(Cb + (long long)(r0 + ty * 4 + r) * ld + c0 + tx * 4) =
             make_float4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
-    if (PANEL || Ib != Jb) {
+    if (Ib != Jb) {
         float *Tb = D + (long long)Jb * TB * ld + (long long)Ib * TB;
 #pragma unroll
         for (int t = 0; t < 4; ++t)
             *reinterpret_cast<float4 *>(Tb + (long long)(c0 + tx * 4 + t) * ld + r0 + ty * 4) =
                 make_float4(acc[0][t], acc[1][t], acc[2][t], acc[3][t]);
     }
-    if (PANEL) {
-        int finite = 0;
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-            for (int t = 0; t < 4; ++t) finite |= (acc[r][t] < INFINITY) ? 1 : 0;
-        finite = __syncthreads_or(finite);
-        if (tid == 0 && finite) flags[Jb] = 1;
-    }
+}
+
+// Workgroups per 128 x 128 tile: one, or split (SPLIT 2) into four remainder
+// quadrants or two panel column halves.
+template <bool PANEL, int SPLIT>
+constexpr int fw_parts() {
+    return SPLIT == 1 ? 1 : (PANEL ? 2 : 4);
 }
 
 template <bool PANEL, int SPLIT>
 __global__ void __launch_bounds__(256) k_fw_minplus(float *__restrict__ D, int ld, int kb, int nt,
                                                      int *__restrict__ flags) {
-    if (SPLIT == 1)
+    constexpr int parts = fw_parts<PANEL, SPLIT>();
+    if constexpr (SPLIT == 1)
         fw_minplus_tile<PANEL>(D, ld, kb, nt, flags, (int)blockIdx.x);
+    else if constexpr (PANEL)
+        fw_panel_half(D, ld, kb, flags, (int)blockIdx.x / parts, (int)blockIdx.x % parts, nullptr);
     else
-        fw_minplus_quad<PANEL>(D, ld, kb, nt, flags, (int)blockIdx.x / 4, (int)blockIdx.x % 4, nullptr);
+        fw_minplus_quad(D, ld, kb, nt, flags, (int)blockIdx.x / parts, (int)blockIdx.x % parts, nullptr);
 }
 
 // Batched panel / remainder launch: prefix[a] is the first tile of active job
-// a; SPLIT 2 runs four quadrant workgroups per tile.
+// a; fw_parts workgroups per tile.
 template <bool PANEL, int SPLIT>
 __global__ void __launch_bounds__(256) k_fw_minplus_b(const FwJob *__restrict__ jobs, const int *__restrict__ active,
                                                        const int *__restrict__ prefix, int n_active, int kb,
                                                        int *__restrict__ flags, unsigned long long *__restrict__ stats) {
-    const int b = (int)blockIdx.x / (SPLIT * SPLIT);
+    constexpr int parts = fw_parts<PANEL, SPLIT>();
+    const int b = (int)blockIdx.x / parts;
     int lo = 0, hi = n_active - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
@@ -360,11 +460,12 @@ __global__ void __launch_bounds__(256) k_fw_minplus_b(const FwJob *__restrict__ 
             hi = mid - 1;
     }
     const FwJob j = jobs[active[lo]];
-    if (SPLIT == 1)
+    if constexpr (SPLIT == 1)
         fw_minplus_tile<PANEL>(j.D, j.ld, kb, j.nt, flags + j.flag_off, b - prefix[lo], stats);
+    else if constexpr (PANEL)
+        fw_panel_half(j.D, j.ld, kb, flags + j.flag_off, b - prefix[lo], (int)blockIdx.x % parts, stats);
     else
-        fw_minplus_quad<PANEL>(j.D, j.ld, kb, j.nt, flags + j.flag_off, b - prefix[lo],
-                               (int)blockIdx.x % (SPLIT * SPLIT), stats);
+        fw_minplus_quad(j.D, j.ld, kb, j.nt, flags + j.flag_off, b - prefix[lo], (int)blockIdx.x % parts, stats);
 }
 
 // ------------------------------------------------------------------ tuning ----
@@ -397,7 +498,7 @@ int32_t fw_solve_impl(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld) {
         if (nt == 1) break;
         ktimer_begin(ctx, "k_fw_panel", &e2);
         if (nt - 1 < tc.split_below)
-            k_fw_minplus<true, 2><<<4 * (nt - 1), 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
+            k_fw_minplus<true, 2><<<2 * (nt - 1), 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
         else
             k_fw_minplus<true, 1><<<nt - 1, 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
         LG_LAUNCHED("k_fw_panel");
@@ -491,7 +592,7 @@ static int32_t fw_batched_run(lg_ctx *ctx, const std::vector<float *> &Ds, const
             if (panel_blocks[kb] == 0) continue;
             if (timers) ktimer_begin(ctx, "k_fw_panel", &e2);
             if (panel_blocks[kb] < tc.split_below && panel_blocks[kb] < (1 << 29))
-                k_fw_minplus_b<true, 2><<<4 * panel_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + n, n, kb,
+                k_fw_minplus_b<true, 2><<<2 * panel_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + n, n, kb,
                                                                                      d_flags, stats);
             else
                 k_fw_minplus_b<true, 1><<<panel_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + n, n, kb,

@@ -372,6 +372,7 @@ struct Pipeline {
     int *parent = nullptr, *root = nullptr;
     uint64_t *keys = nullptr, *keys_sorted = nullptr;
     int *flag = nullptr, *cid = nullptr, *member = nullptr, *local = nullptr, *comp_off = nullptr;
+    int *level = nullptr;   // breadth-first level of each vertex inside its component (bfs_order)
     int *big_list = nullptr, *counters = nullptr;
     int *cls16 = nullptr, *cls48 = nullptr, *cls128 = nullptr;  // components of 2-32, 33-64, 65-128 vertices
     void *tmp_vertex = nullptr;
@@ -384,9 +385,10 @@ struct Pipeline {
            aw_cap = 0;
     // large components
     float *big_T = nullptr;
-    double *big_sums = nullptr;
+    float *big_ln = nullptr;  // normalised vertex light of the large components (3 per row)
     unsigned long long *stats = nullptr;  // ST_COUNT counters; kernels add to them when stats_on
     bool stats_on = false;
+    bool bfs_order = false;  // large components closed in breadth-first vertex order
     double *ray = nullptr;  // per (vertex, light sample) shadowed intensity
     size_t ray_cap = 0;
     // neighbour grid
@@ -397,7 +399,7 @@ struct Pipeline {
     size_t cells_cap = 0, cend_cap = 0;
     long long total_cells = 0;
     uint8_t *big_meta = nullptr;
-    size_t bigT_cap = 0, bigsums_cap = 0, bigmeta_cap = 0;
+    size_t bigT_cap = 0, bigln_cap = 0, bigmeta_cap = 0;
 
     int32_t reserve_vertices(lg_ctx *ctx, int V, int S);
     void release_vertex(lg_ctx *ctx);

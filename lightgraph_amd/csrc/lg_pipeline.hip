@@ -749,6 +749,71 @@ __global__ void k_comp_local(int V, const SolveDesc *__restrict__ sd, const int 
     }
 }
 
+// ------------------------------------------------ breadth-first order ------
+// Optional vertex order of the large components (LG_SOLVE_BFS_ORDER): members
+// sorted by (breadth-first level from the component's smallest vertex, vertex
+// index) instead of by vertex index.  The blocked closure of a geometric graph
+// then meets few finite panel tiles ahead of the pivot row: the vertices of
+// rows 0..k form a connected ball and only its rim reaches the later rows, so
+// most remainder tiles hold +INF operands and are skipped.  Distances are the
+// same shortest paths; the pivot order, and so the FP32 rounding of their
+// sums, follows the new order.
+
+// One workgroup per large component (slot b of big_list, counters[0] of them):
+// level-synchronous breadth-first search over the finite-edge adjacency.  A
+// vertex reached in pass L gets level L + 1 (every writer stores the same value);
+// the loop ends after the first pass that reaches nothing new, at most m passes.
+__global__ void __launch_bounds__(1024) k_bfs_levels(const int *__restrict__ counters,
+                                                     const int *__restrict__ big_list,
+                                                     const int *__restrict__ comp_off, const int *__restrict__ member,
+                                                     const int *__restrict__ aoff, const int *__restrict__ anb,
+                                                     int *__restrict__ level) {
+    const int b = (int)blockIdx.x;
+    if (b >= counters[0]) return;
+    const int c = big_list[2 * b], m = big_list[2 * b + 1];
+    const int a0 = comp_off[c];
+    constexpr int kUnseen = 0x7fffffff;
+    for (int i = threadIdx.x; i < m; i += blockDim.x) level[member[a0 + i]] = (i == 0) ? 0 : kUnseen;
+    __syncthreads();
+    for (int L = 0; L < m; ++L) {
+        int more = 0;
+        for (int i = threadIdx.x; i < m; i += blockDim.x) {
+            const int v = member[a0 + i];
+            if (level[v] != L) continue;
+            const int p1 = aoff[v + 1];
+            for (int p = aoff[v]; p < p1; ++p) {
+                const int u = anb[p];
+                if (level[u] == kUnseen) {
+                    level[u] = L + 1;
+                    more = 1;
+                }
+            }
+        }
+        if (!__syncthreads_or(more)) break;
+    }
+}
+
+// Sort keys (root, level, vertex); vertices of the other components keep level 0.
+__global__ void k_keys_bfs(int V, int bits, const int *__restrict__ root, const int *__restrict__ level,
+                           uint64_t *__restrict__ keys) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const uint64_t lmax = (1ull << bits) - 1ull;
+    const uint64_t l = (uint64_t)(unsigned)level[v] < lmax ? (uint64_t)(unsigned)level[v] : lmax;
+    keys[v] = ((((uint64_t)root[v] << bits) | l) << bits) | (uint64_t)v;
+}
+
+// member / local from the re-sorted keys; the components keep their ranges.
+__global__ void k_member_local(int V, int bits, const uint64_t *__restrict__ sorted, const int *__restrict__ flag,
+                               const int *__restrict__ cid_excl, const int *__restrict__ comp_off,
+                               int *__restrict__ member, int *__restrict__ local) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= V) return;
+    const int v = (int)(sorted[k] & ((1ull << bits) - 1ull));
+    member[k] = v;
+    local[v] = k - comp_off[cid_excl[k] + flag[k] - 1];
+}
+
 // ------------------------------------------------ small-component closure ----
 // One workgroup per component of m <= 128 vertices: build the m x m optical
 // depth matrix in LDS from the adjacency, close it pivot by pivot (ascending
@@ -974,10 +1039,13 @@ __global__ void __launch_bounds__(256) k_big_fill(const BigJob *__restrict__ job
     for (int p = aoff[v] + threadIdx.x; p < aoff[v + 1]; p += blockDim.x) row[local[anb[p]]] = aw[p];
 }
 
-// per-row sums of exp(ext_c * D) over the component (D symmetric: = column sums)
+// per-row sums of exp(ext_c * D) over the component (D symmetric: = column
+// sums), and from them the vertex's normalised light L_i / max(colsum_i, 1),
+// the factor that every row of the diffusion multiplies column i by.
 __global__ void __launch_bounds__(256) k_big_rowsum(const BigJob *__restrict__ jobs, const int *__restrict__ m_prefix,
-                                                    int J, const float *__restrict__ T, float e0, float e1, float e2,
-                                                    double *__restrict__ sums) {
+                                                    int J, const float *__restrict__ T, const int *__restrict__ member,
+                                                    const float *__restrict__ L, float e0, float e1, float e2,
+                                                    float *__restrict__ ln) {
     const int q = big_find(m_prefix, J, (int)blockIdx.x);
     const BigJob jb = jobs[q];
     const int i = (int)blockIdx.x - m_prefix[q];
@@ -999,16 +1067,21 @@ __global__ void __launch_bounds__(256) k_big_rowsum(const BigJob *__restrict__ j
             for (int c = 0; c < 3; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0)
-        for (int c = 0; c < 3; ++c) sums[3 * (long long)(jb.row0 + i) + c] = s[c][0];
+    if (threadIdx.x == 0) {
+        const int vi = member[jb.a0 + i];
+        for (int c = 0; c < 3; ++c) {
+            const double sc = s[c][0];
+            ln[3 * (long long)(jb.row0 + i) + c] = (float)((double)L[3 * vi + c] / (sc < 1.0 ? 1.0 : sc));
+        }
+    }
 }
 
-// diffuse_i = sum_j (L_j / max(colsum_j, 1)) * exp(ext * D_ij), scattered to the
-// component's global vertex ids.
+// diffuse_i = sum_j ln_j * exp(ext * D_ij), scattered to the component's global
+// vertex ids.
 __global__ void __launch_bounds__(256) k_big_diffuse(const BigJob *__restrict__ jobs, const int *__restrict__ m_prefix,
                                                      int J, const float *__restrict__ T, const int *__restrict__ member,
-                                                     const float *__restrict__ L, const double *__restrict__ sums,
-                                                     float e0, float e1, float e2, float *__restrict__ out) {
+                                                     const float *__restrict__ ln, float e0, float e1, float e2,
+                                                     float *__restrict__ out) {
     const int q = big_find(m_prefix, J, (int)blockIdx.x);
     const BigJob jb = jobs[q];
     const int i = (int)blockIdx.x - m_prefix[q];
@@ -1016,13 +1089,10 @@ __global__ void __launch_bounds__(256) k_big_diffuse(const BigJob *__restrict__ 
     double a0 = 0.0, a1 = 0.0, a2 = 0.0;
     for (int j = threadIdx.x; j < jb.m; j += blockDim.x) {
         const float d = row[j];
-        const int vj = member[jb.a0 + j];
-        const double *sj = sums + 3 * (long long)(jb.row0 + j);
-        float ln[3];
-        for (int c = 0; c < 3; ++c) ln[c] = (float)((double)L[3 * vj + c] / (sj[c] < 1.0 ? 1.0 : sj[c]));
-        a0 += (double)(ln[0] * trans_f(e0, d));
-        a1 += (double)(ln[1] * trans_f(e1, d));
-        a2 += (double)(ln[2] * trans_f(e2, d));
+        const float *lj = ln + 3 * (long long)(jb.row0 + j);
+        a0 += (double)(lj[0] * trans_f(e0, d));
+        a1 += (double)(lj[1] * trans_f(e1, d));
+        a2 += (double)(lj[2] * trans_f(e2, d));
     }
     __shared__ double s[3][256];
     s[0][threadIdx.x] = a0;
@@ -1136,6 +1206,7 @@ int32_t Pipeline::reserve_vertices(lg_ctx *ctx, int V, int S) {
     A(&cid, (size_t)v_cap + 1);
     A(&member, (size_t)v_cap);
     A(&local, (size_t)v_cap);
+    A(&level, (size_t)v_cap);
     A(&comp_off, (size_t)v_cap + 1);
     // counters[0..15] then the (component, size) pairs of the large components:
     // at most v_cap / (kSmallComp + 1) of them
@@ -1168,7 +1239,7 @@ int32_t Pipeline::reserve_vertices(lg_ctx *ctx, int V, int S) {
 void Pipeline::release_vertex(lg_ctx *ctx) {
     void *ps[] = {sd,     sid,  rowcnt,      off,    colcnt,       risky, visit,     visit_pos, visit_list,
                   adeg,   aoff, acur,        parent, root,         keys,  keys_sorted, flag,    cid,
-                  member, local, comp_off,   counters,   tmp_vertex, cd,     ckeys, ckeys_sorted,
+                  member, local, level, comp_off, counters, tmp_vertex, cd,  ckeys, ckeys_sorted,
                   cvals,  cvals_sorted, cls16, cls48, cls128};
     for (void *p : ps) dfree(ctx, p);
     cls16 = cls48 = cls128 = nullptr;
@@ -1177,7 +1248,7 @@ void Pipeline::release_vertex(lg_ctx *ctx) {
     ckeys = ckeys_sorted = nullptr;
     cvals = cvals_sorted = nullptr;
     sid = rowcnt = off = colcnt = visit = visit_pos = visit_list = adeg = aoff = acur = parent = root = nullptr;
-    flag = cid = member = local = comp_off = big_list = counters = nullptr;
+    flag = cid = member = local = level = comp_off = big_list = counters = nullptr;
     risky = nullptr;
     keys = keys_sorted = nullptr;
     tmp_vertex = nullptr;
@@ -1186,7 +1257,7 @@ void Pipeline::release_vertex(lg_ctx *ctx) {
 
 void Pipeline::release(lg_ctx *ctx) {
     release_vertex(ctx);
-    void *ps[] = {cand, accepted, epos, tmp_cand, edges_, w_, anb, aw, big_T, big_sums, big_meta, ray, cstart, cend,
+    void *ps[] = {cand, accepted, epos, tmp_cand, edges_, w_, anb, aw, big_T, big_ln, big_meta, ray, cstart, cend,
                   stats};
     for (void *p : ps) dfree(ctx, p);
     stats = nullptr;
@@ -1195,11 +1266,11 @@ void Pipeline::release(lg_ctx *ctx) {
     cand = accepted = epos = edges_ = anb = nullptr;
     w_ = aw = big_T = nullptr;
     big_meta = nullptr;
-    big_sums = nullptr;
+    big_ln = nullptr;
     ray = nullptr;
     tmp_cand = nullptr;
     cand_cap = accepted_cap = epos_cap = tmp_cand_cap = edges_cap = w_cap = anb_cap = aw_cap = 0;
-    bigT_cap = bigsums_cap = bigmeta_cap = ray_cap = 0;
+    bigT_cap = bigln_cap = bigmeta_cap = ray_cap = 0;
 }
 
 // Clears up to 8 int regions in one launch (a stage's counters and flags
@@ -1486,11 +1557,23 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     k_comp_local<<<vb, 256, 0, st>>>(V, pl.sd, pl.sid, pl.flag, pl.cid, pl.comp_off, pl.member, pl.local, pl.big_list,
                                      pl.counters, pl.cls16, pl.cls48, pl.cls128, L, out, pl.cid + V, d_nedges);
     LG_LAUNCHED("k_comp_local");
+    const int max_big = V / (kSmallComp + 1);
+    if (pl.bfs_order && max_big > 0) {
+        LG_CHECK_ARG(3 * bits <= 64, "breadth-first order: %d vertices in one call exceed 2^21", V);
+        LG_TRY(zero_ints(ctx, {{pl.level, (size_t)V}}));
+        k_bfs_levels<<<(unsigned)max_big, 1024, 0, st>>>(pl.counters, pl.big_list, pl.comp_off, pl.member, pl.aoff,
+                                                          pl.anb, pl.level);
+        k_keys_bfs<<<vb, 256, 0, st>>>(V, bits, pl.root, pl.level, pl.keys);
+        sbytes = pl.tmp_vertex_bytes;
+        LG_HIP(rocprim::radix_sort_keys(pl.tmp_vertex, sbytes, pl.keys, pl.keys_sorted, (size_t)V, 0, 3 * bits, st));
+        k_member_local<<<vb, 256, 0, st>>>(V, bits, pl.keys_sorted, pl.flag, pl.cid, pl.comp_off, pl.member,
+                                           pl.local);
+        LG_LAUNCHED("k_member_local");
+    }
     ktimer_end(ctx, "k_comp_sort", ev_s);
     // counts and the large-component pairs come back in one synchronisation
     // (counters[0..3], component count, edge count, then the large-component
     // pairs: big_list = counters + 16)
-    const int max_big = V / (kSmallComp + 1);
     std::vector<int> rb((size_t)16 + 2 * (size_t)max_big);
     LG_HIP(hipMemcpyAsync(rb.data(), pl.counters, sizeof(int) * rb.size(), hipMemcpyDeviceToHost, st));
     LG_HIP(hipStreamSynchronize(st));
@@ -1567,7 +1650,7 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     }
     // a0 = comp_off[c]: gathered on the device from the component ids
     LG_TRY(grow(ctx, &pl.big_T, &pl.bigT_cap, (size_t)toff));
-    LG_TRY(grow(ctx, &pl.big_sums, &pl.bigsums_cap, (size_t)3 * mrows));
+    LG_TRY(grow(ctx, &pl.big_ln, &pl.bigln_cap, (size_t)3 * mrows));
     const size_t jb_bytes = sizeof(BigJob) * nbig, pre_bytes = sizeof(int) * nbig;
     LG_TRY(grow(ctx, &pl.big_meta, &pl.bigmeta_cap, jb_bytes + 2 * pre_bytes + sizeof(int) * nbig + 256));
     BigJob *d_jobs = (BigJob *)pl.big_meta;
@@ -1591,9 +1674,9 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     for (int b = 0; b < nbig; ++b) mats[b] = pl.big_T + jobs[b].toff;
     LG_TRY(fw_solve_batched_impl(ctx, mats, ms, pl.stats_on ? pl.stats : nullptr));
     ktimer_begin(ctx, "k_diffuse", &ev);
-    k_big_rowsum<<<mrows, 256, 0, st>>>(d_jobs, d_mp, nbig, pl.big_T, e0, e1, e2, pl.big_sums);
+    k_big_rowsum<<<mrows, 256, 0, st>>>(d_jobs, d_mp, nbig, pl.big_T, pl.member, L, e0, e1, e2, pl.big_ln);
     LG_LAUNCHED("k_big_rowsum");
-    k_big_diffuse<<<mrows, 256, 0, st>>>(d_jobs, d_mp, nbig, pl.big_T, pl.member, L, pl.big_sums, e0, e1, e2, out);
+    k_big_diffuse<<<mrows, 256, 0, st>>>(d_jobs, d_mp, nbig, pl.big_T, pl.member, pl.big_ln, e0, e1, e2, out);
     LG_LAUNCHED("k_big_diffuse");
     ktimer_end(ctx, "k_diffuse", ev);
     if (any_dense) {

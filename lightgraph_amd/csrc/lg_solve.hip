@@ -320,6 +320,7 @@ static int32_t solve_descs(lg_ctx *ctx, lg_solver *s, const std::vector<SolveDes
     const bool t = sp->timing != 0;
     const int dtype = descs[0].dtype;
     pl.stats_on = (sp->flags & LG_SOLVE_STATS) != 0;
+    pl.bfs_order = (sp->flags & LG_SOLVE_BFS_ORDER) != 0;
     if (pl.stats_on) {
         if (!pl.stats) LG_TRY(dalloc(ctx, (size_t)ST_COUNT * kStatSlots, &pl.stats));
         LG_HIP(hipMemsetAsync(pl.stats, 0, sizeof(unsigned long long) * ST_COUNT * kStatSlots, ctx->stream));

@@ -103,6 +103,34 @@ def test_fw_split_and_graph_schedules_are_identical(lga, ctx):
         lga.set_fw_tuning(ctx, saved["split_below"], saved["use_graph"])
 
 
+def test_fw_split_panels_in_large_launches(lga, ctx):
+    """600 copies of one matrix in one batched closure: each split panel launch
+    holds 2400 workgroups, more than the GPU keeps resident, so late workgroups
+    start after others of the same tile have finished writing.  Every copy equals
+    the whole-tile closure of the same input, split or not."""
+    rng = np.random.default_rng(11)
+    W = _random_graph(rng, 300, 3.0, integer=False)
+    Dm = _padded(lga, W)[0]
+    count = 600
+    saved = lga.fw_tuning(ctx)
+    try:
+        results = []
+        for split_below in (0, 1 << 30):
+            lga.set_fw_tuning(ctx, split_below, 0)
+            bufs = [ctx.upload(Dm) for _ in range(count)]
+            lga.fw_solve_batched(ctx, [300] * count, bufs)
+            out = [b.read(np.float32, Dm.shape) for b in bufs]
+            for b in bufs:
+                b.free()
+            results.append(out)
+        want = results[0][0]
+        for out in results:
+            for k, got in enumerate(out):
+                np.testing.assert_array_equal(got, want, err_msg=f"copy {k}")
+    finally:
+        lga.set_fw_tuning(ctx, saved["split_below"], saved["use_graph"])
+
+
 def test_fw_autotune_measures_both_choices(lga, ctx):
     t = lga.fw_autotune(ctx)
     assert t["tuned"] == 1 and len(t["tiles"]) == 12 and t["tiles"][0] == 1 and t["tiles"][-1] == 4095
@@ -170,6 +198,36 @@ def test_solve_batch_equals_plain(lga, ctx, volumes):
         _, lab = connected_components(A, directed=False)
         largest = max(largest, np.bincount(lab).max())
     assert largest > 128, "the batch should hold a component for the blocked Floyd-Warshall"
+    bs.destroy()
+    plain.destroy()
+
+
+def test_solve_batch_bfs_order(lga, ctx, volumes):
+    """Breadth-first pivot order (LG_SOLVE_BFS_ORDER): the batch equals the plain
+    solves in the same order bit for bit, agrees with the ascending order within
+    FP32 rounding, and skips remainder tiles of the large components."""
+    sh = lga.default_shader(absorption=0.1, scattering=(1.0, 0.9, 0.8))
+    p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5)
+    lights = [lga.point_light(0.0, (1.0, 0.9, 0.8), 4.0e4, (64.0, 200.0, 64.0)),
+              lga.parse_light("dir:0:1,0.89,0.79:0.7:-41.8,-180,0")]
+    ns = [512, 1400, 3000, 700, 2100, 900]
+    items = [(g, n, 40 + k) for k, (g, n) in enumerate(zip(volumes, ns))]
+    bs = lga.Solver(ctx, sum(ns))
+    bs.solve_batch(items, sh, lights, p, radius=6.0, seed=854123, order="bfs", stats=True)
+    got = bs.batch_results()
+    tiles_bfs = bs.stats()["fw_tiles"]
+    bs.solve_batch(items, sh, lights, p, radius=6.0, seed=854123, stats=True)
+    asc = bs.batch_results()
+    tiles_asc = bs.stats()["fw_tiles"]
+    assert 0 < tiles_bfs < tiles_asc
+    plain = lga.Solver(ctx, max(ns))
+    for k, (g, n, solve) in enumerate(items):
+        plain.solve(g, sh, lights, p, n=n, radius=6.0, seed=854123, solve=solve, order="bfs")
+        ref = plain.results(want_D=False)
+        for key in ("points", "L", "diffuse", "edges", "weights"):
+            np.testing.assert_array_equal(got[k][key], ref[key], err_msg=f"item {k} ({key})")
+        d, a = got[k]["diffuse"].astype(np.float64), asc[k]["diffuse"].astype(np.float64)
+        assert np.max(np.abs(d - a) / np.maximum(np.abs(a), 1e-6 * np.abs(a).max())) < 1e-5, f"item {k}"
     bs.destroy()
     plain.destroy()
 

@@ -389,3 +389,39 @@ def test_full_solve_giant_component_pred(lga, ctx, oracle, c1):
     assert r["n_components"] < 10
     np.testing.assert_array_equal(r["pred"], _pred_rule(r["D"], r["edges"], r["weights"]))
     s.destroy()
+
+
+def test_full_solve_bfs_order(lga, ctx, oracle, c1):
+    """LG_SOLVE_BFS_ORDER on the C1 graph (one component far above 128 vertices):
+    the same points, edges, weights and vertex light as the ascending order; the
+    closure in breadth-first pivot order agrees with the FP64 oracle and with the
+    ascending order within FP32 rounding (1e-5 relative, same +INF pattern); its
+    predecessors follow the canonical rule; it computes no more remainder tiles."""
+    grid, dense, _ = c1
+    sh = _c1_shader(lga)
+    p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5)
+    lights = [lga.parse_light("dir:0:1,1,1:1:-45,30,0")]
+    n = 1024
+    s = lga.Solver(ctx, n, dense=True, with_pred=True)
+    s.solve(grid, sh, lights, p, n=n, radius=8.0, seed=854123, solve=5, stats=True)
+    ref_order = s.results()
+    tiles_ref = s.stats()["fw_tiles"]
+    s.solve(grid, sh, lights, p, n=n, radius=8.0, seed=854123, solve=5, stats=True, order="bfs")
+    r = s.results()
+    st = s.stats()
+    assert st["big_components"] >= 1 and st["fw_tiles"] <= tiles_ref
+    for k in ("points", "edges", "weights", "L"):
+        np.testing.assert_array_equal(r[k], ref_order[k], err_msg=k)
+    assert np.array_equal(np.isinf(r["D"]), np.isinf(ref_order["D"]))
+    fin = np.isfinite(r["D"])
+    assert rel_err(r["D"][fin], ref_order["D"][fin], floor=1e-30) < 1e-5
+    assert rel_err(r["diffuse"], ref_order["diffuse"]) < 1e-5
+    ref = oracle.solve_points(dense, sh, p, lights, r["points"], radius=8.0, cap=12, want_P=True)
+    tau_ref = np.where(ref["P"][..., 0] > 0, -np.log(np.maximum(ref["P"][..., 0], 1e-300)) / 1.1, np.inf)
+    assert np.array_equal(np.isinf(r["D"]), np.isinf(tau_ref))
+    assert rel_err(r["D"][fin], tau_ref[fin], floor=1e-30) < 1e-5
+    assert rel_err(r["diffuse"], ref["diffuse"]) < 1e-5
+    np.testing.assert_array_equal(r["pred"], _pred_rule(r["D"], r["edges"], r["weights"]))
+    with pytest.raises(ValueError):
+        s.solve(grid, sh, lights, p, n=n, radius=8.0, order="morton")
+    s.destroy()
