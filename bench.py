@@ -276,8 +276,10 @@ def bench_serving(lga, ctx, rates, jobs_per_rate, workers, max_batch, seed=5):
     jobs drawn from 12 seeded blob volumes of side 128..768 (step 64), N in
     [1000, 16000], images 512^2 / 1280x720 / 1920x1080 at 1-4 spp, one of three
     light sets (directional, point, environment); one LightGraph solve per job.
-    Latency = arrival -> films on the host.  The same job list is also run
-    through a one-worker, one-job-per-batch server (the plain path, back to back)."""
+    Latency = arrival -> films on the host.  The served runs close the graphs in
+    breadth-first pivot order; the same job list is also run through a
+    one-worker, one-job-per-batch server in the ascending (drop-in) order: the
+    plain path, back to back."""
     import numpy as np
     rng = np.random.default_rng(seed)
     sides = [int(x) for x in rng.integers(0, (768 - 128) // 64 + 1, 12) * 64 + 128]
@@ -296,14 +298,14 @@ def bench_serving(lga, ctx, rates, jobs_per_rate, workers, max_batch, seed=5):
                   [lga.env_light(env, 0.0, (1, 1, 1), 1.0, samples=1)]]
     images = [(512, 512), (1280, 720), (1920, 1080)]
 
-    def make_jobs(n):
+    def make_jobs(n, order):
         out = []
         for k in range(n):
             v = pool[int(rng.integers(12))]
             iw, ih = images[int(rng.integers(3))]
             cam = lga.camera_setup(cam_translate=(v.n / 2, v.n / 2, 2.8 * v.n), focal=50.0, width=iw, height=ih)
             out.append(lga.job(v, int(rng.integers(1000, 16001)), cam, sh, light_sets[int(rng.integers(3))], mp,
-                               radius=12.0, samples=int(rng.integers(1, 5)), solve_base=k))
+                               radius=12.0, samples=int(rng.integers(1, 5)), solve_base=k, order=order))
         return out
 
     def run(jobs, arrivals, nworkers, nbatch):
@@ -329,12 +331,14 @@ def bench_serving(lga, ctx, rates, jobs_per_rate, workers, max_batch, seed=5):
            "images": ["512x512", "1280x720", "1920x1080"], "spp": [1, 4], "workers": workers, "max_batch": max_batch,
            "rates": {}}
     for lam in rates:
-        jobs = make_jobs(jobs_per_rate)
+        jobs = make_jobs(jobs_per_rate, "bfs")
         arrivals = np.cumsum(rng.exponential(1.0 / lam, jobs_per_rate))
         out["rates"][str(lam)] = run(jobs, arrivals, workers, max_batch)
-    jobs = make_jobs(jobs_per_rate)
+    state = rng.bit_generator.state
+    jobs = make_jobs(jobs_per_rate, "bfs")
     out["saturated"] = run(jobs, np.zeros(len(jobs)), workers, max_batch)
-    out["plain"] = run(jobs, np.zeros(len(jobs)), 1, 1)
+    rng.bit_generator.state = state  # the same jobs in the drop-in pivot order
+    out["plain"] = run(make_jobs(jobs_per_rate, "reference"), np.zeros(len(jobs)), 1, 1)
     out["served_vs_plain"] = round(out["saturated"]["solves_per_s"] / out["plain"]["solves_per_s"], 3)
     env.destroy()
     return out

@@ -423,6 +423,8 @@ typedef struct lg_job {
     lg_march_params march;
     lg_camera camera;
     lg_light lights[LG_MAX_LIGHTS];
+    int32_t solve_flags;          /* 0 or LG_SOLVE_BFS_ORDER; jobs share a solve
+                                   * launch only with equal flags                  */
 } lg_job;
 
 typedef struct lg_job_result {

@@ -571,9 +571,12 @@ def render_workspace(width, height, samples, n_lights, max_points_per_ray=0):
 
 # ------------------------------------------------------------- serving -------
 def job(scene, points, camera, shader, lights, march, iterations=1, radius=12.0, max_connect=12, samples=1,
-        seed=854123, env_seed=4564446, render_seed=1, solve_base=0, pts_per_kernel=3, scatter_res_factor=4):
-    """An lg_job: volume -> `iterations` solves -> splat -> render."""
+        seed=854123, env_seed=4564446, render_seed=1, solve_base=0, pts_per_kernel=3, scatter_res_factor=4,
+        order="reference"):
+    """An lg_job: volume -> `iterations` solves -> splat -> render.  order: the
+    pivot order of the solves ("reference" or "bfs", as in Solver.solve)."""
     j = _abi.lg_job()
+    j.solve_flags = _order_flag(order)
     j.scene, j.points, j.iterations, j.radius, j.max_connect = scene, points, iterations, radius, max_connect
     j.samples, j.seed, j.env_seed, j.render_seed, j.solve_base = samples, seed, env_seed, render_seed, solve_base
     j.pts_per_kernel, j.scatter_res_factor = pts_per_kernel, scatter_res_factor

@@ -116,7 +116,8 @@ class lg_job(C.Structure):
                 ("max_connect", C.c_int32), ("samples", C.c_int32), ("seed", C.c_uint64), ("env_seed", C.c_uint64),
                 ("render_seed", C.c_uint64), ("solve_base", C.c_uint32), ("pts_per_kernel", C.c_int32),
                 ("scatter_res_factor", C.c_int32), ("n_lights", C.c_int32), ("shader", lg_shader),
-                ("march", lg_march_params), ("camera", lg_camera), ("lights", lg_light * LG_MAX_LIGHTS_)]
+                ("march", lg_march_params), ("camera", lg_camera), ("lights", lg_light * LG_MAX_LIGHTS_),
+                ("solve_flags", C.c_int32)]
 
 
 class lg_job_result(C.Structure):

@@ -258,23 +258,24 @@ __device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, i
     }
 }
 
-// Panel update of one 128 x 64 column half h of the row-panel tile (kb, Jb),
-// 8 x 4 outputs per thread: two workgroups share a panel tile when a launch
-// holds too few tiles to fill the GPU.  The panel tile is its own Y operand, so
-// a workgroup may only read the columns it writes: a split by rows would let one
-// half read rows that the other has already rewritten.  Pivots fold in the same
-// pairs and order as fw_minplus_tile (bit-identical results).  A half with a
-// finite value sets flags[Jb] to 1; the caller has cleared the flags.
-__device__ __forceinline__ void fw_panel_half(float *__restrict__ D, int ld, int kb, int *__restrict__ flags,
-                                              int tile, int h, unsigned long long *__restrict__ stats) {
-    const int Jb = tile < kb ? tile : tile + 1;
+// Panel update of one 128 x 64 column half h (columns 64 h..) of the row-panel
+// tile (kb, Jb), 8 x 4 outputs per thread: two workgroups share a panel tile
+// when a launch holds too few tiles to fill the GPU.  The panel tile is its own
+// Y operand, so a workgroup may only read the columns it writes: a split by
+// rows would let one part read rows that another part has already rewritten.
+// Pivots fold in the same pairs and order as fw_minplus_tile (bit-identical
+// results).  A half that is all +INF stays so and is skipped; a half with a
+// finite value sets flags[Jb] to 1 (the caller has cleared the flags).
+__device__ __forceinline__ void fw_panel_half(float *__restrict__ D, int ld, int kb, int *__restrict__ flags, int tile,
+                                              int h, unsigned long long *__restrict__ stats) {
+    const int Ib = kb, Jb = tile < kb ? tile : tile + 1;
     __shared__ float Xs[KC][TB];
     __shared__ float Ys[KC][64];
     const int tid = threadIdx.x;
     const int ty = tid >> 4, tx = tid & 15;
     const int c0 = 64 * h;
     float acc[8][4];
-    float *Cb = D + (long long)kb * TB * ld + (long long)Jb * TB + c0;
+    float *Cb = D + (long long)Ib * TB * ld + (long long)Jb * TB + c0;
     int finite = 0;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
@@ -286,10 +287,10 @@ __device__ __forceinline__ void fw_panel_half(float *__restrict__ D, int ld, int
         acc[r][3] = v.w;
         finite |= (v.x < INFINITY) | (v.y < INFINITY) | (v.z < INFINITY) | (v.w < INFINITY);
     }
-    // all +INF (the half is its own Y operand): nothing changes
-    if (!__syncthreads_or(finite)) return;
+    if (!__syncthreads_or(finite)) return;  // all +INF: nothing changes
     if (stats && h == 0 && threadIdx.x == 0) stat_add(stats, ST_FW_PANEL_TILES, 1ull);
-    const float *Xg = D + (long long)kb * TB * ld + (long long)kb * TB;
+    const float *Xg = D + (long long)kb * TB * ld + (long long)Ib * TB;
+    const float *Yg = D + (long long)kb * TB * ld + (long long)Jb * TB + c0;
     for (int kc = 0; kc < TB; kc += KC) {
         __syncthreads();
 #pragma unroll
@@ -304,7 +305,7 @@ __device__ __forceinline__ void fw_panel_half(float *__restrict__ D, int ld, int
             const int idx = tid + q * 256;
             const int kk = idx >> 4, c4 = idx & 15;
             *reinterpret_cast<float4 *>(&Ys[kk][c4 * 4]) =
-                *reinterpret_cast<const float4 *>(Cb + (long long)(kc + kk) * ld + c4 * 4);
+                *reinterpret_cast<const float4 *>(Yg + (long long)(kc + kk) * ld + c4 * 4);
         }
         __syncthreads();
 #pragma unroll 1  // unrolled, the compiler folds only a third of the pairs into v_min3
@@ -325,8 +326,8 @@ __device__ __forceinline__ void fw_panel_half(float *__restrict__ D, int ld, int
                 for (int t = 0; t < 4; ++t) acc[r][t] = fminf(acc[r][t], fminf(a[r] + b[t], c[r] + d[t]));
         }
     }
-    // every read of this half is done (the last chunk was staged before the
-    // barrier above), so the in-place writes cannot feed this workgroup's loads
+    // every global read of this workgroup happened before the last barrier, so
+    // the in-place writes below cannot feed its own loads
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
         const int row = (r < 4) ? (ty * 4 + r) : (64 + ty * 4 + r - 4);
@@ -334,7 +335,7 @@ __device__ __forceinline__ void fw_panel_half(float *__restrict__ D, int ld, int
             make_float4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
     }
     // transpose into the column panel tile (Jb, kb)
-    float *Tb = D + (long long)Jb * TB * ld + (long long)kb * TB;
+    float *Tb = D + (long long)Jb * TB * ld + (long long)Ib * TB;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
         const long long col = c0 + tx * 4 + t;
@@ -424,8 +425,9 @@ __device__ __forceinline__ void fw_minplus_quad(float *__restrict__ D, int ld, i
     }
 }
 
-// Workgroups per 128 x 128 tile: one, or split (SPLIT 2) into four remainder
-// quadrants or two panel column halves.
+// Workgroups per 128 x 128 tile: one, or split (SPLIT 2) into two panel column
+// halves or four remainder quadrants.  (Remainder halves measured slower than
+// quadrants at every launch size.)
 template <bool PANEL, int SPLIT>
 constexpr int fw_parts() {
     return SPLIT == 1 ? 1 : (PANEL ? 2 : 4);

@@ -86,7 +86,8 @@ double now_s(std::chrono::steady_clock::time_point t0) {
 
 // jobs that can share one lg_solve_batch
 bool same_solve_setup(const lg_job &a, const lg_job &b) {
-    return a.scene.dtype == b.scene.dtype && a.max_connect == b.max_connect && a.radius == b.radius &&
+    return a.scene.dtype == b.scene.dtype && a.solve_flags == b.solve_flags && a.max_connect == b.max_connect &&
+           a.radius == b.radius &&
            a.seed == b.seed && a.env_seed == b.env_seed && a.n_lights == b.n_lights &&
            std::memcmp(&a.shader, &b.shader, sizeof(lg_shader)) == 0 &&
            std::memcmp(&a.march, &b.march, sizeof(lg_march_params)) == 0 &&
@@ -312,6 +313,7 @@ struct Worker {
         sp.radius = h.radius;
         sp.seed = h.seed;
         sp.env_seed = h.env_seed;
+        sp.flags = h.solve_flags & LG_SOLVE_BFS_ORDER;
         if (rc == LG_OK)
             rc = lg_solve_batch(ctx, solver, items.data(), (int32_t)items.size(), &h.shader, h.lights, h.n_lights,
                                 &h.march, &sp);

@@ -27,6 +27,37 @@ def test_library_exports_every_declared_symbol(lga):
     assert lga.lib().lg_abi_version() == 1
 
 
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror of a header struct has the C size and field offsets
+    (gcc compiles a probe against include/lightgraph.h)."""
+    import shutil
+    import subprocess
+    from lightgraph_amd import _abi
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not found")
+    structs = {n: c for n, c in vars(_abi).items()
+               if n.startswith("lg_") and isinstance(c, type) and issubclass(c, C.Structure)}
+    assert len(structs) >= 15
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "lightgraph.h"', "int main(void) {"]
+    for n, c in sorted(structs.items()):
+        lines.append(f'    printf("{n} size %zu\\n", sizeof({n}));')
+        for f in c._fields_:
+            lines.append(f'    printf("{n} {f[0]} %zu\\n", offsetof({n}, {f[0]}));')
+    lines += ["    return 0;", "}"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        n, field, v = line.split()
+        got[(n, field)] = int(v)
+    for n, c in structs.items():
+        assert got[(n, "size")] == C.sizeof(c), f"{n}: C {got[(n, 'size')]} B, ctypes {C.sizeof(c)} B"
+        for f in c._fields_:
+            assert got[(n, f[0])] == getattr(c, f[0]).offset, f"{n}.{f[0]}"
+
+
 def test_shader_defaults_match_reference(lga):
     s = lga.default_shader()
     # Shader.h:123-138

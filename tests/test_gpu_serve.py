@@ -17,7 +17,7 @@ def _setup(lga):
     return sh, p, lights
 
 
-def _job(lga, kind, size, seed, n, w, h, spp, iterations=1, solve_base=0):
+def _job(lga, kind, size, seed, n, w, h, spp, iterations=1, solve_base=0, order="reference"):
     sh, p, lights = _setup(lga)
     if kind == "sphere":
         spec = lga.scene_spec(lga.LG_SCENE_SPHERE_FBM, size, seed=seed, octaves=4, radius=0.375 * size)
@@ -26,15 +26,16 @@ def _job(lga, kind, size, seed, n, w, h, spp, iterations=1, solve_base=0):
                               axis_max=size / 4, active_fraction=0.3)
     cam = lga.camera_setup(cam_translate=(size / 2, size / 2, 2.6 * size), focal=60.0, width=w, height=h)
     return lga.job(spec, n, cam, sh, lights, p, iterations=iterations, radius=6.0, samples=spp,
-                   solve_base=solve_base)
+                   solve_base=solve_base, order=order)
 
 
 JOBS = [("sphere", 64, 1, 600, 48, 40, 1, 2), ("blobs", 96, 7, 1500, 64, 48, 2, 1), ("blobs", 80, 8, 900, 40, 32, 1, 1),
         ("sphere", 64, 2, 700, 32, 32, 3, 2)]
 
 
-def test_alone_equals_mixed_and_direct(lga, ctx):
-    jobs = [_job(lga, *j) for j in JOBS]
+@pytest.mark.parametrize("order", ["reference", "bfs"])
+def test_alone_equals_mixed_and_direct(lga, ctx, order):
+    jobs = [_job(lga, *j, order=order) for j in JOBS]
     srv = lga.Server(8 * GB, workers=2, max_batch=4)
     alone = []
     for j in jobs:
@@ -57,7 +58,7 @@ def test_alone_equals_mixed_and_direct(lga, ctx):
     spec = lga.scene_spec(lga.LG_SCENE_SPHERE_FBM, size, seed=seed, octaves=4, radius=0.375 * size)
     grid, _ = lga.Grid.generate(ctx, spec)
     s = lga.Solver(ctx, n * iters)
-    s.solve_batch([(grid, n, i) for i in range(iters)], sh, lights, p, radius=6.0)
+    s.solve_batch([(grid, n, i) for i in range(iters)], sh, lights, p, radius=6.0, order=order)
     res = s.batch_results()
     pts = np.concatenate([r["points"] for r in res])
     dif = np.concatenate([r["diffuse"] for r in res])
