@@ -68,6 +68,11 @@ struct KernelTimer {
 struct lg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream for work that overlaps the main stream inside one call (a
+    // solve's vertex light runs beside its graph stages); always joined back
+    // into `stream` before the call returns
+    hipStream_t side = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint64_t budget = 0;         // 0 = unlimited
     uint64_t in_use = 0;
     std::map<void *, size_t> allocs;

@@ -416,8 +416,11 @@ int32_t pl_light(lg_ctx *ctx, Pipeline &pl, int V, int dtype, const LightConsts 
 int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float *pts, double r2, int max_connect,
                  int *Ktot_out);
 int32_t pl_march(lg_ctx *ctx, Pipeline &pl, int dtype, const MarchConsts &mc, const float *pts, int Ktot);
+// L_ready: when not null, an event after which L is complete (the stream
+// waits on it before the first kernel that reads L)
 int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, int V, int Ktot,
-                      const double ext[3], const float *L, float *out, PipelineResult *res);
+                      const double ext[3], const float *L, float *out, PipelineResult *res,
+                      hipEvent_t L_ready = nullptr);
 
 // helpers from lg_solve.hip
 int32_t fw_init_impl(lg_ctx *ctx, int n, float *d_D, int ld);

@@ -412,7 +412,7 @@ __global__ void __launch_bounds__(64) k_greedy(const SolveDesc *__restrict__ sd,
                 const uint64_t m = __ballot(elig);
                 const int need = cap - ci;
                 sel = elig && __popcll(m & ((1ull << lane) - 1ull)) < need;
-                taken = min(__popcll(m), need);
+                taken = min((int)__popcll(m), need);
             } else {
                 sel = elig;
             }
@@ -489,7 +489,7 @@ __global__ void __launch_bounds__(64) k_greedy_staged(const SolveDesc *__restric
                         const uint64_t m = __ballot(elig);
                         const int need = cap - ci;
                         sel = elig && __popcll(m & ((1ull << lane) - 1ull)) < need;
-                        taken = min(__popcll(m), need);
+                        taken = min((int)__popcll(m), need);
                     } else {
                         sel = elig;
                     }
@@ -569,7 +569,7 @@ __global__ void __launch_bounds__(64) k_greedy_staged(const SolveDesc *__restric
                     const uint64_t m = __ballot(elig);
                     const int need = cap - ci;
                     sel = elig && __popcll(m & ((1ull << lane) - 1ull)) < need;
-                    taken = min(__popcll(m), need);
+                    taken = min((int)__popcll(m), need);
                 } else {
                     sel = elig;
                 }
@@ -760,36 +760,53 @@ __global__ void k_comp_local(int V, const SolveDesc *__restrict__ sd, const int 
 // sums, follows the new order.
 
 // One workgroup per large component (slot b of big_list, counters[0] of them):
-// level-synchronous breadth-first search over the finite-edge adjacency.  A
-// vertex reached in pass L gets level L + 1 (every writer stores the same value);
-// the loop ends after the first pass that reaches nothing new, at most m passes.
+// level-synchronous breadth-first search over the finite-edge adjacency with
+// frontier queues (qa, qb: V ints each; the component uses its member range
+// [a0, a0 + m) of both).  A vertex is claimed by the CAS that moves it from
+// unseen to L + 1, so it enters the next frontier once; the queue order varies
+// between runs, the levels do not.  Every member is reachable from the first
+// (the component's smallest vertex), so each frontier holds at most m vertices.
 __global__ void __launch_bounds__(1024) k_bfs_levels(const int *__restrict__ counters,
                                                      const int *__restrict__ big_list,
                                                      const int *__restrict__ comp_off, const int *__restrict__ member,
                                                      const int *__restrict__ aoff, const int *__restrict__ anb,
-                                                     int *__restrict__ level) {
+                                                     int *__restrict__ level, int *__restrict__ qa,
+                                                     int *__restrict__ qb) {
     const int b = (int)blockIdx.x;
     if (b >= counters[0]) return;
     const int c = big_list[2 * b], m = big_list[2 * b + 1];
     const int a0 = comp_off[c];
     constexpr int kUnseen = 0x7fffffff;
+    __shared__ int n_next;
     for (int i = threadIdx.x; i < m; i += blockDim.x) level[member[a0 + i]] = (i == 0) ? 0 : kUnseen;
+    __threadfence();  // the CAS below runs at device scope
+    int *cur = qa + a0, *nxt = qb + a0;
+    if (threadIdx.x == 0) {
+        cur[0] = member[a0];
+        n_next = 0;
+    }
     __syncthreads();
-    for (int L = 0; L < m; ++L) {
-        int more = 0;
-        for (int i = threadIdx.x; i < m; i += blockDim.x) {
-            const int v = member[a0 + i];
-            if (level[v] != L) continue;
+    int n_cur = 1;
+    for (int L = 0; n_cur > 0 && L < m; ++L) {
+        for (int t = threadIdx.x; t < n_cur; t += blockDim.x) {
+            const int v = cur[t];
             const int p1 = aoff[v + 1];
             for (int p = aoff[v]; p < p1; ++p) {
                 const int u = anb[p];
-                if (level[u] == kUnseen) {
-                    level[u] = L + 1;
-                    more = 1;
+                if (atomicCAS(&level[u], kUnseen, L + 1) == kUnseen) {
+                    const int slot = atomicAdd(&n_next, 1);
+                    if (slot < m) nxt[slot] = u;
                 }
             }
         }
-        if (!__syncthreads_or(more)) break;
+        __syncthreads();
+        n_cur = n_next < m ? n_next : m;
+        __syncthreads();
+        if (threadIdx.x == 0) n_next = 0;
+        int *t = cur;
+        cur = nxt;
+        nxt = t;
+        __syncthreads();
     }
 }
 
@@ -1524,7 +1541,7 @@ int32_t pl_march(lg_ctx *ctx, Pipeline &pl, int dtype, const MarchConsts &mc, co
 
 // Components, closure (c) and diffusion; dense outputs when the descriptors ask.
 int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, int V, int Ktot,
-                      const double ext[3], const float *L, float *out, PipelineResult *res) {
+                      const double ext[3], const float *L, float *out, PipelineResult *res, hipEvent_t L_ready) {
     hipStream_t st = ctx->stream;
     const unsigned vb = (unsigned)((V + 255) / 256);
     const int *d_nedges = pl.epos + Ktot;
@@ -1554,6 +1571,7 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     k_comp_flags<<<vb, 256, 0, st>>>(V, bits, pl.keys_sorted, pl.flag, pl.member);
     LG_TRY(scan_excl(ctx, pl, pl.flag, pl.cid, (size_t)V + 1, false));
     k_comp_offsets<<<vb, 256, 0, st>>>(V, pl.flag, pl.cid, pl.comp_off, pl.cid + V);
+    if (L_ready) LG_HIP(hipStreamWaitEvent(st, L_ready, 0));  // k_comp_local is the first reader of L
     k_comp_local<<<vb, 256, 0, st>>>(V, pl.sd, pl.sid, pl.flag, pl.cid, pl.comp_off, pl.member, pl.local, pl.big_list,
                                      pl.counters, pl.cls16, pl.cls48, pl.cls128, L, out, pl.cid + V, d_nedges);
     LG_LAUNCHED("k_comp_local");
@@ -1561,8 +1579,10 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     if (pl.bfs_order && max_big > 0) {
         LG_CHECK_ARG(3 * bits <= 64, "breadth-first order: %d vertices in one call exceed 2^21", V);
         LG_TRY(zero_ints(ctx, {{pl.level, (size_t)V}}));
+        // the frontier queues borrow the 64-bit key buffer (2V ints), refilled below
+        int *qa = reinterpret_cast<int *>(pl.keys);
         k_bfs_levels<<<(unsigned)max_big, 1024, 0, st>>>(pl.counters, pl.big_list, pl.comp_off, pl.member, pl.aoff,
-                                                          pl.anb, pl.level);
+                                                          pl.anb, pl.level, qa, qa + V);
         k_keys_bfs<<<vb, 256, 0, st>>>(V, bits, pl.root, pl.level, pl.keys);
         sbytes = pl.tmp_vertex_bytes;
         LG_HIP(rocprim::radix_sort_keys(pl.tmp_vertex, sbytes, pl.keys, pl.keys_sorted, (size_t)V, 0, 3 * bits, st));

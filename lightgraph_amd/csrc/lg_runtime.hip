@@ -293,6 +293,16 @@ int32_t lg_ctx_create(int32_t device, uint64_t hbm_budget_bytes, lg_ctx **out) {
         set_error("hipStreamCreate failed: %s", hipGetErrorString(e));
         return LG_ERR_HIP;
     }
+    if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+        if (ctx->side) (void)hipStreamDestroy(ctx->side);
+        if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        set_error("lg_ctx_create: side stream or events could not be created");
+        return LG_ERR_HIP;
+    }
     for (auto &m : ctx->marks) (void)hipEventCreate(&m);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) ctx->cu_count = prop.multiProcessorCount;
@@ -303,6 +313,7 @@ int32_t lg_ctx_create(int32_t device, uint64_t hbm_budget_bytes, lg_ctx **out) {
 int32_t lg_ctx_destroy(lg_ctx *ctx) {
     if (!ctx) return LG_OK;
     (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->side);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->fw_graph) (void)hipGraphExecDestroy(ctx->fw_graph);
     for (auto &kv : ctx->allocs) (void)hipFree(kv.first);
@@ -315,6 +326,9 @@ int32_t lg_ctx_destroy(lg_ctx *ctx) {
     }
     for (auto e : ctx->ktimer.pool) (void)hipEventDestroy(e);
     destroy_stage_pipeline(ctx);
+    (void)hipEventDestroy(ctx->ev_fork);
+    (void)hipEventDestroy(ctx->ev_join);
+    (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LG_OK;

@@ -330,14 +330,42 @@ static int32_t solve_descs(lg_ctx *ctx, lg_solver *s, const std::vector<SolveDes
     LG_TRY(pl_begin(ctx, pl, descs, &V, &nmax));
     if (!pp.external_points) LG_TRY(pl_sample(ctx, pl, V, dtype, pp.seed, pp.pts));
     if (t) LG_HIP(hipEventRecord(s->ev[1], ctx->stream));
-    LG_TRY(pl_light(ctx, pl, V, dtype, pp.lc, pp.lights, pp.env_seed, pp.pts, pp.L));
+    // The vertex light needs only the points, and its first reader is the
+    // component stage: it runs on the side stream beside the graph, edge march
+    // and component sort (with per-stage timing on, the stages run in sequence
+    // so that each interval times one stage).
+    const bool overlap = !t;
+    hipEvent_t L_ready = nullptr;
+    if (overlap) {
+        LG_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+        LG_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+        std::swap(ctx->stream, ctx->side);
+        const int32_t rc = pl_light(ctx, pl, V, dtype, pp.lc, pp.lights, pp.env_seed, pp.pts, pp.L);
+        std::swap(ctx->stream, ctx->side);
+        LG_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+        L_ready = ctx->ev_join;
+        if (rc != LG_OK) {
+            (void)hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
+            return rc;
+        }
+    } else {
+        LG_TRY(pl_light(ctx, pl, V, dtype, pp.lc, pp.lights, pp.env_seed, pp.pts, pp.L));
+    }
+    // every return from here on leaves the main stream ordered after the light
+    struct JoinGuard {
+        lg_ctx *ctx;
+        hipEvent_t ev;
+        ~JoinGuard() {
+            if (ev) (void)hipStreamWaitEvent(ctx->stream, ev, 0);
+        }
+    } join{ctx, L_ready};
     if (t) LG_HIP(hipEventRecord(s->ev[2], ctx->stream));
     LG_TRY(pl_graph(ctx, pl, V, (int)descs.size(), nmax, pp.pts, pp.r2, pp.max_connect, &Ktot));
     if (t) LG_HIP(hipEventRecord(s->ev[3], ctx->stream));
     LG_TRY(pl_march(ctx, pl, dtype, pp.mc, pp.pts, Ktot));
     if (t) LG_HIP(hipEventRecord(s->ev[4], ctx->stream));
     PipelineResult res;
-    LG_TRY(pl_components(ctx, pl, descs, V, Ktot, pp.ext, pp.L, pp.out, &res));
+    LG_TRY(pl_components(ctx, pl, descs, V, Ktot, pp.ext, pp.L, pp.out, &res, L_ready));
     if (t) LG_HIP(hipEventRecord(s->ev[5], ctx->stream));
     s->timed = t;
     s->buf.n = V;
