@@ -69,8 +69,10 @@ struct lg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     // second stream for work that overlaps the main stream inside one call (a
-    // solve's vertex light runs beside its graph stages); always joined back
-    // into `stream` before the call returns
+    // solve's vertex light beside its graph stages, the small-component
+    // closures beside the blocked one); always joined back into `stream` before
+    // the call returns.  The events are re-recorded per fork: a stream wait
+    // binds to the record current when the wait is enqueued.
     hipStream_t side = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     uint64_t budget = 0;         // 0 = unlimited

@@ -1618,28 +1618,54 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
         any_dense = any_dense || d.dense_D || d.dense_pred;
     }
 
-    // (c) + diffusion: small components in one launch
+    // (c) + diffusion: small components, one launch per size class.  They share
+    // no output with the large components (disjoint rows of out and of the dense
+    // matrices), so when large ones follow they run on the side stream, beside
+    // the blocked closure, and the main stream joins them before returning.
     const float e0 = (float)ext[0], e1 = (float)ext[1], e2 = (float)ext[2];
-    ktimer_begin(ctx, "k_comp_small", &ev);
-    if (n128 > 0) {
-        k_comp_small<kSmallComp><<<n128, 1024, 0, st>>>(pl.sd, pl.sid, pl.cls128, pl.comp_off, pl.member, pl.local,
-                                                        pl.aoff, pl.anb, pl.aw, L, e0, e1, e2, out,
-            pl.stats_on ? pl.stats : nullptr);
-        LG_LAUNCHED("k_comp_small<128>");
+    const bool side = nbig > 0 && (n128 + n48 + n16) > 0;
+    auto small_closures = [&]() -> int32_t {
+        ktimer_begin(ctx, "k_comp_small", &ev);
+        if (n128 > 0) {
+            k_comp_small<kSmallComp><<<n128, 1024, 0, ctx->stream>>>(
+                pl.sd, pl.sid, pl.cls128, pl.comp_off, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, L, e0, e1, e2, out,
+                pl.stats_on ? pl.stats : nullptr);
+            LG_LAUNCHED("k_comp_small<128>");
+        }
+        if (n48 > 0) {
+            k_comp_small<64><<<n48, 256, 0, ctx->stream>>>(
+                pl.sd, pl.sid, pl.cls48, pl.comp_off, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, L, e0, e1, e2, out,
+                pl.stats_on ? pl.stats : nullptr);
+            LG_LAUNCHED("k_comp_small<64>");
+        }
+        if (n16 > 0) {
+            k_comp_small<32><<<n16, 64, 0, ctx->stream>>>(
+                pl.sd, pl.sid, pl.cls16, pl.comp_off, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, L, e0, e1, e2, out,
+                pl.stats_on ? pl.stats : nullptr);
+            LG_LAUNCHED("k_comp_small<32>");
+        }
+        ktimer_end(ctx, "k_comp_small", ev);
+        return LG_OK;
+    };
+    struct JoinGuard {
+        lg_ctx *ctx;
+        bool on;
+        ~JoinGuard() {
+            if (on) (void)hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
+        }
+    } join{ctx, false};
+    if (side) {
+        LG_HIP(hipEventRecord(ctx->ev_fork, st));
+        LG_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+        std::swap(ctx->stream, ctx->side);
+        const int32_t rc = small_closures();
+        std::swap(ctx->stream, ctx->side);
+        LG_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+        join.on = true;
+        if (rc != LG_OK) return rc;
+    } else {
+        LG_TRY(small_closures());
     }
-    if (n48 > 0) {
-        k_comp_small<64><<<n48, 256, 0, st>>>(pl.sd, pl.sid, pl.cls48, pl.comp_off, pl.member, pl.local, pl.aoff,
-                                               pl.anb, pl.aw, L, e0, e1, e2, out,
-            pl.stats_on ? pl.stats : nullptr);
-        LG_LAUNCHED("k_comp_small<64>");
-    }
-    if (n16 > 0) {
-        k_comp_small<32><<<n16, 64, 0, st>>>(pl.sd, pl.sid, pl.cls16, pl.comp_off, pl.member, pl.local, pl.aoff,
-                                              pl.anb, pl.aw, L, e0, e1, e2, out,
-            pl.stats_on ? pl.stats : nullptr);
-        LG_LAUNCHED("k_comp_small<32>");
-    }
-    ktimer_end(ctx, "k_comp_small", ev);
     if (nbig == 0) {
         res->edges = pl.edges_;
         res->w = pl.w_;
