@@ -222,6 +222,31 @@ int32_t lg_grid_info_get(const lg_grid *grid, lg_grid_info *info);
 int32_t lg_grid_read_dense(lg_ctx *ctx, const lg_grid *grid, float *h_out);
 int32_t lg_grid_destroy(lg_ctx *ctx, lg_grid *grid);
 
+/* ------------------------------------------------ reference-mode sampler ---- */
+/* Active-voxel bounding box at voxel resolution, inclusive (voxels != 0):
+ * grid->evalActiveVoxelBoundingBox() of main.cc:512.  An empty grid gives
+ * vmin = {0,0,0}, vmax = {-1,-1,-1}. */
+int32_t lg_grid_active_bbox(lg_ctx *ctx, const lg_grid *grid, int32_t vmin[3], int32_t vmax[3]);
+
+/* The reference's adaptive dart throwing, LightGraph::scatterPoints
+ * (LightGraph.h:206-274), for `count` iterations at once (one workgroup each,
+ * as the tbb::parallel_for over iterations of main.cc:534-538).  Iteration k
+ * draws from std::mt19937(854123 + h_iters[k]) through
+ * uniform_real_distribution<float>(0,1) (RandGen.h:16-44); candidates fall in
+ * [bbox_min, bbox_max] (world; the reference's mBbox); a candidate is rejected
+ * when the density there is below 0.001 or an accepted point is closer than the
+ * adaptive minimum distance; the squared distance is rescaled every
+ * scatter_count/100 throws by accepted/expected; throwing stops once more than
+ * `target` points are accepted.  h_min_dist[k]: the starting minimum distance
+ * in, sqrt of the squared distance at the last acceptance out (minDistVec[k]).
+ * d_points_out: count x out_cap x 3 floats (world, acceptance order);
+ * h_counts[k]: points accepted (only the first out_cap are written).  The
+ * per-iteration state is reset on each call (the reference carries pcSize over
+ * within one TBB range).  scatter_count >= 100, target <= 2047. */
+int32_t lg_dart_throw(lg_ctx *ctx, const lg_grid *density, const double bbox_min[3], const double bbox_max[3],
+                      int32_t count, const uint32_t *h_iters, uint32_t scatter_count, uint32_t target,
+                      float *h_min_dist, float *d_points_out, int32_t out_cap, int32_t *h_counts);
+
 /* ---------------------------------------------------------- solve stages ---- */
 /* Kernel (a): density-weighted point sampling over the dense bricks.  Replaces
  * the dart thrower LightGraph::scatterPoints (LightGraph.h:206-274) with the

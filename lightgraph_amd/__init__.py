@@ -287,6 +287,46 @@ def sample_points(ctx, grid, seed, solve, n, out_buf):
     _check(lib().lg_sample_points(ctx.handle, grid.handle, seed, solve, n, out_buf.ptr), "sample_points")
 
 
+def active_bbox(ctx, grid):
+    """Inclusive voxel bbox of the voxels != 0 (evalActiveVoxelBoundingBox)."""
+    lo, hi = (C.c_int32 * 3)(), (C.c_int32 * 3)()
+    _check(lib().lg_grid_active_bbox(ctx.handle, grid.handle, lo, hi), "grid_active_bbox")
+    return tuple(lo), tuple(hi)
+
+
+def dart_throw(ctx, grid, iters, scatter_count=1000000, target=750, min_dist=None, bbox=None, out_cap=None):
+    """LightGraph::scatterPoints (the reference's dart thrower) for each
+    iteration index in `iters`.  bbox: (min, max) world corners, by default the
+    active-voxel box of the grid (main.cc:512); min_dist: starting minimum
+    distance per iteration (a float or a sequence), by default
+    MIN_DIST_MULT (0.13) * mean bbox side in float arithmetic (main.cc:514-518).
+    Returns (list of
+    (count, 3) float32 point arrays, array of final minimum distances)."""
+    iters = [int(i) for i in iters]
+    k = len(iters)
+    if bbox is None:
+        lo, hi = active_bbox(ctx, grid)
+        desc = grid.info().desc
+        o, v = list(desc.origin), desc.voxel_size
+        bbox = ([o[a] + v * lo[a] for a in range(3)], [o[a] + v * hi[a] for a in range(3)])
+    bmin, bmax = (C.c_double * 3)(*bbox[0]), (C.c_double * 3)(*bbox[1])
+    if min_dist is None:
+        size = np.array([np.float32(bbox[1][a] - bbox[0][a]) for a in range(3)], np.float32)
+        avg = np.float32(np.float32(size[0] + size[1]) + size[2]) / np.float32(3.0)
+        min_dist = np.float32(np.float32(0.13) * avg)
+    md = np.broadcast_to(np.asarray(min_dist, np.float32), (k,)).astype(np.float32)
+    cap = int(out_cap or target + 1)
+    out = DeviceBuffer(ctx, k * cap * 3 * 4)
+    counts = (C.c_int32 * k)()
+    mdc = (C.c_float * k)(*md.tolist())
+    _check(lib().lg_dart_throw(ctx.handle, grid.handle, bmin, bmax, k, (C.c_uint32 * k)(*iters), scatter_count,
+                               target, mdc, out.ptr, cap, counts), "dart_throw")
+    allp = out.read(np.float32, (k, cap, 3))
+    out.free()
+    pts = [allp[i, :min(counts[i], cap)].copy() for i in range(k)]
+    return pts, np.array(mdc[:], np.float32)
+
+
 def build_edges(ctx, pts_buf, n, radius, max_connect, edges_buf, edge_cap):
     ne = C.c_int32()
     _check(lib().lg_build_edges(ctx.handle, pts_buf.ptr, n, radius, max_connect, edges_buf.ptr, edge_cap,

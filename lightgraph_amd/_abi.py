@@ -172,6 +172,9 @@ SIGNATURES = {
     "lg_grid_read_dense": (I32, [P, P, P]),
     "lg_grid_destroy": (I32, [P, P]),
     "lg_sample_points": (I32, [P, P, U64, U32, I32, P]),
+    "lg_grid_active_bbox": (I32, [P, P, C.POINTER(I32), C.POINTER(I32)]),
+    "lg_dart_throw": (I32, [P, P, DP, DP, I32, C.POINTER(U32), U32, U32, C.POINTER(C.c_float), P, I32,
+                            C.POINTER(I32)]),
     "lg_build_edges": (I32, [P, P, I32, D, I32, P, I32, C.POINTER(I32)]),
     "lg_march_edges": (I32, [P, P, C.POINTER(lg_shader), C.POINTER(lg_march_params), P, P, I32, P, P, I32]),
     "lg_vertex_light": (I32, [P, P, P, C.POINTER(lg_shader), C.POINTER(lg_light), I32,

@@ -910,7 +910,8 @@ int32_t lgo_dart_throw(const lg_grid_desc *d, const float *dense, uint32_t iter,
         const Vec3 pt(np[0], np[1], np[2]);
         if (pcSize > target) break;
         if ((n + 1) % batch == 0) {
-            const uint32_t expected = (uint32_t)(target * std::pow((double)n / (double)scatter_count, 0.17));
+            // PC_GROWTH_FUNC_POWER is a float constant (LightGraph.h:28): pow gets (double)0.17f
+            const uint32_t expected = (uint32_t)(target * std::pow((double)n / (double)scatter_count, (double)0.17f));
             const double ratio = pcSize / (double)expected;
             minDistSquared *= ratio;
         }
