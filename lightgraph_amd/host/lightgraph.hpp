@@ -204,6 +204,11 @@ struct SceneSettings {
     int32_t maxConnect = 12; // GRAPH_MAX_CONNECT (main.cc:62); <= 0: uncapped
     uint64_t seed = 854123;  // RandGen base seed (RandGen.h:28-29)
     uint64_t envSeed = 4564446;
+    // "density": the density-weighted sampler (points per iteration = `points`);
+    // "dart": the reference's dart thrower (LightGraph::scatterPoints), target
+    // `points`, radius 1.7 x the thrower's final minimum distance per iteration
+    std::string sampler = "density";
+    uint32_t scatterCount = 1000000;  // PC_SCATTER_COUNT (main.cc:61): throws per iteration
 
     lg_march_params march() const {
         lg_march_params p{};
@@ -238,8 +243,64 @@ class LightGraph {
         return 1.7 * 0.13 * side * std::cbrt(750.0 / std::max<uint32_t>(opts_.points, 1));
     }
 
+    // One iteration's dart-thrown points (world, acceptance order) and the
+    // thrower's final minimum distance (mMinDistVec[i] after scatterPoints).
+    struct DartSet {
+        std::vector<float> pts;
+        float min_dist = 0.0f;
+        size_t size() const { return pts.size() / 3; }
+        // searchRadiusConnect = mMinDistVec[i] * CONNECT_SEARCH_RADIUS_MULT (LightGraph.h:169)
+        double radius() const { return (double)(min_dist * 1.7f); }
+    };
+
+    // LightGraph::scatterPoints for iterations [begin, end): the darts fall in
+    // the active-voxel box (main.cc:512) and start from MIN_DIST_MULT x its mean
+    // side (main.cc:514-518), in the reference's float arithmetic.
+    std::vector<DartSet> dartThrow(size_t begin, size_t end) const {
+        Context &ctx = grid_.context();
+        int32_t lo[3], hi[3];
+        check(lg_grid_active_bbox(ctx.get(), grid_.get(), lo, hi), "lg_grid_active_bbox");
+        const lg_grid_info gi = grid_.info();
+        double bmin[3], bmax[3];
+        float side[3];
+        for (int c = 0; c < 3; ++c) {
+            bmin[c] = gi.desc.origin[c] + gi.desc.voxel_size * lo[c];
+            bmax[c] = gi.desc.origin[c] + gi.desc.voxel_size * hi[c];
+            side[c] = (float)(bmax[c] - bmin[c]);
+        }
+        const float avgLength = (side[0] + side[1] + side[2]) / 3.0f;
+        const float minDist = 0.13f * avgLength;
+        const size_t count = end - begin;
+        const int32_t cap = (int32_t)opts_.points + 1;
+        std::vector<uint32_t> iters(count);
+        std::vector<float> md(count, minDist);
+        std::vector<int32_t> counts(count, 0);
+        for (size_t k = 0; k < count; ++k) iters[k] = (uint32_t)(begin + k);
+        void *d = nullptr;
+        check(lg_malloc(ctx.get(), count * (size_t)cap * 3 * sizeof(float), &d), "lg_malloc");
+        std::unique_ptr<void, std::function<void(void *)>> dguard(d, [&ctx](void *p) { lg_free(ctx.get(), p); });
+        check(lg_dart_throw(ctx.get(), grid_.get(), bmin, bmax, (int32_t)count, iters.data(), opts_.scatterCount,
+                            opts_.points, md.data(), (float *)d, cap, counts.data()),
+              "lg_dart_throw");
+        std::vector<float> all(count * (size_t)cap * 3);
+        check(lg_memcpy_d2h(ctx.get(), all.data(), d, all.size() * sizeof(float)), "d2h dart points");
+        std::vector<DartSet> out(count);
+        for (size_t k = 0; k < count; ++k) {
+            const size_t n = (size_t)std::min(counts[k], cap);
+            const float *p = all.data() + k * (size_t)cap * 3;
+            out[k].pts.assign(p, p + 3 * n);
+            out[k].min_dist = md[k];
+            if (opts_.verbose) std::cout << "PointCloud Size: " << counts[k] << std::endl;
+        }
+        return out;
+    }
+
     void operator()(size_t begin, size_t end) const {
         if (end <= begin) return;
+        if (opts_.sampler == "dart") {
+            solveDart(begin, end);
+            return;
+        }
         Context &ctx = grid_.context();
         const size_t count = end - begin;
         const size_t total = count * opts_.points;
@@ -286,10 +347,12 @@ class LightGraph {
     // again alone with the dense optical-depth output (the batched solve gives
     // the same numbers); transmittance T_c = exp(extinction_c D) per channel,
     // normalised by max(1, column sum) as diffuseLight does (LightGraph.h:563-571).
-    void dumpGeo(size_t iter) const {
+    // With `dart`, the iteration's dart-thrown points and radius are used.
+    void dumpGeo(size_t iter, const DartSet *dart = nullptr) const {
         const auto t0 = std::chrono::steady_clock::now();
         Context &ctx = grid_.context();
-        const uint32_t n = opts_.points;
+        const uint32_t n = dart ? (uint32_t)dart->size() : opts_.points;
+        if (n == 0) return;
         lg_solver *solver = nullptr;
         check(lg_solver_create(ctx.get(), (int32_t)n, 0, LG_SOLVER_DENSE_D, &solver), "lg_solver_create");
         std::unique_ptr<lg_solver, std::function<void(lg_solver *)>> guard(
@@ -297,11 +360,18 @@ class LightGraph {
         lg_solve_params sp{};
         sp.n = (int32_t)n;
         sp.max_connect = opts_.maxConnect;
-        sp.radius = radius();
+        sp.radius = dart ? dart->radius() : radius();
         sp.seed = opts_.seed;
         sp.env_seed = opts_.envSeed;
         sp.solve = (uint32_t)iter;
         sp.flags = LG_SOLVE_DENSE;
+        if (dart) {
+            lg_solver_buffers pb{};
+            check(lg_solver_get(solver, &pb), "lg_solver_get");
+            check(lg_memcpy_h2d(ctx.get(), pb.d_points, dart->pts.data(), dart->pts.size() * sizeof(float)),
+                  "h2d points");
+            sp.external_points = 1;
+        }
         const lg_march_params mp = opts_.march();
         check(lg_solve(ctx.get(), solver, grid_.get(), temp_ ? temp_->get() : nullptr, &shader_.params(),
                        opts_.light.data(), (int32_t)opts_.light.size(), &mp, &sp),
@@ -357,6 +427,47 @@ class LightGraph {
     }
 
    private:
+    // The reference's per-iteration loop (LightGraph.h:161-194) on dart-thrown
+    // points: each iteration is solved alone with its own point count and radius.
+    void solveDart(size_t begin, size_t end) const {
+        Context &ctx = grid_.context();
+        const std::vector<DartSet> sets = dartThrow(begin, end);
+        lg_solver *solver = nullptr;
+        check(lg_solver_create(ctx.get(), (int32_t)opts_.points + 1, 0, 0, &solver), "lg_solver_create");
+        std::unique_ptr<lg_solver, std::function<void(lg_solver *)>> guard(
+            solver, [&ctx](lg_solver *s) { lg_solver_destroy(ctx.get(), s); });
+        lg_solver_buffers b{};
+        check(lg_solver_get(solver, &b), "lg_solver_get");
+        const lg_march_params mp = opts_.march();
+        for (size_t k = 0; k < sets.size(); ++k) {
+            const DartSet &set = sets[k];
+            const size_t n = set.size();
+            std::vector<Vec3d> &P = points_[begin + k], &D = diffuse_[begin + k];
+            P.resize(n);
+            D.resize(n);
+            if (n == 0) continue;
+            check(lg_memcpy_h2d(ctx.get(), b.d_points, set.pts.data(), set.pts.size() * sizeof(float)), "h2d points");
+            lg_solve_params sp{};
+            sp.n = (int32_t)n;
+            sp.max_connect = opts_.maxConnect;
+            sp.radius = set.radius();
+            sp.seed = opts_.seed;
+            sp.env_seed = opts_.envSeed;
+            sp.solve = (uint32_t)(begin + k);
+            sp.external_points = 1;
+            check(lg_solve(ctx.get(), solver, grid_.get(), temp_ ? temp_->get() : nullptr, &shader_.params(),
+                           opts_.light.data(), (int32_t)opts_.light.size(), &mp, &sp),
+                  "lg_solve");
+            std::vector<float> dif(3 * n);
+            check(lg_memcpy_d2h(ctx.get(), dif.data(), b.d_diffuse, dif.size() * 4), "d2h diffuse");
+            for (size_t v = 0; v < n; ++v) {
+                P[v] = Vec3d(set.pts[3 * v], set.pts[3 * v + 1], set.pts[3 * v + 2]);
+                D[v] = Vec3d(dif[3 * v], dif[3 * v + 1], dif[3 * v + 2]);
+            }
+            if (opts_.geoDump) dumpGeo(begin + k, &set);
+        }
+    }
+
     const DenseBrickGrid &grid_;
     const DenseBrickGrid *temp_;
     const SceneSettings &opts_;

@@ -62,6 +62,8 @@ void printHelp() {
         {"-points", "Graph points per LightGraph iteration", "int"},
         {"-radius", "Graph edge radius in world units (0: from the volume bbox)", "float"},
         {"-maxConnect", "Maximum edges per graph point (<= 0: uncapped)", "int"},
+        {"-sampler", "Graph points: density (density-weighted) or dart (the reference's dart thrower)", "string"},
+        {"-scatterCount", "Dart throws per iteration with -sampler dart", "int"},
         {"-precision", "Density brick storage: fp32 | fp16 | bf16", "string"},
         {"-compression", "EXR compression: zip | zips | none", "string"},
     };
@@ -142,6 +144,8 @@ int main(int argc, char *argv[]) {
             else if (arg == "-points") opts.points = (uint32_t)std::atoi(next().c_str());
             else if (arg == "-radius") opts.radius = std::atof(next().c_str());
             else if (arg == "-maxConnect") opts.maxConnect = std::atoi(next().c_str());
+            else if (arg == "-sampler") opts.sampler = next();
+            else if (arg == "-scatterCount") opts.scatterCount = (uint32_t)std::atoi(next().c_str());
             else if (arg == "-precision") precision = next();
             else if (arg == "-compression") opts.compression = next();
             else {
@@ -157,6 +161,10 @@ int main(int argc, char *argv[]) {
             return 1;
         }
         if (imgFilename.empty()) imgFilename = scene + ".exr";
+        if (opts.sampler != "density" && opts.sampler != "dart") {
+            cout << "-sampler: " << opts.sampler << " is not density or dart" << endl;
+            return 1;
+        }
 
         lg_scene_spec spec{};
         spec.n = sceneSize;
@@ -233,7 +241,10 @@ int main(int argc, char *argv[]) {
         cout << "LightGraphs Calculation: " << secondsSince(start1) << " sec" << endl;
         if (opts.verbose) {
             cout << "Point Cloud size: " << pointCloudCombined.size() << endl;
-            cout << "Edge radius: " << lg.radius() << endl;
+            if (opts.sampler == "dart")
+                cout << "Edge radius: 1.7 x each iteration's dart-throwing minimum distance" << endl;
+            else
+                cout << "Edge radius: " << lg.radius() << endl;
         }
 
         // RASTERIZE LIGHTGRAPH TO GRID (main.cc:560-589)

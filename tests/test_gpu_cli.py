@@ -134,3 +134,46 @@ def test_cli_geo_dump(lga, ctx, tmp_path):
         np.testing.assert_allclose(got, want, rtol=2e-5, atol=1e-9)
     s.destroy()
     grid.destroy()
+
+
+def test_cli_dart_sampler(lga, ctx, tmp_path):
+    """-sampler dart: each iteration's points come from the reference's dart
+    thrower (lg_dart_throw; bit-exact against the oracle in test_gpu_dart.py)
+    and its graph radius is 1.7 x the thrower's final minimum distance
+    (LightGraph.h:169-170); the dumped points, edges, light and radiance equal
+    the API's solve on those points."""
+    iters = 2
+    args = ["-scene", "sphere", "-sceneSize", "48", "-sceneSeed", "5", "-sampler", "dart", "-scatterCount", "100000",
+            "-iter", str(iters), "-resWidth", "16", "-resHeight", "12", "-camPos", "23.5,23.5,120",
+            "-light", "dir:0:1,1,1:1:-45,30,0", "-scattering", "1,0.9,0.8", "-absorption", "0.1",
+            "-exrOut", str(tmp_path / "o.exr"), "-geoDump"]
+    r = subprocess.run([os.path.abspath(CLI), *args], capture_output=True, text=True, timeout=180, cwd=tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    d = tmp_path / "geoDump"
+
+    def load(kind, i, dtype=float):
+        return np.loadtxt(d / f"sphere_{kind}_{i:04d}.txt", delimiter=",", dtype=dtype, ndmin=2)
+
+    spec = lga.scene_spec(lga.LG_SCENE_SPHERE_FBM, 48, seed=5, octaves=4, radius=0.375 * 48)
+    grid, _ = lga.Grid.generate(ctx, spec)
+    pts, md = lga.dart_throw(ctx, grid, range(iters), scatter_count=100000, target=750)
+    sh = lga.default_shader(absorption=0.1, scattering=(1.0, 0.9, 0.8))
+    p = lga.march_params()
+    lights = [lga.parse_light("dir:0:1,1,1:1:-45,30,0")]
+    for i in range(iters):
+        n = len(pts[i])
+        assert 100 < n <= 751
+        np.testing.assert_allclose(load("pc", i), pts[i], rtol=1e-5, atol=1e-30)
+        s = lga.Solver(ctx, n)
+        s.write_points(pts[i])
+        radius = float(np.float32(md[i]) * np.float32(1.7))
+        s.solve(grid, sh, lights, p, n=n, radius=radius, solve=i, external_points=True)
+        res = s.results(want_D=False)
+        np.testing.assert_array_equal(load("edge", i, int), res["edges"])
+        np.testing.assert_allclose(load("light", i), res["L"], rtol=1e-5, atol=1e-30)
+        np.testing.assert_allclose(load("diffuse", i), res["diffuse"], rtol=1e-5, atol=1e-30)
+        s.destroy()
+    grid.destroy()
+    bad = subprocess.run([os.path.abspath(CLI), *args[:6], "-sampler", "poisson"], capture_output=True, text=True,
+                         timeout=60, cwd=tmp_path)
+    assert bad.returncode != 0 and "not density or dart" in bad.stdout
