@@ -441,6 +441,43 @@ def kernel_roofline(times, work, workload):
     return roof
 
 
+def bench_dart(lga, ctx, grid, iters, scatter, cpu):
+    """The reference's sampler (dart throwing, LightGraph::scatterPoints) on the
+    C2 volume: `iters` iterations of `scatter` throws each in one lg_dart_throw
+    call, timed with the context's markers; with `cpu`, one iteration of the
+    oracle's sequential restatement on the host for comparison."""
+    import numpy as np
+    lga.dart_throw(ctx, grid, [0], scatter_count=scatter, target=750)  # warm-up
+    ctx.sync()
+    ctx.mark(6)
+    pts, md = lga.dart_throw(ctx, grid, range(iters), scatter_count=scatter, target=750)
+    ctx.mark(7)
+    ctx.sync()
+    ms = ctx.elapsed_ms(6, 7)
+    out = {"workload": "c2_dart_throw", "iterations": iters, "throws_per_iteration": scatter, "target": 750,
+           "ms": round(ms, 3), "iterations_per_s": round(iters * 1000.0 / ms, 2),
+           "points_per_iteration": [int(np.min([len(p) for p in pts])), int(np.max([len(p) for p in pts]))]}
+    if cpu:
+        from oracle import lgoracle
+        dense = grid.read_dense()
+        t0 = time.perf_counter()
+        want, _ = lgoracle.dart_throw(dense, 0, scatter_count=scatter, target=750,
+                                      min_dist=float(_dart_start(lga, ctx, grid)))
+        dt = time.perf_counter() - t0
+        out["cpu_oracle_s_per_iteration"] = round(dt, 3)
+        out["cpu_equal_to_gpu_iteration_0"] = bool(np.array_equal(want, pts[0]))
+    return out
+
+
+def _dart_start(lga, ctx, grid):
+    """MIN_DIST_MULT x mean side of the active-voxel box (main.cc:512-518)."""
+    import numpy as np
+    lo, hi = lga.active_bbox(ctx, grid)
+    desc = grid.info().desc
+    size = [np.float32(desc.voxel_size * (hi[a] - lo[a])) for a in range(3)]
+    return np.float32(0.13) * (np.float32(np.float32(size[0] + size[1]) + size[2]) / np.float32(3.0))
+
+
 def cpu_baseline(lga, ctx, grid, shader, params, lights, n_points, radius, budget_s):
     """The oracle's serial solve (max-product Floyd-Warshall on Vec3 transmittance,
     as the reference) on the same volume and points, one solve per host thread,
@@ -491,6 +528,8 @@ def main():
     ap.add_argument("--serve-workers", type=int, default=3)
     ap.add_argument("--serve-batch", type=int, default=8)
     ap.add_argument("--no-fw-dense", action="store_true", help="skip the dense n x n Floyd-Warshall timing")
+    ap.add_argument("--dart-iters", type=int, default=64,
+                    help="iterations of the reference's dart thrower (1e6 throws each) to time; 0: skip")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device three times with k_copy "
                          "(the known-size kernel that traffic counters are scaled against)")
@@ -558,6 +597,8 @@ def main():
     if args.render_reps > 0:
         line["render"] = bench_render(lga, ctx, grid, shader, lights, solver, n, args.render_reps)
     solver.destroy()
+    if args.dart_iters > 0:
+        line["dart_throw"] = bench_dart(lga, ctx, grid, args.dart_iters, 1000000, not args.no_cpu_baseline)
     if args.batch_volumes > 0:
         line["batched"] = bench_batched(lga, ctx, shader, params, lights, args.batch_volumes, args.batch_reps)
     t = lga.fw_tuning(ctx)
