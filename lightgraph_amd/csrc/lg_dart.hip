@@ -28,6 +28,8 @@ namespace {
 
 constexpr int kDartThreads = 256;
 constexpr int kDartMaxPoints = 2048;  // accepted points kept in LDS (the reference asserts < 1000)
+constexpr int kCellG = 16;            // collision cells per axis
+constexpr int kCells = kCellG * kCellG * kCellG;
 constexpr int kMtN = 624;
 constexpr int kChunk = kMtN / 3;      // candidates per twist
 constexpr uint32_t kUpper = 0x80000000u, kLower = 0x7fffffffu, kMatrixA = 0x9908b0dfu;
@@ -35,6 +37,7 @@ constexpr uint32_t kUpper = 0x80000000u, kLower = 0x7fffffffu, kMatrixA = 0x9908
 struct DartConsts {
     double lo[3];     // mBbox.min() (world)
     float sz[3];      // Vec3s(mBbox.max() - mBbox.min())
+    double cell_min;  // smallest collision cell: the longest box side / kCellG
     uint32_t scatter; // mScatterCount
     uint32_t target;  // TARGET_PC_SIZE
     uint32_t batch;   // mScatterCount / PC_GROWTH_NUMBER_OF_ESTIMATE
@@ -80,6 +83,37 @@ __device__ __forceinline__ float mt_uniform(uint32_t y) {
     return r >= 1.0f ? 0x1.fffffep-1f : r;
 }
 
+// Collision cells of side h >= sqrt(msq) (1 + 2^-10): a point closer than the
+// minimum distance differs from the candidate by less than h on every axis, so
+// their floor((x - lo) / h) indices differ by at most one (the margin covers the
+// rounding of the two quotients) and the 27 cells around the candidate hold it.
+__device__ __forceinline__ double cell_side(double msq, double cell_min) {
+    return fmax(sqrt(msq) * (1.0 + 0x1p-10), cell_min);  // fmax drops a NaN distance
+}
+
+__device__ __forceinline__ int cell_axis(float x, double lo, double h) {
+    const double q = floor(((double)x - lo) / h);
+    return q < 0.0 ? 0 : (q > (double)(kCellG - 1) ? kCellG - 1 : (int)q);
+}
+
+__device__ __forceinline__ int cell_of(const float *p, const DartConsts &c, double h) {
+    return (cell_axis(p[2], c.lo[2], h) * kCellG + cell_axis(p[1], c.lo[1], h)) * kCellG + cell_axis(p[0], c.lo[0], h);
+}
+
+__device__ __forceinline__ bool near_any(const float *p, const float (*acc)[3], const int *head, const int *next,
+                                         const DartConsts &c, double h, double msq) {
+    const int cx = cell_axis(p[0], c.lo[0], h), cy = cell_axis(p[1], c.lo[1], h), cz = cell_axis(p[2], c.lo[2], h);
+    const double x = p[0], y = p[1], z = p[2];
+    for (int z0 = max(cz - 1, 0); z0 <= min(cz + 1, kCellG - 1); ++z0)
+        for (int y0 = max(cy - 1, 0); y0 <= min(cy + 1, kCellG - 1); ++y0)
+            for (int x0 = max(cx - 1, 0); x0 <= min(cx + 1, kCellG - 1); ++x0)
+                for (int i = head[(z0 * kCellG + y0) * kCellG + x0]; i >= 0; i = next[i]) {
+                    const double dx = x - (double)acc[i][0], dy = y - (double)acc[i][1], dz = z - (double)acc[i][2];
+                    if (dx * dx + dy * dy + dz * dz < msq) return true;
+                }
+    return false;
+}
+
 __device__ __forceinline__ bool too_close(const float *p, const float (*acc)[3], int i0, int i1, double msq) {
     const double x = p[0], y = p[1], z = p[2];
     for (int i = i0; i < i1; ++i) {
@@ -99,8 +133,11 @@ __global__ void __launch_bounds__(kDartThreads) k_dart_throw(lgd::GridView g, Da
     __shared__ float cand[kChunk][3];
     __shared__ float acc[kDartMaxPoints][3];
     __shared__ uint8_t ok[kChunk];  // passes the density test and misses every point accepted before its run
+    __shared__ int head[kCells];    // per collision cell: last accepted point in it (-1: none)
+    __shared__ int next[kDartMaxPoints];
     __shared__ int s_pc, s_stop;
     __shared__ float s_msq, s_last;
+    __shared__ double s_h;          // side of the cells the accepted points are filed under
     const int s = (int)blockIdx.x, tid = (int)threadIdx.x, lane = tid & 63;
     if (tid == 0) {
         uint32_t x = 854123u + iters[s];  // RandGen.h:28-29
@@ -114,6 +151,7 @@ __global__ void __launch_bounds__(kDartThreads) k_dart_throw(lgd::GridView g, Da
         s_last = s_msq;
         s_pc = 0;
         s_stop = 0;
+        s_h = -1.0;
     }
     __syncthreads();
     for (uint32_t n0 = 0; n0 < c.scatter && !s_stop; n0 += kChunk) {
@@ -139,12 +177,22 @@ __global__ void __launch_bounds__(kDartThreads) k_dart_throw(lgd::GridView g, Da
             __syncthreads();
             const int p0 = s_pc;
             const double msq = (double)s_msq;
+            const double h = cell_side(msq, c.cell_min);
+            if (h != s_h) {
+                // the distance changed at a rescale point: file the accepted points again
+                __syncthreads();
+                for (int i = tid; i < kCells; i += kDartThreads) head[i] = -1;
+                __syncthreads();
+                for (int i = tid; i < p0; i += kDartThreads) next[i] = atomicExch(&head[cell_of(acc[i], c, h)], i);
+                if (tid == 0) s_h = h;
+                __syncthreads();
+            }
             if (tid >= a && tid < b) {
                 const float *p = cand[tid];
                 const float rho = lgd::trilinear<T>(g, ((double)p[0] - g.origin[0]) * g.inv_voxel,
                                                     ((double)p[1] - g.origin[1]) * g.inv_voxel,
                                                     ((double)p[2] - g.origin[2]) * g.inv_voxel);
-                ok[tid] = !(rho < 0.001f) && !too_close(p, acc, 0, p0, msq);  // PC_DENSITY_CUTOFF
+                ok[tid] = !(rho < 0.001f) && !near_any(p, acc, head, next, c, h, msq);  // PC_DENSITY_CUTOFF
             }
             __syncthreads();
             if (tid < 64) {
@@ -168,6 +216,11 @@ __global__ void __launch_bounds__(kDartThreads) k_dart_throw(lgd::GridView g, Da
                             break;
                         }
                         if (lane < 3) acc[pc][lane] = cand[kk][lane];
+                        if (lane == 0) {
+                            const int cell = cell_of(cand[kk], c, h);
+                            next[pc] = head[cell];
+                            head[cell] = pc;
+                        }
                         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                         __builtin_amdgcn_wave_barrier();
                         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -294,6 +347,8 @@ int32_t lg_dart_throw(lg_ctx *ctx, const lg_grid *density, const double bbox_min
         c.lo[a] = bbox_min[a];
         c.sz[a] = (float)(bbox_max[a] - bbox_min[a]);
     }
+    c.cell_min = std::max(std::max((double)c.sz[0], (double)c.sz[1]), (double)c.sz[2]) / kCellG;
+    if (!(c.cell_min > 0.0)) c.cell_min = 1.0;  // a flat or empty box: one cell covers it
     c.scatter = scatter_count;
     c.target = target;
     c.batch = scatter_count / 100u;  // PC_GROWTH_NUMBER_OF_ESTIMATE
