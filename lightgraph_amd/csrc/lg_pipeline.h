@@ -373,12 +373,17 @@ struct Pipeline {
     uint64_t *keys = nullptr, *keys_sorted = nullptr;
     int *flag = nullptr, *cid = nullptr, *member = nullptr, *local = nullptr, *comp_off = nullptr;
     int *level = nullptr;   // breadth-first level of each vertex inside its component (bfs_order)
+    int *coloff = nullptr, *colfill = nullptr;  // risky-column slot lists of the parallel greedy
+    int *greedy_err = nullptr;                  // 1: a greedy wait exceeded its bound
     int *big_list = nullptr, *counters = nullptr;
     int *cls16 = nullptr, *cls48 = nullptr, *cls128 = nullptr;  // components of 2-32, 33-64, 65-128 vertices
     void *tmp_vertex = nullptr;
     size_t tmp_vertex_bytes = 0;
     // candidate / edge sized
     int *cand = nullptr, *accepted = nullptr, *epos = nullptr, *edges_ = nullptr, *anb = nullptr;
+    int *colslot = nullptr;     // risky-column slot lists of the parallel greedy
+    int2 *slot_info = nullptr;  // per candidate slot: (j, rank in j's column or -1)
+    size_t colslot_cap = 0, slot_info_cap = 0;
     float *w_ = nullptr, *aw = nullptr;
     uint8_t *tmp_cand = nullptr;
     size_t cand_cap = 0, accepted_cap = 0, epos_cap = 0, tmp_cand_cap = 0, edges_cap = 0, w_cap = 0, anb_cap = 0,

@@ -587,6 +587,156 @@ __global__ void __launch_bounds__(64) k_greedy_staged(const SolveDesc *__restric
     }
 }
 
+// ------------------------------------------- greedy cap, rows in parallel ----
+// The greedy sweep's decisions for row i read and change only the degree
+// counters of its capped ("risky") vertices: i itself and its risky candidates
+// j.  Row i therefore only has to follow the earlier rows that touch one of
+// those vertices; rows touching disjoint vertex sets may be decided at once.
+// For a candidate slot k = (a, j) with j risky, rank[k] = the number of rows
+// a' < a that also hold j (slots are row-major, so the slot order of a column
+// is its row order); the last toucher of a risky vertex i is row i itself,
+// after colcnt[i] earlier rows.
+__global__ void k_col_fill(int K, const int *__restrict__ cand, const uint8_t *__restrict__ risky,
+                           const int *__restrict__ coloff, int *__restrict__ fill, int *__restrict__ colslot) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int j = cand[k];
+    if (risky[j]) colslot[coloff[j] + atomicAdd(&fill[j], 1)] = k;
+}
+
+// slot[k] = (j, rank[k] if j is risky else -1): what the greedy needs of a
+// candidate, in one 8-byte load.  One thread per slot.
+__global__ void k_col_rank(int K, const int *__restrict__ cand, const uint8_t *__restrict__ risky,
+                           const int *__restrict__ coloff, const int *__restrict__ colcnt,
+                           const int *__restrict__ colslot, int2 *__restrict__ slot) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const int j = cand[k];
+    int r = -1;
+    if (risky[j]) {
+        const int c0 = coloff[j], c1 = c0 + colcnt[j];
+        r = 0;
+        for (int c = c0; c < c1; ++c) r += colslot[c] < k ? 1 : 0;
+    }
+    slot[k] = make_int2(j, r);
+}
+
+// One workgroup of kGreedyWaves waves per solve; wave w decides rows w, w + W,
+// ... of the solve's contested rows in order.  Each vertex has one LDS word:
+// rows that have finished with it (<< 8) | its accepted degree.  A row waits
+// (per 64-candidate chunk) until each risky candidate's word shows rank[k]
+// finished rows, and, for a risky row vertex, colcnt[i]; it then decides
+// exactly as k_greedy does and publishes each risky candidate's word (finished
+// + 1, new degree) with a release store, accepted or not.  The earliest
+// unfinished row never waits on an unfinished row, so the sweep always advances;
+// a wait longer than kGreedySpin polls sets *err (the host then fails the
+// solve) instead of spinning on.
+constexpr int kGreedyWaves = 16;
+constexpr int kGreedySpin = 1 << 22;
+
+__device__ __forceinline__ uint32_t lds_acquire(uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_release(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Poll until every lane's condition holds.  Bounded: the first wave of the
+// workgroup that polls kGreedySpin times sets *abort, and every wave then stops
+// waiting at once (false: the decisions are void and the host fails the solve).
+template <typename F>
+__device__ __forceinline__ bool wave_wait(F &&ready, uint32_t *abort) {
+    for (int it = 0; it < kGreedySpin; ++it) {
+        if (__ballot(!ready()) == 0ull) return true;
+        if (lds_acquire(abort) != 0u) return false;
+        __builtin_amdgcn_s_sleep(1);
+    }
+    lds_release(abort, 1u);
+    return false;
+}
+
+// A contested row as a wave holds it: the header in every lane, the first 64
+// candidate slots one per lane (longer rows load the rest when they get there).
+struct GreedyRow {
+    int i, k0, k1, self_want;
+    bool ri;
+    int2 sl;
+};
+
+__device__ __forceinline__ GreedyRow greedy_row(int r, int hi, int lane, const int *__restrict__ visit_list,
+                                                const int *__restrict__ off, const uint8_t *__restrict__ risky,
+                                                const int *__restrict__ colcnt, const int2 *__restrict__ slot) {
+    GreedyRow g{};
+    g.k0 = g.k1 = 0;
+    if (r >= hi) return g;
+    g.i = visit_list[r];
+    g.k0 = off[g.i];
+    g.k1 = off[g.i + 1];
+    g.ri = risky[g.i] != 0;
+    g.self_want = colcnt[g.i];
+    g.sl = g.k0 + lane < g.k1 ? slot[g.k0 + lane] : make_int2(0, -1);
+    return g;
+}
+
+__global__ void __launch_bounds__(64 * kGreedyWaves) k_greedy_waves(
+    const SolveDesc *__restrict__ sd, int cap, const int *__restrict__ off, const int2 *__restrict__ slot,
+    const uint8_t *__restrict__ risky, const int *__restrict__ colcnt, const int *__restrict__ visit_list,
+    const int *__restrict__ n_visit, int *__restrict__ accepted, int *__restrict__ err) {
+    extern __shared__ uint32_t word[];
+    __shared__ uint32_t abort;
+    const int s = blockIdx.x;
+    const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    const int n = sd[s].n, v0 = sd[s].voff;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) word[k] = 0u;
+    if (threadIdx.x == 0) abort = 0u;
+    __syncthreads();
+    const int nv = *n_visit;
+    const int lo = lower_bound_i32(visit_list, nv, v0), hi = lower_bound_i32(visit_list, nv, v0 + n);
+    bool ok = true;
+    GreedyRow nxt = greedy_row(lo + w, hi, lane, visit_list, off, risky, colcnt, slot);
+    for (int r = lo + w; r < hi; r += kGreedyWaves) {
+        const GreedyRow row = nxt;
+        // the wave's next row loads while this one waits and decides
+        nxt = greedy_row(r + kGreedyWaves, hi, lane, visit_list, off, risky, colcnt, slot);
+        const int il = row.i - v0;
+        int ci = 0;
+        if (row.ri) {
+            const uint32_t want = (uint32_t)row.self_want;
+            ok = ok && wave_wait([&] { return lane != 0 || (lds_acquire(&word[il]) >> 8) == want; }, &abort);
+            ci = (int)(lds_acquire(&word[il]) & 0xffu);
+        }
+        for (int base = row.k0; base < row.k1; base += 64) {
+            const int k = base + lane;
+            const bool valid = k < row.k1;
+            const int2 sl = base == row.k0 ? row.sl : (valid ? slot[k] : make_int2(0, -1));
+            const bool rj = valid && sl.y >= 0;
+            const int jl = sl.x - v0;
+            const uint32_t want = rj ? (uint32_t)sl.y : 0u;
+            ok = ok && wave_wait([&] { return !rj || (lds_acquire(&word[jl]) >> 8) == want; }, &abort);
+            const uint32_t wj = rj ? lds_acquire(&word[jl]) : 0u;
+            const int cj = (int)(wj & 0xffu);
+            const bool contested = valid && (row.ri || rj);
+            const bool open = !(row.ri && ci >= cap);
+            const bool elig = open && contested && (!rj || cj < cap);
+            bool sel;
+            int taken = 0;
+            if (row.ri) {
+                const uint64_t m = __ballot(elig);
+                const int need = cap - ci;
+                sel = elig && __popcll(m & ((1ull << lane) - 1ull)) < need;
+                taken = min((int)__popcll(m), need);
+            } else {
+                sel = elig;
+            }
+            if (sel) accepted[k] = 1;
+            if (rj) lds_release(&word[jl], ((want + 1u) << 8) | (uint32_t)(sel ? cj + 1 : cj));
+            if (open) ci += taken;
+        }
+        if (row.ri && lane == 0) lds_release(&word[il], (lds_acquire(&word[il]) & ~0xffu) | (uint32_t)ci);
+    }
+    if (!ok && lane == 0) atomicOr(err, 1);
+}
+
 // Edge list (i, j) in row-major order from the accepted candidates.
 __global__ void k_edges(int V, const int *__restrict__ off, const int *__restrict__ cand,
                         const int *__restrict__ accepted, const int *__restrict__ epos, int *__restrict__ edges) {
@@ -718,11 +868,13 @@ __global__ void k_comp_local(int V, const SolveDesc *__restrict__ sd, const int 
                              const int *__restrict__ comp_off, const int *__restrict__ member, int *__restrict__ local,
                              int *__restrict__ big_list, int *__restrict__ counters, int *__restrict__ cls16,
                              int *__restrict__ cls48, int *__restrict__ cls128, const float *__restrict__ L,
-                             float *__restrict__ out, const int *__restrict__ n_comp, const int *__restrict__ n_edges) {
+                             float *__restrict__ out, const int *__restrict__ n_comp, const int *__restrict__ n_edges,
+                             const int *__restrict__ greedy_err) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) {  // for the host's one readback
         counters[8] = *n_comp;
         counters[9] = *n_edges;
+        counters[10] = *greedy_err;
     }
     if (k >= V) return;
     const int c = cid_excl[k] + flag[k] - 1;
@@ -793,6 +945,8 @@ __global__ void __launch_bounds__(1024) k_bfs_levels(const int *__restrict__ cou
             const int p1 = aoff[v + 1];
             for (int p = aoff[v]; p < p1; ++p) {
                 const int u = anb[p];
+                // most neighbours were reached earlier: read before claiming
+                if (__hip_atomic_load(&level[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != kUnseen) continue;
                 if (atomicCAS(&level[u], kUnseen, L + 1) == kUnseen) {
                     const int slot = atomicAdd(&n_next, 1);
                     if (slot < m) nxt[slot] = u;
@@ -1224,6 +1378,9 @@ int32_t Pipeline::reserve_vertices(lg_ctx *ctx, int V, int S) {
     A(&member, (size_t)v_cap);
     A(&local, (size_t)v_cap);
     A(&level, (size_t)v_cap);
+    A(&coloff, (size_t)v_cap + 1);
+    A(&colfill, (size_t)v_cap);
+    A(&greedy_err, 1);
     A(&comp_off, (size_t)v_cap + 1);
     // counters[0..15] then the (component, size) pairs of the large components:
     // at most v_cap / (kSmallComp + 1) of them
@@ -1256,7 +1413,8 @@ int32_t Pipeline::reserve_vertices(lg_ctx *ctx, int V, int S) {
 void Pipeline::release_vertex(lg_ctx *ctx) {
     void *ps[] = {sd,     sid,  rowcnt,      off,    colcnt,       risky, visit,     visit_pos, visit_list,
                   adeg,   aoff, acur,        parent, root,         keys,  keys_sorted, flag,    cid,
-                  member, local, level, comp_off, counters, tmp_vertex, cd,  ckeys, ckeys_sorted,
+                  member, local, level, coloff, colfill, greedy_err, comp_off, counters, tmp_vertex, cd,
+                  ckeys, ckeys_sorted,
                   cvals,  cvals_sorted, cls16, cls48, cls128};
     for (void *p : ps) dfree(ctx, p);
     cls16 = cls48 = cls128 = nullptr;
@@ -1266,6 +1424,7 @@ void Pipeline::release_vertex(lg_ctx *ctx) {
     cvals = cvals_sorted = nullptr;
     sid = rowcnt = off = colcnt = visit = visit_pos = visit_list = adeg = aoff = acur = parent = root = nullptr;
     flag = cid = member = local = level = comp_off = big_list = counters = nullptr;
+    coloff = colfill = greedy_err = nullptr;
     risky = nullptr;
     keys = keys_sorted = nullptr;
     tmp_vertex = nullptr;
@@ -1274,13 +1433,15 @@ void Pipeline::release_vertex(lg_ctx *ctx) {
 
 void Pipeline::release(lg_ctx *ctx) {
     release_vertex(ctx);
-    void *ps[] = {cand, accepted, epos, tmp_cand, edges_, w_, anb, aw, big_T, big_ln, big_meta, ray, cstart, cend,
-                  stats};
+    void *ps[] = {cand,  accepted, epos, tmp_cand, edges_, w_,   anb,    aw,      big_T,
+                  big_ln, big_meta, ray, cstart,   cend,   stats, colslot, slot_info};
     for (void *p : ps) dfree(ctx, p);
     stats = nullptr;
     cstart = cend = nullptr;
     cells_cap = cend_cap = 0;
-    cand = accepted = epos = edges_ = anb = nullptr;
+    cand = accepted = epos = edges_ = anb = colslot = nullptr;
+    slot_info = nullptr;
+    colslot_cap = slot_info_cap = 0;
     w_ = aw = big_T = nullptr;
     big_meta = nullptr;
     big_ln = nullptr;
@@ -1491,7 +1652,11 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
     LG_TRY(grow(ctx, &pl.edges_, &pl.edges_cap, (size_t)2 * Ktot + 2));
     LG_TRY(grow(ctx, &pl.w_, &pl.w_cap, (size_t)Ktot + 1));
     const int cap = (max_connect > 0 && max_connect < 255) ? max_connect : 0;
-    LG_TRY(zero_ints(ctx, {{cap > 0 ? pl.colcnt : nullptr, (size_t)V}, {pl.visit + V, 1}, {pl.accepted + Ktot, 1}}));
+    LG_TRY(zero_ints(ctx, {{cap > 0 ? pl.colcnt : nullptr, (size_t)V},
+                           {pl.visit + V, 1},
+                           {pl.accepted + Ktot, 1},
+                           {cap > 0 ? pl.colfill : nullptr, (size_t)V},
+                           {pl.greedy_err, 1}}));
     k_cand<true><<<wb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted, pl.off,
                                      nullptr, pl.cand);
     LG_LAUNCHED("k_cand");
@@ -1507,7 +1672,21 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
         k_compact_rows<<<vb, 256, 0, st>>>(V, pl.visit, pl.visit_pos, pl.visit_list);
         hipEvent_t ev_g;
         ktimer_begin(ctx, "k_greedy", &ev_g);
-        if (nmax <= 48 * 1024)
+        const size_t word_bytes = (size_t)nmax * sizeof(uint32_t);
+        // dense graphs (many contested rows) decide rows in parallel; sparse ones
+        // touch few rows, and the single-wave sweep avoids the rank launches
+        if (word_bytes <= 60 * 1024 && (long long)Ktot >= 4ll * V) {
+            // rows in parallel: column ranks of the risky slots, then the waves
+            LG_TRY(grow(ctx, &pl.colslot, &pl.colslot_cap, (size_t)Ktot + 1));
+            LG_TRY(grow(ctx, &pl.slot_info, &pl.slot_info_cap, (size_t)Ktot + 1));
+            LG_TRY(scan_excl(ctx, pl, pl.colcnt, pl.coloff, (size_t)V, false));
+            const unsigned kb = (unsigned)((Ktot + 255) / 256 + 1);
+            k_col_fill<<<kb, 256, 0, st>>>(Ktot, pl.cand, pl.risky, pl.coloff, pl.colfill, pl.colslot);
+            k_col_rank<<<kb, 256, 0, st>>>(Ktot, pl.cand, pl.risky, pl.coloff, pl.colcnt, pl.colslot, pl.slot_info);
+            k_greedy_waves<<<S, 64 * kGreedyWaves, word_bytes, st>>>(pl.sd, cap, pl.off, pl.slot_info, pl.risky,
+                                                                     pl.colcnt, pl.visit_list, pl.visit_pos + V,
+                                                                     pl.accepted, pl.greedy_err);
+        } else if (nmax <= 48 * 1024)
             k_greedy_staged<<<S, 64, (size_t)nmax, st>>>(pl.sd, cap, pl.off, pl.cand, pl.risky, pl.visit_list,
                                                          pl.visit_pos + V, pl.accepted);
         else
@@ -1573,7 +1752,8 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     k_comp_offsets<<<vb, 256, 0, st>>>(V, pl.flag, pl.cid, pl.comp_off, pl.cid + V);
     if (L_ready) LG_HIP(hipStreamWaitEvent(st, L_ready, 0));  // k_comp_local is the first reader of L
     k_comp_local<<<vb, 256, 0, st>>>(V, pl.sd, pl.sid, pl.flag, pl.cid, pl.comp_off, pl.member, pl.local, pl.big_list,
-                                     pl.counters, pl.cls16, pl.cls48, pl.cls128, L, out, pl.cid + V, d_nedges);
+                                     pl.counters, pl.cls16, pl.cls48, pl.cls128, L, out, pl.cid + V, d_nedges,
+                                     pl.greedy_err);
     LG_LAUNCHED("k_comp_local");
     const int max_big = V / (kSmallComp + 1);
     if (pl.bfs_order && max_big > 0) {
@@ -1598,6 +1778,10 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     LG_HIP(hipMemcpyAsync(rb.data(), pl.counters, sizeof(int) * rb.size(), hipMemcpyDeviceToHost, st));
     LG_HIP(hipStreamSynchronize(st));
     ktimer_end(ctx, "k_components", ev);
+    if (rb[10] != 0) {
+        set_error("pl_graph: a row of the parallel greedy cap waited past its bound");
+        return LG_ERR_HIP;
+    }
     const int C = rb[8], nbig = rb[0], n16 = rb[1], n48 = rb[2], n128 = rb[3];
     res->n_edges_total = rb[9];
     res->n_components = C;

@@ -424,9 +424,14 @@ int32_t lg_build_edges(lg_ctx *ctx, const float *d_pts, int32_t n, double radius
     const float rf = (float)radius;
     const float r2f = rf * rf;  // float, as searchRadiusConnectSquared (LightGraph.h:171)
     LG_TRY(pl_graph(ctx, pl, V, 1, nmax, d_pts, (double)r2f, max_connect, &Ktot));
-    int32_t ne = 0;
+    int32_t ne = 0, gerr = 0;
     LG_HIP(hipMemcpyAsync(&ne, pl.epos + Ktot, 4, hipMemcpyDeviceToHost, ctx->stream));
+    LG_HIP(hipMemcpyAsync(&gerr, pl.greedy_err, 4, hipMemcpyDeviceToHost, ctx->stream));
     LG_HIP(hipStreamSynchronize(ctx->stream));
+    if (gerr != 0) {
+        set_error("lg_build_edges: a row of the parallel greedy cap waited past its bound");
+        return LG_ERR_HIP;
+    }
     *n_edges_out = ne;
     if (ne > edge_cap) {
         set_error("lg_build_edges: %d edges exceed edge_cap %d", ne, edge_cap);
