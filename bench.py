@@ -85,11 +85,15 @@ def batch_workload(lga, ctx, count, seed=3000):
     return grids, [int(x) for x in sides], [int(x) for x in ns]
 
 
+ORDERS = ("reference", "bfs", "nd")
+
+
 def bench_batched(lga, ctx, shader, params, lights, count, reps):
-    """C3: the 64 volumes solved one by one (plain, ascending pivot order: the
-    drop-in path) and as one batch, each in both pivot orders (ascending and
-    breadth-first, LG_SOLVE_BFS_ORDER).  Every speedup is against the plain
-    ascending-order pass; the roofline is that of the breadth-first batch."""
+    """C3: the 64 volumes solved one by one (plain) and as one batch, each in the
+    three pivot orders: ascending (the reference's, the drop-in path),
+    breadth-first (LG_SOLVE_BFS_ORDER) and nested dissection (LG_SOLVE_ND_ORDER).
+    Every speedup is against the plain ascending-order pass; the roofline is that
+    of the nested-dissection batch."""
     import numpy as np
     radius = 12.0
     grids, sides, ns = batch_workload(lga, ctx, count)
@@ -116,22 +120,20 @@ def bench_batched(lga, ctx, shader, params, lights, count, reps):
                 worst = max(worst, float(np.max(np.abs(a64 - b64) / np.maximum(np.abs(b64), 1e-6 * np.abs(b64).max()))))
         return worst
 
-    ref = run_plain("reference", keep=True)
-    ref_bfs = run_plain("bfs", keep=True)
+    plain_out = {order: run_plain(order, keep=True) for order in ORDERS}
     diffs = {}
-    for order in ("reference", "bfs"):
+    for order in ORDERS:
         run_batch(order)
         got = [r["diffuse"] for r in batch.batch_results()]
-        same = ref if order == "reference" else ref_bfs
         diffs[order] = {"max_abs_diff_vs_plain_same_order": max(float(np.max(np.abs(a - b))) if len(a) else 0.0
-                                                                for a, b in zip(got, same)),
-                        "max_rel_diff_vs_plain_reference_order": rel_diff(got, ref)}
+                                                                for a, b in zip(got, plain_out[order])),
+                        "max_rel_diff_vs_plain_reference_order": rel_diff(got, plain_out["reference"])}
     ms = {}
     for name, fn in (("plain", run_plain), ("batched", run_batch)):
-        for order in ("reference", "bfs"):
+        for order in ORDERS:
             ctx.sync()
             ctx.mark(2)
-            if name == "batched" and order == "bfs":
+            if name == "batched" and order == "nd":
                 ctx.kernel_time_reset()
                 ctx.kernel_timing(True)
             for _ in range(reps):
@@ -142,12 +144,12 @@ def bench_batched(lga, ctx, shader, params, lights, count, reps):
             ms[(name, order)] = ctx.elapsed_ms(2, 3) / reps
     btimes = kernel_times(ctx, KERNELS + STAGES)
     tiles = {}
-    for order in ("reference", "bfs"):
+    for order in ORDERS:
         run_batch(order, stats=True)
         st = batch.stats()
         tiles[order] = int(st["fw_tiles"])
     bwork = {k: v * reps for k, v in st.items()}
-    broof = kernel_roofline(btimes, bwork, "c3_batch_bfs_order")
+    broof = kernel_roofline(btimes, bwork, "c3_batch_nd_order")
     plain.destroy()
     batch.destroy()
     for g in grids:
@@ -159,15 +161,16 @@ def bench_batched(lga, ctx, shader, params, lights, count, reps):
             "ms": {f"{a}_{b}": round(t, 3) for (a, b), t in ms.items()},
             "solves_per_s": per_s,
             "plain_ms": round(t_plain, 3), "batched_ms": round(ms[("batched", "reference")], 3),
-            "batched_bfs_ms": round(ms[("batched", "bfs")], 3),
+            "batched_nd_ms": round(ms[("batched", "nd")], 3),
             "plain_solves_per_s": per_s["plain_reference"],
             "batched_solves_per_s": per_s["batched_reference"],
-            "batched_bfs_solves_per_s": per_s["batched_bfs"],
+            "batched_nd_solves_per_s": per_s["batched_nd"],
             "speedup": round(t_plain / ms[("batched", "reference")], 3),
             "speedup_bfs": round(t_plain / ms[("batched", "bfs")], 3),
+            "speedup_nd": round(t_plain / ms[("batched", "nd")], 3),
             "speedup_note": "plain = one lg_solve per volume in ascending pivot order (the drop-in path); "
-                            "speedup = batched ascending order, speedup_bfs = batched breadth-first order, "
-                            "both over plain ascending order",
+                            "speedup / speedup_bfs / speedup_nd = the batch in ascending / breadth-first / "
+                            "nested-dissection order, each over plain ascending order",
             "fw_remainder_tiles_per_pass": tiles,
             "diffs": diffs,
             "max_abs_diff_batched_vs_plain": diffs["reference"]["max_abs_diff_vs_plain_same_order"],
@@ -277,7 +280,7 @@ def bench_serving(lga, ctx, rates, jobs_per_rate, workers, max_batch, seed=5):
     [1000, 16000], images 512^2 / 1280x720 / 1920x1080 at 1-4 spp, one of three
     light sets (directional, point, environment); one LightGraph solve per job.
     Latency = arrival -> films on the host.  The served runs close the graphs in
-    breadth-first pivot order; the same job list is also run through a
+    nested-dissection pivot order; the same job list is also run through a
     one-worker, one-job-per-batch server in the ascending (drop-in) order: the
     plain path, back to back."""
     import numpy as np
@@ -331,11 +334,11 @@ def bench_serving(lga, ctx, rates, jobs_per_rate, workers, max_batch, seed=5):
            "images": ["512x512", "1280x720", "1920x1080"], "spp": [1, 4], "workers": workers, "max_batch": max_batch,
            "rates": {}}
     for lam in rates:
-        jobs = make_jobs(jobs_per_rate, "bfs")
+        jobs = make_jobs(jobs_per_rate, "nd")
         arrivals = np.cumsum(rng.exponential(1.0 / lam, jobs_per_rate))
         out["rates"][str(lam)] = run(jobs, arrivals, workers, max_batch)
     state = rng.bit_generator.state
-    jobs = make_jobs(jobs_per_rate, "bfs")
+    jobs = make_jobs(jobs_per_rate, "nd")
     out["saturated"] = run(jobs, np.zeros(len(jobs)), workers, max_batch)
     rng.bit_generator.state = state  # the same jobs in the drop-in pivot order
     out["plain"] = run(make_jobs(jobs_per_rate, "reference"), np.zeros(len(jobs)), 1, 1)

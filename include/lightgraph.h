@@ -330,7 +330,7 @@ typedef struct lg_solve_params {
     int32_t timing;        /* 1: record per-stage HIP events                       */
     int32_t flags;         /* LG_SOLVE_DENSE: also write the dense n x ld outputs;
                             * LG_SOLVE_STATS: count the work (lg_solver_stats);
-                            * LG_SOLVE_BFS_ORDER: see below                           */
+                            * LG_SOLVE_BFS_ORDER, LG_SOLVE_ND_ORDER: see below        */
 } lg_solve_params;
 
 /* LG_SOLVE_BFS_ORDER (no reference counterpart; off in the drop-in path): close
@@ -342,7 +342,16 @@ typedef struct lg_solve_params {
  * agree with the ascending order to FP32 rounding, bit-exact for integer
  * weights).  Points, edges and the outputs keep the vertex indices of the call;
  * at most 2^21 vertices per call. */
-enum { LG_SOLVE_DENSE = 1, LG_SOLVE_STATS = 2, LG_SOLVE_BFS_ORDER = 4 };
+/* LG_SOLVE_ND_ORDER (no reference counterpart; off in the drop-in path; not
+ * with LG_SOLVE_BFS_ORDER): nested-dissection vertex order for the same
+ * components.  Each component's vertex list is split recursively at the
+ * midpoint of its points' longest bounding-box side; the left vertices with a
+ * neighbour on the right (the separator) are ordered after both halves, down to
+ * parts of 256 vertices, each part in ascending vertex order.  While the closure
+ * pivots through one half the other stays unreachable, so far more remainder
+ * tiles are skipped than in breadth-first order; the same FP32 rounding note
+ * applies. */
+enum { LG_SOLVE_DENSE = 1, LG_SOLVE_STATS = 2, LG_SOLVE_BFS_ORDER = 4, LG_SOLVE_ND_ORDER = 8 };
 
 /* Work done by the last lg_solve / lg_solve_batch call, counted on the device
  * when it ran with LG_SOLVE_STATS (otherwise the counters read 0).  The counts are
@@ -448,7 +457,7 @@ typedef struct lg_job {
     lg_march_params march;
     lg_camera camera;
     lg_light lights[LG_MAX_LIGHTS];
-    int32_t solve_flags;          /* 0 or LG_SOLVE_BFS_ORDER; jobs share a solve
+    int32_t solve_flags;          /* 0, LG_SOLVE_BFS_ORDER or LG_SOLVE_ND_ORDER; jobs share a solve
                                    * launch only with equal flags                  */
 } lg_job;
 

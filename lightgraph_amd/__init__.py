@@ -397,15 +397,14 @@ def diffuse(ctx, n, D_buf, ld, sigma_t, L_buf, out_buf):
     _check(lib().lg_diffuse(ctx.handle, n, D_buf.ptr, ld, sig, L_buf.ptr, out_buf.ptr), "diffuse")
 
 
-LG_SOLVE_DENSE, LG_SOLVE_STATS, LG_SOLVE_BFS_ORDER = 1, 2, 4
+LG_SOLVE_DENSE, LG_SOLVE_STATS, LG_SOLVE_BFS_ORDER, LG_SOLVE_ND_ORDER = 1, 2, 4, 8
+ORDERS = {"reference": 0, "bfs": LG_SOLVE_BFS_ORDER, "nd": LG_SOLVE_ND_ORDER}
 
 
 def _order_flag(order):
-    if order == "reference":
-        return 0
-    if order == "bfs":
-        return LG_SOLVE_BFS_ORDER
-    raise ValueError(f"order must be 'reference' or 'bfs', not {order!r}")
+    if order not in ORDERS:
+        raise ValueError(f"order must be one of {sorted(ORDERS)}, not {order!r}")
+    return ORDERS[order]
 
 
 class Solver:
@@ -424,8 +423,9 @@ class Solver:
               temperature=None, env_seed=4564446, external_points=False, timing=False, dense=None, stats=False,
               order="reference"):
         """order: "reference" closes each component in ascending vertex order (the
-        reference's pivot order); "bfs" closes the components of more than 128
-        vertices in breadth-first order (LG_SOLVE_BFS_ORDER)."""
+        reference's pivot order); "bfs" and "nd" close the components of more
+        than 128 vertices in breadth-first (LG_SOLVE_BFS_ORDER) or
+        nested-dissection (LG_SOLVE_ND_ORDER) order."""
         sp = lg_solve_params()
         sp.n, sp.max_connect, sp.radius, sp.seed, sp.env_seed = n, max_connect, radius, seed, env_seed
         sp.solve, sp.external_points, sp.timing = solve, 1 if external_points else 0, 1 if timing else 0

@@ -394,6 +394,7 @@ struct Pipeline {
     unsigned long long *stats = nullptr;  // ST_COUNT counters; kernels add to them when stats_on
     bool stats_on = false;
     bool bfs_order = false;  // large components closed in breadth-first vertex order
+    bool nd_order = false;   // large components closed in nested-dissection vertex order
     double *ray = nullptr;  // per (vertex, light sample) shadowed intensity
     size_t ray_cap = 0;
     // neighbour grid
@@ -425,7 +426,7 @@ int32_t pl_march(lg_ctx *ctx, Pipeline &pl, int dtype, const MarchConsts &mc, co
 // waits on it before the first kernel that reads L)
 int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, int V, int Ktot,
                       const double ext[3], const float *L, float *out, PipelineResult *res,
-                      hipEvent_t L_ready = nullptr);
+                      hipEvent_t L_ready = nullptr, const float *pts = nullptr);
 
 // helpers from lg_solve.hip
 int32_t fw_init_impl(lg_ctx *ctx, int n, float *d_D, int ld);

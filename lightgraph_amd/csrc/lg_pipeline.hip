@@ -964,6 +964,164 @@ __global__ void __launch_bounds__(1024) k_bfs_levels(const int *__restrict__ cou
     }
 }
 
+// Nested-dissection vertex order of the large components (LG_SOLVE_ND_ORDER):
+// one workgroup per component splits its member list recursively.  A segment
+// is cut at the midpoint of its points' longest bounding-box side; its left
+// vertices with a neighbour on the right (in the same segment) form the
+// separator; the segment is stably partitioned into [left - separator | right |
+// separator] and the first two parts are split again down to kNdLeaf vertices.
+// Separators come after both halves they separate, so while the blocked closure
+// pivots through one half, the other half's rows stay +INF in the panel and
+// their tiles are skipped.  Starting from ascending vertex order, every part
+// stays ascending: the order is a function of the points and the graph only.
+constexpr int kNdLeaf = 256;
+constexpr int kNdStack = 64;
+
+__device__ __forceinline__ float block_reduce_min(float v, float *red) {
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    float r = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) r = fminf(r, red[i]);
+    return r;
+}
+
+__device__ __forceinline__ int block_excl_scan(int x, int *red, int *total) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int inc = x;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += y;
+    }
+    __syncthreads();
+    if (lane == 63) red[w] = inc;
+    __syncthreads();
+    int before = 0, all = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+        if (i < w) before += red[i];
+        all += red[i];
+    }
+    *total = all;
+    return before + inc - x;
+}
+
+__global__ void __launch_bounds__(1024) k_nd_order(const int *__restrict__ counters, const int *__restrict__ big_list,
+                                                   const int *__restrict__ comp_off, int *__restrict__ member,
+                                                   int *__restrict__ local, const float *__restrict__ pts,
+                                                   const int *__restrict__ aoff, const int *__restrict__ anb,
+                                                   int *__restrict__ segtag, int *__restrict__ cls,
+                                                   int *__restrict__ tmp) {
+    const int b = (int)blockIdx.x;
+    if (b >= counters[0]) return;
+    const int c = big_list[2 * b], m = big_list[2 * b + 1];
+    const int a0 = comp_off[c];
+    int *perm = member + a0;
+    int *scratch = tmp + a0;
+    __shared__ float redf[16];
+    __shared__ int redi[16];
+    __shared__ int st_lo[kNdStack], st_hi[kNdStack];
+    __shared__ int sp;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    if (tid == 0) {
+        st_lo[0] = 0;
+        st_hi[0] = m;
+        sp = 1;
+    }
+    __syncthreads();
+    int segid = 0;
+    while (true) {
+        __syncthreads();
+        if (sp == 0) break;
+        const int lo = st_lo[sp - 1], hi = st_hi[sp - 1];
+        __syncthreads();
+        if (tid == 0) --sp;
+        const int n = hi - lo;
+        if (n <= kNdLeaf) continue;
+        ++segid;
+        // tag the segment's vertices, bounding box of their points
+        float mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int i = lo + tid; i < hi; i += nt) {
+            const int v = perm[i];
+            segtag[v] = segid;
+            for (int a = 0; a < 3; ++a) {
+                mn[a] = fminf(mn[a], pts[3 * v + a]);
+                mx[a] = fmaxf(mx[a], pts[3 * v + a]);
+            }
+        }
+        float bmn[3], bmx[3];
+        for (int a = 0; a < 3; ++a) {
+            bmn[a] = block_reduce_min(mn[a], redf);
+            bmx[a] = -block_reduce_min(-mx[a], redf);
+        }
+        int ax = 0;
+        for (int a = 1; a < 3; ++a)
+            if (bmx[a] - bmn[a] > bmx[ax] - bmn[ax]) ax = a;
+        const float mid = 0.5f * (bmn[ax] + bmx[ax]);
+        // side of every vertex (0 left, 1 right), then the separator (2)
+        int nright = 0;
+        for (int i = lo + tid; i < hi; i += nt) {
+            const int v = perm[i];
+            const int sd = pts[3 * v + ax] < mid ? 0 : 1;
+            cls[v] = sd;
+            nright += sd;
+        }
+        __threadfence_block();
+        int right_all = 0;
+        (void)block_excl_scan(nright, redi, &right_all);
+        if (right_all == 0 || right_all == n) continue;  // all on one side: a leaf
+        __syncthreads();
+        // separator: left vertices with a right neighbour in this segment get
+        // class 2 (a left vertex turning 0 -> 2 leaves every "is right" test alone)
+        for (int i = lo + tid; i < hi; i += nt) {
+            const int v = perm[i];
+            if (cls[v] != 0) continue;
+            bool sep = false;
+            for (int p = aoff[v]; p < aoff[v + 1] && !sep; ++p) {
+                const int u = anb[p];
+                sep = segtag[u] == segid && cls[u] == 1;
+            }
+            if (sep) cls[v] = 2;
+        }
+        __threadfence_block();
+        __syncthreads();
+        // stable partition [left - separator | right | separator] through scratch
+        int counts[3] = {0, 0, 0};
+        for (int i = lo + tid; i < hi; i += nt) ++counts[cls[perm[i]]];
+        int tot[3];
+        for (int q = 0; q < 3; ++q) (void)block_excl_scan(counts[q], redi, &tot[q]);
+        int carry[3] = {0, 0, 0};
+        const int start[3] = {lo, lo + tot[0], lo + tot[0] + tot[1]};
+        for (int base = lo; base < hi; base += nt) {
+            const int i = base + tid;
+            const int q = i < hi ? cls[perm[i]] : -1;
+            for (int qq = 0; qq < 3; ++qq) {
+                int t = 0;
+                const int off = block_excl_scan(q == qq ? 1 : 0, redi, &t);
+                if (q == qq) scratch[start[qq] - lo + carry[qq] + off] = perm[i];
+                carry[qq] += t;
+            }
+        }
+        __syncthreads();
+        for (int i = lo + tid; i < hi; i += nt) perm[i] = scratch[i - lo];
+        __syncthreads();
+        if (tid == 0) {
+            if (tot[1] > kNdLeaf && sp < kNdStack) {
+                st_lo[sp] = lo + tot[0];
+                st_hi[sp] = lo + tot[0] + tot[1];
+                ++sp;
+            }
+            if (tot[0] > kNdLeaf && sp < kNdStack) {
+                st_lo[sp] = lo;
+                st_hi[sp] = lo + tot[0];
+                ++sp;
+            }
+        }
+    }
+    for (int i = tid; i < m; i += nt) local[perm[i]] = i;
+}
+
 // Sort keys (root, level, vertex); vertices of the other components keep level 0.
 __global__ void k_keys_bfs(int V, int bits, const int *__restrict__ root, const int *__restrict__ level,
                            uint64_t *__restrict__ keys) {
@@ -1720,7 +1878,8 @@ int32_t pl_march(lg_ctx *ctx, Pipeline &pl, int dtype, const MarchConsts &mc, co
 
 // Components, closure (c) and diffusion; dense outputs when the descriptors ask.
 int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, int V, int Ktot,
-                      const double ext[3], const float *L, float *out, PipelineResult *res, hipEvent_t L_ready) {
+                      const double ext[3], const float *L, float *out, PipelineResult *res, hipEvent_t L_ready,
+                      const float *pts) {
     hipStream_t st = ctx->stream;
     const unsigned vb = (unsigned)((V + 255) / 256);
     const int *d_nedges = pl.epos + Ktot;
@@ -1756,7 +1915,14 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
                                      pl.greedy_err);
     LG_LAUNCHED("k_comp_local");
     const int max_big = V / (kSmallComp + 1);
-    if (pl.bfs_order && max_big > 0) {
+    if (pl.nd_order && max_big > 0) {
+        LG_CHECK_ARG(pts, "pl_components: nested-dissection order needs the points");
+        // segment tags in level[], classes and the partition scratch in the key buffer
+        int *qa = reinterpret_cast<int *>(pl.keys);
+        k_nd_order<<<(unsigned)max_big, 1024, 0, st>>>(pl.counters, pl.big_list, pl.comp_off, pl.member, pl.local, pts,
+                                                        pl.aoff, pl.anb, pl.level, qa, qa + V);
+        LG_LAUNCHED("k_nd_order");
+    } else if (pl.bfs_order && max_big > 0) {
         LG_CHECK_ARG(3 * bits <= 64, "breadth-first order: %d vertices in one call exceed 2^21", V);
         LG_TRY(zero_ints(ctx, {{pl.level, (size_t)V}}));
         // the frontier queues borrow the 64-bit key buffer (2V ints), refilled below

@@ -313,7 +313,7 @@ struct Worker {
         sp.radius = h.radius;
         sp.seed = h.seed;
         sp.env_seed = h.env_seed;
-        sp.flags = h.solve_flags & LG_SOLVE_BFS_ORDER;
+        sp.flags = h.solve_flags & (LG_SOLVE_BFS_ORDER | LG_SOLVE_ND_ORDER);
         if (rc == LG_OK)
             rc = lg_solve_batch(ctx, solver, items.data(), (int32_t)items.size(), &h.shader, h.lights, h.n_lights,
                                 &h.march, &sp);

@@ -320,7 +320,10 @@ static int32_t solve_descs(lg_ctx *ctx, lg_solver *s, const std::vector<SolveDes
     const bool t = sp->timing != 0;
     const int dtype = descs[0].dtype;
     pl.stats_on = (sp->flags & LG_SOLVE_STATS) != 0;
+    LG_CHECK_ARG((sp->flags & (LG_SOLVE_BFS_ORDER | LG_SOLVE_ND_ORDER)) != (LG_SOLVE_BFS_ORDER | LG_SOLVE_ND_ORDER),
+                 "lg_solve: LG_SOLVE_BFS_ORDER and LG_SOLVE_ND_ORDER exclude each other");
     pl.bfs_order = (sp->flags & LG_SOLVE_BFS_ORDER) != 0;
+    pl.nd_order = (sp->flags & LG_SOLVE_ND_ORDER) != 0;
     if (pl.stats_on) {
         if (!pl.stats) LG_TRY(dalloc(ctx, (size_t)ST_COUNT * kStatSlots, &pl.stats));
         LG_HIP(hipMemsetAsync(pl.stats, 0, sizeof(unsigned long long) * ST_COUNT * kStatSlots, ctx->stream));
@@ -365,7 +368,7 @@ static int32_t solve_descs(lg_ctx *ctx, lg_solver *s, const std::vector<SolveDes
     LG_TRY(pl_march(ctx, pl, dtype, pp.mc, pp.pts, Ktot));
     if (t) LG_HIP(hipEventRecord(s->ev[4], ctx->stream));
     PipelineResult res;
-    LG_TRY(pl_components(ctx, pl, descs, V, Ktot, pp.ext, pp.L, pp.out, &res, L_ready));
+    LG_TRY(pl_components(ctx, pl, descs, V, Ktot, pp.ext, pp.L, pp.out, &res, L_ready, pp.pts));
     if (t) LG_HIP(hipEventRecord(s->ev[5], ctx->stream));
     s->timed = t;
     s->buf.n = V;

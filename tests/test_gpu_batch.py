@@ -202,10 +202,12 @@ def test_solve_batch_equals_plain(lga, ctx, volumes):
     plain.destroy()
 
 
-def test_solve_batch_bfs_order(lga, ctx, volumes):
-    """Breadth-first pivot order (LG_SOLVE_BFS_ORDER): the batch equals the plain
-    solves in the same order bit for bit, agrees with the ascending order within
-    FP32 rounding, and skips remainder tiles of the large components."""
+@pytest.mark.parametrize("order", ["bfs", "nd"])
+def test_solve_batch_bfs_order(lga, ctx, volumes, order):
+    """Breadth-first and nested-dissection pivot orders (LG_SOLVE_BFS_ORDER,
+    LG_SOLVE_ND_ORDER): the batch equals the plain solves in the same order bit
+    for bit, agrees with the ascending order within FP32 rounding, and skips
+    remainder tiles of the large components."""
     sh = lga.default_shader(absorption=0.1, scattering=(1.0, 0.9, 0.8))
     p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5)
     lights = [lga.point_light(0.0, (1.0, 0.9, 0.8), 4.0e4, (64.0, 200.0, 64.0)),
@@ -213,7 +215,7 @@ def test_solve_batch_bfs_order(lga, ctx, volumes):
     ns = [512, 1400, 3000, 700, 2100, 900]
     items = [(g, n, 40 + k) for k, (g, n) in enumerate(zip(volumes, ns))]
     bs = lga.Solver(ctx, sum(ns))
-    bs.solve_batch(items, sh, lights, p, radius=6.0, seed=854123, order="bfs", stats=True)
+    bs.solve_batch(items, sh, lights, p, radius=6.0, seed=854123, order=order, stats=True)
     got = bs.batch_results()
     tiles_bfs = bs.stats()["fw_tiles"]
     bs.solve_batch(items, sh, lights, p, radius=6.0, seed=854123, stats=True)
@@ -222,7 +224,7 @@ def test_solve_batch_bfs_order(lga, ctx, volumes):
     assert 0 < tiles_bfs < tiles_asc
     plain = lga.Solver(ctx, max(ns))
     for k, (g, n, solve) in enumerate(items):
-        plain.solve(g, sh, lights, p, n=n, radius=6.0, seed=854123, solve=solve, order="bfs")
+        plain.solve(g, sh, lights, p, n=n, radius=6.0, seed=854123, solve=solve, order=order)
         ref = plain.results(want_D=False)
         for key in ("points", "L", "diffuse", "edges", "weights"):
             np.testing.assert_array_equal(got[k][key], ref[key], err_msg=f"item {k} ({key})")

@@ -33,7 +33,7 @@ JOBS = [("sphere", 64, 1, 600, 48, 40, 1, 2), ("blobs", 96, 7, 1500, 64, 48, 2, 
         ("sphere", 64, 2, 700, 32, 32, 3, 2)]
 
 
-@pytest.mark.parametrize("order", ["reference", "bfs"])
+@pytest.mark.parametrize("order", ["reference", "bfs", "nd"])
 def test_alone_equals_mixed_and_direct(lga, ctx, order):
     jobs = [_job(lga, *j, order=order) for j in JOBS]
     srv = lga.Server(8 * GB, workers=2, max_batch=4)

@@ -393,12 +393,14 @@ def test_full_solve_giant_component_pred(lga, ctx, oracle, c1):
     s.destroy()
 
 
-def test_full_solve_bfs_order(lga, ctx, oracle, c1):
-    """LG_SOLVE_BFS_ORDER on the C1 graph (one component far above 128 vertices):
-    the same points, edges, weights and vertex light as the ascending order; the
-    closure in breadth-first pivot order agrees with the FP64 oracle and with the
-    ascending order within FP32 rounding (1e-5 relative, same +INF pattern); its
-    predecessors follow the canonical rule; it computes no more remainder tiles."""
+@pytest.mark.parametrize("order", ["bfs", "nd"])
+def test_full_solve_bfs_order(lga, ctx, oracle, c1, order):
+    """LG_SOLVE_BFS_ORDER / LG_SOLVE_ND_ORDER on the C1 graph (one component far
+    above 128 vertices): the same points, edges, weights and vertex light as the
+    ascending order; the closure in the other pivot order agrees with the FP64
+    oracle and with the ascending order within FP32 rounding (1e-5 relative, same
+    +INF pattern); its predecessors follow the canonical rule; it computes no
+    more remainder tiles."""
     grid, dense, _ = c1
     sh = _c1_shader(lga)
     p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5)
@@ -408,7 +410,7 @@ def test_full_solve_bfs_order(lga, ctx, oracle, c1):
     s.solve(grid, sh, lights, p, n=n, radius=8.0, seed=854123, solve=5, stats=True)
     ref_order = s.results()
     tiles_ref = s.stats()["fw_tiles"]
-    s.solve(grid, sh, lights, p, n=n, radius=8.0, seed=854123, solve=5, stats=True, order="bfs")
+    s.solve(grid, sh, lights, p, n=n, radius=8.0, seed=854123, solve=5, stats=True, order=order)
     r = s.results()
     st = s.stats()
     assert st["big_components"] >= 1 and st["fw_tiles"] <= tiles_ref
