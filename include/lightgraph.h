@@ -4,7 +4,7 @@
  * The reference (asiroisvigneux/LightGraph) is a single C++ program; its hot path
  * is the LightGraph estimator (src/LightGraph.h), the scatter-grid splat
  * (src/ScatterGrid.h) and the per-pixel march (src/Renderer.h).  This header is the
- * device boundary that the C++ host classes (lightgraph_amd/csrc/host) call.  Every
+ * device boundary that the C++ host classes (lightgraph_amd/host) call.  Every
  * entry point names the reference code it replaces.
  *
  * Conventions
