@@ -87,9 +87,23 @@ __device__ __forceinline__ float to_f32(float v) { return v; }
 __device__ __forceinline__ float to_f32(_Float16 v) { return (float)v; }
 __device__ __forceinline__ float to_f32(bf16_t v) { return __uint_as_float(((uint32_t)v.v) << 16); }
 
+// Element e of a raw buffer resource over brick storage of type T (32-bit
+// byte offsets from the uniform base).
+template <typename T> __device__ __forceinline__ float buf_tap(__amdgpu_buffer_rsrc_t r, uint32_t e);
+template <> __device__ __forceinline__ float buf_tap<float>(__amdgpu_buffer_rsrc_t r, uint32_t e) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, e * 4u, 0, 0));
+}
+template <> __device__ __forceinline__ float buf_tap<_Float16>(__amdgpu_buffer_rsrc_t r, uint32_t e) {
+    return (float)__builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(r, e * 2u, 0, 0));
+}
+template <> __device__ __forceinline__ float buf_tap<bf16_t>(__amdgpu_buffer_rsrc_t r, uint32_t e) {
+    return __uint_as_float(((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, e * 2u, 0, 0)) << 16);
+}
+
 struct GridView {
     const void *data;
     const uint8_t *occ;
+    const uint8_t *skip;   // per storage brick: radius of the uniform box around it
     int NBx, NBy, NBz;     // storage bricks per axis (with the 1-brick border)
     int nbx, nby, nbz;     // bricks per axis
     long long sy, sz;      // element stride of one brick step in y and z
@@ -98,6 +112,8 @@ struct GridView {
     double origin[3];
     double voxel, inv_voxel;
     int empty;             // 1: no active brick (every ray misses)
+    int off32;             // 1: the storage is below 4 GiB (taps use buffer loads)
+    uint32_t data_bytes;   // storage bytes when off32
 };
 
 // Element offset of voxel (x,y,z); callers keep x in [-8, 8*nbx+7] etc.
@@ -128,19 +144,42 @@ __device__ __forceinline__ float trilinear(const GridView &g, double x, double y
     const int cx = clampi(ix, -8, 8 * g.nbx + 6);
     const int cy = clampi(iy, -8, 8 * g.nby + 6);
     const int cz = clampi(iz, -8, 8 * g.nbz + 6);
-    const T *d = static_cast<const T *>(g.data);
-    const long long b = voxel_offset(g, cx, cy, cz);
-    const long long dx = ((cx & 7) == 7) ? (512 - 7) : 1;
-    const long long dy = ((cy & 7) == 7) ? (g.sy - 56) : 8;
-    const long long dz = ((cz & 7) == 7) ? (g.sz - 448) : 64;
-    const float d000 = to_f32(d[b]);
-    const float d001 = to_f32(d[b + dz]);
-    const float d010 = to_f32(d[b + dy]);
-    const float d011 = to_f32(d[b + dy + dz]);
-    const float d100 = to_f32(d[b + dx]);
-    const float d101 = to_f32(d[b + dx + dz]);
-    const float d110 = to_f32(d[b + dx + dy]);
-    const float d111 = to_f32(d[b + dx + dy + dz]);
+    float d000, d001, d010, d011, d100, d101, d110, d111;
+    if (g.off32) {
+        // Storage below 4 GiB: raw buffer loads with 32-bit offsets (stride 0,
+        // num_records = storage bytes; 0x20000 is the gfx9 raw-buffer format word).
+        const __amdgpu_buffer_rsrc_t r =
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(g.data), (short)0, (int)g.data_bytes, 0x00020000);
+        const uint32_t brick = ((uint32_t)((cz >> 3) + 1) * (uint32_t)g.NBy + (uint32_t)((cy >> 3) + 1)) *
+                                   (uint32_t)g.NBx + (uint32_t)((cx >> 3) + 1);
+        const uint32_t b = (brick << 9) + ((cz & 7) << 6) + ((cy & 7) << 3) + (cx & 7);
+        const uint32_t dx = ((cx & 7) == 7) ? (512u - 7u) : 1u;
+        const uint32_t dy = ((cy & 7) == 7) ? ((uint32_t)g.sy - 56u) : 8u;
+        const uint32_t dz = ((cz & 7) == 7) ? ((uint32_t)g.sz - 448u) : 64u;
+        auto at = [r](uint32_t e) { return buf_tap<T>(r, e); };
+        d000 = at(b);
+        d001 = at(b + dz);
+        d010 = at(b + dy);
+        d011 = at(b + dy + dz);
+        d100 = at(b + dx);
+        d101 = at(b + dx + dz);
+        d110 = at(b + dx + dy);
+        d111 = at(b + dx + dy + dz);
+    } else {
+        const T *d = static_cast<const T *>(g.data);
+        const long long b = voxel_offset(g, cx, cy, cz);
+        const long long dx = ((cx & 7) == 7) ? (512 - 7) : 1;
+        const long long dy = ((cy & 7) == 7) ? (g.sy - 56) : 8;
+        const long long dz = ((cz & 7) == 7) ? (g.sz - 448) : 64;
+        d000 = to_f32(d[b]);
+        d001 = to_f32(d[b + dz]);
+        d010 = to_f32(d[b + dy]);
+        d011 = to_f32(d[b + dy + dz]);
+        d100 = to_f32(d[b + dx]);
+        d101 = to_f32(d[b + dx + dz]);
+        d110 = to_f32(d[b + dx + dy]);
+        d111 = to_f32(d[b + dx + dy + dz]);
+    }
     const float c00 = box_lerp(d000, d001, w);
     const float c01 = box_lerp(d010, d011, w);
     const float c10 = box_lerp(d100, d101, w);
@@ -155,6 +194,23 @@ __device__ __forceinline__ bool brick_exists(const GridView &g, int vx, int vy, 
     const int bx = (vx >> 3) + 1, by = (vy >> 3) + 1, bz = (vz >> 3) + 1;
     if (bx < 0 || by < 0 || bz < 0 || bx >= g.NBx || by >= g.NBy || bz >= g.NBz) return false;
     return g.occ[((long long)bz * g.NBy + by) * g.NBx + bx] != 0;
+}
+
+// Per storage brick, the Chebyshev radius (in bricks) of the uniform box around
+// it: an empty brick holds d in [1, kSkipCap], the distance to the nearest
+// existing brick; an existing brick holds kSkipExists + d, d the distance to
+// the nearest empty brick.  Every brick closer than d has the same state.
+constexpr int kSkipCap = 16;
+constexpr int kSkipExists = 128;
+// Smallest d at which isect_march jumps over the box around a cell.
+constexpr int kSkipMin = 2;
+
+// Box code of the brick cell at (vx,vy,vz) (multiples of 8): cells outside the
+// storage read 1 (empty, nothing known around them).
+__device__ __forceinline__ int brick_skip(const GridView &g, int vx, int vy, int vz) {
+    const int bx = (vx >> 3) + 1, by = (vy >> 3) + 1, bz = (vz >> 3) + 1;
+    if (bx < 0 || by < 0 || bz < 0 || bx >= g.NBx || by >= g.NBy || bz >= g.NBz) return 1;
+    return g.skip[((long long)bz * g.NBy + by) * g.NBx + bx];
 }
 
 // ------------------------------------------------------------------ rays ----
@@ -249,6 +305,21 @@ __device__ __forceinline__ int dda_min_axis(double n0, double n1, double n2) {
 // One VolumeRayIntersector::march (Renderer.h:345-369) over the brick level:
 // the next span of consecutive existing 8^3 bricks along the ray.  Returns true
 // with [t0, t1] when a span longer than Delta<double> was found.
+//
+// The walk is DDA<Ray, 3>::step over the brick cells: crossing m of axis c is
+// at next0[c] + m * delta[c] and MinIndex takes the first (ties z, y, x).
+// Each iteration reads the box code of the current cell; when its uniform box
+// has radius R = d - 1 >= kSkipMin - 1 the walk through the box only moves the
+// DDA state (inside an empty box no span is open, inside an existing one the
+// open span goes on), so every crossing up to and including the first that
+// leaves the box is taken at once: along each axis that is crossing m + R, the
+// exit is the first of the three in MinIndex order, and the other axes take
+// the crossings that precede it (at most R each, counted by binary search on
+// the closed form).  R = 0 is the single DDA step.  The crossings, cells and
+// times are those of the cell-by-cell walk.  Every cell read has an entry time
+// within the clip box, so it lies inside the storage (one border brick on each
+// side): the brick index is kept as one linear offset, range-checked only as
+// a guard.
 __device__ __forceinline__ bool isect_march(const GridView &g, Isect &s, double &t0, double &t1) {
     double st0 = -1.0, st1 = -1.0;
     if (s.t1 - s.t0 > kDeltaF) {
@@ -257,35 +328,57 @@ __device__ __forceinline__ bool isect_march(const GridView &g, Isect &s, double 
         double next0[3], delta[3];
         int vox[3], step[3];
         dda_setup(s, mT0, vox, step, next0, delta);
-        double n0 = next0[0], n1 = next0[1], n2 = next0[2];
+        const int sx = 1, sy = g.NBx, sz = g.NBx * g.NBy;
+        const int nlin = sz * g.NBz;
+        int lin = ((vox[2] >> 3) + 1) * sz + ((vox[1] >> 3) + 1) * sy + ((vox[0] >> 3) + 1);
+        const int l0 = step[0] > 0 ? sx : (step[0] < 0 ? -sx : 0);
+        const int l1 = step[1] > 0 ? sy : (step[1] < 0 ? -sy : 0);
+        const int l2 = step[2] > 0 ? sz : (step[2] < 0 ? -sz : 0);
         int m0 = 0, m1 = 0, m2 = 0;
         for (;;) {
-            if (brick_exists(g, vox[0], vox[1], vox[2])) {
-                if (st0 < 0) st0 = mT0;
-            } else if (st0 >= 0) {
+            const bool inside = (unsigned)lin < (unsigned)nlin;
+            const int code = inside ? (int)g.skip[inside ? lin : 0] : 1;
+            const bool exists = code >= kSkipExists;
+            if (!exists && st0 >= 0 && mT0 - st0 > kDeltaD) {
                 st1 = mT0;
-                if (st1 - st0 > kDeltaD) break;
-                st0 = st1 = -1.0;
+                break;
             }
-            // DDA::step (constant indices keep the state in registers)
-            const int ax = dda_min_axis(n0, n1, n2);
-            if (ax == 0) {
-                mT0 = n0;
-                n0 = next0[0] + (double)(++m0) * delta[0];
-                vox[0] += step[0];
-            } else if (ax == 1) {
-                mT0 = n1;
-                n1 = next0[1] + (double)(++m1) * delta[1];
-                vox[1] += step[1];
-            } else {
-                mT0 = n2;
-                n2 = next0[2] + (double)(++m2) * delta[2];
-                vox[2] += step[2];
-            }
-            if (!(mT0 <= mT1)) {
+            st0 = exists ? (st0 < 0 ? mT0 : st0) : -1.0;
+            const int d = exists ? code - kSkipExists : code;
+            const int R = d >= kSkipMin ? d - 1 : 0;
+            const double e0 = next0[0] + (double)(m0 + R) * delta[0];
+            const double e1 = next0[1] + (double)(m1 + R) * delta[1];
+            const double e2 = next0[2] + (double)(m2 + R) * delta[2];
+            const int ax = dda_min_axis(e0, e1, e2);
+            const double ex = ax == 0 ? e0 : (ax == 1 ? e1 : e2);
+            mT0 = ex;
+            if (!(ex <= mT1)) {  // the walk ends inside the box (or at this step)
                 if (st0 >= 0) st1 = mT1;
                 break;
             }
+            int k0 = 0, k1 = 0, k2 = 0;
+            if (R > 0) {
+                auto taken = [&](int b, int m) {
+                    int lo = 0, hi = R;
+                    while (lo < hi) {
+                        const int mid = (lo + hi) >> 1;
+                        const double c = next0[b] + (double)(m + mid) * delta[b];
+                        if (c < ex || (c == ex && b > ax)) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    return lo;
+                };
+                k0 = ax == 0 ? 0 : taken(0, m0);
+                k1 = ax == 1 ? 0 : taken(1, m1);
+                k2 = ax == 2 ? 0 : taken(2, m2);
+            }
+            k0 += ax == 0 ? R + 1 : 0;
+            k1 += ax == 1 ? R + 1 : 0;
+            k2 += ax == 2 ? R + 1 : 0;
+            m0 += k0;
+            m1 += k1;
+            m2 += k2;
+            lin += __mul24(k0, l0) + __mul24(k1, l1) + __mul24(k2, l2);
         }
     }
     if (st1 > 0) {

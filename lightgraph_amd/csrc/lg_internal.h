@@ -110,6 +110,7 @@ struct lg_grid {
     int NB[3] = {0, 0, 0};       // storage bricks per axis (nb+2)
     void *data = nullptr;        // NB[0]*NB[1]*NB[2]*512 elements of desc.dtype
     uint8_t *occ = nullptr;      // NB product, 1 = brick exists (has an active voxel)
+    uint8_t *skip = nullptr;     // NB product, box code (lgd::brick_skip): radius of the uniform box
     int active_bricks = 0;
     int bmin[3] = {0, 0, 0}, bmax[3] = {-1, -1, -1};  // active brick bbox (inclusive)
     uint64_t active_voxels = 0;
@@ -208,6 +209,7 @@ inline lgd::GridView make_view(const lg_grid *g) {
     lgd::GridView v{};
     v.data = g->data;
     v.occ = g->occ;
+    v.skip = g->skip;
     v.NBx = g->NB[0];
     v.NBy = g->NB[1];
     v.NBz = g->NB[2];
@@ -224,6 +226,9 @@ inline lgd::GridView make_view(const lg_grid *g) {
     v.voxel = g->desc.voxel_size;
     v.inv_voxel = 1.0 / g->desc.voxel_size;
     v.empty = g->active_bricks == 0;
+    const size_t data_bytes = grid_storage_elems(g) * dtype_bytes(g->desc.dtype);
+    v.off32 = data_bytes < (1ull << 32);
+    v.data_bytes = v.off32 ? (uint32_t)data_bytes : 0u;
     return v;
 }
 

@@ -165,6 +165,45 @@ __global__ void k_brick_occupancy(const T *__restrict__ data, long long nbricks,
     }
 }
 
+// One axis of a max-norm distance transform over the storage bricks:
+// out = min over |k| < kSkipCap of max(|k|, in at offset k along the axis),
+// capped at kSkipCap (the max-norm distance separates into one pass per axis).
+// Pass 0 reads the occupancy: 0 where occ == target, kSkipCap elsewhere.
+// Neighbours outside the storage are left out (the border bricks inside it
+// are empty and nearer).
+__global__ void k_brick_dist(const uint8_t *__restrict__ in, uint8_t *__restrict__ out, int NBx, int NBy, int NBz,
+                             int axis, int from_occ, int target) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long nb = (long long)NBx * NBy * NBz;
+    if (i >= nb) return;
+    const int bx = (int)(i % NBx);
+    const int by = (int)((i / NBx) % NBy);
+    const int bz = (int)(i / ((long long)NBx * NBy));
+    const int c = axis == 0 ? bx : (axis == 1 ? by : bz);
+    const int n = axis == 0 ? NBx : (axis == 1 ? NBy : NBz);
+    const long long stride = axis == 0 ? 1 : (axis == 1 ? (long long)NBx : (long long)NBx * NBy);
+    int best = lgd::kSkipCap;
+    for (int k = -(lgd::kSkipCap - 1); k < lgd::kSkipCap; ++k) {
+        const int cc = c + k;
+        if (cc < 0 || cc >= n) continue;
+        const uint8_t v = in[i + k * stride];
+        const int d = from_occ ? (((v != 0) == (target != 0)) ? 0 : lgd::kSkipCap) : (int)v;
+        const int ak = k < 0 ? -k : k;
+        const int m = d > ak ? d : ak;
+        best = m < best ? m : best;
+    }
+    out[i] = (uint8_t)best;
+}
+
+// Box codes: an empty brick keeps its distance to the nearest existing brick,
+// an existing one gets kSkipExists + its distance to the nearest empty brick.
+__global__ void k_brick_code(const uint8_t *__restrict__ occ, const uint8_t *__restrict__ to_existing,
+                             const uint8_t *__restrict__ to_empty, long long nb, uint8_t *__restrict__ code) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    code[i] = occ[i] ? (uint8_t)(lgd::kSkipExists + to_empty[i]) : to_existing[i];
+}
+
 // Brick storage back to a dense float volume (x fastest).
 template <typename T>
 __global__ void k_bricks_to_slab(const T *__restrict__ data, int z0, int nzs, GridDims d,
@@ -221,8 +260,13 @@ int32_t grid_finalize(lg_ctx *ctx, lg_grid *g) {
     const long long nbricks = (long long)g->NB[0] * g->NB[1] * g->NB[2];
     unsigned long long *d_cnt = nullptr;
     void *s = nullptr;
-    LG_TRY(scratch(ctx, 256, &s));
+    LG_TRY(scratch(ctx, 256 + 2 * (size_t)nbricks, &s));
     d_cnt = (unsigned long long *)s;
+    uint8_t *d_a = (uint8_t *)s + 256, *d_b = d_a + nbricks;
+    if (!g->skip) {
+        LG_TRY(dmalloc(ctx, (size_t)nbricks, (void **)&g->skip));
+        g->bytes += (uint64_t)nbricks;
+    }
     LG_HIP(hipMemsetAsync(d_cnt, 0, sizeof(unsigned long long), ctx->stream));
     switch (g->desc.dtype) {
         case LG_F32:
@@ -236,6 +280,20 @@ int32_t grid_finalize(lg_ctx *ctx, lg_grid *g) {
             break;
     }
     LG_LAUNCHED("k_brick_occupancy");
+    {
+        const unsigned blocks = (unsigned)((nbricks + 255) / 256);
+        const int NBx = g->NB[0], NBy = g->NB[1], NBz = g->NB[2];
+        // distance to the nearest existing brick into a, to the nearest empty one into b
+        k_brick_dist<<<blocks, 256, 0, ctx->stream>>>(g->occ, d_a, NBx, NBy, NBz, 0, 1, 1);
+        k_brick_dist<<<blocks, 256, 0, ctx->stream>>>(d_a, g->skip, NBx, NBy, NBz, 1, 0, 0);
+        k_brick_dist<<<blocks, 256, 0, ctx->stream>>>(g->skip, d_a, NBx, NBy, NBz, 2, 0, 0);
+        k_brick_dist<<<blocks, 256, 0, ctx->stream>>>(g->occ, d_b, NBx, NBy, NBz, 0, 1, 0);
+        k_brick_dist<<<blocks, 256, 0, ctx->stream>>>(d_b, g->skip, NBx, NBy, NBz, 1, 0, 0);
+        k_brick_dist<<<blocks, 256, 0, ctx->stream>>>(g->skip, d_b, NBx, NBy, NBz, 2, 0, 0);
+        LG_LAUNCHED("k_brick_dist");
+        k_brick_code<<<blocks, 256, 0, ctx->stream>>>(g->occ, d_a, d_b, nbricks, g->skip);
+        LG_LAUNCHED("k_brick_code");
+    }
     std::vector<uint8_t> occ((size_t)nbricks);
     unsigned long long cnt = 0;
     LG_HIP(hipMemcpyAsync(occ.data(), g->occ, nbricks, hipMemcpyDeviceToHost, ctx->stream));
@@ -550,6 +608,7 @@ int32_t lg_grid_destroy(lg_ctx *ctx, lg_grid *g) {
     LG_CHECK_ARG(ctx, "lg_grid_destroy: ctx is NULL");
     dfree(ctx, g->data);
     dfree(ctx, g->occ);
+    dfree(ctx, g->skip);
     dfree(ctx, g->active_list);
     dfree(ctx, g->mass_prefix);
     delete g;
