@@ -13,6 +13,7 @@ struct DevLight {
     int type, samples;
     float intensity;
     int pad;
+    double inv_samples;  // 1/samples when samples is a power of two (exact), else 0
     double color[3];
     double dir[3];
     double pos[3];
@@ -40,6 +41,7 @@ inline int32_t make_dev_lights(const lg_light *lights, int n, DevLights *out) {
         LG_CHECK_ARG(s.samples >= 1, "lights: samples must be >= 1");
         d.type = s.type;
         d.samples = s.samples;
+        d.inv_samples = (s.samples & (s.samples - 1)) == 0 ? 1.0 / (double)s.samples : 0.0;
         d.intensity = s.intensity;
         for (int c = 0; c < 3; ++c) {
             d.color[c] = s.color[c];

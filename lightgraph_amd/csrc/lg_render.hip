@@ -324,9 +324,15 @@ __device__ __forceinline__ void render_ramp(const RenderConsts &c, double x, dou
 // then holds points at the same depth of neighbouring rays, whose shadow rays
 // are alike, and reads and writes its points in order.
 struct ShadePoint {
-    double wp[3];  // world position
+    double ip[3];  // index-space position (world: origin + voxel * ip)
     uint32_t ray;  // r = pixel * samples + sample
     uint32_t ord;  // ordinal of the shading point on its ray
+};
+// What the combine phase takes from the primary march at a shading point: the
+// optical depth before the point, its density and the march step it was taken
+// with (same slot as the ShadePoint).
+struct ShadeExtra {
+    double tau_before, pd, step;
 };
 struct RayOut {
     double lum[3], dl[3], sl[3], el[3];  // beauty, direct, scatter, emission
@@ -413,22 +419,29 @@ __device__ __forceinline__ void ray_pixel(const RenderConsts &c, uint32_t r, uin
     i = (int)(pix - (uint32_t)j * (uint32_t)c.width);
 }
 
-// Phase count: shading points of every ray (0 when no light needs them).
+// Phase count: shading points of every ray, and the camera-ray and
+// primary-sample counters.
 template <typename T>
-__global__ void __launch_bounds__(256) k_render_count(lgd::GridView g, RenderConsts c, int n_lights, uint32_t k0,
-                                                      uint32_t k1, uint32_t nrays, int *__restrict__ cnt) {
+__global__ void __launch_bounds__(256) k_render_count(lgd::GridView g, RenderConsts c, uint32_t k0, uint32_t k1,
+                                                      uint32_t nrays, int *__restrict__ cnt,
+                                                      unsigned long long *__restrict__ stats) {
     const uint32_t r = blockIdx.x * 256u + threadIdx.x;
-    if (r >= nrays) return;
-    uint32_t pix, s;
-    int i, j;
-    ray_pixel(c, r, pix, s, i, j);
-    lgd::Stream rng;
-    rng.init(pix, s, lgd::DOM_RENDER, ((uint64_t)k1 << 32) | k0);
-    lgd::Isect pr;
     uint32_t n = 0, smp = 0;
-    if (n_lights > 0 && camera_ray(g, c, i, j, rng, pr))
-        (void)primary_march<T>(g, c, pr, rng, &smp, [&](const double *, double, double, double) { ++n; });
-    cnt[r] = (int)n;
+    if (r < nrays) {
+        uint32_t pix, s;
+        int i, j;
+        ray_pixel(c, r, pix, s, i, j);
+        lgd::Stream rng;
+        rng.init(pix, s, lgd::DOM_RENDER, ((uint64_t)k1 << 32) | k0);
+        lgd::Isect pr;
+        if (camera_ray(g, c, i, j, rng, pr))
+            (void)primary_march<T>(g, c, pr, rng, &smp, [&](const double *, double, double, double) { ++n; });
+        cnt[r] = (int)n;
+    }
+    if (stats) {
+        render_stat(stats, RS_CAMERA_RAYS, r < nrays ? 1 : 0);
+        render_stat(stats, RS_PRIMARY_SAMPLES, smp);
+    }
 }
 
 // Shading points of each block of 64 pixels.
@@ -494,7 +507,8 @@ __device__ __forceinline__ int ord_pos(const OrdTable &t, int k) {
 template <typename T>
 __global__ void __launch_bounds__(256) k_render_fill(lgd::GridView g, RenderConsts c, uint32_t k0, uint32_t k1,
                                                      uint32_t ray0, uint32_t nrays, const int *__restrict__ cnt,
-                                                     const int *__restrict__ off, ShadePoint *__restrict__ pts) {
+                                                     const int *__restrict__ off, ShadePoint *__restrict__ pts,
+                                                     ShadeExtra *__restrict__ ext) {
     __shared__ OrdTable tab[4];
     const uint32_t rl = blockIdx.x * 256u + threadIdx.x;
     const uint32_t r = ray0 + rl;
@@ -513,12 +527,17 @@ __global__ void __launch_bounds__(256) k_render_fill(lgd::GridView g, RenderCons
     lgd::Isect pr;
     if (!camera_ray(g, c, i, j, rng, pr)) return;
     uint32_t ord = 0, smp = 0;
-    (void)primary_march<T>(g, c, pr, rng, &smp, [&](const double *ip, double, double, double) {
+    (void)primary_march<T>(g, c, pr, rng, &smp, [&](const double *ip, double pd, double step, double tau_before) {
         if ((int)ord < qn) {
-            ShadePoint &p = pts[base + ord_pos(t, (int)ord)];
-            for (int k = 0; k < 3; ++k) p.wp[k] = g.origin[k] + g.voxel * ip[k];
+            const int q = base + ord_pos(t, (int)ord);
+            ShadePoint &p = pts[q];
+            for (int k = 0; k < 3; ++k) p.ip[k] = ip[k];
             p.ray = r;
             p.ord = ord;
+            ShadeExtra &e = ext[q];
+            e.tau_before = tau_before;
+            e.pd = pd;
+            e.step = step;
         }
         ++ord;
     });
@@ -575,6 +594,8 @@ __global__ void __launch_bounds__(256, 4) k_render_shadow(lgd::GridView g, Rende
     if (li < L.n && q < P) {
         const lgl::DevLight &lt = L.l[li];
         const ShadePoint sp = pts[q];
+        const double wp[3] = {g.origin[0] + g.voxel * sp.ip[0], g.origin[1] + g.voxel * sp.ip[1],
+                              g.origin[2] + g.voxel * sp.ip[2]};
         const uint32_t pix = sp.ray / (uint32_t)c.samples;
         const uint32_t tag = shadow_tag(sp.ray - pix * (uint32_t)c.samples, li);
         lgd::Stream rng;
@@ -587,12 +608,12 @@ __global__ void __launch_bounds__(256, 4) k_render_shadow(lgd::GridView g, Rende
                 lgd::Stream env;
                 env.init(pix, sp.ord, lgd::DOM_RENDER_ENV | tag, ((uint64_t)ek1 << 32) | ek0);
                 if (lt.type == LG_LIGHT_ENV && ls > 0) env.skip(2u * (uint32_t)ls);
-                lgl::illuminate(lt, sp.wp, env, ldir, inten, dist);
+                lgl::illuminate(lt, wp, env, ldir, inten, dist);
             }
             double tau_s;
             bool sdrop;
             ++rays;
-            if (!render_shadow<T>(g, c, sp.wp, ldir, dist, rng, &tau_s, &sdrop, &smp)) continue;
+            if (!render_shadow<T>(g, c, wp, ldir, dist, rng, &tau_s, &sdrop, &smp)) continue;
             if (sdrop) continue;
             for (int a = 0; a < 3; ++a) tl[a] += inten[a] * (double)expf((float)(c.ext[a] * tau_s));
         }
@@ -606,14 +627,21 @@ __global__ void __launch_bounds__(256, 4) k_render_shadow(lgd::GridView g, Rende
     }
 }
 
-// Phase combine over rays [ray0, ray0 + nrays) (Renderer.h:580-668).
-template <typename T>
-__global__ void __launch_bounds__(256, 3) k_render_combine(lgd::GridView g, lgd::GridView tg, int has_temp,
+// Phase combine over rays [ray0, ray0 + nrays) (Renderer.h:580-668): the
+// shading points of each ray in march order, from the records the fill phase
+// stored (no second march).  Per point: transmittance to the point and the
+// absorbed fraction of its step, the light sums of the shadow phase, the
+// scatter-grid and emission terms; the optical depth after the last point is
+// the ray's (the march adds pd * step after each point and stops there at a
+// dropout).
+__global__ void __launch_bounds__(256) k_render_combine(lgd::GridView g, lgd::GridView tg, int has_temp,
                                                         CoarseView sc, int has_scatter, RenderConsts c,
-                                                        lgl::DevLights L, uint32_t k0, uint32_t k1, uint32_t ray0,
-                                                        uint32_t nrays, const int *__restrict__ cnt,
-                                                        const int *__restrict__ off, const double *__restrict__ sums,
-                                                        uint32_t P_pad, RayOut *__restrict__ rout,
+                                                        lgl::DevLights L, uint32_t ray0, uint32_t nrays,
+                                                        const int *__restrict__ cnt, const int *__restrict__ off,
+                                                        const ShadePoint *__restrict__ pts,
+                                                        const ShadeExtra *__restrict__ ext,
+                                                        const double *__restrict__ sums, uint32_t P_pad,
+                                                        RayOut *__restrict__ rout,
                                                         unsigned long long *__restrict__ stats) {
     __shared__ OrdTable tab[4];
     const uint32_t rl = blockIdx.x * 256u + threadIdx.x;
@@ -623,75 +651,66 @@ __global__ void __launch_bounds__(256, 3) k_render_combine(lgd::GridView g, lgd:
     OrdTable &tb = tab[threadIdx.x >> 6];
     ord_table_build(tb, qn);
     __syncthreads();
-    uint32_t cam = 0, smp = 0, npts = 0;
     if (valid) {
-        const uint32_t r = ray0 + rl;
-        uint32_t pix, s;
-        int i, j;
-        ray_pixel(c, r, pix, s, i, j);
-        lgd::Stream rng;
-        rng.init(pix, s, lgd::DOM_RENDER, ((uint64_t)k1 << 32) | k0);
         RayOut o{};
-        o.tr[0] = o.tr[1] = o.tr[2] = 1.0;
-        lgd::Isect pr;
-        cam = 1;
-        if (camera_ray(g, c, i, j, rng, pr)) {
-            int ord = 0;
-            const double tau = primary_march<T>(g, c, pr, rng, &smp, [&](const double *ip, double pd, double step,
-                                                                         double tau_before) {
-                const double wp[3] = {g.origin[0] + g.voxel * ip[0], g.origin[1] + g.voxel * ip[1],
-                                      g.origin[2] + g.voxel * ip[2]};
-                double tf[3], om[3];
-                for (int k = 0; k < 3; ++k) {
-                    tf[k] = (double)expf((float)(c.ext[k] * tau_before));
-                    om[k] = (double)(-expm1f((float)(c.ext[k] * pd * step)));
-                }
-                if (ord < qn) {
-                    const uint32_t q = (uint32_t)(base + ord_pos(tb, ord));
-                    for (int li = 0; li < L.n; ++li) {
-                        const double *tl = sums + 3ull * ((uint64_t)li * P_pad + q);
-                        const double ns = (double)L.l[li].samples;
-                        for (int k = 0; k < 3; ++k) {
-                            const double v = c.albedo[k] * (tl[k] / ns) * tf[k] * om[k];
-                            o.lum[k] += v;
-                            o.dl[k] += v;
-                        }
-                    }
-                }
-                ++ord;
-                double msc[3] = {0, 0, 0};
-                if (has_scatter) {
-                    float sv[3];
-                    coarse_sample(sc, wp[0] * sc.inv_cv, wp[1] * sc.inv_cv, wp[2] * sc.inv_cv, sv);
-                    const double pc = pd < 0.0 ? 0.0 : (pd > 1.0 ? 1.0 : pd);
-                    const double mask = (double)powf((float)pc, (float)c.mask_pow);
-                    for (int k = 0; k < 3; ++k) msc[k] = (double)sv[k] * c.inv_scat[k] * mask * c.scatter_scale;
-                }
-                double emi[3] = {0, 0, 0};
-                if (c.ramp_n > 0) {
-                    double temp = 0.0;
-                    if (has_temp) temp = (double)lgd::trilinear<float>(tg, ip[0], ip[1], ip[2]);
-                    render_ramp(c, temp, emi);
-                    for (int k = 0; k < 3; ++k) emi[k] *= c.emission_scale;
+        double tau = 0.0;
+        for (int ord = 0; ord < qn; ++ord) {
+            const uint32_t q = (uint32_t)(base + ord_pos(tb, ord));
+            const ShadeExtra e = ext[q];
+            const double ip[3] = {pts[q].ip[0], pts[q].ip[1], pts[q].ip[2]};
+            const double pd = e.pd, step = e.step, tau_before = e.tau_before;
+            const double wp[3] = {g.origin[0] + g.voxel * ip[0], g.origin[1] + g.voxel * ip[1],
+                                  g.origin[2] + g.voxel * ip[2]};
+            double tf[3], om[3];
+            for (int k = 0; k < 3; ++k) {
+                tf[k] = (double)expf((float)(c.ext[k] * tau_before));
+                om[k] = (double)(-expm1f((float)(c.ext[k] * pd * step)));
+            }
+            for (int li = 0; li < L.n; ++li) {
+                const double *tl = sums + 3ull * ((uint64_t)li * P_pad + q);
+                double a[3];
+                const double inv = L.l[li].inv_samples;
+                if (inv != 0.0) {
+                    // x * 2^-k is x / 2^k exactly (both correctly rounded)
+                    for (int k = 0; k < 3; ++k) a[k] = tl[k] * inv;
+                } else {
+                    const double ns = (double)L.l[li].samples;
+                    for (int k = 0; k < 3; ++k) a[k] = tl[k] / ns;
                 }
                 for (int k = 0; k < 3; ++k) {
-                    const double ts = msc[k] * tf[k] * om[k];
-                    const double te = emi[k] * tf[k] * om[k];
-                    o.sl[k] += ts;
-                    o.el[k] += te;
-                    o.lum[k] += ts + te;
+                    const double v = c.albedo[k] * a[k] * tf[k] * om[k];
+                    o.lum[k] += v;
+                    o.dl[k] += v;
                 }
-            });
-            npts = (uint32_t)ord;
-            for (int k = 0; k < 3; ++k) o.tr[k] = (double)expf((float)(c.ext[k] * tau));  // kept at dropout
+            }
+            double msc[3] = {0, 0, 0};
+            if (has_scatter) {
+                float sv[3];
+                coarse_sample(sc, wp[0] * sc.inv_cv, wp[1] * sc.inv_cv, wp[2] * sc.inv_cv, sv);
+                const double pc = pd < 0.0 ? 0.0 : (pd > 1.0 ? 1.0 : pd);
+                const double mask = (double)powf((float)pc, (float)c.mask_pow);
+                for (int k = 0; k < 3; ++k) msc[k] = (double)sv[k] * c.inv_scat[k] * mask * c.scatter_scale;
+            }
+            double emi[3] = {0, 0, 0};
+            if (c.ramp_n > 0) {
+                double temp = 0.0;
+                if (has_temp) temp = (double)lgd::trilinear<float>(tg, ip[0], ip[1], ip[2]);
+                render_ramp(c, temp, emi);
+                for (int k = 0; k < 3; ++k) emi[k] *= c.emission_scale;
+            }
+            for (int k = 0; k < 3; ++k) {
+                const double ts = msc[k] * tf[k] * om[k];
+                const double te = emi[k] * tf[k] * om[k];
+                o.sl[k] += ts;
+                o.el[k] += te;
+                o.lum[k] += ts + te;
+            }
+            tau = tau_before + pd * step;
         }
+        for (int k = 0; k < 3; ++k) o.tr[k] = (double)expf((float)(c.ext[k] * tau));  // kept at dropout
         rout[rl] = o;
     }
-    if (stats) {
-        render_stat(stats, RS_CAMERA_RAYS, cam);
-        render_stat(stats, RS_PRIMARY_SAMPLES, smp);
-        render_stat(stats, RS_SHADING_POINTS, npts);
-    }
+    if (stats) render_stat(stats, RS_SHADING_POINTS, (unsigned long long)qn);
 }
 
 // Phase aa over pixels [pix0, pix0 + npix): AA samples summed in order.
@@ -728,7 +747,7 @@ __global__ void __launch_bounds__(256) k_render_aa(RenderConsts c, uint32_t pix0
 // kRenderPointBytes of shading-point data (a single block may exceed the latter).
 constexpr uint32_t kRenderBlockPixels = 64;
 constexpr uint64_t kRenderRayCap = 1u << 20;
-constexpr uint64_t kRenderPointBytes = 512ull << 20;
+constexpr uint64_t kRenderPointBytes = 1ull << 30;
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
@@ -746,8 +765,8 @@ inline RenderLayout render_layout(int w, int h, int samples, int n_lights) {
     l.nblocks = (l.npix + kRenderBlockPixels - 1) / kRenderBlockPixels;
     l.rays_per_block = (uint64_t)kRenderBlockPixels * (uint64_t)samples;
     l.ray_cap = std::min(std::max(kRenderRayCap, l.rays_per_block), l.nblocks * l.rays_per_block);
-    l.point_bytes = sizeof(ShadePoint) + 24ull * (uint64_t)n_lights;
-    l.point_cap = n_lights > 0 ? kRenderPointBytes / l.point_bytes : 0;
+    l.point_bytes = sizeof(ShadePoint) + sizeof(ShadeExtra) + 24ull * (uint64_t)n_lights;
+    l.point_cap = kRenderPointBytes / l.point_bytes;
     l.bsum_off = align256(4 * l.nrays);
     l.head_bytes = l.bsum_off + align256(8 * l.nblocks);
     size_t sb = 0;
@@ -1139,7 +1158,7 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
     ktimer_begin(ctx, "k_render_count", &ev);
     for_dtype_r(density->desc.dtype, [&](auto tag) {
         using T = decltype(tag);
-        k_render_count<T><<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(g, c, L.n, u0, u1, nrays, cnt);
+        k_render_count<T><<<(nrays + 255) / 256, 256, 0, ctx->stream>>>(g, c, u0, u1, nrays, cnt, stats);
     });
     LG_LAUNCHED("k_render_count");
     ktimer_end(ctx, "k_render_count", ev);
@@ -1177,7 +1196,8 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
                  kRenderBlockPixels);
     const uint64_t pad_cap = (max_points + 63) & ~63ull;
     const size_t pts_off = lay.ray_area_bytes;
-    const size_t sum_off = pts_off + align256(sizeof(ShadePoint) * max_points);
+    const size_t ext_off = pts_off + align256(sizeof(ShadePoint) * max_points);
+    const size_t sum_off = ext_off + align256(sizeof(ShadeExtra) * max_points);
     const size_t body_bytes = sum_off + align256(24ull * (uint64_t)L.n * pad_cap);
     void *body = nullptr;
     LG_TRY(scratch(ctx, body_bytes, &body, 1));
@@ -1185,6 +1205,7 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
     void *scan_tmp = (char *)body + lay.scan_off;
     RayOut *rout = (RayOut *)((char *)body + lay.out_off);
     ShadePoint *pts = (ShadePoint *)((char *)body + pts_off);
+    ShadeExtra *ext = (ShadeExtra *)((char *)body + ext_off);
     double *sums = (double *)((char *)body + sum_off);
     for (const Chunk &ch : chunks) {
         const uint32_t pix0 = ch.b0 * kRenderBlockPixels;
@@ -1198,26 +1219,27 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
             ktimer_begin(ctx, "k_render_fill", &ev);
             for_dtype_r(density->desc.dtype, [&](auto tag) {
                 using T = decltype(tag);
-                k_render_fill<T><<<(nr + 255) / 256, 256, 0, ctx->stream>>>(g, c, u0, u1, ray0, nr, cnt, off, pts);
+                k_render_fill<T><<<(nr + 255) / 256, 256, 0, ctx->stream>>>(g, c, u0, u1, ray0, nr, cnt, off, pts,
+                                                                            ext);
             });
             LG_LAUNCHED("k_render_fill");
             ktimer_end(ctx, "k_render_fill", ev);
             const uint64_t tasks = (uint64_t)L.n * P_pad;
-            ktimer_begin(ctx, "k_render_shadow", &ev);
-            for_dtype_r(density->desc.dtype, [&](auto tag) {
-                using T = decltype(tag);
-                k_render_shadow<T><<<(unsigned)((tasks + 255) / 256), 256, 0, ctx->stream>>>(
-                    g, c, L, u0, u1, e0, e1, pts, P, P_pad, sums, stats);
-            });
-            LG_LAUNCHED("k_render_shadow");
-            ktimer_end(ctx, "k_render_shadow", ev);
+            if (tasks > 0) {
+                ktimer_begin(ctx, "k_render_shadow", &ev);
+                for_dtype_r(density->desc.dtype, [&](auto tag) {
+                    using T = decltype(tag);
+                    k_render_shadow<T><<<(unsigned)((tasks + 255) / 256), 256, 0, ctx->stream>>>(
+                        g, c, L, u0, u1, e0, e1, pts, P, P_pad, sums, stats);
+                });
+                LG_LAUNCHED("k_render_shadow");
+                ktimer_end(ctx, "k_render_shadow", ev);
+            }
         }
         ktimer_begin(ctx, "k_render_combine", &ev);
-        for_dtype_r(density->desc.dtype, [&](auto tag) {
-            using T = decltype(tag);
-            k_render_combine<T><<<(nr + 255) / 256, 256, 0, ctx->stream>>>(
-                g, tg, temperature ? 1 : 0, sc, has_sc, c, L, u0, u1, ray0, nr, cnt, off, sums, P_pad, rout, stats);
-        });
+        k_render_combine<<<(nr + 255) / 256, 256, 0, ctx->stream>>>(g, tg, temperature ? 1 : 0, sc, has_sc, c, L,
+                                                                    ray0, nr, cnt, off, pts, ext, sums, P_pad, rout,
+                                                                    stats);
         LG_LAUNCHED("k_render_combine");
         ktimer_end(ctx, "k_render_combine", ev);
         k_render_aa<<<(pix1 - pix0 + 255) / 256, 256, 0, ctx->stream>>>(
