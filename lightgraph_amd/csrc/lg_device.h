@@ -28,8 +28,11 @@ struct U4 {
 __device__ __forceinline__ U4 philox(U4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; ++r) {
-        const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
-        const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+        // one 32 x 32 -> 64 product per multiplier (a single v_mad_u64_u32
+        // instead of a mul_lo / mul_hi pair)
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
+        const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+        const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
         c = U4{hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0};
         k0 += 0x9E3779B9u;
         k1 += 0xBB67AE85u;
@@ -133,8 +136,10 @@ __device__ __forceinline__ float box_lerp(float a, float b, double w) {
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // BoxSampler::sample at index-space position (x,y,z): background 0 outside the
-// stored voxels.
-template <typename T>
+// stored voxels.  kBufferTaps: raw buffer loads with 32-bit offsets when the
+// storage allows (faster for per-lane rays; the wave-per-ray light march,
+// whose lanes tap neighbouring voxels, runs faster on 64-bit addresses).
+template <typename T, bool kBufferTaps = true>
 __device__ __forceinline__ float trilinear(const GridView &g, double x, double y, double z) {
     const double fx = floor(x), fy = floor(y), fz = floor(z);
     const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
@@ -145,7 +150,7 @@ __device__ __forceinline__ float trilinear(const GridView &g, double x, double y
     const int cy = clampi(iy, -8, 8 * g.nby + 6);
     const int cz = clampi(iz, -8, 8 * g.nbz + 6);
     float d000, d001, d010, d011, d100, d101, d110, d111;
-    if (g.off32) {
+    if (kBufferTaps && g.off32) {
         // Storage below 4 GiB: raw buffer loads with 32-bit offsets (stride 0,
         // num_records = storage bytes; 0x20000 is the gfx9 raw-buffer format word).
         const __amdgpu_buffer_rsrc_t r =

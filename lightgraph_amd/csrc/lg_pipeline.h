@@ -184,8 +184,8 @@ __device__ __forceinline__ bool shadow_tau_wave(const lgd::GridView &g, const Li
             *nsamp += __popcll(__ballot(act));
             double x = 0.0;
             if (act) {
-                const float rho =
-                    lgd::trilinear<T>(g, is.e[0] + is.d[0] * t, is.e[1] + is.d[1] * t, is.e[2] + is.d[2] * t);
+                const float rho = lgd::trilinear<T, false>(g, is.e[0] + is.d[0] * t, is.e[1] + is.d[1] * t,
+                                                           is.e[2] + is.d[2] * t);
                 double dens = (double)rho * lc.ds_sds_scds[0] * lc.ds_sds_scds[1] * lc.ds_sds_scds[2];
                 dens = (lc.dmax < dens) ? lc.dmax : dens;
                 if (!(dens < lc.cutoff)) x = dens * lc.h;
