@@ -188,6 +188,10 @@ int32_t lg_ctx_stream(lg_ctx *ctx, void **hip_stream_out);
 int32_t lg_ctx_bytes_in_use(lg_ctx *ctx, uint64_t *bytes);
 /* Per-kernel HIP-event timing of the library's own launches (for benchmarks). */
 int32_t lg_ctx_set_kernel_timing(lg_ctx *ctx, int32_t enable);
+/* While timing is on, time only the launches of the named kernel (NULL or "":
+ * every kernel), so that a timed region carries two events per launch of that
+ * kernel and none elsewhere.  No reference counterpart. */
+int32_t lg_ctx_set_kernel_timing_only(lg_ctx *ctx, const char *kernel);
 int32_t lg_kernel_time(lg_ctx *ctx, const char *kernel, uint64_t *launches, double *total_ms);
 int32_t lg_kernel_time_reset(lg_ctx *ctx);
 

@@ -193,7 +193,10 @@ class Context:
         _check(lib().lg_ctx_bytes_in_use(self.handle, C.byref(v)), "bytes_in_use")
         return v.value
 
-    def kernel_timing(self, enable=True):
+    def kernel_timing(self, enable=True, only=None):
+        """Per-kernel HIP-event timing; only: time the launches of that kernel alone."""
+        _check(lib().lg_ctx_set_kernel_timing_only(self.handle, only.encode() if only else None),
+               "kernel_timing_only")
         _check(lib().lg_ctx_set_kernel_timing(self.handle, 1 if enable else 0), "kernel_timing")
 
     def kernel_time(self, name):

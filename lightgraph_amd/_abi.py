@@ -155,6 +155,7 @@ SIGNATURES = {
     "lg_ctx_stream": (I32, [P, PP]),
     "lg_ctx_bytes_in_use": (I32, [P, C.POINTER(U64)]),
     "lg_ctx_set_kernel_timing": (I32, [P, I32]),
+    "lg_ctx_set_kernel_timing_only": (I32, [P, C.c_char_p]),
     "lg_kernel_time": (I32, [P, C.c_char_p, C.POINTER(U64), DP]),
     "lg_kernel_time_reset": (I32, [P]),
     "lg_malloc": (I32, [P, SZ, PP]),

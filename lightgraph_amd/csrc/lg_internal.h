@@ -79,6 +79,7 @@ struct lg_ctx {
     uint64_t in_use = 0;
     std::map<void *, size_t> allocs;
     bool timing = false;
+    std::string timing_only;  // non-empty: time only this kernel's launches
     lg::KernelTimer ktimer;
     hipEvent_t marks[16] = {};
     // scratch reused by entry points that need temporary device memory (slot 0),

@@ -153,8 +153,8 @@ __device__ __forceinline__ int cell_axis(double p, double lo, double inv, int n)
 }
 
 // Block s: bounding box of solve s's points and its cell grid.
-__global__ void __launch_bounds__(256) k_cell_setup(const SolveDesc *__restrict__ sd, const float *__restrict__ pts,
-                                                    double cell_min, CellDesc *__restrict__ cd) {
+__global__ void __launch_bounds__(1024) k_cell_setup(const SolveDesc *__restrict__ sd, const float *__restrict__ pts,
+                                                     double cell_min, CellDesc *__restrict__ cd) {
     const SolveDesc &d = sd[blockIdx.x];
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int v = d.voff + threadIdx.x; v < d.voff + d.n; v += blockDim.x)
@@ -162,20 +162,27 @@ __global__ void __launch_bounds__(256) k_cell_setup(const SolveDesc *__restrict_
             lo[c] = fminf(lo[c], pts[3 * v + c]);
             hi[c] = fmaxf(hi[c], pts[3 * v + c]);
         }
-    __shared__ float s_lo[3][256], s_hi[3][256];
+    // min / max within each wave by shuffles, then across the waves in LDS
+    __shared__ float s_lo[3][16], s_hi[3][16];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
     for (int c = 0; c < 3; ++c) {
-        s_lo[c][threadIdx.x] = lo[c];
-        s_hi[c][threadIdx.x] = hi[c];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            lo[c] = fminf(lo[c], __shfl_xor(lo[c], o, 64));
+            hi[c] = fmaxf(hi[c], __shfl_xor(hi[c], o, 64));
+        }
+        if (lane == 0) {
+            s_lo[c][w] = lo[c];
+            s_hi[c][w] = hi[c];
+        }
     }
     __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            for (int c = 0; c < 3; ++c) {
-                s_lo[c][threadIdx.x] = fminf(s_lo[c][threadIdx.x], s_lo[c][threadIdx.x + o]);
-                s_hi[c][threadIdx.x] = fmaxf(s_hi[c][threadIdx.x], s_hi[c][threadIdx.x + o]);
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c)
+            for (int k = 1; k < nw; ++k) {
+                s_lo[c][0] = fminf(s_lo[c][0], s_lo[c][k]);
+                s_hi[c][0] = fmaxf(s_hi[c][0], s_hi[c][k]);
             }
-        __syncthreads();
-    }
     if (threadIdx.x == 0) {
         CellDesc c{};
         for (int a = 0; a < 3; ++a) {
@@ -1648,21 +1655,33 @@ static int32_t zero_ints(lg_ctx *ctx, std::initializer_list<std::pair<int *, siz
 constexpr size_t kSmallScan = 65536;
 
 __global__ void __launch_bounds__(1024) k_scan_small(const int *__restrict__ in, int *__restrict__ out, int n) {
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
+    // thread t scans its run of consecutive elements; the run totals are
+    // scanned within each wave by shuffles and across the 16 waves in LDS
+    __shared__ int wsum[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int per = (n + 1023) / 1024;
     const int b = t * per, e = min(b + per, n);
     int sum = 0;
     for (int i = b; i < e; ++i) sum += in[i];
-    part[t] = sum;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const int v = t >= o ? part[t - o] : 0;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    int v = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(v, o, 64);
+        if (lane >= o) v += y;
     }
-    int run = part[t] - sum;
+    if (lane == 63) wsum[w] = v;
+    __syncthreads();
+    if (w == 0) {
+        int x = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+        for (int o = 1; o < 16; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        if (lane < 16) wsum[lane] = x;
+    }
+    __syncthreads();
+    int run = v - sum + (w > 0 ? wsum[w - 1] : 0);
     for (int i = b; i < e; ++i) {
         const int x = in[i];
         out[i] = run;
@@ -1782,7 +1801,7 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
     const double cell_min = sqrt(r2) * (1.0 + 1e-6) + 1e-30;
     hipEvent_t ev_c;
     ktimer_begin(ctx, "k_cand", &ev_c);
-    k_cell_setup<<<S, 256, 0, st>>>(pl.sd, pts, cell_min, pl.cd);
+    k_cell_setup<<<S, 1024, 0, st>>>(pl.sd, pts, cell_min, pl.cd);
     LG_LAUNCHED("k_cell_setup");
     k_cell_keys<<<vb, 256, 0, st>>>(pl.cd, pl.sid, V, pts, pl.ckeys, pl.cvals);
     LG_LAUNCHED("k_cell_keys");

@@ -79,9 +79,10 @@ static hipEvent_t ktimer_event(lg_ctx *ctx) {
     return e;
 }
 
-void ktimer_begin(lg_ctx *ctx, const char *, hipEvent_t *a) {
+void ktimer_begin(lg_ctx *ctx, const char *name, hipEvent_t *a) {
     *a = nullptr;
     if (!ctx->timing) return;
+    if (!ctx->timing_only.empty() && ctx->timing_only != name) return;
     *a = ktimer_event(ctx);
     if (*a) (void)hipEventRecord(*a, ctx->stream);
 }
@@ -413,6 +414,12 @@ int32_t lg_ctx_bytes_in_use(lg_ctx *ctx, uint64_t *bytes) {
 int32_t lg_ctx_set_kernel_timing(lg_ctx *ctx, int32_t enable) {
     LG_CHECK_ARG(ctx, "lg_ctx_set_kernel_timing: ctx is NULL");
     ctx->timing = enable != 0;
+    return LG_OK;
+}
+
+int32_t lg_ctx_set_kernel_timing_only(lg_ctx *ctx, const char *kernel) {
+    LG_CHECK_ARG(ctx, "lg_ctx_set_kernel_timing_only: ctx is NULL");
+    ctx->timing_only = kernel ? kernel : "";
     return LG_OK;
 }
 
