@@ -288,13 +288,15 @@ int32_t lg_vertex_light(lg_ctx *ctx, const lg_grid *density, const lg_grid *temp
                         const lg_march_params *params, uint64_t env_seed, uint32_t solve,
                         const float *d_pts, int32_t n, float *d_L_out);
 
-/* Fill D (n x ld, ld >= n) with +INF off the diagonal and 0 on it. */
+/* Fill D (ld rows of ld floats, ld >= n) with +INF off the diagonal and 0 on it. */
 int32_t lg_fw_init(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld);
 /* Required leading dimension of D for lg_fw_solve with n vertices. */
 int32_t lg_fw_ld(int32_t n);
 
-/* Kernel (c): all-pairs min-plus closure of the symmetric matrix D (n x ld,
- * ld = lg_fw_ld(n), zero diagonal, +INF = no edge) in place.  Replaces
+/* Kernel (c): all-pairs min-plus closure of the symmetric matrix D (ld rows of
+ * ld floats, ld = lg_fw_ld(n), zero diagonal, +INF = no edge; rows and columns
+ * n..ld-1 are padding, +INF off the diagonal as lg_fw_init leaves them, and are
+ * never taken as pivots) in place.  Replaces
  * LightGraph::floydAlgo (LightGraph.h:512-542): the reference's per-channel
  * (max, x) Floyd-Warshall on transmittance equals (min, +) on tau because
  * T_c = exp(-sigma_t[c]*tau) with a global sigma_t.  Pivots are taken in
@@ -388,7 +390,7 @@ int32_t lg_solve(lg_ctx *ctx, lg_solver *solver, const lg_grid *density,
 /* Many independent min-plus closures of uneven n (lg_fw_solve for each matrix)
  * in one launch sequence: each 128-pivot round closes the pivot tiles of every
  * matrix still that large in one launch, then all their panels, then all their
- * remainder tiles.  d_D[g] is n[g] x ld[g] with ld[g] = lg_fw_ld(n[g]).  The host
+ * remainder tiles.  d_D[g] is ld[g] x ld[g] with ld[g] = lg_fw_ld(n[g]).  The host
  * arrays are read during the call; the device matrices are updated in place.
  * Replaces the per-solve floydAlgo calls of a tbb::parallel_for over solves
  * (LightGraph.h:512-542 under main.cc:534-538). */

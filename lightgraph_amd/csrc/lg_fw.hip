@@ -40,7 +40,19 @@ constexpr int KC = 32;       // pivots staged in LDS per step
 // and G[r][x] = P_r[x] folded with min(., G[q][k+r] + G[q][x]) for q < r, where
 // P_r is row k+r before the group -- the same sums in the same operand order as
 // the one-pivot-at-a-time loop, so the result is bit-identical to it.
-__device__ __forceinline__ void fw_pivot_tile(float *__restrict__ D, int ld, int kb) {
+// Pivots of round kb that are vertices (the last round of an n-vertex matrix
+// holds n - kb * TB of them; the rest are padding, +INF rows and columns, which
+// relax nothing), and the staged pivot chunks that cover them.
+__device__ __forceinline__ int fw_round_pivots(int n, int kb) {
+    const int m = n - kb * TB;
+    return m < TB ? m : TB;
+}
+__device__ __forceinline__ int fw_round_kend(int n, int kb) {
+    return ((fw_round_pivots(n, kb) + KC - 1) / KC) * KC;
+}
+
+// m: pivots of the tile to take (the rest are padding)
+__device__ __forceinline__ void fw_pivot_tile(float *__restrict__ D, int ld, int kb, int m) {
     __shared__ float P[4][TB];  // rows of the group before it
     __shared__ float G[4][TB];  // the group's pivot rows
     const int tid = threadIdx.x;
@@ -55,7 +67,8 @@ __device__ __forceinline__ void fw_pivot_tile(float *__restrict__ D, int ld, int
         v[a][2] = q.z;
         v[a][3] = q.w;
     }
-    for (int g = 0; g < TB / 4; ++g) {
+    const int groups = (m + 3) / 4;
+    for (int g = 0; g < groups; ++g) {
         const int k = 4 * g;
         if (rg == g) {  // the owners of rows k..k+3 publish them
 #pragma unroll
@@ -111,15 +124,15 @@ __device__ __forceinline__ void fw_pivot_tile(float *__restrict__ D, int ld, int
 
 // One matrix: also clears the tile flags for the round's panel launch.
 __global__ void __launch_bounds__(1024) k_fw_pivot(float *__restrict__ D, int ld, int kb, int *__restrict__ flags,
-                                                   int nt) {
+                                                   int nt, int n) {
     for (int t = threadIdx.x; t < nt; t += blockDim.x) flags[t] = 0;
-    fw_pivot_tile(D, ld, kb);
+    fw_pivot_tile(D, ld, kb, fw_round_pivots(n, kb));
 }
 
 // One matrix of a batch: D (ld x ld, ld = 128 * nt), its tile flags at flags[flag_off].
 struct FwJob {
     float *D;
-    int ld, nt, flag_off, pad;
+    int ld, nt, flag_off, n;  // n: vertices (rows past n are padding)
 };
 
 // Batched pivot closure: block b closes the pivot tile of job active[b].
@@ -130,7 +143,7 @@ __global__ void __launch_bounds__(1024) k_fw_pivot_b(const FwJob *__restrict__ j
     const FwJob j = jobs[active[blockIdx.x]];
     if (stats && threadIdx.x == 0) stat_add(stats, ST_FW_PIVOTS, 1ull);
     for (int t = threadIdx.x; t < j.nt; t += blockDim.x) flags[j.flag_off + t] = 0;
-    fw_pivot_tile(j.D, j.ld, kb);
+    fw_pivot_tile(j.D, j.ld, kb, fw_round_pivots(j.n, kb));
 }
 
 __device__ __forceinline__ void tri_decode(int p, int T, int &a, int &b) {
@@ -147,8 +160,9 @@ __device__ __forceinline__ void tri_decode(int p, int T, int &a, int &b) {
 }
 
 template <bool PANEL>
-__device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, int kb, int nt, int *__restrict__ flags,
-                                                int tile, unsigned long long *__restrict__ stats = nullptr) {
+__device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, int kb, int nt, int kend,
+                                                int *__restrict__ flags, int tile,
+                                                unsigned long long *__restrict__ stats = nullptr) {
     int Ib, Jb;
     if (PANEL) {
         Ib = kb;
@@ -194,7 +208,7 @@ __device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, i
     }
     const float *Xg = D + (long long)kb * TB * ld + (long long)Ib * TB;
     const float *Yg = D + (long long)kb * TB * ld + (long long)Jb * TB;
-    for (int kc = 0; kc < TB; kc += KC) {
+    for (int kc = 0; kc < kend; kc += KC) {
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -266,8 +280,8 @@ __device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, i
 // Pivots fold in the same pairs and order as fw_minplus_tile (bit-identical
 // results).  A half that is all +INF stays so and is skipped; a half with a
 // finite value sets flags[Jb] to 1 (the caller has cleared the flags).
-__device__ __forceinline__ void fw_panel_half(float *__restrict__ D, int ld, int kb, int *__restrict__ flags, int tile,
-                                              int h, unsigned long long *__restrict__ stats) {
+__device__ __forceinline__ void fw_panel_half(float *__restrict__ D, int ld, int kb, int kend, int *__restrict__ flags,
+                                              int tile, int h, unsigned long long *__restrict__ stats) {
     const int Ib = kb, Jb = tile < kb ? tile : tile + 1;
     __shared__ float Xs[KC][TB];
     __shared__ float Ys[KC][64];
@@ -291,7 +305,7 @@ __device__ __forceinline__ void fw_panel_half(float *__restrict__ D, int ld, int
     if (stats && h == 0 && threadIdx.x == 0) stat_add(stats, ST_FW_PANEL_TILES, 1ull);
     const float *Xg = D + (long long)kb * TB * ld + (long long)Ib * TB;
     const float *Yg = D + (long long)kb * TB * ld + (long long)Jb * TB + c0;
-    for (int kc = 0; kc < TB; kc += KC) {
+    for (int kc = 0; kc < kend; kc += KC) {
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -358,8 +372,9 @@ __device__ __forceinline__ void fw_panel_half(float *__restrict__ D, int ld, int
 // Its operands are panel tiles, which no remainder workgroup writes.  Each
 // output folds the pivots in the same pairs and order as fw_minplus_tile, so
 // the result is bit-identical.
-__device__ __forceinline__ void fw_minplus_quad(float *__restrict__ D, int ld, int kb, int nt, int *__restrict__ flags,
-                                                int tile, int q, unsigned long long *__restrict__ stats) {
+__device__ __forceinline__ void fw_minplus_quad(float *__restrict__ D, int ld, int kb, int nt, int kend,
+                                                int *__restrict__ flags, int tile, int q,
+                                                unsigned long long *__restrict__ stats) {
     int a, b;
     tri_decode(tile, nt - 1, a, b);
     const int Ib = a < kb ? a : a + 1;
@@ -384,7 +399,7 @@ __device__ __forceinline__ void fw_minplus_quad(float *__restrict__ D, int ld, i
     const float *Xg = D + (long long)kb * TB * ld + (long long)Ib * TB + r0;
     const float *Yg = D + (long long)kb * TB * ld + (long long)Jb * TB + c0;
 
-    for (int kc = 0; kc < TB; kc += KC) {
+    for (int kc = 0; kc < kend; kc += KC) {
         __syncthreads();
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
@@ -435,14 +450,15 @@ constexpr int fw_parts() {
 
 template <bool PANEL, int SPLIT>
 __global__ void __launch_bounds__(256) k_fw_minplus(float *__restrict__ D, int ld, int kb, int nt,
-                                                     int *__restrict__ flags) {
+                                                     int *__restrict__ flags, int n) {
     constexpr int parts = fw_parts<PANEL, SPLIT>();
+    const int kend = fw_round_kend(n, kb);
     if constexpr (SPLIT == 1)
-        fw_minplus_tile<PANEL>(D, ld, kb, nt, flags, (int)blockIdx.x);
+        fw_minplus_tile<PANEL>(D, ld, kb, nt, kend, flags, (int)blockIdx.x);
     else if constexpr (PANEL)
-        fw_panel_half(D, ld, kb, flags, (int)blockIdx.x / parts, (int)blockIdx.x % parts, nullptr);
+        fw_panel_half(D, ld, kb, kend, flags, (int)blockIdx.x / parts, (int)blockIdx.x % parts, nullptr);
     else
-        fw_minplus_quad(D, ld, kb, nt, flags, (int)blockIdx.x / parts, (int)blockIdx.x % parts, nullptr);
+        fw_minplus_quad(D, ld, kb, nt, kend, flags, (int)blockIdx.x / parts, (int)blockIdx.x % parts, nullptr);
 }
 
 // Batched panel / remainder launch: prefix[a] is the first tile of active job
@@ -462,12 +478,14 @@ __global__ void __launch_bounds__(256) k_fw_minplus_b(const FwJob *__restrict__ 
             hi = mid - 1;
     }
     const FwJob j = jobs[active[lo]];
+    const int kend = fw_round_kend(j.n, kb);
     if constexpr (SPLIT == 1)
-        fw_minplus_tile<PANEL>(j.D, j.ld, kb, j.nt, flags + j.flag_off, b - prefix[lo], stats);
+        fw_minplus_tile<PANEL>(j.D, j.ld, kb, j.nt, kend, flags + j.flag_off, b - prefix[lo], stats);
     else if constexpr (PANEL)
-        fw_panel_half(j.D, j.ld, kb, flags + j.flag_off, b - prefix[lo], (int)blockIdx.x % parts, stats);
+        fw_panel_half(j.D, j.ld, kb, kend, flags + j.flag_off, b - prefix[lo], (int)blockIdx.x % parts, stats);
     else
-        fw_minplus_quad(j.D, j.ld, kb, j.nt, flags + j.flag_off, b - prefix[lo], (int)blockIdx.x % parts, stats);
+        fw_minplus_quad(j.D, j.ld, kb, j.nt, kend, flags + j.flag_off, b - prefix[lo], (int)blockIdx.x % parts,
+                        stats);
 }
 
 // ------------------------------------------------------------------ tuning ----
@@ -494,22 +512,22 @@ int32_t fw_solve_impl(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld) {
     for (int kb = 0; kb < nt; ++kb) {
         hipEvent_t e1, e2, e3;
         ktimer_begin(ctx, "k_fw_pivot", &e1);
-        k_fw_pivot<<<1, 1024, 0, ctx->stream>>>(d_D, ld, kb, flags, nt);
+        k_fw_pivot<<<1, 1024, 0, ctx->stream>>>(d_D, ld, kb, flags, nt, n);
         LG_LAUNCHED("k_fw_pivot");
         ktimer_end(ctx, "k_fw_pivot", e1);
         if (nt == 1) break;
         ktimer_begin(ctx, "k_fw_panel", &e2);
         if (nt - 1 < tc.split_below)
-            k_fw_minplus<true, 2><<<2 * (nt - 1), 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
+            k_fw_minplus<true, 2><<<2 * (nt - 1), 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags, n);
         else
-            k_fw_minplus<true, 1><<<nt - 1, 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
+            k_fw_minplus<true, 1><<<nt - 1, 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags, n);
         LG_LAUNCHED("k_fw_panel");
         ktimer_end(ctx, "k_fw_panel", e2);
         ktimer_begin(ctx, "k_fw_minplus", &e3);
         if (P < tc.split_below && 4 * P < (1LL << 31))
-            k_fw_minplus<false, 2><<<(unsigned)(4 * P), 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
+            k_fw_minplus<false, 2><<<(unsigned)(4 * P), 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags, n);
         else
-            k_fw_minplus<false, 1><<<(unsigned)P, 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags);
+            k_fw_minplus<false, 1><<<(unsigned)P, 256, 0, ctx->stream>>>(d_D, ld, kb, nt, flags, n);
         LG_LAUNCHED("k_fw_minplus");
         ktimer_end(ctx, "k_fw_minplus", e3);
     }
@@ -538,7 +556,7 @@ static int32_t fw_batched_run(lg_ctx *ctx, const std::vector<float *> &Ds, const
     for (int g = 0; g < count; ++g) {
         if (ns[g] <= 1) continue;
         const int ld = fw_ld(ns[g]);
-        jobs.push_back(FwJob{Ds[g], ld, ld / TB, flag_total, 0});
+        jobs.push_back(FwJob{Ds[g], ld, ld / TB, flag_total, ns[g]});
         flag_total += ld / TB;
         max_nt = std::max(max_nt, ld / TB);
     }
@@ -725,7 +743,7 @@ static int32_t fw_autotune_run(lg_ctx *ctx, lg_fw_tuning *t) {
     for (int k = 0; k < npts && rc == LG_OK; ++k) {
         const int nt = nts[k];
         const int T = (nt - 1) * nt / 2;
-        FwJob job{D, ld, nt, 0, 0};
+        FwJob job{D, ld, nt, 0, nt * TB};
         int ap[2] = {0, 0};
         if (hipMemcpyAsync(meta, &job, sizeof(job), hipMemcpyHostToDevice, st) != hipSuccess ||
             hipMemcpyAsync(meta + 32, ap, sizeof(ap), hipMemcpyHostToDevice, st) != hipSuccess) {
