@@ -1654,15 +1654,25 @@ static int32_t zero_ints(lg_ctx *ctx, std::initializer_list<std::pair<int *, siz
 // thread writes its run.  One dispatch instead of a device-wide scan's two.
 constexpr size_t kSmallScan = 65536;
 
+constexpr int kScanStage = 12288;  // inputs up to this size are staged through LDS
+
 __global__ void __launch_bounds__(1024) k_scan_small(const int *__restrict__ in, int *__restrict__ out, int n) {
     // thread t scans its run of consecutive elements; the run totals are
-    // scanned within each wave by shuffles and across the 16 waves in LDS
+    // scanned within each wave by shuffles and across the 16 waves in LDS.
+    // Inputs of at most kScanStage elements move through LDS with coalesced
+    // loads and stores (the runs are strided across the lanes).
+    __shared__ int buf[kScanStage];
     __shared__ int wsum[16];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const bool staged = n <= kScanStage;
+    if (staged) {
+        for (int i = t; i < n; i += 1024) buf[i] = in[i];
+        __syncthreads();
+    }
     const int per = (n + 1023) / 1024;
     const int b = t * per, e = min(b + per, n);
     int sum = 0;
-    for (int i = b; i < e; ++i) sum += in[i];
+    for (int i = b; i < e; ++i) sum += staged ? buf[i] : in[i];
     int v = sum;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
@@ -1682,10 +1692,20 @@ __global__ void __launch_bounds__(1024) k_scan_small(const int *__restrict__ in,
     }
     __syncthreads();
     int run = v - sum + (w > 0 ? wsum[w - 1] : 0);
-    for (int i = b; i < e; ++i) {
-        const int x = in[i];
-        out[i] = run;
-        run += x;
+    if (staged) {
+        for (int i = b; i < e; ++i) {
+            const int x = buf[i];
+            buf[i] = run;
+            run += x;
+        }
+        __syncthreads();
+        for (int i = t; i < n; i += 1024) out[i] = buf[i];
+    } else {
+        for (int i = b; i < e; ++i) {
+            const int x = in[i];
+            out[i] = run;
+            run += x;
+        }
     }
 }
 
