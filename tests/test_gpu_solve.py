@@ -116,12 +116,14 @@ def test_edge_march_parity(lga, ctx, oracle, c1):
         b.free()
 
 
-def test_edge_march_dense_dropouts(lga, ctx, oracle, c1):
+# an arbitrary step and a power of two
+@pytest.mark.parametrize("step", [0.7, 0.5])
+def test_edge_march_dense_dropouts(lga, ctx, oracle, c1, step):
     grid, dense, _ = c1
     pts = oracle.sample_points(dense, 5, 0, 512)
     edges = oracle.build_edges(pts, 20.0, 0)
     sh = lga.default_shader(absorption=0.5, scattering=2.0, scatter_density_scale=1.0)
-    p = lga.march_params(cutoff=0.01, shadow_step=0.7)
+    p = lga.march_params(cutoff=0.01, shadow_step=step)
     pb, ebuf = ctx.upload(pts), ctx.upload(edges)
     wb = ctx.alloc(len(edges) * 4)
     lga.march_edges(ctx, grid, sh, p, pb, ebuf, len(edges), w_buf=wb)
