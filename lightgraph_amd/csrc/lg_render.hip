@@ -747,7 +747,7 @@ __global__ void __launch_bounds__(256) k_render_aa(RenderConsts c, uint32_t pix0
 // kRenderPointBytes of shading-point data (a single block may exceed the latter).
 constexpr uint32_t kRenderBlockPixels = 64;
 constexpr uint64_t kRenderRayCap = 1u << 20;
-constexpr uint64_t kRenderPointBytes = 1ull << 30;
+constexpr uint64_t kRenderPointBytes = 2ull << 30;
 
 inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
