@@ -10,8 +10,8 @@ light.  All three runs share the points sampled on the FP32 volume, so the graph
 is the same and only the brick precision differs.
 
 Default size: 128^3, N = 2048, 128 x 64 map, 80 x 60 image.  LG_PRECISION_FULL=1
-runs the full configuration (1024^3, N = 12288, 1024 x 512 map, 160 x 90 image)
-and writes the table to gpurun_out/precision.json."""
+runs the full configuration (1024^3, N = 12288, 1024 x 512 map, 160 x 90 image);
+LG_PRECISION_JSON=<path> also writes the table there."""
 import json
 import os
 
@@ -154,9 +154,10 @@ def test_precision_errors_vs_fp64_oracle(lga, ctx, oracle, setup):
            "note": "(max, 99.9th percentile, mean) relative error (floor 1e-6 of the reference maximum), "
                    "largest error / reference maximum"}
     print(json.dumps(out))
-    if FULL:
-        os.makedirs("gpurun_out", exist_ok=True)
-        with open(os.path.join("gpurun_out", "precision.json"), "w") as f:
+    path = os.environ.get("LG_PRECISION_JSON")
+    if path:
+        os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+        with open(path, "w") as f:
             json.dump(out, f, indent=1)
     assert ref["pix"][0][..., 3].max() > 0.3 and ref["pix"][3][..., :3].max() > 0
     for name, bars in BARS.items():

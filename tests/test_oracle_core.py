@@ -1,6 +1,6 @@
 """Pins of the CPU oracle against known answers, closed forms and brute force.
 
-The reference has no tests or fixtures (SURVEY: none exist), so every check here
+The reference ships no tests or fixtures, so every check here
 comes from a published KAT, a library result (SciPy), a closed form or an
 exhaustive enumeration on a tiny case.  CPU only.
 """
