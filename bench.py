@@ -16,7 +16,7 @@ Beside the headline, the line carries:
     largest difference between a graph's batched and plain radiance (0 when
     the batch reproduces every graph exactly).
 
-Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
+Prints one JSON line.  --gpus above 1 is refused: the benchmark runs on one GPU.
 """
 import argparse
 import json
@@ -538,7 +538,7 @@ def main():
                          "(the known-size kernel that traffic counters are scaled against)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "the benchmark runs on one GPU; --gpus must be 1", "n_gpus": args.gpus}))
         return 2
 
     import lightgraph_amd as lga

@@ -41,7 +41,7 @@ enum {
     LG_ERR_NOMEM = -2,       /* device allocation failed                      */
     LG_ERR_HIP = -3,         /* HIP runtime error                             */
     LG_ERR_BUDGET = -4,      /* request exceeds the context's HBM budget      */
-    LG_ERR_UNSUPPORTED = -5, /* valid request this build does not support     */
+    LG_ERR_UNSUPPORTED = -5, /* valid request this library does not support   */
     LG_ERR_OVERFLOW = -6     /* an output capacity given by the caller is too small */
 };
 

@@ -178,7 +178,7 @@ class Shader {
 };
 
 // SceneSettings (SceneSettings.h:21-63): the same fields and defaults; lights are
-// C-ABI light records.  The last block holds this build's additions: graph size
+// C-ABI light records.  The last block holds this library's additions: graph size
 // and radius (the reference derives them from its dart thrower) and the scene.
 struct SceneSettings {
     std::string camera = "perspective";

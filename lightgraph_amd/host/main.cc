@@ -20,7 +20,7 @@ namespace {
 
 void printHelp() {
     const char *lines[][3] = {
-        {"-vdbIn", "Input vdb file to read and render (not read by this build: use -scene)", "string"},
+        {"-vdbIn", "Input vdb file to read and render (not read by this library: use -scene)", "string"},
         {"-exrOut", "Output exr file to write to disk", "string"},
         {"-camPos", "Camera position", "float,float,float"},
         {"-camRot", "Camera rotation", "float,float,float"},
@@ -155,7 +155,7 @@ int main(int argc, char *argv[]) {
         }
         if (scene.empty()) {
             if (!vdbFilename.empty())
-                cout << "-vdbIn: this build reads no vdb files; use -scene sphere|blobs|plume" << endl;
+                cout << "-vdbIn: this library reads no vdb files; use -scene sphere|blobs|plume" << endl;
             else
                 cout << "no input: give -scene sphere|blobs|plume" << endl;
             return 1;
