@@ -143,7 +143,9 @@ template <typename T, bool kBufferTaps = true>
 __device__ __forceinline__ float trilinear(const GridView &g, double x, double y, double z) {
     const double fx = floor(x), fy = floor(y), fz = floor(z);
     const int ix = (int)fx, iy = (int)fy, iz = (int)fz;
-    const double u = x - (double)ix, v = y - (double)iy, w = z - (double)iz;
+    // x - floor(x) is x - double(int(floor(x))) for |x| < 2^31, without the
+    // conversions back
+    const double u = x - fx, v = y - fy, w = z - fz;
     // Out-of-range corners read the zero border: clamping keeps both taps of an
     // axis inside the border bricks, where every value is 0.
     const int cx = clampi(ix, -8, 8 * g.nbx + 6);
