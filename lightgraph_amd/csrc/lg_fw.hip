@@ -30,6 +30,8 @@ namespace lg {
 
 constexpr int TB = FW_TILE;  // 128
 constexpr int KC = 32;       // pivots staged in LDS per step
+// Panel launches with fewer half-tile workgroups than this run in column quarters.
+constexpr int kQuarterPanelBelow = 256;
 
 // Closure of the symmetric pivot tile, four pivots per round trip through LDS.
 // Thread t holds the 4 x 4 block (rows 4*(t/32).., cols 4*(t%32)..) in registers.
@@ -366,6 +368,90 @@ __device__ __forceinline__ void fw_panel_half(float *__restrict__ D, int ld, int
     if (tid == 0 && finite) flags[Jb] = 1;
 }
 
+// fw_panel_half over a 32-column quarter (q = 0..3) of the panel tile, for
+// panel launches too small to fill the GPU: the columns of a panel tile are
+// independent (column j reads only column j of the tile and the pivot tile),
+// and each output folds the pivots in the same pairs and order as
+// fw_minplus_tile, so the result is bit-identical.  256 threads, 4 x 4 outputs
+// each.
+__device__ __forceinline__ void fw_panel_quarter(float *__restrict__ D, int ld, int kb, int kend,
+                                                 int *__restrict__ flags, int tile, int q,
+                                                 unsigned long long *__restrict__ stats) {
+    const int Ib = kb, Jb = tile < kb ? tile : tile + 1;
+    __shared__ float Xs[KC][TB];
+    __shared__ float Ys[KC][32];
+    const int tid = threadIdx.x;
+    const int ty = tid >> 3, tx = tid & 7;
+    const int c0 = 32 * q;
+    float acc[4][4];
+    float *Cb = D + (long long)Ib * TB * ld + (long long)Jb * TB + c0;
+    int finite = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const float4 v = *reinterpret_cast<const float4 *>(Cb + (long long)(ty * 4 + r) * ld + tx * 4);
+        acc[r][0] = v.x;
+        acc[r][1] = v.y;
+        acc[r][2] = v.z;
+        acc[r][3] = v.w;
+        finite |= (v.x < INFINITY) | (v.y < INFINITY) | (v.z < INFINITY) | (v.w < INFINITY);
+    }
+    if (!__syncthreads_or(finite)) return;  // all +INF: nothing changes
+    if (stats && q == 0 && threadIdx.x == 0) stat_add(stats, ST_FW_PANEL_TILES, 1ull);
+    const float *Xg = D + (long long)kb * TB * ld + (long long)Ib * TB;
+    const float *Yg = D + (long long)kb * TB * ld + (long long)Jb * TB + c0;
+    for (int kc = 0; kc < kend; kc += KC) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int idx = tid + u * 256;
+            const int kk = idx >> 5, c4 = idx & 31;
+            *reinterpret_cast<float4 *>(&Xs[kk][c4 * 4]) =
+                *reinterpret_cast<const float4 *>(Xg + (long long)(kc + kk) * ld + c4 * 4);
+        }
+        {
+            const int kk = tid >> 3, c4 = tid & 7;
+            *reinterpret_cast<float4 *>(&Ys[kk][c4 * 4]) =
+                *reinterpret_cast<const float4 *>(Yg + (long long)(kc + kk) * ld + c4 * 4);
+        }
+        __syncthreads();
+#pragma unroll 1
+        for (int kk = 0; kk < KC; kk += 2) {
+            const float4 xa = *reinterpret_cast<const float4 *>(&Xs[kk][ty * 4]);
+            const float4 yb = *reinterpret_cast<const float4 *>(&Ys[kk][tx * 4]);
+            const float4 xc = *reinterpret_cast<const float4 *>(&Xs[kk + 1][ty * 4]);
+            const float4 yd = *reinterpret_cast<const float4 *>(&Ys[kk + 1][tx * 4]);
+            const float a[4] = {xa.x, xa.y, xa.z, xa.w};
+            const float b[4] = {yb.x, yb.y, yb.z, yb.w};
+            const float c[4] = {xc.x, xc.y, xc.z, xc.w};
+            const float d[4] = {yd.x, yd.y, yd.z, yd.w};
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc[r][t] = fminf(acc[r][t], fminf(a[r] + b[t], c[r] + d[t]));
+        }
+    }
+    // every global read of this workgroup happened before the last barrier, so
+    // the in-place writes below cannot feed its own loads
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+        *reinterpret_cast<float4 *>(Cb + (long long)(ty * 4 + r) * ld + tx * 4) =
+            make_float4(acc[r][0], acc[r][1], acc[r][2], acc[r][3]);
+    // transpose into the column panel tile (Jb, kb)
+    float *Tb = D + (long long)Jb * TB * ld + (long long)Ib * TB;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const long long col = c0 + tx * 4 + t;
+        *reinterpret_cast<float4 *>(Tb + col * ld + ty * 4) = make_float4(acc[0][t], acc[1][t], acc[2][t], acc[3][t]);
+    }
+    finite = 0;
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) finite |= (acc[r][t] < INFINITY) ? 1 : 0;
+    finite = __syncthreads_or(finite);
+    if (tid == 0 && finite) flags[Jb] = 1;
+}
+
 // The remainder update for one 64 x 64 quadrant q (rows 64 (q >> 1)..,
 // columns 64 (q & 1).. of the output tile), 4 x 4 outputs per thread: four
 // workgroups share a tile when a launch holds too few tiles to fill the GPU.
@@ -445,7 +531,7 @@ __device__ __forceinline__ void fw_minplus_quad(float *__restrict__ D, int ld, i
 // quadrants at every launch size.)
 template <bool PANEL, int SPLIT>
 constexpr int fw_parts() {
-    return SPLIT == 1 ? 1 : (PANEL ? 2 : 4);
+    return SPLIT == 1 ? 1 : (PANEL ? (SPLIT == 4 ? 4 : 2) : 4);
 }
 
 template <bool PANEL, int SPLIT>
@@ -481,6 +567,8 @@ __global__ void __launch_bounds__(256) k_fw_minplus_b(const FwJob *__restrict__ 
     const int kend = fw_round_kend(j.n, kb);
     if constexpr (SPLIT == 1)
         fw_minplus_tile<PANEL>(j.D, j.ld, kb, j.nt, kend, flags + j.flag_off, b - prefix[lo], stats);
+    else if constexpr (PANEL && SPLIT == 4)
+        fw_panel_quarter(j.D, j.ld, kb, kend, flags + j.flag_off, b - prefix[lo], (int)blockIdx.x % parts, stats);
     else if constexpr (PANEL)
         fw_panel_half(j.D, j.ld, kb, kend, flags + j.flag_off, b - prefix[lo], (int)blockIdx.x % parts, stats);
     else
@@ -611,7 +699,11 @@ static int32_t fw_batched_run(lg_ctx *ctx, const std::vector<float *> &Ds, const
             if (timers) ktimer_end(ctx, "k_fw_pivot", e1);
             if (panel_blocks[kb] == 0) continue;
             if (timers) ktimer_begin(ctx, "k_fw_panel", &e2);
-            if (panel_blocks[kb] < tc.split_below && panel_blocks[kb] < (1 << 29))
+            if (2 * panel_blocks[kb] < kQuarterPanelBelow)
+                // too few halves to fill the GPU: column quarters
+                k_fw_minplus_b<true, 4><<<4 * panel_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + n, n, kb,
+                                                                                     d_flags, stats);
+            else if (panel_blocks[kb] < tc.split_below && panel_blocks[kb] < (1 << 29))
                 k_fw_minplus_b<true, 2><<<2 * panel_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + n, n, kb,
                                                                                      d_flags, stats);
             else
