@@ -990,6 +990,21 @@ Scene makeScene(const Grid *den, const Grid *tmp, const lg_shader *sh, const lg_
 }
 }  // namespace
 
+// ColorRamp::getColor (Shader.h:63-92) of the shader's temperature ramp at x.
+extern "C" void lgo_ramp_color(const lg_shader *sh, double x, double out[3]) {
+    const Vec3 c = Shading{sh}.emissionColor(x);
+    for (int a = 0; a < 3; ++a) out[a] = c[a];
+}
+
+// Distribution1D::SampleContinuous (Sampling.h:104-120) over a CDF of count+1
+// entries: the continuous sample in [0, 1) and the segment it fell in.
+extern "C" double lgo_sample1d(const double *cdf, int32_t count, double u, int32_t *off) {
+    int o = 0;
+    const double x = sample1d(cdf, count, u, &o);
+    if (off) *off = o;
+    return x;
+}
+
 extern "C" void lgo_env_cdf(int32_t w, int32_t h, const float *rgb, double *cond, double *marg) {
     lg_light l{};
     l.type = LG_LIGHT_ENV;
@@ -1134,11 +1149,14 @@ void lgo_splat(const float *pts, const float *rad, int64_t count, double radius,
 // steps from (pix, q, 5 | k << 8 | s << 12; seed) and its env-map samples from
 // (pix, q, 3 | k << 8 | s << 12; env_seed), light samples in order.  The
 // reference draws all of these from one engine per TBB range (Renderer.h:518).
-void lgo_render(const lg_grid_desc *d, const float *dense, const float *temp_dense, const float *scatter_rgb,
-                const int32_t sn[3], const int32_t si0[3], double scv, const lg_camera *cam, const lg_shader *sh,
-                const lg_light *lights, int32_t nl, const float *const *env_rgb, const int32_t *env_w,
-                const int32_t *env_h, const lg_march_params *mp, int32_t samples, uint64_t seed, uint64_t env_seed,
-                int32_t threads, float *films) {
+// lgo_render_rows renders image rows [row0, row1) only (the pixels and streams
+// those rows have in the full image); films then hold 4 consecutive
+// width*(row1-row0) RGBA arrays.
+void lgo_render_rows(const lg_grid_desc *d, const float *dense, const float *temp_dense, const float *scatter_rgb,
+                     const int32_t sn[3], const int32_t si0[3], double scv, const lg_camera *cam, const lg_shader *sh,
+                     const lg_light *lights, int32_t nl, const float *const *env_rgb, const int32_t *env_w,
+                     const int32_t *env_h, const lg_march_params *mp, int32_t samples, uint64_t seed,
+                     uint64_t env_seed, int32_t row0, int32_t row1, int32_t threads, float *films) {
     Grid g(*d, dense);
     std::unique_ptr<Grid> tg;
     if (temp_dense) tg.reset(new Grid(*d, temp_dense));
@@ -1153,7 +1171,11 @@ void lgo_render(const lg_grid_desc *d, const float *dense, const float *temp_den
         coarse.rgb = scatter_rgb;
     }
     const Camera camera{cam};
-    const int W = cam->width, H = cam->height;
+    const int W = cam->width;
+    row0 = std::max(row0, 0);
+    row1 = std::min(row1, (int32_t)cam->height);
+    if (row1 <= row0) return;
+    const int H = row1 - row0;
     float *rgbF = films, *dirF = films + 4 * (size_t)W * H, *scaF = films + 8 * (size_t)W * H,
           *emiF = films + 12 * (size_t)W * H;
     const Vec3 One(1.0);
@@ -1257,7 +1279,7 @@ void lgo_render(const lg_grid_desc *d, const float *dense, const float *temp_den
                     emitLumiAA += emitLumi * ratioAA;
                     pTransAA += pTrans * ratioAA;
                 }
-                const size_t o = 4 * (size_t)pix;
+                const size_t o = 4 * ((size_t)(j - row0) * W + (size_t)i);
                 for (int c = 0; c < 3; ++c) {
                     rgbF[o + c] = (float)pLumiAA[c];
                     dirF[o + c] = (float)directLumiAA[c];
@@ -1269,16 +1291,25 @@ void lgo_render(const lg_grid_desc *d, const float *dense, const float *temp_den
             }
     };
     if (threads <= 1) {
-        rows(0, H);
+        rows(row0, row1);
     } else {
+        // rows dealt round-robin: the cost of a row depends on what it crosses
         std::vector<std::thread> pool;
-        const int per = (H + threads - 1) / threads;
-        for (int t = 0; t < threads; ++t) {
-            const int a = t * per, b = std::min(H, a + per);
-            if (a < b) pool.emplace_back(rows, a, b);
-        }
+        for (int t = 0; t < threads && t < H; ++t)
+            pool.emplace_back([&, t] {
+                for (int j = row0 + t; j < row1; j += threads) rows(j, j + 1);
+            });
         for (auto &th : pool) th.join();
     }
+}
+
+void lgo_render(const lg_grid_desc *d, const float *dense, const float *temp_dense, const float *scatter_rgb,
+                const int32_t sn[3], const int32_t si0[3], double scv, const lg_camera *cam, const lg_shader *sh,
+                const lg_light *lights, int32_t nl, const float *const *env_rgb, const int32_t *env_w,
+                const int32_t *env_h, const lg_march_params *mp, int32_t samples, uint64_t seed, uint64_t env_seed,
+                int32_t threads, float *films) {
+    lgo_render_rows(d, dense, temp_dense, scatter_rgb, sn, si0, scv, cam, sh, lights, nl, env_rgb, env_w, env_h, mp,
+                    samples, seed, env_seed, 0, cam->height, threads, films);
 }
 
 // computeVoxelSize restatement for the splat radius (see lg_splat_radius).

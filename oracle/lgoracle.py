@@ -50,6 +50,8 @@ def lib():
             "lgo_solve_points": (I32, [P, P, P, P, P, P, I32, P, P, P, U64, U32, P, I32, D, I32, P, I32, P, P, P]),
             "lgo_dart_throw": (I32, [P, P, U32, U32, U32, P, P, I32]),
             "lgo_env_cdf": (None, [I32, I32, P, P, P]),
+            "lgo_ramp_color": (None, [P, D, P]),
+            "lgo_sample1d": (D, [P, I32, D, P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -249,6 +251,20 @@ def env_cdf(rgb):
     return cond, marg
 
 
+def ramp_color(shader, x):
+    out = np.zeros(3, dtype=np.float64)
+    lib().lgo_ramp_color(C.byref(shader), float(x), _p(out))
+    return out
+
+
+def sample1d(cdf, u):
+    """(x in [0, 1), segment) of Distribution1D::SampleContinuous over cdf[0..count]."""
+    cdf = np.ascontiguousarray(cdf, dtype=np.float64)
+    off = C.c_int32(0)
+    x = lib().lgo_sample1d(_p(cdf), len(cdf) - 1, float(u), C.byref(off))
+    return x, off.value
+
+
 def _sig_render():
     L = lib()
     if not hasattr(L, "_render_sig"):
@@ -258,6 +274,8 @@ def _sig_render():
         L.lgo_splat.argtypes = [P, P, I64, D, P, P, D, P, P]
         L.lgo_render.restype = None
         L.lgo_render.argtypes = [P, P, P, P, P, P, D, P, P, P, I32, P, P, P, P, I32, U64, U64, I32, P]
+        L.lgo_render_rows.restype = None
+        L.lgo_render_rows.argtypes = [P, P, P, P, P, P, D, P, P, P, I32, P, P, P, P, I32, U64, U64, I32, I32, I32, P]
         L.lgo_point_voxel_size.restype = D
         L.lgo_point_voxel_size.argtypes = [P, I64]
         L._render_sig = True
@@ -291,8 +309,9 @@ def point_voxel_size(pts):
 
 
 def render(dense, camera, shader, lights, params, samples, seed=1, env_seed=4564446, temp_dense=None,
-           scatter=None, envs=None, threads=8, voxel_size=1.0, origin=(0, 0, 0)):
-    """scatter = (rgb[z,y,x,3], n_xyz, i0_xyz, cv) or None.  Returns films [4, H, W, 4]."""
+           scatter=None, envs=None, threads=8, voxel_size=1.0, origin=(0, 0, 0), rows=None):
+    """scatter = (rgb[z,y,x,3], n_xyz, i0_xyz, cv) or None.  Returns films [4, H, W, 4];
+    with rows=(j0, j1) only those image rows, films [4, j1 - j0, W, 4]."""
     dense = np.ascontiguousarray(dense, dtype=np.float32)
     tdense = np.ascontiguousarray(temp_dense, dtype=np.float32) if temp_dense is not None else None
     d = _desc(dense, voxel_size, origin)
@@ -304,8 +323,9 @@ def render(dense, camera, shader, lights, params, samples, seed=1, env_seed=4564
         sn = np.asarray(scatter[1], dtype=np.int32)
         si0 = np.asarray(scatter[2], dtype=np.int32)
         scv = float(scatter[3])
-    films = np.zeros((4, camera.height, camera.width, 4), dtype=np.float32)
-    _sig_render().lgo_render(C.byref(d), _p(dense), _p(tdense), _p(srgb), _p(sn), _p(si0), scv, C.byref(camera),
-                             C.byref(shader), arr, len(lights), rgb, w, h, C.byref(params), samples, seed, env_seed,
-                             threads, _p(films))
+    j0, j1 = (0, camera.height) if rows is None else (max(0, rows[0]), min(camera.height, rows[1]))
+    films = np.zeros((4, max(0, j1 - j0), camera.width, 4), dtype=np.float32)
+    _sig_render().lgo_render_rows(C.byref(d), _p(dense), _p(tdense), _p(srgb), _p(sn), _p(si0), scv,
+                                  C.byref(camera), C.byref(shader), arr, len(lights), rgb, w, h, C.byref(params),
+                                  samples, seed, env_seed, j0, j1, threads, _p(films))
     return films

@@ -9,9 +9,10 @@ and a 5000x sun disk, importance-sampled through the FP64 CDF) and a directional
 light.  All three runs share the points sampled on the FP32 volume, so the graph
 is the same and only the brick precision differs.
 
-Default size: 128^3, N = 2048, 128 x 64 map, 80 x 60 image.  LG_PRECISION_FULL=1
-runs the full configuration (1024^3, N = 12288, 1024 x 512 map, 160 x 90 image);
-LG_PRECISION_JSON=<path> also writes the table there."""
+Default size: the full configuration (1024^3, N = 12288, 1024 x 512 map,
+160 x 90 image).  LG_PRECISION_SMALL=1 runs a reduced one (128^3, N = 2048,
+128 x 64 map, 80 x 60 image); LG_PRECISION_JSON=<path> also writes the table
+there."""
 import json
 import os
 
@@ -22,7 +23,7 @@ from scipy.sparse.csgraph import dijkstra
 
 pytestmark = pytest.mark.gpu
 
-FULL = os.environ.get("LG_PRECISION_FULL") == "1"
+FULL = os.environ.get("LG_PRECISION_SMALL") != "1"
 SIZE, NPTS, ENV_W, IMG = (1024, 12288, 1024, (160, 90)) if FULL else (128, 2048, 128, (80, 60))
 RAMP = "0.0,0.33,0.66,1.0/0,0,0:1.0,0.076,0.0:1.0,0.322,0.0:1.0,0.45,0.15/0.35/1.5"
 

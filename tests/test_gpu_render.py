@@ -113,3 +113,64 @@ def test_scatter_topology_and_splat_parity(lga, ctx, oracle, scene):
     pb.free()
     rb.free()
     sg.destroy()
+
+
+def test_envmap_cdf_device_equals_oracle(lga, ctx, oracle):
+    """The FP64 importance-sampling CDF built on the device (Light.h:161-176,
+    Sampling.h:76-167) for a 1024 x 512 HDR map, sky gradient plus a 5000x sun
+    disk, equals the oracle's (same sequential sums)."""
+    h, w = 512, 1024
+    y = np.linspace(0, 1, h, dtype=np.float32)[:, None, None]
+    sky = np.broadcast_to(np.concatenate([0.25 + 0.5 * y, 0.35 + 0.45 * y, 0.8 - 0.2 * y], -1), (h, w, 3)).copy()
+    yy, xx = np.mgrid[0:h, 0:w]
+    sky[(xx - 0.7 * w) ** 2 + (yy - 0.22 * h) ** 2 <= (0.012 * w) ** 2 + 1] *= 5000.0
+    env = lga.EnvMap(ctx, sky)
+    cond, marg, rowint, total = env.read_cdf()
+    env.destroy()
+    want_cond, want_marg = oracle.env_cdf(sky)
+    np.testing.assert_allclose(cond, want_cond, rtol=0, atol=1e-14)
+    np.testing.assert_allclose(marg, want_marg, rtol=0, atol=1e-14)
+    # the sun's rows hold most of the marginal mass
+    sun_rows = slice(int(0.22 * h) - 13, int(0.22 * h) + 14)
+    assert marg[sun_rows.stop] - marg[sun_rows.start] > 0.5
+
+
+def test_ramp_knots_on_device(lga, ctx, oracle):
+    """Emission only (no lights): the vertex light of points in a constant
+    temperature grid is ramp(T) x emissionScale, at the knot temperatures of
+    scenes/explo.sh:1 and between them."""
+    ramp = "0.0,0.33,0.66,1.0/0,0,0:1.0,0.076,0.0:1.0,0.322,0.0:1.0,0.45,0.15/0.35/1.5"
+    sh = lga.default_shader(absorption=0.1, scattering=1.0, emission_scale=3.0, ramp=ramp)
+    p = lga.march_params()
+    dens = lga.Grid.from_dense(ctx, np.full((16, 16, 16), 0.5, np.float32))
+    pts = np.array([[8.0, 8.0, 8.0], [5.5, 9.25, 3.0]], np.float32)
+    pb, ob = ctx.upload(pts), ctx.alloc(len(pts) * 12)
+    for T in (0.0, 0.35, 0.35 + 0.33 * 1.15, 0.35 + 0.495 * 1.15, 0.35 + 0.66 * 1.15, 1.5, 3.0):
+        tg = lga.Grid.from_dense(ctx, np.full((16, 16, 16), T, np.float32))
+        lga.vertex_light(ctx, dens, tg, sh, [], p, 4564446, 0, pb, len(pts), ob)
+        got = ob.read(np.float32, (len(pts), 3))
+        want = oracle.ramp_color(sh, float(np.float32(T))) * 3.0
+        for row in got:
+            np.testing.assert_allclose(row, want, rtol=2e-7, atol=1e-7)
+        tg.destroy()
+    pb.free()
+    ob.free()
+    dens.destroy()
+
+
+@pytest.mark.parametrize("voxel", [(37, 20, 9), (8, 8, 8)])
+def test_scatter_topology_single_voxel_on_device(lga, ctx, voxel):
+    """One active voxel v: the coarse voxel floor((v + 2) / 4) of the factor-4
+    grid and its 26 neighbours, nothing else."""
+    a = np.zeros((64, 64, 64), np.float32)
+    a[voxel[2], voxel[1], voxel[0]] = 0.7
+    g = lga.Grid.from_dense(ctx, a)
+    sg = lga.ScatterGrid(ctx, g, 4)
+    _, act = sg.read()
+    i0 = tuple(sg.sg.i0)
+    want = np.zeros_like(act)
+    c = [(voxel[k] + 2) // 4 - i0[k] for k in range(3)]
+    want[max(c[2] - 1, 0):c[2] + 2, max(c[1] - 1, 0):c[1] + 2, max(c[0] - 1, 0):c[0] + 2] = 1
+    np.testing.assert_array_equal(act, want)
+    sg.destroy()
+    g.destroy()

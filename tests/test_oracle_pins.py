@@ -1,0 +1,215 @@
+"""Closed-form pins of the oracle functions that no reference test or fixture
+covers: the environment-map sampling CDF and its inverse (Sampling.h:76-167,
+Light.h:161-176), the temperature colour ramp at the knots of
+scenes/explo.sh:1 (Shader.h:56-92), the point voxel-size heuristic on lattices
+whose answer is exact (LightGraph.h:713-714) and the scatter-grid topology of
+single-voxel and single-brick grids (ScatterGrid.h:105-144).
+
+Each expected value below is derived by hand from the definition, not by
+re-running the oracle's arithmetic, so that a dropped term, an off-by-one index
+or a transposed axis in the oracle fails here."""
+import math
+
+import numpy as np
+import pytest
+
+import lightgraph_amd as lga
+
+EXPLO_RAMP = "0.0,0.33,0.66,1.0/0,0,0:1.0,0.076,0.0:1.0,0.322,0.0:1.0,0.45,0.15/0.35/1.5"
+
+
+def _sine_prefix(m, a):
+    """sum_{v=0}^{m-1} sin(v a) in closed form."""
+    if m <= 1:
+        return 0.0
+    return math.sin(m * a / 2.0) * math.sin((m - 1) * a / 2.0) / math.sin(a / 2.0)
+
+
+# ------------------------------------------------------------- env CDF -----
+def test_env_cdf_constant_map(oracle):
+    """A constant map: every conditional CDF is i / w; the marginal follows
+    the sin(theta) row weights, sum_{v<i} sin(pi v/(h-1)) / sum_all."""
+    h, w = 32, 64
+    rgb = np.full((h, w, 3), 2.0, np.float32)
+    cond, marg = oracle.env_cdf(rgb)
+    want_row = np.arange(w + 1) / w
+    for v in range(h):
+        np.testing.assert_allclose(cond[v], want_row, rtol=0, atol=1e-12)
+    a = math.pi / (h - 1)
+    total = _sine_prefix(h, a)
+    want = np.array([_sine_prefix(i, a) / total for i in range(h + 1)])
+    # the row weights are single-precision sines (Light.h:164-172)
+    np.testing.assert_allclose(marg, want, rtol=0, atol=1e-6)
+    assert marg[0] == 0.0 and marg[h] == 1.0
+
+
+def test_env_cdf_separable_map(oracle):
+    """rgb = f(u) g(v) with f(u) = 1 + u and g(v) = v + 1: the conditional CDF
+    of every row with weight is (i + i(i-1)/2) / (w + w(w-1)/2), independent of
+    the row; the marginal is the cumulative g(v) sin(theta_v)."""
+    h, w = 24, 40
+    f = 1.0 + np.arange(w)
+    g = 1.0 + np.arange(h)
+    rgb = np.repeat((g[:, None] * f[None, :])[..., None], 3, axis=2).astype(np.float32)
+    cond, marg = oracle.env_cdf(rgb)
+    i = np.arange(w + 1, dtype=np.float64)
+    want_row = (i + i * (i - 1) / 2) / (w + w * (w - 1) / 2)
+    for v in range(1, h - 1):
+        np.testing.assert_allclose(cond[v], want_row, rtol=0, atol=1e-6)
+    # row 0 has sin(0) = 0: no weight, the uniform CDF (Sampling.h:88-90)
+    np.testing.assert_allclose(cond[0], i / w, rtol=0, atol=1e-15)
+    wts = g * np.sin(np.pi * np.arange(h) / (h - 1))
+    want = np.concatenate([[0.0], np.cumsum(wts)]) / wts.sum()
+    np.testing.assert_allclose(marg, want, rtol=0, atol=1e-6)
+
+
+def test_env_cdf_transposed_map_differs(oracle):
+    """Rows are latitude (sin weighted) and columns longitude: a map that is
+    bright in one column keeps a uniform marginal shape over rows, one bright
+    row moves the marginal."""
+    h, w = 16, 32
+    col = np.full((h, w, 3), 0.1, np.float32)
+    col[:, 5] = 10.0
+    row = np.full((h, w, 3), 0.1, np.float32)
+    row[5, :] = 10.0
+    c1, m1 = oracle.env_cdf(col)
+    c2, m2 = oracle.env_cdf(row)
+    # bright column: every weighted row jumps at column 5 -> 6
+    assert np.all(np.diff(c1[1:h - 1, 5:7], axis=1)[:, 0] > 0.5)
+    # bright row: the marginal jumps at row 5 -> 6
+    assert m2[6] - m2[5] > 0.5 and m1[6] - m1[5] < 0.2
+
+
+def test_sample1d_inverse_closed_form(oracle):
+    """Distribution1D::SampleContinuous inverts the piecewise-linear CDF: a
+    uniform CDF gives x = u; two levels a | b (n/2 segments each) give
+    x = u (a+b) / (2a) below the knee a / (a+b) and 0.5 + (u - a/(a+b)) (a+b) / (2b)
+    above it."""
+    n = 10
+    uni = np.arange(n + 1) / n
+    for u in (0.0, 0.05, 0.37, 0.5, 0.999):
+        x, off = oracle.sample1d(uni, u)
+        assert x == pytest.approx(u, abs=1e-15)
+        assert off == min(int(u * n), n - 1)
+    a, b = 3.0, 1.0
+    func = np.array([a] * (n // 2) + [b] * (n // 2))
+    cdf = np.concatenate([[0.0], np.cumsum(func)]) / func.sum()
+    knee = a / (a + b)
+    for u in np.linspace(0.0, 0.9999, 41):
+        want = u * (a + b) / (2 * a) if u < knee else 0.5 + (u - knee) * (a + b) / (2 * b)
+        x, off = oracle.sample1d(cdf, u)
+        assert x == pytest.approx(want, abs=1e-12), u
+        assert off == min(int(want * n), n - 1)
+
+
+# ------------------------------------------------------------------ ramp ----
+def test_ramp_knots_of_explo(oracle):
+    """scenes/explo.sh:1 ramp: knots at 0, .33, .66, 1 of the remapped
+    temperature (T - 0.35) / (1.5 - 0.35), colours black, (1, .076, 0),
+    (1, .322, 0), (1, .45, .15); clamped below 0.35 and above 1.5; linear
+    between knots."""
+    sh = lga.default_shader(ramp=EXPLO_RAMP)
+    pos = [0.0, 0.33, 0.66, 1.0]
+    col = np.array([[0, 0, 0], [1.0, 0.076, 0.0], [1.0, 0.322, 0.0], [1.0, 0.45, 0.15]])
+    t_in, t_out = 0.35, 1.5
+    for p, c in zip(pos, col):
+        np.testing.assert_allclose(oracle.ramp_color(sh, t_in + p * (t_out - t_in)), c, rtol=0, atol=1e-12)
+    np.testing.assert_array_equal(oracle.ramp_color(sh, 0.0), col[0])
+    np.testing.assert_array_equal(oracle.ramp_color(sh, -5.0), col[0])
+    np.testing.assert_array_equal(oracle.ramp_color(sh, 1500.0), col[3])
+    for k in range(3):
+        mid = t_in + 0.5 * (pos[k] + pos[k + 1]) * (t_out - t_in)
+        np.testing.assert_allclose(oracle.ramp_color(sh, mid), 0.5 * (col[k] + col[k + 1]), rtol=0, atol=1e-12)
+    # a quarter of the way from knot 2 to knot 3
+    q = t_in + (0.66 + 0.25 * 0.34) * (t_out - t_in)
+    np.testing.assert_allclose(oracle.ramp_color(sh, q), 0.75 * col[2] + 0.25 * col[3], rtol=0, atol=1e-12)
+
+
+def test_ramp_empty_and_single(oracle):
+    sh = lga.default_shader()
+    np.testing.assert_array_equal(oracle.ramp_color(sh, 0.9), [0.0, 0.0, 0.0])
+    one = lga.default_shader(ramp="0.5/0.2,0.4,0.6/0/1")
+    np.testing.assert_allclose(oracle.ramp_color(one, 0.0), [0.2, 0.4, 0.6], rtol=0, atol=0)
+    np.testing.assert_allclose(oracle.ramp_color(one, 7.0), [0.2, 0.4, 0.6], rtol=0, atol=0)
+
+
+# ------------------------------------------------------ point voxel size ----
+def _lattice(m, spacing):
+    g = np.arange(m, dtype=np.float32) * spacing
+    z, y, x = np.meshgrid(g, g, g, indexing="ij")
+    return np.stack([x.ravel(), y.ravel(), z.ravel()], 1).astype(np.float32)
+
+
+def test_point_voxel_size_lattices(oracle):
+    """Hand-derived answers of the volume / occupied-voxel heuristic with 8
+    points per voxel:
+      * 16^3 lattice, spacing 1: 4096 points, target 512, box 15^3 = 3375;
+        first size cbrt(3375/512) = 1.875; rounding to 1.875 puts coordinates
+        0..15 into 9 cells per axis, 729 cells of volume 4805 > 3375: stop at 1.875;
+      * 8^3 lattice, spacing 2: target 64, box 14^3 = 2744, cbrt(2744/64) = 3.5;
+        coordinates 0, 2, .., 14 fall into 5 cells per axis, 125 cells of
+        volume 5359 > 2744: stop at 3.5;
+      * 9 points on a line x = 0..8: the box is flat, its volume becomes 8^3
+        (longest side cubed), target 1, size 8; two cells, volume 1024 > 512: 8;
+      * one point: 0.1."""
+    assert oracle.point_voxel_size(_lattice(16, 1.0)) == 1.875
+    assert oracle.point_voxel_size(_lattice(8, 2.0)) == 3.5
+    line = np.zeros((9, 3), np.float32)
+    line[:, 0] = np.arange(9)
+    assert oracle.point_voxel_size(line) == 8.0
+    assert oracle.point_voxel_size(np.array([[3.0, 4.0, 5.0]], np.float32)) == np.float32(0.1)
+    # translation by a multiple of the answer changes nothing
+    assert oracle.point_voxel_size(_lattice(16, 1.0) + np.float32(30.0)) == 1.875
+
+
+def test_splat_radius_uses_voxel_size():
+    """lg_splat_radius (host) gives the same voxel size on the exact lattice and
+    the radius (3 k vs^3 / (32 pi))^(1/3) of ScatterGrid.h:71-73."""
+    r, vs = lga.splat_radius(_lattice(16, 1.0), 3)
+    assert vs == 1.875
+    assert r == pytest.approx((3.0 * 3 * 1.875 ** 3 / (32.0 * math.pi)) ** (1.0 / 3.0), rel=1e-12)
+
+
+# ------------------------------------------------------ scatter topology ----
+def _closed_form_topology(on_voxels, f, n, i0):
+    """Coarse voxel C (index units of the coarse grid) covers density voxels
+    [f C - f/2, f C + f/2) for even f (footprint centred on C f), so voxel v lies
+    in the footprint of C = floor((v + f/2) / f); the 26-neighbour dilation
+    then adds every C' with |C' - C| <= 1 per axis."""
+    out = np.zeros((n[2], n[1], n[0]), np.uint8)
+    for v in on_voxels:
+        c = [(v[a] + f // 2) // f for a in range(3)]
+        for dz in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dx in (-1, 0, 1):
+                    x, y, z = c[0] + dx - i0[0], c[1] + dy - i0[1], c[2] + dz - i0[2]
+                    if 0 <= x < n[0] and 0 <= y < n[1] and 0 <= z < n[2]:
+                        out[z, y, x] = 1
+    return out
+
+
+@pytest.mark.parametrize("voxel", [(37, 20, 9), (0, 0, 0), (63, 1, 30)])
+def test_scatter_topology_single_voxel(oracle, voxel):
+    dense = np.zeros((64, 64, 64), np.float32)
+    dense[voxel[2], voxel[1], voxel[0]] = 0.5
+    n, i0 = (20, 20, 20), (-2, -2, -2)
+    got = oracle.scatter_topology(dense, 4, n, i0, 4.0)
+    want = _closed_form_topology([voxel], 4, n, i0)
+    np.testing.assert_array_equal(got, want)
+    assert got.sum() == 27
+
+
+def test_scatter_topology_single_brick(oracle):
+    """Brick b (voxels 8b..8b+7 per axis) touches the footprints of coarse
+    voxels 2b, 2b+1, 2b+2; dilated: 2b-1 .. 2b+3, 5 per axis, 125 in all."""
+    dense = np.zeros((64, 64, 64), np.float32)
+    b = (1, 3, 5)
+    dense[8 * b[2]:8 * b[2] + 8, 8 * b[1]:8 * b[1] + 8, 8 * b[0]:8 * b[0] + 8] = 1.0
+    n, i0 = (20, 20, 20), (-2, -2, -2)
+    got = oracle.scatter_topology(dense, 4, n, i0, 4.0)
+    want = np.zeros_like(got)
+    sl = [slice(2 * b[a] - 1 - i0[a], 2 * b[a] + 4 - i0[a]) for a in range(3)]
+    want[sl[2], sl[1], sl[0]] = 1
+    np.testing.assert_array_equal(got, want)
+    assert got.sum() == 125
+    assert np.zeros_like(got).sum() == oracle.scatter_topology(np.zeros_like(dense), 4, n, i0, 4.0).sum()
