@@ -313,6 +313,20 @@ int32_t lg_fw_solve(lg_ctx *ctx, int32_t n, float *d_D, int32_t ld, int32_t *d_p
 int32_t lg_diffuse(lg_ctx *ctx, int32_t n, const float *d_D, int32_t ld,
                    const double sigma_t[3], const float *d_L, float *d_out);
 
+/* lg_diffuse without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments and
+ * results as lg_diffuse (bit for bit); the column sums and normalised light
+ * live in a fixed per-GPU scratch array inside the library, sized for
+ * n <= LG_DIFFUSE_NOALLOC_MAX_N (2.25 MiB, held by each GPU the library runs on
+ * until the process ends); larger n returns LG_ERR_INVALID.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams, other
+ * contexts, other threads) race and must be serialised by the caller.  Kernel
+ * timing does not cover this entry point. */
+#define LG_DIFFUSE_NOALLOC_MAX_N 65536
+int32_t lg_diffuse_noalloc(lg_ctx *ctx, int32_t n, const float *d_D, int32_t ld,
+                           const double sigma_t[3], const float *d_L, float *d_out);
+
 /* ------------------------------------------------------------ full solve ---- */
 /* Per-solve workspace for graphs (or batches, lg_solve_batch) of up to n_max
  * vertices in total, n_max <= 2^24.  flags: LG_SOLVER_DENSE_D allocates the dense

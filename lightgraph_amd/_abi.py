@@ -184,6 +184,7 @@ SIGNATURES = {
     "lg_fw_ld": (I32, [I32]),
     "lg_fw_solve": (I32, [P, I32, P, I32, P, P, P, I32]),
     "lg_diffuse": (I32, [P, I32, P, I32, DP, P, P]),
+    "lg_diffuse_noalloc": (I32, [P, I32, P, I32, DP, P, P]),
     "lg_solver_create": (I32, [P, I32, I32, I32, PP]),
     "lg_solver_destroy": (I32, [P, P]),
     "lg_solver_get": (I32, [P, C.POINTER(lg_solver_buffers)]),

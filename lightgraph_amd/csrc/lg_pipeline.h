@@ -431,7 +431,7 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
 // helpers from lg_solve.hip
 int32_t fw_init_impl(lg_ctx *ctx, int n, float *d_D, int ld);
 int32_t diffuse_impl(lg_ctx *ctx, int n, const float *d_D, int ld, const double sigma_t[3], const float *d_L,
-                     float *d_out, double *d_sums, float *d_Ln);
+                     float *d_out, double *d_sums, float *d_Ln, bool timed);
 SolveDesc make_desc(lg_grid *density, const lg_grid *temperature, int n, int voff, uint32_t solve);
 
 }  // namespace lg

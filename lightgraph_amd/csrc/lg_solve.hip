@@ -4,11 +4,21 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "lg_internal.h"
 #include "lg_lights.h"
 #include "lg_pipeline.h"
+
+// Scratch of lg_diffuse_noalloc: the column sums (3 doubles per vertex) then the
+// normalised light (3 floats per vertex) of up to LG_DIFFUSE_NOALLOC_MAX_N
+// vertices, one copy per GPU for the life of the process.  Host code never
+// passes `scratch` itself to a kernel: its device address comes from
+// hipGetSymbolAddress, once per device.
+static constexpr size_t kNoAllocSums = (size_t)LG_DIFFUSE_NOALLOC_MAX_N * 3 * sizeof(double);
+static constexpr size_t kNoAllocBytes = kNoAllocSums + (size_t)LG_DIFFUSE_NOALLOC_MAX_N * 3 * sizeof(float);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 
 namespace lg {
 
@@ -281,18 +291,45 @@ __global__ void k_diffuse_out(const float *__restrict__ D, int n, int ld, float 
 }
 
 int32_t diffuse_impl(lg_ctx *ctx, int n, const float *d_D, int ld, const double sigma_t[3], const float *d_L,
-                     float *d_out, double *d_sums, float *d_Ln) {
+                     float *d_out, double *d_sums, float *d_Ln, bool timed) {
     if (n <= 0) return LG_OK;
     const float e0 = (float)(-sigma_t[0]), e1 = (float)(-sigma_t[1]), e2 = (float)(-sigma_t[2]);
-    hipEvent_t ev;
-    ktimer_begin(ctx, "k_diffuse", &ev);
+    hipEvent_t ev = nullptr;
+    if (timed) ktimer_begin(ctx, "k_diffuse", &ev);
     k_rowsum<<<n, 256, 0, ctx->stream>>>(d_D, n, ld, e0, e1, e2, d_sums);
     LG_LAUNCHED("k_rowsum");
     k_lnorm<<<(n + 255) / 256, 256, 0, ctx->stream>>>(d_L, d_sums, n, d_Ln);
     LG_LAUNCHED("k_lnorm");
     k_diffuse_out<<<n, 256, 0, ctx->stream>>>(d_D, n, ld, e0, e1, e2, d_Ln, d_out);
     LG_LAUNCHED("k_diffuse_out");
-    ktimer_end(ctx, "k_diffuse", ev);
+    if (timed) ktimer_end(ctx, "k_diffuse", ev);
+    return LG_OK;
+}
+
+static constexpr int kNoAllocDevices = 64;
+static std::mutex g_noalloc_mu;
+static void *g_noalloc_ptr[kNoAllocDevices];
+
+// Device address of `scratch` on the context's device (looked up on the first
+// call on that device, with the device current).
+static int32_t noalloc_scratch(lg_ctx *ctx, unsigned char **out) {
+    LG_CHECK_ARG(ctx->device >= 0 && ctx->device < kNoAllocDevices, "device %d outside the scratch table",
+                 ctx->device);
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[ctx->device]) {
+        int prev = 0;
+        LG_HIP(hipGetDevice(&prev));
+        LG_HIP(hipSetDevice(ctx->device));
+        void *ptr = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(::scratch));
+        (void)hipSetDevice(prev);
+        if (err != hipSuccess || !ptr) {
+            set_error("hipGetSymbolAddress(scratch) failed on device %d: %s", ctx->device, hipGetErrorString(err));
+            return LG_ERR_HIP;
+        }
+        g_noalloc_ptr[ctx->device] = ptr;
+    }
+    *out = static_cast<unsigned char *>(g_noalloc_ptr[ctx->device]);
     return LG_OK;
 }
 
@@ -497,10 +534,24 @@ int32_t lg_diffuse(lg_ctx *ctx, int32_t n, const float *d_D, int32_t ld, const d
     float *ln = nullptr;
     LG_TRY(dalloc(ctx, (size_t)n * 3, &sums));
     int32_t rc = dalloc(ctx, (size_t)n * 3, &ln);
-    if (rc == LG_OK) rc = diffuse_impl(ctx, n, d_D, ld, sigma_t, d_L, d_out, sums, ln);
+    if (rc == LG_OK) rc = diffuse_impl(ctx, n, d_D, ld, sigma_t, d_L, d_out, sums, ln, true);
     dfree(ctx, sums);
     dfree(ctx, ln);
     return rc;
+}
+
+int32_t lg_diffuse_noalloc(lg_ctx *ctx, int32_t n, const float *d_D, int32_t ld, const double sigma_t[3],
+                           const float *d_L, float *d_out) {
+    LG_CHECK_ARG(ctx && d_D && sigma_t && d_L && d_out && ld >= n && n >= 0, "lg_diffuse_noalloc: bad argument");
+    LG_CHECK_ARG(n <= LG_DIFFUSE_NOALLOC_MAX_N, "lg_diffuse_noalloc: n=%d above the scratch limit of %d vertices",
+                 n, LG_DIFFUSE_NOALLOC_MAX_N);
+    if (n == 0) return LG_OK;
+    unsigned char *base = nullptr;
+    LG_TRY(noalloc_scratch(ctx, &base));
+    // k_rowsum writes every sum it reads back and k_lnorm every normalised
+    // entry, so nothing left by an earlier call is read
+    return diffuse_impl(ctx, n, d_D, ld, sigma_t, d_L, d_out, reinterpret_cast<double *>(base),
+                        reinterpret_cast<float *>(base + kNoAllocSums), false);
 }
 
 int32_t lg_solver_create(lg_ctx *ctx, int32_t n_max, int32_t edge_cap, int32_t flags, lg_solver **out) {
