@@ -35,22 +35,32 @@ KERNELS = ("k_light_rays", "k_march_edges", "k_fw_minplus", "k_fw_panel", "k_fw_
 STAGES = ("k_vertex_light", "k_graph", "k_cand", "k_greedy", "k_components", "k_union_find", "k_comp_sort")
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
-# kernel -> (bound, work counter of lg_solver_stats, algorithmic FP32 ops per unit)
-#   marches: 32 FP32 ops per trilinear sample (index transform + 8-tap trilinear 23,
-#            scale/clamp/floor 3, accumulate 2, amortised exp 4);
-#   min-plus: one add + one min per (i, j, k) of each executed 128^3 tile update;
-#   small components: one add + one min per (i, j, k) of each component's closure.
-# kernel: (bound, work counter, algorithmic amount per unit).  The marches are
-# gathers: per trilinear sample 8 taps of 4 B (FP32 bricks) against HBM; the
-# Floyd-Warshall kernels are 2 VALU ops (add + min) per (i, j, k) update.
-ROOF = {
-    "k_light_rays": ("hbm", "shadow_samples", 32.0),
-    "k_march_edges": ("hbm", "edge_samples", 32.0),
-    "k_fw_minplus": ("valu", "fw_tiles", 2.0 * 128 ** 3),
-    "k_fw_panel": ("valu", "fw_panel_tiles", 2.0 * 128 ** 3),
-    "k_fw_pivot": ("valu", "fw_pivots", 2.0 * 128 ** 3),
-    "k_comp_small": ("valu", "small_updates", 2.0),
+# VALU issue ceiling without packed FP32: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
+# instruction per SIMD every 4 cycles = 614.4e9 wave instructions/s = 39.3e12 lane
+# instructions/s.  A min-plus update costs 1.5 of them (two adds and one
+# v_min3_f32 per pivot pair), so Floyd-Warshall tops out at 26.2e12 updates/s =
+# 52.4 T ops/s (add + min).
+VALU_LANE_INSTR_PER_S = 256 * 4 * 2.4e9 * 16
+FW_OPS_CEILING_TFLOPS = VALU_LANE_INSTR_PER_S / 1.5 * 2 / 1e12
+# Marches: t_roof = max(B_comp / 8 TB/s, S * 32 / 157.3 TFLOP/s) with
+# S trilinear samples at 32 FP32 ops each and B_comp the compulsory bytes: the
+# distinct 8^3 bricks the samples' taps read (2 KiB each in FP32) plus inputs and
+# outputs.  kernel -> (sample counter, brick counter, input + output bytes)
+MARCH = {
+    # points in (12 B each), one FP64 RGB sum out per ray (24 B)
+    "k_light_rays": ("shadow_samples", "light_bricks", lambda w: 12.0 * w.get("points", 0) + 24.0 * w.get("shadow_rays", 0)),
+    # points in, edges in (8 B), optical depth out (4 B)
+    "k_march_edges": ("edge_samples", "edge_bricks", lambda w: 12.0 * w.get("points", 0) + 12.0 * w.get("edges", 0)),
 }
+# Floyd-Warshall and the LDS closures: 2 ops (add + min) per executed update
+FW = {
+    "k_fw_minplus": "fw_updates",
+    "k_fw_panel": "fw_panel_updates",
+    "k_fw_pivot": "fw_pivot_updates",
+    "k_comp_small": "small_updates",
+}
+BRICK_BYTES = 512 * 4
+OPS_PER_SAMPLE = 32.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic.json")
 RENDER_KERNELS = ("k_render_count", "k_render_fill", "k_render_shadow", "k_render_combine")
 
@@ -147,7 +157,7 @@ def bench_batched(lga, ctx, shader, params, lights, count, reps):
     for order in ORDERS:
         run_batch(order, stats=True)
         st = batch.stats()
-        tiles[order] = int(st["fw_tiles"])
+        tiles[order] = int(st["fw_updates"])
     bwork = {k: v * reps for k, v in st.items()}
     broof = kernel_roofline(btimes, bwork, "c3_batch_nd_order")
     plain.destroy()
@@ -171,7 +181,7 @@ def bench_batched(lga, ctx, shader, params, lights, count, reps):
             "speedup_note": "plain = one lg_solve per volume in ascending pivot order (the drop-in path); "
                             "speedup / speedup_bfs / speedup_nd = the batch in ascending / breadth-first / "
                             "nested-dissection order, each over plain ascending order",
-            "fw_remainder_tiles_per_pass": tiles,
+            "fw_remainder_updates_per_pass": tiles,
             "diffs": diffs,
             "max_abs_diff_batched_vs_plain": diffs["reference"]["max_abs_diff_vs_plain_same_order"],
             "roofline": broof,
@@ -220,6 +230,16 @@ def bench_render(lga, ctx, grid, shader, lights, solver, n, reps):
         launches, tot = ctx.kernel_time(nm)
         if launches:
             phases[nm] = {"launches": launches, "ms": round(tot, 3)}
+    # and one counting the distinct bricks each shadow launch reads (the marking
+    # slows it: not timed)
+    ctx.brick_counting(True)
+    ctx.kernel_timing(True)
+    lga.render(ctx, grid, sg, None, cam, shader, lights, rp, spp, films, seed=1)
+    ctx.sync()
+    ctx.kernel_timing(False)
+    ctx.brick_counting(False)
+    bricks = lga.render_stats(ctx)["shadow_bricks"]
+    ctx.kernel_time_reset()
     films.free()
     sg.destroy()
     pb.free()
@@ -227,17 +247,44 @@ def bench_render(lga, ctx, grid, shader, lights, solver, n, reps):
     out = {"workload": "c2_render", "width": w, "height": h, "spp": spp, "ms": round(ms, 3),
            "camera_rays_per_s": round(w * h * spp / (ms * 1e-3), 1), "splat_ms": round(splat_ms, 4),
            "primary_step": 1.0, "shadow_step": 3.0, "reps": reps, "mean_alpha": round(alpha_mean, 4),
-           "phases": phases, "work": {k: v for k, v in st.items() if k != "shadow_lane_slots"},
+           "phases": phases, "work": {k: v for k, v in st.items() if k not in ("shadow_lane_slots", "shadow_bricks")},
            "shadow_lane_efficiency": round(st["shadow_samples"] / max(st["shadow_lane_slots"], 1), 4)}
-    sh_ms = phases.get("k_render_shadow", {}).get("ms")
-    if sh_ms:
-        # the shadow phase: trilinear samples of the shadow marches, 8 taps of 4 B each
-        achieved = st["shadow_samples"] * 32.0 / (sh_ms * 1e-3) / 1e9
-        out["roofline"] = {"kernel": "k_render_shadow", "bound": "hbm", "achieved": round(achieved, 2),
-                           "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
-                           "traffic": None, "work_unit": "shadow_samples", "bytes_per_unit": 32.0,
-                           "shadow_samples_per_s": round(st["shadow_samples"] / (sh_ms * 1e-3), 1),
-                           "note": "issue-bound on VALU (profiles/k_render_c2_1080p_kernel_trace.md); the taps hit L2"}
+    sh = phases.get("k_render_shadow")
+    if sh:
+        # the shadow phase (kernel d's dominant kernel): shadow-ray trilinear
+        # samples and the bricks they read; inputs 32 B per shading point, light
+        # sums out 24 B per (light, point)
+        io = 32.0 * st["shading_points"] + 24.0 * len(lights) * st["shading_points"]
+        roof = {"kernel": "k_render_shadow", "traffic": None, "launches": sh["launches"],
+                "avg_launch_ms": round(sh["ms"] / sh["launches"], 4)}
+        roof.update(march_roofline(st["shadow_samples"], bricks, io, sh["ms"] * 1e-3))
+        roof["shadow_samples_per_s"] = round(st["shadow_samples"] / (sh["ms"] * 1e-3), 1)
+        out["roofline"] = roof
+    return out
+
+
+def bench_iter(lga, ctx, grid, shader, params, lights, n, radius, iters_list, reps=2):
+    """The reference's own shape: `-iter` solves of one volume (main.cc:534-538,
+    scenes/*.sh use 25..500), as LightGraph::operator()(0, iter) issues them --
+    one lg_solve_batch of `iter` solves of the production volume -- in the
+    ascending (drop-in) and nested-dissection pivot orders."""
+    out = {"workload": "c2_iter_batch", "points_per_solve": n, "runs": {}}
+    big = lga.Solver(ctx, n * max(iters_list))
+    for iters in iters_list:
+        items = [(grid, n, k) for k in range(iters)]
+        row = {}
+        for order in ("reference", "nd"):
+            big.solve_batch(items, shader, lights, params, radius=radius, order=order)  # warm-up
+            ctx.sync()
+            ctx.mark(8)
+            for _ in range(reps):
+                big.solve_batch(items, shader, lights, params, radius=radius, order=order)
+            ctx.mark(9)
+            ctx.sync()
+            ms = ctx.elapsed_ms(8, 9) / reps
+            row[order] = {"ms": round(ms, 3), "solves_per_s": round(iters * 1000.0 / ms, 1)}
+        out["runs"][str(iters)] = row
+    big.destroy()
     return out
 
 
@@ -270,7 +317,11 @@ def bench_fw_dense(lga, ctx, solver, n):
     return {"workload": "c2_fw_dense", "n": n, "ms": round(ms, 3),
             "algorithmic_tflops": round(alg / (ms * 1e-3) / 1e12, 2),
             "frac_of_fp32_peak": round(alg / (ms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS, 4),
-            "note": "2 n^3 ops as the reference's triple loop; the symmetric blocked kernel computes half the tiles",
+            "executed_tflops": round(alg / 2 / (ms * 1e-3) / 1e12, 2),
+            "issue_ceiling_tflops": round(FW_OPS_CEILING_TFLOPS, 2),
+            "executed_frac_of_issue_ceiling": round(alg / 2 / (ms * 1e-3) / 1e12 / FW_OPS_CEILING_TFLOPS, 4),
+            "note": "2 n^3 ops as the reference's triple loop; the symmetric blocked kernel computes half the "
+                    "tiles (executed = n^3 ops); the issue ceiling is 1.5 VALU instructions per update",
             "kernels": {k: {"launches": v["launches"], "avg_ms": v["avg_ms"]} for k, v in t.items()}}
 
 
@@ -411,36 +462,63 @@ def kernel_times(ctx, names):
     return out
 
 
+def march_roofline(samples, bricks, io_bytes, t_s):
+    """t_roof / t for one march launch (or a sum of launches): the compulsory-byte
+    term against HBM and the trilinear-sample term against the FP32 peak; the
+    bound is the larger term."""
+    b_comp = bricks * BRICK_BYTES + io_bytes
+    t_hbm = b_comp / (PEAK_HBM_GBS * 1e9)
+    t_valu = samples * OPS_PER_SAMPLE / (PEAK_FP32_TFLOPS * 1e12)
+    if t_hbm >= t_valu:
+        achieved = b_comp / t_s / 1e9
+        out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+               "frac": round(achieved / PEAK_HBM_GBS, 4)}
+    else:
+        achieved = samples * OPS_PER_SAMPLE / t_s / 1e12
+        out = {"bound": "valu", "achieved": round(achieved, 4), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+               "frac": round(achieved / PEAK_FP32_TFLOPS, 5)}
+    out.update({"compulsory_bytes": int(b_comp), "bricks": int(bricks), "samples": int(samples),
+                "t_roof_hbm_us": round(t_hbm * 1e6, 3), "t_roof_valu_us": round(t_valu * 1e6, 3),
+                "issued_gather_gbs": round(samples * 8 * 4 / t_s / 1e9, 1)})
+    return out
+
+
 def kernel_roofline(times, work, workload):
-    """Roofline object for the kernel with the largest total time among KERNELS:
-    achieved = algorithmic work of its launches (device work counters of the same
-    solves) / their summed HIP-event duration."""
+    """Roofline object for the kernel with the largest total time among KERNELS,
+    over the launches of the timed region: the marches by SURVEY 8(d)'s formula
+    (compulsory bytes or trilinear samples, whichever bounds), the Floyd-Warshall
+    kernels by executed min-plus updates (2 ops each) against the FP32 peak and
+    the no-packed-math VALU issue ceiling.  Work comes from the device counters
+    of the same solves."""
     cands = [(t["total_ms"], nm) for nm, t in times.items() if nm in KERNELS]
     if not cands:
         return None
     _, name = max(cands)
     t = times[name]
+    t_s = t["total_ms"] * 1e-3
     roof = {"kernel": name, "bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
             "traffic": None, "avg_launch_ms": t["avg_ms"], "launches": t["launches"]}
-    if name in ROOF:
-        bound, key, per = ROOF[name]
-        units = work.get(key, 0)
-        if bound == "hbm":
-            achieved = units * per / (t["total_ms"] * 1e-3) / 1e9
-            roof.update({"bound": "hbm", "achieved": round(achieved, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                         "frac": round(achieved / PEAK_HBM_GBS, 4), "work_unit": key,
-                         "units_per_launch": round(units / t["launches"], 1), "bytes_per_unit": per})
-        else:
-            achieved = units * per / (t["total_ms"] * 1e-3) / 1e12
-            roof.update({"bound": bound, "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
-                         "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "work_unit": key,
-                         "units_per_launch": round(units / t["launches"], 1), "ops_per_unit": per})
+    if name in MARCH:
+        skey, bkey, io = MARCH[name]
+        roof.update(march_roofline(work.get(skey, 0), work.get(bkey, 0), io(work), t_s))
+        roof["per_launch"] = {"samples": round(work.get(skey, 0) / t["launches"], 1),
+                              "bricks": round(work.get(bkey, 0) / t["launches"], 1),
+                              "compulsory_bytes": round(roof["compulsory_bytes"] / t["launches"], 1)}
+    elif name in FW:
+        ops = 2.0 * work.get(FW[name], 0)
+        achieved = ops / t_s / 1e12
+        roof.update({"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "work_unit": FW[name] + " x 2 ops",
+                     "issue_ceiling_tflops": round(FW_OPS_CEILING_TFLOPS, 2),
+                     "frac_of_issue_ceiling": round(achieved / FW_OPS_CEILING_TFLOPS, 4),
+                     "ops_per_launch": round(ops / t["launches"], 1)})
     if os.path.exists(TRAFFIC_FILE):
         with open(TRAFFIC_FILE) as f:
             tr = json.load(f).get(workload, {}).get(name)
         if tr:
             roof["traffic"] = tr["hbm_bytes_per_launch"]
             roof["traffic_gbs"] = round(tr["hbm_bytes_per_launch"] / (t["avg_ms"] * 1e-3) / 1e9, 1)
+            roof["traffic_source"] = tr.get("source")
     return roof
 
 
@@ -481,36 +559,56 @@ def _dart_start(lga, ctx, grid):
     return np.float32(0.13) * (np.float32(np.float32(size[0] + size[1]) + size[2]) / np.float32(3.0))
 
 
-def cpu_baseline(lga, ctx, grid, shader, params, lights, n_points, radius, budget_s):
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(lga, ctx, grid, shader, params, lights, n_points, radius, rounds=3):
     """The oracle's serial solve (max-product Floyd-Warshall on Vec3 transmittance,
-    as the reference) on the same volume and points, one solve per host thread,
-    as many threads as cores (the reference parallelises across solves)."""
-    import numpy as np
+    FP64, built -O3 like the reference) on the same volume and points, one solve
+    per host thread, as the reference parallelises across solves
+    (main.cc:534-538).  Threads: the host CPUs this process may run on, capped
+    at 16, the host share the GPU box gives one GPU.  Median of `rounds` rounds
+    after one warm-up round."""
     from oracle import lgoracle
     dense = grid.read_dense()
-    cores = min(16, os.cpu_count() or 1)
+    visible = os.cpu_count() or 1
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else visible
+    cores = max(1, min(16, allowed))
     solver = lga.Solver(ctx, n_points)
     pts = []
     for s in range(cores):
         solver.solve(grid, shader, lights, params, n=n_points, radius=radius, seed=854123, solve=1000 + s)
         pts.append(solver.results(want_D=False)["points"])
     solver.destroy()
-    done = []
 
     def one(k):
         lgoracle.solve_points(dense, shader, params, lights, pts[k], radius=radius, cap=12)
-        done.append(k)
 
-    t0 = time.perf_counter()
-    th = [threading.Thread(target=one, args=(k,)) for k in range(cores)]
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
-    dt = time.perf_counter() - t0
-    return {"value": round(len(done) / dt, 4), "unit": "solves/s", "cores": cores, "kind": "port",
-            "sample": f"{cores} concurrent full oracle solves (N={n_points}, same volume, points and lights), "
-                      f"{dt:.1f} s wall"}
+    walls = []
+    for r in range(rounds + 1):
+        t0 = time.perf_counter()
+        th = [threading.Thread(target=one, args=(k,)) for k in range(cores)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        if r > 0:
+            walls.append(time.perf_counter() - t0)
+    walls.sort()
+    dt = walls[len(walls) // 2]
+    return {"value": round(cores / dt, 4), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} concurrent full oracle solves (N={n_points}, same volume, points and lights) per "
+                      f"round, median of {rounds} rounds after a warm-up: {dt:.2f} s wall",
+            "cpu_model": _cpu_model(), "host_cpus_visible": visible, "host_cpus_allowed": allowed,
+            "build": "g++ -O3 -ffp-contract=off, FP64"}
 
 
 def main():
@@ -520,7 +618,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--points", type=int, default=8192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-budget", type=float, default=60.0)
     ap.add_argument("--batch-volumes", type=int, default=64)
     ap.add_argument("--batch-reps", type=int, default=3)
     ap.add_argument("--latency-solves", type=int, default=50)
@@ -531,6 +628,8 @@ def main():
     ap.add_argument("--serve-workers", type=int, default=3)
     ap.add_argument("--serve-batch", type=int, default=8)
     ap.add_argument("--no-fw-dense", action="store_true", help="skip the dense n x n Floyd-Warshall timing")
+    ap.add_argument("--iters", default="25,100,500",
+                    help="solves per lg_solve_batch of the production volume (the reference's -iter); empty: skip")
     ap.add_argument("--dart-iters", type=int, default=64,
                     help="iterations of the reference's dart thrower (1e6 throws each) to time; 0: skip")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -608,6 +707,9 @@ def main():
             "latency": {"p50_ms": pct(0.50), "p99_ms": pct(0.99), "solves": len(lat), "mode": "one solve alone"}}
     if not args.no_fw_dense:
         line["fw_dense"] = bench_fw_dense(lga, ctx, solver, n)
+    iters = [int(x) for x in args.iters.split(",") if x]
+    if iters:
+        line["iter_batch"] = bench_iter(lga, ctx, grid, shader, params, lights, n, radius, iters)
     if args.render_reps > 0:
         line["render"] = bench_render(lga, ctx, grid, shader, lights, solver, n, args.render_reps)
     solver.destroy()
@@ -626,7 +728,7 @@ def main():
         rates = [float(x) for x in args.serve_rates.split(",") if x]
         line["serving"] = bench_serving(lga, ctx, rates, args.serve_jobs, args.serve_workers, args.serve_batch)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(lga, ctx, grid, shader, params, lights, n, radius, args.cpu_budget)
+        line["cpu_baseline"] = cpu_baseline(lga, ctx, grid, shader, params, lights, n, radius)
     else:
         line["cpu_baseline"] = None
     if args.copy_calibration_mib > 0:

@@ -193,6 +193,10 @@ int32_t lg_ctx_set_kernel_timing(lg_ctx *ctx, int32_t enable);
  * kernel and none elsewhere.  No reference counterpart. */
 int32_t lg_ctx_set_kernel_timing_only(lg_ctx *ctx, const char *kernel);
 int32_t lg_kernel_time(lg_ctx *ctx, const char *kernel, uint64_t *launches, double *total_ms);
+/* Renders made while kernel timing is on also count the distinct bricks each
+ * shadow launch reads (lg_render_stats.shadow_bricks); the marking slows the
+ * shadow phase, so a render counted this way is not the one to time. */
+int32_t lg_ctx_set_brick_counting(lg_ctx *ctx, int32_t enable);
 int32_t lg_kernel_time_reset(lg_ctx *ctx);
 
 /* Device memory, charged against the context budget. */
@@ -382,12 +386,19 @@ typedef struct lg_solve_stats {
     uint64_t edge_samples;    /* trilinear samples of the edge march                    */
     uint64_t shadow_rays;     /* vertex-light shadow rays                               */
     uint64_t shadow_samples;  /* their trilinear samples                                */
-    uint64_t fw_tiles;        /* 128x128x128 min-plus remainder tile updates (kernel c) */
-    uint64_t fw_panel_tiles;  /* the same for the row-panel tiles                       */
-    uint64_t fw_pivots;       /* 128x128 pivot-tile closures                            */
+    /* Floyd-Warshall (kernel c): executed (i, j, k) min-plus updates, one add and
+     * one min each; the pivots of a round are counted in the 32-pivot chunks that
+     * run, each tile part (quadrant, half, quarter) counts what it computed */
+    uint64_t fw_updates;         /* remainder tiles                                     */
+    uint64_t fw_panel_updates;   /* row-panel tiles                                     */
+    uint64_t fw_pivot_updates;   /* pivot-tile closures (pivots x 128 x 128)            */
     uint64_t small_updates;   /* sum of m^3 over the components closed in LDS (m <= 128) */
     uint64_t components;      /* connected components                                   */
     uint64_t big_components;  /* components above 128 vertices                          */
+    /* compulsory bytes of the marches: distinct 8^3 bricks whose voxels the
+     * samples' trilinear taps read, summed over the grids of the call */
+    uint64_t light_bricks;    /* vertex-light shadow rays                               */
+    uint64_t edge_bricks;     /* edge march                                             */
 } lg_solve_stats;
 int32_t lg_solver_stats(lg_ctx *ctx, const lg_solver *solver, lg_solve_stats *out);
 enum { LG_SOLVER_DENSE_D = 1, LG_SOLVER_PRED = 2 };
@@ -567,6 +578,8 @@ typedef struct lg_render_stats {
     uint64_t shadow_lane_slots; /* 64 x the most samples of a lane, summed over the
                                  * shadow waves: shadow_samples / this = lane efficiency */
     uint64_t chunks;           /* shading-point chunks the render ran in              */
+    uint64_t shadow_bricks;    /* distinct 8^3 bricks the shadow taps read, summed over
+                                * the shadow launches (with lg_ctx_set_brick_counting) */
 } lg_render_stats;
 int32_t lg_render_get_stats(lg_ctx *ctx, lg_render_stats *out);
 

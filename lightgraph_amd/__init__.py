@@ -199,6 +199,11 @@ class Context:
                "kernel_timing_only")
         _check(lib().lg_ctx_set_kernel_timing(self.handle, 1 if enable else 0), "kernel_timing")
 
+    def brick_counting(self, enable=True):
+        """Renders under kernel timing also count the distinct bricks of each
+        shadow launch (render_stats()["shadow_bricks"]); slows those renders."""
+        _check(lib().lg_ctx_set_brick_counting(self.handle, 1 if enable else 0), "brick_counting")
+
     def kernel_time(self, name):
         n = C.c_uint64()
         ms = C.c_double()

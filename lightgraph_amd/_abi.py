@@ -94,7 +94,8 @@ class lg_solve_params(C.Structure):
 
 class lg_solve_stats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("points", "edges", "edge_samples", "shadow_rays", "shadow_samples",
-                                          "fw_tiles", "fw_panel_tiles", "fw_pivots", "small_updates", "components", "big_components")]
+                                          "fw_updates", "fw_panel_updates", "fw_pivot_updates", "small_updates", "components",
+                                          "big_components", "light_bricks", "edge_bricks")]
 
 
 class lg_fw_tuning(C.Structure):
@@ -105,7 +106,7 @@ class lg_fw_tuning(C.Structure):
 
 class lg_render_stats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("camera_rays", "primary_samples", "shading_points", "shadow_rays",
-                                          "shadow_samples", "shadow_lane_slots", "chunks")]
+                                          "shadow_samples", "shadow_lane_slots", "chunks", "shadow_bricks")]
 
 
 LG_MAX_LIGHTS_ = 8
@@ -156,6 +157,7 @@ SIGNATURES = {
     "lg_ctx_bytes_in_use": (I32, [P, C.POINTER(U64)]),
     "lg_ctx_set_kernel_timing": (I32, [P, I32]),
     "lg_ctx_set_kernel_timing_only": (I32, [P, C.c_char_p]),
+    "lg_ctx_set_brick_counting": (I32, [P, I32]),
     "lg_kernel_time": (I32, [P, C.c_char_p, C.POINTER(U64), DP]),
     "lg_kernel_time_reset": (I32, [P]),
     "lg_malloc": (I32, [P, SZ, PP]),

@@ -117,12 +117,30 @@ struct GridView {
     int empty;             // 1: no active brick (every ray misses)
     int off32;             // 1: the storage is below 4 GiB (taps use buffer loads)
     uint32_t data_bytes;   // storage bytes when off32
+    uint8_t *touched;      // work counters only: per storage brick, set when a tap reads it
 };
 
 // Element offset of voxel (x,y,z); callers keep x in [-8, 8*nbx+7] etc.
 __device__ __forceinline__ long long voxel_offset(const GridView &g, int x, int y, int z) {
     const long long brick = ((long long)((z >> 3) + 1) * g.NBy + ((y >> 3) + 1)) * g.NBx + ((x >> 3) + 1);
     return (brick << 9) + ((z & 7) << 6) + ((y & 7) << 3) + (x & 7);
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// Work counters: flags the storage bricks that the 8 taps of a trilinear
+// sample at index-space (x, y, z) read (clamped as trilinear clamps them).
+__device__ __forceinline__ void mark_taps(const GridView &g, double x, double y, double z) {
+    const int cx = clampi((int)floor(x), -8, 8 * g.nbx + 6);
+    const int cy = clampi((int)floor(y), -8, 8 * g.nby + 6);
+    const int cz = clampi((int)floor(z), -8, 8 * g.nbz + 6);
+    for (int dz = 0; dz < 2; ++dz)
+        for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx) {
+                const long long b = ((long long)(((cz + dz) >> 3) + 1) * g.NBy + (((cy + dy) >> 3) + 1)) * g.NBx +
+                                    (((cx + dx) >> 3) + 1);
+                g.touched[b] = 1;
+            }
 }
 
 // OpenVDB BoxSampler::trilinearInterpolation on a FloatGrid: each lerp is
@@ -133,7 +151,6 @@ __device__ __forceinline__ float box_lerp(float a, float b, double w) {
     return a + (float)t;
 }
 
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
 // BoxSampler::sample at index-space position (x,y,z): background 0 outside the
 // stored voxels.  kBufferTaps: raw buffer loads with 32-bit offsets when the

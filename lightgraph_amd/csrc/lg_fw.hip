@@ -143,7 +143,8 @@ __global__ void __launch_bounds__(1024) k_fw_pivot_b(const FwJob *__restrict__ j
                                                      int kb, int *__restrict__ flags,
                                                      unsigned long long *__restrict__ stats) {
     const FwJob j = jobs[active[blockIdx.x]];
-    if (stats && threadIdx.x == 0) stat_add(stats, ST_FW_PIVOTS, 1ull);
+    if (stats && threadIdx.x == 0)
+        stat_add(stats, ST_FW_PIVOTS, (unsigned long long)fw_round_pivots(j.n, kb) * TB * TB);
     for (int t = threadIdx.x; t < j.nt; t += blockDim.x) flags[j.flag_off + t] = 0;
     fw_pivot_tile(j.D, j.ld, kb, fw_round_pivots(j.n, kb));
 }
@@ -176,7 +177,7 @@ __device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, i
         Jb = b < kb ? b : b + 1;
         if (!flags[Ib] || !flags[Jb]) return;
     }
-    if (!PANEL && stats && threadIdx.x == 0) stat_add(stats, ST_FW_TILES, 1ull);
+    if (!PANEL && stats && threadIdx.x == 0) stat_add(stats, ST_FW_TILES, (unsigned long long)kend * TB * TB);
     __shared__ float Xs[KC][TB];
     __shared__ float Ys[KC][TB];
     const int tid = threadIdx.x;
@@ -206,7 +207,7 @@ __device__ __forceinline__ void fw_minplus_tile(float *__restrict__ D, int ld, i
 #pragma unroll
             for (int t = 0; t < 8; ++t) finite |= (acc[r][t] < INFINITY) ? 1 : 0;
         if (!__syncthreads_or(finite)) return;
-        if (stats && threadIdx.x == 0) stat_add(stats, ST_FW_PANEL_TILES, 1ull);
+        if (stats && threadIdx.x == 0) stat_add(stats, ST_FW_PANEL_TILES, (unsigned long long)kend * TB * TB);
     }
     const float *Xg = D + (long long)kb * TB * ld + (long long)Ib * TB;
     const float *Yg = D + (long long)kb * TB * ld + (long long)Jb * TB;
@@ -304,7 +305,7 @@ __device__ __forceinline__ void fw_panel_half(float *__restrict__ D, int ld, int
         finite |= (v.x < INFINITY) | (v.y < INFINITY) | (v.z < INFINITY) | (v.w < INFINITY);
     }
     if (!__syncthreads_or(finite)) return;  // all +INF: nothing changes
-    if (stats && h == 0 && threadIdx.x == 0) stat_add(stats, ST_FW_PANEL_TILES, 1ull);
+    if (stats && threadIdx.x == 0) stat_add(stats, ST_FW_PANEL_TILES, (unsigned long long)kend * TB * 64);
     const float *Xg = D + (long long)kb * TB * ld + (long long)Ib * TB;
     const float *Yg = D + (long long)kb * TB * ld + (long long)Jb * TB + c0;
     for (int kc = 0; kc < kend; kc += KC) {
@@ -396,7 +397,7 @@ __device__ __forceinline__ void fw_panel_quarter(float *__restrict__ D, int ld, 
         finite |= (v.x < INFINITY) | (v.y < INFINITY) | (v.z < INFINITY) | (v.w < INFINITY);
     }
     if (!__syncthreads_or(finite)) return;  // all +INF: nothing changes
-    if (stats && q == 0 && threadIdx.x == 0) stat_add(stats, ST_FW_PANEL_TILES, 1ull);
+    if (stats && threadIdx.x == 0) stat_add(stats, ST_FW_PANEL_TILES, (unsigned long long)kend * TB * 32);
     const float *Xg = D + (long long)kb * TB * ld + (long long)Ib * TB;
     const float *Yg = D + (long long)kb * TB * ld + (long long)Jb * TB + c0;
     for (int kc = 0; kc < kend; kc += KC) {
@@ -466,7 +467,7 @@ __device__ __forceinline__ void fw_minplus_quad(float *__restrict__ D, int ld, i
     const int Ib = a < kb ? a : a + 1;
     const int Jb = b < kb ? b : b + 1;
     if (!flags[Ib] || !flags[Jb]) return;
-    if (stats && q == 0 && threadIdx.x == 0) stat_add(stats, ST_FW_TILES, 1ull);
+    if (stats && threadIdx.x == 0) stat_add(stats, ST_FW_TILES, (unsigned long long)kend * 64 * 64);
     __shared__ float Xs[KC][64];
     __shared__ float Ys[KC][64];
     const int tid = threadIdx.x;

@@ -79,6 +79,7 @@ struct lg_ctx {
     uint64_t in_use = 0;
     std::map<void *, size_t> allocs;
     bool timing = false;
+    bool count_bricks = false;  // renders under timing also count distinct shadow bricks
     std::string timing_only;  // non-empty: time only this kernel's launches
     lg::KernelTimer ktimer;
     hipEvent_t marks[16] = {};
@@ -120,6 +121,9 @@ struct lg_grid {
     int32_t *active_list = nullptr;   // storage indices of active bricks, ascending
     uint64_t *mass_prefix = nullptr;  // exclusive prefix of brick masses (active_bricks+1)
     uint64_t total_mass = 0;
+    // work counters: one byte per storage brick, set by the marches' taps while
+    // a call counts its work (allocated on the first such call)
+    uint8_t *touched = nullptr;
 };
 
 struct lg_envmap {
@@ -173,7 +177,7 @@ int32_t grid_build_sampling_tables(lg_ctx *ctx, lg_grid *g);
 // Each counter has kStatSlots slots picked by block index (spreads the atomics);
 // the host sums them.
 enum : int { ST_EDGE_SAMPLES = 0, ST_SHADOW_RAYS, ST_SHADOW_SAMPLES, ST_FW_TILES, ST_FW_PIVOTS, ST_SMALL_UPDATES,
-             ST_FW_PANEL_TILES, ST_COUNT = 8 };
+             ST_FW_PANEL_TILES, ST_LIGHT_BRICKS, ST_EDGE_BRICKS, ST_COUNT = 10 };
 constexpr int kStatSlots = 64;
 __device__ __forceinline__ void stat_add(unsigned long long *stats, int counter, unsigned long long v) {
     atomicAdd(&stats[counter * kStatSlots + (blockIdx.x & (kStatSlots - 1))], v);
@@ -204,6 +208,13 @@ inline size_t dtype_bytes(int dtype) { return dtype == LG_F32 ? 4 : 2; }
 #include "lg_device.h"
 
 namespace lg {
+
+// Distinct-brick counters of the marches (LG_SOLVE_STATS, render statistics):
+// grid_touch_begin allocates the grid's per-brick flags if needed and clears
+// them; grid_touch_count adds the number of flagged bricks to counter `counter`
+// of `stats` (kStatSlots slots per counter) and clears the flags again.
+int32_t grid_touch_begin(lg_ctx *ctx, lg_grid *g);
+int32_t grid_touch_count(lg_ctx *ctx, const lg_grid *g, unsigned long long *stats, int counter);
 
 // Kernel-side view of a grid.
 inline lgd::GridView make_view(const lg_grid *g) {

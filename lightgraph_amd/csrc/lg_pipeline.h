@@ -70,7 +70,8 @@ LightConsts light_consts(const lg_shader *sh, const lg_march_params *mp);
 
 // Edge optical depth (raymarchConnection, LightGraph.h:364-406): +INF when the
 // ray misses the grid bbox or the dropout |T|^2 < cutoff fires.
-template <typename T>
+// kCount: also flag the bricks the taps read (g.touched, work counters).
+template <typename T, bool kCount = false>
 __device__ __forceinline__ float march_edge_one(const lgd::GridView &g, const MarchConsts &mc, const float *pi,
                                                 const float *pj, int *nsamp = nullptr) {
     const double sp[3] = {(double)pi[0], (double)pi[1], (double)pi[2]};
@@ -90,7 +91,9 @@ __device__ __forceinline__ float march_edge_one(const lgd::GridView &g, const Ma
         for (double t = mc.h * ceil(t0 / mc.h); t <= t1; t += mc.h) {
             ++count;
             if (nsamp) *nsamp = count;
-            const float rho = lgd::trilinear<T>(g, s.e[0] + s.d[0] * t, s.e[1] + s.d[1] * t, s.e[2] + s.d[2] * t);
+            const double x = s.e[0] + s.d[0] * t, y = s.e[1] + s.d[1] * t, z = s.e[2] + s.d[2] * t;
+            if (kCount) lgd::mark_taps(g, x, y, z);
+            const float rho = lgd::trilinear<T>(g, x, y, z);
             double dens = (double)rho * mc.ds_sds_scds[0] * mc.ds_sds_scds[1] * mc.ds_sds_scds[2];
             dens = (mc.dmax < dens) ? mc.dmax : dens;
             if (dens < mc.cutoff && mc.sdm < mc.cutoff) continue;
@@ -143,7 +146,9 @@ __device__ __forceinline__ bool shadow_tau(const lgd::GridView &g, const LightCo
     *dropped = false;
     while (lgd::isect_march(g, is, t0, t1)) {
         for (double t = lc.h * ceil(t0 / lc.h); t <= t1; t += lc.h) {
-            const float rho = lgd::trilinear<T>(g, is.e[0] + is.d[0] * t, is.e[1] + is.d[1] * t, is.e[2] + is.d[2] * t);
+            const double x = is.e[0] + is.d[0] * t, y = is.e[1] + is.d[1] * t, z = is.e[2] + is.d[2] * t;
+            if (g.touched) lgd::mark_taps(g, x, y, z);
+            const float rho = lgd::trilinear<T>(g, x, y, z);
             double dens = (double)rho * lc.ds_sds_scds[0] * lc.ds_sds_scds[1] * lc.ds_sds_scds[2];
             dens = (lc.dmax < dens) ? lc.dmax : dens;
             if (dens < lc.cutoff) continue;
@@ -165,7 +170,7 @@ __device__ __forceinline__ bool shadow_tau(const lgd::GridView &g, const LightCo
 // summation order differs only in rounding), and the dropout test fires at the
 // first sample whose running depth exceeds tau_star.  Every lane returns the
 // same values.
-template <typename T>
+template <typename T, bool kCount = false>
 __device__ __forceinline__ bool shadow_tau_wave(const lgd::GridView &g, const LightConsts &lc, const double p[3],
                                                 const double ldir[3], double distance, double *tau_out,
                                                 bool *dropped, int *nsamp) {
@@ -184,8 +189,9 @@ __device__ __forceinline__ bool shadow_tau_wave(const lgd::GridView &g, const Li
             *nsamp += __popcll(__ballot(act));
             double x = 0.0;
             if (act) {
-                const float rho = lgd::trilinear<T, false>(g, is.e[0] + is.d[0] * t, is.e[1] + is.d[1] * t,
-                                                           is.e[2] + is.d[2] * t);
+                const double px = is.e[0] + is.d[0] * t, py = is.e[1] + is.d[1] * t, pz = is.e[2] + is.d[2] * t;
+                if (kCount) lgd::mark_taps(g, px, py, pz);
+                const float rho = lgd::trilinear<T, false>(g, px, py, pz);
                 double dens = (double)rho * lc.ds_sds_scds[0] * lc.ds_sds_scds[1] * lc.ds_sds_scds[2];
                 dens = (lc.dmax < dens) ? lc.dmax : dens;
                 if (!(dens < lc.cutoff)) x = dens * lc.h;
@@ -349,8 +355,6 @@ struct PipelineParams {
     MarchConsts mc;
     LightConsts lc;
     lgl::DevLights lights;
-    hipEvent_t ev_after_sample = nullptr;
-    hipEvent_t ev_after_light = nullptr;
 };
 
 struct PipelineResult {
@@ -412,8 +416,6 @@ struct Pipeline {
     void release(lg_ctx *ctx);
 };
 
-int32_t run_pipeline(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, const PipelineParams &pp,
-                     PipelineResult *res);
 // stages of run_pipeline
 int32_t pl_begin(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, int *V_out, int *nmax_out);
 int32_t pl_sample(lg_ctx *ctx, Pipeline &pl, int V, int dtype, uint64_t seed, float *pts);

@@ -82,7 +82,7 @@ __global__ void k_light(const SolveDesc *__restrict__ sd, const int *__restrict_
 // Vertex lighting for exact steps: one wave per (vertex, light sample) shadow
 // ray, then one lane per vertex sums the samples in the order of
 // computeLighting (LightGraph.h:446-497).
-template <typename T>
+template <typename T, bool kCount>
 __global__ void __launch_bounds__(256) k_light_rays(const SolveDesc *__restrict__ sd, const int *__restrict__ sid,
                                                     int V, int nt, LightConsts lc, lgl::DevLights L, uint32_t ek0,
                                                     uint32_t ek1, const float *__restrict__ pts,
@@ -104,7 +104,7 @@ __global__ void __launch_bounds__(256) k_light_rays(const SolveDesc *__restrict_
     bool dropped;
     double r[3] = {0.0, 0.0, 0.0};
     int ns = 0;
-    if (shadow_tau_wave<T>(d.g, lc, p, ldir, distance, &tau, &dropped, &ns) && !dropped)
+    if (shadow_tau_wave<T, kCount>(d.g, lc, p, ldir, distance, &tau, &dropped, &ns) && !dropped)
         for (int k = 0; k < 3; ++k) r[k] = inten[k] * (double)expf((float)(lc.ext[k] * tau));
     if ((threadIdx.x & 63) == 0) {
         for (int k = 0; k < 3; ++k) ray[3 * w + k] = r[k];
@@ -758,7 +758,7 @@ __global__ void k_edges(int V, const int *__restrict__ off, const int *__restric
 }
 
 // ---------------------------------------------------------- edge march ------
-template <typename T>
+template <typename T, bool kCount>
 __global__ void k_march(const SolveDesc *__restrict__ sd, const int *__restrict__ sid, MarchConsts mc,
                         const float *__restrict__ pts, const int *__restrict__ edges, int edge_limit,
                         const int *__restrict__ n_edges, float *__restrict__ w, unsigned long long *__restrict__ stats) {
@@ -766,7 +766,7 @@ __global__ void k_march(const SolveDesc *__restrict__ sd, const int *__restrict_
     int ns = 0;
     if (e < min(*n_edges, edge_limit)) {
         const int i = edges[2 * e], j = edges[2 * e + 1];
-        w[e] = march_edge_one<T>(sd[sid[i]].g, mc, pts + 3 * (long long)i, pts + 3 * (long long)j, &ns);
+        w[e] = march_edge_one<T, kCount>(sd[sid[i]].g, mc, pts + 3 * (long long)i, pts + 3 * (long long)j, &ns);
     }
     if (stats) {  // one atomic per wave
         unsigned long long v = (unsigned long long)ns;
@@ -1787,8 +1787,12 @@ int32_t pl_light(lg_ctx *ctx, Pipeline &pl, int V, int dtype, const LightConsts 
             ktimer_begin(ctx, "k_light_rays", &ev_r);
             for_dtype(dtype, [&](auto tag) {
                 using T = decltype(tag);
-                k_light_rays<T><<<(unsigned)((rays + 3) / 4), 256, 0, ctx->stream>>>(
-                    pl.sd, pl.sid, V, nt, lc, lights, ek0, ek1, pts, pl.ray, pl.stats_on ? pl.stats : nullptr);
+                if (pl.stats_on)
+                    k_light_rays<T, true><<<(unsigned)((rays + 3) / 4), 256, 0, ctx->stream>>>(
+                        pl.sd, pl.sid, V, nt, lc, lights, ek0, ek1, pts, pl.ray, pl.stats);
+                else
+                    k_light_rays<T, false><<<(unsigned)((rays + 3) / 4), 256, 0, ctx->stream>>>(
+                        pl.sd, pl.sid, V, nt, lc, lights, ek0, ek1, pts, pl.ray, nullptr);
             });
             LG_LAUNCHED("k_light_rays");
             ktimer_end(ctx, "k_light_rays", ev_r);
@@ -1906,8 +1910,12 @@ int32_t pl_march(lg_ctx *ctx, Pipeline &pl, int dtype, const MarchConsts &mc, co
     if (Ktot > 0) {
         for_dtype(dtype, [&](auto tag) {
             using T = decltype(tag);
-            k_march<T><<<(unsigned)((Ktot + 127) / 128), 128, 0, ctx->stream>>>(
-                pl.sd, pl.sid, mc, pts, pl.edges_, Ktot, pl.epos + Ktot, pl.w_, pl.stats_on ? pl.stats : nullptr);
+            if (pl.stats_on)
+                k_march<T, true><<<(unsigned)((Ktot + 127) / 128), 128, 0, ctx->stream>>>(
+                    pl.sd, pl.sid, mc, pts, pl.edges_, Ktot, pl.epos + Ktot, pl.w_, pl.stats);
+            else
+                k_march<T, false><<<(unsigned)((Ktot + 127) / 128), 128, 0, ctx->stream>>>(
+                    pl.sd, pl.sid, mc, pts, pl.edges_, Ktot, pl.epos + Ktot, pl.w_, nullptr);
         });
         LG_LAUNCHED("k_march");
     }
@@ -2129,21 +2137,6 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
 void destroy_stage_pipeline(lg_ctx *ctx) {
     delete static_cast<Pipeline *>(ctx->stage_pipeline);
     ctx->stage_pipeline = nullptr;
-}
-
-int32_t run_pipeline(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, const PipelineParams &pp,
-                     PipelineResult *res) {
-    int V = 0, nmax = 0, Ktot = 0;
-    LG_TRY(pl_begin(ctx, pl, descs, &V, &nmax));
-    const int dtype = descs[0].dtype;
-    if (!pp.external_points) LG_TRY(pl_sample(ctx, pl, V, dtype, pp.seed, pp.pts));
-    if (pp.ev_after_sample) LG_HIP(hipEventRecord(pp.ev_after_sample, ctx->stream));
-    LG_TRY(pl_light(ctx, pl, V, dtype, pp.lc, pp.lights, pp.env_seed, pp.pts, pp.L));
-    if (pp.ev_after_light) LG_HIP(hipEventRecord(pp.ev_after_light, ctx->stream));
-    LG_TRY(pl_graph(ctx, pl, V, (int)descs.size(), nmax, pp.pts, pp.r2, pp.max_connect, &Ktot));
-    LG_TRY(pl_march(ctx, pl, dtype, pp.mc, pp.pts, Ktot));
-    LG_TRY(pl_components(ctx, pl, descs, V, Ktot, pp.ext, pp.L, pp.out, res));
-    return LG_OK;
 }
 
 }  // namespace lg

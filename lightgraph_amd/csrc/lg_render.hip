@@ -340,7 +340,7 @@ struct RayOut {
 };
 
 enum : int { RS_CAMERA_RAYS = 0, RS_PRIMARY_SAMPLES, RS_SHADING_POINTS, RS_SHADOW_RAYS, RS_SHADOW_SAMPLES,
-             RS_SHADOW_LANE_SLOTS, RS_COUNT };
+             RS_SHADOW_LANE_SLOTS, RS_SHADOW_BRICKS, RS_COUNT };
 
 __device__ __forceinline__ void render_stat(unsigned long long *stats, int counter, unsigned long long v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -547,7 +547,7 @@ __global__ void __launch_bounds__(256) k_render_fill(lgd::GridView g, RenderCons
 // each span and after each sample that passed the cutoff.  Returns false when
 // the ray misses the bbox; *tau_out receives the optical depth, *dropped the
 // dropout (transmittance 0).
-template <typename T>
+template <typename T, bool kCount>
 __device__ __forceinline__ bool render_shadow(const lgd::GridView &g, const RenderConsts &c, const double p[3],
                                               const double ldir[3], double distance, lgd::Stream &rng,
                                               double *tau_out, bool *dropped, uint32_t *samples) {
@@ -560,7 +560,9 @@ __device__ __forceinline__ bool render_shadow(const lgd::GridView &g, const Rend
     while (lgd::isect_march(g, is, t0, t1)) {
         double step = c.step1 * (1.0 + ((double)rng.next() - 0.5) * 0.5);
         for (double t = c.step1 * ceil(t0 / c.step1); t <= t1; t += step) {
-            const float rho = lgd::trilinear<T>(g, is.e[0] + is.d[0] * t, is.e[1] + is.d[1] * t, is.e[2] + is.d[2] * t);
+            const double x = is.e[0] + is.d[0] * t, y = is.e[1] + is.d[1] * t, z = is.e[2] + is.d[2] * t;
+            if (kCount) lgd::mark_taps(g, x, y, z);
+            const float rho = lgd::trilinear<T>(g, x, y, z);
             ++*samples;
             double dens = (double)rho * c.ds * c.sds;
             dens = (c.dmax < dens) ? c.dmax : dens;
@@ -581,7 +583,7 @@ __device__ __forceinline__ bool render_shadow(const lgd::GridView &g, const Rend
 // Phase shadow: task t = light * P_pad + point (P_pad a multiple of 64, so a
 // wave holds one light); out[3 t ..] = the sum over the light's samples of
 // intensity * transmittance (Renderer.h:609-643).
-template <typename T>
+template <typename T, bool kCount>
 __global__ void __launch_bounds__(256, 4) k_render_shadow(lgd::GridView g, RenderConsts c, lgl::DevLights L,
                                                        uint32_t k0, uint32_t k1, uint32_t ek0, uint32_t ek1,
                                                        const ShadePoint *__restrict__ pts, uint32_t P,
@@ -613,7 +615,7 @@ __global__ void __launch_bounds__(256, 4) k_render_shadow(lgd::GridView g, Rende
             double tau_s;
             bool sdrop;
             ++rays;
-            if (!render_shadow<T>(g, c, wp, ldir, dist, rng, &tau_s, &sdrop, &smp)) continue;
+            if (!render_shadow<T, kCount>(g, c, wp, ldir, dist, rng, &tau_s, &sdrop, &smp)) continue;
             if (sdrop) continue;
             for (int a = 0; a < 3; ++a) tl[a] += inten[a] * (double)expf((float)(c.ext[a] * tau_s));
         }
@@ -1133,7 +1135,7 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
     c.height = camera->height;
     c.samples = samples;
     c.ratio_aa = 1.0 / (double)samples;
-    const lgd::GridView g = make_view(density);
+    lgd::GridView g = make_view(density);
     const lgd::GridView tg = temperature ? make_view(temperature) : lgd::GridView{};
     const CoarseView sc = scatter && scatter->d_rgb ? coarse_view(scatter) : CoarseView{};
     const int has_sc = (scatter && scatter->d_rgb) ? 1 : 0;
@@ -1145,6 +1147,12 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
         if (!ctx->render_stats) LG_TRY(dalloc(ctx, (size_t)RS_COUNT * kStatSlots, &ctx->render_stats));
         stats = ctx->render_stats;
         LG_HIP(hipMemsetAsync(stats, 0, sizeof(unsigned long long) * RS_COUNT * kStatSlots, ctx->stream));
+    }
+    // distinct bricks read by each shadow launch (lg_ctx_set_brick_counting)
+    lg_grid *touch = (stats && ctx->count_bricks) ? const_cast<lg_grid *>(density) : nullptr;
+    if (touch) {
+        LG_TRY(grid_touch_begin(ctx, touch));
+        g.touched = touch->touched;
     }
     if (nrays == 0) return LG_OK;
     // phase count over the whole image, then the shading points of every block of pixels
@@ -1229,11 +1237,16 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
                 ktimer_begin(ctx, "k_render_shadow", &ev);
                 for_dtype_r(density->desc.dtype, [&](auto tag) {
                     using T = decltype(tag);
-                    k_render_shadow<T><<<(unsigned)((tasks + 255) / 256), 256, 0, ctx->stream>>>(
-                        g, c, L, u0, u1, e0, e1, pts, P, P_pad, sums, stats);
+                    if (touch)
+                        k_render_shadow<T, true><<<(unsigned)((tasks + 255) / 256), 256, 0, ctx->stream>>>(
+                            g, c, L, u0, u1, e0, e1, pts, P, P_pad, sums, stats);
+                    else
+                        k_render_shadow<T, false><<<(unsigned)((tasks + 255) / 256), 256, 0, ctx->stream>>>(
+                            g, c, L, u0, u1, e0, e1, pts, P, P_pad, sums, stats);
                 });
                 LG_LAUNCHED("k_render_shadow");
                 ktimer_end(ctx, "k_render_shadow", ev);
+                if (touch) LG_TRY(grid_touch_count(ctx, touch, stats, RS_SHADOW_BRICKS));
             }
         }
         ktimer_begin(ctx, "k_render_combine", &ev);
@@ -1269,6 +1282,7 @@ int32_t lg_render_get_stats(lg_ctx *ctx, lg_render_stats *out) {
     out->shadow_rays = v[RS_SHADOW_RAYS];
     out->shadow_samples = v[RS_SHADOW_SAMPLES];
     out->shadow_lane_slots = v[RS_SHADOW_LANE_SLOTS];
+    out->shadow_bricks = v[RS_SHADOW_BRICKS];
     return LG_OK;
 }
 

@@ -326,6 +326,41 @@ int32_t grid_finalize(lg_ctx *ctx, lg_grid *g) {
     return LG_OK;
 }
 
+// ------------------------------------------------------ brick counters ----
+__global__ void __launch_bounds__(256) k_touch_count(uint8_t *__restrict__ flags, size_t n,
+                                                     unsigned long long *__restrict__ stats, int counter) {
+    unsigned long long c = 0;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        c += flags[i] != 0;
+        flags[i] = 0;
+    }
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+    if ((threadIdx.x & 63) == 0 && c) stat_add(stats, counter, c);
+}
+
+__global__ void __launch_bounds__(256) k_touch_clear(uint8_t *__restrict__ flags, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) flags[i] = 0;
+}
+
+static size_t grid_storage_bricks(const lg_grid *g) { return (size_t)g->NB[0] * g->NB[1] * g->NB[2]; }
+
+int32_t grid_touch_begin(lg_ctx *ctx, lg_grid *g) {
+    const size_t n = grid_storage_bricks(g);
+    if (!g->touched) LG_TRY(dalloc(ctx, n, &g->touched));
+    k_touch_clear<<<(unsigned)std::min<size_t>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(g->touched, n);
+    LG_LAUNCHED("k_touch_clear");
+    return LG_OK;
+}
+
+int32_t grid_touch_count(lg_ctx *ctx, const lg_grid *g, unsigned long long *stats, int counter) {
+    if (!g->touched) return LG_OK;
+    const size_t n = grid_storage_bricks(g);
+    k_touch_count<<<(unsigned)std::min<size_t>((n + 255) / 256, 1024), 256, 0, ctx->stream>>>(g->touched, n, stats,
+                                                                                             counter);
+    LG_LAUNCHED("k_touch_count");
+    return LG_OK;
+}
+
 }  // namespace lg
 
 using namespace lg;
@@ -420,6 +455,12 @@ int32_t lg_ctx_set_kernel_timing(lg_ctx *ctx, int32_t enable) {
 int32_t lg_ctx_set_kernel_timing_only(lg_ctx *ctx, const char *kernel) {
     LG_CHECK_ARG(ctx, "lg_ctx_set_kernel_timing_only: ctx is NULL");
     ctx->timing_only = kernel ? kernel : "";
+    return LG_OK;
+}
+
+int32_t lg_ctx_set_brick_counting(lg_ctx *ctx, int32_t enable) {
+    LG_CHECK_ARG(ctx, "lg_ctx_set_brick_counting: ctx is NULL");
+    ctx->count_bricks = enable != 0;
     return LG_OK;
 }
 
@@ -618,6 +659,7 @@ int32_t lg_grid_destroy(lg_ctx *ctx, lg_grid *g) {
     dfree(ctx, g->skip);
     dfree(ctx, g->active_list);
     dfree(ctx, g->mass_prefix);
+    dfree(ctx, g->touched);
     delete g;
     return LG_OK;
 }

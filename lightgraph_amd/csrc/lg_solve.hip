@@ -335,9 +335,12 @@ static int32_t noalloc_scratch(lg_ctx *ctx, unsigned char **out) {
 
 // Runs the pipeline over the solves in descs (vertex offsets voff, one shared
 // shader / light set / march parameters) into the solver's buffers.
+// With LG_SOLVE_STATS, `grids` are the call's distinct density grids, whose
+// per-brick flags the descriptors' views point at: the light and the edge march
+// run one after the other and each counts the distinct bricks its taps read.
 static int32_t solve_descs(lg_ctx *ctx, lg_solver *s, const std::vector<SolveDesc> &descs, const lg_shader *shader,
                            const lg_light *lights, int32_t n_lights, const lg_march_params *params,
-                           const lg_solve_params *sp) {
+                           const lg_solve_params *sp, const std::vector<lg_grid *> &grids) {
     Pipeline &pl = *static_cast<Pipeline *>(s->pipeline);
     PipelineParams pp{};
     pp.pts = s->buf.d_points;
@@ -374,7 +377,7 @@ static int32_t solve_descs(lg_ctx *ctx, lg_solver *s, const std::vector<SolveDes
     // component stage: it runs on the side stream beside the graph, edge march
     // and component sort (with per-stage timing on, the stages run in sequence
     // so that each interval times one stage).
-    const bool overlap = !t;
+    const bool overlap = !t && !pl.stats_on;
     hipEvent_t L_ready = nullptr;
     if (overlap) {
         LG_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
@@ -390,6 +393,8 @@ static int32_t solve_descs(lg_ctx *ctx, lg_solver *s, const std::vector<SolveDes
         }
     } else {
         LG_TRY(pl_light(ctx, pl, V, dtype, pp.lc, pp.lights, pp.env_seed, pp.pts, pp.L));
+        if (pl.stats_on)
+            for (lg_grid *g : grids) LG_TRY(grid_touch_count(ctx, g, pl.stats, ST_LIGHT_BRICKS));
     }
     // every return from here on leaves the main stream ordered after the light
     struct JoinGuard {
@@ -403,6 +408,8 @@ static int32_t solve_descs(lg_ctx *ctx, lg_solver *s, const std::vector<SolveDes
     LG_TRY(pl_graph(ctx, pl, V, (int)descs.size(), nmax, pp.pts, pp.r2, pp.max_connect, &Ktot));
     if (t) LG_HIP(hipEventRecord(s->ev[3], ctx->stream));
     LG_TRY(pl_march(ctx, pl, dtype, pp.mc, pp.pts, Ktot));
+    if (pl.stats_on)
+        for (lg_grid *g : grids) LG_TRY(grid_touch_count(ctx, g, pl.stats, ST_EDGE_BRICKS));
     if (t) LG_HIP(hipEventRecord(s->ev[4], ctx->stream));
     PipelineResult res;
     LG_TRY(pl_components(ctx, pl, descs, V, Ktot, pp.ext, pp.L, pp.out, &res, L_ready, pp.pts));
@@ -606,9 +613,11 @@ int32_t lg_solver_stats(lg_ctx *ctx, const lg_solver *s, lg_solve_stats *out) {
     out->edge_samples = h[ST_EDGE_SAMPLES];
     out->shadow_rays = h[ST_SHADOW_RAYS];
     out->shadow_samples = h[ST_SHADOW_SAMPLES];
-    out->fw_tiles = h[ST_FW_TILES];
-    out->fw_panel_tiles = h[ST_FW_PANEL_TILES];
-    out->fw_pivots = h[ST_FW_PIVOTS];
+    out->fw_updates = h[ST_FW_TILES];
+    out->fw_panel_updates = h[ST_FW_PANEL_TILES];
+    out->fw_pivot_updates = h[ST_FW_PIVOTS];
+    out->light_bricks = h[ST_LIGHT_BRICKS];
+    out->edge_bricks = h[ST_EDGE_BRICKS];
     out->small_updates = h[ST_SMALL_UPDATES];
     return LG_OK;
 }
@@ -674,7 +683,13 @@ int32_t lg_solve(lg_ctx *ctx, lg_solver *s, const lg_grid *density, const lg_gri
     d.ld = fw_ld(n);
     d.dense_D = dense ? s->buf.d_D : nullptr;
     d.dense_pred = (dense && s->with_pred) ? s->buf.d_pred : nullptr;
-    return solve_descs(ctx, s, {d}, shader, lights, n_lights, params, sp);
+    std::vector<lg_grid *> grids;
+    if (sp->flags & LG_SOLVE_STATS) {
+        LG_TRY(grid_touch_begin(ctx, g));
+        d.g.touched = g->touched;
+        grids.push_back(g);
+    }
+    return solve_descs(ctx, s, {d}, shader, lights, n_lights, params, sp, grids);
 }
 
 int32_t lg_solve_batch(lg_ctx *ctx, lg_solver *s, const lg_batch_item *items, int32_t count, const lg_shader *shader,
@@ -685,6 +700,7 @@ int32_t lg_solve_batch(lg_ctx *ctx, lg_solver *s, const lg_batch_item *items, in
     LG_CHECK_ARG((sp->flags & LG_SOLVE_DENSE) == 0, "lg_solve_batch: dense outputs are per-solve only (use lg_solve)");
     std::vector<SolveDesc> descs;
     descs.reserve((size_t)count);
+    std::vector<lg_grid *> grids;
     long long V = 0;
     for (int k = 0; k < count; ++k) {
         const lg_batch_item &it = items[k];
@@ -697,11 +713,18 @@ int32_t lg_solve_batch(lg_ctx *ctx, lg_solver *s, const lg_batch_item *items, in
         LG_TRY(grid_build_sampling_tables(ctx, g));
         SolveDesc d = make_desc(g, it.temperature, it.n, (int)V, it.solve);
         d.ld = fw_ld(it.n);
+        if (sp->flags & LG_SOLVE_STATS) {
+            if (std::find(grids.begin(), grids.end(), g) == grids.end()) {
+                LG_TRY(grid_touch_begin(ctx, g));
+                grids.push_back(g);
+            }
+            d.g.touched = g->touched;
+        }
         descs.push_back(d);
         V += it.n;
         LG_CHECK_ARG(V <= s->n_max, "lg_solve_batch: %lld points exceed the solver's n_max %d", V, s->n_max);
     }
-    return solve_descs(ctx, s, descs, shader, lights, n_lights, params, sp);
+    return solve_descs(ctx, s, descs, shader, lights, n_lights, params, sp, grids);
 }
 
 }  // extern "C"

@@ -217,10 +217,10 @@ def test_solve_batch_bfs_order(lga, ctx, volumes, order):
     bs = lga.Solver(ctx, sum(ns))
     bs.solve_batch(items, sh, lights, p, radius=6.0, seed=854123, order=order, stats=True)
     got = bs.batch_results()
-    tiles_bfs = bs.stats()["fw_tiles"]
+    tiles_bfs = bs.stats()["fw_updates"]
     bs.solve_batch(items, sh, lights, p, radius=6.0, seed=854123, stats=True)
     asc = bs.batch_results()
-    tiles_asc = bs.stats()["fw_tiles"]
+    tiles_asc = bs.stats()["fw_updates"]
     assert 0 < tiles_bfs < tiles_asc
     plain = lga.Solver(ctx, max(ns))
     for k, (g, n, solve) in enumerate(items):
@@ -269,8 +269,33 @@ def test_solve_stats_counts(lga, ctx, volumes):
     small = sizes[(sizes >= 2) & (sizes <= 128)].astype(np.int64)
     assert st["small_updates"] == int((small ** 3).sum())
     assert st["big_components"] == int((sizes > 128).sum())
-    assert (st["fw_tiles"] + st["fw_panel_tiles"] > 0) == (st["big_components"] > 0)
+    assert (st["fw_updates"] + st["fw_panel_updates"] > 0) == (st["big_components"] > 0)
+    # whole 32-pivot chunks of 128 x 128 tile parts
+    for k in ("fw_updates", "fw_panel_updates"):
+        assert st[k] % (32 * 32 * 64) == 0
+    storage = (np.array(volumes[1].read_dense().shape) // 8 + 2).prod()
+    assert 0 < st["light_bricks"] <= storage and 0 < st["edge_bricks"] <= storage
     # without the flag the counters are not collected
     s.solve(volumes[1], sh, lights, p, n=n, radius=6.0, seed=854123, solve=3)
     assert s.stats()["shadow_samples"] == 0
     s.destroy()
+
+
+def test_light_bricks_closed_form(lga, ctx):
+    """One vertex at (20.5, 20.3, 20.7) of a uniform 64^3 grid lit by a point
+    light straight above it: the shadow ray's samples run up the line x = 20.5,
+    z = 20.7 from y = 20.8 to 63.8 in steps of 0.5, so their taps read the
+    bricks (2, 2..8, 2) -- the last one the zero border above the grid --
+    7 distinct bricks; with no edges the edge march reads none."""
+    g = lga.Grid.from_dense(ctx, np.full((64, 64, 64), 0.01, np.float32))
+    sh = lga.default_shader(absorption=0.1, scattering=1.0)
+    p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5)
+    lights = [lga.point_light(0.0, (1, 1, 1), 1.0, (20.5, 200.0, 20.7))]
+    s = lga.Solver(ctx, 4)
+    s.write_points(np.array([[20.5, 20.3, 20.7]], np.float32))
+    s.solve(g, sh, lights, p, n=1, radius=4.0, external_points=True, stats=True)
+    st = s.stats()
+    assert st["shadow_rays"] == 1 and st["shadow_samples"] == 87
+    assert st["light_bricks"] == 7 and st["edge_bricks"] == 0
+    s.destroy()
+    g.destroy()

@@ -411,11 +411,11 @@ def test_full_solve_bfs_order(lga, ctx, oracle, c1, order):
     s = lga.Solver(ctx, n, dense=True, with_pred=True)
     s.solve(grid, sh, lights, p, n=n, radius=8.0, seed=854123, solve=5, stats=True)
     ref_order = s.results()
-    tiles_ref = s.stats()["fw_tiles"]
+    tiles_ref = s.stats()["fw_updates"]
     s.solve(grid, sh, lights, p, n=n, radius=8.0, seed=854123, solve=5, stats=True, order=order)
     r = s.results()
     st = s.stats()
-    assert st["big_components"] >= 1 and st["fw_tiles"] <= tiles_ref
+    assert st["big_components"] >= 1 and st["fw_updates"] <= tiles_ref
     for k in ("points", "edges", "weights", "L"):
         np.testing.assert_array_equal(r[k], ref_order[k], err_msg=k)
     assert np.array_equal(np.isinf(r["D"]), np.isinf(ref_order["D"]))
