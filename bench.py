@@ -30,8 +30,8 @@ sys.path.insert(0, ROOT)
 
 # Kernel-level timers (one kernel, or one kernel per size class) that can carry
 # the roofline, and stage timers reported beside them.
-KERNELS = ("k_light_rays", "k_march_edges", "k_fw_minplus", "k_fw_panel", "k_fw_pivot", "k_comp_small",
-           "k_sample_points", "k_big_fill", "k_diffuse")
+KERNELS = ("k_light_rays", "k_march_edges", "k_fw_minplus", "k_fw_panel", "k_fw_pivot", "k_comp_small_128",
+           "k_comp_small_64", "k_comp_small_32", "k_sample_points", "k_big_fill", "k_diffuse")
 STAGES = ("k_vertex_light", "k_graph", "k_cand", "k_greedy", "k_components", "k_union_find", "k_comp_sort")
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
@@ -54,10 +54,12 @@ MARCH = {
 }
 # Floyd-Warshall and the LDS closures: 2 ops (add + min) per executed update
 FW = {
-    "k_fw_minplus": "fw_updates",
-    "k_fw_panel": "fw_panel_updates",
-    "k_fw_pivot": "fw_pivot_updates",
-    "k_comp_small": "small_updates",
+    "k_fw_minplus": lambda w: w.get("fw_updates", 0),
+    "k_fw_panel": lambda w: w.get("fw_panel_updates", 0),
+    "k_fw_pivot": lambda w: w.get("fw_pivot_updates", 0),
+    "k_comp_small_128": lambda w: w.get("small_updates", 0) - w.get("small_updates_64", 0) - w.get("small_updates_32", 0),
+    "k_comp_small_64": lambda w: w.get("small_updates_64", 0),
+    "k_comp_small_32": lambda w: w.get("small_updates_32", 0),
 }
 BRICK_BYTES = 512 * 4
 OPS_PER_SAMPLE = 32.0
@@ -505,10 +507,10 @@ def kernel_roofline(times, work, workload):
                               "bricks": round(work.get(bkey, 0) / t["launches"], 1),
                               "compulsory_bytes": round(roof["compulsory_bytes"] / t["launches"], 1)}
     elif name in FW:
-        ops = 2.0 * work.get(FW[name], 0)
+        ops = 2.0 * FW[name](work)
         achieved = ops / t_s / 1e12
         roof.update({"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP32_TFLOPS, 4), "work_unit": FW[name] + " x 2 ops",
+                     "frac": round(achieved / PEAK_FP32_TFLOPS, 5), "work_unit": "executed min-plus updates x 2 ops",
                      "issue_ceiling_tflops": round(FW_OPS_CEILING_TFLOPS, 2),
                      "frac_of_issue_ceiling": round(achieved / FW_OPS_CEILING_TFLOPS, 4),
                      "ops_per_launch": round(ops / t["launches"], 1)})

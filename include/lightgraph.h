@@ -399,6 +399,10 @@ typedef struct lg_solve_stats {
      * samples' trilinear taps read, summed over the grids of the call */
     uint64_t light_bricks;    /* vertex-light shadow rays                               */
     uint64_t edge_bricks;     /* edge march                                             */
+    /* the part of small_updates closed by the 33..64- and 2..32-vertex kernels
+     * (the rest by the 65..128-vertex one) */
+    uint64_t small_updates_64;
+    uint64_t small_updates_32;
 } lg_solve_stats;
 int32_t lg_solver_stats(lg_ctx *ctx, const lg_solver *solver, lg_solve_stats *out);
 enum { LG_SOLVER_DENSE_D = 1, LG_SOLVER_PRED = 2 };

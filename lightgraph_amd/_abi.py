@@ -95,7 +95,8 @@ class lg_solve_params(C.Structure):
 class lg_solve_stats(C.Structure):
     _fields_ = [(k, C.c_uint64) for k in ("points", "edges", "edge_samples", "shadow_rays", "shadow_samples",
                                           "fw_updates", "fw_panel_updates", "fw_pivot_updates", "small_updates", "components",
-                                          "big_components", "light_bricks", "edge_bricks")]
+                                          "big_components", "light_bricks", "edge_bricks", "small_updates_64",
+                                          "small_updates_32")]
 
 
 class lg_fw_tuning(C.Structure):

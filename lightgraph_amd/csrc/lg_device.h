@@ -128,6 +128,16 @@ __device__ __forceinline__ long long voxel_offset(const GridView &g, int x, int 
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
+// A value every lane of the wave holds equally, moved to scalar registers (a
+// wave-per-ray march keeps its ray's state there instead of in 64 VGPR copies).
+__device__ __forceinline__ double wave_uniform(double x) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
+
 // Work counters: flags the storage bricks that the 8 taps of a trilinear
 // sample at index-space (x, y, z) read (clamped as trilinear clamps them).
 __device__ __forceinline__ void mark_taps(const GridView &g, double x, double y, double z) {
@@ -423,6 +433,30 @@ __device__ __forceinline__ bool isect_march(const GridView &g, Isect &s, double 
 // of the walk and loads its brick flag; the span rules run on the flag mask.
 // The cells, times and spans are those of isect_march.  Workgroups of at most
 // 256 threads (a 64-entry exchange buffer per wave).
+// The number of j in [0, 64) whose crossing next0 + (m + j) * delta comes
+// before time c (a tie counts as before when tie_first: the crossing's axis
+// comes first in MinIndex order).  The crossings are the very values the lanes
+// compute, a nondecreasing sequence in j, so the count is the closed form
+// ceil((c - next0) / delta - m), with inv_delta = |d| / 8 for 1 / delta (a few
+// ulps apart), settled by comparing against the crossing on each side of it:
+// the estimate is off by at most one.  (The one case it can miss further, c
+// infinite against a degenerate axis, is a crossing that never falls in the
+// batch: its rank is at least 64 through the ray's finite axes.)
+__device__ __forceinline__ int crossings_before(double next0, double delta, double inv_delta, int m, double c,
+                                                bool tie_first) {
+    auto before = [&](int j) {
+        const double v = next0 + (double)(m + j) * delta;
+        return v < c || (v == c && tie_first);
+    };
+    const double est = (c - next0) * inv_delta - (double)m;
+    int n = !(est > 0.0) ? 0 : (est >= 64.0 ? 64 : (int)ceil(est));
+    if (n > 0 && !before(n - 1))
+        --n;
+    else if (n < 64 && before(n))
+        ++n;
+    return n;
+}
+
 __device__ __forceinline__ bool isect_march_wave(const GridView &g, Isect &s, double &t0, double &t1) {
     __shared__ double ev_t[4][64];
     __shared__ int ev_ax[4][64];
@@ -433,6 +467,15 @@ __device__ __forceinline__ bool isect_march_wave(const GridView &g, Isect &s, do
         double next0[3], delta[3];
         int vox[3], step[3];
         dda_setup(s, s.t0, vox, step, next0, delta);
+        for (int a = 0; a < 3; ++a) {  // one ray per wave: the walk's state is the wave's
+            vox[a] = wave_uniform(vox[a]);
+            step[a] = wave_uniform(step[a]);
+            next0[a] = wave_uniform(next0[a]);
+            delta[a] = wave_uniform(delta[a]);
+        }
+        // estimates only (crossings_before settles every count exactly)
+        const double inv_delta[3] = {wave_uniform(fabs(s.d[0]) * 0.125), wave_uniform(fabs(s.d[1]) * 0.125),
+                                     wave_uniform(fabs(s.d[2]) * 0.125)};
         int mc[3] = {0, 0, 0};  // crossings taken per axis
         double tcur = s.t0;     // entry time of the current cell (cell 0 of the batch)
         bool found = false, done = false;
@@ -444,19 +487,8 @@ __device__ __forceinline__ bool isect_march_wave(const GridView &g, Isect &s, do
             int rank[3];
             for (int a = 0; a < 3; ++a) {
                 int r = lane;
-                for (int b = 0; b < 3; ++b) {
-                    if (b == a) continue;
-                    // count of j in [0, 64) with (c_b[j], b) before (c[a], a): ties put z, y, x first
-                    int lo = 0, hi = 64;
-                    for (int it = 0; it < 7; ++it) {
-                        const int mid = (lo + hi) >> 1;
-                        const double v = __shfl(c[b], mid < 64 ? mid : 63, 64);
-                        const bool before = mid < hi && (v < c[a] || (v == c[a] && b > a));
-                        lo = before ? mid + 1 : lo;
-                        hi = before ? hi : mid;
-                    }
-                    r += lo;
-                }
+                for (int b = 0; b < 3; ++b)
+                    if (b != a) r += crossings_before(next0[b], delta[b], inv_delta[b], mc[b], c[a], b > a);
                 rank[a] = r;
             }
             __builtin_amdgcn_wave_barrier();
@@ -491,13 +523,13 @@ __device__ __forceinline__ bool isect_march_wave(const GridView &g, Isect &s, do
                     if (!rem) break;
                     k = __ffsll((long long)rem) - 1;
                     if (k >= K) break;
-                    st0 = __shfl(tcell, k, 64);
+                    st0 = wave_uniform(__shfl(tcell, k, 64));
                     ++k;
                 } else {
                     const unsigned long long rem = ~E & V & from;
                     if (!rem) break;
                     k = __ffsll((long long)rem) - 1;
-                    st1 = __shfl(tcell, k, 64);
+                    st1 = wave_uniform(__shfl(tcell, k, 64));
                     if (st1 - st0 > kDeltaD) {
                         found = true;
                         break;
@@ -513,7 +545,7 @@ __device__ __forceinline__ bool isect_march_wave(const GridView &g, Isect &s, do
                 break;
             }
             // next batch: cell 64, entered at event 63
-            const double t63 = __shfl(te, 63, 64);
+            const double t63 = wave_uniform(__shfl(te, 63, 64));
             vox[0] += __popcll(mx) * step[0];
             vox[1] += __popcll(my) * step[1];
             vox[2] += __popcll(mz) * step[2];

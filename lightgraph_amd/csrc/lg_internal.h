@@ -177,7 +177,8 @@ int32_t grid_build_sampling_tables(lg_ctx *ctx, lg_grid *g);
 // Each counter has kStatSlots slots picked by block index (spreads the atomics);
 // the host sums them.
 enum : int { ST_EDGE_SAMPLES = 0, ST_SHADOW_RAYS, ST_SHADOW_SAMPLES, ST_FW_TILES, ST_FW_PIVOTS, ST_SMALL_UPDATES,
-             ST_FW_PANEL_TILES, ST_LIGHT_BRICKS, ST_EDGE_BRICKS, ST_COUNT = 10 };
+             ST_FW_PANEL_TILES, ST_LIGHT_BRICKS, ST_EDGE_BRICKS, ST_SMALL_UPDATES_64, ST_SMALL_UPDATES_32,
+             ST_COUNT = 12 };
 constexpr int kStatSlots = 64;
 __device__ __forceinline__ void stat_add(unsigned long long *stats, int counter, unsigned long long v) {
     atomicAdd(&stats[counter * kStatSlots + (blockIdx.x & (kStatSlots - 1))], v);

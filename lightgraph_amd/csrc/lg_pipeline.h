@@ -178,11 +178,19 @@ __device__ __forceinline__ bool shadow_tau_wave(const lgd::GridView &g, const Li
     const double sd[3] = {-ldir[0], -ldir[1], -ldir[2]};
     lgd::Isect is;
     if (!lgd::isect_world_ray(g, is, p, sd, lgd::kDeltaD, distance)) return false;
+    for (int c = 0; c < 3; ++c) {  // the wave's ray: scalar registers
+        is.e[c] = lgd::wave_uniform(is.e[c]);
+        is.d[c] = lgd::wave_uniform(is.d[c]);
+        is.inv[c] = lgd::wave_uniform(is.inv[c]);
+    }
+    is.t0 = lgd::wave_uniform(is.t0);
+    is.t1 = lgd::wave_uniform(is.t1);
+    is.tmax = lgd::wave_uniform(is.tmax);
     double tau = 0.0;
     double t0, t1;
     *dropped = false;
     while (lgd::isect_march_wave(g, is, t0, t1)) {
-        const double ts = lc.h * ceil(t0 / lc.h);
+        const double ts = lgd::wave_uniform(lc.h * ceil(t0 / lc.h));
         for (int base = 0;; base += 64) {
             const double t = ts + (double)(base + lane) * lc.h;
             const bool act = t <= t1;
@@ -207,7 +215,7 @@ __device__ __forceinline__ bool shadow_tau_wave(const lgd::GridView &g, const Li
                 *tau_out = tau;
                 return true;
             }
-            tau += __shfl(pre, 63, 64);
+            tau = lgd::wave_uniform(tau + __shfl(pre, 63, 64));
             if (!__shfl((int)act, 63, 64)) break;  // t grows with the lane: the span is done
         }
     }

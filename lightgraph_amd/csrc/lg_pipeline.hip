@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(256) k_light_rays(const SolveDesc *__restrict_
                                                     int V, int nt, LightConsts lc, lgl::DevLights L, uint32_t ek0,
                                                     uint32_t ek1, const float *__restrict__ pts,
                                                     double *__restrict__ ray, unsigned long long *__restrict__ stats) {
-    const long long w = (long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const long long w = (long long)blockIdx.x * (blockDim.x >> 6) + lgd::wave_uniform((int)(threadIdx.x >> 6));
     if (w >= (long long)V * nt) return;
     const int v = (int)(w / nt), t = (int)(w - (long long)v * nt);
     int li;
@@ -100,6 +100,7 @@ __global__ void __launch_bounds__(256) k_light_rays(const SolveDesc *__restrict_
     rng.skip(ew);
     double ldir[3], inten[3], distance;
     lgl::illuminate(L.l[li], p, rng, ldir, inten, distance);
+    for (int k = 0; k < 3; ++k) inten[k] = lgd::wave_uniform(inten[k]);
     double tau;
     bool dropped;
     double r[3] = {0.0, 0.0, 0.0};
@@ -1182,7 +1183,10 @@ __global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
     if (m > MS) return;
     const int tid = threadIdx.x;
     const int rg = tid / CG, cg = tid % CG;
-    if (stats && tid == 0) stat_add(stats, ST_SMALL_UPDATES, (unsigned long long)m * m * m);
+    if (stats && tid == 0) {
+        stat_add(stats, ST_SMALL_UPDATES, (unsigned long long)m * m * m);
+        if (MS <= 64) stat_add(stats, MS <= 32 ? ST_SMALL_UPDATES_32 : ST_SMALL_UPDATES_64, (unsigned long long)m * m * m);
+    }
     for (int a = tid; a < m; a += NT) mem[a] = member[a0 + a];
     for (int e = tid; e < MS * MS; e += NT) {
         const int i = e / MS, j = e - i * MS;
@@ -2022,26 +2026,31 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     const float e0 = (float)ext[0], e1 = (float)ext[1], e2 = (float)ext[2];
     const bool side = nbig > 0 && (n128 + n48 + n16) > 0;
     auto small_closures = [&]() -> int32_t {
-        ktimer_begin(ctx, "k_comp_small", &ev);
+        // three kernels, one per size class, each timed on its own
         if (n128 > 0) {
+            ktimer_begin(ctx, "k_comp_small_128", &ev);
             k_comp_small<kSmallComp><<<n128, 1024, 0, ctx->stream>>>(
                 pl.sd, pl.sid, pl.cls128, pl.comp_off, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, L, e0, e1, e2, out,
                 pl.stats_on ? pl.stats : nullptr);
             LG_LAUNCHED("k_comp_small<128>");
+            ktimer_end(ctx, "k_comp_small_128", ev);
         }
         if (n48 > 0) {
+            ktimer_begin(ctx, "k_comp_small_64", &ev);
             k_comp_small<64><<<n48, 256, 0, ctx->stream>>>(
                 pl.sd, pl.sid, pl.cls48, pl.comp_off, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, L, e0, e1, e2, out,
                 pl.stats_on ? pl.stats : nullptr);
             LG_LAUNCHED("k_comp_small<64>");
+            ktimer_end(ctx, "k_comp_small_64", ev);
         }
         if (n16 > 0) {
+            ktimer_begin(ctx, "k_comp_small_32", &ev);
             k_comp_small<32><<<n16, 64, 0, ctx->stream>>>(
                 pl.sd, pl.sid, pl.cls16, pl.comp_off, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, L, e0, e1, e2, out,
                 pl.stats_on ? pl.stats : nullptr);
             LG_LAUNCHED("k_comp_small<32>");
+            ktimer_end(ctx, "k_comp_small_32", ev);
         }
-        ktimer_end(ctx, "k_comp_small", ev);
         return LG_OK;
     };
     struct JoinGuard {

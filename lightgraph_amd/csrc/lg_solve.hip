@@ -619,6 +619,8 @@ int32_t lg_solver_stats(lg_ctx *ctx, const lg_solver *s, lg_solve_stats *out) {
     out->light_bricks = h[ST_LIGHT_BRICKS];
     out->edge_bricks = h[ST_EDGE_BRICKS];
     out->small_updates = h[ST_SMALL_UPDATES];
+    out->small_updates_64 = h[ST_SMALL_UPDATES_64];
+    out->small_updates_32 = h[ST_SMALL_UPDATES_32];
     return LG_OK;
 }
 

@@ -268,6 +268,8 @@ def test_solve_stats_counts(lga, ctx, volumes):
     assert st["edge_samples"] > 0
     small = sizes[(sizes >= 2) & (sizes <= 128)].astype(np.int64)
     assert st["small_updates"] == int((small ** 3).sum())
+    assert st["small_updates_32"] == int((small[small <= 32] ** 3).sum())
+    assert st["small_updates_64"] == int((small[(small > 32) & (small <= 64)] ** 3).sum())
     assert st["big_components"] == int((sizes > 128).sum())
     assert (st["fw_updates"] + st["fw_panel_updates"] > 0) == (st["big_components"] > 0)
     # whole 32-pivot chunks of 128 x 128 tile parts
