@@ -429,9 +429,11 @@ int32_t lg_fw_solve_batched(lg_ctx *ctx, int32_t count, const int32_t *h_n,
 /* Tile and schedule choices of the batched Floyd-Warshall, picked by a
  * measurement on the device (no reference counterpart: the reference runs one
  * scalar loop per solve).  split_below: panel and remainder launches of fewer
- * 128 x 128 tiles run two 128 x 64 column-half workgroups per panel tile and
- * four 64 x 64 quadrant workgroups per remainder tile (results are identical
- * either way).  use_graph: the rounds of a batch are captured once into a
+ * 128 x 128 tiles run split: a panel tile as two 128 x 64 column-half
+ * workgroups (four 128 x 32 column quarters when the launch has fewer than 128
+ * panel tiles) and a remainder tile as four 64 x 64 quadrant workgroups;
+ * split_below = 0 keeps one workgroup per tile everywhere (results are
+ * identical either way).  use_graph: the rounds of a batch are captured once into a
  * hipGraph and replayed while the schedule's shape repeats.  The first batched
  * closure on a device runs lg_fw_autotune unless the environment sets
  * LG_FW_AUTOTUNE=0 (then the defaults hold); LG_FW_SPLIT_BELOW and LG_FW_GRAPH

@@ -700,11 +700,12 @@ static int32_t fw_batched_run(lg_ctx *ctx, const std::vector<float *> &Ds, const
             if (timers) ktimer_end(ctx, "k_fw_pivot", e1);
             if (panel_blocks[kb] == 0) continue;
             if (timers) ktimer_begin(ctx, "k_fw_panel", &e2);
-            if (2 * panel_blocks[kb] < kQuarterPanelBelow)
+            const bool split_panel = panel_blocks[kb] < tc.split_below && panel_blocks[kb] < (1 << 29);
+            if (split_panel && 2 * panel_blocks[kb] < kQuarterPanelBelow)
                 // too few halves to fill the GPU: column quarters
                 k_fw_minplus_b<true, 4><<<4 * panel_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + n, n, kb,
                                                                                      d_flags, stats);
-            else if (panel_blocks[kb] < tc.split_below && panel_blocks[kb] < (1 << 29))
+            else if (split_panel)
                 k_fw_minplus_b<true, 2><<<2 * panel_blocks[kb], 256, 0, ctx->stream>>>(d_jobs, act, act + n, n, kb,
                                                                                      d_flags, stats);
             else

@@ -1,12 +1,18 @@
 // Host-side parameter construction mirroring the reference's Shader, ColorRamp,
 // light and CLI parsing code (Shader.h, Light.h, main.cc:214-319).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <sstream>
 #include <string>
+#include <unordered_set>
 #include <vector>
 
 #include "lightgraph.h"
+
+namespace lg {
+void set_error(const char *fmt, ...);  // the library's thread-local error message (lg_last_error)
+}
 
 namespace {
 
@@ -265,6 +271,131 @@ int32_t lg_light_parse(const char *text, lg_light *out) {
         return fail("bad light number");
     }
     return fail(f[0] + " is not a valid light type!");
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------- camera and splat ----
+// Pure host computations of the render and splat set-up; errors go to
+// lg_last_error like the device entry points'.
+namespace {
+int32_t arg_error(const char *msg) {
+    lg::set_error("%s", msg);
+    return LG_ERR_INVALID;
+}
+}  // namespace
+
+extern "C" {
+
+int32_t lg_camera_setup(const double rot[3], const double tr[3], double yrot, double focal, double aperture,
+                        double znear, double zfar, int32_t width, int32_t height, lg_camera *out) {
+    if (!rot || !tr || !out) return arg_error("lg_camera_setup: NULL argument");
+    if (width <= 0 || height <= 0) return arg_error("lg_camera_setup: image size must be positive");
+    if (!(znear > 0 && zfar > znear)) return arg_error("lg_camera_setup: need 0 < znear < zfar");
+    std::memset(out, 0, sizeof(*out));
+    // main.cc:98-102: rotation = camRotate + (0, camYRotate, 0),
+    // translation = camTranslate * yRotXform(camYRotate)
+    double Y[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    post_rotate(Y, 1, (yrot / 180.0) * M_PI);
+    double t[3];
+    for (int c = 0; c < 3; ++c) t[c] = tr[0] * Y[c] + tr[1] * Y[4 + c] + tr[2] * Y[8 + c] + Y[12 + c];
+    const double r[3] = {rot[0], rot[1] + yrot, rot[2]};
+    double M[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    // PerspectiveCamera ctor (Renderer.h:166-169)
+    post_rotate(M, 0, r[0] * M_PI / 180.0);
+    post_rotate(M, 1, r[1] * M_PI / 180.0);
+    post_rotate(M, 2, r[2] * M_PI / 180.0);
+    M[12] += t[0];
+    M[13] += t[1];
+    M[14] += t[2];
+    std::memcpy(out->screen_to_world, M, sizeof(M));
+    out->scale_width = 0.5 * aperture / focal;
+    out->scale_height = out->scale_width * double(height) / double(width);
+    out->znear = znear;
+    out->zfar = zfar;
+    out->width = width;
+    out->height = height;
+    return LG_OK;
+}
+
+// computeVoxelSize(points, 8) (used by LightGraph::buildVdbPointGrid,
+// LightGraph.h:713-714) restated: voxel volume = bbox volume / (count / 8),
+// refined by counting the occupied voxels of a cell-centred mask until the
+// estimate stops shrinking by more than 10%, truncated to 5 decimals.  Then the
+// splat radius of ScatterGrid.h:71-73.  Parity unpinned against OpenVDB.
+int32_t lg_splat_radius(const float *h_pts, int64_t count, int64_t kernel_total, double *radius_out,
+                        double *voxel_out) {
+    if (!h_pts || !radius_out || count < 0 || kernel_total < 0) return arg_error("lg_splat_radius: bad argument");
+    float vs = 0.1f;
+    if (count > 1) {
+        int64_t target = count / 8;
+        if (target == 0) target = 1;
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int64_t i = 0; i < count; ++i)
+            for (int c = 0; c < 3; ++c) {
+                lo[c] = std::min(lo[c], (double)h_pts[3 * i + c]);
+                hi[c] = std::max(hi[c], (double)h_pts[3 * i + c]);
+            }
+        double ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+        bool coincident = ext[0] <= 1e-7 && ext[1] <= 1e-7 && ext[2] <= 1e-7;
+        if (!coincident) {
+            double volume = ext[0] * ext[1] * ext[2];
+            if (volume <= 1e-7) {  // collinear or coplanar points
+                std::sort(ext, ext + 3);
+                volume = (ext[1] <= 1e-7) ? ext[2] * ext[2] * ext[2] : ext[2] * ext[2] * ext[1];
+            }
+            auto from_volume = [&](double vol, float &v) {
+                double vv = vol / (double)target;
+                bool ok = true;
+                if (vv < 3e-15) {
+                    vv = 3e-15;
+                    ok = false;
+                }
+                v = (float)std::pow(vv, 1.0 / 3.0);
+                return ok;
+            };
+            double prev_volume = volume;
+            if (from_volume(volume, vs)) {
+                int64_t prev_count = 0, vcount = 1;
+                std::unordered_set<uint64_t> cells;
+                while (vcount > prev_count) {
+                    cells.clear();
+                    const double inv = 1.0 / (double)vs;
+                    for (int64_t i = 0; i < count; ++i) {
+                        uint64_t key = 0;
+                        for (int c = 0; c < 3; ++c) {
+                            const int64_t q = (int64_t)std::floor((double)h_pts[3 * i + c] * inv + 0.5);
+                            key = key * 2097152ull + (uint64_t)((q + 1048576) & 2097151);
+                        }
+                        cells.insert(key);
+                    }
+                    prev_count = vcount;
+                    vcount = (int64_t)cells.size();
+                    volume = (double)(vs * vs * vs * (float)vcount);
+                    if (volume >= prev_volume) break;
+                    prev_volume = volume;
+                    const float prev_vs = vs;
+                    if (!from_volume(volume, vs)) {
+                        vs = prev_vs;
+                        break;
+                    }
+                    if (vs / prev_vs > 0.9f) break;
+                }
+            }
+            for (int d = 5; d < 11; ++d) {  // truncate for readability (5 decimals)
+                const double t = std::pow(10.0, d);
+                const float tv = (float)(std::floor((double)vs * t + 0.5) / t);
+                if (tv != 0.0f) {
+                    vs = tv;
+                    break;
+                }
+            }
+        }
+    }
+    const double v = (double)vs;
+    if (voxel_out) *voxel_out = v;
+    *radius_out = std::pow((3.0 * (double)kernel_total * v * v * v) / (32.0 * M_PI), 1.0 / 3.0);
+    return LG_OK;
 }
 
 }  // extern "C"

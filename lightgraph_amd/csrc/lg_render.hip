@@ -4,7 +4,6 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
-#include <unordered_set>
 #include <vector>
 
 #include <rocprim/device/device_scan.hpp>
@@ -13,37 +12,6 @@
 #include "lg_lights.h"
 
 namespace lg {
-
-// ------------------------------------------------------------- camera ------
-// OpenVDB Mat4 (row-major, row-vector convention): M = M * R(axis, angle).
-static void post_rotate(double M[16], int axis, double angle) {
-    const double c = cos(angle), s = sin(angle);
-    double R[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-    if (axis == 0) {
-        R[5] = c;
-        R[6] = s;
-        R[9] = -s;
-        R[10] = c;
-    } else if (axis == 1) {
-        R[0] = c;
-        R[2] = -s;
-        R[8] = s;
-        R[10] = c;
-    } else {
-        R[0] = c;
-        R[1] = s;
-        R[4] = -s;
-        R[5] = c;
-    }
-    double out[16];
-    for (int r = 0; r < 4; ++r)
-        for (int k = 0; k < 4; ++k) {
-            double acc = 0.0;
-            for (int m = 0; m < 4; ++m) acc += M[4 * r + m] * R[4 * m + k];
-            out[4 * r + k] = acc;
-        }
-    memcpy(M, out, sizeof(out));
-}
 
 // ------------------------------------------------------------ env map ------
 // Conditional CDFs, one lane per row, sequential FP64 prefix (Sampling.h:81-93
@@ -797,37 +765,6 @@ using namespace lg;
 
 extern "C" {
 
-int32_t lg_camera_setup(const double rot[3], const double tr[3], double yrot, double focal, double aperture,
-                        double znear, double zfar, int32_t width, int32_t height, lg_camera *out) {
-    LG_CHECK_ARG(rot && tr && out, "lg_camera_setup: NULL argument");
-    LG_CHECK_ARG(width > 0 && height > 0, "lg_camera_setup: image size must be positive");
-    LG_CHECK_ARG(znear > 0 && zfar > znear, "lg_camera_setup: need 0 < znear < zfar");
-    memset(out, 0, sizeof(*out));
-    // main.cc:98-102: rotation = camRotate + (0, camYRotate, 0),
-    // translation = camTranslate * yRotXform(camYRotate)
-    double Y[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-    post_rotate(Y, 1, (yrot / 180.0) * M_PI);
-    double t[3];
-    for (int c = 0; c < 3; ++c) t[c] = tr[0] * Y[c] + tr[1] * Y[4 + c] + tr[2] * Y[8 + c] + Y[12 + c];
-    const double r[3] = {rot[0], rot[1] + yrot, rot[2]};
-    double M[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
-    // PerspectiveCamera ctor (Renderer.h:166-169)
-    post_rotate(M, 0, r[0] * M_PI / 180.0);
-    post_rotate(M, 1, r[1] * M_PI / 180.0);
-    post_rotate(M, 2, r[2] * M_PI / 180.0);
-    M[12] += t[0];
-    M[13] += t[1];
-    M[14] += t[2];
-    memcpy(out->screen_to_world, M, sizeof(M));
-    out->scale_width = 0.5 * aperture / focal;
-    out->scale_height = out->scale_width * double(height) / double(width);
-    out->znear = znear;
-    out->zfar = zfar;
-    out->width = width;
-    out->height = height;
-    return LG_OK;
-}
-
 int32_t lg_envmap_create(lg_ctx *ctx, int32_t w, int32_t h, const float *h_rgb, lg_envmap **out) {
     LG_CHECK_ARG(ctx && h_rgb && out && w >= 2 && h >= 2, "lg_envmap_create: bad argument");
     *out = nullptr;
@@ -896,86 +833,6 @@ int32_t lg_envmap_read_cdf(lg_ctx *ctx, const lg_envmap *e, double *cond, double
     if (rowint) LG_HIP(hipMemcpyAsync(rowint, e->rowint, (size_t)e->height * 8, hipMemcpyDeviceToHost, ctx->stream));
     LG_HIP(hipStreamSynchronize(ctx->stream));
     if (total) *total = e->total;
-    return LG_OK;
-}
-
-// computeVoxelSize(points, 8) (used by LightGraph::buildVdbPointGrid,
-// LightGraph.h:713-714) restated: voxel volume = bbox volume / (count / 8),
-// refined by counting the occupied voxels of a cell-centred mask until the
-// estimate stops shrinking by more than 10%, truncated to 5 decimals.  Then the
-// splat radius of ScatterGrid.h:71-73.  Parity unpinned against OpenVDB.
-int32_t lg_splat_radius(const float *h_pts, int64_t count, int64_t kernel_total, double *radius_out,
-                        double *voxel_out) {
-    LG_CHECK_ARG(h_pts && radius_out && count >= 0 && kernel_total >= 0, "lg_splat_radius: bad argument");
-    float vs = 0.1f;
-    if (count > 1) {
-        int64_t target = count / 8;
-        if (target == 0) target = 1;
-        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-        for (int64_t i = 0; i < count; ++i)
-            for (int c = 0; c < 3; ++c) {
-                lo[c] = std::min(lo[c], (double)h_pts[3 * i + c]);
-                hi[c] = std::max(hi[c], (double)h_pts[3 * i + c]);
-            }
-        double ext[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
-        bool coincident = ext[0] <= 1e-7 && ext[1] <= 1e-7 && ext[2] <= 1e-7;
-        if (!coincident) {
-            double volume = ext[0] * ext[1] * ext[2];
-            if (volume <= 1e-7) {  // collinear or coplanar points
-                std::sort(ext, ext + 3);
-                volume = (ext[1] <= 1e-7) ? ext[2] * ext[2] * ext[2] : ext[2] * ext[2] * ext[1];
-            }
-            auto from_volume = [&](double vol, float &v) {
-                double vv = vol / (double)target;
-                bool ok = true;
-                if (vv < 3e-15) {
-                    vv = 3e-15;
-                    ok = false;
-                }
-                v = (float)pow(vv, 1.0 / 3.0);
-                return ok;
-            };
-            double prev_volume = volume;
-            if (from_volume(volume, vs)) {
-                int64_t prev_count = 0, vcount = 1;
-                std::unordered_set<uint64_t> cells;
-                while (vcount > prev_count) {
-                    cells.clear();
-                    const double inv = 1.0 / (double)vs;
-                    for (int64_t i = 0; i < count; ++i) {
-                        uint64_t key = 0;
-                        for (int c = 0; c < 3; ++c) {
-                            const int64_t q = (int64_t)floor((double)h_pts[3 * i + c] * inv + 0.5);
-                            key = key * 2097152ull + (uint64_t)((q + 1048576) & 2097151);
-                        }
-                        cells.insert(key);
-                    }
-                    prev_count = vcount;
-                    vcount = (int64_t)cells.size();
-                    volume = (double)(vs * vs * vs * (float)vcount);
-                    if (volume >= prev_volume) break;
-                    prev_volume = volume;
-                    const float prev_vs = vs;
-                    if (!from_volume(volume, vs)) {
-                        vs = prev_vs;
-                        break;
-                    }
-                    if (vs / prev_vs > 0.9f) break;
-                }
-            }
-            for (int d = 5; d < 11; ++d) {  // truncate for readability (5 decimals)
-                const double t = pow(10.0, d);
-                const float tv = (float)(floor((double)vs * t + 0.5) / t);
-                if (tv != 0.0f) {
-                    vs = tv;
-                    break;
-                }
-            }
-        }
-    }
-    const double v = (double)vs;
-    *voxel_out = v;
-    *radius_out = pow((3.0 * (double)kernel_total * v * v * v) / (32.0 * M_PI), 1.0 / 3.0);
     return LG_OK;
 }
 

@@ -317,10 +317,12 @@ class LightGraph {
         sp.radius = radius();
         sp.seed = opts_.seed;
         sp.env_seed = opts_.envSeed;
+        sp.timing = opts_.verbose ? 1 : 0;  // stage events (the stages then run one after another)
         const lg_march_params mp = opts_.march();
         check(lg_solve_batch(ctx.get(), solver, items.data(), (int32_t)count, &shader_.params(),
                              opts_.light.data(), (int32_t)opts_.light.size(), &mp, &sp),
               "lg_solve_batch");
+        if (opts_.verbose) printStageTimes(solver, count);
         lg_solver_buffers b{};
         check(lg_solver_get(solver, &b), "lg_solver_get");
         std::vector<float> pts(total * 3), dif(total * 3);
@@ -338,6 +340,24 @@ class LightGraph {
         }
         if (opts_.geoDump)
             for (size_t k = 0; k < count; ++k) dumpGeo(begin + k);
+    }
+
+    // The per-stage lines of -v (LightGraph.h:268-272, 312-316, 412-416,
+    // 499-503, 536-540, 581-585), once for the whole batch of solves: the
+    // stages run as one launch sequence for every solve of the range.  The
+    // shortest paths and the diffusion are one stage here (the closures of the
+    // components carry the diffusion with them), reported on the first line.
+    void printStageTimes(const lg_solver *solver, size_t count) const {
+        lg_solve_timing t{};
+        check(lg_solver_timing(solver, &t), "lg_solver_timing");
+        const std::string of = count > 1 ? " (" + std::to_string(count) + " solves)" : "";
+        std::cout << "Point Cloud Creation: " << t.sample_ms * 1e-3 << " sec" << of << std::endl;
+        std::cout << "Graph Creation: " << t.graph_ms * 1e-3 << " sec" << of << std::endl;
+        std::cout << "Raymarch Graph Edges: " << t.march_ms * 1e-3 << " sec" << of << std::endl;
+        std::cout << "Vertex Lighting Calculation: " << t.light_ms * 1e-3 << " sec" << of << std::endl;
+        std::cout << "Shortest Path Finding: " << (t.fw_ms + t.diffuse_ms) * 1e-3 << " sec" << of
+                  << " (with the radiance diffusion)" << std::endl;
+        std::cout << "Radiance Diffusion: 0 sec (done with the shortest paths)" << std::endl;
     }
 
     // LightGraph::outputDebugGeo (LightGraph.h:604-696): five comma-separated
@@ -378,6 +398,8 @@ class LightGraph {
               "lg_solve");
         lg_solver_buffers b{};
         check(lg_solver_get(solver, &b), "lg_solver_get");
+        if (!b.d_D || b.ld < (int32_t)n || b.n != (int32_t)n)
+            throw std::runtime_error("geoDump: the solve returned no dense optical depths");
         std::vector<float> pts((size_t)n * 3), L((size_t)n * 3), dif((size_t)n * 3), Dm((size_t)n * b.ld);
         std::vector<int32_t> edges((size_t)b.n_edges * 2);
         check(lg_memcpy_d2h(ctx.get(), pts.data(), b.d_points, pts.size() * 4), "d2h points");
@@ -455,9 +477,11 @@ class LightGraph {
             sp.env_seed = opts_.envSeed;
             sp.solve = (uint32_t)(begin + k);
             sp.external_points = 1;
+            sp.timing = opts_.verbose ? 1 : 0;
             check(lg_solve(ctx.get(), solver, grid_.get(), temp_ ? temp_->get() : nullptr, &shader_.params(),
                            opts_.light.data(), (int32_t)opts_.light.size(), &mp, &sp),
                   "lg_solve");
+            if (opts_.verbose) printStageTimes(solver, 1);
             std::vector<float> dif(3 * n);
             check(lg_memcpy_d2h(ctx.get(), dif.data(), b.d_diffuse, dif.size() * 4), "d2h diffuse");
             for (size_t v = 0; v < n; ++v) {

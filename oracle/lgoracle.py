@@ -11,7 +11,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liblgoracle.so")
+# LG_ORACLE_LIB: another build of the same source (the sanitizer test uses one)
+LIB_PATH = os.environ.get("LG_ORACLE_LIB") or os.path.join(HERE, "liblgoracle.so")
 _LIB = None
 
 P = C.c_void_p
