@@ -1,8 +1,9 @@
 // Command line of the LightGraph renderer (main.cc of the reference): the same
 // options and help aliases, the stages in the same order with the same stage
-// timing lines, on the MI355X path.  Additions: -light point:..., procedural
-// scenes (-scene, -sceneSize, -sceneSeed), graph size and radius (-points,
-// -radius, -maxConnect) and brick precision (-precision).
+// timing lines, on the MI355X path.  -vdbIn reads the density and temperature
+// FloatGrids of a .vdb file (host/vdb.cpp) into dense bricks.  Additions:
+// -light point:..., procedural scenes (-scene, -sceneSize, -sceneSeed), graph
+// size and radius (-points, -radius, -maxConnect) and brick precision (-precision).
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -10,6 +11,7 @@
 #include <sstream>
 
 #include "lightgraph.hpp"
+#include "vdb.hpp"
 
 using namespace lightgraph;
 using std::cout;
@@ -20,7 +22,7 @@ namespace {
 
 void printHelp() {
     const char *lines[][3] = {
-        {"-vdbIn", "Input vdb file to read and render (not read by this library: use -scene)", "string"},
+        {"-vdbIn", "Input vdb file to read and render", "string"},
         {"-exrOut", "Output exr file to write to disk", "string"},
         {"-camPos", "Camera position", "float,float,float"},
         {"-camRot", "Camera rotation", "float,float,float"},
@@ -153,14 +155,21 @@ int main(int argc, char *argv[]) {
                 return 1;
             }
         }
-        if (scene.empty()) {
-            if (!vdbFilename.empty())
-                cout << "-vdbIn: this library reads no vdb files; use -scene sphere|blobs|plume" << endl;
-            else
-                cout << "no input: give -scene sphere|blobs|plume" << endl;
+        if (scene.empty() && vdbFilename.empty()) {
+            cout << "no input: give -vdbIn file.vdb or -scene sphere|blobs|plume" << endl;
             return 1;
         }
-        if (imgFilename.empty()) imgFilename = scene + ".exr";
+        if (!scene.empty() && !vdbFilename.empty()) {
+            cout << "give either -vdbIn or -scene, not both" << endl;
+            return 1;
+        }
+        // the dumps' and default image's stem: the vdb file's stem, or the scene's name
+        string stem = scene;
+        if (!vdbFilename.empty()) {
+            stem = vdbFilename.substr(vdbFilename.find_last_of('/') + 1);
+            if (stem.size() > 4 && stem.compare(stem.size() - 4, 4, ".vdb") == 0) stem.resize(stem.size() - 4);
+        }
+        if (imgFilename.empty()) imgFilename = stem + ".exr";
         if (opts.sampler != "density" && opts.sampler != "dart") {
             cout << "-sampler: " << opts.sampler << " is not density or dart" << endl;
             return 1;
@@ -186,10 +195,16 @@ int main(int argc, char *argv[]) {
         } else if (scene == "plume") {
             spec.kind = LG_SCENE_PLUME;
             spec.with_temperature = 1;
-        } else {
+        } else if (!scene.empty()) {
             throw Error("-scene must be sphere, blobs or plume", LG_ERR_INVALID);
         }
-        opts.geoName = scene;  // the dumps' stem (the reference uses the vdb file's stem)
+        opts.geoName = stem;
+        // LOAD INPUT GRIDS (main.cc:484-502): the file is read before the device is set up
+        VdbVolume vdb;
+        if (!vdbFilename.empty()) {
+            cout << "Reading in: " << vdbFilename << endl;
+            vdb = readVdbVolume(vdbFilename);
+        }
         Context ctx(0);
         // lights: dir/point parse on the host; env needs its map on the device
         std::vector<std::unique_ptr<EnvMap>> envMaps;
@@ -221,8 +236,31 @@ int main(int argc, char *argv[]) {
             opts.light.push_back(l);
         }
 
-        cout << "Generating: " << scene << " " << sceneSize << "^3 (" << precision << " bricks)" << endl;
-        auto grids = DenseBrickGrid::procedural(ctx, spec);
+        std::pair<DenseBrickGrid, std::unique_ptr<DenseBrickGrid>> grids{DenseBrickGrid(ctx, nullptr), nullptr};
+        if (!vdbFilename.empty()) {
+            const VdbFloatGrid &d = vdb.density;
+            const Vec3d origin(d.translation[0] + d.voxel_size * d.dense_min[0],
+                               d.translation[1] + d.voxel_size * d.dense_min[1],
+                               d.translation[2] + d.voxel_size * d.dense_min[2]);
+            grids.first = DenseBrickGrid::fromDense(ctx, d.dims[0], d.dims[1], d.dims[2], d.dense.data(), d.voxel_size,
+                                                    origin, spec.dtype);
+            if (vdb.has_temperature)
+                grids.second.reset(new DenseBrickGrid(DenseBrickGrid::fromDense(
+                    ctx, d.dims[0], d.dims[1], d.dims[2], vdb.temperature.data(), d.voxel_size, origin, LG_F32)));
+            if (opts.verbose) {
+                cout << "Input Density Grid Voxel size: " << d.voxel_size << endl;
+                cout << "Input Density Grid: " << d.dims[0] << "x" << d.dims[1] << "x" << d.dims[2] << " voxels from "
+                     << d.leaves << " leaves, " << (d.empty ? "no active voxel" : "active box") << " [" << d.dense_min[0]
+                     << "," << d.dense_min[1] << "," << d.dense_min[2] << "]" << endl;
+                if (vdb.has_temperature)
+                    cout << "Input Temperature Grid: " << (vdb.temperature_resampled ? "resampled" : "same transform")
+                         << endl;
+            }
+            vdb = VdbVolume();  // the host copies are on the device now
+        } else {
+            cout << "Generating: " << scene << " " << sceneSize << "^3 (" << precision << " bricks)" << endl;
+            grids = DenseBrickGrid::procedural(ctx, spec);
+        }
         DenseBrickGrid &grid = grids.first;
         const DenseBrickGrid *tempGrid = grids.second.get();
 

@@ -43,7 +43,9 @@ def test_no_input_fails():
     r = _run("-resWidth", "16")
     assert r.returncode == 1
     r = _run("-vdbIn", "/nonexistent.vdb")
-    assert r.returncode == 1 and "-scene" in r.stdout
+    assert r.returncode == 1 and "vdb: cannot open /nonexistent.vdb" in r.stderr
+    r = _run("-vdbIn", "/nonexistent.vdb", "-scene", "sphere")
+    assert r.returncode == 1 and "either -vdbIn or -scene" in r.stdout
 
 
 def test_bad_values_fail_cleanly():

@@ -10,6 +10,8 @@ the address, undefined-behaviour and leak sanitizers:
   csrc/lg_host.cpp) end to end on the fake device, sanitized: env light read
   from an EXR, point and directional lights, ramp, -v, -geoDump, the 13-channel
   EXR written and read back, the dart sampler, help and option errors;
+* -vdbIn: density and temperature read from a .vdb file (host/vdb.cpp) through
+  the same front end, sanitized;
 * the oracle (oracle/lgoracle.cpp), sanitized, running the oracle's CPU tests.
 
 Builds go to a temporary directory; g++ compiles the host sources directly."""
@@ -62,8 +64,9 @@ def test_server_admission_and_failures(tmp_path, sanitized):
 def cli_san(tmp_path_factory):
     d = tmp_path_factory.mktemp("cli")
     src = [os.path.join(HOST, "fake_abi.cpp"), os.path.join(ROOT, "lightgraph_amd", "csrc", "lg_host.cpp"),
-           os.path.join(ROOT, "lightgraph_amd", "host", "main.cc"), os.path.join(ROOT, "lightgraph_amd", "host", "exr.cpp")]
-    return _build(d / "lightgraph_san", src, SAN, libs=["-lz"])
+           os.path.join(ROOT, "lightgraph_amd", "host", "main.cc"), os.path.join(ROOT, "lightgraph_amd", "host", "exr.cpp"),
+           os.path.join(ROOT, "lightgraph_amd", "host", "vdb.cpp")]
+    return _build(d / "lightgraph_san", src, SAN, libs=["-lz", "-ldl"])
 
 
 def _clean(r):
@@ -128,3 +131,41 @@ def test_oracle_under_sanitizers(tmp_path):
     out = r.stdout + r.stderr
     assert r.returncode == 0, out[-4000:]
     assert "runtime error" not in out and "ERROR: AddressSanitizer" not in out
+
+
+def test_cli_vdb_input_under_sanitizers(cli_san, tmp_path):
+    """-vdbIn: density + temperature FloatGrids read from a .vdb file
+    (tests/vdbio.py writes it), the stages run, the image and the geometry dumps
+    are named after the file's stem; a missing density grid and a missing file
+    are errors."""
+    sys.path.insert(0, os.path.dirname(__file__))
+    import exrio
+    import vdbio
+    rng = np.random.default_rng(2)
+    z, y, x = np.mgrid[0:24, 0:24, 0:24]
+    r = np.sqrt((x - 11.5) ** 2 + (y - 11.5) ** 2 + (z - 11.5) ** 2)
+    dens = np.where(r < 10, 0.5 + 0.5 * rng.random(r.shape), 0.0).astype(np.float32)
+    temp = np.where(r < 10, 800.0 * (1 - r / 10), 0.0).astype(np.float32)
+    g = vdbio.Grid.from_dense("density", dens, dens > 0, (-12, -12, -12), voxel_size=0.5, translation=(1, 2, 3))
+    t = vdbio.Grid.from_dense("temperature", temp, temp > 0, (-12, -12, -12), voxel_size=0.5, translation=(1, 2, 3))
+    path = vdbio.write(str(tmp_path / "ball.vdb"), [g, t])
+    ramp = "0.0,0.33,0.66,1.0/0,0,0:1.0,0.076,0.0:1.0,0.322,0.0:1.0,0.45,0.15/0.35/1.5"
+    r = _run([cli_san, "-vdbIn", path, "-points", "200", "-iter", "2", "-resWidth", "24", "-resHeight", "16",
+              "-camPos", "1,2,30", "-focal", "50", "-light", "dir:0:1,1,1:1:-45,30,0", "-tempColorRamp", ramp,
+              "-v", "-geoDump"], cwd=str(tmp_path))
+    out = _clean(r)
+    assert r.returncode == 0, out[-4000:]
+    assert f"Reading in: {path}" in out and "Input Density Grid Voxel size: 0.5" in out
+    assert "Input Temperature Grid: same transform" in out
+    chans, _ = exrio.read(str(tmp_path / "ball.exr"))
+    assert chans["R"].shape == (16, 24)
+    assert all(f.startswith("ball_") for f in os.listdir(tmp_path / "geoDump"))
+    # no density grid; no file; both inputs
+    only_t = vdbio.write(str(tmp_path / "t.vdb"), [t])
+    for args in (["-vdbIn", only_t], ["-vdbIn", str(tmp_path / "none.vdb")],
+                 ["-vdbIn", path, "-scene", "sphere"]):
+        r = _run([cli_san, *args, "-resWidth", "8", "-resHeight", "8", "-exrOut", str(tmp_path / "x.exr")],
+                 cwd=str(tmp_path))
+        _clean(r)
+        assert r.returncode != 0, args
+    assert not os.path.exists(tmp_path / "x.exr")
