@@ -333,9 +333,11 @@ def bench_serving(lga, ctx, rates, jobs_per_rate, workers, max_batch, seed=5):
     [1000, 16000], images 512^2 / 1280x720 / 1920x1080 at 1-4 spp, one of three
     light sets (directional, point, environment); one LightGraph solve per job.
     Latency = arrival -> films on the host.  The served runs close the graphs in
-    nested-dissection pivot order; the same job list is also run through a
-    one-worker, one-job-per-batch server in the ascending (drop-in) order: the
-    plain path, back to back."""
+    nested-dissection pivot order; the same job list is also run, in the
+    ascending (drop-in) pivot order, through a one-worker, one-job-per-batch
+    server ("plain": volumes, solver and buffers stay resident) and job by job
+    the way one command-line run does it ("dropin": everything built and freed
+    per job), back to back."""
     import numpy as np
     rng = np.random.default_rng(seed)
     sides = [int(x) for x in rng.integers(0, (768 - 128) // 64 + 1, 12) * 64 + 128]
@@ -368,11 +370,13 @@ def bench_serving(lga, ctx, rates, jobs_per_rate, workers, max_batch, seed=5):
         srv = lga.Server(160 << 30, workers=nworkers, max_batch=nbatch)
         t0 = time.perf_counter()
         ids = []
-        for j, a in zip(jobs, arrivals):
+        for k, (j, a) in enumerate(zip(jobs, arrivals)):
             dt = a - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
             ids.append(srv.submit(j))
+            if k % 50 == 49:
+                print(f"serving: {k + 1} of {len(jobs)} jobs submitted", file=sys.stderr, flush=True)
         rs = [srv.wait(i) for i in ids]
         st = srv.stats()
         srv.close()
@@ -390,12 +394,46 @@ def bench_serving(lga, ctx, rates, jobs_per_rate, workers, max_batch, seed=5):
         jobs = make_jobs(jobs_per_rate, "nd")
         arrivals = np.cumsum(rng.exponential(1.0 / lam, jobs_per_rate))
         out["rates"][str(lam)] = run(jobs, arrivals, workers, max_batch)
+    def dropin(jobs):
+        """Each job the way one command-line run does it, back to back: build the
+        volume, a solver, the scatter grid and the films, solve, splat, render,
+        read the films back, free everything."""
+        t0 = time.perf_counter()
+        for j in jobs:
+            grid, temp = lga.Grid.generate(ctx, j.scene)
+            lights = [j.lights[k] for k in range(j.n_lights)]
+            n = j.points
+            s = lga.Solver(ctx, n)
+            s.solve(grid, j.shader, lights, j.march, n=n, radius=j.radius, seed=j.seed, solve=j.solve_base,
+                    max_connect=j.max_connect, temperature=temp, env_seed=j.env_seed)
+            r = s.results(want_D=False)
+            radius, _ = lga.splat_radius(r["points"], j.pts_per_kernel)
+            sg = lga.ScatterGrid(ctx, grid, j.scatter_res_factor)
+            pb, rb = ctx.upload(r["points"]), ctx.upload(r["diffuse"])
+            sg.splat(pb, rb, len(r["points"]), radius)
+            films = lga.Films(ctx, j.camera.width, j.camera.height)
+            lga.render(ctx, grid, sg, temp, j.camera, j.shader, lights, j.march, j.samples, films, seed=j.render_seed,
+                       env_seed=j.env_seed)
+            films.read()
+            for x in (films, pb, rb):
+                x.free()
+            sg.destroy()
+            s.destroy()
+            grid.destroy()
+            if temp is not None:
+                temp.destroy()
+        ctx.sync()
+        return {"jobs": len(jobs), "solves_per_s": round(len(jobs) / (time.perf_counter() - t0), 3)}
+
     state = rng.bit_generator.state
     jobs = make_jobs(jobs_per_rate, "nd")
     out["saturated"] = run(jobs, np.zeros(len(jobs)), workers, max_batch)
     rng.bit_generator.state = state  # the same jobs in the drop-in pivot order
-    out["plain"] = run(make_jobs(jobs_per_rate, "reference"), np.zeros(len(jobs)), 1, 1)
+    plain_jobs = make_jobs(jobs_per_rate, "reference")
+    out["plain"] = run(plain_jobs, np.zeros(len(jobs)), 1, 1)
+    out["dropin"] = dropin(plain_jobs)
     out["served_vs_plain"] = round(out["saturated"]["solves_per_s"] / out["plain"]["solves_per_s"], 3)
+    out["served_vs_dropin"] = round(out["saturated"]["solves_per_s"] / out["dropin"]["solves_per_s"], 3)
     env.destroy()
     return out
 
