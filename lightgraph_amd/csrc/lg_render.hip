@@ -550,45 +550,52 @@ __device__ __forceinline__ bool render_shadow(const lgd::GridView &g, const Rend
 
 // Phase shadow: task t = light * P_pad + point (P_pad a multiple of 64, so a
 // wave holds one light); out[3 t ..] = the sum over the light's samples of
-// intensity * transmittance (Renderer.h:609-643).
+// intensity * transmittance (Renderer.h:609-643).  The intensity of the sample
+// being marched and the running sum wait in LDS during the march, which leaves
+// registers for five waves per SIMD.
 template <typename T, bool kCount>
-__global__ void __launch_bounds__(256, 4) k_render_shadow(lgd::GridView g, RenderConsts c, lgl::DevLights L,
+__global__ void __launch_bounds__(256, 5) k_render_shadow(lgd::GridView g, RenderConsts c, lgl::DevLights L,
                                                        uint32_t k0, uint32_t k1, uint32_t ek0, uint32_t ek1,
                                                        const ShadePoint *__restrict__ pts, uint32_t P,
                                                        uint32_t P_pad, double *__restrict__ out,
                                                        unsigned long long *__restrict__ stats) {
+    __shared__ double s_inten[3 * 256], s_sum[3 * 256];
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     const int li = __builtin_amdgcn_readfirstlane((int)(t / P_pad));
     const uint32_t q = t - (uint32_t)li * P_pad;
+    double *const inten = s_inten + 3 * threadIdx.x;
+    double *const sum = s_sum + 3 * threadIdx.x;
     uint32_t rays = 0, smp = 0;
     if (li < L.n && q < P) {
         const lgl::DevLight &lt = L.l[li];
-        const ShadePoint sp = pts[q];
-        const double wp[3] = {g.origin[0] + g.voxel * sp.ip[0], g.origin[1] + g.voxel * sp.ip[1],
-                              g.origin[2] + g.voxel * sp.ip[2]};
-        const uint32_t pix = sp.ray / (uint32_t)c.samples;
-        const uint32_t tag = shadow_tag(sp.ray - pix * (uint32_t)c.samples, li);
+        const uint32_t ray = pts[q].ray, ord = pts[q].ord;
+        const uint32_t pix = ray / (uint32_t)c.samples;
+        const uint32_t tag = shadow_tag(ray - pix * (uint32_t)c.samples, li);
         lgd::Stream rng;
-        rng.init(pix, sp.ord, lgd::DOM_RENDER_SHADOW | tag, ((uint64_t)k1 << 32) | k0);
-        double tl[3] = {0, 0, 0};
+        rng.init(pix, ord, lgd::DOM_RENDER_SHADOW | tag, ((uint64_t)k1 << 32) | k0);
+        for (int a = 0; a < 3; ++a) sum[a] = 0.0;
         for (int ls = 0; ls < lt.samples; ++ls) {
-            double ldir[3], inten[3], dist;
+            const double wp[3] = {g.origin[0] + g.voxel * pts[q].ip[0], g.origin[1] + g.voxel * pts[q].ip[1],
+                                  g.origin[2] + g.voxel * pts[q].ip[2]};
+            double ldir[3], dist;
             {
                 // an env sample draws 2 values: sample ls starts at draw 2 ls
                 lgd::Stream env;
-                env.init(pix, sp.ord, lgd::DOM_RENDER_ENV | tag, ((uint64_t)ek1 << 32) | ek0);
+                env.init(pix, ord, lgd::DOM_RENDER_ENV | tag, ((uint64_t)ek1 << 32) | ek0);
                 if (lt.type == LG_LIGHT_ENV && ls > 0) env.skip(2u * (uint32_t)ls);
-                lgl::illuminate(lt, wp, env, ldir, inten, dist);
+                double in[3];
+                lgl::illuminate(lt, wp, env, ldir, in, dist);
+                for (int a = 0; a < 3; ++a) inten[a] = in[a];
             }
             double tau_s;
             bool sdrop;
             ++rays;
             if (!render_shadow<T, kCount>(g, c, wp, ldir, dist, rng, &tau_s, &sdrop, &smp)) continue;
             if (sdrop) continue;
-            for (int a = 0; a < 3; ++a) tl[a] += inten[a] * (double)expf((float)(c.ext[a] * tau_s));
+            for (int a = 0; a < 3; ++a) sum[a] += inten[a] * (double)expf((float)(c.ext[a] * tau_s));
         }
         double *o = out + 3ull * t;
-        for (int a = 0; a < 3; ++a) o[a] = tl[a];
+        for (int a = 0; a < 3; ++a) o[a] = sum[a];
     }
     if (stats) {
         render_stat(stats, RS_SHADOW_RAYS, rays);
