@@ -35,14 +35,17 @@ KERNELS = ("k_light_rays", "k_march_edges", "k_fw_minplus", "k_fw_panel", "k_fw_
 STAGES = ("k_vertex_light", "k_graph", "k_cand", "k_greedy", "k_components", "k_union_find", "k_comp_sort")
 PEAK_HBM_GBS = 8000.0
 PEAK_FP32_TFLOPS = 157.3
-# VALU issue ceiling without packed FP32: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64
-# instruction per SIMD every 4 cycles = 614.4e9 wave instructions/s = 39.3e12 lane
-# instructions/s.  A min-plus update costs 1.5 of them (two adds and one
-# v_min3_f32 per pivot pair), so Floyd-Warshall tops out at 26.2e12 updates/s =
-# 52.4 T ops/s (add + min).
-VALU_LANE_INSTR_PER_S = 256 * 4 * 2.4e9 * 16
+# VALU issue ceiling for 32-bit operations: 256 CUs x 4 SIMDs x 2.4 GHz, a wave64
+# FP32/INT32 instruction every 2 cycles per SIMD (32 lanes a cycle: the 157.3
+# TFLOP/s FP32 peak is 256 x 4 x 32 lanes x 2 x 2.4 GHz) = 1228.8e9 wave
+# instructions/s = 78.6e12 lane instructions/s; FP64 runs at half that.  The
+# dense min-plus kernel measures 8.93e11 wave instructions/s under SQ_INSTS_VALU,
+# above what a 4-cycle issue would allow.  A min-plus update costs 1.5 32-bit
+# instructions (two adds and one v_min3_f32 per pivot pair), so Floyd-Warshall
+# tops out at 52.4e12 updates/s = 104.9 T ops/s (add + min).
+VALU_LANE_INSTR_PER_S = 256 * 4 * 2.4e9 * 32
 FW_OPS_CEILING_TFLOPS = VALU_LANE_INSTR_PER_S / 1.5 * 2 / 1e12
-# Marches (SURVEY 8(d)): t_roof = max(B_comp / 8 TB/s, S * 32 / 157.3 TFLOP/s) with
+# Marches: t_roof = max(B_comp / 8 TB/s, S * 32 / 157.3 TFLOP/s) with
 # S trilinear samples at 32 FP32 ops each and B_comp the compulsory bytes: the
 # distinct 8^3 bricks the samples' taps read (2 KiB each in FP32) plus inputs and
 # outputs.  kernel -> (sample counter, brick counter, input + output bytes)
@@ -261,6 +264,9 @@ def bench_render(lga, ctx, grid, shader, lights, solver, n, reps):
                 "avg_launch_ms": round(sh["ms"] / sh["launches"], 4)}
         roof.update(march_roofline(st["shadow_samples"], bricks, io, sh["ms"] * 1e-3))
         roof["shadow_samples_per_s"] = round(st["shadow_samples"] / (sh["ms"] * 1e-3), 1)
+        # what the counters show holds it (profiles/k_render_c2_1080p_kernel_trace.md):
+        # neither term of t_roof but the latency of dependent loads
+        roof["limiter"] = "latency: dependent brick-code and tap loads, 5 waves/SIMD, 0.58 lane efficiency"
         out["roofline"] = roof
     return out
 
@@ -525,10 +531,10 @@ def march_roofline(samples, bricks, io_bytes, t_s):
 
 def kernel_roofline(times, work, workload):
     """Roofline object for the kernel with the largest total time among KERNELS,
-    over the launches of the timed region: the marches by SURVEY 8(d)'s formula
-    (compulsory bytes or trilinear samples, whichever bounds), the Floyd-Warshall
-    kernels by executed min-plus updates (2 ops each) against the FP32 peak and
-    the no-packed-math VALU issue ceiling.  Work comes from the device counters
+    over the launches of the timed region: the marches by t_roof = max(B_comp / 8
+    TB/s, S * 32 / 157.3 TFLOP/s) (compulsory bytes or trilinear samples,
+    whichever bounds), the Floyd-Warshall kernels by executed min-plus updates
+    (2 ops each) against the FP32 peak and the 32-bit VALU issue ceiling.  Work comes from the device counters
     of the same solves."""
     cands = [(t["total_ms"], nm) for nm, t in times.items() if nm in KERNELS]
     if not cands:
