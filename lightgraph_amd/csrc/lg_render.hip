@@ -575,8 +575,13 @@ __global__ void __launch_bounds__(256, 5) k_render_shadow(lgd::GridView g, Rende
         rng.init(pix, ord, lgd::DOM_RENDER_SHADOW | tag, ((uint64_t)k1 << 32) | k0);
         for (int a = 0; a < 3; ++a) sum[a] = 0.0;
         for (int ls = 0; ls < lt.samples; ++ls) {
-            const double wp[3] = {g.origin[0] + g.voxel * pts[q].ip[0], g.origin[1] + g.voxel * pts[q].ip[1],
-                                  g.origin[2] + g.voxel * pts[q].ip[2]};
+            // the position is re-read each light sample and kept opaque, so that
+            // the compiler does not hoist the ray set-up's per-point values out
+            // of this loop and hold them (in spills) across the march
+            double ip[3] = {pts[q].ip[0], pts[q].ip[1], pts[q].ip[2]};
+            asm volatile("" : "+v"(ip[0]), "+v"(ip[1]), "+v"(ip[2]));
+            const double wp[3] = {g.origin[0] + g.voxel * ip[0], g.origin[1] + g.voxel * ip[1],
+                                  g.origin[2] + g.voxel * ip[2]};
             double ldir[3], dist;
             {
                 // an env sample draws 2 values: sample ls starts at draw 2 ls
