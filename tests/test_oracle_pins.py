@@ -63,6 +63,28 @@ def test_env_cdf_separable_map(oracle):
     np.testing.assert_allclose(marg, want, rtol=0, atol=1e-6)
 
 
+def test_env_cdf_luminance_weights(oracle):
+    """Channels that vary differently: R = 1 + u, G = 2, B = 3 (u the column).
+    The luminance is Y(u) = 0.2126 (1 + u) + 0.7152 * 2 + 0.0722 * 3 in every
+    row, so each weighted row's CDF is the prefix of Y over columns divided by
+    its total (the row's sin(theta) weight cancels), and the marginal is the
+    plain sin(theta) prefix of the constant map."""
+    h, w = 12, 20
+    u = np.arange(w, dtype=np.float64)
+    rgb = np.empty((h, w, 3), np.float32)
+    rgb[..., 0] = (1.0 + u)[None, :]
+    rgb[..., 1] = 2.0
+    rgb[..., 2] = 3.0
+    cond, marg = oracle.env_cdf(rgb)
+    y = 0.2126 * (1.0 + u) + 0.7152 * 2.0 + 0.0722 * 3.0
+    want_row = np.concatenate([[0.0], np.cumsum(y)]) / y.sum()
+    for v in range(1, h - 1):
+        np.testing.assert_allclose(cond[v], want_row, rtol=0, atol=1e-6)
+    a = math.pi / (h - 1)
+    want = np.array([_sine_prefix(i, a) / _sine_prefix(h, a) for i in range(h + 1)])
+    np.testing.assert_allclose(marg, want, rtol=0, atol=1e-6)
+
+
 def test_env_cdf_transposed_map_differs(oracle):
     """Rows are latitude (sin weighted) and columns longitude: a map that is
     bright in one column keeps a uniform marginal shape over rows, one bright
