@@ -220,6 +220,67 @@ __global__ void k_cell_keys(const CellDesc *__restrict__ cd, const int *__restri
     vals[v] = v;
 }
 
+// The neighbour grid of a small call in one workgroup: a counting sort of the
+// points by cell (LDS counters, ranks within a cell from the atomics), the
+// cell ranges, and the sorted point list.  Points keep no order inside a cell:
+// k_cand reads whole x-runs of cells and sorts each row itself.
+constexpr int kCellSortCells = 32768;  // LDS counters (128 KiB)
+constexpr int kCellSortMaxV = 65536;   // ranks in 16 bits
+
+__global__ void __launch_bounds__(1024) k_cell_sort_small(const CellDesc *__restrict__ cd, const int *__restrict__ sid,
+                                                          int V, const float *__restrict__ pts, int cells,
+                                                          uint32_t *__restrict__ packed, int *__restrict__ cstart,
+                                                          int *__restrict__ cend, int *__restrict__ svals,
+                                                          int *__restrict__ zero_word) {
+    __shared__ int cnt[kCellSortCells];
+    __shared__ int wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) *zero_word = 0;  // the caller's count slot after the last vertex
+    for (int k = tid; k < cells; k += 1024) cnt[k] = 0;
+    __syncthreads();
+    for (int v = tid; v < V; v += 1024) {
+        const CellDesc c = cd[sid[v]];
+        const int x = cell_axis((double)pts[3 * v], c.lo[0], c.inv[0], c.n[0]);
+        const int y = cell_axis((double)pts[3 * v + 1], c.lo[1], c.inv[1], c.n[1]);
+        const int z = cell_axis((double)pts[3 * v + 2], c.lo[2], c.inv[2], c.n[2]);
+        const int key = c.base + (z * c.n[1] + y) * c.n[0] + x;
+        const int r = atomicAdd(&cnt[key], 1);
+        packed[v] = ((uint32_t)key << 16) | (uint32_t)r;
+    }
+    __syncthreads();
+    // exclusive scan of the counters in place: a run of `per` cells per thread
+    const int per = (cells + 1023) / 1024;
+    const int k0 = min(tid * per, cells), k1 = min(k0 + per, cells);
+    int run = 0;
+    for (int k = k0; k < k1; ++k) run += cnt[k];
+    int incl = run;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int base = 0;
+    for (int k = 0; k < w; ++k) base += wsum[k];
+    int acc = base + incl - run;
+    for (int k = k0; k < k1; ++k) {
+        const int n = cnt[k];
+        cnt[k] = acc;
+        acc += n;
+    }
+    __syncthreads();
+    for (int k = tid; k < cells; k += 1024) {
+        const int a = cnt[k], b = k + 1 < cells ? cnt[k + 1] : V;
+        cstart[k] = b > a ? a : 0;
+        cend[k] = b > a ? b : 0;
+    }
+    for (int v = tid; v < V; v += 1024) {
+        const uint32_t pk = packed[v];
+        svals[cnt[pk >> 16] + (int)(pk & 0xffffu)] = v;
+    }
+}
+
 __global__ void k_cell_bounds(int V, const uint32_t *__restrict__ sk, int *__restrict__ cstart, int *__restrict__ cend) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= V) return;
@@ -1831,19 +1892,26 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
     ktimer_begin(ctx, "k_cand", &ev_c);
     k_cell_setup<<<S, 1024, 0, st>>>(pl.sd, pts, cell_min, pl.cd);
     LG_LAUNCHED("k_cell_setup");
-    k_cell_keys<<<vb, 256, 0, st>>>(pl.cd, pl.sid, V, pts, pl.ckeys, pl.cvals);
-    LG_LAUNCHED("k_cell_keys");
-    int kbits = 1;
-    while ((1ll << kbits) < pl.total_cells) ++kbits;
-    size_t sbytes = pl.tmp_vertex_bytes;
-    LG_HIP(rocprim::radix_sort_pairs(pl.tmp_vertex, sbytes, pl.ckeys, pl.ckeys_sorted, pl.cvals, pl.cvals_sorted,
-                                     (size_t)V, 0, kbits, st));
     LG_TRY(grow(ctx, &pl.cstart, &pl.cells_cap, (size_t)pl.total_cells));
     LG_TRY(grow(ctx, &pl.cend, &pl.cend_cap, (size_t)pl.total_cells));
-    LG_TRY(zero_ints(ctx, {{pl.cstart, (size_t)pl.total_cells}, {pl.cend, (size_t)pl.total_cells},
-                           {pl.rowcnt + V, 1}}));
-    k_cell_bounds<<<vb, 256, 0, st>>>(V, pl.ckeys_sorted, pl.cstart, pl.cend);
-    LG_LAUNCHED("k_cell_bounds");
+    if (pl.total_cells <= kCellSortCells && V <= kCellSortMaxV) {
+        // one workgroup: counting sort, cell ranges and sorted list in one launch
+        k_cell_sort_small<<<1, 1024, 0, st>>>(pl.cd, pl.sid, V, pts, (int)pl.total_cells, pl.ckeys, pl.cstart,
+                                              pl.cend, pl.cvals_sorted, pl.rowcnt + V);
+        LG_LAUNCHED("k_cell_sort_small");
+    } else {
+        k_cell_keys<<<vb, 256, 0, st>>>(pl.cd, pl.sid, V, pts, pl.ckeys, pl.cvals);
+        LG_LAUNCHED("k_cell_keys");
+        int kbits = 1;
+        while ((1ll << kbits) < pl.total_cells) ++kbits;
+        size_t sbytes = pl.tmp_vertex_bytes;
+        LG_HIP(rocprim::radix_sort_pairs(pl.tmp_vertex, sbytes, pl.ckeys, pl.ckeys_sorted, pl.cvals,
+                                         pl.cvals_sorted, (size_t)V, 0, kbits, st));
+        LG_TRY(zero_ints(ctx, {{pl.cstart, (size_t)pl.total_cells}, {pl.cend, (size_t)pl.total_cells},
+                               {pl.rowcnt + V, 1}}));
+        k_cell_bounds<<<vb, 256, 0, st>>>(V, pl.ckeys_sorted, pl.cstart, pl.cend);
+        LG_LAUNCHED("k_cell_bounds");
+    }
     k_cand<false><<<wb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted, nullptr,
                                       pl.rowcnt, nullptr);
     LG_LAUNCHED("k_cand");
