@@ -298,3 +298,50 @@ def test_density_and_temperature(dump, tmp_path):
     path4 = vdbio.write(str(tmp_path / "v4.vdb"), [temp])
     r = _run(dump, "volume", path4, tmp_path / "a.raw", tmp_path / "b.raw")
     assert r.returncode == 2 and "no grid named density" in r.stderr
+
+
+def test_corrupted_files_fail_cleanly(tmp_path):
+    """Bytes flipped, zeroed or cut anywhere in a valid file: the reader either
+    reads a grid or throws its error (exit status 0 or 2), with no crash, no
+    undefined behaviour and no hang.  Run without the address sanitizer and
+    under a 4 GiB address-space limit, so that a corrupted size asking for a
+    huge allocation ends in std::bad_alloc (caught: status 2)."""
+    import resource
+    exe = str(tmp_path / "vdb_dump_ub")
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=undefined", "-fno-sanitize-recover=undefined",
+           "-I", os.path.join(ROOT, "lightgraph_amd", "host"), os.path.join(ROOT, "tests", "host", "vdb_dump.cpp"),
+           os.path.join(ROOT, "lightgraph_amd", "host", "vdb.cpp"), "-o", exe, "-lz", "-ldl"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    rng = np.random.default_rng(9)
+    v, act = _sparse_cloud(rng, (10, 12, 14))
+    g = vdbio.Grid.from_dense("density", v, act, (-5, 3, -9), voxel_size=0.5, translation=(1, 2, 3))
+    g.tiles8[(64, 64, 64)] = (0.5, True)
+    t = vdbio.Grid.from_dense("temperature", v * 100, act, (-5, 3, -9), voxel_size=0.5, translation=(1, 2, 3))
+    good = open(vdbio.write(str(tmp_path / "good.vdb"), [g, t]), "rb").read()
+
+    def limit():
+        resource.setrlimit(resource.RLIMIT_AS, (4 << 30, 4 << 30))
+
+    outcomes = {0: 0, 2: 0}
+    for k in range(150):
+        b = bytearray(good)
+        kind = k % 3
+        if kind == 0:  # flip a few bytes
+            for _ in range(int(rng.integers(1, 4))):
+                b[int(rng.integers(0, len(b)))] ^= int(rng.integers(1, 256))
+        elif kind == 1:  # zero a run
+            a, n = int(rng.integers(0, len(b))), int(rng.integers(1, 16))
+            b[a:a + n] = bytes(len(b[a:a + n]))
+        else:  # cut
+            b = b[:int(rng.integers(0, len(b)))]
+        f = tmp_path / f"bad{k}.vdb"
+        f.write_bytes(bytes(b))
+        for args in (["grid", f, "density", tmp_path / "o.raw"], ["volume", f, tmp_path / "d.raw", tmp_path / "t.raw"]):
+            r = subprocess.run([exe, *map(str, args)], capture_output=True, text=True, errors="replace", timeout=60,
+                               preexec_fn=limit,
+                               env=dict(os.environ, UBSAN_OPTIONS="print_stacktrace=1"))
+            assert r.returncode in (0, 2), (k, args[0], r.returncode, r.stderr[-2000:])
+            assert "runtime error" not in r.stderr, (k, r.stderr[-2000:])
+            outcomes[r.returncode] += 1
+    assert outcomes[2] > 0
