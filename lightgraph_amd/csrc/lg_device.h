@@ -138,6 +138,44 @@ __device__ __forceinline__ double wave_uniform(double x) {
 }
 __device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// x of a source lane chosen by a DPP control word, both 32-bit halves moved by
+// the row permute network (no LDS); lanes without a source, or outside
+// row_mask, get 0.
+template <int kCtrl, int kRowMask, typename V>
+__device__ __forceinline__ V dpp_move(V x) {
+    static_assert(sizeof(V) == 8, "64-bit values");
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, kCtrl, kRowMask, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), kCtrl, kRowMask, 0xf, true);
+    return __builtin_bit_cast(V, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+}
+
+// Inclusive prefix sum of x over the 64 lanes in lane order: shifts by 1, 2, 4
+// and 8 inside each row of 16 lanes, then the row totals carried across rows
+// with the row broadcasts (lane 15 of a row to the next row, lane 31 to rows 2
+// and 3).  Every step adds a value or an exact zero.
+template <typename V>
+__device__ __forceinline__ V wave_prefix_sum(V x) {
+    V s = x;
+    s += dpp_move<0x111, 0xf>(s);  // row_shr:1
+    s += dpp_move<0x112, 0xf>(s);  // row_shr:2
+    s += dpp_move<0x114, 0xf>(s);  // row_shr:4
+    s += dpp_move<0x118, 0xf>(s);  // row_shr:8
+    s += dpp_move<0x142, 0xa>(s);  // row_bcast:15 into rows 1 and 3
+    s += dpp_move<0x143, 0xc>(s);  // row_bcast:31 into rows 2 and 3
+    return s;
+}
+
+// Value of lane `lane` of the wave, in scalar registers.
+template <typename V>
+__device__ __forceinline__ V wave_lane64(V x, int lane) {
+    static_assert(sizeof(V) == 8, "64-bit values");
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __builtin_bit_cast(V, ((unsigned long long)hi << 32) | lo);
+}
+
 // Work counters: flags the storage bricks that the 8 taps of a trilinear
 // sample at index-space (x, y, z) read (clamped as trilinear clamps them).
 __device__ __forceinline__ void mark_taps(const GridView &g, double x, double y, double z) {
@@ -504,8 +542,7 @@ __device__ __forceinline__ bool isect_march_wave(const GridView &g, Isect &s, do
             // tcur (k = 0) or at event k - 1
             const double te = ev_t[wv][lane];
             const int ea = ev_ax[wv][lane];
-            const double tprev = __shfl_up(te, 1, 64);  // every lane takes part in the shuffle
-            const double tcell = lane == 0 ? tcur : tprev;
+            const double tcell = lane == 0 ? tcur : ev_t[wv][lane - 1];
             const unsigned long long mx = __ballot(ea == 0), my = __ballot(ea == 1), mz = __ballot(ea == 2);
             const unsigned long long below = (1ull << lane) - 1ull;  // events before cell `lane`
             const int cx = __popcll(mx & below), cy = __popcll(my & below), cz = __popcll(mz & below);

@@ -194,7 +194,8 @@ __device__ __forceinline__ bool shadow_tau_wave(const lgd::GridView &g, const Li
         for (int base = 0;; base += 64) {
             const double t = ts + (double)(base + lane) * lc.h;
             const bool act = t <= t1;
-            *nsamp += __popcll(__ballot(act));
+            const unsigned long long amask = __ballot(act);
+            *nsamp += __popcll(amask);
             double x = 0.0;
             if (act) {
                 const double px = is.e[0] + is.d[0] * t, py = is.e[1] + is.d[1] * t, pz = is.e[2] + is.d[2] * t;
@@ -204,19 +205,14 @@ __device__ __forceinline__ bool shadow_tau_wave(const lgd::GridView &g, const Li
                 dens = (lc.dmax < dens) ? lc.dmax : dens;
                 if (!(dens < lc.cutoff)) x = dens * lc.h;
             }
-            double pre = x;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const double y = __shfl_up(pre, o, 64);
-                if (lane >= o) pre += y;
-            }
+            const double pre = lgd::wave_prefix_sum(x);
             if (__ballot(act && tau + pre > lc.tau_star) != 0ull) {
                 *dropped = true;
                 *tau_out = tau;
                 return true;
             }
-            tau = lgd::wave_uniform(tau + __shfl(pre, 63, 64));
-            if (!__shfl((int)act, 63, 64)) break;  // t grows with the lane: the span is done
+            tau = tau + lgd::wave_lane64(pre, 63);
+            if (!((amask >> 63) & 1ull)) break;  // t grows with the lane: the span is done
         }
     }
     *tau_out = tau;
@@ -329,16 +325,13 @@ __device__ __forceinline__ void sample_point_wave(const SolveDesc &d, int i, uin
     for (int c = 0; c < 8; ++c) {
         const float f = lgd::to_f32(p[64 * c + lane]);
         uint64_t q = (f > 0.0f) ? (uint64_t)fmin((double)f * 16777216.0, 549755813888.0 * 16777216.0) : 0ull;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint64_t t = __shfl_up(q, o, 64);
-            if (lane >= o) q += t;
-        }
+        q = lgd::wave_prefix_sum(q);
         const unsigned long long hit = __ballot(local < carry + q);
         if (hit) {
             v = 64 * c + (__ffsll((long long)hit) - 1);
             break;
         }
-        carry += __shfl(q, 63, 64);
+        carry += lgd::wave_lane64(q, 63);
     }
     if (lane != 0) return;
     const int sbx = sb % g.NBx, sby = (sb / g.NBx) % g.NBy, sbz = sb / (g.NBx * g.NBy);
