@@ -143,11 +143,16 @@ __device__ __forceinline__ int wave_uniform(int x) { return __builtin_amdgcn_rea
 // row_mask, get 0.
 template <int kCtrl, int kRowMask, typename V>
 __device__ __forceinline__ V dpp_move(V x) {
-    static_assert(sizeof(V) == 8, "64-bit values");
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, kCtrl, kRowMask, 0xf, true);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), kCtrl, kRowMask, 0xf, true);
-    return __builtin_bit_cast(V, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    if constexpr (sizeof(V) == 8) {
+        const unsigned long long b = __builtin_bit_cast(unsigned long long, x);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, kCtrl, kRowMask, 0xf, true);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), kCtrl, kRowMask, 0xf, true);
+        return __builtin_bit_cast(V, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    } else {
+        static_assert(sizeof(V) == 4, "32- or 64-bit values");
+        return __builtin_bit_cast(
+            V, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), kCtrl, kRowMask, 0xf, true));
+    }
 }
 
 // Inclusive prefix sum of x over the 64 lanes in lane order: shifts by 1, 2, 4
@@ -163,6 +168,19 @@ __device__ __forceinline__ V wave_prefix_sum(V x) {
     s += dpp_move<0x118, 0xf>(s);  // row_shr:8
     s += dpp_move<0x142, 0xa>(s);  // row_bcast:15 into rows 1 and 3
     s += dpp_move<0x143, 0xc>(s);  // row_bcast:31 into rows 2 and 3
+    return s;
+}
+
+// Inclusive prefix maximum of non-negative ints over the 64 lanes in lane order
+// (the same DPP steps; lanes without a source read 0, the identity here).
+__device__ __forceinline__ int wave_prefix_max(int x) {
+    int s = x;
+    s = max(s, dpp_move<0x111, 0xf>(s));
+    s = max(s, dpp_move<0x112, 0xf>(s));
+    s = max(s, dpp_move<0x114, 0xf>(s));
+    s = max(s, dpp_move<0x118, 0xf>(s));
+    s = max(s, dpp_move<0x142, 0xa>(s));
+    s = max(s, dpp_move<0x143, 0xc>(s));
     return s;
 }
 

@@ -253,12 +253,7 @@ __global__ void __launch_bounds__(1024) k_cell_sort_small(const CellDesc *__rest
     const int k0 = min(tid * per, cells), k1 = min(k0 + per, cells);
     int run = 0;
     for (int k = k0; k < k1; ++k) run += cnt[k];
-    int incl = run;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y;
-    }
+    const int incl = lgd::wave_prefix_sum(run);
     if (lane == 63) wsum[w] = incl;
     __syncthreads();
     int base = 0;
@@ -529,12 +524,7 @@ __global__ void __launch_bounds__(64) k_greedy_staged(const SolveDesc *__restric
             len = off[i + 1] - k0;
             rf = risky[i];
         }
-        int pre = len;  // inclusive scan
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const int y = __shfl_up(pre, o, 64);
-            if (lane >= o) pre += y;
-        }
+        const int pre = lgd::wave_prefix_sum(len);  // inclusive scan
         const int pos = pre - len;
         const uint64_t fits = __ballot(have && pre <= kGreedyBuf);
         const int rows = __popcll(fits);  // rows form a prefix: pre grows with the lane
@@ -589,12 +579,7 @@ __global__ void __launch_bounds__(64) k_greedy_staged(const SolveDesc *__restric
 #pragma unroll
         for (int u = 0; u < kGreedyBuf / 64; ++u) {
             const int q = lane + 64 * u;
-            int r = q < total ? (int)cbuf[q] : 0;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int y = __shfl_up(r, o, 64);
-                if (lane >= o) r = max(r, y);
-            }
+            int r = lgd::wave_prefix_max(q < total ? (int)cbuf[q] : 0);
             r = max(r, carry);
             carry = __shfl(r, 63, 64);
             const int kb = __shfl(k0, r, 64), pb = __shfl(pos, r, 64);  // all lanes active
@@ -1059,11 +1044,7 @@ __device__ __forceinline__ float block_reduce_min(float v, float *red) {
 
 __device__ __forceinline__ int block_excl_scan(int x, int *red, int *total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int inc = x;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += y;
-    }
+    const int inc = lgd::wave_prefix_sum(x);
     __syncthreads();
     if (lane == 63) red[w] = inc;
     __syncthreads();
@@ -1738,21 +1719,11 @@ __global__ void __launch_bounds__(1024) k_scan_small(const int *__restrict__ in,
     const int b = t * per, e = min(b + per, n);
     int sum = 0;
     for (int i = b; i < e; ++i) sum += staged ? buf[i] : in[i];
-    int v = sum;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(v, o, 64);
-        if (lane >= o) v += y;
-    }
+    const int v = lgd::wave_prefix_sum(sum);
     if (lane == 63) wsum[w] = v;
     __syncthreads();
     if (w == 0) {
-        int x = lane < 16 ? wsum[lane] : 0;
-#pragma unroll
-        for (int o = 1; o < 16; o <<= 1) {
-            const int y = __shfl_up(x, o, 64);
-            if (lane >= o) x += y;
-        }
+        const int x = lgd::wave_prefix_sum(lane < 16 ? wsum[lane] : 0);
         if (lane < 16) wsum[lane] = x;
     }
     __syncthreads();
