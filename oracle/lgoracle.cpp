@@ -398,6 +398,46 @@ double sample1d(const double *cdf, int count, double u, int *off) {
 float sinF(float x) { return (float)std::sin((double)x); }
 float cosF(float x) { return (float)std::cos((double)x); }
 
+// EnvLight::illuminate (Light.h:192-231) for the two uniform draws r0, r1:
+// the continuous sample (s0, s1) of the map's 2D distribution, the bilinear
+// texel colour at (s0 w, s1 h) and the direction -(cos t sin p, cos p,
+// sin t sin p) with t = 2 pi s0, p = pi s1, through the light's transform.
+void envIlluminate(const lg_light &L, const EnvMap *env, float r0, float r1, Vec3 &lightDir, Vec3 &lightIntensity,
+                   float *s0_out = nullptr, float *s1_out = nullptr) {
+    const Vec3 color(L.color[0], L.color[1], L.color[2]);
+    int v = 0;
+    const float s1 = (float)sample1d(env->marg.data(), env->h, r1, &v);
+    const float s0 = (float)sample1d(env->cond.data() + (size_t)v * (env->w + 1), env->w, r0, nullptr);
+    if (s0_out) *s0_out = s0;
+    if (s1_out) *s1_out = s1;
+    int col_lo = (int)std::floor(s0 * env->w), row_lo = (int)std::floor(s1 * env->h);
+    int col_hi = (int)std::ceil(s0 * env->w), row_hi = (int)std::ceil(s1 * env->h);
+    const float col_ratio = s0 * env->w - (float)col_lo;
+    const float row_ratio = s1 * env->h - (float)row_lo;
+    col_lo = std::min(std::max(col_lo, 0), env->w - 1);
+    col_hi = std::min(std::max(col_hi, 0), env->w - 1);
+    row_lo = std::min(std::max(row_lo, 0), env->h - 1);
+    row_hi = std::min(std::max(row_hi, 0), env->h - 1);
+    const float theta = s0 * M_PI * 2.0f;
+    const float phi = s1 * M_PI;
+    const double sinPhi = sinF(phi);
+    Vec3 sd((double)cosF(theta) * sinPhi, (double)cosF(phi), (double)sinF(theta) * sinPhi);
+    const double *m = L.xform;
+    Vec3 t(sd[0] * m[0] + sd[1] * m[4] + sd[2] * m[8] + m[12], sd[0] * m[1] + sd[1] * m[5] + sd[2] * m[9] + m[13],
+           sd[0] * m[2] + sd[1] * m[6] + sd[2] * m[10] + m[14]);
+    sd = t / t.length();
+    auto px = [&](int r, int c) {
+        const float *p = env->rgb + 3 * ((size_t)r * env->w + c);
+        return Vec3(p[0], p[1], p[2]);
+    };
+    const Vec3 img = px(row_lo, col_lo) * (double)((1.0f - row_ratio) * (1.0f - col_ratio)) +
+                     px(row_lo, col_hi) * (double)((1.0f - row_ratio) * col_ratio) +
+                     px(row_hi, col_lo) * (double)(row_ratio * (1.0f - col_ratio)) +
+                     px(row_hi, col_hi) * (double)(row_ratio * col_ratio);
+    lightDir = -sd;
+    lightIntensity = img * color * (double)L.intensity;
+}
+
 // Light::illuminate (Light.h:79-89, 118-126, 161-231)
 void illuminate(const lg_light &L, const EnvMap *env, const Vec3 &hitPos, CounterStream *rng, Vec3 &lightDir,
                 Vec3 &lightIntensity, double &distance) {
@@ -415,35 +455,7 @@ void illuminate(const lg_light &L, const EnvMap *env, const Vec3 &hitPos, Counte
     } else {
         const float r0 = (*rng)();
         const float r1 = (*rng)();
-        int v = 0;
-        const float s1 = (float)sample1d(env->marg.data(), env->h, r1, &v);
-        const float s0 = (float)sample1d(env->cond.data() + (size_t)v * (env->w + 1), env->w, r0, nullptr);
-        int col_lo = (int)std::floor(s0 * env->w), row_lo = (int)std::floor(s1 * env->h);
-        int col_hi = (int)std::ceil(s0 * env->w), row_hi = (int)std::ceil(s1 * env->h);
-        const float col_ratio = s0 * env->w - (float)col_lo;
-        const float row_ratio = s1 * env->h - (float)row_lo;
-        col_lo = std::min(std::max(col_lo, 0), env->w - 1);
-        col_hi = std::min(std::max(col_hi, 0), env->w - 1);
-        row_lo = std::min(std::max(row_lo, 0), env->h - 1);
-        row_hi = std::min(std::max(row_hi, 0), env->h - 1);
-        const float theta = s0 * M_PI * 2.0f;
-        const float phi = s1 * M_PI;
-        const double sinPhi = sinF(phi);
-        Vec3 sd((double)cosF(theta) * sinPhi, (double)cosF(phi), (double)sinF(theta) * sinPhi);
-        const double *m = L.xform;
-        Vec3 t(sd[0] * m[0] + sd[1] * m[4] + sd[2] * m[8] + m[12], sd[0] * m[1] + sd[1] * m[5] + sd[2] * m[9] + m[13],
-               sd[0] * m[2] + sd[1] * m[6] + sd[2] * m[10] + m[14]);
-        sd = t / t.length();
-        auto px = [&](int r, int c) {
-            const float *p = env->rgb + 3 * ((size_t)r * env->w + c);
-            return Vec3(p[0], p[1], p[2]);
-        };
-        const Vec3 img = px(row_lo, col_lo) * (double)((1.0f - row_ratio) * (1.0f - col_ratio)) +
-                         px(row_lo, col_hi) * (double)((1.0f - row_ratio) * col_ratio) +
-                         px(row_hi, col_lo) * (double)(row_ratio * (1.0f - col_ratio)) +
-                         px(row_hi, col_hi) * (double)(row_ratio * col_ratio);
-        lightDir = -sd;
-        lightIntensity = img * color * (double)L.intensity;
+        envIlluminate(L, env, r0, r1, lightDir, lightIntensity);
         distance = DBL_MAX;
     }
 }
@@ -1014,6 +1026,21 @@ extern "C" void lgo_env_cdf(int32_t w, int32_t h, const float *rgb, double *cond
     Scene sc = makeScene(nullptr, nullptr, nullptr, nullptr, &l, 1, maps, ws, hs);
     memcpy(cond, sc.envs[0].cond.data(), sc.envs[0].cond.size() * 8);
     memcpy(marg, sc.envs[0].marg.data(), sc.envs[0].marg.size() * 8);
+}
+
+// EnvLight::illuminate for given draws (r0, r1): direction, intensity and the
+// continuous sample (s0, s1) of the map's distribution.
+extern "C" void lgo_env_illuminate(int32_t w, int32_t h, const float *rgb, const lg_light *L, float r0, float r1,
+                                   double *dir, double *inten, float *s01) {
+    const float *maps[1] = {rgb};
+    const int ws[1] = {w}, hs[1] = {h};
+    Scene sc = makeScene(nullptr, nullptr, nullptr, nullptr, L, 1, maps, ws, hs);
+    Vec3 d, I;
+    envIlluminate(*L, &sc.envs[0], r0, r1, d, I, &s01[0], &s01[1]);
+    for (int c = 0; c < 3; ++c) {
+        dir[c] = d[c];
+        inten[c] = I[c];
+    }
 }
 
 // ====================================================== splat and render =====

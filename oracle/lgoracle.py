@@ -21,6 +21,7 @@ U32 = C.c_uint32
 U64 = C.c_uint64
 I64 = C.c_int64
 D = C.c_double
+F = C.c_float
 
 
 def build():
@@ -51,6 +52,7 @@ def lib():
             "lgo_solve_points": (I32, [P, P, P, P, P, P, I32, P, P, P, U64, U32, P, I32, D, I32, P, I32, P, P, P]),
             "lgo_dart_throw": (I32, [P, P, U32, U32, U32, P, P, I32]),
             "lgo_env_cdf": (None, [I32, I32, P, P, P]),
+            "lgo_env_illuminate": (None, [I32, I32, P, P, F, F, P, P, P]),
             "lgo_ramp_color": (None, [P, D, P]),
             "lgo_sample1d": (D, [P, I32, D, P]),
         }
@@ -250,6 +252,17 @@ def env_cdf(rgb):
     marg = np.zeros(h + 1, dtype=np.float64)
     lib().lgo_env_cdf(w, h, _p(rgb), _p(cond), _p(marg))
     return cond, marg
+
+
+def env_illuminate(rgb, light, r0, r1):
+    """EnvLight::illuminate with the draws (r0, r1): (direction, intensity, (s0, s1))."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+    h, w = rgb.shape[:2]
+    d = np.zeros(3, dtype=np.float64)
+    it = np.zeros(3, dtype=np.float64)
+    s = np.zeros(2, dtype=np.float32)
+    lib().lgo_env_illuminate(w, h, _p(rgb), C.byref(light), float(r0), float(r1), _p(d), _p(it), _p(s))
+    return d, it, (float(s[0]), float(s[1]))
 
 
 def ramp_color(shader, x):

@@ -1,6 +1,6 @@
 """Closed-form pins of the oracle functions that no reference test or fixture
 covers: the environment-map sampling CDF and its inverse (Sampling.h:76-167,
-Light.h:161-176), the temperature colour ramp at the knots of
+Light.h:161-176), the env light's texel lookup and direction (Light.h:192-231), the temperature colour ramp at the knots of
 scenes/explo.sh:1 (Shader.h:56-92), the point voxel-size heuristic on lattices
 whose answer is exact (LightGraph.h:713-714) and the scatter-grid topology of
 single-voxel and single-brick grids (ScatterGrid.h:105-144).
@@ -122,6 +122,86 @@ def test_sample1d_inverse_closed_form(oracle):
         x, off = oracle.sample1d(cdf, u)
         assert x == pytest.approx(want, abs=1e-12), u
         assert off == min(int(want * n), n - 1)
+
+
+# --------------------------------------------------- env texel lookup ------
+def _env_light(color=(1.0, 1.0, 1.0), intensity=1.0, xform=None):
+    from lightgraph_amd import _abi
+    l = _abi.lg_light()
+    l.type = _abi.LG_LIGHT_ENV
+    l.samples = 1
+    l.intensity = intensity
+    for c in range(3):
+        l.color[c] = color[c]
+    m = np.eye(4) if xform is None else np.asarray(xform, np.float64)
+    for k in range(16):
+        l.xform[k] = float(m.reshape(-1)[k])
+    return l
+
+
+_DRAWS = [(a, b) for a in np.linspace(0.03, 0.97, 9) for b in np.linspace(0.04, 0.96, 9)]
+
+
+def test_env_texel_constant_map(oracle):
+    """A constant map returns its colour times the light's colour and
+    intensity wherever it is sampled: the four bilinear weights sum to one
+    (Light.h:204-225)."""
+    h, w = 16, 32
+    rgb = np.empty((h, w, 3), np.float32)
+    rgb[...] = (0.25, 2.0, 7.5)
+    light = _env_light(color=(0.5, 1.0, 2.0), intensity=3.0)
+    want = np.array([0.25 * 0.5 * 3.0, 2.0 * 1.0 * 3.0, 7.5 * 2.0 * 3.0])
+    for r0, r1 in _DRAWS:
+        _, inten, _ = oracle.env_illuminate(rgb, light, r0, r1)
+        np.testing.assert_allclose(inten, want, rtol=1e-6)
+
+
+def test_env_texel_bilinear_reproduces_affine_maps(oracle):
+    """Texel (row i, column j) = A + B j + C i, with different B and C per
+    channel: bilinear interpolation at the continuous sample (s0 w, s1 h)
+    returns A + B s0 w + C s1 h exactly wherever the upper neighbours lie
+    inside the map (column s0 w <= w - 1, row s1 h <= h - 1).  A transposed
+    lookup, swapped ratios or a floor/ceil mix-up break the equality."""
+    h, w = 12, 20
+    A = np.array([1.0, 2.0, 3.0])
+    B = np.array([0.5, 0.0, 0.125])
+    Cr = np.array([0.0, 0.75, 0.25])
+    i = np.arange(h)[:, None, None]
+    j = np.arange(w)[None, :, None]
+    rgb = (A + B * j + Cr * i).astype(np.float32)
+    light = _env_light()
+    inside = 0
+    for r0, r1 in _DRAWS:
+        _, inten, (s0, s1) = oracle.env_illuminate(rgb, light, r0, r1)
+        u, v = float(s0) * w, float(s1) * h
+        if u > w - 1 or v > h - 1:
+            continue
+        inside += 1
+        np.testing.assert_allclose(inten, A + B * u + Cr * v, rtol=2e-6)
+    assert inside >= 40
+
+
+def test_env_direction_closed_form(oracle):
+    """The light direction is -(cos t sin p, cos p, sin t sin p) with
+    t = 2 pi s0, p = pi s1 (Light.h:198-202), through the light's transform
+    in row-vector form: a 90 degree turn about y (rows (0,0,-1), (0,1,0),
+    (1,0,0)) maps (x, y, z) to (z, y, -x), and the transform's translation
+    is applied to the direction before it is normalised (Mat4::transform)."""
+    rgb = np.ones((8, 16, 3), np.float32)
+    rgb[2, 5] = 40.0
+    turn = np.array([[0, 0, -1, 0], [0, 1, 0, 0], [1, 0, 0, 0], [0, 0, 0, 1]], np.float64)
+    shift = np.eye(4)
+    shift[3, :3] = (0.0, 2.0, 0.0)
+    for r0, r1 in _DRAWS[::5]:
+        d0, _, (s0, s1) = oracle.env_illuminate(rgb, _env_light(), r0, r1)
+        t, p = 2.0 * math.pi * s0, math.pi * s1
+        sd = np.array([math.cos(t) * math.sin(p), math.cos(p), math.sin(t) * math.sin(p)])
+        np.testing.assert_allclose(d0, -sd, atol=2e-6)
+        d1, _, _ = oracle.env_illuminate(rgb, _env_light(xform=turn), r0, r1)
+        np.testing.assert_allclose(d1, -np.array([sd[2], sd[1], -sd[0]]), atol=2e-6)
+        d2, _, _ = oracle.env_illuminate(rgb, _env_light(xform=shift), r0, r1)
+        moved = sd + np.array([0.0, 2.0, 0.0])
+        np.testing.assert_allclose(d2, -moved / np.linalg.norm(moved), atol=2e-6)
 
 
 # ------------------------------------------------------------------ ramp ----
