@@ -529,17 +529,20 @@ def march_roofline(samples, bricks, io_bytes, t_s):
     return out
 
 
-def kernel_roofline(times, work, workload):
-    """Roofline object for the kernel with the largest total time among KERNELS,
-    over the launches of the timed region: the marches by t_roof = max(B_comp / 8
-    TB/s, S * 32 / 157.3 TFLOP/s) (compulsory bytes or trilinear samples,
-    whichever bounds), the Floyd-Warshall kernels by executed min-plus updates
-    (2 ops each) against the FP32 peak and the 32-bit VALU issue ceiling.  Work comes from the device counters
-    of the same solves."""
-    cands = [(t["total_ms"], nm) for nm, t in times.items() if nm in KERNELS]
-    if not cands:
+def kernel_roofline(times, work, workload, name=None):
+    """Roofline object for one kernel (by default the one with the largest total
+    time among KERNELS) over its launches in `times`: the marches by t_roof =
+    max(B_comp / 8 TB/s, S * 32 / 157.3 TFLOP/s) (compulsory bytes or trilinear
+    samples, whichever bounds), the Floyd-Warshall kernels by executed min-plus
+    updates (2 ops each) against the FP32 peak and the 32-bit VALU issue
+    ceiling.  Work comes from the device counters of the same solves."""
+    if name is None:
+        cands = [(t["total_ms"], nm) for nm, t in times.items() if nm in KERNELS]
+        if not cands:
+            return None
+        _, name = max(cands)
+    if name not in times:
         return None
-    _, name = max(cands)
     t = times[name]
     t_s = t["total_ms"] * 1e-3
     roof = {"kernel": name, "bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
@@ -732,6 +735,13 @@ def main():
         for key, v in solver.stats().items():
             work[key] = work.get(key, 0) + v
     roof = kernel_roofline(timed, work, WORKLOAD["workload"])
+    # every roofline kernel of the solve, from the pass with every kernel timed
+    # (its per-kernel events add a little to each launch)
+    kroofs = {}
+    for nm in KERNELS:
+        r = kernel_roofline(times, work, WORKLOAD["workload"], name=nm)
+        if r and r.get("frac") is not None:
+            kroofs[nm] = {k: r.get(k) for k in ("bound", "frac", "achieved", "unit", "avg_launch_ms", "launches")}
     b = solver.buffers()
     lat = []
     for k in range(args.latency_solves):
@@ -749,6 +759,7 @@ def main():
             "vs_baseline": None, "dtype": "fp32 (fp64 ray/DDA positions)", "data": "synthetic",
             "config": dict(WORKLOAD, edges_last_solve=b.n_edges), "roofline": roof,
             "kernels": {k: {"launches": v["launches"], "avg_ms": v["avg_ms"]} for k, v in times.items()},
+            "kernel_rooflines": kroofs,
             "work_per_solve": {k: round(v / args.steps, 1) for k, v in work.items()},
             "latency": {"p50_ms": pct(0.50), "p99_ms": pct(0.99), "solves": len(lat), "mode": "one solve alone"}}
     if not args.no_fw_dense:

@@ -317,8 +317,22 @@ constexpr double kDeltaD = 1e-9;  // math::Delta<double>
 constexpr double kDeltaF = 1e-5;  // math::Delta<float> (Ray::valid)
 constexpr double kDblMax = 1.7976931348623157e308;
 
+// 1/x from v_rcp_f64 refined by two Newton steps (x finite and nonzero): within
+// an ulp of the correctly rounded quotient, for a fraction of a division's cost.
+__device__ __forceinline__ double rcp_newton(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+}
+
 // Ray(eye_w, dir_w, t0, t1).worldToIndex(grid) followed by clip(mBBox)
 // (VolumeRayIntersector::hitGridBbox, Renderer.h:341-343).  Returns false on a miss.
+// kRcp: the unit direction and its inverse from reciprocals (one per
+// component and one of the length) instead of OpenVDB's four divisions; the
+// times then move by at most an ulp, which the pixel tolerance absorbs.
+template <bool kRcp = false>
 __device__ __forceinline__ bool isect_world_ray(const GridView &g, Isect &s, const double ew[3],
                                                 const double dw[3], double t0w, double t1w) {
     if (g.empty) return false;
@@ -328,9 +342,17 @@ __device__ __forceinline__ bool isect_world_ray(const GridView &g, Isect &s, con
         di[c] = dw[c] * g.inv_voxel;
     }
     const double len = sqrt(di[0] * di[0] + di[1] * di[1] + di[2] * di[2]);
-    for (int c = 0; c < 3; ++c) {
-        s.d[c] = di[c] / len;
-        s.inv[c] = 1.0 / s.d[c];
+    if (kRcp) {
+        const double rl = rcp_newton(len);
+        for (int c = 0; c < 3; ++c) {
+            s.d[c] = di[c] * rl;
+            s.inv[c] = s.d[c] == 0.0 ? (signbit(s.d[c]) ? -INFINITY : INFINITY) : rcp_newton(s.d[c]);
+        }
+    } else {
+        for (int c = 0; c < 3; ++c) {
+            s.d[c] = di[c] / len;
+            s.inv[c] = 1.0 / s.d[c];
+        }
     }
     double t0 = len * t0w, t1 = len * t1w;
     // Ray::intersects(bbox, t0, t1)
@@ -469,149 +491,6 @@ __device__ __forceinline__ bool isect_march(const GridView &g, Isect &s, double 
             m1 += k1;
             m2 += k2;
             lin += __mul24(k0, l0) + __mul24(k1, l1) + __mul24(k2, l2);
-        }
-    }
-    if (st1 > 0) {
-        s.t0 = st1 + kDeltaD;
-        s.t1 = s.tmax;
-    }
-    t0 = st0;
-    t1 = st1;
-    return (st1 - st0) > kDeltaD;
-}
-
-// isect_march for a wave whose 64 lanes share one ray (the same s in every
-// lane; every lane gets the same span).  Cells are taken 64 at a time: lane l
-// times the l-th coming crossing of each axis, each crossing's rank in the
-// merged order is its index plus the crossings of the other axes before it
-// (found by binary search over the lanes, ties ordered z, y, x as MinIndex
-// does), the crossings are placed by rank, and lane k then holds the k-th cell
-// of the walk and loads its brick flag; the span rules run on the flag mask.
-// The cells, times and spans are those of isect_march.  Workgroups of at most
-// 256 threads (a 64-entry exchange buffer per wave).
-// The number of j in [0, 64) whose crossing next0 + (m + j) * delta comes
-// before time c (a tie counts as before when tie_first: the crossing's axis
-// comes first in MinIndex order).  The crossings are the very values the lanes
-// compute, a nondecreasing sequence in j, so the count is the closed form
-// ceil((c - next0) / delta - m), with inv_delta = |d| / 8 for 1 / delta (a few
-// ulps apart), settled by comparing against the crossing on each side of it:
-// the estimate is off by at most one.  (The one case it can miss further, c
-// infinite against a degenerate axis, is a crossing that never falls in the
-// batch: its rank is at least 64 through the ray's finite axes.)
-__device__ __forceinline__ int crossings_before(double next0, double delta, double inv_delta, int m, double c,
-                                                bool tie_first) {
-    auto before = [&](int j) {
-        const double v = next0 + (double)(m + j) * delta;
-        return v < c || (v == c && tie_first);
-    };
-    const double est = (c - next0) * inv_delta - (double)m;
-    int n = !(est > 0.0) ? 0 : (est >= 64.0 ? 64 : (int)ceil(est));
-    if (n > 0 && !before(n - 1))
-        --n;
-    else if (n < 64 && before(n))
-        ++n;
-    return n;
-}
-
-__device__ __forceinline__ bool isect_march_wave(const GridView &g, Isect &s, double &t0, double &t1) {
-    __shared__ double ev_t[4][64];
-    __shared__ int ev_ax[4][64];
-    const int lane = threadIdx.x & 63, wv = (threadIdx.x >> 6) & 3;
-    double st0 = -1.0, st1 = -1.0;
-    if (s.t1 - s.t0 > kDeltaF) {
-        const double mT1 = s.t1;
-        double next0[3], delta[3];
-        int vox[3], step[3];
-        dda_setup(s, s.t0, vox, step, next0, delta);
-        for (int a = 0; a < 3; ++a) {  // one ray per wave: the walk's state is the wave's
-            vox[a] = wave_uniform(vox[a]);
-            step[a] = wave_uniform(step[a]);
-            next0[a] = wave_uniform(next0[a]);
-            delta[a] = wave_uniform(delta[a]);
-        }
-        // estimates only (crossings_before settles every count exactly)
-        const double inv_delta[3] = {wave_uniform(fabs(s.d[0]) * 0.125), wave_uniform(fabs(s.d[1]) * 0.125),
-                                     wave_uniform(fabs(s.d[2]) * 0.125)};
-        int mc[3] = {0, 0, 0};  // crossings taken per axis
-        double tcur = s.t0;     // entry time of the current cell (cell 0 of the batch)
-        bool found = false, done = false;
-        while (!found && !done) {
-            // lane l: crossing mc + l of each axis
-            double c[3];
-            for (int a = 0; a < 3; ++a) c[a] = next0[a] + (double)(mc[a] + lane) * delta[a];
-            // rank of crossing (a, lane) = lane + crossings of the other axes before it
-            int rank[3];
-            for (int a = 0; a < 3; ++a) {
-                int r = lane;
-                for (int b = 0; b < 3; ++b)
-                    if (b != a) r += crossings_before(next0[b], delta[b], inv_delta[b], mc[b], c[a], b > a);
-                rank[a] = r;
-            }
-            __builtin_amdgcn_wave_barrier();
-            for (int a = 0; a < 3; ++a)
-                if (rank[a] < 64) {
-                    ev_t[wv][rank[a]] = c[a];
-                    ev_ax[wv][rank[a]] = a;
-                }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            // event k (k = 0..63) is the crossing into cell k + 1; cell k is entered at
-            // tcur (k = 0) or at event k - 1
-            const double te = ev_t[wv][lane];
-            const int ea = ev_ax[wv][lane];
-            const double tcell = lane == 0 ? tcur : ev_t[wv][lane - 1];
-            const unsigned long long mx = __ballot(ea == 0), my = __ballot(ea == 1), mz = __ballot(ea == 2);
-            const unsigned long long below = (1ull << lane) - 1ull;  // events before cell `lane`
-            const int cx = __popcll(mx & below), cy = __popcll(my & below), cz = __popcll(mz & below);
-            const int vx = vox[0] + cx * step[0], vy = vox[1] + cy * step[1], vz = vox[2] + cz * step[2];
-            const bool valid = tcell <= mT1;  // cell 0 always (t1 - t0 > Delta<float>)
-            const bool ex = valid && brick_exists(g, vx, vy, vz);
-            const unsigned long long V = __ballot(valid), E = __ballot(ex);
-            const int K = __popcll(V);  // cells of this batch on the ray (a prefix)
-            // span rules over cells 0..K-1 in order
-            int k = 0;
-            while (k < K) {
-                const unsigned long long from = (k >= 64) ? 0ull : ~((1ull << k) - 1ull);
-                if (st0 < 0) {
-                    const unsigned long long rem = E & from;
-                    if (!rem) break;
-                    k = __ffsll((long long)rem) - 1;
-                    if (k >= K) break;
-                    st0 = wave_uniform(__shfl(tcell, k, 64));
-                    ++k;
-                } else {
-                    const unsigned long long rem = ~E & V & from;
-                    if (!rem) break;
-                    k = __ffsll((long long)rem) - 1;
-                    st1 = wave_uniform(__shfl(tcell, k, 64));
-                    if (st1 - st0 > kDeltaD) {
-                        found = true;
-                        break;
-                    }
-                    st0 = st1 = -1.0;
-                    ++k;
-                }
-            }
-            if (found) break;
-            if (K < 64) {  // the walk left [t0, t1] inside this batch
-                if (st0 >= 0) st1 = mT1;
-                done = true;
-                break;
-            }
-            // next batch: cell 64, entered at event 63
-            const double t63 = wave_uniform(__shfl(te, 63, 64));
-            vox[0] += __popcll(mx) * step[0];
-            vox[1] += __popcll(my) * step[1];
-            vox[2] += __popcll(mz) * step[2];
-            mc[0] += __popcll(mx);
-            mc[1] += __popcll(my);
-            mc[2] += __popcll(mz);
-            tcur = t63;
-            if (!(tcur <= mT1)) {
-                if (st0 >= 0) st1 = mT1;
-                done = true;
-            }
         }
     }
     if (st1 > 0) {

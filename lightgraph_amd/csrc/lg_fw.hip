@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "lg_internal.h"
+#include "lg_fw_group.h"
 
 namespace lg {
 
@@ -33,15 +34,9 @@ constexpr int KC = 32;       // pivots staged in LDS per step
 // Panel launches with fewer half-tile workgroups than this run in column quarters.
 constexpr int kQuarterPanelBelow = 256;
 
-// Closure of the symmetric pivot tile, four pivots per round trip through LDS.
-// Thread t holds the 4 x 4 block (rows 4*(t/32).., cols 4*(t%32)..) in registers.
-// For the pivot group k..k+3 let G[r] be row k+r after pivots k..k+r-1 (the row
-// the sequential loop uses at pivot k+r); by symmetry it is also column k+r, so
-// the four sequential steps give exactly
-//   D_ij <- min(D_ij, G[0][i] + G[0][j], ..., G[3][i] + G[3][j]),
-// and G[r][x] = P_r[x] folded with min(., G[q][k+r] + G[q][x]) for q < r, where
-// P_r is row k+r before the group -- the same sums in the same operand order as
-// the one-pivot-at-a-time loop, so the result is bit-identical to it.
+// Closure of the symmetric pivot tile, kPivotGroup pivots per round trip through
+// LDS (lg_fw_group.h).  Thread t holds the 4 x 4 block (rows 4*(t/32).., cols
+// 4*(t%32)..) in registers.
 // Pivots of round kb that are vertices (the last round of an n-vertex matrix
 // holds n - kb * TB of them; the rest are padding, +INF rows and columns, which
 // relax nothing), and the staged pivot chunks that cover them.
@@ -55,8 +50,9 @@ __device__ __forceinline__ int fw_round_kend(int n, int kb) {
 
 // m: pivots of the tile to take (the rest are padding)
 __device__ __forceinline__ void fw_pivot_tile(float *__restrict__ D, int ld, int kb, int m) {
-    __shared__ float P[4][TB];  // rows of the group before it
-    __shared__ float G[4][TB];  // the group's pivot rows
+    constexpr int KG = kPivotGroup;
+    __shared__ float P[KG][TB];  // rows of the group before it
+    __shared__ float G[KG][TB];  // the group's pivot rows
     const int tid = threadIdx.x;
     const int rg = tid >> 5, cg = tid & 31;
     float *base = D + (long long)kb * TB * ld + (long long)kb * TB;
@@ -69,54 +65,19 @@ __device__ __forceinline__ void fw_pivot_tile(float *__restrict__ D, int ld, int
         v[a][2] = q.z;
         v[a][3] = q.w;
     }
-    const int groups = (m + 3) / 4;
+    const int groups = (m + KG - 1) / KG;
     for (int g = 0; g < groups; ++g) {
-        const int k = 4 * g;
-        if (rg == g) {  // the owners of rows k..k+3 publish them
+        const int k = KG * g;
+        if (rg / (KG / 4) == g) {  // the owners of rows k..k+KG-1 publish them
+            const int r0 = 4 * (rg % (KG / 4));
 #pragma unroll
             for (int a = 0; a < 4; ++a)
-                *reinterpret_cast<float4 *>(&P[a][4 * cg]) = make_float4(v[a][0], v[a][1], v[a][2], v[a][3]);
+                *reinterpret_cast<float4 *>(&P[r0 + a][4 * cg]) = make_float4(v[a][0], v[a][1], v[a][2], v[a][3]);
         }
         __syncthreads();
-        if (tid < TB) {
-            const int x = tid;
-            // s[q][r] = G[q][k+r], computed in the same order by every thread
-            float s01 = P[0][k + 1], s02 = P[0][k + 2], s03 = P[0][k + 3];
-            float s12 = fminf(P[1][k + 2], s01 + s02);
-            float s13 = fminf(P[1][k + 3], s01 + s03);
-            float s23 = fminf(fminf(P[2][k + 3], s02 + s03), s12 + s13);
-            const float g0 = P[0][x];
-            const float g1 = fminf(P[1][x], s01 + g0);
-            const float g2 = fminf(fminf(P[2][x], s02 + g0), s12 + g1);
-            const float g3 = fminf(fminf(fminf(P[3][x], s03 + g0), s13 + g1), s23 + g2);
-            G[0][x] = g0;
-            G[1][x] = g1;
-            G[2][x] = g2;
-            G[3][x] = g3;
-        }
+        if (tid < TB) pivot_group_rows<KG, TB>(P, G, k, tid);
         __syncthreads();
-        float gr[4][4], gc[4][4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float4 a = *reinterpret_cast<const float4 *>(&G[q][4 * rg]);
-            const float4 b = *reinterpret_cast<const float4 *>(&G[q][4 * cg]);
-            gr[q][0] = a.x;
-            gr[q][1] = a.y;
-            gr[q][2] = a.z;
-            gr[q][3] = a.w;
-            gc[q][0] = b.x;
-            gc[q][1] = b.y;
-            gc[q][2] = b.z;
-            gc[q][3] = b.w;
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const float m01 = fminf(gr[0][a] + gc[0][b], gr[1][a] + gc[1][b]);
-                const float m23 = fminf(gr[2][a] + gc[2][b], gr[3][a] + gc[3][b]);
-                v[a][b] = fminf(v[a][b], fminf(m01, m23));
-            }
+        pivot_group_update<KG, TB>(G, rg, cg, v);
     }
 #pragma unroll
     for (int a = 0; a < 4; ++a)

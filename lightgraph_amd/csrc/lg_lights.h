@@ -93,6 +93,9 @@ __device__ __forceinline__ double sample_1d(const double *cdf, int count, double
 __device__ __forceinline__ float sin_f(float x) { return (float)sin((double)x); }
 __device__ __forceinline__ float cos_f(float x) { return (float)cos((double)x); }
 
+// kRcp: a point light's direction and falloff from reciprocals (see
+// lgd::isect_world_ray) instead of four divisions.
+template <bool kRcp = false>
 __device__ __forceinline__ void illuminate(const DevLight &lt, const double p[3], lgd::Stream &rng, double ldir[3],
                                            double inten[3], double &distance) {
     if (lt.type == LG_LIGHT_DIRECTIONAL) {  // DistantLight::illuminate, Light.h:118-126
@@ -107,9 +110,17 @@ __device__ __forceinline__ void illuminate(const DevLight &lt, const double p[3]
         const float r2 = (float)(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
         distance = (double)sqrtf(r2);
         const double fall = 1.0 + 4.0 * M_PI * (double)r2;
-        for (int c = 0; c < 3; ++c) {
-            ldir[c] = v[c] / distance;
-            inten[c] = (lt.color[c] * (double)lt.intensity) / fall;
+        if (kRcp) {
+            const double rd = lgd::rcp_newton(distance), rf = lgd::rcp_newton(fall);
+            for (int c = 0; c < 3; ++c) {
+                ldir[c] = v[c] * rd;
+                inten[c] = (lt.color[c] * (double)lt.intensity) * rf;
+            }
+        } else {
+            for (int c = 0; c < 3; ++c) {
+                ldir[c] = v[c] / distance;
+                inten[c] = (lt.color[c] * (double)lt.intensity) / fall;
+            }
         }
     } else {  // EnvLight::illuminate, Light.h:161-231
         const float u0 = rng.next();

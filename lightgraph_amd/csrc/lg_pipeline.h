@@ -164,16 +164,20 @@ __device__ __forceinline__ bool shadow_tau(const lgd::GridView &g, const LightCo
     return true;
 }
 
-// Wave-cooperative form of shadow_tau for exact steps: the 64 lanes of a wave
-// take consecutive samples t = ts + k*h of one span, the optical depth is an FP64
-// wave prefix sum (same sample set and densities as the sequential loop; the
-// summation order differs only in rounding), and the dropout test fires at the
-// first sample whose running depth exceeds tau_star.  Every lane returns the
-// same values.
-template <typename T, bool kCount = false>
-__device__ __forceinline__ bool shadow_tau_wave(const lgd::GridView &g, const LightConsts &lc, const double p[3],
-                                                const double ldir[3], double distance, double *tau_out,
-                                                bool *dropped, int *nsamp) {
+// Vertex-light march for steps that are multiples of 2^-30 (computeLighting's
+// shadow ray, LightGraph.h:449-488).  Every sample of every span sits at
+// t = k h, so sample k is taken exactly when the 8^3 brick holding its position
+// exists: the spans are the runs of existing bricks along the ray.  (A sample
+// exactly on a span's closing plane, a span shorter than Delta<double> and a
+// span that opens within Delta<float> of the ray's end are the exceptions; each
+// needs an exact tie between two FP64 times.)  The 64 lanes take 64 consecutive
+// k, each loading its brick's box code together with its 8 taps: one memory
+// round trip per 64 samples and no brick walk.  A batch whose samples all lie in
+// empty bricks jumps over the empty box around its last sample.
+template <typename T, bool kCount>
+__device__ __forceinline__ bool shadow_tau_cells(const lgd::GridView &g, const LightConsts &lc, const double p[3],
+                                                 const double ldir[3], double distance, double *tau_out,
+                                                 bool *dropped, int *nsamp) {
     const int lane = threadIdx.x & 63;
     const double sd[3] = {-ldir[0], -ldir[1], -ldir[2]};
     lgd::Isect is;
@@ -184,36 +188,64 @@ __device__ __forceinline__ bool shadow_tau_wave(const lgd::GridView &g, const Li
         is.inv[c] = lgd::wave_uniform(is.inv[c]);
     }
     is.t0 = lgd::wave_uniform(is.t0);
-    is.t1 = lgd::wave_uniform(is.t1);
     is.tmax = lgd::wave_uniform(is.tmax);
-    double tau = 0.0;
-    double t0, t1;
     *dropped = false;
-    while (lgd::isect_march_wave(g, is, t0, t1)) {
-        const double ts = lgd::wave_uniform(lc.h * ceil(t0 / lc.h));
-        for (int base = 0;; base += 64) {
-            const double t = ts + (double)(base + lane) * lc.h;
-            const bool act = t <= t1;
-            const unsigned long long amask = __ballot(act);
-            *nsamp += __popcll(amask);
-            double x = 0.0;
-            if (act) {
-                const double px = is.e[0] + is.d[0] * t, py = is.e[1] + is.d[1] * t, pz = is.e[2] + is.d[2] * t;
-                if (kCount) lgd::mark_taps(g, px, py, pz);
-                const float rho = lgd::trilinear<T, false>(g, px, py, pz);
-                double dens = (double)rho * lc.ds_sds_scds[0] * lc.ds_sds_scds[1] * lc.ds_sds_scds[2];
-                dens = (lc.dmax < dens) ? lc.dmax : dens;
-                if (!(dens < lc.cutoff)) x = dens * lc.h;
-            }
-            const double pre = lgd::wave_prefix_sum(x);
-            if (__ballot(act && tau + pre > lc.tau_star) != 0ull) {
-                *dropped = true;
-                *tau_out = tau;
-                return true;
-            }
-            tau = tau + lgd::wave_lane64(pre, 63);
-            if (!((amask >> 63) & 1ull)) break;  // t grows with the lane: the span is done
+    double tau = 0.0;
+    *tau_out = 0.0;
+    if (!(is.tmax - is.t0 > lgd::kDeltaF)) return true;  // isect_march: no span on a ray this short
+    const double h = lc.h;
+    double kb = lgd::wave_uniform(ceil(is.t0 / h));
+    for (;;) {
+        const double t = (kb + (double)lane) * h;
+        const bool act = t <= is.tmax;
+        const double px = is.e[0] + is.d[0] * t, py = is.e[1] + is.d[1] * t, pz = is.e[2] + is.d[2] * t;
+        const int bx = ((int)floor(px)) >> 3, by = ((int)floor(py)) >> 3, bz = ((int)floor(pz)) >> 3;
+        // positions up to the clipped end lie inside the storage (its border included)
+        const int code = act ? (int)g.skip[((long long)(bz + 1) * g.NBy + (by + 1)) * g.NBx + (bx + 1)] : 0;
+        double x = 0.0;
+        float rho = 0.0f;
+        if (act) rho = lgd::trilinear<T, false>(g, px, py, pz);
+        const bool take = act && code >= lgd::kSkipExists;
+        if (take) {
+            double dens = (double)rho * lc.ds_sds_scds[0] * lc.ds_sds_scds[1] * lc.ds_sds_scds[2];
+            dens = (lc.dmax < dens) ? lc.dmax : dens;
+            if (!(dens < lc.cutoff)) x = dens * h;
         }
+        const unsigned long long tmask = __ballot(take);
+        const double pre = lgd::wave_prefix_sum(x);
+        const unsigned long long dmask = __ballot(take && tau + pre > lc.tau_star);
+        // the samples the sequential march takes: up to the one that drops the ray
+        const unsigned long long upto = dmask ? (2ull << (__ffsll((long long)dmask) - 1)) - 1ull : ~0ull;
+        *nsamp += __popcll(tmask & upto);
+        if (kCount && take && ((upto >> lane) & 1ull)) lgd::mark_taps(g, px, py, pz);
+        if (dmask != 0ull) {
+            *dropped = true;
+            *tau_out = tau;
+            return true;
+        }
+        tau = tau + lgd::wave_lane64(pre, 63);
+        if (!((__ballot(act) >> 63) & 1ull)) break;  // the ray ends inside this batch
+        double knext = kb + 64.0;
+        if (tmask == 0ull) {
+            // all 64 samples in empty bricks: the box of Chebyshev radius d - 1
+            // bricks around the last one is empty (its code d >= 2)
+            const int d = __builtin_amdgcn_readlane(code, 63);
+            if (d >= lgd::kSkipMin) {
+                const int cb[3] = {__builtin_amdgcn_readlane(bx, 63), __builtin_amdgcn_readlane(by, 63),
+                                   __builtin_amdgcn_readlane(bz, 63)};
+                double texit = lgd::kDblMax;
+                for (int c = 0; c < 3; ++c) {
+                    if (is.d[c] == 0.0) continue;
+                    const double plane = 8.0 * (double)(is.d[c] > 0.0 ? cb[c] + d : cb[c] - d + 1);
+                    const double tc = (plane - is.e[c]) * is.inv[c];
+                    texit = tc < texit ? tc : texit;
+                }
+                // every sample before texit (less a rounding margin) is in the box
+                const double kj = floor((texit - 1e-9 * (1.0 + fabs(texit))) / h);
+                knext = kj > knext ? kj : knext;
+            }
+        }
+        kb = lgd::wave_uniform(knext);
     }
     *tau_out = tau;
     return true;

@@ -28,6 +28,7 @@
 #include "lg_internal.h"
 #include "lg_lights.h"
 #include "lg_pipeline.h"
+#include "lg_fw_group.h"
 
 namespace lg {
 
@@ -105,7 +106,7 @@ __global__ void __launch_bounds__(256) k_light_rays(const SolveDesc *__restrict_
     bool dropped;
     double r[3] = {0.0, 0.0, 0.0};
     int ns = 0;
-    if (shadow_tau_wave<T, kCount>(d.g, lc, p, ldir, distance, &tau, &dropped, &ns) && !dropped)
+    if (shadow_tau_cells<T, kCount>(d.g, lc, p, ldir, distance, &tau, &dropped, &ns) && !dropped)
         for (int k = 0; k < 3; ++k) r[k] = inten[k] * (double)expf((float)(lc.ext[k] * tau));
     if ((threadIdx.x & 63) == 0) {
         for (int k = 0; k < 3; ++k) ray[3 * w + k] = r[k];
@@ -1200,8 +1201,8 @@ __global__ void k_member_local(int V, int bits, const uint64_t *__restrict__ sor
 // One workgroup of (MS/4)^2 threads per component of at most MS vertices.  The
 // component's optical-depth matrix is built in LDS from the adjacency (padding
 // vertices are isolated: +INF rows, zero diagonal), then closed in registers
-// (4 x 4 per thread) with the four-pivots-per-round-trip scheme of the pivot
-// tile (fw_pivot_tile): pivots in ascending vertex order, bit-identical to the
+// (4 x 4 per thread) in pivot groups like the pivot tile (fw_pivot_tile,
+// lg_fw_group.h): pivots in ascending vertex order, bit-identical to the
 // one-pivot loop.  Diffusion (LightGraph.h:554-587) follows from the registers:
 // column sums = row sums (D is symmetric), each reduced over the row's lanes in
 // a fixed order.
@@ -1214,9 +1215,10 @@ __global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
     unsigned long long *__restrict__ stats) {
     constexpr int CG = MS / 4;  // column groups = lanes sharing a row group
     constexpr int NT = CG * CG;
+    constexpr int KG = kPivotGroup;
     __shared__ float T[MS][MS + 1];
-    __shared__ float P[4][MS];
-    __shared__ float G[4][MS];
+    __shared__ float P[KG][MS];
+    __shared__ float G[KG][MS];
     __shared__ int mem[MS];
     __shared__ float Ln[MS][3];
     const int c = cls[blockIdx.x];
@@ -1245,41 +1247,20 @@ __global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) v[a][b] = T[4 * rg + a][4 * cg + b];
-    const int groups = (m + 3) / 4;
+    const int groups = (m + KG - 1) / KG;
     for (int g = 0; g < groups; ++g) {
-        const int k = 4 * g;
-        if (rg == g) {
+        const int k = KG * g;
+        if (rg / (KG / 4) == g) {  // the owners of rows k..k+KG-1 publish them
+            const int r0 = 4 * (rg % (KG / 4));
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
-                for (int b = 0; b < 4; ++b) P[a][4 * cg + b] = v[a][b];
+                for (int b = 0; b < 4; ++b) P[r0 + a][4 * cg + b] = v[a][b];
         }
         __syncthreads();
-        if (tid < MS) {
-            const int x = tid;
-            const float s01 = P[0][k + 1], s02 = P[0][k + 2], s03 = P[0][k + 3];
-            const float s12 = fminf(P[1][k + 2], s01 + s02);
-            const float s13 = fminf(P[1][k + 3], s01 + s03);
-            const float s23 = fminf(fminf(P[2][k + 3], s02 + s03), s12 + s13);
-            const float g0 = P[0][x];
-            const float g1 = fminf(P[1][x], s01 + g0);
-            const float g2 = fminf(fminf(P[2][x], s02 + g0), s12 + g1);
-            const float g3 = fminf(fminf(fminf(P[3][x], s03 + g0), s13 + g1), s23 + g2);
-            G[0][x] = g0;
-            G[1][x] = g1;
-            G[2][x] = g2;
-            G[3][x] = g3;
-        }
+        if (tid < MS) pivot_group_rows<KG, MS>(P, G, k, tid);
         __syncthreads();
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int i = 4 * rg + a, j = 4 * cg + b;
-                const float m01 = fminf(G[0][i] + G[0][j], G[1][i] + G[1][j]);
-                const float m23 = fminf(G[2][i] + G[2][j], G[3][i] + G[3][j]);
-                v[a][b] = fminf(v[a][b], fminf(m01, m23));
-            }
+        pivot_group_update<KG, MS>(G, rg, cg, v);
     }
     // diffusion: two reductions over the row's lanes (transmittances kept in
     // registers between them where the register budget allows)
