@@ -458,7 +458,8 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
                  int *Ktot_out);
 int32_t pl_march(lg_ctx *ctx, Pipeline &pl, int dtype, const MarchConsts &mc, const float *pts, int Ktot);
 // L_ready: when not null, an event after which L is complete (the stream
-// waits on it before the first kernel that reads L)
+// waits on it before the first kernel that reads L).  Runs after pl_graph on the
+// same pipeline: pl_graph's edge-list launch clears this stage's counters.
 int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, int V, int Ktot,
                       const double ext[3], const float *L, float *out, PipelineResult *res,
                       hipEvent_t L_ready = nullptr, const float *pts = nullptr);

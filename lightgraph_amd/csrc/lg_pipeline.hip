@@ -155,9 +155,8 @@ __device__ __forceinline__ int cell_axis(double p, double lo, double inv, int n)
 }
 
 // Block s: bounding box of solve s's points and its cell grid.
-__global__ void __launch_bounds__(1024) k_cell_setup(const SolveDesc *__restrict__ sd, const float *__restrict__ pts,
-                                                     double cell_min, CellDesc *__restrict__ cd) {
-    const SolveDesc &d = sd[blockIdx.x];
+__device__ __forceinline__ void cell_setup_block(const SolveDesc &d, const float *__restrict__ pts, double cell_min,
+                                                 CellDesc *__restrict__ out, CellDesc *__restrict__ out_lds) {
     float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     for (int v = d.voff + threadIdx.x; v < d.voff + d.n; v += blockDim.x)
         for (int c = 0; c < 3; ++c) {
@@ -205,42 +204,44 @@ __global__ void __launch_bounds__(1024) k_cell_setup(const SolveDesc *__restrict
             c.n[a] = n;
         }
         c.base = d.cell_base;
-        cd[blockIdx.x] = c;
+        *out = c;
+        if (out_lds) *out_lds = c;
     }
 }
 
-__global__ void k_cell_keys(const CellDesc *__restrict__ cd, const int *__restrict__ sid, int V,
-                            const float *__restrict__ pts, uint32_t *__restrict__ keys, int *__restrict__ vals) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= V) return;
-    const CellDesc c = cd[sid[v]];
-    const int x = cell_axis((double)pts[3 * v], c.lo[0], c.inv[0], c.n[0]);
-    const int y = cell_axis((double)pts[3 * v + 1], c.lo[1], c.inv[1], c.n[1]);
-    const int z = cell_axis((double)pts[3 * v + 2], c.lo[2], c.inv[2], c.n[2]);
-    keys[v] = (uint32_t)(c.base + (z * c.n[1] + y) * c.n[0] + x);
-    vals[v] = v;
+__global__ void __launch_bounds__(1024) k_cell_setup(const SolveDesc *__restrict__ sd, const float *__restrict__ pts,
+                                                     double cell_min, CellDesc *__restrict__ cd) {
+    cell_setup_block(sd[blockIdx.x], pts, cell_min, cd + blockIdx.x, nullptr);
 }
 
 // The neighbour grid of a small call in one workgroup: a counting sort of the
 // points by cell (LDS counters, ranks within a cell from the atomics), the
 // cell ranges, and the sorted point list.  Points keep no order inside a cell:
-// k_cand reads whole x-runs of cells and sorts each row itself.
+// k_cand reads whole x-runs of cells and sorts each row itself.  With `setup`
+// (a one-solve call) the workgroup first derives that solve's grid itself
+// (cell_setup_block), which saves the k_cell_setup launch.
 constexpr int kCellSortCells = 32768;  // LDS counters (128 KiB)
 constexpr int kCellSortMaxV = 65536;   // ranks in 16 bits
 
-__global__ void __launch_bounds__(1024) k_cell_sort_small(const CellDesc *__restrict__ cd, const int *__restrict__ sid,
+__global__ void __launch_bounds__(1024) k_cell_sort_small(CellDesc *__restrict__ cd, const int *__restrict__ sid,
                                                           int V, const float *__restrict__ pts, int cells,
                                                           uint32_t *__restrict__ packed, int *__restrict__ cstart,
                                                           int *__restrict__ cend, int *__restrict__ svals,
-                                                          int *__restrict__ zero_word) {
+                                                          int *__restrict__ zero_word, const SolveDesc *__restrict__ sd,
+                                                          double cell_min, int setup) {
     __shared__ int cnt[kCellSortCells];
     __shared__ int wsum[16];
+    __shared__ CellDesc s_cd;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (setup) {
+        cell_setup_block(sd[0], pts, cell_min, cd, &s_cd);
+        __syncthreads();
+    }
     if (tid == 0) *zero_word = 0;  // the caller's count slot after the last vertex
     for (int k = tid; k < cells; k += 1024) cnt[k] = 0;
     __syncthreads();
     for (int v = tid; v < V; v += 1024) {
-        const CellDesc c = cd[sid[v]];
+        const CellDesc c = setup ? s_cd : cd[sid[v]];
         const int x = cell_axis((double)pts[3 * v], c.lo[0], c.inv[0], c.n[0]);
         const int y = cell_axis((double)pts[3 * v + 1], c.lo[1], c.inv[1], c.n[1]);
         const int z = cell_axis((double)pts[3 * v + 2], c.lo[2], c.inv[2], c.n[2]);
@@ -277,6 +278,35 @@ __global__ void __launch_bounds__(1024) k_cell_sort_small(const CellDesc *__rest
     }
 }
 
+__global__ void k_cell_keys(const CellDesc *__restrict__ cd, const int *__restrict__ sid, int V,
+                            const float *__restrict__ pts, uint32_t *__restrict__ keys, int *__restrict__ vals) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const CellDesc c = cd[sid[v]];
+    const int x = cell_axis((double)pts[3 * v], c.lo[0], c.inv[0], c.n[0]);
+    const int y = cell_axis((double)pts[3 * v + 1], c.lo[1], c.inv[1], c.n[1]);
+    const int z = cell_axis((double)pts[3 * v + 2], c.lo[2], c.inv[2], c.n[2]);
+    keys[v] = (uint32_t)(c.base + (z * c.n[1] + y) * c.n[0] + x);
+    vals[v] = v;
+}
+
+struct ZeroList {
+    int *p[8];
+    long long n[8];
+    int count;
+};
+
+// Clears the regions of z with the whole grid (a kernel that also does other
+// work clears buffers its own launch does not read).
+__device__ __forceinline__ void grid_zero(const ZeroList &z) {
+    const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (int r = 0; r < z.count; ++r)
+        for (long long i = i0; i < z.n[r]; i += stride) z.p[r][i] = 0;
+}
+
+__global__ void k_zero(ZeroList z) { grid_zero(z); }
+
 __global__ void k_cell_bounds(int V, const uint32_t *__restrict__ sk, int *__restrict__ cstart, int *__restrict__ cend) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= V) return;
@@ -301,8 +331,9 @@ __global__ void __launch_bounds__(256) k_cand(const SolveDesc *__restrict__ sd, 
                                               const int *__restrict__ sid, int V, const float *__restrict__ pts,
                                               double r2, const int *__restrict__ cstart, const int *__restrict__ cend,
                                               const int *__restrict__ svals, const int *__restrict__ off,
-                                              int *__restrict__ count, int *__restrict__ fill) {
+                                              int *__restrict__ count, int *__restrict__ fill, ZeroList zero) {
     __shared__ int buf[4][kCandLds];
+    grid_zero(zero);  // the next stage's counters (not read here)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = blockIdx.x * 4 + wv;
     if (i >= V) return;
@@ -389,17 +420,19 @@ __global__ void k_colcount(int V, const int *__restrict__ off, const int *__rest
     for (int k = off[i]; k < off[i + 1]; ++k) atomicAdd(&colcnt[cand[k]], 1);
 }
 
-__global__ void k_risky(int V, int cap, const int *__restrict__ off, const int *__restrict__ colcnt,
-                        uint8_t *__restrict__ risky) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= V) return;
-    risky[i] = (off[i + 1] - off[i] + colcnt[i]) > cap ? 1 : 0;
+// A vertex is risky when its candidate degree (row plus column) exceeds the cap:
+// only its rows can lose edges to the greedy sweep.
+__device__ __forceinline__ bool vertex_risky(int v, int cap, const int *__restrict__ off,
+                                             const int *__restrict__ colcnt) {
+    return (off[v + 1] - off[v] + colcnt[v]) > cap;
 }
 
 // accepted = 1 for candidates with no risky endpoint; visit = rows the greedy
-// sweep has to look at.  Uncapped (cap <= 0): everything accepted.
+// sweep has to look at; risky[i] for the sweep.  Uncapped (cap <= 0):
+// everything accepted.
 __global__ void k_classify(int V, int cap, const int *__restrict__ off, const int *__restrict__ cand,
-                           const uint8_t *__restrict__ risky, int *__restrict__ accepted, int *__restrict__ visit) {
+                           const int *__restrict__ colcnt, uint8_t *__restrict__ risky, int *__restrict__ accepted,
+                           int *__restrict__ visit) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= V) return;
     const int s = off[i], e = off[i + 1];
@@ -408,10 +441,11 @@ __global__ void k_classify(int V, int cap, const int *__restrict__ off, const in
         visit[i] = 0;
         return;
     }
-    const bool ri = risky[i] != 0;
+    const bool ri = vertex_risky(i, cap, off, colcnt);
+    risky[i] = ri ? 1 : 0;
     bool any = false;
     for (int k = s; k < e; ++k) {
-        const bool rj = risky[cand[k]] != 0;
+        const bool rj = vertex_risky(cand[k], cap, off, colcnt);
         accepted[k] = (!ri && !rj) ? 1 : 0;
         any |= rj;
     }
@@ -794,7 +828,9 @@ __global__ void __launch_bounds__(64 * kGreedyWaves) k_greedy_waves(
 
 // Edge list (i, j) in row-major order from the accepted candidates.
 __global__ void k_edges(int V, const int *__restrict__ off, const int *__restrict__ cand,
-                        const int *__restrict__ accepted, const int *__restrict__ epos, int *__restrict__ edges) {
+                        const int *__restrict__ accepted, const int *__restrict__ epos, int *__restrict__ edges,
+                        ZeroList zero) {
+    grid_zero(zero);  // the component stage's counters (not read here)
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= V) return;
     for (int k = off[i]; k < off[i + 1]; ++k)
@@ -823,31 +859,11 @@ __global__ void k_march(const SolveDesc *__restrict__ sd, const int *__restrict_
     }
 }
 
-// ----------------------------------------------------------- adjacency ------
-__global__ void k_adj_deg(const int *__restrict__ edges, const float *__restrict__ w, const int *__restrict__ n_edges,
-                          int *__restrict__ deg) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= *n_edges || !(w[e] < INFINITY)) return;
-    atomicAdd(&deg[edges[2 * e]], 1);
-    atomicAdd(&deg[edges[2 * e + 1]], 1);
-}
-
-__global__ void k_adj_fill(const int *__restrict__ edges, const float *__restrict__ w, const int *__restrict__ n_edges,
-                           const int *__restrict__ aoff, int *__restrict__ cursor, int *__restrict__ nb,
-                           float *__restrict__ nw) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= *n_edges || !(w[e] < INFINITY)) return;
-    const int i = edges[2 * e], j = edges[2 * e + 1];
-    const float we = w[e];
-    int p = aoff[i] + atomicAdd(&cursor[i], 1);
-    nb[p] = j;
-    nw[p] = we;
-    p = aoff[j] + atomicAdd(&cursor[j], 1);
-    nb[p] = i;
-    nw[p] = we;
-}
-
-// ----------------------------------------------------------- components -----
+// ------------------------------------------------ adjacency and components --
+// Lock-free union-find over the finite edges of every solve of the call, one
+// thread per edge: hook the larger root under the smaller with a CAS that only
+// succeeds while it is still a root, so one pass joins every edge.  The final root
+// of a component is its smallest vertex, whatever the interleaving.
 __device__ __forceinline__ int uf_find(int *P, int x) {
     for (;;) {
         const int p = atomic_load_i32(&P[x]);
@@ -859,20 +875,8 @@ __device__ __forceinline__ int uf_find(int *P, int x) {
     }
 }
 
-// Lock-free union-find over the finite edges of every solve of the call, one
-// thread per edge: hook the larger root under the smaller with a CAS that only
-// succeeds while it is still a root, so one pass joins every edge.  The final root
-// of a component is its smallest vertex, whatever the interleaving.
-__global__ void k_uf_init(int V, int *__restrict__ P) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < V) atomic_store_i32(&P[v], v);
-}
-
-__global__ void k_uf_union(const int *__restrict__ edges, const float *__restrict__ w,
-                           const int *__restrict__ n_edges, int *__restrict__ P) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= *n_edges || !(w[e] < INFINITY)) return;
-    int a = edges[2 * e], b = edges[2 * e + 1];
+// Joins the trees of a and b: the larger root is linked below the smaller.
+__device__ __forceinline__ void uf_link(int *__restrict__ P, int a, int b) {
     for (;;) {
         a = uf_find(P, a);
         b = uf_find(P, b);
@@ -886,15 +890,44 @@ __global__ void k_uf_union(const int *__restrict__ edges, const float *__restric
     }
 }
 
-__global__ void k_uf_root(int V, int *__restrict__ P, int *__restrict__ root) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v < V) root[v] = uf_find(P, v);
+// Degrees of the finite edges; thread v < V also starts the union-find forest
+// (parent[v] = v) that k_adj_fill_union links.
+__global__ void k_adj_deg(const int *__restrict__ edges, const float *__restrict__ w, const int *__restrict__ n_edges,
+                          int *__restrict__ deg, int V, int *__restrict__ P) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < V) atomic_store_i32(&P[e], e);
+    if (e >= *n_edges || !(w[e] < INFINITY)) return;
+    atomicAdd(&deg[edges[2 * e]], 1);
+    atomicAdd(&deg[edges[2 * e + 1]], 1);
 }
 
-__global__ void k_keys(int V, int bits, const int *__restrict__ root, uint64_t *__restrict__ keys) {
+// Adjacency lists of the finite edges, and each such edge's union in the
+// union-find forest (parents started by k_adj_deg).
+__global__ void k_adj_fill_union(const int *__restrict__ edges, const float *__restrict__ w,
+                                 const int *__restrict__ n_edges, const int *__restrict__ aoff, int *__restrict__ cursor,
+                                 int *__restrict__ nb, float *__restrict__ nw, int *__restrict__ P) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= *n_edges || !(w[e] < INFINITY)) return;
+    const int i = edges[2 * e], j = edges[2 * e + 1];
+    const float we = w[e];
+    int p = aoff[i] + atomicAdd(&cursor[i], 1);
+    nb[p] = j;
+    nw[p] = we;
+    p = aoff[j] + atomicAdd(&cursor[j], 1);
+    nb[p] = i;
+    nw[p] = we;
+    uf_link(P, i, j);
+}
+
+// Each vertex's root (its component's smallest vertex) and its sort key
+// (root, vertex).
+__global__ void k_uf_root_keys(int V, int bits, int *__restrict__ P, int *__restrict__ root,
+                               uint64_t *__restrict__ keys) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
-    keys[v] = ((uint64_t)root[v] << bits) | (uint64_t)v;
+    const int r = uf_find(P, v);
+    root[v] = r;
+    keys[v] = ((uint64_t)r << bits) | (uint64_t)v;
 }
 
 __global__ void k_comp_flags(int V, int bits, const uint64_t *__restrict__ sorted, int *__restrict__ flag,
@@ -1380,20 +1413,19 @@ __device__ __forceinline__ int big_find(const int *__restrict__ prefix, int J, i
     return lo;
 }
 
-__global__ void k_big_a0(int J, const int *__restrict__ comp, const int *__restrict__ comp_off,
-                         BigJob *__restrict__ jobs) {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b < J) jobs[b].a0 = comp_off[comp[b]];
-}
-
-// rows of ldm: +INF off the diagonal, 0 on it, then the component's edges.
-__global__ void __launch_bounds__(256) k_big_fill(const BigJob *__restrict__ jobs, const int *__restrict__ ld_prefix,
-                                                  int J, const int *__restrict__ member, const int *__restrict__ local,
+// rows of ldm: +INF off the diagonal, 0 on it, then the component's edges.  The
+// component's first member a0 = comp_off[comp[q]] is gathered here, and row 0
+// records it in the job table for the diffusion kernels.
+__global__ void __launch_bounds__(256) k_big_fill(BigJob *__restrict__ jobs, const int *__restrict__ ld_prefix,
+                                                  int J, const int *__restrict__ comp, const int *__restrict__ comp_off,
+                                                  const int *__restrict__ member, const int *__restrict__ local,
                                                   const int *__restrict__ aoff, const int *__restrict__ anb,
                                                   const float *__restrict__ aw, float *__restrict__ T) {
     const int q = big_find(ld_prefix, J, (int)blockIdx.x);
-    const BigJob jb = jobs[q];
+    BigJob jb = jobs[q];
+    jb.a0 = comp_off[comp[q]];
     const int r = (int)blockIdx.x - ld_prefix[q];
+    if (r == 0 && threadIdx.x == 0) jobs[q].a0 = jb.a0;
     float *row = T + jb.toff + (long long)r * jb.ldm;
     for (int j = threadIdx.x; j < jb.ldm; j += blockDim.x) row[j] = (j == r) ? 0.0f : INFINITY;
     __syncthreads();
@@ -1645,20 +1677,8 @@ void Pipeline::release(lg_ctx *ctx) {
 
 // Clears up to 8 int regions in one launch (a stage's counters and flags
 // together instead of one fill dispatch each).
-struct ZeroList {
-    int *p[8];
-    long long n[8];
-    int count;
-};
-
-__global__ void k_zero(ZeroList z) {
-    const long long i0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long long stride = (long long)gridDim.x * blockDim.x;
-    for (int r = 0; r < z.count; ++r)
-        for (long long i = i0; i < z.n[r]; i += stride) z.p[r][i] = 0;
-}
-
-static int32_t zero_ints(lg_ctx *ctx, std::initializer_list<std::pair<int *, size_t>> regions) {
+// Up to 8 int regions to clear (null or empty ones are skipped).
+static int32_t zero_list(std::initializer_list<std::pair<int *, size_t>> regions, ZeroList *out, size_t *most_out) {
     ZeroList z{};
     size_t most = 0;
     for (const auto &r : regions) {
@@ -1669,6 +1689,15 @@ static int32_t zero_ints(lg_ctx *ctx, std::initializer_list<std::pair<int *, siz
         ++z.count;
         most = std::max(most, r.second);
     }
+    *out = z;
+    if (most_out) *most_out = most;
+    return LG_OK;
+}
+
+static int32_t zero_ints(lg_ctx *ctx, std::initializer_list<std::pair<int *, size_t>> regions) {
+    ZeroList z{};
+    size_t most = 0;
+    LG_TRY(zero_list(regions, &z, &most));
     if (z.count == 0) return LG_OK;
     const unsigned blocks = (unsigned)std::min<size_t>(1024, (most + 255) / 256);
     k_zero<<<blocks, 256, 0, ctx->stream>>>(z);
@@ -1842,14 +1871,19 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
     const double cell_min = sqrt(r2) * (1.0 + 1e-6) + 1e-30;
     hipEvent_t ev_c;
     ktimer_begin(ctx, "k_cand", &ev_c);
-    k_cell_setup<<<S, 1024, 0, st>>>(pl.sd, pts, cell_min, pl.cd);
-    LG_LAUNCHED("k_cell_setup");
     LG_TRY(grow(ctx, &pl.cstart, &pl.cells_cap, (size_t)pl.total_cells));
     LG_TRY(grow(ctx, &pl.cend, &pl.cend_cap, (size_t)pl.total_cells));
-    if (pl.total_cells <= kCellSortCells && V <= kCellSortMaxV) {
+    const bool small_sort = pl.total_cells <= kCellSortCells && V <= kCellSortMaxV;
+    // one solve with the small sort: its workgroup derives the grid too
+    const int fused_setup = (small_sort && S == 1) ? 1 : 0;
+    if (!fused_setup) {
+        k_cell_setup<<<S, 1024, 0, st>>>(pl.sd, pts, cell_min, pl.cd);
+        LG_LAUNCHED("k_cell_setup");
+    }
+    if (small_sort) {
         // one workgroup: counting sort, cell ranges and sorted list in one launch
         k_cell_sort_small<<<1, 1024, 0, st>>>(pl.cd, pl.sid, V, pts, (int)pl.total_cells, pl.ckeys, pl.cstart,
-                                              pl.cend, pl.cvals_sorted, pl.rowcnt + V);
+                                              pl.cend, pl.cvals_sorted, pl.rowcnt + V, pl.sd, cell_min, fused_setup);
         LG_LAUNCHED("k_cell_sort_small");
     } else {
         k_cell_keys<<<vb, 256, 0, st>>>(pl.cd, pl.sid, V, pts, pl.ckeys, pl.cvals);
@@ -1865,7 +1899,7 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
         LG_LAUNCHED("k_cell_bounds");
     }
     k_cand<false><<<wb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted, nullptr,
-                                      pl.rowcnt, nullptr);
+                                      pl.rowcnt, nullptr, ZeroList{});
     LG_LAUNCHED("k_cand");
     LG_TRY(scan_excl(ctx, pl, pl.rowcnt, pl.off, (size_t)V + 1, false));
     int Ktot = 0;
@@ -1877,20 +1911,20 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
     LG_TRY(grow(ctx, &pl.edges_, &pl.edges_cap, (size_t)2 * Ktot + 2));
     LG_TRY(grow(ctx, &pl.w_, &pl.w_cap, (size_t)Ktot + 1));
     const int cap = (max_connect > 0 && max_connect < 255) ? max_connect : 0;
-    LG_TRY(zero_ints(ctx, {{cap > 0 ? pl.colcnt : nullptr, (size_t)V},
-                           {pl.visit + V, 1},
-                           {pl.accepted + Ktot, 1},
-                           {cap > 0 ? pl.colfill : nullptr, (size_t)V},
-                           {pl.greedy_err, 1}}));
+    // the fill pass also clears the counters of the cap stage
+    ZeroList zl{};
+    LG_TRY(zero_list({{cap > 0 ? pl.colcnt : nullptr, (size_t)V},
+                      {pl.visit + V, 1},
+                      {pl.accepted + Ktot, 1},
+                      {cap > 0 ? pl.colfill : nullptr, (size_t)V},
+                      {pl.greedy_err, 1}},
+                     &zl, nullptr));
     k_cand<true><<<wb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted, pl.off,
-                                     nullptr, pl.cand);
+                                     nullptr, pl.cand, zl);
     LG_LAUNCHED("k_cand");
     ktimer_end(ctx, "k_cand", ev_c);
-    if (cap > 0) {
-        k_colcount<<<vb, 256, 0, st>>>(V, pl.off, pl.cand, pl.colcnt);
-        k_risky<<<vb, 256, 0, st>>>(V, cap, pl.off, pl.colcnt, pl.risky);
-    }
-    k_classify<<<vb, 256, 0, st>>>(V, cap, pl.off, pl.cand, pl.risky, pl.accepted, pl.visit);
+    if (cap > 0) k_colcount<<<vb, 256, 0, st>>>(V, pl.off, pl.cand, pl.colcnt);
+    k_classify<<<vb, 256, 0, st>>>(V, cap, pl.off, pl.cand, pl.colcnt, pl.risky, pl.accepted, pl.visit);
     LG_LAUNCHED("k_classify");
     if (cap > 0) {
         LG_TRY(scan_excl(ctx, pl, pl.visit, pl.visit_pos, (size_t)V + 1, false));
@@ -1921,7 +1955,11 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
         ktimer_end(ctx, "k_greedy", ev_g);
     }
     LG_TRY(scan_excl(ctx, pl, pl.accepted, pl.epos, (size_t)Ktot + 1, true));
-    k_edges<<<vb, 256, 0, st>>>(V, pl.off, pl.cand, pl.accepted, pl.epos, pl.edges_);
+    // the edge list's launch also clears the component stage's counters
+    ZeroList zc{};
+    LG_TRY(zero_list({{pl.adeg, (size_t)V + 1}, {pl.acur, (size_t)V}, {pl.flag + V, 1}, {pl.counters, 16}}, &zc,
+                     nullptr));
+    k_edges<<<vb, 256, 0, st>>>(V, pl.off, pl.cand, pl.accepted, pl.epos, pl.edges_, zc);
     LG_LAUNCHED("k_edges");
     ktimer_end(ctx, "k_graph", ev);
     *Ktot_out = Ktot;
@@ -1956,25 +1994,24 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     const int *d_nedges = pl.epos + Ktot;
     hipEvent_t ev;
     ktimer_begin(ctx, "k_components", &ev);
-    LG_TRY(zero_ints(ctx, {{pl.adeg, (size_t)V + 1}, {pl.acur, (size_t)V}, {pl.flag + V, 1}, {pl.counters, 16}}));
+    // adeg, acur, flag[V] and the counters were cleared by pl_graph's k_edges launch
     const unsigned eb = (unsigned)((Ktot + 255) / 256 + 1);
-    k_adj_deg<<<eb, 256, 0, st>>>(pl.edges_, pl.w_, d_nedges, pl.adeg);
+    // adjacency (degrees, scan, lists) and the union-find forest ride the same
+    // passes over the edges
+    hipEvent_t ev_u;
+    ktimer_begin(ctx, "k_union_find", &ev_u);
+    k_adj_deg<<<std::max(eb, vb), 256, 0, st>>>(pl.edges_, pl.w_, d_nedges, pl.adeg, V, pl.parent);
     LG_TRY(scan_excl(ctx, pl, pl.adeg, pl.aoff, (size_t)V + 1, false));
     LG_TRY(grow(ctx, &pl.anb, &pl.anb_cap, (size_t)2 * Ktot + 2));
     LG_TRY(grow(ctx, &pl.aw, &pl.aw_cap, (size_t)2 * Ktot + 2));
-    k_adj_fill<<<eb, 256, 0, st>>>(pl.edges_, pl.w_, d_nedges, pl.aoff, pl.acur, pl.anb, pl.aw);
-    hipEvent_t ev_u;
-    ktimer_begin(ctx, "k_union_find", &ev_u);
-    k_uf_init<<<vb, 256, 0, st>>>(V, pl.parent);
-    k_uf_union<<<eb, 256, 0, st>>>(pl.edges_, pl.w_, d_nedges, pl.parent);
-    k_uf_root<<<vb, 256, 0, st>>>(V, pl.parent, pl.root);
-    LG_LAUNCHED("k_uf_root");
+    k_adj_fill_union<<<eb, 256, 0, st>>>(pl.edges_, pl.w_, d_nedges, pl.aoff, pl.acur, pl.anb, pl.aw, pl.parent);
+    LG_LAUNCHED("k_adj_fill_union");
     ktimer_end(ctx, "k_union_find", ev_u);
     hipEvent_t ev_s;
     ktimer_begin(ctx, "k_comp_sort", &ev_s);
     int bits = 1;
     while ((1 << bits) < V) ++bits;
-    k_keys<<<vb, 256, 0, st>>>(V, bits, pl.root, pl.keys);
+    k_uf_root_keys<<<vb, 256, 0, st>>>(V, bits, pl.parent, pl.root, pl.keys);
     size_t sbytes = pl.tmp_vertex_bytes;
     LG_HIP(rocprim::radix_sort_keys(pl.tmp_vertex, sbytes, pl.keys, pl.keys_sorted, (size_t)V, 0, 2 * bits, st));
     k_comp_flags<<<vb, 256, 0, st>>>(V, bits, pl.keys_sorted, pl.flag, pl.member);
@@ -2120,7 +2157,7 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
         toff += (long long)ldm * ldm;
         ms[b] = m;
     }
-    // a0 = comp_off[c]: gathered on the device from the component ids
+    // a0 = comp_off[c]: gathered on the device (k_big_fill) from the component ids
     LG_TRY(grow(ctx, &pl.big_T, &pl.bigT_cap, (size_t)toff));
     LG_TRY(grow(ctx, &pl.big_ln, &pl.bigln_cap, (size_t)3 * mrows));
     const size_t jb_bytes = sizeof(BigJob) * nbig, pre_bytes = sizeof(int) * nbig;
@@ -2137,10 +2174,9 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     int *up_comp = (int *)(up.data() + jb_bytes + 2 * pre_bytes);
     for (int b = 0; b < nbig; ++b) up_comp[b] = big[b].first;
     LG_HIP(hipMemcpyAsync(d_jobs, up.data(), up.size(), hipMemcpyHostToDevice, st));
-    k_big_a0<<<(nbig + 255) / 256, 256, 0, st>>>(nbig, d_comp, pl.comp_off, d_jobs);
-    LG_LAUNCHED("k_big_a0");
     ktimer_begin(ctx, "k_big_fill", &ev);
-    k_big_fill<<<rows, 256, 0, st>>>(d_jobs, d_ldp, nbig, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, pl.big_T);
+    k_big_fill<<<rows, 256, 0, st>>>(d_jobs, d_ldp, nbig, d_comp, pl.comp_off, pl.member, pl.local, pl.aoff, pl.anb,
+                                     pl.aw, pl.big_T);
     LG_LAUNCHED("k_big_fill");
     ktimer_end(ctx, "k_big_fill", ev);
     for (int b = 0; b < nbig; ++b) mats[b] = pl.big_T + jobs[b].toff;
