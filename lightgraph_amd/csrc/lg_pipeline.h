@@ -2,6 +2,8 @@
 // lighting device functions, and the reusable device workspace.
 #pragma once
 
+#include <functional>
+
 #include <vector>
 
 #include "lg_internal.h"
@@ -454,8 +456,10 @@ int32_t pl_begin(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs,
 int32_t pl_sample(lg_ctx *ctx, Pipeline &pl, int V, int dtype, uint64_t seed, float *pts);
 int32_t pl_light(lg_ctx *ctx, Pipeline &pl, int V, int dtype, const LightConsts &lc, const lgl::DevLights &lights,
                  uint64_t env_seed, const float *pts, float *L);
+// after_grid (may be null) runs on the host right after the neighbour-grid
+// launches are queued (the solve forks its vertex light there).
 int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float *pts, double r2, int max_connect,
-                 int *Ktot_out);
+                 int *Ktot_out, const std::function<int32_t()> *after_grid = nullptr);
 int32_t pl_march(lg_ctx *ctx, Pipeline &pl, int dtype, const MarchConsts &mc, const float *pts, int Ktot);
 // L_ready: when not null, an event after which L is complete (the stream
 // waits on it before the first kernel that reads L).  Runs after pl_graph on the

@@ -1860,7 +1860,7 @@ int32_t pl_light(lg_ctx *ctx, Pipeline &pl, int V, int dtype, const LightConsts 
 
 // Candidate CSR, greedy cap and the edge list (pl.edges_, count at pl.epos[Ktot]).
 int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float *pts, double r2, int max_connect,
-                 int *Ktot_out) {
+                 int *Ktot_out, const std::function<int32_t()> *after_grid) {
     hipStream_t st = ctx->stream;
     const unsigned vb = (unsigned)((V + 255) / 256);
     hipEvent_t ev;
@@ -1898,6 +1898,7 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
         k_cell_bounds<<<vb, 256, 0, st>>>(V, pl.ckeys_sorted, pl.cstart, pl.cend);
         LG_LAUNCHED("k_cell_bounds");
     }
+    if (after_grid && *after_grid) LG_TRY((*after_grid)());
     k_cand<false><<<wb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted, nullptr,
                                       pl.rowcnt, nullptr, ZeroList{});
     LG_LAUNCHED("k_cand");

@@ -10,6 +10,7 @@
 #include "lg_internal.h"
 #include "lg_lights.h"
 #include "lg_pipeline.h"
+#include <functional>
 
 // Scratch of lg_diffuse_noalloc: the column sums (3 doubles per vertex) then the
 // normalised light (3 floats per vertex) of up to LG_DIFFUSE_NOALLOC_MAX_N
@@ -375,44 +376,47 @@ static int32_t solve_descs(lg_ctx *ctx, lg_solver *s, const std::vector<SolveDes
     if (t) LG_HIP(hipEventRecord(s->ev[1], ctx->stream));
     // The vertex light needs only the points, and its first reader is the
     // component stage: it runs on the side stream beside the graph, edge march
-    // and component sort (with per-stage timing on, the stages run in sequence
-    // so that each interval times one stage).
+    // and component sort, forked once the neighbour grid is queued (the grid's
+    // one-workgroup sort of a small call runs alone, before the light's waves
+    // take every CU).  With per-stage timing on, the stages run in sequence so
+    // that each interval times one stage.
     const bool overlap = !t && !pl.stats_on;
-    hipEvent_t L_ready = nullptr;
-    if (overlap) {
-        LG_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
-        LG_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
-        std::swap(ctx->stream, ctx->side);
-        const int32_t rc = pl_light(ctx, pl, V, dtype, pp.lc, pp.lights, pp.env_seed, pp.pts, pp.L);
-        std::swap(ctx->stream, ctx->side);
-        LG_HIP(hipEventRecord(ctx->ev_join, ctx->side));
-        L_ready = ctx->ev_join;
-        if (rc != LG_OK) {
-            (void)hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
-            return rc;
-        }
-    } else {
-        LG_TRY(pl_light(ctx, pl, V, dtype, pp.lc, pp.lights, pp.env_seed, pp.pts, pp.L));
-        if (pl.stats_on)
-            for (lg_grid *g : grids) LG_TRY(grid_touch_count(ctx, g, pl.stats, ST_LIGHT_BRICKS));
-    }
-    // every return from here on leaves the main stream ordered after the light
+    // every return after the fork leaves the main stream ordered after the light
     struct JoinGuard {
         lg_ctx *ctx;
         hipEvent_t ev;
         ~JoinGuard() {
             if (ev) (void)hipStreamWaitEvent(ctx->stream, ev, 0);
         }
-    } join{ctx, L_ready};
+    } join{ctx, nullptr};
+    std::function<int32_t()> fork_light;
+    if (overlap) {
+        fork_light = [&]() -> int32_t {
+            LG_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+            LG_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+            std::swap(ctx->stream, ctx->side);
+            const int32_t rc = pl_light(ctx, pl, V, dtype, pp.lc, pp.lights, pp.env_seed, pp.pts, pp.L);
+            std::swap(ctx->stream, ctx->side);
+            LG_HIP(hipEventRecord(ctx->ev_join, ctx->side));
+            join.ev = ctx->ev_join;
+            return rc;
+        };
+    } else {
+        LG_TRY(pl_light(ctx, pl, V, dtype, pp.lc, pp.lights, pp.env_seed, pp.pts, pp.L));
+        if (pl.stats_on)
+            for (lg_grid *g : grids) LG_TRY(grid_touch_count(ctx, g, pl.stats, ST_LIGHT_BRICKS));
+    }
     if (t) LG_HIP(hipEventRecord(s->ev[2], ctx->stream));
-    LG_TRY(pl_graph(ctx, pl, V, (int)descs.size(), nmax, pp.pts, pp.r2, pp.max_connect, &Ktot));
+    LG_TRY(pl_graph(ctx, pl, V, (int)descs.size(), nmax, pp.pts, pp.r2, pp.max_connect, &Ktot,
+                    overlap ? &fork_light : nullptr));
+    if (overlap && !join.ev) LG_TRY(fork_light());  // a graph stage that queued no grid
     if (t) LG_HIP(hipEventRecord(s->ev[3], ctx->stream));
     LG_TRY(pl_march(ctx, pl, dtype, pp.mc, pp.pts, Ktot));
     if (pl.stats_on)
         for (lg_grid *g : grids) LG_TRY(grid_touch_count(ctx, g, pl.stats, ST_EDGE_BRICKS));
     if (t) LG_HIP(hipEventRecord(s->ev[4], ctx->stream));
     PipelineResult res;
-    LG_TRY(pl_components(ctx, pl, descs, V, Ktot, pp.ext, pp.L, pp.out, &res, L_ready, pp.pts));
+    LG_TRY(pl_components(ctx, pl, descs, V, Ktot, pp.ext, pp.L, pp.out, &res, join.ev, pp.pts));
     if (t) LG_HIP(hipEventRecord(s->ev[5], ctx->stream));
     s->timed = t;
     s->buf.n = V;
