@@ -1,9 +1,11 @@
 """Device marches over a sparse volume against the oracle: three small blobs in
 a 160^3 grid, far apart, so that rays cross wide empty regions where the
 brick walk jumps over empty boxes (the spans, step origins and jitter draws
-must stay those of the cell-by-cell walk).  Bars as in the dense tests: edge
-optical depths within float32 rounding of the oracle's FP64 value, every
-render AOV within 1e-4 relative."""
+must stay those of the cell-by-cell walk) and the vertex light's batches jump
+over the empty box around their last sample (no sample in an existing brick
+may be skipped, the one-brick specks included).  Bars as in the dense tests:
+edge optical depths within float32 rounding of the oracle's FP64 value, vertex
+light within 1e-5, every render AOV within 1e-4 relative."""
 import numpy as np
 import pytest
 
@@ -55,6 +57,32 @@ def test_sparse_edge_march_parity(lga, ctx, oracle, sparse):
     assert rel_err(w[fin], tau[fin], floor=1e-30) < 2e-7
     for b in (pb, ebuf, wb):
         b.free()
+
+
+@pytest.mark.parametrize("step", [0.5, 2.0])
+def test_sparse_vertex_light_parity(lga, ctx, oracle, sparse, step):
+    """Points of all three blobs lit by point lights placed beyond the other
+    blobs (each shadow ray leaves its blob, crosses a gap and enters another
+    blob) and by a directional light: the batch jumps over empty boxes and the
+    samples in existing bricks after them equal the oracle's span walk."""
+    grid, dense = sparse
+    pts = oracle.sample_points(dense, 91, 0, 900)
+    sh = lga.default_shader(absorption=0.1, scattering=1.0)
+    p = lga.march_params(cutoff=0.005, shadow_step=step)
+    c = np.array(CENTRES)
+    lights = [lga.point_light(0.0, (1.0, 0.9, 0.8), 3e4, tuple(c[0] + 2.0 * (c[1] - c[0]))),
+              lga.point_light(0.0, (0.8, 0.9, 1.0), 3e4, tuple(c[1] + 2.0 * (c[2] - c[1]))),
+              lga.point_light(0.0, (0.9, 1.0, 0.9), 3e4, tuple(c[2] + 2.0 * (c[0] - c[2]))),
+              lga.parse_light("dir:0:1,1,1:1:-45,30,0")]
+    pb = ctx.upload(pts)
+    lb = ctx.alloc(len(pts) * 12)
+    lga.vertex_light(ctx, grid, None, sh, lights, p, 4564446, 0, pb, len(pts), lb)
+    got = lb.read(np.float32, (len(pts), 3))
+    want = oracle.vertex_light(dense, sh, p, lights, pts)
+    assert np.all(want > 0)
+    assert rel_err(got, want) < 1e-5
+    pb.free()
+    lb.free()
 
 
 @pytest.mark.parametrize("rotate", [(0.0, 0.0, 0.0), (6.0, -8.0, 3.0)])
