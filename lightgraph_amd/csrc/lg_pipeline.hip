@@ -1264,7 +1264,12 @@ __global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
         stat_add(stats, ST_SMALL_UPDATES, (unsigned long long)m * m * m);
         if (MS <= 64) stat_add(stats, MS <= 32 ? ST_SMALL_UPDATES_32 : ST_SMALL_UPDATES_64, (unsigned long long)m * m * m);
     }
-    for (int a = tid; a < m; a += NT) mem[a] = member[a0 + a];
+    for (int a = tid; a < m; a += NT) {
+        const int vm = member[a0 + a];
+        mem[a] = vm;
+        // the member's light, read early (the diffusion's first pass divides it)
+        for (int q = 0; q < 3; ++q) Ln[a][q] = L[3 * vm + q];
+    }
     for (int e = tid; e < MS * MS; e += NT) {
         const int i = e / MS, j = e - i * MS;
         T[i][j] = (i == j) ? 0.0f : INFINITY;
@@ -1295,58 +1300,40 @@ __global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
         __syncthreads();
         pivot_group_update<KG, MS>(G, rg, cg, v);
     }
-    // diffusion: two reductions over the row's lanes (transmittances kept in
-    // registers between them where the register budget allows)
+    // diffusion, one channel at a time: the row's 16 transmittances stay in
+    // registers between the two reductions over the row's lanes (column sums =
+    // row sums, D being symmetric), each in a fixed order
     const float ex[3] = {e0, e1, e2};
-    constexpr bool kKeep = MS < 128;
-    float P3[3][4][4];
-    auto tr = [&](int q, int a, int b) {
-        return kKeep ? P3[q][a][b] : ((4 * cg + b < m) ? trans_f(ex[q], v[a][b]) : 0.0f);
-    };
-    if (kKeep) {
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+        float tq[4][4];
 #pragma unroll
         for (int a = 0; a < 4; ++a)
 #pragma unroll
-            for (int b = 0; b < 4; ++b)
+            for (int b = 0; b < 4; ++b) tq[a][b] = (4 * cg + b < m) ? trans_f(ex[q], v[a][b]) : 0.0f;
 #pragma unroll
-                for (int q = 0; q < 3; ++q) P3[q][a][b] = (4 * cg + b < m) ? trans_f(ex[q], v[a][b]) : 0.0f;
-    }
+        for (int a = 0; a < 4; ++a) {
+            double sq = 0.0;
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        double sq[3];
+            for (int b = 0; b < 4; ++b) sq += (double)tq[a][b];
 #pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            sq[q] = 0.0;
-#pragma unroll
-            for (int b = 0; b < 4; ++b) sq[q] += (double)tr(q, a, b);
-#pragma unroll
-            for (int o = CG / 2; o > 0; o >>= 1) sq[q] += __shfl_xor(sq[q], o, CG);
+            for (int o = CG / 2; o > 0; o >>= 1) sq += __shfl_xor(sq, o, CG);
+            const int i = 4 * rg + a;
+            if (cg == 0 && i < m) Ln[i][q] = (float)((double)Ln[i][q] / (sq < 1.0 ? 1.0 : sq));
         }
-        const int i = 4 * rg + a;
-        if (cg == 0 && i < m) {
-            const int vi = mem[i];
-            for (int q = 0; q < 3; ++q) Ln[i][q] = (float)((double)L[3 * vi + q] / (sq[q] < 1.0 ? 1.0 : sq[q]));
-        }
-    }
-    __syncthreads();
+        __syncthreads();
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        double oq[3];
-#pragma unroll
-        for (int q = 0; q < 3; ++q) {
-            oq[q] = 0.0;
+        for (int a = 0; a < 4; ++a) {
+            double oq = 0.0;
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
                 const int j = 4 * cg + b;
-                if (j < m) oq[q] += (double)Ln[j][q] * (double)tr(q, a, b);
+                if (j < m) oq += (double)Ln[j][q] * (double)tq[a][b];
             }
 #pragma unroll
-            for (int o = CG / 2; o > 0; o >>= 1) oq[q] += __shfl_xor(oq[q], o, CG);
-        }
-        const int i = 4 * rg + a;
-        if (cg == 0 && i < m) {
-            const int vi = mem[i];
-            for (int q = 0; q < 3; ++q) out[3 * vi + q] = (float)oq[q];
+            for (int o = CG / 2; o > 0; o >>= 1) oq += __shfl_xor(oq, o, CG);
+            const int i = 4 * rg + a;
+            if (cg == 0 && i < m) out[3 * mem[i] + q] = (float)oq;
         }
     }
     const SolveDesc &d = sd[sid[mem[0]]];
