@@ -201,8 +201,11 @@ __device__ __forceinline__ bool shadow_tau_cells(const lgd::GridView &g, const L
         const double t = (kb + (double)lane) * h;
         const bool act = t <= is.tmax;
         const double px = is.e[0] + is.d[0] * t, py = is.e[1] + is.d[1] * t, pz = is.e[2] + is.d[2] * t;
-        const int bx = ((int)floor(px)) >> 3, by = ((int)floor(py)) >> 3, bz = ((int)floor(pz)) >> 3;
-        // positions up to the clipped end lie inside the storage (its border included)
+        // positions up to the clipped end lie inside the storage (its border
+        // included); the clamp only guards the index against rounding
+        const int bx = lgd::clampi(((int)floor(px)) >> 3, -1, g.NBx - 2);
+        const int by = lgd::clampi(((int)floor(py)) >> 3, -1, g.NBy - 2);
+        const int bz = lgd::clampi(((int)floor(pz)) >> 3, -1, g.NBz - 2);
         const int code = act ? (int)g.skip[((long long)(bz + 1) * g.NBy + (by + 1)) * g.NBx + (bx + 1)] : 0;
         double x = 0.0;
         float rho = 0.0f;
