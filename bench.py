@@ -735,6 +735,14 @@ def main():
         for key, v in solver.stats().items():
             work[key] = work.get(key, 0) + v
     roof = kernel_roofline(timed, work, WORKLOAD["workload"])
+    if roof:
+        # how dominant the dominant kernel is: its share of the kernel time of
+        # the pass with every kernel timed (the two streams' kernels summed)
+        total = sum(t["total_ms"] for nm, t in times.items() if nm in KERNELS)
+        share = times[dominant]["total_ms"] / total if total > 0 else None
+        roof["share_of_roofline_kernels_time"] = round(share, 4) if share is not None else None
+        roof["note"] = ("largest of the solve's march and closure kernels by total time; the plain solve is a chain of "
+                        "~46 small launches, see kernel_rooflines for every kernel's fraction")
     # every roofline kernel of the solve, from the pass with every kernel timed
     # (its per-kernel events add a little to each launch)
     kroofs = {}
