@@ -439,6 +439,12 @@ struct Pipeline {
     bool nd_order = false;   // large components closed in nested-dissection vertex order
     double *ray = nullptr;  // per (vertex, light sample) shadowed intensity
     size_t ray_cap = 0;
+    // spatial ray order of large calls: keys, sorted keys, vertices, sorted vertices
+    // (four runs of lsort_cap / 4 words) and the sort's temporary storage
+    uint32_t *lkeys = nullptr;
+    size_t lsort_cap = 0;
+    uint8_t *tmp_light = nullptr;
+    size_t tmp_light_cap = 0;
     // neighbour grid
     CellDesc *cd = nullptr;
     uint32_t *ckeys = nullptr, *ckeys_sorted = nullptr;
@@ -464,8 +470,9 @@ int32_t pl_light(lg_ctx *ctx, Pipeline &pl, int V, int dtype, const LightConsts 
 int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float *pts, double r2, int max_connect,
                  int *Ktot_out, const std::function<int32_t()> *after_grid = nullptr);
 int32_t pl_march(lg_ctx *ctx, Pipeline &pl, int dtype, const MarchConsts &mc, const float *pts, int Ktot);
-// L_ready: when not null, an event after which L is complete (the stream
-// waits on it before the first kernel that reads L).  Runs after pl_graph on the
+// L_ready: when not null, an event after which L is complete (each stream
+// waits on it before its first kernel that reads L: the component search and
+// the blocked closure of the large components run before it).  Runs after pl_graph on the
 // same pipeline: pl_graph's edge-list launch clears this stage's counters.
 int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &descs, int V, int Ktot,
                       const double ext[3], const float *L, float *out, PipelineResult *res,

@@ -80,17 +80,57 @@ __global__ void k_light(const SolveDesc *__restrict__ sd, const int *__restrict_
     for (int k = 0; k < 3; ++k) out[3 * v + k] = (float)acc[k];
 }
 
+// Morton key of a vertex's position on a 32^3 lattice over its grid's clip box
+// (15 bits): the order in which large calls trace their shadow rays.
+__device__ __forceinline__ uint32_t morton_spread5(uint32_t x) {
+    x &= 0x1fu;
+    x = (x | (x << 8)) & 0x0000f00fu;
+    x = (x | (x << 4)) & 0x000c30c3u;
+    x = (x | (x << 2)) & 0x00249249u;
+    return x;
+}
+
+__global__ void k_light_keys(const SolveDesc *__restrict__ sd, const int *__restrict__ sid, int V,
+                             const float *__restrict__ pts, uint32_t *__restrict__ keys, int *__restrict__ vals) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V) return;
+    const lgd::GridView &g = sd[sid[v]].g;
+    uint32_t key = 0;
+    for (int c = 0; c < 3; ++c) {
+        const double f = floor(((double)pts[3 * v + c] - g.bbmin[c]) * 32.0 / (g.bbmax[c] - g.bbmin[c]));
+        const uint32_t q = f > 0.0 ? (f < 31.0 ? (uint32_t)f : 31u) : 0u;
+        key |= morton_spread5(q) << c;
+    }
+    keys[v] = key;
+    vals[v] = v;
+}
+
 // Vertex lighting for exact steps: one wave per (vertex, light sample) shadow
 // ray, then one lane per vertex sums the samples in the order of
-// computeLighting (LightGraph.h:446-497).
+// computeLighting (LightGraph.h:446-497).  Without `order` the waves take the
+// rays vertex-major.  With it (large calls) they take them light-sample-major,
+// the vertices of each sample in the given spatial order, so that concurrent
+// rays share bricks in L2 (workgroups stay round-robin over the XCDs: giving
+// each XCD one contiguous run of the order measured slower).  Which wave traces
+// a ray does not change the ray or where its result goes.
 template <typename T, bool kCount>
 __global__ void __launch_bounds__(256) k_light_rays(const SolveDesc *__restrict__ sd, const int *__restrict__ sid,
                                                     int V, int nt, LightConsts lc, lgl::DevLights L, uint32_t ek0,
                                                     uint32_t ek1, const float *__restrict__ pts,
-                                                    double *__restrict__ ray, unsigned long long *__restrict__ stats) {
+                                                    const int *__restrict__ order, double *__restrict__ ray,
+                                                    unsigned long long *__restrict__ stats) {
     const long long w = (long long)blockIdx.x * (blockDim.x >> 6) + lgd::wave_uniform((int)(threadIdx.x >> 6));
     if (w >= (long long)V * nt) return;
-    const int v = (int)(w / nt), t = (int)(w - (long long)v * nt);
+    int v, t;
+    if (order) {
+        t = (int)(w / V);
+        v = order[w - (long long)t * V];
+    } else {
+        v = (int)(w / nt);
+        t = (int)(w - (long long)v * nt);
+    }
+    v = lgd::wave_uniform(v);
+    const long long slot = (long long)v * nt + t;
     int li;
     uint32_t ew;
     light_task(L, t, &li, &ew);
@@ -109,7 +149,7 @@ __global__ void __launch_bounds__(256) k_light_rays(const SolveDesc *__restrict_
     if (shadow_tau_cells<T, kCount>(d.g, lc, p, ldir, distance, &tau, &dropped, &ns) && !dropped)
         for (int k = 0; k < 3; ++k) r[k] = inten[k] * (double)expf((float)(lc.ext[k] * tau));
     if ((threadIdx.x & 63) == 0) {
-        for (int k = 0; k < 3; ++k) ray[3 * w + k] = r[k];
+        for (int k = 0; k < 3; ++k) ray[3 * slot + k] = r[k];
         if (stats) {
             stat_add(stats, ST_SHADOW_RAYS, 1ull);
             stat_add(stats, ST_SHADOW_SAMPLES, (unsigned long long)ns);
@@ -322,50 +362,68 @@ __device__ __forceinline__ double pdist2(const float *__restrict__ pts, int i, i
     return dx * dx + dy * dy + dz * dz;
 }
 
-constexpr int kCandLds = 512;  // candidates per row sorted in LDS; longer rows take the ordered scan
+constexpr int kCandLds = 512;  // candidates per wave sorted in LDS; longer rows take the ordered scan
+constexpr int kCandGroupedFromV = 65536;  // calls with this many vertices search with 4 lanes per vertex
+constexpr int kLightSortFromV = 65536;    // calls with this many vertices trace their shadow rays in spatial order
 
-// One wave per global vertex i.  FILL = false: count candidates into count[i];
-// FILL = true: write them, ascending, at off[i].
-template <bool FILL>
+// G lanes per global vertex i (64 / G vertices per wave; G = 64 for small calls,
+// where few vertices have to fill the GPU, 4 for large ones, where the rows are
+// short and a whole wave per row leaves most lanes idle).  FILL = false: count
+// candidates into count[i]; FILL = true: write them, ascending, at off[i].
+template <bool FILL, int G>
 __global__ void __launch_bounds__(256) k_cand(const SolveDesc *__restrict__ sd, const CellDesc *__restrict__ cd,
                                               const int *__restrict__ sid, int V, const float *__restrict__ pts,
                                               double r2, const int *__restrict__ cstart, const int *__restrict__ cend,
                                               const int *__restrict__ svals, const int *__restrict__ off,
                                               int *__restrict__ count, int *__restrict__ fill, ZeroList zero) {
+    static_assert(G == 64 || G == 32 || G == 16 || G == 8 || G == 4, "lanes per vertex");
+    constexpr int GPW = 64 / G;          // vertices per wave
+    constexpr int CAP = kCandLds / GPW;  // LDS slots per vertex
     __shared__ int buf[4][kCandLds];
     grid_zero(zero);  // the next stage's counters (not read here)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int i = blockIdx.x * 4 + wv;
-    if (i >= V) return;
-    const CellDesc c = cd[sid[i]];
-    const int cx = cell_axis((double)pts[3 * i], c.lo[0], c.inv[0], c.n[0]);
-    const int cy = cell_axis((double)pts[3 * i + 1], c.lo[1], c.inv[1], c.n[1]);
-    const int cz = cell_axis((double)pts[3 * i + 2], c.lo[2], c.inv[2], c.n[2]);
-    const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < c.n[0] ? cx + 1 : c.n[0] - 1;
-    // the 9 x-runs as position ranges of the sorted points
+    const int grp = lane / G, gl = lane % G;
+    const int i = (blockIdx.x * 4 + wv) * GPW + grp;
+    if (__builtin_amdgcn_readfirstlane((blockIdx.x * 4 + wv) * GPW) >= V) return;  // the wave's first vertex
+    const bool live = i < V;
+    const uint64_t gmask = G == 64 ? ~0ull : (((1ull << G) - 1ull) << (grp * G));
     int rs[9], pre[10];
     pre[0] = 0;
+    int total = 0;
+    if (live) {
+        const CellDesc c = cd[sid[i]];
+        const int cx = cell_axis((double)pts[3 * i], c.lo[0], c.inv[0], c.n[0]);
+        const int cy = cell_axis((double)pts[3 * i + 1], c.lo[1], c.inv[1], c.n[1]);
+        const int cz = cell_axis((double)pts[3 * i + 2], c.lo[2], c.inv[2], c.n[2]);
+        const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < c.n[0] ? cx + 1 : c.n[0] - 1;
+        // the 9 x-runs as position ranges of the sorted points
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-        const int y = cy + (r % 3) - 1, z = cz + (r / 3) - 1;
-        int s0 = 0x7fffffff, e0 = 0;
-        if (y >= 0 && y < c.n[1] && z >= 0 && z < c.n[2]) {
-            const int k0 = c.base + (z * c.n[1] + y) * c.n[0];
-            for (int x = x0; x <= x1; ++x) {
-                const int e = cend[k0 + x];
-                if (e > 0) {
-                    s0 = min(s0, cstart[k0 + x]);
-                    e0 = max(e0, e);
+        for (int r = 0; r < 9; ++r) {
+            const int y = cy + (r % 3) - 1, z = cz + (r / 3) - 1;
+            int s0 = 0x7fffffff, e0 = 0;
+            if (y >= 0 && y < c.n[1] && z >= 0 && z < c.n[2]) {
+                const int k0 = c.base + (z * c.n[1] + y) * c.n[0];
+                for (int x = x0; x <= x1; ++x) {
+                    const int e = cend[k0 + x];
+                    if (e > 0) {
+                        s0 = min(s0, cstart[k0 + x]);
+                        e0 = max(e0, e);
+                    }
                 }
             }
+            rs[r] = s0;
+            pre[r + 1] = pre[r] + (e0 > 0 ? e0 - s0 : 0);
         }
-        rs[r] = s0;
-        pre[r + 1] = pre[r] + (e0 > 0 ? e0 - s0 : 0);
+        total = pre[9];
     }
-    const int total = pre[9];
+    // the wave walks as many G-wide steps as its longest group needs
+    int tmax = total;
+    if (G < 64)
+        for (int o = 32; o >= G; o >>= 1) tmax = max(tmax, __shfl_xor(tmax, o, 64));
     int cnt = 0;
-    for (int f0 = 0; f0 < total; f0 += 64) {
-        const int f = f0 + lane;
+    int *gbuf = buf[wv] + grp * CAP;
+    for (int f0 = 0; f0 < tmax; f0 += G) {
+        const int f = f0 + gl;
         bool hit = false;
         int j = -1;
         if (f < total) {
@@ -376,39 +434,40 @@ __global__ void __launch_bounds__(256) k_cand(const SolveDesc *__restrict__ sd, 
             j = svals[pos];
             hit = j > i && pdist2(pts, i, j) < r2;
         }
-        const uint64_t m = __ballot(hit);
+        const uint64_t m = __ballot(hit) & gmask;
         if (FILL && hit) {
             const int slot = cnt + __popcll(m & ((1ull << lane) - 1ull));
-            if (slot < kCandLds) buf[wv][slot] = j;
+            if (slot < CAP) gbuf[slot] = j;
         }
         cnt += __popcll(m);
     }
     if (!FILL) {
-        if (lane == 0) count[i] = cnt;
+        if (live && gl == 0) count[i] = cnt;
         return;
     }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (!live) return;
     const int o = off[i];
-    if (cnt <= kCandLds) {
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (cnt <= CAP) {
         // rank of each candidate among the row's (distinct) candidates
-        for (int e = lane; e < cnt; e += 64) {
-            const int val = buf[wv][e];
+        for (int e = gl; e < cnt; e += G) {
+            const int val = gbuf[e];
             int rank = 0;
-            for (int x = 0; x < cnt; ++x) rank += buf[wv][x] < val;
+            for (int x = 0; x < cnt; ++x) rank += gbuf[x] < val;
             fill[o + rank] = val;
         }
         return;
     }
-    // long row: ordered scan over the whole solve
+    // long row: ordered scan over the whole solve, G points at a time
     const SolveDesc &d = sd[sid[i]];
     const int end = d.voff + d.n;
     int w = 0;
-    for (int base = i + 1; base < end; base += 64) {
-        const int jj = base + lane;
+    for (int base = i + 1; base < end; base += G) {
+        const int jj = base + gl;
         const bool h = jj < end && pdist2(pts, i, jj) < r2;
-        const uint64_t m = __ballot(h);
+        const uint64_t m = __ballot(h) & gmask;
         if (h) fill[o + w + __popcll(m & ((1ull << lane) - 1ull))] = jj;
         w += __popcll(m);
     }
@@ -949,15 +1008,16 @@ __global__ void k_comp_offsets(int V, const int *__restrict__ flag, const int *_
 
 // local index of every vertex inside its component, the (component, size) pairs
 // of the components too large for the LDS kernels, and the lists of the other
-// components by size class (2..32, 33..64, 65..128).  An isolated vertex has
-// P = [1], so its diffused radiance is its own light; it is written here.
+// components by size class (2..32, 33..64, 65..128), and the isolated vertices,
+// listed from the end of the 2..32 list (cls16[V - 1 - slot]: that list holds at
+// most (V - n1) / 2 components, so the two never meet).  Reads no vertex light,
+// so it need not wait for it.
 __global__ void k_comp_local(int V, const SolveDesc *__restrict__ sd, const int *__restrict__ sid,
                              const int *__restrict__ flag, const int *__restrict__ cid_excl,
                              const int *__restrict__ comp_off, const int *__restrict__ member, int *__restrict__ local,
                              int *__restrict__ big_list, int *__restrict__ counters, int *__restrict__ cls16,
-                             int *__restrict__ cls48, int *__restrict__ cls128, const float *__restrict__ L,
-                             float *__restrict__ out, const int *__restrict__ n_comp, const int *__restrict__ n_edges,
-                             const int *__restrict__ greedy_err) {
+                             int *__restrict__ cls48, int *__restrict__ cls128, const int *__restrict__ n_comp,
+                             const int *__restrict__ n_edges, const int *__restrict__ greedy_err) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k == 0) {  // for the host's one readback
         counters[8] = *n_comp;
@@ -981,7 +1041,7 @@ __global__ void k_comp_local(int V, const SolveDesc *__restrict__ sd, const int 
         cls16[atomicAdd(&counters[1], 1)] = c;
     } else {
         const int v = member[k];
-        for (int q = 0; q < 3; ++q) out[3 * v + q] = L[3 * v + q];
+        cls16[V - 1 - atomicAdd(&counters[4], 1)] = v;
         const SolveDesc &d = sd[sid[v]];
         const int li = v - d.voff;
         if (d.dense_D) d.dense_D[(long long)li * d.ld + li] = 0.0f;
@@ -1245,7 +1305,7 @@ __global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
     const int *__restrict__ comp_off, const int *__restrict__ member, const int *__restrict__ local,
     const int *__restrict__ aoff, const int *__restrict__ anb, const float *__restrict__ aw,
     const float *__restrict__ L, float e0, float e1, float e2, float *__restrict__ out,
-    unsigned long long *__restrict__ stats) {
+    unsigned long long *__restrict__ stats, int ncls, const int *__restrict__ single_end, int nsingle) {
     constexpr int CG = MS / 4;  // column groups = lanes sharing a row group
     constexpr int NT = CG * CG;
     constexpr int KG = kPivotGroup;
@@ -1254,6 +1314,16 @@ __global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
     __shared__ float G[KG][MS];
     __shared__ int mem[MS];
     __shared__ float Ln[MS][3];
+    if ((int)blockIdx.x >= ncls) {
+        // workgroups past the list: isolated vertices, whose P = [1] makes the
+        // diffused radiance the vertex light itself (single_end[-1 - i])
+        const int idx = ((int)blockIdx.x - ncls) * NT + (int)threadIdx.x;
+        if (idx < nsingle) {
+            const int v = single_end[-1 - idx];
+            for (int q = 0; q < 3; ++q) out[3 * v + q] = L[3 * v + q];
+        }
+        return;
+    }
     const int c = cls[blockIdx.x];
     const int a0 = comp_off[c];
     const int m = comp_off[c + 1] - a0;
@@ -1645,7 +1715,7 @@ void Pipeline::release_vertex(lg_ctx *ctx) {
 void Pipeline::release(lg_ctx *ctx) {
     release_vertex(ctx);
     void *ps[] = {cand,  accepted, epos, tmp_cand, edges_, w_,   anb,    aw,      big_T,
-                  big_ln, big_meta, ray, cstart,   cend,   stats, colslot, slot_info};
+                  big_ln, big_meta, ray, cstart,   cend,   stats, colslot, slot_info, lkeys, tmp_light};
     for (void *p : ps) dfree(ctx, p);
     stats = nullptr;
     cstart = cend = nullptr;
@@ -1657,6 +1727,9 @@ void Pipeline::release(lg_ctx *ctx) {
     big_meta = nullptr;
     big_ln = nullptr;
     ray = nullptr;
+    lkeys = nullptr;
+    tmp_light = nullptr;
+    lsort_cap = tmp_light_cap = 0;
     tmp_cand = nullptr;
     cand_cap = accepted_cap = epos_cap = tmp_cand_cap = edges_cap = w_cap = anb_cap = aw_cap = 0;
     bigT_cap = bigln_cap = bigmeta_cap = ray_cap = 0;
@@ -1816,16 +1889,32 @@ int32_t pl_light(lg_ctx *ctx, Pipeline &pl, int V, int dtype, const LightConsts 
         const long long rays = (long long)V * nt;
         if (rays > 0) {
             LG_TRY(grow(ctx, &pl.ray, &pl.ray_cap, (size_t)rays * 3));
+            const int *order = nullptr;
+            if (V >= kLightSortFromV) {
+                // spatial order of the rays: Morton keys of the vertices, sorted
+                LG_TRY(grow(ctx, &pl.lkeys, &pl.lsort_cap, (size_t)V * 4));
+                uint32_t *keys = pl.lkeys, *keys_sorted = pl.lkeys + pl.lsort_cap / 4;
+                int *vals = (int *)(pl.lkeys + 2 * (pl.lsort_cap / 4)), *vals_sorted = (int *)(pl.lkeys + 3 * (pl.lsort_cap / 4));
+                size_t sbytes = 0;
+                LG_HIP(rocprim::radix_sort_pairs(nullptr, sbytes, keys, keys_sorted, vals, vals_sorted, (size_t)V, 0, 15,
+                                                 ctx->stream));
+                LG_TRY(grow(ctx, &pl.tmp_light, &pl.tmp_light_cap, sbytes + 256));
+                k_light_keys<<<(unsigned)((V + 255) / 256), 256, 0, ctx->stream>>>(pl.sd, pl.sid, V, pts, keys, vals);
+                LG_LAUNCHED("k_light_keys");
+                LG_HIP(rocprim::radix_sort_pairs(pl.tmp_light, sbytes, keys, keys_sorted, vals, vals_sorted, (size_t)V, 0,
+                                                 15, ctx->stream));
+                order = vals_sorted;
+            }
             hipEvent_t ev_r;
             ktimer_begin(ctx, "k_light_rays", &ev_r);
             for_dtype(dtype, [&](auto tag) {
                 using T = decltype(tag);
                 if (pl.stats_on)
                     k_light_rays<T, true><<<(unsigned)((rays + 3) / 4), 256, 0, ctx->stream>>>(
-                        pl.sd, pl.sid, V, nt, lc, lights, ek0, ek1, pts, pl.ray, pl.stats);
+                        pl.sd, pl.sid, V, nt, lc, lights, ek0, ek1, pts, order, pl.ray, pl.stats);
                 else
                     k_light_rays<T, false><<<(unsigned)((rays + 3) / 4), 256, 0, ctx->stream>>>(
-                        pl.sd, pl.sid, V, nt, lc, lights, ek0, ek1, pts, pl.ray, nullptr);
+                        pl.sd, pl.sid, V, nt, lc, lights, ek0, ek1, pts, order, pl.ray, nullptr);
             });
             LG_LAUNCHED("k_light_rays");
             ktimer_end(ctx, "k_light_rays", ev_r);
@@ -1886,8 +1975,15 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
         LG_LAUNCHED("k_cell_bounds");
     }
     if (after_grid && *after_grid) LG_TRY((*after_grid)());
-    k_cand<false><<<wb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted, nullptr,
-                                      pl.rowcnt, nullptr, ZeroList{});
+    // rows are short: large calls give each vertex 4 lanes, small ones a wave
+    const bool grouped = V >= kCandGroupedFromV;
+    const unsigned cb = grouped ? (unsigned)((V + 63) / 64) : wb;
+    if (grouped)
+        k_cand<false, 4><<<cb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted,
+                                              nullptr, pl.rowcnt, nullptr, ZeroList{});
+    else
+        k_cand<false, 64><<<cb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted,
+                                              nullptr, pl.rowcnt, nullptr, ZeroList{});
     LG_LAUNCHED("k_cand");
     LG_TRY(scan_excl(ctx, pl, pl.rowcnt, pl.off, (size_t)V + 1, false));
     int Ktot = 0;
@@ -1907,8 +2003,12 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
                       {cap > 0 ? pl.colfill : nullptr, (size_t)V},
                       {pl.greedy_err, 1}},
                      &zl, nullptr));
-    k_cand<true><<<wb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted, pl.off,
-                                     nullptr, pl.cand, zl);
+    if (grouped)
+        k_cand<true, 4><<<cb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted,
+                                             pl.off, nullptr, pl.cand, zl);
+    else
+        k_cand<true, 64><<<cb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted,
+                                             pl.off, nullptr, pl.cand, zl);
     LG_LAUNCHED("k_cand");
     ktimer_end(ctx, "k_cand", ev_c);
     if (cap > 0) k_colcount<<<vb, 256, 0, st>>>(V, pl.off, pl.cand, pl.colcnt);
@@ -2005,10 +2105,8 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     k_comp_flags<<<vb, 256, 0, st>>>(V, bits, pl.keys_sorted, pl.flag, pl.member);
     LG_TRY(scan_excl(ctx, pl, pl.flag, pl.cid, (size_t)V + 1, false));
     k_comp_offsets<<<vb, 256, 0, st>>>(V, pl.flag, pl.cid, pl.comp_off, pl.cid + V);
-    if (L_ready) LG_HIP(hipStreamWaitEvent(st, L_ready, 0));  // k_comp_local is the first reader of L
     k_comp_local<<<vb, 256, 0, st>>>(V, pl.sd, pl.sid, pl.flag, pl.cid, pl.comp_off, pl.member, pl.local, pl.big_list,
-                                     pl.counters, pl.cls16, pl.cls48, pl.cls128, L, out, pl.cid + V, d_nedges,
-                                     pl.greedy_err);
+                                     pl.counters, pl.cls16, pl.cls48, pl.cls128, pl.cid + V, d_nedges, pl.greedy_err);
     LG_LAUNCHED("k_comp_local");
     const int max_big = V / (kSmallComp + 1);
     if (pl.nd_order && max_big > 0) {
@@ -2034,7 +2132,7 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     }
     ktimer_end(ctx, "k_comp_sort", ev_s);
     // counts and the large-component pairs come back in one synchronisation
-    // (counters[0..3], component count, edge count, then the large-component
+    // (counters[0..4], component count, edge count, then the large-component
     // pairs: big_list = counters + 16)
     std::vector<int> rb((size_t)16 + 2 * (size_t)max_big);
     LG_HIP(hipMemcpyAsync(rb.data(), pl.counters, sizeof(int) * rb.size(), hipMemcpyDeviceToHost, st));
@@ -2044,7 +2142,7 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
         set_error("pl_graph: a row of the parallel greedy cap waited past its bound");
         return LG_ERR_HIP;
     }
-    const int C = rb[8], nbig = rb[0], n16 = rb[1], n48 = rb[2], n128 = rb[3];
+    const int C = rb[8], nbig = rb[0], n16 = rb[1], n48 = rb[2], n128 = rb[3], n1 = rb[4];
     res->n_edges_total = rb[9];
     res->n_components = C;
     res->n_big = nbig;
@@ -2069,14 +2167,16 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     // matrices), so when large ones follow they run on the side stream, beside
     // the blocked closure, and the main stream joins them before returning.
     const float e0 = (float)ext[0], e1 = (float)ext[1], e2 = (float)ext[2];
-    const bool side = nbig > 0 && (n128 + n48 + n16) > 0;
+    const bool side = nbig > 0 && (n128 + n48 + n16 + n1) > 0;
     auto small_closures = [&]() -> int32_t {
-        // three kernels, one per size class, each timed on its own
+        // the first readers of the vertex light: three kernels, one per size
+        // class, each timed on its own (the isolated vertices ride the last)
+        if (L_ready) LG_HIP(hipStreamWaitEvent(ctx->stream, L_ready, 0));
         if (n128 > 0) {
             ktimer_begin(ctx, "k_comp_small_128", &ev);
             k_comp_small<kSmallComp><<<n128, 1024, 0, ctx->stream>>>(
                 pl.sd, pl.sid, pl.cls128, pl.comp_off, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, L, e0, e1, e2, out,
-                pl.stats_on ? pl.stats : nullptr);
+                pl.stats_on ? pl.stats : nullptr, n128, nullptr, 0);
             LG_LAUNCHED("k_comp_small<128>");
             ktimer_end(ctx, "k_comp_small_128", ev);
         }
@@ -2084,15 +2184,15 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
             ktimer_begin(ctx, "k_comp_small_64", &ev);
             k_comp_small<64><<<n48, 256, 0, ctx->stream>>>(
                 pl.sd, pl.sid, pl.cls48, pl.comp_off, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, L, e0, e1, e2, out,
-                pl.stats_on ? pl.stats : nullptr);
+                pl.stats_on ? pl.stats : nullptr, n48, nullptr, 0);
             LG_LAUNCHED("k_comp_small<64>");
             ktimer_end(ctx, "k_comp_small_64", ev);
         }
-        if (n16 > 0) {
+        if (n16 + n1 > 0) {
             ktimer_begin(ctx, "k_comp_small_32", &ev);
-            k_comp_small<32><<<n16, 64, 0, ctx->stream>>>(
+            k_comp_small<32><<<(unsigned)(n16 + (n1 + 63) / 64), 64, 0, ctx->stream>>>(
                 pl.sd, pl.sid, pl.cls16, pl.comp_off, pl.member, pl.local, pl.aoff, pl.anb, pl.aw, L, e0, e1, e2, out,
-                pl.stats_on ? pl.stats : nullptr);
+                pl.stats_on ? pl.stats : nullptr, n16, pl.cls16 + V, n1);
             LG_LAUNCHED("k_comp_small<32>");
             ktimer_end(ctx, "k_comp_small_32", ev);
         }
@@ -2170,6 +2270,7 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     for (int b = 0; b < nbig; ++b) mats[b] = pl.big_T + jobs[b].toff;
     LG_TRY(fw_solve_batched_impl(ctx, mats, ms, pl.stats_on ? pl.stats : nullptr));
     ktimer_begin(ctx, "k_diffuse", &ev);
+    if (L_ready) LG_HIP(hipStreamWaitEvent(st, L_ready, 0));  // the closure above needs no vertex light
     k_big_rowsum<<<mrows, 256, 0, st>>>(d_jobs, d_mp, nbig, pl.big_T, pl.member, L, e0, e1, e2, pl.big_ln);
     LG_LAUNCHED("k_big_rowsum");
     k_big_diffuse<<<mrows, 256, 0, st>>>(d_jobs, d_mp, nbig, pl.big_T, pl.member, pl.big_ln, e0, e1, e2, out);

@@ -234,6 +234,33 @@ def test_solve_batch_bfs_order(lga, ctx, volumes, order):
     plain.destroy()
 
 
+def test_solve_batch_large_call_equals_plain(lga, ctx, volumes):
+    """A call of 81,920 vertices (40 solves of 2048 points over the six volumes)
+    takes the large-call forms of two stages -- the candidate search with 4
+    lanes per vertex and the shadow rays traced light by light in Morton order
+    of their vertices -- and each solve still equals its plain solve bit for
+    bit (with the work counters on, too)."""
+    sh = lga.default_shader(absorption=0.1, scattering=(1.0, 0.9, 0.8))
+    p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5)
+    lights = [lga.point_light(0.0, (1.0, 0.9, 0.8), 4.0e4, (64.0, 200.0, 64.0)),
+              lga.parse_light("dir:0:1,0.89,0.79:0.7:-41.8,-180,0")]
+    n = 2048
+    items = [(volumes[k % len(volumes)], n, 200 + k) for k in range(40)]
+    assert len(items) * n >= 65536
+    bs = lga.Solver(ctx, n * len(items))
+    plain = lga.Solver(ctx, n)
+    for stats in (False, True):
+        bs.solve_batch(items, sh, lights, p, radius=6.0, seed=854123, stats=stats)
+        got = bs.batch_results()
+        for k, (g, nk, solve) in enumerate(items):
+            plain.solve(g, sh, lights, p, n=nk, radius=6.0, seed=854123, solve=solve)
+            ref = plain.results(want_D=False)
+            for key in ("points", "L", "diffuse", "edges", "weights"):
+                np.testing.assert_array_equal(got[k][key], ref[key], err_msg=f"item {k} ({key}, stats {stats})")
+    bs.destroy()
+    plain.destroy()
+
+
 def test_solve_batch_rejects_mixed_dtypes(lga, ctx, volumes):
     d = volumes[0].read_dense()
     g16 = lga.Grid.from_dense(ctx, d, dtype=lga.LG_F16)
