@@ -175,7 +175,11 @@ __device__ __forceinline__ bool shadow_tau(const lgd::GridView &g, const LightCo
 // needs an exact tie between two FP64 times.)  The 64 lanes take 64 consecutive
 // k, each loading its brick's box code together with its 8 taps: one memory
 // round trip per 64 samples and no brick walk.  A batch whose samples all lie in
-// empty bricks jumps over the empty box around its last sample.
+// empty bricks jumps over the empty box around its last sample.  The optical
+// depth is summed in rows of 16 samples aligned to multiples of 16 in k: within
+// a row by a prefix sum, rows in order.  shadow_tau_rows sums the same rows in
+// the same order (skipped and idle samples add exact zeros), so both forms give
+// every ray the same bits.
 template <typename T, bool kCount>
 __device__ __forceinline__ bool shadow_tau_cells(const lgd::GridView &g, const LightConsts &lc, const double p[3],
                                                  const double ldir[3], double distance, double *tau_out,
@@ -196,10 +200,14 @@ __device__ __forceinline__ bool shadow_tau_cells(const lgd::GridView &g, const L
     *tau_out = 0.0;
     if (!(is.tmax - is.t0 > lgd::kDeltaF)) return true;  // isect_march: no span on a ray this short
     const double h = lc.h;
-    double kb = lgd::wave_uniform(ceil(is.t0 / h));
+    // steps start at multiples of 16 (samples before the first k are idle), so
+    // that the 16-sample rows below are the same whatever the lanes per ray
+    const double kfirst = lgd::wave_uniform(ceil(is.t0 / h));
+    double kb = floor(kfirst * (1.0 / 16.0)) * 16.0;
     for (;;) {
-        const double t = (kb + (double)lane) * h;
-        const bool act = t <= is.tmax;
+        const double k = kb + (double)lane;
+        const double t = k * h;
+        const bool act = k >= kfirst && t <= is.tmax;
         const double px = is.e[0] + is.d[0] * t, py = is.e[1] + is.d[1] * t, pz = is.e[2] + is.d[2] * t;
         // positions up to the clipped end lie inside the storage (its border
         // included); the clamp only guards the index against rounding
@@ -217,8 +225,20 @@ __device__ __forceinline__ bool shadow_tau_cells(const lgd::GridView &g, const L
             if (!(dens < lc.cutoff)) x = dens * h;
         }
         const unsigned long long tmask = __ballot(take);
-        const double pre = lgd::wave_prefix_sum(x);
-        const unsigned long long dmask = __ballot(take && tau + pre > lc.tau_star);
+        // optical depth in rows of 16 samples: a prefix sum within each row
+        // (DPP row_shr 1, 2, 4, 8), the rows added in order
+        double pre = x;
+        pre += lgd::dpp_move<0x111, 0xf>(pre);
+        pre += lgd::dpp_move<0x112, 0xf>(pre);
+        pre += lgd::dpp_move<0x114, 0xf>(pre);
+        pre += lgd::dpp_move<0x118, 0xf>(pre);
+        double trow[5];
+        trow[0] = tau;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) trow[r + 1] = trow[r] + lgd::wave_lane64(pre, 16 * r + 15);
+        const int row = lane >> 4;
+        const double tbase = row == 0 ? trow[0] : (row == 1 ? trow[1] : (row == 2 ? trow[2] : trow[3]));
+        const unsigned long long dmask = __ballot(take && tbase + pre > lc.tau_star);
         // the samples the sequential march takes: up to the one that drops the ray
         const unsigned long long upto = dmask ? (2ull << (__ffsll((long long)dmask) - 1)) - 1ull : ~0ull;
         *nsamp += __popcll(tmask & upto);
@@ -228,7 +248,7 @@ __device__ __forceinline__ bool shadow_tau_cells(const lgd::GridView &g, const L
             *tau_out = tau;
             return true;
         }
-        tau = tau + lgd::wave_lane64(pre, 63);
+        tau = trow[4];
         if (!((__ballot(act) >> 63) & 1ull)) break;  // the ray ends inside this batch
         double knext = kb + 64.0;
         if (tmask == 0ull) {
@@ -247,13 +267,110 @@ __device__ __forceinline__ bool shadow_tau_cells(const lgd::GridView &g, const L
                 }
                 // every sample before texit (less a rounding margin) is in the box
                 const double kj = floor((texit - 1e-9 * (1.0 + fabs(texit))) / h);
-                knext = kj > knext ? kj : knext;
+                knext = kj > knext ? floor(kj * (1.0 / 16.0)) * 16.0 : knext;
             }
         }
         kb = lgd::wave_uniform(knext);
     }
     *tau_out = tau;
     return true;
+}
+
+// The same march with one 16-lane DPP row per ray (four rays per wave): each
+// row takes 16 consecutive k per step, from a multiple of 16, and sums them as
+// shadow_tau_cells sums its rows, so each ray's optical depth, dropout and
+// sample count are bit-identical to that form's.  Rows whose ray has ended (or
+// that hold no ray: live false on entry) keep running the wave's shuffles with
+// no samples.
+template <typename T, bool kCount>
+__device__ __forceinline__ void shadow_tau_rows(const lgd::GridView &g, const LightConsts &lc, const double p[3],
+                                                const double ldir[3], double distance, bool live, bool *hit,
+                                                double *tau_out, bool *dropped, int *nsamp) {
+    const int lane = threadIdx.x & 63, gl = lane & 15, row = lane >> 4;
+    const double sd[3] = {-ldir[0], -ldir[1], -ldir[2]};
+    lgd::Isect is;
+    *hit = false;
+    *dropped = false;
+    *tau_out = 0.0;
+    if (live) {
+        *hit = lgd::isect_world_ray(g, is, p, sd, lgd::kDeltaD, distance);
+        // isect_march: no span on a ray this short
+        live = *hit && is.tmax - is.t0 > lgd::kDeltaF;
+    }
+    const double h = lc.h;
+    const double kfirst = live ? ceil(is.t0 / h) : 0.0;
+    double kb = floor(kfirst * (1.0 / 16.0)) * 16.0;  // rows aligned as in shadow_tau_cells
+    double tau = 0.0;
+    for (;;) {
+        if (__ballot(live) == 0ull) break;
+        const double k = kb + (double)gl;
+        const double t = k * h;
+        const bool act = live && k >= kfirst && t <= is.tmax;
+        double px = 0.0, py = 0.0, pz = 0.0;
+        int bx = 0, by = 0, bz = 0, code = 0;
+        double x = 0.0;
+        bool take = false;
+        if (act) {
+            px = is.e[0] + is.d[0] * t;
+            py = is.e[1] + is.d[1] * t;
+            pz = is.e[2] + is.d[2] * t;
+            bx = lgd::clampi(((int)floor(px)) >> 3, -1, g.NBx - 2);
+            by = lgd::clampi(((int)floor(py)) >> 3, -1, g.NBy - 2);
+            bz = lgd::clampi(((int)floor(pz)) >> 3, -1, g.NBz - 2);
+            code = (int)g.skip[((long long)(bz + 1) * g.NBy + (by + 1)) * g.NBx + (bx + 1)];
+            const float rho = lgd::trilinear<T, false>(g, px, py, pz);
+            take = code >= lgd::kSkipExists;
+            if (take) {
+                double dens = (double)rho * lc.ds_sds_scds[0] * lc.ds_sds_scds[1] * lc.ds_sds_scds[2];
+                dens = (lc.dmax < dens) ? lc.dmax : dens;
+                if (!(dens < lc.cutoff)) x = dens * h;
+            }
+        }
+        double pre = x;
+        pre += lgd::dpp_move<0x111, 0xf>(pre);  // row_shr:1
+        pre += lgd::dpp_move<0x112, 0xf>(pre);  // row_shr:2
+        pre += lgd::dpp_move<0x114, 0xf>(pre);  // row_shr:4
+        pre += lgd::dpp_move<0x118, 0xf>(pre);  // row_shr:8
+        const unsigned tm = (unsigned)(__ballot(take) >> (16 * row)) & 0xffffu;
+        const unsigned dm = (unsigned)(__ballot(take && tau + pre > lc.tau_star) >> (16 * row)) & 0xffffu;
+        const unsigned am = (unsigned)(__ballot(act) >> (16 * row)) & 0xffffu;
+        const double last = __shfl(pre, 15, 16);
+        const int lcode = __shfl(code, 15, 16), lbx = __shfl(bx, 15, 16), lby = __shfl(by, 15, 16),
+                  lbz = __shfl(bz, 15, 16);
+        if (!live) continue;
+        // the samples the sequential march takes: up to the one that drops the ray
+        const unsigned upto = dm ? (2u << (__ffs((int)dm) - 1)) - 1u : 0xffffu;
+        *nsamp += __popc(tm & upto);
+        if (kCount && take && ((upto >> gl) & 1u)) lgd::mark_taps(g, px, py, pz);
+        if (dm != 0u) {
+            *dropped = true;
+            live = false;
+            continue;
+        }
+        tau = tau + last;
+        if (!((am >> 15) & 1u)) {  // the ray ends inside this step
+            live = false;
+            continue;
+        }
+        double knext = kb + 16.0;
+        if (tm == 0u && lcode >= lgd::kSkipMin) {
+            // all 16 samples in empty bricks: the box of Chebyshev radius d - 1
+            // bricks around the last one is empty (its code d >= 2)
+            const int cb[3] = {lbx, lby, lbz};
+            double texit = lgd::kDblMax;
+            for (int c = 0; c < 3; ++c) {
+                if (is.d[c] == 0.0) continue;
+                const double plane = 8.0 * (double)(is.d[c] > 0.0 ? cb[c] + lcode : cb[c] - lcode + 1);
+                const double tc = (plane - is.e[c]) * is.inv[c];
+                texit = tc < texit ? tc : texit;
+            }
+            // every sample before texit (less a rounding margin) is in the box
+            const double kj = floor((texit - 1e-9 * (1.0 + fabs(texit))) / h);
+            knext = kj > knext ? floor(kj * (1.0 / 16.0)) * 16.0 : knext;
+        }
+        kb = knext;
+    }
+    *tau_out = tau;
 }
 
 // Emission term of computeLighting (LightGraph.h:494-496).

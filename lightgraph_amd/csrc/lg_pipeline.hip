@@ -157,6 +157,63 @@ __global__ void __launch_bounds__(256) k_light_rays(const SolveDesc *__restrict_
     }
 }
 
+// The 16-lane form for large calls: DPP row r of the wave (lanes 16r..16r+15)
+// traces ray 4 * wave + r, in the same order (vertex-major, or sample-major over
+// `order`), with the same results as k_light_rays.  Run alone it is slower (a
+// C2 solve's 16K rays fill only a quarter of the waves); beside the main
+// stream's work of a large batch it leaves that work more of the GPU.
+template <typename T, bool kCount>
+__global__ void __launch_bounds__(256) k_light_rays16(const SolveDesc *__restrict__ sd, const int *__restrict__ sid,
+                                                      int V, int nt, LightConsts lc, lgl::DevLights L, uint32_t ek0,
+                                                      uint32_t ek1, const float *__restrict__ pts,
+                                                      const int *__restrict__ order, double *__restrict__ ray,
+                                                      unsigned long long *__restrict__ stats) {
+    const long long w0 = ((long long)blockIdx.x * (blockDim.x >> 6) + lgd::wave_uniform((int)(threadIdx.x >> 6))) * 4;
+    const long long total = (long long)V * nt;
+    if (w0 >= total) return;  // whole waves leave together
+    const int lane = threadIdx.x & 63;
+    const long long w = w0 + (lane >> 4);
+    const bool valid = w < total;
+    double ldir[3] = {0.0, 0.0, 1.0}, inten[3] = {0.0, 0.0, 0.0}, distance = 0.0;
+    double p[3] = {0.0, 0.0, 0.0};
+    long long slot = 0;
+    const SolveDesc *d = sd;
+    if (valid) {
+        int v, t;
+        if (order) {
+            t = (int)(w / V);
+            v = order[w - (long long)t * V];
+        } else {
+            v = (int)(w / nt);
+            t = (int)(w - (long long)v * nt);
+        }
+        slot = (long long)v * nt + t;
+        int li;
+        uint32_t ew;
+        light_task(L, t, &li, &ew);
+        d = &sd[sid[v]];
+        for (int c = 0; c < 3; ++c) p[c] = (double)pts[3 * v + c];
+        lgd::Stream rng;
+        rng.init((uint32_t)(v - d->voff), d->solve, lgd::DOM_VERTEX_ENV, ((uint64_t)ek1 << 32) | ek0);
+        rng.skip(ew);
+        lgl::illuminate(L.l[li], p, rng, ldir, inten, distance);
+    }
+    double tau;
+    bool hit, dropped;
+    int ns = 0;
+    shadow_tau_rows<T, kCount>(d->g, lc, p, ldir, distance, valid, &hit, &tau, &dropped, &ns);
+    if (valid && (lane & 15) == 0) {
+        double r[3] = {0.0, 0.0, 0.0};
+        if (hit && !dropped)
+            for (int k = 0; k < 3; ++k) r[k] = inten[k] * (double)expf((float)(lc.ext[k] * tau));
+        for (int k = 0; k < 3; ++k) ray[3 * slot + k] = r[k];
+        if (stats) {
+            stat_add(stats, ST_SHADOW_RAYS, 1ull);
+            stat_add(stats, ST_SHADOW_SAMPLES, (unsigned long long)ns);
+        }
+    }
+}
+
 __global__ void k_light_combine(const SolveDesc *__restrict__ sd, const int *__restrict__ sid, int V, int nt,
                                 LightConsts lc, lgl::DevLights L, const float *__restrict__ pts,
                                 const double *__restrict__ ray, float *__restrict__ out) {
@@ -365,6 +422,7 @@ __device__ __forceinline__ double pdist2(const float *__restrict__ pts, int i, i
 constexpr int kCandLds = 512;  // candidates per wave sorted in LDS; longer rows take the ordered scan
 constexpr int kCandGroupedFromV = 65536;  // calls with this many vertices search with 4 lanes per vertex
 constexpr int kLightSortFromV = 65536;    // calls with this many vertices trace their shadow rays in spatial order
+constexpr int kLightRows16FromV = 65536;  // calls with this many vertices trace 4 shadow rays per wave
 
 // G lanes per global vertex i (64 / G vertices per wave; G = 64 for small calls,
 // where few vertices have to fill the GPU, 4 for large ones, where the rows are
@@ -1909,7 +1967,15 @@ int32_t pl_light(lg_ctx *ctx, Pipeline &pl, int V, int dtype, const LightConsts 
             ktimer_begin(ctx, "k_light_rays", &ev_r);
             for_dtype(dtype, [&](auto tag) {
                 using T = decltype(tag);
-                if (pl.stats_on)
+                if (V >= kLightRows16FromV) {
+                    const unsigned gb = (unsigned)((rays + 15) / 16);
+                    if (pl.stats_on)
+                        k_light_rays16<T, true><<<gb, 256, 0, ctx->stream>>>(pl.sd, pl.sid, V, nt, lc, lights, ek0,
+                                                                           ek1, pts, order, pl.ray, pl.stats);
+                    else
+                        k_light_rays16<T, false><<<gb, 256, 0, ctx->stream>>>(pl.sd, pl.sid, V, nt, lc, lights, ek0,
+                                                                            ek1, pts, order, pl.ray, nullptr);
+                } else if (pl.stats_on)
                     k_light_rays<T, true><<<(unsigned)((rays + 3) / 4), 256, 0, ctx->stream>>>(
                         pl.sd, pl.sid, V, nt, lc, lights, ek0, ek1, pts, order, pl.ray, pl.stats);
                 else

@@ -237,9 +237,11 @@ def test_solve_batch_bfs_order(lga, ctx, volumes, order):
 def test_solve_batch_large_call_equals_plain(lga, ctx, volumes):
     """A call of 81,920 vertices (40 solves of 2048 points over the six volumes)
     takes the large-call forms of two stages -- the candidate search with 4
-    lanes per vertex and the shadow rays traced light by light in Morton order
-    of their vertices -- and each solve still equals its plain solve bit for
-    bit (with the work counters on, too)."""
+    lanes per vertex, and the shadow rays traced light by light in Morton order
+    of their vertices, four rays per wave, where a plain solve gives each ray a
+    wave -- and each solve still equals its plain solve bit for bit; with the
+    work counters on, the batch's shadow rays and samples are the plain
+    solves' sums."""
     sh = lga.default_shader(absorption=0.1, scattering=(1.0, 0.9, 0.8))
     p = lga.march_params(cutoff=0.005, primary_step=1.0, shadow_step=0.5)
     lights = [lga.point_light(0.0, (1.0, 0.9, 0.8), 4.0e4, (64.0, 200.0, 64.0)),
@@ -252,11 +254,21 @@ def test_solve_batch_large_call_equals_plain(lga, ctx, volumes):
     for stats in (False, True):
         bs.solve_batch(items, sh, lights, p, radius=6.0, seed=854123, stats=stats)
         got = bs.batch_results()
+        want = {"shadow_rays": 0, "shadow_samples": 0}
         for k, (g, nk, solve) in enumerate(items):
-            plain.solve(g, sh, lights, p, n=nk, radius=6.0, seed=854123, solve=solve)
+            plain.solve(g, sh, lights, p, n=nk, radius=6.0, seed=854123, solve=solve, stats=stats)
             ref = plain.results(want_D=False)
             for key in ("points", "L", "diffuse", "edges", "weights"):
                 np.testing.assert_array_equal(got[k][key], ref[key], err_msg=f"item {k} ({key}, stats {stats})")
+            if stats:
+                st = plain.stats()
+                for key in want:
+                    want[key] += st[key]
+        if stats:
+            st = bs.stats()
+            assert want["shadow_rays"] == len(items) * n * 2
+            for key in want:
+                assert st[key] == want[key], key
     bs.destroy()
     plain.destroy()
 
