@@ -749,7 +749,8 @@ def main():
     for nm in KERNELS:
         r = kernel_roofline(times, work, WORKLOAD["workload"], name=nm)
         if r and r.get("frac") is not None:
-            kroofs[nm] = {k: r.get(k) for k in ("bound", "frac", "achieved", "unit", "avg_launch_ms", "launches")}
+            kroofs[nm] = {k: r.get(k) for k in ("bound", "frac", "achieved", "unit", "avg_launch_ms", "launches",
+                                                "traffic", "traffic_gbs") if k in r}
     b = solver.buffers()
     lat = []
     for k in range(args.latency_solves):
