@@ -89,6 +89,8 @@ struct lg_ctx {
     size_t scratch_bytes = 0;
     void *scratch1 = nullptr;
     size_t scratch1_bytes = 0;
+    // device copy of the last blob scene's ellipsoids (lg_grid_generate)
+    void *blobs_dev = nullptr;
     // work counters of the last lg_render (kernel timing on), device u64 slots
     unsigned long long *render_stats = nullptr;
     uint64_t render_chunks = 0;

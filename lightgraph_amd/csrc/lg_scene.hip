@@ -59,12 +59,13 @@ static std::vector<lgn::Blob> make_blobs(const lg_scene_spec &s) {
     return out;
 }
 
-__constant__ lgn::Blob c_blobs[64];
+constexpr int kMaxBlobs = 64;  // lg_scene_spec.blobs is at most this
 
 // One thread per voxel of the (unpadded) grid, written into storage brick layout.
-// mode 0: sphere*fBm, 1: raw blob field, 2: plume density (+temperature).
-__global__ void k_generate(int mode, int n, int octaves, uint32_t seed, float radius, int nblobs, int NBx, int NBy,
-                           float *__restrict__ out, float *__restrict__ temp_out) {
+// mode 0: sphere*fBm, 1: raw blob field (blobs: the context's device copy of the
+// scene's ellipsoids), 2: plume density (+temperature).
+__global__ void k_generate(int mode, int n, int octaves, uint32_t seed, float radius, const lgn::Blob *__restrict__ blobs,
+                           int nblobs, int NBx, int NBy, float *__restrict__ out, float *__restrict__ temp_out) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const long long total = (long long)n * n * n;
     if (i >= total) return;
@@ -73,7 +74,7 @@ __global__ void k_generate(int mode, int n, int octaves, uint32_t seed, float ra
     if (mode == 0) {
         v = lgn::sphere_fbm(x, y, z, 0.5f * (float)(n - 1), radius, octaves, seed);
     } else if (mode == 1) {
-        v = lgn::blobs_raw(x, y, z, c_blobs, nblobs, octaves, seed);
+        v = lgn::blobs_raw(x, y, z, blobs, nblobs, octaves, seed);
     } else {
         lgn::plume(x, y, z, n, octaves, seed, &v, &t);
     }
@@ -137,6 +138,7 @@ static int32_t check_spec(const lg_scene_spec *s) {
     LG_CHECK_ARG(s->kind >= LG_SCENE_SPHERE_FBM && s->kind <= LG_SCENE_PLUME, "scene: unknown kind %d", s->kind);
     LG_CHECK_ARG(s->octaves >= 1 && s->octaves <= 12, "scene: octaves out of range");
     LG_CHECK_ARG(s->kind != LG_SCENE_BLOBS_FBM || (s->blobs >= 1 && s->blobs <= 64), "scene: blobs must be in [1, 64]");
+    static_assert(kMaxBlobs == 64, "the bound checked above");
     return LG_OK;
 }
 
@@ -163,9 +165,14 @@ int32_t lg_grid_generate(lg_ctx *ctx, const lg_scene_spec *spec, lg_grid **densi
     const int mode = spec->kind == LG_SCENE_SPHERE_FBM ? 0 : (spec->kind == LG_SCENE_BLOBS_FBM ? 1 : 2);
     std::vector<lgn::Blob> blobs;
     if (rc == LG_OK && mode == 1) {
+        // each context keeps its own device copy: contexts on other threads
+        // (the serving workers) generate their volumes at the same time
         blobs = make_blobs(*spec);
-        if (hipMemcpyToSymbolAsync(HIP_SYMBOL(c_blobs), blobs.data(), blobs.size() * sizeof(lgn::Blob), 0,
-                                   hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+        if (!ctx->blobs_dev) rc = dmalloc(ctx, sizeof(lgn::Blob) * kMaxBlobs, &ctx->blobs_dev);
+        if (rc == LG_OK && (blobs.size() > (size_t)kMaxBlobs ||
+                            hipMemcpyAsync(ctx->blobs_dev, blobs.data(), blobs.size() * sizeof(lgn::Blob),
+                                           hipMemcpyHostToDevice, ctx->stream) != hipSuccess ||
+                            hipStreamSynchronize(ctx->stream) != hipSuccess)) {
             set_error("blob upload failed");
             rc = LG_ERR_HIP;
         }
@@ -174,8 +181,9 @@ int32_t lg_grid_generate(lg_ctx *ctx, const lg_scene_spec *spec, lg_grid **densi
     const long long elems = (long long)grid_storage_elems(g);
     if (rc == LG_OK) {
         k_generate<<<(unsigned)((total + 255) / 256), 256, 0, ctx->stream>>>(
-            mode, spec->n, spec->octaves, (uint32_t)spec->seed, (float)spec->radius, (int)blobs.size(), g->NB[0],
-            g->NB[1], (float *)g->data, t ? (float *)t->data : nullptr);
+            mode, spec->n, spec->octaves, (uint32_t)spec->seed, (float)spec->radius,
+            (const lgn::Blob *)ctx->blobs_dev, (int)blobs.size(), g->NB[0], g->NB[1], (float *)g->data,
+            t ? (float *)t->data : nullptr);
         if (hipGetLastError() != hipSuccess) {
             set_error("k_generate launch failed");
             rc = LG_ERR_HIP;

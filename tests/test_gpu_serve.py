@@ -101,3 +101,41 @@ def test_budget_admission(lga):
         srv.submit(big)
     assert e.value.code == -4
     srv.close()
+
+
+def test_concurrent_scene_generation(lga):
+    """Two contexts on two threads generate different blob scenes at the same
+    time (as serving workers do); each volume equals the one generated alone.
+    The blob table lives per context: a table shared by the process let one
+    thread's volume be generated with the other's blobs."""
+    import threading
+
+    specs = [lga.scene_spec(lga.LG_SCENE_BLOBS_FBM, 128, seed=900 + k, octaves=4, blobs=20 - 8 * k, axis_min=6.0,
+                            axis_max=30.0, active_fraction=0.3) for k in range(2)]
+    want = []
+    with lga.Context(0) as c:
+        for spec in specs:
+            g, _ = lga.Grid.generate(c, spec)
+            want.append(g.read_dense())
+            g.destroy()
+    assert not np.array_equal(want[0], want[1])
+    errors = []
+
+    def work(k):
+        try:
+            with lga.Context(0) as c:
+                for _ in range(8):
+                    g, _ = lga.Grid.generate(c, specs[k])
+                    got = g.read_dense()
+                    g.destroy()
+                    if not np.array_equal(got, want[k]):
+                        errors.append(k)
+        except Exception as e:  # reported below, not lost in the thread
+            errors.append(repr(e))
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert errors == []
