@@ -474,10 +474,15 @@ __device__ __forceinline__ void sample_point_wave(const SolveDesc &d, int i, uin
     const int sb = d.active_list[lo];
     const uint64_t local = target - d.mass_prefix[lo];
     const T *p = static_cast<const T *>(g.data) + ((long long)sb << 9);
+    // the brick's 512 voxels in one round trip: 8 loads in flight per lane
+    float fv[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) fv[c] = lgd::to_f32(p[64 * c + lane]);
     int v = 511;
     uint64_t carry = 0;
+#pragma unroll
     for (int c = 0; c < 8; ++c) {
-        const float f = lgd::to_f32(p[64 * c + lane]);
+        const float f = fv[c];
         uint64_t q = (f > 0.0f) ? (uint64_t)fmin((double)f * 16777216.0, 549755813888.0 * 16777216.0) : 0ull;
         q = lgd::wave_prefix_sum(q);
         const unsigned long long hit = __ballot(local < carry + q);
