@@ -421,6 +421,10 @@ __device__ __forceinline__ double pdist2(const float *__restrict__ pts, int i, i
 
 constexpr int kCandLds = 512;  // candidates per wave sorted in LDS; longer rows take the ordered scan
 constexpr int kCandGroupedFromV = 65536;  // calls with this many vertices search with 4 lanes per vertex
+// the fill pass of such calls: 4 lanes per vertex up to this mean row length,
+// 16 up to the next, else a wave
+constexpr double kCandFill4MaxRow = 3.0;
+constexpr double kCandFill16MaxRow = 16.0;
 constexpr int kLightSortFromV = 65536;    // calls with this many vertices trace their shadow rays in spatial order
 constexpr int kLightRows16FromV = 65536;  // calls with this many vertices trace 4 shadow rays per wave
 
@@ -2069,12 +2073,20 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
                       {cap > 0 ? pl.colfill : nullptr, (size_t)V},
                       {pl.greedy_err, 1}},
                      &zl, nullptr));
-    if (grouped)
-        k_cand<true, 4><<<cb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted,
+    // the fill ranks each row's candidates in LDS, O(row^2 / lanes) per lane:
+    // long rows (dense volumes) take more lanes per vertex
+    const double avg_row = (double)Ktot / (double)V;
+    const int gf = !grouped ? 64 : (avg_row <= kCandFill4MaxRow ? 4 : (avg_row <= kCandFill16MaxRow ? 16 : 64));
+    const unsigned fb = (unsigned)((V + 4 * (64 / gf) - 1) / (4 * (64 / gf)));
+    if (gf == 4)
+        k_cand<true, 4><<<fb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted,
                                              pl.off, nullptr, pl.cand, zl);
+    else if (gf == 16)
+        k_cand<true, 16><<<fb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted,
+                                              pl.off, nullptr, pl.cand, zl);
     else
-        k_cand<true, 64><<<cb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted,
-                                             pl.off, nullptr, pl.cand, zl);
+        k_cand<true, 64><<<fb, 256, 0, st>>>(pl.sd, pl.cd, pl.sid, V, pts, r2, pl.cstart, pl.cend, pl.cvals_sorted,
+                                              pl.off, nullptr, pl.cand, zl);
     LG_LAUNCHED("k_cand");
     ktimer_end(ctx, "k_cand", ev_c);
     if (cap > 0) k_colcount<<<vb, 256, 0, st>>>(V, pl.off, pl.cand, pl.colcnt);
