@@ -72,9 +72,11 @@ struct lg_ctx {
     // solve's vertex light beside its graph stages, the small-component
     // closures beside the blocked one); always joined back into `stream` before
     // the call returns.  The events are re-recorded per fork: a stream wait
-    // binds to the record current when the wait is enqueued.
+    // binds to the record current when the wait is enqueued.  ev_light marks
+    // the end of a solve's vertex light alone (the readers of the light wait
+    // for it), ev_join the end of everything forked.
     hipStream_t side = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_light = nullptr;
     uint64_t budget = 0;         // 0 = unlimited
     uint64_t in_use = 0;
     std::map<void *, size_t> allocs;

@@ -389,9 +389,11 @@ int32_t lg_ctx_create(int32_t device, uint64_t hbm_budget_bytes, lg_ctx **out) {
     }
     if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_light, hipEventDisableTiming) != hipSuccess) {
         if (ctx->side) (void)hipStreamDestroy(ctx->side);
         if (ctx->ev_fork) (void)hipEventDestroy(ctx->ev_fork);
+        if (ctx->ev_join) (void)hipEventDestroy(ctx->ev_join);
         (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         set_error("lg_ctx_create: side stream or events could not be created");
@@ -422,6 +424,7 @@ int32_t lg_ctx_destroy(lg_ctx *ctx) {
     destroy_stage_pipeline(ctx);
     (void)hipEventDestroy(ctx->ev_fork);
     (void)hipEventDestroy(ctx->ev_join);
+    (void)hipEventDestroy(ctx->ev_light);
     (void)hipStreamDestroy(ctx->side);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

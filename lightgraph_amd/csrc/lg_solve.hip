@@ -397,8 +397,8 @@ static int32_t solve_descs(lg_ctx *ctx, lg_solver *s, const std::vector<SolveDes
             std::swap(ctx->stream, ctx->side);
             const int32_t rc = pl_light(ctx, pl, V, dtype, pp.lc, pp.lights, pp.env_seed, pp.pts, pp.L);
             std::swap(ctx->stream, ctx->side);
-            LG_HIP(hipEventRecord(ctx->ev_join, ctx->side));
-            join.ev = ctx->ev_join;
+            LG_HIP(hipEventRecord(ctx->ev_light, ctx->side));
+            join.ev = ctx->ev_light;
             return rc;
         };
     } else {
