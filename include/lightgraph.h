@@ -567,8 +567,10 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
 /* Device scratch an lg_render of this size keeps on its context (for admission
  * control), given the most shading points one camera ray can take (<= 0:
  * unknown; a ray through n voxels at primary step h takes at most n / (0.75 h)
- * + 1).  It is exceeded only when one block of 64 pixels holds more than 2 GiB
- * of shading points.  No reference counterpart. */
+ * + 1).  A render of more than one chunk of pixels keeps two chunk workspaces
+ * (consecutive chunks run on the context's two streams).  It is exceeded only
+ * when one block of 64 pixels holds more than 2 GiB of shading points.  No
+ * reference counterpart. */
 int32_t lg_render_workspace(int32_t width, int32_t height, int32_t samples, int32_t n_lights,
                             int64_t max_points_per_ray, uint64_t *bytes);
 

@@ -1077,15 +1077,38 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
     const size_t ext_off = pts_off + align256(sizeof(ShadePoint) * max_points);
     const size_t sum_off = ext_off + align256(sizeof(ShadeExtra) * max_points);
     const size_t body_bytes = sum_off + align256(24ull * (uint64_t)L.n * pad_cap);
-    void *body = nullptr;
-    LG_TRY(scratch(ctx, body_bytes, &body, 1));
-    int *off = (int *)body;
-    void *scan_tmp = (char *)body + lay.scan_off;
-    RayOut *rout = (RayOut *)((char *)body + lay.out_off);
-    ShadePoint *pts = (ShadePoint *)((char *)body + pts_off);
-    ShadeExtra *ext = (ShadeExtra *)((char *)body + ext_off);
-    double *sums = (double *)((char *)body + sum_off);
-    for (const Chunk &ch : chunks) {
+    // Several chunks alternate between two workspaces and the context's two
+    // streams, so that one chunk's bandwidth-bound phases (fill, combine) run
+    // beside the other's latency-bound shadow march.  Chunks share only
+    // read-only inputs and write disjoint pixels.  With kernel timing on (per
+    // phase times, brick counting) the chunks run in order on one stream.
+    const bool two = chunks.size() > 1 && !ctx->timing && !touch;
+    void *body0 = nullptr;
+    LG_TRY(scratch(ctx, (two ? 2 : 1) * body_bytes, &body0, 1));
+    struct SideJoin {
+        lg_ctx *ctx;
+        bool on;
+        ~SideJoin() {
+            if (on && hipEventRecord(ctx->ev_join, ctx->side) == hipSuccess)
+                (void)hipStreamWaitEvent(ctx->stream, ctx->ev_join, 0);
+        }
+    } join{ctx, false};
+    if (two) {
+        LG_HIP(hipEventRecord(ctx->ev_fork, ctx->stream));
+        LG_HIP(hipStreamWaitEvent(ctx->side, ctx->ev_fork, 0));
+        join.on = true;
+    }
+    for (size_t ci = 0; ci < chunks.size(); ++ci) {
+        const Chunk &ch = chunks[ci];
+        const bool alt = two && (ci & 1);
+        hipStream_t st = alt ? ctx->side : ctx->stream;
+        void *body = (char *)body0 + (alt ? body_bytes : 0);
+        int *off = (int *)body;
+        void *scan_tmp = (char *)body + lay.scan_off;
+        RayOut *rout = (RayOut *)((char *)body + lay.out_off);
+        ShadePoint *pts = (ShadePoint *)((char *)body + pts_off);
+        ShadeExtra *ext = (ShadeExtra *)((char *)body + ext_off);
+        double *sums = (double *)((char *)body + sum_off);
         const uint32_t pix0 = ch.b0 * kRenderBlockPixels;
         const uint32_t pix1 = std::min<uint64_t>((uint64_t)ch.b1 * kRenderBlockPixels, lay.npix);
         const uint32_t ray0 = pix0 * (uint32_t)samples, nr = (pix1 - pix0) * (uint32_t)samples;
@@ -1093,11 +1116,11 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
         if (P > 0) {
             size_t tb = lay.scan_bytes;
             LG_HIP(rocprim::exclusive_scan(scan_tmp, tb, cnt + ray0, off, 0, (size_t)nr, rocprim::plus<int>(),
-                                           ctx->stream));
+                                           st));
             ktimer_begin(ctx, "k_render_fill", &ev);
             for_dtype_r(density->desc.dtype, [&](auto tag) {
                 using T = decltype(tag);
-                k_render_fill<T><<<(nr + 255) / 256, 256, 0, ctx->stream>>>(g, c, u0, u1, ray0, nr, cnt, off, pts,
+                k_render_fill<T><<<(nr + 255) / 256, 256, 0, st>>>(g, c, u0, u1, ray0, nr, cnt, off, pts,
                                                                             ext);
             });
             LG_LAUNCHED("k_render_fill");
@@ -1108,10 +1131,10 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
                 for_dtype_r(density->desc.dtype, [&](auto tag) {
                     using T = decltype(tag);
                     if (touch)
-                        k_render_shadow<T, true><<<(unsigned)((tasks + 255) / 256), 256, 0, ctx->stream>>>(
+                        k_render_shadow<T, true><<<(unsigned)((tasks + 255) / 256), 256, 0, st>>>(
                             g, c, L, u0, u1, e0, e1, pts, P, P_pad, sums, stats);
                     else
-                        k_render_shadow<T, false><<<(unsigned)((tasks + 255) / 256), 256, 0, ctx->stream>>>(
+                        k_render_shadow<T, false><<<(unsigned)((tasks + 255) / 256), 256, 0, st>>>(
                             g, c, L, u0, u1, e0, e1, pts, P, P_pad, sums, stats);
                 });
                 LG_LAUNCHED("k_render_shadow");
@@ -1120,12 +1143,12 @@ int32_t lg_render(lg_ctx *ctx, const lg_grid *density, const lg_scatter_grid *sc
             }
         }
         ktimer_begin(ctx, "k_render_combine", &ev);
-        k_render_combine<<<(nr + 255) / 256, 256, 0, ctx->stream>>>(g, tg, temperature ? 1 : 0, sc, has_sc, c, L,
+        k_render_combine<<<(nr + 255) / 256, 256, 0, st>>>(g, tg, temperature ? 1 : 0, sc, has_sc, c, L,
                                                                     ray0, nr, cnt, off, pts, ext, sums, P_pad, rout,
                                                                     stats);
         LG_LAUNCHED("k_render_combine");
         ktimer_end(ctx, "k_render_combine", ev);
-        k_render_aa<<<(pix1 - pix0 + 255) / 256, 256, 0, ctx->stream>>>(
+        k_render_aa<<<(pix1 - pix0 + 255) / 256, 256, 0, st>>>(
             c, pix0, pix1 - pix0, rout, (float4 *)films->d_rgb, (float4 *)films->d_direct, (float4 *)films->d_scatter,
             (float4 *)films->d_emission);
         LG_LAUNCHED("k_render_aa");
@@ -1166,7 +1189,11 @@ int32_t lg_render_workspace(int32_t width, int32_t height, int32_t samples, int3
         const uint64_t chunk_rays = std::max<uint64_t>(l.ray_cap, l.rays_per_block);
         pts = std::min<uint64_t>(pts, chunk_rays * (uint64_t)max_points_per_ray);
     }
-    *bytes = l.head_bytes + l.ray_area_bytes + align256(pts * l.point_bytes) + 1024;
+    // more than one chunk (more rays than one chunk takes, or possibly more
+    // points): lg_render keeps two chunk workspaces
+    const bool two = l.nrays > l.ray_cap || max_points_per_ray <= 0 ||
+                     (uint64_t)max_points_per_ray * l.nrays > l.point_cap;
+    *bytes = l.head_bytes + (two ? 2 : 1) * (l.ray_area_bytes + align256(pts * l.point_bytes)) + 1024;
     return LG_OK;
 }
 
