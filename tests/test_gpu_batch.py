@@ -107,7 +107,8 @@ def test_fw_split_panels_in_large_launches(lga, ctx):
     """600 copies of one matrix in one batched closure: each split panel launch
     holds 2400 workgroups, more than the GPU keeps resident, so late workgroups
     start after others of the same tile have finished writing.  Every copy equals
-    the whole-tile closure of the same input, split or not."""
+    the whole-tile closure of the same input, split or not, and the one-matrix
+    closure of it."""
     rng = np.random.default_rng(11)
     W = _random_graph(rng, 300, 3.0, integer=False)
     Dm = _padded(lga, W)[0]
@@ -124,6 +125,10 @@ def test_fw_split_panels_in_large_launches(lga, ctx):
                 b.free()
             results.append(out)
         want = results[0][0]
+        single = ctx.upload(Dm)
+        lga.fw_solve(ctx, 300, single, Dm.shape[0])
+        np.testing.assert_array_equal(single.read(np.float32, Dm.shape), want)
+        single.free()
         for out in results:
             for k, got in enumerate(out):
                 np.testing.assert_array_equal(got, want, err_msg=f"copy {k}")
