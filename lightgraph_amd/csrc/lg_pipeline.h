@@ -561,6 +561,14 @@ struct Pipeline {
     bool nd_order = false;   // large components closed in nested-dissection vertex order
     double *ray = nullptr;  // per (vertex, light sample) shadowed intensity
     size_t ray_cap = 0;
+    // pinned host words for the solve's two readbacks (candidate count, component
+    // counts): a copy into pageable memory goes through a staging buffer
+    int *h_pin = nullptr;
+    size_t h_pin_cap = 0;
+    int32_t pinned_words(size_t n, int **out);
+    ~Pipeline() {
+        if (h_pin) (void)hipHostFree(h_pin);
+    }
     // spatial ray order of large calls: keys, sorted keys, vertices, sorted vertices
     // (four runs of lsort_cap / 4 words) and the sort's temporary storage
     uint32_t *lkeys = nullptr;

@@ -1774,8 +1774,24 @@ void Pipeline::release_vertex(lg_ctx *ctx) {
     v_cap = s_cap = 0;
 }
 
+int32_t Pipeline::pinned_words(size_t n, int **out) {
+    if (n > h_pin_cap) {
+        if (h_pin) (void)hipHostFree(h_pin);
+        h_pin = nullptr;
+        h_pin_cap = 0;
+        const size_t want = std::max<size_t>(n + n / 4, 256);
+        LG_HIP(hipHostMalloc((void **)&h_pin, want * sizeof(int), hipHostMallocDefault));
+        h_pin_cap = want;
+    }
+    *out = h_pin;
+    return LG_OK;
+}
+
 void Pipeline::release(lg_ctx *ctx) {
     release_vertex(ctx);
+    if (h_pin) (void)hipHostFree(h_pin);
+    h_pin = nullptr;
+    h_pin_cap = 0;
     void *ps[] = {cand,  accepted, epos, tmp_cand, edges_, w_,   anb,    aw,      big_T,
                   big_ln, big_meta, ray, cstart,   cend,   stats, colslot, slot_info, lkeys, tmp_light};
     for (void *p : ps) dfree(ctx, p);
@@ -2056,9 +2072,11 @@ int32_t pl_graph(lg_ctx *ctx, Pipeline &pl, int V, int S, int nmax, const float 
                                               nullptr, pl.rowcnt, nullptr, ZeroList{});
     LG_LAUNCHED("k_cand");
     LG_TRY(scan_excl(ctx, pl, pl.rowcnt, pl.off, (size_t)V + 1, false));
-    int Ktot = 0;
-    LG_HIP(hipMemcpyAsync(&Ktot, pl.off + V, sizeof(int), hipMemcpyDeviceToHost, st));
+    int *hk = nullptr;
+    LG_TRY(pl.pinned_words(16, &hk));
+    LG_HIP(hipMemcpyAsync(hk, pl.off + V, sizeof(int), hipMemcpyDeviceToHost, st));
     LG_HIP(hipStreamSynchronize(st));
+    const int Ktot = hk[0];
     LG_TRY(grow(ctx, &pl.cand, &pl.cand_cap, (size_t)Ktot + 1));
     LG_TRY(grow(ctx, &pl.accepted, &pl.accepted_cap, (size_t)Ktot + 1));
     LG_TRY(grow(ctx, &pl.epos, &pl.epos_cap, (size_t)Ktot + 1));
@@ -2212,9 +2230,12 @@ int32_t pl_components(lg_ctx *ctx, Pipeline &pl, const std::vector<SolveDesc> &d
     // counts and the large-component pairs come back in one synchronisation
     // (counters[0..4], component count, edge count, then the large-component
     // pairs: big_list = counters + 16)
-    std::vector<int> rb((size_t)16 + 2 * (size_t)max_big);
-    LG_HIP(hipMemcpyAsync(rb.data(), pl.counters, sizeof(int) * rb.size(), hipMemcpyDeviceToHost, st));
+    const size_t nrb = (size_t)16 + 2 * (size_t)max_big;
+    int *hrb = nullptr;
+    LG_TRY(pl.pinned_words(nrb, &hrb));
+    LG_HIP(hipMemcpyAsync(hrb, pl.counters, sizeof(int) * nrb, hipMemcpyDeviceToHost, st));
     LG_HIP(hipStreamSynchronize(st));
+    const std::vector<int> rb(hrb, hrb + nrb);
     ktimer_end(ctx, "k_components", ev);
     if (rb[10] != 0) {
         set_error("pl_graph: a row of the parallel greedy cap waited past its bound");
