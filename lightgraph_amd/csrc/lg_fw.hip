@@ -51,10 +51,12 @@ __device__ __forceinline__ int fw_round_kend(int n, int kb) {
 // m: pivots of the tile to take (the rest are padding)
 __device__ __forceinline__ void fw_pivot_tile(float *__restrict__ D, int ld, int kb, int m) {
     constexpr int KG = kPivotGroup;
-    __shared__ float P[KG][TB];  // rows of the group before it
-    __shared__ float G[KG][TB];  // the group's pivot rows
+    static_assert(KG == 8 && TB == 128, "the owners of a pivot group are one wave");
+    __shared__ float P[KG][TB];     // rows of the group before it
+    __shared__ float G[2][KG][TB];  // the group's pivot rows, alternating buffers
     const int tid = threadIdx.x;
     const int rg = tid >> 5, cg = tid & 31;
+    const int wave = tid >> 6, lane = tid & 63;
     float *base = D + (long long)kb * TB * ld + (long long)kb * TB;
     float v[4][4];
 #pragma unroll
@@ -65,19 +67,26 @@ __device__ __forceinline__ void fw_pivot_tile(float *__restrict__ D, int ld, int
         v[a][2] = q.z;
         v[a][3] = q.w;
     }
+    // Group g's rows k..k+7 are held by wave g (row groups 2g, 2g+1): it
+    // publishes them and derives the group's pivot rows itself, so one
+    // workgroup barrier per group suffices.  G alternates between two buffers:
+    // a wave writes G[g & 1] again only at group g + 2, after every wave has
+    // passed group g + 1's barrier and so finished reading it.
     const int groups = (m + KG - 1) / KG;
     for (int g = 0; g < groups; ++g) {
         const int k = KG * g;
-        if (rg / (KG / 4) == g) {  // the owners of rows k..k+KG-1 publish them
+        if (wave == g) {
             const int r0 = 4 * (rg % (KG / 4));
 #pragma unroll
             for (int a = 0; a < 4; ++a)
                 *reinterpret_cast<float4 *>(&P[r0 + a][4 * cg]) = make_float4(v[a][0], v[a][1], v[a][2], v[a][3]);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            pivot_group_rows_wave128<KG>(P, G[g & 1], k, lane);
         }
         __syncthreads();
-        if (tid < TB) pivot_group_rows<KG, TB>(P, G, k, tid);
-        __syncthreads();
-        pivot_group_update<KG, TB>(G, rg, cg, v);
+        pivot_group_update<KG, TB>(G[g & 1], rg, cg, v);
     }
 #pragma unroll
     for (int a = 0; a < 4; ++a)

@@ -46,6 +46,42 @@ __device__ __forceinline__ void pivot_group_rows(const float (*P)[W], float (*G)
     for (int r = 0; r < KG; ++r) G[r][x] = g[r];
 }
 
+// pivot_group_rows for a 128-column tile computed by one wave (columns lane
+// and lane + 64): with 4 x 4 blocks over 32 column groups, the owners of a
+// group's 8 rows are exactly one wave, which can then publish its rows and
+// derive G without a workgroup barrier in between.  Same operations, same
+// order, per column as pivot_group_rows.
+template <int KG>
+__device__ __forceinline__ void pivot_group_rows_wave128(const float (*P)[128], float (*G)[128], int k, int lane) {
+    float s[KG][KG];
+#pragma unroll
+    for (int r = 1; r < KG; ++r) s[0][r] = P[0][k + r];
+#pragma unroll
+    for (int q = 1; q < KG - 1; ++q)
+#pragma unroll
+        for (int r = q + 1; r < KG; ++r) {
+            float t = P[q][k + r];
+#pragma unroll
+            for (int p = 0; p < q; ++p) t = fminf(t, s[p][q] + s[p][r]);
+            s[q][r] = t;
+        }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int x = lane + 64 * h;
+        float g[KG];
+        g[0] = P[0][x];
+#pragma unroll
+        for (int r = 1; r < KG; ++r) {
+            float t = P[r][x];
+#pragma unroll
+            for (int q = 0; q < r; ++q) t = fminf(t, s[q][r] + g[q]);
+            g[r] = t;
+        }
+#pragma unroll
+        for (int r = 0; r < KG; ++r) G[r][x] = g[r];
+    }
+}
+
 // The group's update of the 4 x 4 block at rows 4*rg.., columns 4*cg..: two
 // pivots folded per v_min3_f32.
 template <int KG, int W>

@@ -1373,7 +1373,7 @@ __global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
     constexpr int KG = kPivotGroup;
     __shared__ float T[MS][MS + 1];
     __shared__ float P[KG][MS];
-    __shared__ float G[KG][MS];
+    __shared__ float G[2][KG][MS];  // alternating: see fw_pivot_tile
     __shared__ int mem[MS];
     __shared__ float Ln[MS][3];
     if ((int)blockIdx.x >= ncls) {
@@ -1420,17 +1420,34 @@ __global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
     const int groups = (m + KG - 1) / KG;
     for (int g = 0; g < groups; ++g) {
         const int k = KG * g;
-        if (rg / (KG / 4) == g) {  // the owners of rows k..k+KG-1 publish them
-            const int r0 = 4 * (rg % (KG / 4));
+        if constexpr (MS == 128) {
+            // the owners of rows k..k+KG-1 are wave g: it publishes them and
+            // derives the group's pivot rows without a workgroup barrier
+            if ((tid >> 6) == g) {
+                const int r0 = 4 * (rg % (KG / 4));
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+                for (int a = 0; a < 4; ++a)
 #pragma unroll
-                for (int b = 0; b < 4; ++b) P[r0 + a][4 * cg + b] = v[a][b];
+                    for (int b = 0; b < 4; ++b) P[r0 + a][4 * cg + b] = v[a][b];
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                pivot_group_rows_wave128<KG>(P, G[g & 1], k, tid & 63);
+            }
+            __syncthreads();
+        } else {
+            if (rg / (KG / 4) == g) {  // the owners of rows k..k+KG-1 publish them
+                const int r0 = 4 * (rg % (KG / 4));
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) P[r0 + a][4 * cg + b] = v[a][b];
+            }
+            __syncthreads();
+            if (tid < MS) pivot_group_rows<KG, MS>(P, G[g & 1], k, tid);
+            __syncthreads();
         }
-        __syncthreads();
-        if (tid < MS) pivot_group_rows<KG, MS>(P, G, k, tid);
-        __syncthreads();
-        pivot_group_update<KG, MS>(G, rg, cg, v);
+        pivot_group_update<KG, MS>(G[g & 1], rg, cg, v);
     }
     // diffusion, one channel at a time: the row's 16 transmittances stay in
     // registers between the two reductions over the row's lanes (column sums =
