@@ -1350,6 +1350,27 @@ __global__ void k_member_local(int V, int bits, const uint64_t *__restrict__ sor
 }
 
 // ------------------------------------------------ small-component closure ----
+// Sum of x over each group of CG consecutive lanes (a row of the closure's
+// register tile), valid at the group's last lane.  Groups of 16 and 32 lanes
+// are DPP rows: prefix sums within the rows (row_shr 1, 2, 4, 8), then for 32
+// row 0's total added into row 1 (row_bcast 15), in a fixed order and without
+// the LDS round trips of lane shuffles; smaller groups use shuffles.
+template <int CG>
+__device__ __forceinline__ double row_group_sum(double x) {
+    if constexpr (CG == 16 || CG == 32) {
+        x += lgd::dpp_move<0x111, 0xf>(x);
+        x += lgd::dpp_move<0x112, 0xf>(x);
+        x += lgd::dpp_move<0x114, 0xf>(x);
+        x += lgd::dpp_move<0x118, 0xf>(x);
+        if constexpr (CG == 32) x += lgd::dpp_move<0x142, 0xa>(x);
+        return x;
+    } else {
+#pragma unroll
+        for (int o = CG / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, CG);
+        return x;
+    }
+}
+
 // One workgroup per component of m <= 128 vertices: build the m x m optical
 // depth matrix in LDS from the adjacency, close it pivot by pivot (ascending
 // vertex order), then diffuse (LightGraph.h:554-579) and write the outputs.
@@ -1465,10 +1486,9 @@ __global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
             double sq = 0.0;
 #pragma unroll
             for (int b = 0; b < 4; ++b) sq += (double)tq[a][b];
-#pragma unroll
-            for (int o = CG / 2; o > 0; o >>= 1) sq += __shfl_xor(sq, o, CG);
+            sq = row_group_sum<CG>(sq);
             const int i = 4 * rg + a;
-            if (cg == 0 && i < m) Ln[i][q] = (float)((double)Ln[i][q] / (sq < 1.0 ? 1.0 : sq));
+            if (cg == CG - 1 && i < m) Ln[i][q] = (float)((double)Ln[i][q] / (sq < 1.0 ? 1.0 : sq));
         }
         __syncthreads();
 #pragma unroll
@@ -1479,10 +1499,9 @@ __global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
                 const int j = 4 * cg + b;
                 if (j < m) oq += (double)Ln[j][q] * (double)tq[a][b];
             }
-#pragma unroll
-            for (int o = CG / 2; o > 0; o >>= 1) oq += __shfl_xor(oq, o, CG);
+            oq = row_group_sum<CG>(oq);
             const int i = 4 * rg + a;
-            if (cg == 0 && i < m) out[3 * mem[i] + q] = (float)oq;
+            if (cg == CG - 1 && i < m) out[3 * mem[i] + q] = (float)oq;
         }
     }
     const SolveDesc &d = sd[sid[mem[0]]];
