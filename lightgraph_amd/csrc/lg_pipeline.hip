@@ -1589,6 +1589,21 @@ __global__ void __launch_bounds__(256) k_big_fill(BigJob *__restrict__ jobs, con
     for (int p = aoff[v] + threadIdx.x; p < aoff[v + 1]; p += blockDim.x) row[local[anb[p]]] = aw[p];
 }
 
+// Sums of three values over a 256-thread workgroup, at thread 0: each wave
+// reduces by DPP prefix sums (its total at lane 63), then thread 0 adds the
+// four waves' totals in order.  One barrier, a fixed order.
+__device__ __forceinline__ void block256_sum3(double v[3], double (*red)[4]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const double x = lgd::wave_prefix_sum(v[c]);
+        if ((threadIdx.x & 63) == 63) red[c][threadIdx.x >> 6] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = ((red[c][0] + red[c][1]) + red[c][2]) + red[c][3];
+}
+
 // per-row sums of exp(ext_c * D) over the component (D symmetric: = column
 // sums), and from them the vertex's normalised light L_i / max(colsum_i, 1),
 // the factor that every row of the diffusion multiplies column i by.
@@ -1607,20 +1622,13 @@ __global__ void __launch_bounds__(256) k_big_rowsum(const BigJob *__restrict__ j
         a1 += trans_f(e1, d);
         a2 += trans_f(e2, d);
     }
-    __shared__ double s[3][256];
-    s[0][threadIdx.x] = a0;
-    s[1][threadIdx.x] = a1;
-    s[2][threadIdx.x] = a2;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            for (int c = 0; c < 3; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + o];
-        __syncthreads();
-    }
+    __shared__ double s[3][4];
+    double a[3] = {(double)a0, (double)a1, (double)a2};
+    block256_sum3(a, s);
     if (threadIdx.x == 0) {
         const int vi = member[jb.a0 + i];
         for (int c = 0; c < 3; ++c) {
-            const double sc = s[c][0];
+            const double sc = a[c];
             ln[3 * (long long)(jb.row0 + i) + c] = (float)((double)L[3 * vi + c] / (sc < 1.0 ? 1.0 : sc));
         }
     }
@@ -1644,19 +1652,12 @@ __global__ void __launch_bounds__(256) k_big_diffuse(const BigJob *__restrict__ 
         a1 += (double)(lj[1] * trans_f(e1, d));
         a2 += (double)(lj[2] * trans_f(e2, d));
     }
-    __shared__ double s[3][256];
-    s[0][threadIdx.x] = a0;
-    s[1][threadIdx.x] = a1;
-    s[2][threadIdx.x] = a2;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            for (int c = 0; c < 3; ++c) s[c][threadIdx.x] += s[c][threadIdx.x + o];
-        __syncthreads();
-    }
+    __shared__ double s[3][4];
+    double a[3] = {(double)a0, (double)a1, (double)a2};
+    block256_sum3(a, s);
     if (threadIdx.x == 0) {
         const int vi = member[jb.a0 + i];
-        for (int c = 0; c < 3; ++c) out[3 * vi + c] = (float)s[c][0];
+        for (int c = 0; c < 3; ++c) out[3 * vi + c] = (float)a[c];
     }
 }
 
