@@ -1428,9 +1428,16 @@ __global__ void __launch_bounds__((MS / 4) * (MS / 4)) k_comp_small(
         T[i][j] = (i == j) ? 0.0f : INFINITY;
     }
     __syncthreads();
-    for (int a = tid; a < m; a += NT) {
-        const int v = mem[a];
-        for (int p = aoff[v]; p < aoff[v + 1]; ++p) T[a][local[anb[p]]] = aw[p];
+    {
+        // NT / MS lanes per row walk its adjacency together: each lane's loads
+        // (neighbour, its local index, weight) depend on one another, and a
+        // row's edges are distinct, so the lanes write distinct entries
+        constexpr int LPR = NT / MS;
+        const int a = tid / LPR, l = tid % LPR;
+        if (a < m) {
+            const int v = mem[a];
+            for (int p = aoff[v] + l; p < aoff[v + 1]; p += LPR) T[a][local[anb[p]]] = aw[p];
+        }
     }
     __syncthreads();
     float v[4][4];
