@@ -428,9 +428,10 @@ constexpr double kCandFill16MaxRow = 16.0;
 constexpr int kLightSortFromV = 65536;    // calls with this many vertices trace their shadow rays in spatial order
 constexpr int kLightRows16FromV = 65536;  // calls with this many vertices trace 4 shadow rays per wave
 
-// G lanes per global vertex i (64 / G vertices per wave; G = 64 for small calls,
-// where few vertices have to fill the GPU, 4 for large ones, where the rows are
-// short and a whole wave per row leaves most lanes idle).  FILL = false: count
+// G lanes per global vertex i (64 / G vertices per wave).  Small calls, where
+// few vertices have to fill the GPU, take a wave per vertex; large ones count
+// with 4 lanes (their rows are short: a wave per row leaves most lanes idle)
+// and fill with 4, 16 or 64 by the mean row length.  FILL = false: count
 // candidates into count[i]; FILL = true: write them, ascending, at off[i].
 template <bool FILL, int G>
 __global__ void __launch_bounds__(256) k_cand(const SolveDesc *__restrict__ sd, const CellDesc *__restrict__ cd,
